@@ -1,0 +1,215 @@
+// dnum.hpp — device arithmetic of the lookahead path (HIP, gfx950).
+//
+// Every FP32 expression is written in the evaluation order the host side of
+// this library documents (left to right, no contraction: the kernels are built
+// with -ffp-contract=off), exact products use __builtin_fmaf, and volume sums
+// go into an int128 fixed-point accumulator so that the order in which lanes
+// and waves combine their partial sums cannot change a single bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../host/cost.hpp"
+
+namespace dk {
+
+typedef __int128 i128;
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr float PC_EPS_F = 1e-6f;  // src/gpu/shaders.ts:162
+
+// ---- fixed point: value * 2^64 truncated toward zero
+__device__ __forceinline__ i128 fix_f32(float x) {
+  const uint32_t b = __float_as_uint(x);
+  uint32_t e = (b >> 23) & 0xffu;
+  uint32_t m = b & 0x7fffffu;
+  if (e == 0) {
+    if (m == 0) return 0;
+    e = 1;
+  } else {
+    m |= 0x800000u;
+  }
+  const int sh = (int)e - 86;
+  i128 v;
+  if (sh >= 0) v = (i128)m << sh;
+  else if (sh > -24) v = (i128)(m >> (-sh));
+  else v = 0;
+  return (b >> 31) ? -v : v;
+}
+__device__ __forceinline__ double fix_to_f64(i128 a) {
+  const int64_t hi = (int64_t)(a >> 64);
+  const uint64_t lo = (uint64_t)a;
+  return (double)hi + (double)lo * 0x1p-64;
+}
+__device__ __forceinline__ i128 fix_abs(i128 a) { return a < 0 ? -a : a; }
+
+__device__ __forceinline__ void acc_triple(i128& acc, float x, float y, float z, bool neg) {
+  const float p = x * y;
+  const float e = __builtin_fmaf(x, y, -p);
+  const float p2 = p * z;
+  const float e2 = __builtin_fmaf(p, z, -p2);
+  const float e3 = e * z;
+  const i128 s = fix_f32(p2) + fix_f32(e2) + fix_f32(e3);
+  acc += neg ? -s : s;
+}
+__device__ __forceinline__ void acc_det(i128& acc, const float* a, const float* b, const float* c, bool neg) {
+  acc_triple(acc, a[0], b[1], c[2], neg);
+  acc_triple(acc, a[0], b[2], c[1], !neg);
+  acc_triple(acc, a[1], b[2], c[0], neg);
+  acc_triple(acc, a[1], b[0], c[2], !neg);
+  acc_triple(acc, a[2], b[0], c[1], neg);
+  acc_triple(acc, a[2], b[1], c[0], !neg);
+}
+__device__ __forceinline__ void acc_det_anchor(i128& acc, const float* a, const float* b, const float* c, const float* o) {
+  acc_det(acc, a, b, c, false);
+  acc_det(acc, o, b, c, true);
+  acc_det(acc, a, o, c, true);
+  acc_det(acc, a, b, o, true);
+}
+
+// ---- wave (64-lane) primitives
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+// number of set bits of `mask` below this lane
+__device__ __forceinline__ uint32_t prefix_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+// make this wave's prior LDS/global writes visible to its other lanes
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ i128 shfl_xor_i128(i128 v, int m) {
+  const uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
+  const uint64_t lo2 = __shfl_xor(lo, m, 64), hi2 = __shfl_xor(hi, m, 64);
+  return (i128)(((unsigned __int128)hi2 << 64) | lo2);
+}
+__device__ __forceinline__ i128 wave_sum_i128(i128 v) {
+  for (int m = 32; m >= 1; m >>= 1) v += shfl_xor_i128(v, m);
+  return v;
+}
+__device__ __forceinline__ float wave_min_f(float v) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float o = __shfl_xor(v, m, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ float wave_max_f(float v) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float o = __shfl_xor(v, m, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) {
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    const uint32_t o = __shfl_xor(v, m, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+// (value, index): largest value, then lowest index
+__device__ __forceinline__ void wave_argmax(float& v, uint32_t& i) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float ov = __shfl_xor(v, m, 64);
+    const uint32_t oi = __shfl_xor(i, m, 64);
+    if (ov > v || (ov == v && oi < i)) {
+      v = ov;
+      i = oi;
+    }
+  }
+}
+// (value, index): smallest value, then lowest index
+__device__ __forceinline__ void wave_argmin(float& v, uint32_t& i) {
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float ov = __shfl_xor(v, m, 64);
+    const uint32_t oi = __shfl_xor(i, m, 64);
+    if (ov < v || (ov == v && oi < i)) {
+      v = ov;
+      i = oi;
+    }
+  }
+}
+
+// ---- planes
+__device__ __forceinline__ float plane_eval(const float* pl, float x, float y, float z) {
+  float t = pl[0] * x;
+  t = t + pl[1] * y;
+  t = t + pl[2] * z;
+  return t + pl[3];
+}
+__device__ __forceinline__ int vsign(float d) { return d > PC_EPS_F ? 1 : (d < -PC_EPS_F ? -1 : 0); }
+
+// src/gpu/shaders.ts:116-119 plane_point (FP64 math, f32 result)
+__device__ __forceinline__ void plane_point(const float* pl, float* out) {
+  const double a = pl[0], b = pl[1], c = pl[2];
+  double n2 = a * a + b * b;
+  n2 = n2 + c * c;
+  if (!(n2 > 1e-30)) n2 = 1e-30;
+  const double s = -(double)pl[3] / n2;
+  out[0] = (float)(a * s);
+  out[1] = (float)(b * s);
+  out[2] = (float)(c * s);
+}
+
+// src/gpu/shaders.ts:126-143 anchor_for (FP64 math, f32 result)
+__device__ inline void anchor_for(const float pls[4][4], uint32_t n, float* o) {
+  o[0] = o[1] = o[2] = 0.f;
+  if (n == 0) return;
+  if (n == 1) {
+    plane_point(pls[0], o);
+    return;
+  }
+  if (n >= 3) {
+    const float* p1 = pls[n - 3];
+    const float* p2 = pls[n - 2];
+    const float* p3 = pls[n - 1];
+    const double n1[3] = {p1[0], p1[1], p1[2]}, n2[3] = {p2[0], p2[1], p2[2]}, n3[3] = {p3[0], p3[1], p3[2]};
+    const double c23[3] = {n2[1] * n3[2] - n2[2] * n3[1], n2[2] * n3[0] - n2[0] * n3[2], n2[0] * n3[1] - n2[1] * n3[0]};
+    const double c31[3] = {n3[1] * n1[2] - n3[2] * n1[1], n3[2] * n1[0] - n3[0] * n1[2], n3[0] * n1[1] - n3[1] * n1[0]};
+    const double c12[3] = {n1[1] * n2[2] - n1[2] * n2[1], n1[2] * n2[0] - n1[0] * n2[2], n1[0] * n2[1] - n1[1] * n2[0]};
+    const double det = (n1[0] * c23[0] + n1[1] * c23[1]) + n1[2] * c23[2];
+    if (fabs(det) > 1e-4) {
+      const double e1 = -(double)p1[3], e2 = -(double)p2[3], e3 = -(double)p3[3];
+      for (int k = 0; k < 3; k++) o[k] = (float)(((e1 * c23[k] + e2 * c31[k]) + e3 * c12[k]) / det);
+      return;
+    }
+  }
+  const float* a = pls[n - 2];
+  const float* b = pls[n - 1];
+  const double an[3] = {a[0], a[1], a[2]}, bn[3] = {b[0], b[1], b[2]};
+  const double L[3] = {an[1] * bn[2] - an[2] * bn[1], an[2] * bn[0] - an[0] * bn[2], an[0] * bn[1] - an[1] * bn[0]};
+  const double l2 = (L[0] * L[0] + L[1] * L[1]) + L[2] * L[2];
+  if (l2 > 1e-8) {
+    const double e1 = -(double)a[3], e2 = -(double)b[3];
+    const double bxL[3] = {bn[1] * L[2] - bn[2] * L[1], bn[2] * L[0] - bn[0] * L[2], bn[0] * L[1] - bn[1] * L[0]};
+    const double Lxa[3] = {L[1] * an[2] - L[2] * an[1], L[2] * an[0] - L[0] * an[2], L[0] * an[1] - L[1] * an[0]};
+    for (int k = 0; k < 3; k++) o[k] = (float)((e1 * bxL[k] + e2 * Lxa[k]) / l2);
+    return;
+  }
+  plane_point(b, o);
+}
+
+// src/gpu/shaders.ts:171-181 clip_tri: entries 0..2 = corner, 4+k = crossing on edge k->k+1
+struct Poly {
+  uint32_t n;
+  uint32_t src[4];
+};
+__device__ __forceinline__ Poly clip_tri(const int* sg, int side) {
+  Poly p;
+  p.n = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int q = k == 2 ? 0 : k + 1;
+    const int sp = sg[k] * side, sq = sg[q] * side;
+    if (sp >= 0) p.src[p.n++] = (uint32_t)k;
+    if (sg[k] != 0 && sg[q] != 0 && sp * sq < 0) p.src[p.n++] = 4u + (uint32_t)k;
+  }
+  return p;
+}
+
+}  // namespace dk

@@ -1,0 +1,597 @@
+// gpu_backend.hip — the lookahead engine on one MI355X (host side of the HIP
+// kernels): device pools and tables, level-synchronous tree runs, hull and
+// Hausdorff batches. Mirrors the GpuTree class of src/gpu/tree.ts:116-816.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../host/backend.hpp"
+#include "../host/sampling.hpp"
+#include "k_hd.hpp"
+#include "k_hull.hpp"
+#include "k_tree.hpp"
+
+namespace cuacd {
+
+namespace {
+
+void hipCheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() {}
+  DevBuf(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    if (p) hipCheck(hipFree(p), "hipFree");
+    p = nullptr;
+    n = 0;
+    const size_t c = std::max<size_t>(count, 1);
+    hipCheck(hipMalloc(&p, c * sizeof(T)), "hipMalloc");
+    n = c;
+  }
+};
+
+double nowMs() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Level {
+  uint32_t itemsInBase, nItemsIn, itemsOutBase, jobBase, partBase, nJobs, width, nEdgeCuts, totalWidth;
+};
+
+}  // namespace
+
+class GpuBackend : public Backend {
+ public:
+  explicit GpuBackend(const TreeConfig& cfg) : cfg_(cfg), edgeStride_(edgeStrideOf(cfg)) {
+    if (cfg.width % 3 != 0 || cfg.width2 % 3 != 0) throw std::invalid_argument("width and width2 must be multiples of 3");
+    if (cfg.depth < 1 || cfg.depth > 4) throw std::invalid_argument("depth must be in 1..4");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw DeviceError("no HIP device available");
+    if (cfg.device < 0 || cfg.device >= n) throw std::invalid_argument("device ordinal out of range");
+    hipCheck(hipSetDevice(cfg.device), "hipSetDevice");
+    hipCheck(hipStreamCreate(&stream_), "hipStreamCreate");
+    ctr_.ensure(1);
+  }
+  ~GpuBackend() override {
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+  const TreeConfig& config() const override { return cfg_; }
+  uint32_t planeSlots() const override { return edgeStride_; }
+
+  TreeRunResult run(const std::vector<MeshRef>& seeds, const std::vector<std::vector<float>>& edgePlanes,
+                    const RunOptions& opts) override;
+  std::vector<HullResult> hullMeshes(const std::vector<MeshRef>& meshes) override;
+  std::vector<double> hausdorff(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b) override;
+  std::vector<HullResult> hullsWithHausdorff(const std::vector<MeshRef>& meshes, std::vector<double>* hd) override;
+  std::vector<ItemView> levelItems(uint32_t level) override;
+  bool readPart(uint32_t partIdx, PartView* pv, Mesh* mesh) override;
+
+ private:
+  void sync() { hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+  // host -> device copy ordered on the stream that returns once the host buffer may be reused
+  void h2d(void* dst, const void* src, size_t bytes, const char* what) {
+    if (bytes) hipCheck(hipMemcpyWithStream(dst, src, bytes, hipMemcpyHostToDevice, stream_), what);
+  }
+  void launched(const char* what) { hipCheck(hipGetLastError(), what); }
+  std::vector<Level> layout(uint32_t nSeeds, uint32_t depth, uint32_t w0, uint32_t ne0) const;
+  dk::LevelParams levelParams(const Level& L, uint32_t level, uint32_t depth, uint32_t nSeeds) const;
+  void ensurePools();
+  // hull the given jobs (slab offsets are assigned here, in arena-sized chunks)
+  void runHulls(std::vector<dk::HullJob>& hj, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
+                std::vector<dk::HullResultD>* results);
+  void uploadMeshes(const std::vector<MeshRef>& meshes, std::vector<uint32_t>* vOff, std::vector<uint32_t>* tOff);
+
+  TreeConfig cfg_;
+  uint32_t edgeStride_;
+  hipStream_t stream_ = nullptr;
+  DevBuf<float> verts_;
+  DevBuf<uint32_t> tris_;
+  DevBuf<dk::DEdge> edges_;
+  DevBuf<dk::DPart> parts_;
+  DevBuf<dk::DItem> items_;
+  DevBuf<dk::DJob> jobs_;
+  DevBuf<float4> jobPlanes_, edgePlanes_;
+  DevBuf<dk::SeedState> state_;
+  DevBuf<uint32_t> scratch_;
+  DevBuf<dk::Counters> ctr_;
+  DevBuf<uint32_t> arena_;
+  DevBuf<dk::HullJob> hjobs_;
+  DevBuf<dk::HullResultD> hres_;
+  DevBuf<uint32_t> export_;
+  DevBuf<float> hdVerts_, hdSamples_;
+  DevBuf<uint32_t> hdTris_, hdCounts_;
+  DevBuf<dk::HdMesh> hdMeshes_;
+  DevBuf<dk::HdTile> hdTiles_;
+  // last run
+  std::vector<uint32_t> lastLevelCounts_;
+  uint32_t lastPartCount_ = 0;
+};
+
+std::vector<Level> GpuBackend::layout(uint32_t nSeeds, uint32_t depth, uint32_t w0, uint32_t ne0) const {
+  std::vector<Level> levels;
+  uint32_t itemsInBase = 0, nItemsIn = nSeeds, itemsOutBase = nSeeds, jobBase = 0;
+  for (uint32_t l = 0; l < depth; l++) {
+    const uint32_t width = l == 0 ? w0 : cfg_.width2;
+    const uint32_t nEdgeCuts = l == 0 ? ne0 : 0;
+    const uint32_t totalWidth = width + nEdgeCuts;
+    const uint32_t nJobs = nItemsIn * totalWidth;
+    levels.push_back(Level{itemsInBase, nItemsIn, itemsOutBase, jobBase, nSeeds + 2 * jobBase, nJobs, width, nEdgeCuts, totalWidth});
+    itemsInBase = itemsOutBase;
+    nItemsIn = nJobs;
+    itemsOutBase += nJobs;
+    jobBase += nJobs;
+  }
+  return levels;
+}
+
+dk::LevelParams GpuBackend::levelParams(const Level& L, uint32_t level, uint32_t depth, uint32_t nSeeds) const {
+  dk::LevelParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.itemsInBase = L.itemsInBase;
+  P.nItemsIn = L.nItemsIn;
+  P.itemsOutBase = L.itemsOutBase;
+  P.jobBase = L.jobBase;
+  P.partBase = L.partBase;
+  P.nJobs = L.nJobs;
+  P.width = L.width;
+  P.nEdgeCuts = L.nEdgeCuts;
+  P.totalWidth = L.totalWidth;
+  P.level = level;
+  P.depth = depth;
+  P.lastLevel = level == depth - 1 ? 1 : 0;
+  P.nSeeds = nSeeds;
+  P.edgeStride = edgeStride_;
+  P.prune = (level == depth - 1 && level > 0 && cfg_.pruneLastLevel) ? 1 : 0;
+  P.minEdgeDist = cfg_.minEdgeDist;
+  P.vCap = std::min<uint64_t>(cfg_.vertexCap, 0xffffffffull);
+  P.tCap = std::min<uint64_t>(cfg_.triCap, 0xffffffffull);
+  P.eCap = std::min<uint64_t>(cfg_.edgeCap, 0xffffffffull);
+  P.sCap = std::min<uint64_t>(cfg_.scratchWords, 0xffffffffull);
+  return P;
+}
+
+void GpuBackend::ensurePools() {
+  verts_.ensure(std::min<uint64_t>(cfg_.vertexCap, 0xffffffffull));
+  tris_.ensure(std::min<uint64_t>(cfg_.triCap, 0xffffffffull));
+  edges_.ensure(std::min<uint64_t>(cfg_.edgeCap, 0xffffffffull));
+  scratch_.ensure(std::min<uint64_t>(cfg_.scratchWords, 0xffffffffull));
+}
+
+void GpuBackend::runHulls(std::vector<dk::HullJob>& hj, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
+                          std::vector<dk::HullResultD>* results) {
+  if (hj.empty()) return;
+  const uint64_t budgetWords = std::max<uint64_t>(cfg_.hullBudgetBytes / 4, 1ull << 20);
+  uint64_t maxSlab = 0;
+  for (const dk::HullJob& j : hj) maxSlab = std::max(maxSlab, dk::hullSlabWords(j.nv));
+  arena_.ensure(std::max(budgetWords, maxSlab));
+  hjobs_.ensure(hj.size());
+  hres_.ensure(hj.size());
+  size_t start = 0;
+  while (start < hj.size()) {
+    uint64_t used = 0;
+    size_t end = start;
+    while (end < hj.size()) {
+      const uint64_t w = dk::hullSlabWords(hj[end].nv);
+      if (used + w > arena_.n && end > start) break;
+      hj[end].slabOff = used;
+      hj[end].slabWords = (uint32_t)std::min<uint64_t>(w, 0xffffffffull);
+      used += w;
+      end++;
+    }
+    const uint32_t n = (uint32_t)(end - start);
+    h2d(hjobs_.p + start, hj.data() + start, n * sizeof(dk::HullJob), "upload hull jobs");
+    dk::k_hull<<<(n + dk::HW - 1) / dk::HW, 64 * dk::HW, 0, stream_>>>(hjobs_.p + start, n, verts_.p, arena_.p, parts,
+                                                                        hres_.p + start, exportBuf, items_.p, jobs_.p, HL,
+                                                                        ctr_.p);
+    launched("k_hull");
+    start = end;
+  }
+  if (results) {
+    results->resize(hj.size());
+    hipCheck(hipMemcpyAsync(results->data(), hres_.p, hj.size() * sizeof(dk::HullResultD), hipMemcpyDeviceToHost, stream_),
+             "read hull results");
+    sync();
+  }
+}
+
+TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vector<std::vector<float>>& edgePlanes,
+                              const RunOptions& opts) {
+  TreeRunResult R;
+  const uint32_t depth = opts.depth ? opts.depth : cfg_.depth;
+  const uint32_t width0 = opts.width ? opts.width : cfg_.width;
+  const uint32_t nEdge0 = std::min(opts.nEdgeCuts, edgeStride_);
+  if (width0 % 3 != 0 || width0 > cfg_.width)
+    throw std::invalid_argument("width override must be a multiple of 3 and <= configured width");
+  if (depth < 1 || depth > cfg_.depth) throw std::invalid_argument("depth out of range for this tree");
+  const uint32_t nSeeds = (uint32_t)seeds.size();
+  if (nSeeds == 0) return R;
+  const double t0 = nowMs();
+  const bool refine = opts.refine && opts.refine->rounds > 0 && opts.refine->probes >= 3;
+  const uint32_t nProbes = refine ? std::min(opts.refine->probes, edgeStride_) : 0;
+  const std::vector<Level> treeLevels = layout(nSeeds, depth, width0, nEdge0);
+  const std::vector<Level> probeLevels = refine ? layout(nSeeds, depth, 0, nProbes) : std::vector<Level>();
+  size_t totalJobs = 0, probeJobs = 0;
+  for (const Level& L : treeLevels) totalJobs += L.nJobs;
+  for (const Level& L : probeLevels) probeJobs += L.nJobs;
+  totalJobs = std::max(totalJobs, probeJobs);
+  ensurePools();
+  parts_.ensure(nSeeds + 2 * totalJobs);
+  items_.ensure(nSeeds + totalJobs);
+  jobs_.ensure(totalJobs);
+  jobPlanes_.ensure(totalJobs);
+  edgePlanes_.ensure((size_t)nSeeds * edgeStride_);
+  state_.ensure(nSeeds);
+  // ---- seeds: one packed upload each for vertices, triangles, parts, items
+  std::vector<float> pv;
+  std::vector<uint32_t> pt;
+  std::vector<dk::DPart> sp(nSeeds);
+  std::vector<dk::DItem> si(nSeeds);
+  for (uint32_t k = 0; k < nSeeds; k++) {
+    dk::DPart& p = sp[k];
+    std::memset(&p, 0, sizeof(p));
+    p.vOff = (uint32_t)pv.size();
+    p.nv = seeds[k].nv;
+    p.tOff = (uint32_t)pt.size();
+    p.nt = seeds[k].nt;
+    for (int q = 0; q < 4; q++) p.planeIds[q] = dk::NONE;
+    p.flags = seeds[k].nt == 0 ? 2u : 0u;
+    pv.insert(pv.end(), seeds[k].verts, seeds[k].verts + 3ull * seeds[k].nv);
+    if (seeds[k].nt) pt.insert(pt.end(), seeds[k].tris, seeds[k].tris + 3ull * seeds[k].nt);
+    dk::DItem& it = si[k];
+    std::memset(&it, 0, sizeof(it));
+    it.partIds[0] = k;
+    it.np = 1;
+    it.srcPart = k;
+    it.initialCut = dk::NONE;
+    it.flags = 1;
+    it.parentItem = dk::NONE;
+  }
+  if (pv.size() > verts_.n || pt.size() > tris_.n) throw std::length_error("seed meshes exceed pool capacity");
+  h2d(verts_.p, pv.data(), pv.size() * 4, "upload verts");
+  if (!pt.empty()) h2d(tris_.p, pt.data(), pt.size() * 4, "upload tris");
+  h2d(parts_.p, sp.data(), nSeeds * sizeof(dk::DPart), "upload parts");
+  h2d(items_.p, si.data(), nSeeds * sizeof(dk::DItem), "upload items");
+  std::vector<float4> ep((size_t)nSeeds * edgeStride_, make_float4(0.f, 0.f, 0.f, 0.f));
+  for (uint32_t k = 0; k < nSeeds && k < edgePlanes.size(); k++) {
+    const size_t n = std::min<size_t>(edgePlanes[k].size() / 4, edgeStride_);
+    for (size_t i = 0; i < n; i++) {
+      const float* e = &edgePlanes[k][4 * i];
+      ep[(size_t)k * edgeStride_ + i] = make_float4(e[0], e[1], e[2], e[3]);
+    }
+  }
+  h2d(edgePlanes_.p, ep.data(), ep.size() * sizeof(float4), "upload edge planes");
+  std::vector<dk::SeedState> st(nSeeds);
+  for (uint32_t k = 0; k < nSeeds; k++) {
+    std::memset(&st[k], 0, sizeof(st[k]));
+    st[k].cut = dk::NONE;
+    st[k].spacing = refine && k < opts.refine->spacing.size() ? opts.refine->spacing[k] : 0.0f;
+  }
+  h2d(state_.p, st.data(), nSeeds * sizeof(dk::SeedState), "upload seed state");
+  dk::Counters c0;
+  std::memset(&c0, 0, sizeof(c0));
+  c0.vTop = (uint32_t)pv.size();
+  c0.tTop = (uint32_t)pt.size();
+  h2d(ctr_.p, &c0, offsetof(dk::Counters, hd), "init counters");
+  // ---- stages: the tree, then each refinement round over the probe layout
+  const uint32_t stages = 1 + (refine ? opts.refine->rounds : 0);
+  std::vector<dk::DPart> hostParts;
+  for (uint32_t s = 0; s < stages; s++) {
+    const std::vector<Level>& levels = s == 0 ? treeLevels : probeLevels;
+    if (s > 0)  // pool pointers back to the seeds; flags and counts accumulate
+      h2d(ctr_.p, &c0, 4 * sizeof(uint32_t), "reset counters");
+    for (uint32_t l = 0; l < depth; l++) {
+      const Level& L = levels[l];
+      const dk::LevelParams P = levelParams(L, l, depth, nSeeds);
+      if (L.nJobs == 0) continue;
+      dk::k_plan<<<L.nJobs, dk::BT, 0, stream_>>>(P, items_.p, parts_.p, jobs_.p, jobPlanes_.p, edgePlanes_.p, verts_.p,
+                                                  tris_.p, edges_.p, scratch_.p, ctr_.p);
+      launched("k_plan");
+      dk::k_scan<<<1, dk::ST, 0, stream_>>>(P, items_.p, parts_.p, jobs_.p, ctr_.p);
+      launched("k_scan");
+      dk::k_emit<<<L.nJobs, dk::BT, 0, stream_>>>(P, parts_.p, jobs_.p, jobPlanes_.p, verts_.p, tris_.p, edges_.p,
+                                                  scratch_.p);
+      launched("k_emit");
+      // hull jobs for every child with enough points
+      const uint32_t nChild = 2 * L.nJobs;
+      hostParts.resize(nChild);
+      hipCheck(hipMemcpyAsync(hostParts.data(), parts_.p + L.partBase, nChild * sizeof(dk::DPart), hipMemcpyDeviceToHost,
+                              stream_),
+               "read child parts");
+      sync();
+      std::vector<dk::HullJob> hj;
+      for (uint32_t c = 0; c < nChild; c++) {
+        const dk::DPart& p = hostParts[c];
+        if (!(p.nv >= 4 && (p.nt > 0 || (p.flags & 2u)))) continue;
+        dk::HullJob J;
+        J.part = L.partBase + c;
+        J.vOff = p.vOff;
+        J.nv = p.nv;
+        J.exportOff = dk::NONE;
+        J.slabOff = 0;
+        J.slabWords = 0;
+        J.job = L.jobBase + c / 2;
+        hj.push_back(J);
+      }
+      dk::HullLevel HL;
+      HL.prune = P.prune && nSeeds > 0;
+      HL.itemsInBase = L.itemsInBase;
+      HL.nItemsIn = L.nItemsIn;
+      HL.nSeeds = nSeeds;
+      HL.depth = depth;
+      HL.jobBase = L.jobBase;
+      runHulls(hj, parts_.p, nullptr, HL, nullptr);
+      dk::k_record<<<(L.nJobs + 255) / 256, 256, 0, stream_>>>(P, items_.p, parts_.p);
+      launched("k_record");
+    }
+    dk::EvalParams E;
+    std::memset(&E, 0, sizeof(E));
+    E.depth = depth;
+    E.nSeeds = nSeeds;
+    E.w0t = levels[0].totalWidth;
+    E.nProbes = nProbes;
+    E.mode = s == 0 ? 0 : 1;
+    E.jobBase0 = levels[0].jobBase;
+    E.edgeStride = edgeStride_;
+    for (uint32_t l = 0; l < depth; l++) {
+      E.outBase[l] = levels[l].itemsOutBase;
+      E.nJobs[l] = levels[l].nJobs;
+    }
+    dk::k_evaluate<<<nSeeds, dk::BT, 0, stream_>>>(E, items_.p, jobPlanes_.p, edgePlanes_.p, state_.p);
+    launched("k_evaluate");
+    lastLevelCounts_.assign(1, nSeeds);
+    for (const Level& L : levels) lastLevelCounts_.push_back(L.nJobs);
+  }
+  lastPartCount_ = (uint32_t)(nSeeds + 2 * totalJobs);
+  hipCheck(hipMemcpyAsync(st.data(), state_.p, nSeeds * sizeof(dk::SeedState), hipMemcpyDeviceToHost, stream_),
+           "read seed state");
+  dk::Counters c1;
+  hipCheck(hipMemcpyAsync(&c1, ctr_.p, offsetof(dk::Counters, hd), hipMemcpyDeviceToHost, stream_), "read counters");
+  sync();
+  for (uint32_t k = 0; k < nSeeds; k++) {
+    SeedResult r;
+    r.valid = st[k].valid != 0;
+    r.bestCut = r.valid ? (int32_t)st[k].cut : -1;
+    r.bestCost = r.valid ? st[k].cost : 1e300;
+    for (int q = 0; q < 4; q++) r.plane[q] = r.valid ? st[k].plane[q] : 0.f;
+    R.seeds.push_back(r);
+  }
+  R.overflow = c1.overflow != 0;
+  R.hullFail = c1.hullFail != 0;
+  R.hullFailCount = c1.hullFail;
+  R.npOverflow = c1.npOverflow != 0;
+  R.pruned = c1.pruned;
+  R.hullsDone = c1.hullsDone;
+  for (const Level& L : treeLevels) R.jobsPerLevel.push_back(L.nJobs);
+  R.gpuMs = nowMs() - t0;
+  return R;
+}
+
+std::vector<ItemView> GpuBackend::levelItems(uint32_t level) {
+  std::vector<ItemView> out;
+  if (level >= lastLevelCounts_.size()) return out;
+  size_t base = 0;
+  for (uint32_t l = 0; l < level; l++) base += lastLevelCounts_[l];
+  const uint32_t n = lastLevelCounts_[level];
+  std::vector<dk::DItem> h(n);
+  if (n) {
+    hipCheck(hipMemcpyAsync(h.data(), items_.p + base, n * sizeof(dk::DItem), hipMemcpyDeviceToHost, stream_), "read items");
+    sync();
+  }
+  for (const dk::DItem& it : h) {
+    ItemView v;
+    for (int k = 0; k < 8; k++) v.partIds[k] = it.partIds[k];
+    v.np = it.np;
+    v.srcPart = it.srcPart;
+    v.initialCut = it.initialCut;
+    v.nLevels = it.nLevels;
+    for (int k = 0; k < 4; k++) v.levelCosts[k] = it.levelCosts[k];
+    v.flags = it.flags;
+    v.parentItem = it.parentItem;
+    v.cutIdx = it.cutIdx;
+    out.push_back(v);
+  }
+  return out;
+}
+
+bool GpuBackend::readPart(uint32_t partIdx, PartView* pv, Mesh* mesh) {
+  if (partIdx >= lastPartCount_) return false;
+  dk::DPart p;
+  hipCheck(hipMemcpyAsync(&p, parts_.p + partIdx, sizeof(p), hipMemcpyDeviceToHost, stream_), "read part");
+  sync();
+  pv->nv = p.nv;
+  pv->nt = p.nt;
+  pv->ne = p.ne;
+  pv->meshVol = p.meshVol;
+  pv->hullVol = p.hullVol;
+  for (int k = 0; k < 4; k++) pv->planeIds[k] = p.planeIds[k];
+  pv->nPlanes = p.nPlanes;
+  pv->flags = p.flags;
+  pv->nCand = p.nCand;
+  pv->nIns = p.nIns;
+  if (mesh) {
+    mesh->verts.resize(3ull * p.nv);
+    mesh->tris.resize(3ull * p.nt);
+    if (p.nv)
+      hipCheck(hipMemcpyAsync(mesh->verts.data(), verts_.p + p.vOff, 12ull * p.nv, hipMemcpyDeviceToHost, stream_),
+               "read part verts");
+    if (p.nt)
+      hipCheck(hipMemcpyAsync(mesh->tris.data(), tris_.p + p.tOff, 12ull * p.nt, hipMemcpyDeviceToHost, stream_),
+               "read part tris");
+    sync();
+  }
+  return true;
+}
+
+void GpuBackend::uploadMeshes(const std::vector<MeshRef>& meshes, std::vector<uint32_t>* vOff, std::vector<uint32_t>* tOff) {
+  std::vector<float> pv;
+  std::vector<uint32_t> pt;
+  for (const MeshRef& m : meshes) {
+    vOff->push_back((uint32_t)pv.size());
+    tOff->push_back((uint32_t)pt.size());
+    pv.insert(pv.end(), m.verts, m.verts + 3ull * m.nv);
+    if (m.nt) pt.insert(pt.end(), m.tris, m.tris + 3ull * m.nt);
+  }
+  verts_.ensure(pv.size());
+  tris_.ensure(pt.size());
+  if (!pv.empty()) h2d(verts_.p, pv.data(), pv.size() * 4, "upload verts");
+  if (!pt.empty()) h2d(tris_.p, pt.data(), pt.size() * 4, "upload tris");
+}
+
+std::vector<HullResult> GpuBackend::hullMeshes(const std::vector<MeshRef>& meshes) {
+  std::vector<HullResult> out(meshes.size());
+  if (meshes.empty()) return out;
+  lastLevelCounts_.clear();
+  lastPartCount_ = 0;
+  std::vector<uint32_t> vOff, tOff;
+  uploadMeshes(meshes, &vOff, &tOff);
+  std::vector<dk::HullJob> hj(meshes.size());
+  uint64_t exportTotal = 0;
+  std::vector<uint64_t> exOff(meshes.size());
+  for (size_t k = 0; k < meshes.size(); k++) {
+    exOff[k] = exportTotal;
+    exportTotal += 6ull * std::max<uint32_t>(meshes[k].nv, 4);  // a hull over n points has < 2n faces
+  }
+  if (exportTotal > 0xffffffffull) throw std::length_error("hull export exceeds triangle pool");
+  export_.ensure(exportTotal);
+  for (size_t k = 0; k < meshes.size(); k++) {
+    dk::HullJob& J = hj[k];
+    J.part = (uint32_t)k;
+    J.vOff = vOff[k];
+    J.nv = meshes[k].nv;
+    J.exportOff = (uint32_t)exOff[k];
+    J.slabOff = 0;
+    J.slabWords = 0;
+    J.job = dk::NONE;
+  }
+  dk::HullLevel HL;
+  std::memset(&HL, 0, sizeof(HL));
+  std::vector<dk::HullResultD> res;
+  runHulls(hj, nullptr, export_.p, HL, &res);
+  std::vector<uint32_t> faces(exportTotal);
+  hipCheck(hipMemcpyAsync(faces.data(), export_.p, exportTotal * 4, hipMemcpyDeviceToHost, stream_), "read hull faces");
+  sync();
+  for (size_t k = 0; k < meshes.size(); k++) {
+    HullResult& h = out[k];
+    h.ok = res[k].status <= 1;
+    h.failCode = res[k].status == 3 ? 3 : 0;
+    h.volume = h.ok ? res[k].volume : -1.0;
+    if (h.ok) h.tris.assign(faces.begin() + exOff[k], faces.begin() + exOff[k] + 3ull * res[k].nFaces);
+  }
+  return out;
+}
+
+std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b) {
+  const size_t n = a.size();
+  std::vector<double> out(n, 0.0);
+  if (n == 0) return out;
+  if (n > 1024) throw std::invalid_argument("hausdorff: at most 1024 pairs per call");
+  // meshes 0..n-1 = A, n..2n-1 = B; counts and sample offsets from the host rule
+  std::vector<MeshRef> all(a);
+  all.insert(all.end(), b.begin(), b.end());
+  std::vector<float> pv;
+  std::vector<uint32_t> pt, counts;
+  std::vector<dk::HdMesh> hm(all.size());
+  uint64_t sTotal = 0, tTotal = 0;
+  for (size_t k = 0; k < all.size(); k++) {
+    const MeshRef& m = all[k];
+    dk::HdMesh& M = hm[k];
+    M.vOff = (uint32_t)pv.size();
+    M.tOff = (uint32_t)pt.size();
+    M.nt = m.nt;
+    M.gTri = (uint32_t)tTotal;
+    M.sOff = (uint32_t)sTotal;
+    M.cOff = (uint32_t)counts.size();
+    pv.insert(pv.end(), m.verts, m.verts + 3ull * m.nv);
+    if (m.nt) pt.insert(pt.end(), m.tris, m.tris + 3ull * m.nt);
+    const std::vector<uint32_t> off = m.nt ? sampleOffsets(m, cfg_.hausdorffSamples) : std::vector<uint32_t>(1, 0);
+    counts.insert(counts.end(), off.begin(), off.end());
+    sTotal += off.back();
+    tTotal += m.nt;
+  }
+  if (sTotal > 0x7fffffffull) throw std::length_error("hausdorff sample scratch exceeds capacity");
+  hdVerts_.ensure(pv.size());
+  hdTris_.ensure(pt.size());
+  hdCounts_.ensure(counts.size());
+  hdMeshes_.ensure(hm.size());
+  hdSamples_.ensure(3 * sTotal);
+  h2d(hdVerts_.p, pv.data(), pv.size() * 4, "upload hd verts");
+  if (!pt.empty()) h2d(hdTris_.p, pt.data(), pt.size() * 4, "upload hd tris");
+  h2d(hdCounts_.p, counts.data(), counts.size() * 4, "upload counts");
+  h2d(hdMeshes_.p, hm.data(), hm.size() * sizeof(dk::HdMesh), "upload hd meshes");
+  if (tTotal)
+    dk::k_hd_samples<<<(uint32_t)((tTotal + 255) / 256), 256, 0, stream_>>>(hdMeshes_.p, (uint32_t)hm.size(), (uint32_t)tTotal,
+                                                                          hdVerts_.p, hdTris_.p, hdCounts_.p, hdSamples_.p);
+  launched("k_hd_samples");
+  std::vector<dk::HdTile> tiles;
+  for (size_t p = 0; p < n; p++) {
+    if (a[p].nt == 0 || b[p].nt == 0) continue;  // src/cpu/hausdorff.ts:227
+    for (int dir = 0; dir < 2; dir++) {
+      const dk::HdMesh& S = hm[dir == 0 ? p : n + p];
+      const dk::HdMesh& O = hm[dir == 0 ? n + p : p];
+      const uint32_t ns = counts[S.cOff + S.nt];
+      for (uint32_t s0 = 0; s0 < ns; s0 += dk::HDT) {
+        dk::HdTile t;
+        t.pair = (uint32_t)p;
+        t.sBegin = S.sOff + s0;
+        t.sCount = std::min<uint32_t>(dk::HDT, ns - s0);
+        t.vOff = O.vOff;
+        t.tOff = O.tOff;
+        t.nt = O.nt;
+        tiles.push_back(t);
+      }
+    }
+  }
+  hipCheck(hipMemsetAsync(ctr_.p->hd, 0, n * 4, stream_), "clear hd results");
+  if (!tiles.empty()) {
+    hdTiles_.ensure(tiles.size());
+    h2d(hdTiles_.p, tiles.data(), tiles.size() * sizeof(dk::HdTile), "upload tiles");
+    dk::k_hd_dist<<<(uint32_t)tiles.size(), dk::HDT, 0, stream_>>>(hdTiles_.p, (uint32_t)tiles.size(), hdSamples_.p,
+                                                                  hdVerts_.p, hdTris_.p, ctr_.p->hd);
+    launched("k_hd_dist");
+  }
+  std::vector<uint32_t> bits(n);
+  hipCheck(hipMemcpyAsync(bits.data(), ctr_.p->hd, n * 4, hipMemcpyDeviceToHost, stream_), "read hd results");
+  sync();
+  for (size_t p = 0; p < n; p++) {
+    float f;
+    std::memcpy(&f, &bits[p], 4);
+    out[p] = std::sqrt((double)f);
+  }
+  return out;
+}
+
+std::vector<HullResult> GpuBackend::hullsWithHausdorff(const std::vector<MeshRef>& meshes, std::vector<double>* hd) {
+  std::vector<HullResult> out = hullMeshes(meshes);
+  std::vector<MeshRef> hulls;
+  for (size_t k = 0; k < meshes.size(); k++)
+    hulls.push_back(MeshRef{meshes[k].verts, meshes[k].nv, out[k].tris.data(), (uint32_t)(out[k].tris.size() / 3)});
+  hd->assign(meshes.size(), 0.0);
+  for (size_t s = 0; s < meshes.size(); s += 1024) {
+    const size_t e = std::min(meshes.size(), s + 1024);
+    const std::vector<double> r = hausdorff(std::vector<MeshRef>(meshes.begin() + s, meshes.begin() + e),
+                                            std::vector<MeshRef>(hulls.begin() + s, hulls.begin() + e));
+    for (size_t k = s; k < e; k++) (*hd)[k] = r[k - s];
+  }
+  return out;
+}
+
+}  // namespace cuacd
+
+#define CUACD_API(name) cuacd_##name
+#define CUACD_MAKE_BACKEND(cfg) (new cuacd::GpuBackend(cfg))
+#include "../host/capi_body.inc"

@@ -1,0 +1,136 @@
+// backend.hpp — the operations the decomposer and the merge pass need from the
+// lookahead engine, mirroring the GpuTree class (src/gpu/tree.ts:116-816).
+//
+// The product implements this over HIP kernels; the same host code can be
+// driven by any other implementation of the interface.
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mesh.hpp"
+
+namespace cuacd {
+
+// src/gpu/tree.ts:14-46 TreeConfig / DEFAULT_TREE_CONFIG
+struct TreeConfig {
+  uint32_t width = 60;
+  uint32_t width2 = 5;
+  uint32_t depth = 2;
+  uint32_t nConcaveEdges = 32;
+  float minEdgeDist = 0.0125f;
+  // Pool capacities, sized for one 288 GB MI355X rather than the reference's
+  // WebGPU limits (64M floats / 64M u32 / 1M edges / 48M words / 256 MB).
+  uint64_t vertexCap = 512ull << 20;     // floats
+  uint64_t triCap = 512ull << 20;        // u32
+  uint64_t edgeCap = 16ull << 20;        // edges (8 words each)
+  uint64_t scratchWords = 256ull << 20;  // plan/emit side flags, 2 words per parent vertex per job
+  uint64_t hullBudgetBytes = 4ull << 30;  // in-flight hull slabs
+  uint32_t hullMaxVerts = 4096;
+  bool pruneLastLevel = true;
+  // Hausdorff samples per mesh: 0 = the reference rule max(1000, floor(2000 * area)),
+  // N > 0 = exactly the same distribution rule with res = N.
+  uint32_t hausdorffSamples = 0;
+  int device = 0;  // HIP device ordinal
+};
+
+// A HIP runtime failure (reported as CUACD_ERR_DEVICE at the C ABI).
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// Planes per seed available for edge/probe cuts (src/gpu/tree.ts:145).
+inline uint32_t edgeStrideOf(const TreeConfig& c) {
+  const uint32_t s = 4 * c.nConcaveEdges;
+  return s > 32 ? s : 32;
+}
+
+struct MeshRef {
+  const float* verts;
+  uint32_t nv;
+  const uint32_t* tris;
+  uint32_t nt;
+};
+inline MeshRef refOf(const Mesh& m) { return MeshRef{m.verts.data(), m.nv(), m.tris.data(), m.nt()}; }
+
+// src/gpu/tree.ts:60-67 RefineRequest
+struct RefineRequest {
+  uint32_t rounds = 0;
+  uint32_t probes = 0;
+  std::vector<float> spacing;  // per seed initial half-width (0 disables)
+};
+
+struct RunOptions {
+  uint32_t depth = 0;                // 0 = configured depth
+  uint32_t width = 0;                // 0 = configured width
+  uint32_t nEdgeCuts = 0xffffffffu;  // capped at the plane slots
+  const RefineRequest* refine = nullptr;
+};
+
+// src/gpu/tree.ts:69-73 SeedResult (bestCost in FP64)
+struct SeedResult {
+  int32_t bestCut = -1;
+  double bestCost = 1e300;
+  float plane[4] = {0, 0, 0, 0};
+  bool valid = false;
+};
+
+// src/gpu/tree.ts:48-58 ItemView (level costs in FP64)
+struct ItemView {
+  uint32_t partIds[8];
+  uint32_t np, srcPart, initialCut, nLevels;
+  double levelCosts[4];
+  uint32_t flags, parentItem, cutIdx;
+};
+
+// Part descriptor as read back by readPart (src/gpu/tree.ts:764-778).
+struct PartView {
+  uint32_t nv, nt, ne;
+  double meshVol, hullVol;
+  uint32_t planeIds[4];
+  uint32_t nPlanes, flags;
+  uint32_t nCand;  // points outside the rough hull (pad0 in the reference)
+  uint32_t nIns;   // hull insertions (pad1 in the reference)
+};
+
+// src/gpu/tree.ts:75-85 TreeRunResult
+struct TreeRunResult {
+  std::vector<SeedResult> seeds;
+  bool overflow = false;
+  bool hullFail = false;
+  uint32_t hullFailCount = 0;
+  bool npOverflow = false;
+  uint32_t pruned = 0;
+  std::vector<uint32_t> jobsPerLevel;
+  uint64_t hullsDone = 0;  // child hulls completed (exact or pruned), all stages
+  double gpuMs = 0;
+};
+
+struct HullResult {
+  bool ok = false;
+  double volume = 0;
+  std::vector<uint32_t> tris;  // outward triangles over the mesh's own vertex indices
+  uint32_t failCode = 0;
+};
+
+class Backend {
+ public:
+  virtual ~Backend() {}
+  virtual const TreeConfig& config() const = 0;
+  virtual uint32_t planeSlots() const = 0;
+  // src/gpu/tree.ts:449 run(seeds, edgePlanes, opts)
+  virtual TreeRunResult run(const std::vector<MeshRef>& seeds, const std::vector<std::vector<float>>& edgePlanes,
+                            const RunOptions& opts) = 0;
+  // src/gpu/tree.ts:382 hullMeshes (nt == 0 means a point set)
+  virtual std::vector<HullResult> hullMeshes(const std::vector<MeshRef>& meshes) = 0;
+  // src/gpu/tree.ts:318 hausdorff(pairs); a[k] vs b[k]
+  virtual std::vector<double> hausdorff(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b) = 0;
+  // src/gpu/tree.ts:671 hullsWithHausdorff: hull of each mesh and Hausdorff(mesh, hull)
+  virtual std::vector<HullResult> hullsWithHausdorff(const std::vector<MeshRef>& meshes, std::vector<double>* hd) = 0;
+  // last run's items of one level (0 = seeds) and part descriptors
+  virtual std::vector<ItemView> levelItems(uint32_t level) = 0;
+  virtual bool readPart(uint32_t partIdx, PartView* pv, Mesh* mesh) = 0;
+};
+
+}  // namespace cuacd

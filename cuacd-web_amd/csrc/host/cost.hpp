@@ -1,0 +1,49 @@
+// cost.hpp — the rv cost model (src/part.ts:7-29), evaluated so that the host,
+// the HIP kernels and any CPU restatement produce the same bits.
+//
+// rv = K_RV * cbrt(3/(4 pi) * max(hullVol - meshVol, 0)) in FP64. The cube root
+// is a fixed Newton iteration on IEEE +,-,*,/ only: library cbrt routines
+// differ in the last ulp between the host libm and the device math library,
+// and a one-ulp difference is enough to flip an argmin between equal-looking
+// candidate cuts.
+#pragma once
+#include <cstdint>
+#include <cstring>
+
+#if defined(__HIPCC__)
+#define CUACD_HD __host__ __device__
+#else
+#define CUACD_HD
+#endif
+
+namespace cuacd {
+
+constexpr double K_RV = 0.3;                               // src/part.ts:7
+constexpr double THREE_OVER_4PI = 0.23873241463784300365;  // 3 / (4 pi)
+
+// Cube root of x >= 0 (0 for x <= 0 or tiny x): start at 2^floor(e/3) <= cbrt(x)
+// and run 12 Newton steps; the result is within an ulp or two of cbrt(x).
+CUACD_HD inline double det_cbrt(double x) {
+  if (!(x > 1e-300)) return 0.0;
+  uint64_t b;
+  __builtin_memcpy(&b, &x, 8);
+  const int e = (int)((b >> 52) & 0x7ff) - 1023;
+  const int k = e >= 0 ? e / 3 : -((-e + 2) / 3);
+  const uint64_t yb = (uint64_t)(k + 1023) << 52;
+  double y;
+  __builtin_memcpy(&y, &yb, 8);
+  for (int i = 0; i < 12; i++) {
+    const double y2 = y * y;
+    y = y - (y2 * y - x) / (3.0 * y2);
+  }
+  return y;
+}
+
+// src/part.ts:24-26 rvCost
+CUACD_HD inline double rv_cost(double meshVol, double hullVol) {
+  double gap = hullVol - meshVol;
+  if (!(gap > 0.0)) gap = 0.0;
+  return K_RV * det_cbrt(THREE_OVER_4PI * gap);
+}
+
+}  // namespace cuacd

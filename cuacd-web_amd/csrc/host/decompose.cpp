@@ -1,0 +1,561 @@
+// decompose.cpp — iteration driver and merge pass (host). Restates
+// src/decompose.ts:128-398 and src/merge.ts:78-303.
+#include "decompose.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <set>
+#include <unordered_map>
+
+#include "concave.hpp"
+#include "cut.hpp"
+
+namespace cuacd {
+
+namespace {
+double nowMs() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+TreeConfig treeConfigFor(const DecomposeOptions& o) {
+  TreeConfig c;
+  c.width = o.width;
+  c.width2 = o.width2;
+  c.depth = o.depth;
+  c.nConcaveEdges = o.nConcaveEdges;
+  c.minEdgeDist = (float)std::min(o.threshold / 4, 0.015);
+  if (o.vertexCap) c.vertexCap = o.vertexCap;
+  if (o.triCap) c.triCap = o.triCap;
+  if (o.edgeCap) c.edgeCap = o.edgeCap;
+  if (o.scratchWords) c.scratchWords = o.scratchWords;
+  if (o.hullBudgetBytes) c.hullBudgetBytes = o.hullBudgetBytes;
+  if (o.hullMaxVerts) c.hullMaxVerts = o.hullMaxVerts;
+  if (o.pruneLastLevel >= 0) c.pruneLastLevel = o.pruneLastLevel != 0;
+  c.hausdorffSamples = o.hausdorffSamples;
+  return c;
+}
+
+Mesh compactHull(const float* hv, const std::vector<uint32_t>& tris, double* volume) {
+  std::unordered_map<uint32_t, uint32_t> remap;
+  Mesh m;
+  m.tris.resize(tris.size());
+  for (size_t i = 0; i < tris.size(); i++) {
+    const uint32_t v = tris[i];
+    auto it = remap.find(v);
+    uint32_t r;
+    if (it == remap.end()) {
+      r = m.nv();
+      remap.emplace(v, r);
+      m.verts.push_back(hv[3 * (size_t)v]);
+      m.verts.push_back(hv[3 * (size_t)v + 1]);
+      m.verts.push_back(hv[3 * (size_t)v + 2]);
+    } else {
+      r = it->second;
+    }
+    m.tris[i] = r;
+  }
+  const double sv = signedVolume(m);
+  if (sv < 0)
+    for (size_t i = 0; i + 2 < m.tris.size(); i += 3) std::swap(m.tris[i + 1], m.tris[i + 2]);
+  if (volume) *volume = std::fabs(sv);
+  return m;
+}
+
+// ---- merge pass ------------------------------------------------------------
+
+namespace {
+
+double bboxGap2(const Box3& a, const Box3& b) {
+  double g2 = 0;
+  for (int k = 0; k < 3; k++) {
+    const double g = std::max(0.0, std::max((double)a.lo[k] - b.hi[k], (double)b.lo[k] - a.hi[k]));
+    g2 += g * g;
+  }
+  return g2;
+}
+
+bool triCoplanar(const std::vector<float>& V, const std::vector<uint32_t>& T, size_t t, const Plane& p, double tol) {
+  for (int e = 0; e < 3; e++) {
+    const size_t i = 3 * (size_t)T[t + e];
+    if (std::fabs(p[0] * V[i] + p[1] * V[i + 1] + p[2] * V[i + 2] + p[3]) > tol) return false;
+  }
+  return true;
+}
+
+double coplanarArea(const Mesh& m, const Plane& p, double tol) {
+  const auto& V = m.verts;
+  const auto& T = m.tris;
+  double area = 0;
+  for (size_t t = 0; t + 2 < T.size(); t += 3) {
+    if (!triCoplanar(V, T, t, p, tol)) continue;
+    const size_t a = 3 * (size_t)T[t], b = 3 * (size_t)T[t + 1], c = 3 * (size_t)T[t + 2];
+    const double abx = (double)V[b] - V[a], aby = (double)V[b + 1] - V[a + 1], abz = (double)V[b + 2] - V[a + 2];
+    const double acx = (double)V[c] - V[a], acy = (double)V[c + 1] - V[a + 1], acz = (double)V[c + 2] - V[a + 2];
+    const double x = aby * acz - abz * acy, y = abz * acx - abx * acz, z = abx * acy - aby * acx;
+    area += 0.5 * std::sqrt(x * x + y * y + z * z);
+  }
+  return area;
+}
+
+inline double jsRound(double x) { return std::floor(x + 0.5); }
+
+struct PairEval {
+  double cost = 0;
+  bool hasHull = false;
+  Mesh hull;
+  double hullVol = 0;
+  double hd = 0;
+  bool hasSeam = false;
+  Plane seam{0, 0, 0, 0};
+};
+
+}  // namespace
+
+bool seamPlane(const Part& a, const Part& b, double tol, Plane* out) {
+  const auto& hv = a.hull.verts;
+  const auto& ht = a.hull.tris;
+  const auto& bv = b.hull.verts;
+  std::set<std::array<double, 4>> seen;
+  bool found = false;
+  double bestArea = 0;
+  for (size_t t = 0; t + 2 < ht.size(); t += 3) {
+    const size_t i0 = 3 * (size_t)ht[t], i1 = 3 * (size_t)ht[t + 1], i2 = 3 * (size_t)ht[t + 2];
+    const double e1x = (double)hv[i1] - hv[i0], e1y = (double)hv[i1 + 1] - hv[i0 + 1], e1z = (double)hv[i1 + 2] - hv[i0 + 2];
+    const double e2x = (double)hv[i2] - hv[i0], e2y = (double)hv[i2 + 1] - hv[i0 + 1], e2z = (double)hv[i2 + 2] - hv[i0 + 2];
+    double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const double len = std::sqrt(nx * nx + ny * ny + nz * nz);
+    if (len < 1e-20) continue;
+    nx /= len;
+    ny /= len;
+    nz /= len;
+    const double d = nx * hv[i0] + ny * hv[i0 + 1] + nz * hv[i0 + 2];
+    const std::array<double, 4> key{jsRound(nx * 2000), jsRound(ny * 2000), jsRound(nz * 2000), jsRound(d * 2000)};
+    if (!seen.insert(key).second) continue;
+    bool ok = true;
+    for (size_t j = 0; j + 2 < bv.size(); j += 3)
+      if (nx * bv[j] + ny * bv[j + 1] + nz * bv[j + 2] - d < -tol) {
+        ok = false;
+        break;
+      }
+    if (!ok) continue;
+    const Plane p{nx, ny, nz, -d};
+    const double area = coplanarArea(a.mesh, p, tol) + coplanarArea(b.mesh, p, tol);
+    if (area > bestArea) {
+      bestArea = area;
+      *out = p;
+      found = true;
+    }
+  }
+  return found;
+}
+
+Mesh filteredConcat(const Mesh& a, const Mesh& b, const std::vector<Plane>& planes, double tol) {
+  Mesh m = mergeMeshes(a, b);
+  if (planes.empty()) return m;
+  std::vector<uint32_t> out;
+  out.reserve(m.tris.size());
+  for (size_t t = 0; t + 2 < m.tris.size(); t += 3) {
+    bool drop = false;
+    for (const Plane& p : planes)
+      if (triCoplanar(m.verts, m.tris, t, p, tol)) {
+        drop = true;
+        break;
+      }
+    if (drop) continue;
+    out.push_back(m.tris[t]);
+    out.push_back(m.tris[t + 1]);
+    out.push_back(m.tris[t + 2]);
+  }
+  m.tris.swap(out);
+  return m;
+}
+
+std::vector<Part> mergeHulls(Backend& be, const std::vector<Part>& input, const MergeOptions& o, MergeStats* statsOut) {
+  const double t0 = nowMs();
+  MergeStats stats;
+  std::vector<Part> parts = input;
+  std::unordered_map<uint64_t, Box3> boxes;
+  uint64_t nextId = 1u << 30;
+  for (Part& p : parts)
+    if (p.uid == 0) p.uid = nextId++;
+  auto boxOf = [&](const Part& p) -> const Box3& {
+    auto it = boxes.find(p.uid);
+    if (it == boxes.end()) it = boxes.emplace(p.uid, bbox(p.mesh)).first;
+    return it->second;
+  };
+  std::map<std::pair<uint64_t, uint64_t>, PairEval> cache;
+  auto key = [](const Part& a, const Part& b) {
+    return a.uid < b.uid ? std::make_pair(a.uid, b.uid) : std::make_pair(b.uid, a.uid);
+  };
+  const double halfT2 = (o.threshold / 2) * (o.threshold / 2);
+  for (uint32_t round = 0; round < o.maxRounds && parts.size() > 1; round++) {
+    stats.rounds = round + 1;
+    std::vector<std::pair<uint32_t, uint32_t>> todo, candidates;
+    for (uint32_t i = 0; i < parts.size(); i++) {
+      for (uint32_t j = i + 1; j < parts.size(); j++) {
+        const Part& A = parts[i];
+        const Part& B = parts[j];
+        if (cache.count(key(A, B))) {
+          candidates.push_back({i, j});
+          continue;
+        }
+        if (rv_cost(A.meshVol + B.meshVol, A.hullVol + B.hullVol) >= o.threshold) continue;
+        if (bboxGap2(boxOf(A), boxOf(B)) >= halfT2) continue;
+        todo.push_back({i, j});
+        candidates.push_back({i, j});
+      }
+    }
+    // merged hulls of the two hulls' vertex sets, batched
+    const double th0 = nowMs();
+    std::vector<PairEval> evals(todo.size());
+    std::vector<std::vector<float>> pointSets(todo.size());
+    for (size_t k = 0; k < todo.size(); k++) {
+      const Part& A = parts[todo[k].first];
+      const Part& B = parts[todo[k].second];
+      pointSets[k] = A.hull.verts;
+      pointSets[k].insert(pointSets[k].end(), B.hull.verts.begin(), B.hull.verts.end());
+    }
+    const size_t batch = 1024;
+    for (size_t s = 0; s < todo.size(); s += batch) {
+      const size_t e = std::min(todo.size(), s + batch);
+      std::vector<MeshRef> refs;
+      for (size_t k = s; k < e; k++) refs.push_back(MeshRef{pointSets[k].data(), (uint32_t)(pointSets[k].size() / 3), nullptr, 0});
+      const std::vector<HullResult> hr = be.hullMeshes(refs);
+      for (size_t k = s; k < e; k++) {
+        const HullResult& h = hr[k - s];
+        const Part& A = parts[todo[k].first];
+        const Part& B = parts[todo[k].second];
+        PairEval& ev = evals[k];
+        if (!h.ok) {
+          // a failed merged hull leaves the pair unmerged
+          stats.hullFallbacks++;
+          ev.cost = INFINITY;
+          continue;
+        }
+        ev.hullVol = h.volume;
+        const double rv = rv_cost(A.meshVol + B.meshVol, h.volume);
+        ev.cost = rv;
+        if (rv < o.threshold) {
+          ev.hull = compactHull(pointSets[k].data(), h.tris, nullptr);
+          ev.hasHull = true;
+        }
+      }
+    }
+    stats.hullMs += nowMs() - th0;
+    stats.pairsHulled += (uint32_t)todo.size();
+    // Hausdorff(filtered union, merged hull) for pairs still under the threshold
+    const double thd = nowMs();
+    std::vector<size_t> hdIdx;
+    std::vector<Mesh> hdA;
+    for (size_t k = 0; k < todo.size(); k++) {
+      PairEval& ev = evals[k];
+      if (!ev.hasHull) continue;
+      const Part& A = parts[todo[k].first];
+      const Part& B = parts[todo[k].second];
+      ev.hasSeam = seamPlane(A, B, o.planeTol, &ev.seam);
+      std::vector<Plane> planes;
+      if (ev.hasSeam) planes.push_back(ev.seam);
+      planes.insert(planes.end(), A.seams.begin(), A.seams.end());
+      planes.insert(planes.end(), B.seams.begin(), B.seams.end());
+      Mesh m = filteredConcat(A.mesh, B.mesh, planes, o.planeTol);
+      if (m.tris.empty()) continue;
+      hdIdx.push_back(k);
+      hdA.push_back(std::move(m));
+    }
+    for (size_t s = 0; s < hdIdx.size(); s += batch) {
+      const size_t e = std::min(hdIdx.size(), s + batch);
+      std::vector<MeshRef> ra, rb;
+      for (size_t q = s; q < e; q++) {
+        ra.push_back(refOf(hdA[q]));
+        rb.push_back(refOf(evals[hdIdx[q]].hull));
+      }
+      const std::vector<double> hd = be.hausdorff(ra, rb);
+      for (size_t q = s; q < e; q++) {
+        PairEval& ev = evals[hdIdx[q]];
+        ev.hd = hd[q - s];
+        ev.cost = std::max(ev.cost, ev.hd);
+        if (ev.cost >= o.threshold) ev.hasHull = false;
+      }
+    }
+    stats.hausdorffMs += nowMs() - thd;
+    stats.pairsHausdorff += (uint32_t)hdIdx.size();
+    for (size_t k = 0; k < todo.size(); k++) cache[key(parts[todo[k].first], parts[todo[k].second])] = std::move(evals[k]);
+    // greedy min-cost matching (stable by candidate order on equal costs)
+    struct Ranked {
+      uint32_t i, j;
+      const PairEval* ev;
+    };
+    std::vector<Ranked> ranked;
+    for (const auto& c : candidates) {
+      const PairEval& ev = cache[key(parts[c.first], parts[c.second])];
+      if (ev.cost < o.threshold && ev.hasHull) ranked.push_back({c.first, c.second, &ev});
+    }
+    std::stable_sort(ranked.begin(), ranked.end(), [](const Ranked& x, const Ranked& y) { return x.ev->cost < y.ev->cost; });
+    std::vector<uint8_t> used(parts.size(), 0);
+    std::vector<Part> next;
+    uint32_t merges = 0;
+    for (const Ranked& c : ranked) {
+      if (used[c.i] || used[c.j]) continue;
+      used[c.i] = used[c.j] = 1;
+      const Part& A = parts[c.i];
+      const Part& B = parts[c.j];
+      Part m;
+      m.mesh = mergeMeshes(A.mesh, B.mesh);
+      m.hull = c.ev->hull;
+      m.meshVol = A.meshVol + B.meshVol;
+      m.hullVol = c.ev->hullVol;
+      m.hasHausdorff = true;
+      m.hausdorff = c.ev->hd;
+      m.seams = A.seams;
+      m.seams.insert(m.seams.end(), B.seams.begin(), B.seams.end());
+      if (c.ev->hasSeam) m.seams.push_back(c.ev->seam);
+      m.merged = A.merged + B.merged;
+      m.uid = nextId++;
+      next.push_back(std::move(m));
+      merges++;
+    }
+    if (o.verbose)
+      std::fprintf(stderr, "[merge round %u] parts=%zu candidates=%zu hulled=%zu hausdorff=%zu merged=%u\n", round,
+                   parts.size(), candidates.size(), todo.size(), hdIdx.size(), merges);
+    if (merges == 0) break;
+    for (size_t i = 0; i < parts.size(); i++)
+      if (!used[i]) next.push_back(std::move(parts[i]));
+    parts.swap(next);
+    stats.merges += merges;
+  }
+  stats.totalMs = nowMs() - t0;
+  if (statsOut) *statsOut = stats;
+  return parts;
+}
+
+// ---- decomposer --------------------------------------------------------------
+
+Decomposer::Decomposer(Backend& be, const DecomposeOptions& o) : be_(be), o_(o) {
+  if (o.width % 3 != 0 || o.width2 % 3 != 0) throw std::invalid_argument("width and width2 must be multiples of 3");
+}
+
+void Decomposer::fillHausdorff(std::vector<Part>& parts) {
+  std::vector<size_t> todo;
+  for (size_t i = 0; i < parts.size(); i++)
+    if (!parts[i].hasHausdorff) todo.push_back(i);
+  const size_t batch = 64;
+  for (size_t s = 0; s < todo.size(); s += batch) {
+    const size_t e = std::min(todo.size(), s + batch);
+    std::vector<MeshRef> a, b;
+    for (size_t k = s; k < e; k++) {
+      a.push_back(refOf(parts[todo[k]].mesh));
+      b.push_back(refOf(parts[todo[k]].hull));
+    }
+    const std::vector<double> hd = be_.hausdorff(a, b);
+    for (size_t k = s; k < e; k++) {
+      parts[todo[k]].hausdorff = hd[k - s];
+      parts[todo[k]].hasHausdorff = std::isfinite(hd[k - s]);
+    }
+  }
+}
+
+std::vector<Part> Decomposer::makeParts(const std::vector<Mesh>& meshes) {
+  std::vector<Part> out;
+  const size_t batch = 256;
+  for (size_t s = 0; s < meshes.size(); s += batch) {
+    const size_t e = std::min(meshes.size(), s + batch);
+    std::vector<MeshRef> refs;
+    for (size_t k = s; k < e; k++) refs.push_back(refOf(meshes[k]));
+    std::vector<double> hd;
+    const std::vector<HullResult> hulls = be_.hullsWithHausdorff(refs, &hd);
+    for (size_t k = s; k < e; k++) {
+      const HullResult& h = hulls[k - s];
+      if (!h.ok) throw std::runtime_error("hull kernel failed for a part (code " + std::to_string(h.failCode) + ")");
+      Part p;
+      p.mesh = meshes[k];
+      p.hull = compactHull(meshes[k].verts.data(), h.tris, nullptr);
+      p.meshVol = volume(meshes[k]);
+      p.hullVol = h.volume;
+      p.hasHausdorff = std::isfinite(hd[k - s]);
+      p.hausdorff = p.hasHausdorff ? hd[k - s] : 0;
+      p.uid = nextUid_++;
+      out.push_back(std::move(p));
+    }
+  }
+  return out;
+}
+
+std::vector<float> Decomposer::refineSpacing(const std::vector<const Mesh*>& meshes) const {
+  const double minEdge = std::min(o_.threshold / 4, 0.015);
+  const double cpa = (double)std::max<uint32_t>(1, o_.width / 3);
+  std::vector<float> out;
+  for (const Mesh* m : meshes) {
+    const Box3 b = bbox(*m);
+    double s = INFINITY;
+    for (int a = 0; a < 3; a++) {
+      const double valid = (double)b.hi[a] - (double)b.lo[a] - 2 * minEdge;
+      if (valid > 0) s = std::min(s, valid / (cpa + 1));
+    }
+    out.push_back(std::isfinite(s) ? (float)s : 0.f);
+  }
+  return out;
+}
+
+DecomposeResult Decomposer::decompose(const Mesh& input) {
+  const DecomposeOptions& o = o_;
+  DecomposeResult R;
+  const double tStart = nowMs();
+  if (o.requireClosed && !isClosed(input))
+    throw std::invalid_argument(
+        "input mesh is not closed (open or non-manifold edges); repair or remesh it first, or set requireClosed: false");
+  Mesh mesh = input;
+  if (o.normalize) {
+    mesh = normalize(input, &R.norm);
+    R.normalized = true;
+  }
+  std::vector<Part> parts = makeParts({mesh});
+  DecomposeStats& st = R.stats;
+  for (uint32_t iter = 0; iter < o.maxIters; iter++) {
+    R.iterations = iter + 1;
+    const double tc0 = nowMs();
+    fillHausdorff(parts);
+    st.hausdorffMs += nowMs() - tc0;
+    // candidates ascending by full cost; the last maxNCutting above the threshold
+    struct OC {
+      uint32_t i;
+      double c;
+    };
+    std::vector<OC> order;
+    for (uint32_t i = 0; i < parts.size(); i++) order.push_back({i, partCost(parts[i])});
+    std::stable_sort(order.begin(), order.end(), [](const OC& a, const OC& b) { return a.c < b.c; });
+    std::vector<uint32_t> above;
+    for (const OC& e : order)
+      if (e.c >= o.threshold && !parts[e.i].stuck) above.push_back(e.i);
+    std::vector<uint32_t> cutting(above.begin() + (above.size() > o.maxNCutting ? above.size() - o.maxNCutting : 0),
+                                  above.end());
+    const double worst = order.empty() ? 0 : order.back().c;
+    if (cutting.empty()) break;
+    const double tcc = nowMs();
+    std::vector<std::vector<float>> edgePlanes;
+    for (uint32_t pi : cutting) {
+      if (iter < o.concaveIters && o.nConcaveEdges > 0) {
+        ConcaveOptions co;
+        co.nConcaveEdges = o.nConcaveEdges;
+        co.concaveEps = o.concaveEps;
+        co.concaveThreshold = o.concaveThreshold;
+        co.seed = iter * 7919u + pi;
+        edgePlanes.push_back(concaveEdgePlanes(parts[pi].mesh, co));
+      } else {
+        edgePlanes.emplace_back();
+      }
+    }
+    st.concaveMs += nowMs() - tcc;
+    // tree search, halving the batch on pool overflow
+    std::vector<SeedResult> results(cutting.size());
+    size_t batchN = cutting.size(), start = 0;
+    while (start < cutting.size()) {
+      const size_t e = std::min(cutting.size(), start + batchN);
+      const double tt0 = nowMs();
+      std::vector<MeshRef> seeds;
+      std::vector<const Mesh*> seedMeshes;
+      for (size_t k = start; k < e; k++) {
+        seeds.push_back(refOf(parts[cutting[k]].mesh));
+        seedMeshes.push_back(&parts[cutting[k]].mesh);
+      }
+      const uint32_t nProbes = std::min(o.refineProbes, be_.planeSlots());
+      RefineRequest rr;
+      RunOptions ro;
+      if (o.refineRounds > 0 && nProbes >= 3) {
+        rr.rounds = o.refineRounds;
+        rr.probes = nProbes;
+        rr.spacing = refineSpacing(seedMeshes);
+        ro.refine = &rr;
+      }
+      std::vector<std::vector<float>> ep(edgePlanes.begin() + start, edgePlanes.begin() + e);
+      const TreeRunResult res = be_.run(seeds, ep, ro);
+      st.treeMs += nowMs() - tt0;
+      st.treeRuns++;
+      st.treeGpuMs += res.gpuMs;
+      st.candidateHulls += res.hullsDone;
+      st.pruned += res.pruned;
+      if (o.verbose >= 2)
+        std::fprintf(stderr, "[iter %u] tree run: %zu seeds, hulls %llu, pruned %u, %.1f ms\n", iter, seeds.size(),
+                     (unsigned long long)res.hullsDone, res.pruned, nowMs() - tt0);
+      if (res.overflow) {
+        if (batchN == 1) throw std::runtime_error("GPU pool overflow with a single part; increase tree pool sizes");
+        batchN = std::max<size_t>(1, batchN >> 1);
+        continue;
+      }
+      if (res.npOverflow) throw std::runtime_error("tree depth exceeds LA_MAX_PARTS");
+      if (res.hullFail) st.hullFailRuns++;
+      for (size_t k = start; k < e; k++) results[k] = res.seeds[k - start];
+      start = e;
+    }
+    // apply the winning cuts on the host, then hull all new halves in one batch
+    const double ta0 = nowMs();
+    std::vector<Mesh> halves;
+    std::vector<uint32_t> replaced;
+    for (size_t k = 0; k < cutting.size(); k++) {
+      const uint32_t pi = cutting[k];
+      Part& p = parts[pi];
+      const SeedResult& sr = results[k];
+      CutRecord rec{iter, pi, sr.bestCut, sr.bestCost, {sr.plane[0], sr.plane[1], sr.plane[2], sr.plane[3]}, false};
+      if (!sr.valid) {
+        p.stuck = true;
+        p.stuckReason = "no-plane";
+        R.cuts.push_back(rec);
+        continue;
+      }
+      CutResult r = planeCut(p.mesh, sr.plane);
+      if (!r.ok) {
+        p.stuck = true;
+        p.stuckReason = "cut-failed";
+      } else if (r.pos.verts.size() < 12 || r.neg.verts.size() < 12) {
+        p.stuck = true;
+        p.stuckReason = "degenerate";
+      } else if (o.validateCuts && (!isClosed(r.pos) || !isClosed(r.neg))) {
+        p.stuck = true;
+        p.stuckReason = "not-closed";
+      } else {
+        rec.applied = true;
+        replaced.push_back(pi);
+        halves.push_back(std::move(r.pos));
+        halves.push_back(std::move(r.neg));
+      }
+      R.cuts.push_back(rec);
+    }
+    st.cutMs += nowMs() - ta0;
+    const double th0 = nowMs();
+    std::vector<Part> made = makeParts(halves);
+    st.hullMs += nowMs() - th0;
+    for (size_t k = 0; k < replaced.size(); k++) {
+      parts[replaced[k]] = std::move(made[2 * k]);
+      parts.push_back(std::move(made[2 * k + 1]));
+    }
+    if (o.verbose)
+      std::fprintf(stderr, "[iter %u] parts=%zu cutting=%zu worst=%.4f\n", iter, parts.size(), cutting.size(), worst);
+  }
+  fillHausdorff(parts);
+  if (o.mergeHulls && parts.size() > 1) {
+    const double tm0 = nowMs();
+    MergeOptions mo;
+    mo.threshold = o.threshold;
+    mo.maxRounds = o.mergeRounds;
+    mo.planeTol = o.mergePlaneTol;
+    mo.verbose = o.verbose;
+    parts = mergeHulls(be_, parts, mo, &R.merge);
+    R.merged = true;
+    st.mergeMs = nowMs() - tm0;
+  }
+  if (R.normalized) {
+    for (Part& p : parts) {
+      p.mesh = denormalize(p.mesh, R.norm);
+      p.hull = denormalize(p.hull, R.norm);
+      for (Plane& q : p.seams)
+        q[3] = q[3] * R.norm.scale - (q[0] * R.norm.center[0] + q[1] * R.norm.center[1] + q[2] * R.norm.center[2]);
+    }
+  }
+  R.parts = std::move(parts);
+  st.totalMs = nowMs() - tStart;
+  return R;
+}
+
+}  // namespace cuacd

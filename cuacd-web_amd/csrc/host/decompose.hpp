@@ -1,0 +1,140 @@
+// decompose.hpp — the iteration driver and the merge pass (host), mirroring
+// src/decompose.ts, src/merge.ts and src/part.ts.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "backend.hpp"
+#include "cost.hpp"
+#include "mesh.hpp"
+
+namespace cuacd {
+
+typedef std::array<double, 4> Plane;  // a x + b y + c z + d = 0
+
+// src/part.ts:9-22 Part
+struct Part {
+  Mesh mesh;
+  Mesh hull;
+  double meshVol = 0;
+  double hullVol = 0;
+  bool hasHausdorff = false;
+  double hausdorff = 0;
+  bool stuck = false;
+  std::string stuckReason;  // "no-plane" | "cut-failed" | "not-closed" | "degenerate"
+  std::vector<Plane> seams;
+  uint32_t merged = 1;
+  uint64_t uid = 0;  // identity for the merge pass's pair cache
+};
+
+// src/part.ts:27-29 partCost
+inline double partCost(const Part& p) {
+  const double rv = rv_cost(p.meshVol, p.hullVol);
+  const double hd = p.hasHausdorff ? p.hausdorff : 0.0;
+  return rv > hd ? rv : hd;
+}
+
+// src/decompose.ts:26-108 DecomposeOptions / DEFAULT_OPTIONS (the GPU is the only backend)
+struct DecomposeOptions {
+  double threshold = 0.05;
+  uint32_t maxIters = 100;
+  uint32_t width = 30;
+  uint32_t width2 = 3;
+  uint32_t depth = 2;
+  uint32_t maxNCutting = 16;
+  uint32_t nConcaveEdges = 16;
+  double concaveEps = 0.005;
+  double concaveThreshold = 3.49;
+  uint32_t concaveIters = 10;
+  bool normalize = true;
+  bool requireClosed = true;
+  bool validateCuts = true;
+  uint32_t refineRounds = 2;
+  uint32_t refineProbes = 8;
+  bool mergeHulls = true;
+  uint32_t mergeRounds = 32;
+  double mergePlaneTol = 1e-3;
+  uint32_t verbose = 0;
+  uint32_t hausdorffSamples = 0;  // 0 = reference rule
+  // src/decompose.ts:60 tree overrides (pool capacities etc.); 0 = default
+  uint64_t vertexCap = 0, triCap = 0, edgeCap = 0, scratchWords = 0, hullBudgetBytes = 0;
+  uint32_t hullMaxVerts = 0;
+  int pruneLastLevel = -1;  // -1 = default (on)
+};
+
+// TreeConfig a Decomposer runs with (src/decompose.ts:141-148)
+TreeConfig treeConfigFor(const DecomposeOptions& o);
+
+// src/merge.ts:54-66 MergeStats
+struct MergeStats {
+  uint32_t rounds = 0, merges = 0, pairsHulled = 0, pairsHausdorff = 0, hullFallbacks = 0;
+  double hullMs = 0, hausdorffMs = 0, totalMs = 0;
+};
+
+struct DecomposeStats {
+  double totalMs = 0, hausdorffMs = 0, cutMs = 0, concaveMs = 0, hullMs = 0, treeMs = 0, mergeMs = 0;
+  uint32_t treeRuns = 0, hullFailRuns = 0;
+  uint64_t candidateHulls = 0;  // child hulls completed inside tree runs
+  uint64_t pruned = 0;
+  double treeGpuMs = 0;
+};
+
+// one chosen cut per iteration and cut part (for cut-sequence comparisons)
+struct CutRecord {
+  uint32_t iter;
+  uint32_t part;
+  int32_t bestCut;
+  double bestCost;
+  float plane[4];
+  bool applied;
+};
+
+struct DecomposeResult {
+  std::vector<Part> parts;
+  uint32_t iterations = 0;
+  DecomposeStats stats;
+  bool merged = false;
+  MergeStats merge;
+  bool normalized = false;
+  Normalization norm{1.0, {0, 0, 0}};
+  std::vector<CutRecord> cuts;
+};
+
+// src/merge.ts:25-52 MergeOptions
+struct MergeOptions {
+  double threshold = 0.05;
+  uint32_t maxRounds = 32;
+  double planeTol = 1e-3;
+  uint32_t verbose = 0;
+};
+
+// src/cpu/hull.ts:156-185 compactHull: first-use vertex order, outward by volume sign
+Mesh compactHull(const float* hv, const std::vector<uint32_t>& tris, double* volume);
+
+// src/merge.ts:114-141 seamPlane; false when no separating face carries mesh area
+bool seamPlane(const Part& a, const Part& b, double tol, Plane* out);
+// src/merge.ts:144-157 filteredConcat
+Mesh filteredConcat(const Mesh& a, const Mesh& b, const std::vector<Plane>& planes, double tol);
+// src/merge.ts:170-303 mergeHulls
+std::vector<Part> mergeHulls(Backend& be, const std::vector<Part>& input, const MergeOptions& o, MergeStats* stats);
+
+// src/decompose.ts:128-398 Decomposer
+class Decomposer {
+ public:
+  Decomposer(Backend& be, const DecomposeOptions& o);
+  DecomposeResult decompose(const Mesh& input);
+
+ private:
+  void fillHausdorff(std::vector<Part>& parts);
+  std::vector<Part> makeParts(const std::vector<Mesh>& meshes);
+  std::vector<float> refineSpacing(const std::vector<const Mesh*>& meshes) const;
+  Backend& be_;
+  DecomposeOptions o_;
+  uint64_t nextUid_ = 1;
+};
+
+}  // namespace cuacd

@@ -1,0 +1,149 @@
+"""decompose(mesh, options) and the merge pass (src/decompose.ts, src/merge.ts, src/part.ts)."""
+import ctypes as C
+import math
+from dataclasses import dataclass, field, fields
+
+import numpy as np
+
+from . import _ffi as F
+from ._lib import load_product, check
+from .gputree import _batch
+from .mesh import Mesh
+
+K_RV = 0.3  # src/part.ts:7
+STUCK_REASONS = {0: None, 1: "no-plane", 2: "cut-failed", 3: "not-closed", 4: "degenerate"}
+
+
+def rv_cost(mesh_vol, hull_vol):
+    """src/part.ts:24-26 (the library evaluates it with a deterministic cube root)."""
+    return K_RV * math.pow((3 / (4 * math.pi)) * max(hull_vol - mesh_vol, 0.0), 1 / 3)
+
+
+@dataclass
+class Part:
+    """src/part.ts:9-22 Part."""
+    mesh: Mesh
+    hull: Mesh
+    mesh_vol: float
+    hull_vol: float
+    hausdorff: float
+    stuck: bool = False
+    stuck_reason: str = None
+    seams: list = field(default_factory=list)
+    merged: int = 1
+
+
+def part_cost(p):
+    """src/part.ts:27-29 partCost = max(rv, hausdorff)."""
+    return max(rv_cost(p.mesh_vol, p.hull_vol), p.hausdorff or 0.0)
+
+
+@dataclass
+class DecomposeOptions:
+    """src/decompose.ts:26-108 (GPU backends only; tree pool overrides as flat fields, 0 = default)."""
+    threshold: float = 0.05
+    max_iters: int = 100
+    width: int = 30
+    width2: int = 3
+    depth: int = 2
+    max_n_cutting: int = 16
+    n_concave_edges: int = 16
+    concave_eps: float = 0.005
+    concave_threshold: float = 3.49
+    concave_iters: int = 10
+    normalize: bool = True
+    require_closed: bool = True
+    validate_cuts: bool = True
+    refine_rounds: int = 2
+    refine_probes: int = 8
+    merge_hulls: bool = True
+    merge_rounds: int = 32
+    merge_plane_tol: float = 1e-3
+    verbose: int = 0
+    hausdorff_samples: int = 0
+    vertex_cap: int = 0
+    tri_cap: int = 0
+    edge_cap: int = 0
+    scratch_words: int = 0
+    hull_budget_bytes: int = 0
+    hull_max_verts: int = 0
+    prune_last_level: int = -1
+    device: int = 0
+
+    def raw(self):
+        r = F.DecomposeOptions()
+        for f in fields(self):
+            v = getattr(self, f.name)
+            setattr(r, f.name, int(v) if isinstance(v, bool) else v)
+        return r
+
+
+DEFAULT_OPTIONS = DecomposeOptions()
+
+
+@dataclass
+class DecomposeResult:
+    parts: list
+    iterations: int
+    stats: dict
+    merge: dict
+    norm: tuple
+    cuts: list
+
+
+def _read_result(api, h):
+    parts = []
+    for k in range(api.result_num_parts(h)):
+        d = F.PartDesc()
+        check(api, api.result_part(h, k, C.byref(d)))
+        v = np.zeros(max(3 * d.nv, 1), np.float32)
+        t = np.zeros(max(3 * d.nt, 1), np.uint32)
+        hv = np.zeros(max(3 * d.hull_nv, 1), np.float32)
+        ht = np.zeros(max(3 * d.hull_nt, 1), np.uint32)
+        sm = np.zeros(max(4 * d.n_seams, 1), np.float64)
+        check(api, api.result_part_data(h, k, v.ctypes.data_as(C.POINTER(F.f32)), t.ctypes.data_as(C.POINTER(F.u32)),
+                                        hv.ctypes.data_as(C.POINTER(F.f32)), ht.ctypes.data_as(C.POINTER(F.u32)),
+                                        sm.ctypes.data_as(C.POINTER(F.f64))))
+        parts.append(Part(Mesh(v[: 3 * d.nv], t[: 3 * d.nt]), Mesh(hv[: 3 * d.hull_nv], ht[: 3 * d.hull_nt]),
+                          d.mesh_vol, d.hull_vol, d.hausdorff if d.has_hausdorff else None, bool(d.stuck),
+                          STUCK_REASONS.get(d.stuck_reason), [tuple(sm[4 * i: 4 * i + 4]) for i in range(d.n_seams)],
+                          d.merged))
+    s = F.DecomposeStats()
+    check(api, api.result_stats(h, C.byref(s)))
+    stats = {f[0]: getattr(s, f[0]) for f in F.DecomposeStats._fields_ if f[0] != "norm_center"}
+    merge = dict(rounds=s.merge_rounds, merges=s.merges, pairs_hulled=s.pairs_hulled,
+                 pairs_hausdorff=s.pairs_hausdorff) if s.merged else None
+    cuts = []
+    for k in range(api.result_num_cuts(h)):
+        c = F.CutRecord()
+        check(api, api.result_cut(h, k, C.byref(c)))
+        cuts.append(dict(iter=c.iter, part=c.part, best_cut=c.best_cut, applied=bool(c.applied),
+                         best_cost=c.best_cost, plane=tuple(c.plane)))
+    api.result_free(h)
+    return DecomposeResult(parts, s.iterations, stats, merge, (s.norm_scale, tuple(s.norm_center)), cuts)
+
+
+def decompose(mesh, options=None, api=None, **overrides):
+    """decompose(mesh, options) (src/decompose.ts:407): one device, decompose, release."""
+    api = api or load_product()
+    o = options or DecomposeOptions()
+    for k, v in overrides.items():
+        setattr(o, k, v)
+    raw = o.raw()
+    h = C.c_void_p()
+    check(api, api.decompose(mesh.verts.ctypes.data_as(C.POINTER(F.f32)), mesh.nv,
+                             mesh.tris.ctypes.data_as(C.POINTER(F.u32)), mesh.nt, C.byref(raw), C.byref(h)))
+    return _read_result(api, h)
+
+
+def merge_hulls(meshes, options=None, api=None, **overrides):
+    """mergeHulls (src/merge.ts:170) over parts made from `meshes` (hull + Hausdorff on the GPU)."""
+    api = api or load_product()
+    o = options or DecomposeOptions()
+    for k, v in overrides.items():
+        setattr(o, k, v)
+    raw = o.raw()
+    b = _batch(meshes)
+    h = C.c_void_p()
+    check(api, api.merge_meshes(C.byref(b), C.byref(raw), C.byref(h)))
+    return _read_result(api, h)

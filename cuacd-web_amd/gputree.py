@@ -1,0 +1,220 @@
+"""GpuTree: the lookahead engine (src/gpu/tree.ts:116-816) over the C ABI."""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _ffi as F
+from ._lib import load_product, check, CuacdError
+
+HD_MAX_PAIRS = 1024  # src/gpu/tree.ts:12
+
+
+@dataclass
+class TreeConfig:
+    """src/gpu/tree.ts:14-46 TreeConfig (pool capacities are element counts)."""
+    width: int = 60
+    width2: int = 5
+    depth: int = 2
+    n_concave_edges: int = 32
+    min_edge_dist: float = 0.0125
+    vertex_cap: int = 512 << 20
+    tri_cap: int = 512 << 20
+    edge_cap: int = 16 << 20
+    scratch_words: int = 256 << 20
+    hull_budget_bytes: int = 4 << 30
+    hull_max_verts: int = 4096
+    prune_last_level: bool = True
+    hausdorff_samples: int = 0
+    device: int = 0
+
+
+DEFAULT_TREE_CONFIG = TreeConfig()
+
+
+def _batch(meshes):
+    """Packed upload of a list of Mesh (src/gpu/tree.ts:240-262); keeps the arrays alive."""
+    n = len(meshes)
+    verts = np.ascontiguousarray(np.concatenate([m.verts.reshape(-1) for m in meshes]) if n else np.zeros(0, np.float32),
+                                 dtype=np.float32)
+    tl = [m.tris.reshape(-1) for m in meshes]
+    tris = np.ascontiguousarray(np.concatenate(tl) if n else np.zeros(0, np.uint32), dtype=np.uint32)
+    nv = np.array([m.nv for m in meshes], dtype=np.uint32)
+    nt = np.array([m.nt for m in meshes], dtype=np.uint32)
+    v_off = np.zeros(n, dtype=np.uint64)
+    t_off = np.zeros(n, dtype=np.uint64)
+    if n:
+        v_off[1:] = np.cumsum(nv[:-1].astype(np.uint64))
+        t_off[1:] = np.cumsum(nt[:-1].astype(np.uint64))
+    b = F.MeshBatch(n, verts.ctypes.data_as(C.POINTER(F.f32)), tris.ctypes.data_as(C.POINTER(F.u32)),
+                    v_off.ctypes.data_as(C.POINTER(F.u64)), nv.ctypes.data_as(C.POINTER(F.u32)),
+                    t_off.ctypes.data_as(C.POINTER(F.u64)), nt.ctypes.data_as(C.POINTER(F.u32)))
+    b._keep = (verts, tris, nv, nt, v_off, t_off)
+    return b
+
+
+@dataclass
+class SeedResult:
+    best_cut: int
+    best_cost: float
+    plane: tuple
+
+
+@dataclass
+class TreeRunResult:
+    seeds: list
+    overflow: bool
+    hull_fail: bool
+    hull_fail_count: int
+    np_overflow: bool
+    pruned: int
+    jobs_per_level: list
+    hulls_done: int
+    gpu_ms: float
+    levels: list = field(default_factory=list)
+
+
+class GpuTree:
+    """One engine instance with its device pools; not re-entrant (calls are serialized)."""
+
+    def __init__(self, cfg=None, api=None, **overrides):
+        self.api = api or load_product()
+        c = cfg or TreeConfig()
+        for k, v in overrides.items():
+            setattr(c, k, v)
+        self.cfg = c
+        raw = F.TreeConfig(c.width, c.width2, c.depth, c.n_concave_edges, c.min_edge_dist, c.hull_max_verts,
+                           c.vertex_cap, c.tri_cap, c.edge_cap, c.scratch_words, c.hull_budget_bytes,
+                           1 if c.prune_last_level else 0, c.hausdorff_samples, c.device)
+        h = C.c_void_p()
+        check(self.api, self.api.tree_create(C.byref(raw), C.byref(h)))
+        self._h = h
+
+    def destroy(self):
+        if self._h:
+            self.api.tree_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    @property
+    def plane_slots(self):
+        return self.api.plane_slots(self._h)
+
+    @property
+    def vis_max(self):
+        return self.api.vis_max(self._h)
+
+    def run(self, seeds, edge_planes=None, depth=None, width=None, n_edge_cuts=None, refine=None, read_levels=True):
+        """run(seeds, edgePlanes, opts) (src/gpu/tree.ts:449). refine = dict(rounds, probes, spacing)."""
+        n = len(seeds)
+        b = _batch(seeds)
+        slots = self.plane_slots
+        ep = np.zeros(n * slots * 4, dtype=np.float32)
+        if edge_planes is not None:
+            for k, p in enumerate(edge_planes):
+                p = np.asarray(p, dtype=np.float32).reshape(-1)[: slots * 4]
+                ep[k * slots * 4: k * slots * 4 + p.size] = p
+        spacing = None
+        opts = F.RunOpts(depth or 0, width or 0, 0xFFFFFFFF if n_edge_cuts is None else n_edge_cuts, 0, 0, None)
+        if refine:
+            spacing = np.asarray(refine["spacing"], dtype=np.float32)
+            opts.refine_rounds = refine["rounds"]
+            opts.refine_probes = refine["probes"]
+            opts.refine_spacing = spacing.ctypes.data_as(C.POINTER(F.f32))
+        out = (F.SeedResult * max(n, 1))()
+        st = F.RunStats()
+        check(self.api, self.api.tree_run(self._h, C.byref(b), ep.ctypes.data_as(C.POINTER(F.f32)), C.byref(opts),
+                                          out, C.byref(st)))
+        seeds_out = []
+        for k in range(n):
+            r = out[k]
+            seeds_out.append(SeedResult(r.best_cut if r.valid else -1, r.best_cost if r.valid else float("inf"),
+                                        tuple(r.plane) if r.valid else None))
+        res = TreeRunResult(seeds_out, bool(st.overflow), bool(st.hull_fail), st.hull_fail_count,
+                            bool(st.np_overflow), st.pruned, [x for x in st.jobs_per_level if x], st.hulls_done,
+                            st.gpu_ms)
+        if read_levels and n:
+            res.levels = [self.level_items(l) for l in range(len(res.jobs_per_level) + 1)]
+        return res
+
+    def level_items(self, level):
+        n = F.u32()
+        check(self.api, self.api.tree_level_items(self._h, level, None, 0, C.byref(n)))
+        arr = (F.Item * max(n.value, 1))()
+        check(self.api, self.api.tree_level_items(self._h, level, arr, n.value, C.byref(n)))
+        out = []
+        for i in range(n.value):
+            it = arr[i]
+            out.append(dict(part_ids=list(it.part_ids), np=it.np, src_part=it.src_part, initial_cut=it.initial_cut,
+                            n_levels=it.n_levels, level_costs=list(it.level_costs), flags=it.flags,
+                            parent_item=it.parent_item, cut_idx=it.cut_idx))
+        return out
+
+    def read_part(self, idx, geometry=True):
+        """readPart (src/gpu/tree.ts:764): descriptor dict plus the uncapped mesh."""
+        info = F.PartInfo()
+        check(self.api, self.api.read_part(self._h, idx, C.byref(info), None, 0, None, 0))
+        d = dict(nv=info.nv, nt=info.nt, ne=info.ne, plane_ids=list(info.plane_ids)[: info.n_planes],
+                 n_planes=info.n_planes, flags=info.flags, n_cand=info.n_cand, n_ins=info.n_ins,
+                 mesh_vol=info.mesh_vol, hull_vol=info.hull_vol)
+        if geometry:
+            v = np.zeros(max(info.nv * 3, 1), dtype=np.float32)
+            t = np.zeros(max(info.nt * 3, 1), dtype=np.uint32)
+            check(self.api, self.api.read_part(self._h, idx, C.byref(info), v.ctypes.data_as(C.POINTER(F.f32)),
+                                               info.nv, t.ctypes.data_as(C.POINTER(F.u32)), info.nt))
+            d["verts"] = v[: info.nv * 3].reshape(-1, 3)
+            d["tris"] = t[: info.nt * 3].reshape(-1, 3)
+        return d
+
+    def _faces(self, k):
+        nf = F.u32()
+        p = self.api.last_faces(self._h, k, C.byref(nf))
+        if not nf.value:
+            return np.zeros((0, 3), dtype=np.uint32)
+        return np.ctypeslib.as_array(p, shape=(nf.value * 3,)).copy().reshape(-1, 3)
+
+    def hull_meshes(self, meshes):
+        """hullMeshes (src/gpu/tree.ts:382): [(tris, volume) or None]."""
+        n = len(meshes)
+        if n == 0:
+            return []
+        b = _batch(meshes)
+        vol = np.zeros(n, dtype=np.float64)
+        nf = np.zeros(n, dtype=np.uint32)
+        check(self.api, self.api.hull_meshes(self._h, C.byref(b), vol.ctypes.data_as(C.POINTER(F.f64)),
+                                             nf.ctypes.data_as(C.POINTER(F.u32))))
+        return [None if vol[k] < 0 else (self._faces(k), float(vol[k])) for k in range(n)]
+
+    def hulls_with_hausdorff(self, meshes):
+        """hullsWithHausdorff (src/gpu/tree.ts:671): [(tris, volume, hausdorff) or None]."""
+        n = len(meshes)
+        if n == 0:
+            return []
+        if n > HD_MAX_PAIRS:
+            raise CuacdError(F.ERR_ARG, f"hullsWithHausdorff: at most {HD_MAX_PAIRS} meshes per call")
+        b = _batch(meshes)
+        vol = np.zeros(n, dtype=np.float64)
+        hd = np.zeros(n, dtype=np.float64)
+        nf = np.zeros(n, dtype=np.uint32)
+        check(self.api, self.api.hulls_with_hausdorff(self._h, C.byref(b), vol.ctypes.data_as(C.POINTER(F.f64)),
+                                                      nf.ctypes.data_as(C.POINTER(F.u32)),
+                                                      hd.ctypes.data_as(C.POINTER(F.f64))))
+        return [None if vol[k] < 0 else (self._faces(k), float(vol[k]), float(hd[k])) for k in range(n)]
+
+    def hausdorff(self, pairs):
+        """hausdorff(pairs) (src/gpu/tree.ts:318): float64 array, one per (a, b) pair."""
+        n = len(pairs)
+        if n == 0:
+            return np.zeros(0)
+        if n > HD_MAX_PAIRS:
+            raise CuacdError(F.ERR_ARG, f"hausdorff: at most {HD_MAX_PAIRS} pairs per call")
+        ba = _batch([p[0] for p in pairs])
+        bb = _batch([p[1] for p in pairs])
+        out = np.zeros(n, dtype=np.float64)
+        check(self.api, self.api.hausdorff(self._h, C.byref(ba), C.byref(bb), out.ctypes.data_as(C.POINTER(F.f64))))
+        return out

@@ -1,0 +1,206 @@
+/* cuacd.h — C ABI of the MI355X lookahead decomposition library (libcuacd.so).
+ *
+ * Plain pointers and sizes only. Every entry point returns 0 on success or a
+ * CUACD_ERR_* code; cuacd_last_error() then holds the message, worded like the
+ * Error the reference throws. A handle is not re-entrant: calls on one tree
+ * are serialized by the caller (src/decompose.ts awaits every call), device
+ * memory stays inside the handle between calls (src/gpu/tree.ts:149-178).
+ * Library-owned outputs (faces of the last hull call, meshes) stay valid
+ * until the next call on the same handle or until they are freed.
+ */
+#ifndef CUACD_H
+#define CUACD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUACD_OK 0
+#define CUACD_ERR_ARG 1      /* configuration error (width, depth, >1024 pairs, ...) */
+#define CUACD_ERR_SCRATCH 2  /* capacity error; message contains "scratch" (src/merge.ts:251) */
+#define CUACD_ERR_DEVICE 3   /* HIP runtime failure or no GPU */
+#define CUACD_ERR_RUNTIME 4  /* pool overflow with a single part, np overflow, open input */
+
+typedef struct cuacd_tree cuacd_tree;
+typedef struct cuacd_result cuacd_result;
+typedef struct cuacd_meshes cuacd_meshes;
+
+/* src/gpu/tree.ts:14-46 TreeConfig (+ device ordinal, Hausdorff sample count) */
+typedef struct cuacd_tree_config {
+  uint32_t width, width2, depth, n_concave_edges;
+  float min_edge_dist;
+  uint32_t hull_max_verts;
+  uint64_t vertex_cap, tri_cap, edge_cap, scratch_words, hull_budget_bytes;
+  int32_t prune_last_level;
+  uint32_t hausdorff_samples; /* 0 = max(1000, floor(2000*area)) */
+  int32_t device;
+} cuacd_tree_config;
+
+/* A batch of meshes packed as the reference uploads them (src/gpu/tree.ts:240-262):
+ * mesh k has nv[k] vertices at verts + 3*v_off[k] and nt[k] triangles at
+ * tris + 3*t_off[k] (indices local to the mesh). nt[k] == 0 is a point set. */
+typedef struct cuacd_mesh_batch {
+  uint32_t n;
+  const float* verts;
+  const uint32_t* tris;
+  const uint64_t* v_off;
+  const uint32_t* nv;
+  const uint64_t* t_off;
+  const uint32_t* nt;
+} cuacd_mesh_batch;
+
+/* src/gpu/tree.ts:449-453 run options (refine = RefineRequest, :60-67) */
+typedef struct cuacd_run_opts {
+  uint32_t depth;       /* 0 = configured */
+  uint32_t width;       /* 0 = configured */
+  uint32_t n_edge_cuts; /* 0xffffffff = all plane slots */
+  uint32_t refine_rounds, refine_probes;
+  const float* refine_spacing; /* per seed, may be NULL when refine_rounds == 0 */
+} cuacd_run_opts;
+
+/* src/gpu/tree.ts:69-73 SeedResult */
+typedef struct cuacd_seed_result {
+  int32_t best_cut; /* -1 if none */
+  int32_t valid;
+  double best_cost;
+  float plane[4];
+} cuacd_seed_result;
+
+/* src/gpu/tree.ts:75-85 TreeRunResult flags and stats */
+typedef struct cuacd_run_stats {
+  int32_t overflow, hull_fail, np_overflow;
+  uint32_t hull_fail_count, pruned;
+  uint32_t jobs_per_level[4];
+  uint64_t hulls_done;
+  double gpu_ms;
+} cuacd_run_stats;
+
+/* src/gpu/tree.ts:48-58 ItemView */
+typedef struct cuacd_item {
+  uint32_t part_ids[8];
+  uint32_t np, src_part, initial_cut, n_levels;
+  double level_costs[4];
+  uint32_t flags, parent_item, cut_idx, pad;
+} cuacd_item;
+
+/* src/gpu/tree.ts:764-778 readPart descriptor */
+typedef struct cuacd_part_info {
+  uint32_t nv, nt, ne;
+  uint32_t plane_ids[4];
+  uint32_t n_planes, flags, n_cand, n_ins, pad;
+  double mesh_vol, hull_vol;
+} cuacd_part_info;
+
+const char* cuacd_last_error(void);
+void cuacd_tree_config_default(cuacd_tree_config* cfg);
+
+/* new GpuTree(device, cfg) (src/gpu/tree.ts:139) / destroy() (:813) */
+int cuacd_tree_create(const cuacd_tree_config* cfg, cuacd_tree** out);
+void cuacd_tree_destroy(cuacd_tree* t);
+/* planeSlots (src/gpu/tree.ts:759), visMax (:809) */
+uint32_t cuacd_plane_slots(const cuacd_tree* t);
+uint32_t cuacd_vis_max(const cuacd_tree* t);
+
+/* run(seeds, edgePlanes, opts) (src/gpu/tree.ts:449). edge_planes holds
+ * plane_slots planes (4 floats) per seed; zero planes are skipped; may be NULL. */
+int cuacd_tree_run(cuacd_tree* t, const cuacd_mesh_batch* seeds, const float* edge_planes, const cuacd_run_opts* opts,
+                   cuacd_seed_result* out, cuacd_run_stats* stats);
+/* items of one level of the last run (0 = seeds); *n receives the count */
+int cuacd_tree_level_items(cuacd_tree* t, uint32_t level, cuacd_item* out, uint32_t cap, uint32_t* n);
+/* readPart(partIdx) (src/gpu/tree.ts:764): descriptor, and geometry when the buffers are large enough */
+int cuacd_read_part(cuacd_tree* t, uint32_t part, cuacd_part_info* info, float* verts, uint32_t vcap, uint32_t* tris,
+                    uint32_t tcap);
+
+/* hullMeshes(meshes) (src/gpu/tree.ts:382): volume[k] (< 0 on failure), nfaces[k];
+ * faces via cuacd_last_faces until the next call */
+int cuacd_hull_meshes(cuacd_tree* t, const cuacd_mesh_batch* meshes, double* volume, uint32_t* nfaces);
+/* hullsWithHausdorff(meshes) (src/gpu/tree.ts:671): <= 1024 meshes, also Hausdorff(mesh, hull) */
+int cuacd_hulls_with_hausdorff(cuacd_tree* t, const cuacd_mesh_batch* meshes, double* volume, uint32_t* nfaces,
+                               double* hausdorff);
+const uint32_t* cuacd_last_faces(cuacd_tree* t, uint32_t k, uint32_t* nfaces);
+/* hausdorff(pairs) (src/gpu/tree.ts:318): a[k] vs b[k], <= 1024 pairs */
+int cuacd_hausdorff(cuacd_tree* t, const cuacd_mesh_batch* a, const cuacd_mesh_batch* b, double* out);
+
+/* src/decompose.ts:26-108 DecomposeOptions (GPU backends only) */
+typedef struct cuacd_decompose_options {
+  double threshold;
+  uint32_t max_iters, width, width2, depth, max_n_cutting, n_concave_edges;
+  double concave_eps, concave_threshold;
+  uint32_t concave_iters;
+  int32_t normalize, require_closed, validate_cuts;
+  uint32_t refine_rounds, refine_probes;
+  int32_t merge_hulls;
+  uint32_t merge_rounds;
+  double merge_plane_tol;
+  uint32_t verbose, hausdorff_samples;
+  uint64_t vertex_cap, tri_cap, edge_cap, scratch_words, hull_budget_bytes;
+  uint32_t hull_max_verts;
+  int32_t prune_last_level;
+  int32_t device;
+} cuacd_decompose_options;
+
+/* src/part.ts:9-22 Part, flattened */
+typedef struct cuacd_part_desc {
+  uint32_t nv, nt, hull_nv, hull_nt, n_seams, merged;
+  int32_t stuck;
+  int32_t stuck_reason; /* 0 none, 1 no-plane, 2 cut-failed, 3 not-closed, 4 degenerate */
+  int32_t has_hausdorff, pad;
+  double mesh_vol, hull_vol, hausdorff;
+} cuacd_part_desc;
+
+typedef struct cuacd_decompose_stats {
+  uint32_t iterations, tree_runs, hull_fail_runs, merged;
+  uint64_t candidate_hulls, pruned;
+  double total_ms, hausdorff_ms, cut_ms, concave_ms, hull_ms, tree_ms, merge_ms, tree_gpu_ms;
+  uint32_t merge_rounds, merges, pairs_hulled, pairs_hausdorff;
+  double norm_scale, norm_center[3];
+} cuacd_decompose_stats;
+
+typedef struct cuacd_cut_record {
+  uint32_t iter, part;
+  int32_t best_cut, applied;
+  double best_cost;
+  float plane[4];
+} cuacd_cut_record;
+
+void cuacd_decompose_options_default(cuacd_decompose_options* o);
+/* decompose(mesh, options) (src/decompose.ts:407) */
+int cuacd_decompose(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                    const cuacd_decompose_options* o, cuacd_result** out);
+/* mergeHulls over parts built from the given meshes (src/merge.ts:170) */
+int cuacd_merge_meshes(const cuacd_mesh_batch* meshes, const cuacd_decompose_options* o, cuacd_result** out);
+uint32_t cuacd_result_num_parts(const cuacd_result* r);
+int cuacd_result_part(const cuacd_result* r, uint32_t k, cuacd_part_desc* d);
+/* copies part k: verts[3*nv], tris[3*nt], hull_verts[3*hull_nv], hull_tris[3*hull_nt], seams[4*n_seams]; any may be NULL */
+int cuacd_result_part_data(const cuacd_result* r, uint32_t k, float* verts, uint32_t* tris, float* hull_verts,
+                           uint32_t* hull_tris, double* seams);
+int cuacd_result_stats(const cuacd_result* r, cuacd_decompose_stats* s);
+uint32_t cuacd_result_num_cuts(const cuacd_result* r);
+int cuacd_result_cut(const cuacd_result* r, uint32_t k, cuacd_cut_record* c);
+void cuacd_result_free(cuacd_result* r);
+
+/* ---- host geometry helpers (src/cpu/cut.ts, src/cpu/concave.ts, src/mesh.ts) ---- */
+uint32_t cuacd_meshes_count(const cuacd_meshes* m);
+int cuacd_meshes_get(const cuacd_meshes* m, uint32_t k, const float** verts, uint32_t* nv, const uint32_t** tris,
+                     uint32_t* nt);
+void cuacd_meshes_free(cuacd_meshes* m);
+/* planeCut(mesh, plane) (src/cpu/cut.ts:33): *out gets 2 meshes (pos, neg) when *ok */
+int cuacd_plane_cut(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, const float plane[4],
+                    int32_t* ok, cuacd_meshes** out);
+/* concaveEdgePlanes (src/cpu/concave.ts:25): writes up to cap floats, *n = floats produced */
+int cuacd_concave_edge_planes(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                              uint32_t n_concave_edges, double concave_eps, double concave_threshold, uint32_t seed,
+                              float* out, uint32_t cap, uint32_t* n);
+int32_t cuacd_is_closed(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt);
+double cuacd_signed_volume(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt);
+/* binary STL, welded by exact position (src/mesh.ts:166-212) */
+int cuacd_parse_stl(const uint8_t* buf, uint64_t len, cuacd_meshes** out);
+/* synthetic meshes: kind 0 staircase (steps), 1 blobs (n_blobs, grid), 2 torus (major, minor,
+ * twist), all closed, float32, normalized into the unit cube [0,1]^3 */
+int cuacd_generate(uint32_t kind, uint64_t seed, const double* params, uint32_t n_params, cuacd_meshes** out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,45 @@
+// ohull.hpp — ORACLE exact grid hull (test infrastructure only).
+//
+// Sequential restatement of the hull kernel (src/gpu/shaders.ts:752-1401):
+// points quantized to a 2^21 grid over their bbox, 80 extreme seeds along 40
+// directions, a simplex from the seeds, farthest-first insertion (seeds first,
+// then every point outside the seed hull), orientation by an f32 filter with
+// an exact integer fallback, optional last-level branch-and-bound.
+//
+// Every tie is broken by an order-independent rule (largest distance, then
+// lowest point index; owner face = first face in creation order), so a
+// parallel implementation that follows the same rules reproduces the same
+// face list exactly. The volume is taken over the ORIGINAL float coordinates
+// of the hull triangles with error-free products and exact accumulation.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace orc {
+
+struct HullParams {
+  bool prune = false;    // branch-and-bound after the seed phase
+  double meshVol = 0;    // part volume (for the bound)
+  double partial = 0;    // parent item's level-cost sum
+  double ub = 0;         // best upper bound of the seed
+};
+
+struct HullOut {
+  int status = 0;  // 0 exact hull, 1 degenerate (volume 0, no faces), 2 pruned (rough volume)
+  double volume = 0;
+  std::vector<uint32_t> faces;  // alive faces in creation order, outward
+  uint32_t nCand = 0;           // points outside the seed hull
+  uint32_t nIns = 0;            // insertions
+};
+
+HullOut exactHull(const float* v, uint32_t nv, const HullParams& hp);
+
+// exact (brute-force) check helpers for tests
+// sign of det[b-a; c-a; p-a] on the quantization grid of the point set
+struct Grid {
+  std::vector<int32_t> q;  // 3 per point
+  bool ok;
+};
+Grid quantize(const float* v, uint32_t nv);
+
+}  // namespace orc

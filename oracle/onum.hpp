@@ -1,0 +1,157 @@
+// onum.hpp — ORACLE numerics (test infrastructure only; never linked into the
+// product). Scalar arithmetic of the lookahead path, written out op by op so
+// that the CPU restatement produces the same bits as the GPU kernels:
+// every FP32 expression below is evaluated left to right without contraction
+// (build with -ffp-contract=off), products that must be exact use fmaf, and
+// sums that must not depend on order go into an int128 fixed-point accumulator.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+namespace orc {
+
+typedef __int128 i128;
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr float PC_EPS_F = 1e-6f;  // src/gpu/shaders.ts:162, src/cpu/cut.ts:13
+
+// ---- order-independent fixed-point sums (value * 2^64, truncated toward zero)
+inline i128 fix_f32(float x) {
+  uint32_t b;
+  std::memcpy(&b, &x, 4);
+  uint32_t e = (b >> 23) & 0xffu;
+  uint32_t m = b & 0x7fffffu;
+  if (e == 0) {
+    if (m == 0) return 0;
+    e = 1;
+  } else {
+    m |= 0x800000u;
+  }
+  const int sh = (int)e - 86;  // x * 2^64 = m * 2^(e - 150 + 64)
+  i128 v;
+  if (sh >= 0) v = (i128)m << sh;
+  else if (sh > -24) v = (i128)(m >> (-sh));
+  else v = 0;
+  return (b >> 31) ? -v : v;
+}
+inline double fix_to_f64(i128 a) {
+  const int64_t hi = (int64_t)(a >> 64);
+  const uint64_t lo = (uint64_t)a;
+  return (double)hi + (double)lo * 0x1p-64;
+}
+inline i128 fix_abs(i128 a) { return a < 0 ? -a : a; }
+
+// x*y*z as three f32 terms whose sum is exact up to the last (rounded) term,
+// accumulated with the given sign (error-free TwoProduct via fmaf)
+inline void acc_triple(i128& acc, float x, float y, float z, bool neg) {
+  const float p = x * y;
+  const float e = std::fmaf(x, y, -p);
+  const float p2 = p * z;
+  const float e2 = std::fmaf(p, z, -p2);
+  const float e3 = e * z;
+  const i128 s = fix_f32(p2) + fix_f32(e2) + fix_f32(e3);
+  acc += neg ? -s : s;
+}
+// det[a; b; c] = a . (b x c)
+inline void acc_det(i128& acc, const float* a, const float* b, const float* c, bool neg) {
+  acc_triple(acc, a[0], b[1], c[2], neg);
+  acc_triple(acc, a[0], b[2], c[1], !neg);
+  acc_triple(acc, a[1], b[2], c[0], neg);
+  acc_triple(acc, a[1], b[0], c[2], !neg);
+  acc_triple(acc, a[2], b[0], c[1], neg);
+  acc_triple(acc, a[2], b[1], c[0], !neg);
+}
+// det[a-o; b-o; c-o] by multilinearity (no rounded subtraction)
+inline void acc_det_anchor(i128& acc, const float* a, const float* b, const float* c, const float* o) {
+  acc_det(acc, a, b, c, false);
+  acc_det(acc, o, b, c, true);
+  acc_det(acc, a, o, c, true);
+  acc_det(acc, a, b, o, true);
+}
+
+// ---- plane helpers (f32 as stored in the plane table)
+// signed plane value ((a x + b y) + c z) + w
+inline float plane_eval(const float* pl, const float* v) {
+  float t = pl[0] * v[0];
+  t = t + pl[1] * v[1];
+  t = t + pl[2] * v[2];
+  return t + pl[3];
+}
+// src/gpu/shaders.ts:163 vsign
+inline int vsign(float d) { return d > PC_EPS_F ? 1 : (d < -PC_EPS_F ? -1 : 0); }
+
+// src/gpu/shaders.ts:116-119 plane_point, computed in FP64 and stored as f32
+inline void plane_point(const float* pl, float* out) {
+  const double a = pl[0], b = pl[1], c = pl[2];
+  double n2 = a * a + b * b;
+  n2 = n2 + c * c;
+  if (!(n2 > 1e-30)) n2 = 1e-30;
+  const double s = -(double)pl[3] / n2;
+  out[0] = (float)(a * s);
+  out[1] = (float)(b * s);
+  out[2] = (float)(c * s);
+}
+
+// src/gpu/shaders.ts:126-143 anchor_for: a point on every plane of the part
+// when one exists (3 planes: their common point; 2: a point on their line),
+// FP64 math, f32 result
+inline void anchor_for(const float pls[4][4], uint32_t n, float* o) {
+  o[0] = o[1] = o[2] = 0.f;
+  if (n == 0) return;
+  if (n == 1) {
+    plane_point(pls[0], o);
+    return;
+  }
+  if (n >= 3) {
+    const float* p1 = pls[n - 3];
+    const float* p2 = pls[n - 2];
+    const float* p3 = pls[n - 1];
+    const double n1[3] = {p1[0], p1[1], p1[2]}, n2[3] = {p2[0], p2[1], p2[2]}, n3[3] = {p3[0], p3[1], p3[2]};
+    const double c23[3] = {n2[1] * n3[2] - n2[2] * n3[1], n2[2] * n3[0] - n2[0] * n3[2], n2[0] * n3[1] - n2[1] * n3[0]};
+    const double c31[3] = {n3[1] * n1[2] - n3[2] * n1[1], n3[2] * n1[0] - n3[0] * n1[2], n3[0] * n1[1] - n3[1] * n1[0]};
+    const double c12[3] = {n1[1] * n2[2] - n1[2] * n2[1], n1[2] * n2[0] - n1[0] * n2[2], n1[0] * n2[1] - n1[1] * n2[0]};
+    const double det = (n1[0] * c23[0] + n1[1] * c23[1]) + n1[2] * c23[2];
+    if (std::fabs(det) > 1e-4) {
+      const double e1 = -(double)p1[3], e2 = -(double)p2[3], e3 = -(double)p3[3];
+      for (int k = 0; k < 3; k++) o[k] = (float)(((e1 * c23[k] + e2 * c31[k]) + e3 * c12[k]) / det);
+      return;
+    }
+  }
+  const float* a = pls[n - 2];
+  const float* b = pls[n - 1];
+  const double an[3] = {a[0], a[1], a[2]}, bn[3] = {b[0], b[1], b[2]};
+  const double L[3] = {an[1] * bn[2] - an[2] * bn[1], an[2] * bn[0] - an[0] * bn[2], an[0] * bn[1] - an[1] * bn[0]};
+  const double l2 = (L[0] * L[0] + L[1] * L[1]) + L[2] * L[2];
+  if (l2 > 1e-8) {
+    const double e1 = -(double)a[3], e2 = -(double)b[3];
+    const double bxL[3] = {bn[1] * L[2] - bn[2] * L[1], bn[2] * L[0] - bn[0] * L[2], bn[0] * L[1] - bn[1] * L[0]};
+    const double Lxa[3] = {L[1] * an[2] - L[2] * an[1], L[2] * an[0] - L[0] * an[2], L[0] * an[1] - L[1] * an[0]};
+    for (int k = 0; k < 3; k++) o[k] = (float)((e1 * bxL[k] + e2 * Lxa[k]) / l2);
+    return;
+  }
+  plane_point(b, o);
+}
+
+// ---- deterministic cube root and rv cost (src/part.ts:24-26)
+inline double det_cbrt(double x) {
+  if (!(x > 1e-300)) return 0.0;
+  uint64_t b;
+  std::memcpy(&b, &x, 8);
+  const int e = (int)((b >> 52) & 0x7ff) - 1023;
+  const int k = e >= 0 ? e / 3 : -((-e + 2) / 3);
+  const uint64_t yb = (uint64_t)(k + 1023) << 52;
+  double y;
+  std::memcpy(&y, &yb, 8);
+  for (int i = 0; i < 12; i++) {
+    const double y2 = y * y;
+    y = y - (y2 * y - x) / (3.0 * y2);
+  }
+  return y;
+}
+inline double rv_cost(double meshVol, double hullVol) {
+  double gap = hullVol - meshVol;
+  if (!(gap > 0.0)) gap = 0.0;
+  return 0.3 * det_cbrt(0.23873241463784300365 * gap);
+}
+
+}  // namespace orc

@@ -1,0 +1,103 @@
+"""End to end on the GPU: decompose / merge against the reference's known answers
+(tests/decompose.test.ts, tests/merge.test.ts) and against the oracle-driven run
+of the same host code — identical cut sequences, part counts and volumes."""
+import numpy as np
+import pytest
+
+from fixtures import square_torus, stairs
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(vertex_cap=4 << 20, tri_cap=4 << 20, edge_cap=1 << 18, scratch_words=4 << 20, hull_budget_bytes=256 << 20)
+
+
+def _same(ra, rb):
+    assert len(ra.cuts) == len(rb.cuts)
+    for a, b in zip(ra.cuts, rb.cuts):
+        assert a == b, (a, b)
+    assert len(ra.parts) == len(rb.parts)
+    for pa, pb in zip(ra.parts, rb.parts):
+        assert pa.mesh_vol == pb.mesh_vol and pa.hull_vol == pb.hull_vol
+        assert pa.hausdorff == pb.hausdorff
+        assert np.array_equal(pa.mesh.verts, pb.mesh.verts) and np.array_equal(pa.mesh.tris, pb.mesh.tris)
+        assert np.array_equal(pa.hull.tris, pb.hull.tris)
+
+
+def _both(cw, oracle, mesh, **opts):
+    rg = cw.decompose(mesh, **opts)
+    ro = cw.decompose(mesh, api=oracle, **opts)
+    _same(rg, ro)
+    return rg
+
+
+def test_cube_stays_one_part(cw, oracle):
+    r = _both(cw, oracle, cw.unit_cube(), max_iters=5, threshold=0.05, **SMALL)
+    assert len(r.parts) == 1
+    assert abs(cw.volume(r.parts[0].mesh) - 1) < 1e-6
+
+
+def test_l_shape_two_parts_merged(cw, oracle):
+    # tests/merge.test.ts:88-97: exactly 2 parts with merging
+    r = _both(cw, oracle, cw.l_shape(), max_iters=50, threshold=0.05, width=30, **SMALL)
+    assert len(r.parts) == 2
+    assert abs(sum(cw.volume(p.mesh) for p in r.parts) - 1.5) < 1e-4
+
+
+def test_stairs_volume_preserved(cw, oracle):
+    # tests/decompose.test.ts:33-41 and tests/merge.test.ts:77-86
+    r = _both(cw, oracle, stairs(cw), max_iters=50, threshold=0.05, width=30, **SMALL)
+    assert 3 <= len(r.parts) <= 6
+    for p in r.parts:
+        assert cw.is_closed(p.mesh)
+    assert abs(sum(cw.volume(p.mesh) for p in r.parts) - 6) < 1e-4
+
+
+def test_square_torus(cw, oracle):
+    # tests/decompose.test.ts:43-51 (no merge there)
+    r = _both(cw, oracle, square_torus(cw), max_iters=50, threshold=0.05, width=30, merge_hulls=False, **SMALL)
+    assert len(r.parts) >= 4
+    assert abs(sum(cw.volume(p.mesh) for p in r.parts) - 1.5) < 1e-4
+
+
+def test_merge_two_halves(cw, oracle):
+    # tests/merge.test.ts:33-47
+    pos, neg = cw.plane_cut(cw.unit_cube(), (1, 0, 0, -0.5))
+    rg = cw.merge_hulls([neg, pos], threshold=0.05)
+    ro = cw.merge_hulls([neg, pos], api=oracle, threshold=0.05)
+    _same(rg, ro)
+    assert len(rg.parts) == 1 and rg.parts[0].merged == 2 and len(rg.parts[0].seams) == 1
+    assert abs(rg.parts[0].mesh_vol - 1) < 1e-6 and abs(rg.parts[0].hull_vol - 1) < 1e-5
+    assert rg.merge["merges"] == 1 and rg.merge["rounds"] == 1
+
+
+def test_merge_four_slabs_two_rounds(cw, oracle):
+    # tests/merge.test.ts:64-75
+    rest, slabs = cw.unit_cube(), []
+    for x in (0.25, 0.5, 0.75):
+        pos, neg = cw.plane_cut(rest, (1, 0, 0, -x))
+        slabs.append(neg)
+        rest = pos
+    slabs.append(rest)
+    rg = cw.merge_hulls(slabs, threshold=0.05)
+    ro = cw.merge_hulls(slabs, api=oracle, threshold=0.05)
+    _same(rg, ro)
+    assert len(rg.parts) == 1 and rg.merge["rounds"] == 2 and rg.merge["merges"] == 3
+    assert len(rg.parts[0].seams) == 3
+
+
+def test_merge_l_boxes_stay_apart(cw, oracle):
+    # tests/merge.test.ts:54-62
+    boxes = [cw.make_box((0, 0, 0), (1, 1, 1)), cw.make_box((1, 0, 0), (2, 1, 0.5))]
+    r = cw.merge_hulls(boxes, threshold=0.05)
+    assert len(r.parts) == 2 and r.merge["pairs_hulled"] == 1 and r.merge["merges"] == 0
+
+
+def test_stl_49160_runs(cw, oracle):
+    # tests/decompose.test.ts:53-63 (closed parts, volume within 1e-3), fewer iterations for time
+    import os
+    data = open(os.path.join(os.path.dirname(__file__), "golden", "49160.stl"), "rb").read()
+    mesh = cw.parse_stl(data)
+    assert cw.is_closed(mesh)
+    r = _both(cw, oracle, mesh, max_iters=3, threshold=0.05, width=30, merge_hulls=False, **SMALL)
+    total = sum(cw.volume(p.mesh) for p in r.parts)
+    assert abs(total - cw.volume(mesh)) < cw.volume(mesh) * 1e-3

@@ -1,0 +1,143 @@
+"""GPU (libcuacd.so, HIP) against the oracle on the same inputs: the tree's
+item tables, part volumes and chosen cuts, hull face lists and volumes,
+Hausdorff distances — all bit-exact (integer tables, FP64 bit patterns)."""
+import numpy as np
+import pytest
+
+from fixtures import l_prism, park_miller_cloud, square_torus, stairs
+
+pytestmark = pytest.mark.gpu
+
+
+def _tree_pair(cw, oracle, **cfg):
+    return cw.GpuTree(**cfg), cw.GpuTree(api=oracle, **cfg)
+
+
+def compare_runs(cw, oracle, meshes, edge_planes=None, refine=None, run_kw=None, **cfg):
+    g, o = _tree_pair(cw, oracle, **cfg)
+    run_kw = run_kw or {}
+    rg = g.run(meshes, edge_planes, refine=refine, **run_kw)
+    ro = o.run(meshes, edge_planes, refine=refine, **run_kw)
+    assert not rg.overflow and not rg.np_overflow
+    assert rg.jobs_per_level == ro.jobs_per_level
+    assert rg.hulls_done == ro.hulls_done, (rg.hulls_done, ro.hulls_done)
+    assert rg.pruned == ro.pruned
+    for sg, so in zip(rg.seeds, ro.seeds):
+        assert sg.best_cut == so.best_cut
+        assert sg.best_cost == so.best_cost  # FP64 bits
+        assert sg.plane == so.plane
+    checked = 0
+    for lvl, (lg, lo) in enumerate(zip(rg.levels, ro.levels)):
+        assert len(lg) == len(lo)
+        for i, (a, b) in enumerate(zip(lg, lo)):
+            assert a == b, f"level {lvl} item {i}: gpu {a} oracle {b}"
+            if lvl > 0 and a["flags"] == 1:
+                for p in a["part_ids"][: a["np"]]:
+                    pg, po = g.read_part(p, geometry=False), o.read_part(p, geometry=False)
+                    for k in ("nv", "nt", "ne", "n_planes", "plane_ids", "flags", "mesh_vol", "hull_vol", "n_cand", "n_ins"):
+                        assert pg[k] == po[k], f"part {p} {k}: gpu {pg[k]} oracle {po[k]}"
+                    checked += 1
+    g.destroy()
+    o.destroy()
+    return rg, checked
+
+
+def test_cube_depth1_known_answers(cw, oracle):
+    # tests/gpu_tree.test.ts:8-32
+    rg, checked = compare_runs(cw, oracle, [cw.unit_cube()], run_kw=dict(depth=1), width=3, width2=3, depth=2,
+                               n_concave_edges=0, min_edge_dist=0.01)
+    assert rg.jobs_per_level[0] == 3 + 32
+    assert rg.seeds[0].best_cost == 0.0
+    assert checked == 6
+
+
+def test_cube_depth2_quarters(cw, oracle):
+    # tests/gpu_tree.test.ts:34-60
+    g = cw.GpuTree(width=3, width2=3, depth=2, n_concave_edges=0, min_edge_dist=0.01)
+    res = g.run([cw.unit_cube()], depth=2)
+    assert res.jobs_per_level[1] == (3 + 32) * 3
+    for j in range(9):
+        it = res.levels[2][j]
+        assert it["flags"] == 1 and it["np"] == 3
+        vols = sorted(g.read_part(it["part_ids"][p], geometry=False)["mesh_vol"] for p in range(3))
+        assert abs(vols[0] - 0.25) < 1e-12 and abs(vols[1] - 0.25) < 1e-12 and abs(vols[2] - 0.5) < 1e-12
+    g.destroy()
+    compare_runs(cw, oracle, [cw.unit_cube()], width=3, width2=3, depth=2, n_concave_edges=0, min_edge_dist=0.01)
+
+
+def test_l_shape_best_plane(cw, oracle):
+    # tests/gpu_tree.test.ts:62-79: x=0 or z=0 at cost 0
+    m, _, _ = cw.normalize(cw.l_shape())
+    rg, _ = compare_runs(cw, oracle, [m], width=9, width2=3, depth=2, n_concave_edges=0, min_edge_dist=0.0125)
+    pl = rg.seeds[0].plane
+    assert (abs(pl[0] - 1) < 1e-6 or abs(pl[2] - 1) < 1e-6) and abs(pl[3]) < 1e-3
+    assert rg.seeds[0].best_cost < 1e-3
+
+
+def test_offcentre_box(cw, oracle):
+    # tests/gpu_tree.test.ts:81-107
+    box = cw.make_box((0.2, -0.7, 0.1), (0.9, 0.3, 0.8))
+    compare_runs(cw, oracle, [box], width=6, width2=3, depth=2, n_concave_edges=0, min_edge_dist=0.01)
+
+
+def test_multi_seed_with_edge_planes_and_refine(cw, oracle):
+    meshes = [stairs(cw), square_torus(cw), l_prism(cw), cw.normalize(cw.l_shape())[0]]
+    meshes = [cw.normalize(m)[0] for m in meshes]
+    planes = [cw.concave_edge_planes(m, n_concave_edges=4, seed=7 + i) for i, m in enumerate(meshes)]
+    refine = dict(rounds=2, probes=8, spacing=[0.1, 0.05, 0.08, 0.12])
+    rg, checked = compare_runs(cw, oracle, meshes, planes, refine=refine, width=12, width2=3, depth=3,
+                               n_concave_edges=4, min_edge_dist=0.0125)
+    assert checked > 0
+
+
+def test_blob_depth3(cw, oracle):
+    m = cw.normalize(cw.generate("blobs", 20260201, [5, 20]))[0]
+    assert m.nt > 1000
+    planes = [cw.concave_edge_planes(m, n_concave_edges=8, seed=3)]
+    rg, checked = compare_runs(cw, oracle, [m], planes, refine=dict(rounds=1, probes=4, spacing=[0.05]), width=6,
+                               width2=3, depth=3, n_concave_edges=8, min_edge_dist=0.0125)
+    assert rg.pruned > 0 or checked > 0
+
+
+def _hull_cases(cw):
+    rng = np.random.default_rng(7)
+    cases = [cw.unit_cube().verts, cw.l_shape().verts, park_miller_cloud(1000)]
+    for n in (4, 5, 9, 60, 300, 3000):
+        cases.append(rng.standard_normal((n, 3)).astype(np.float32))
+    cases.append(np.round(rng.random((500, 3)) * 4).astype(np.float32))  # many coplanar / duplicate points
+    cases.append(np.concatenate([rng.random((50, 2)), np.zeros((50, 1))], axis=1).astype(np.float32))  # flat
+    cases.append(np.zeros((3, 3), np.float32))  # too few points
+    sphere = rng.standard_normal((4000, 3))
+    cases.append((sphere / np.linalg.norm(sphere, axis=1, keepdims=True)).astype(np.float32))  # all extreme
+    return [cw.Mesh(v, np.zeros((0, 3), np.uint32)) for v in cases]
+
+
+def test_hull_meshes_bit_exact(cw, oracle):
+    cases = _hull_cases(cw)
+    g, o = _tree_pair(cw, oracle, width=3, width2=3, depth=1, n_concave_edges=0)
+    hg, ho = g.hull_meshes(cases), o.hull_meshes(cases)
+    for k, (a, b) in enumerate(zip(hg, ho)):
+        assert (a is None) == (b is None)
+        if a is None:
+            continue
+        assert a[1] == b[1], f"case {k}: volume {a[1]!r} vs {b[1]!r}"
+        assert np.array_equal(a[0], b[0]), f"case {k}: face lists differ"
+    assert hg[0][1] == 1.0 and hg[1][1] == 1.75  # tests/cpu.test.ts:62-64
+    assert 6.5 < hg[2][1] < 8  # tests/cpu.test.ts:65-75
+
+
+def test_hausdorff_bit_exact(cw, oracle):
+    # tests/cpu.test.ts:78-112 pairs: meshes against their hulls, box against a shifted box
+    g, o = _tree_pair(cw, oracle, width=3, width2=3, depth=1, n_concave_edges=0)
+    meshes = [cw.l_shape(), square_torus(cw), stairs(cw), cw.unit_cube()]
+    hulls = g.hull_meshes(meshes)
+    pairs = [(m, cw.Mesh(m.verts, h[0])) for m, h in zip(meshes, hulls)]
+    pairs.append((cw.unit_cube(), cw.make_box((0.1, 0, 0), (1.1, 1, 1))))
+    dg, do = g.hausdorff(pairs), o.hausdorff(pairs)
+    assert np.array_equal(dg, do), (dg, do)
+    assert 0.3 < dg[0] < 0.6
+    assert dg[3] < 1e-6
+    assert abs(dg[4] - 0.1) < 0.02
+    hw_g, hw_o = g.hulls_with_hausdorff(meshes), o.hulls_with_hausdorff(meshes)
+    for a, b in zip(hw_g, hw_o):
+        assert a[1] == b[1] and a[2] == b[2] and np.array_equal(a[0], b[0])

@@ -1,0 +1,250 @@
+"""The oracle pinned before it is trusted: the reference's known answers
+(tests/gpu_tree.test.ts, tests/cpu.test.ts), closed forms, invariants and
+brute force on small inputs. CPU only."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from fixtures import l_prism, park_miller_cloud, square_torus, stairs
+
+
+def tree(cw, oracle, **cfg):
+    return cw.GpuTree(api=oracle, **cfg)
+
+
+# ---- tree known answers (tests/gpu_tree.test.ts) ---------------------------------------------
+def test_cube_depth1(cw, oracle):
+    t = tree(cw, oracle, width=3, width2=3, depth=2, n_concave_edges=0, min_edge_dist=0.01)
+    res = t.run([cw.unit_cube()], depth=1)
+    assert not res.overflow and not res.hull_fail
+    assert res.jobs_per_level[0] == 3 + t.plane_slots
+    for j in range(3):
+        it = res.levels[1][j]
+        assert it["flags"] == 1 and it["np"] == 2
+        for p in range(2):
+            d = t.read_part(it["part_ids"][p])
+            assert abs(d["mesh_vol"] - 0.5) < 1e-12 and abs(d["hull_vol"] - 0.5) < 1e-12
+            assert d["n_planes"] == 1
+        assert it["level_costs"][0] < 1e-4
+    assert res.seeds[0].best_cost < 1e-4
+
+
+def test_cube_depth2(cw, oracle):
+    t = tree(cw, oracle, width=3, width2=3, depth=2, n_concave_edges=0, min_edge_dist=0.01)
+    res = t.run([cw.unit_cube()], depth=2)
+    assert res.jobs_per_level[1] == (3 + t.plane_slots) * 3
+    for j in range(9):
+        it = res.levels[2][j]
+        assert it["flags"] == 1 and it["np"] == 3
+        vols = sorted(t.read_part(it["part_ids"][p])["mesh_vol"] for p in range(3))
+        assert np.allclose(vols, [0.25, 0.25, 0.5], atol=1e-12)
+        for p in range(3):
+            d = t.read_part(it["part_ids"][p])
+            assert abs(d["hull_vol"] - d["mesh_vol"]) < 2e-4
+
+
+def test_l_shape_best_plane(cw, oracle):
+    m, _, _ = cw.normalize(cw.l_shape())
+    t = tree(cw, oracle, width=9, width2=3, depth=2, n_concave_edges=0, min_edge_dist=0.0125)
+    s = t.run([m], depth=2).seeds[0]
+    pl = s.plane
+    is_x = abs(pl[0] - 1) < 1e-6 and abs(pl[3]) < 1e-3
+    is_z = abs(pl[2] - 1) < 1e-6 and abs(pl[3]) < 1e-3
+    assert is_x or is_z
+    assert s.best_cost < 1e-3
+
+
+def test_offcentre_box_hull_equals_mesh(cw, oracle):
+    box = cw.make_box((0.2, -0.7, 0.1), (0.9, 0.3, 0.8))
+    total = cw.volume(box)  # of the float32 corners
+    t = tree(cw, oracle, width=6, width2=3, depth=2, n_concave_edges=0, min_edge_dist=0.01)
+    res = t.run([box], depth=2)
+    for lvl in (1, 2):
+        for it in res.levels[lvl]:
+            if it["flags"] != 1:
+                continue
+            s = 0.0
+            for p in range(it["np"]):
+                d = t.read_part(it["part_ids"][p])
+                assert abs(d["hull_vol"] - d["mesh_vol"]) < 1e-9
+                s += d["mesh_vol"]
+            assert abs(s - total) < 1e-9
+
+
+def test_children_volumes_sum_to_parent(cw, oracle):
+    """Invariant: the two uncapped halves of every job add up to the parent (exact anchors, <= 2 planes)."""
+    m = cw.normalize(stairs(cw))[0]
+    t = tree(cw, oracle, width=12, width2=3, depth=2, n_concave_edges=0, min_edge_dist=0.0125)
+    res = t.run([m], depth=2)
+    parent = abs(cw.signed_volume(m))
+    for it in res.levels[1]:
+        if it["flags"] != 1:
+            continue
+        v = sum(t.read_part(p, geometry=False)["mesh_vol"] for p in it["part_ids"][:2])
+        assert abs(v - parent) < 1e-9 * parent
+
+
+def test_uncapped_volume_matches_capped_cut(cw, oracle):
+    """Closed form: a tree child's uncapped volume equals the volume of the host's capped cut."""
+    m = cw.normalize(square_torus(cw))[0]
+    t = tree(cw, oracle, width=9, width2=3, depth=1, n_concave_edges=0, min_edge_dist=0.0125)
+    res = t.run([m], depth=1)
+    n = 0
+    for it in res.levels[1]:
+        if it["flags"] != 1:
+            continue
+        pos_id, neg_id = it["part_ids"][0], it["part_ids"][1]
+        job_plane = None
+        # the cut plane: the level-0 job that produced the item
+        cut = it["cut_idx"]
+        cpa = 3
+        axis = cut // cpa
+        lo, hi = m.verts.min(0), m.verts.max(0)
+        lov, hiv = np.float32(lo[axis] + np.float32(0.0125)), np.float32(hi[axis] - np.float32(0.0125))
+        frac = np.float32((cut % cpa + 1.0) / (cpa + 1.0))
+        pos = np.float32(lov + frac * (hiv - lov))
+        job_plane = [0.0, 0.0, 0.0, -float(pos)]
+        job_plane[axis] = 1.0
+        halves = cw.plane_cut(m, job_plane)
+        assert halves is not None
+        # record sorts an item's parts by rv, so compare the pair as a set
+        got = sorted(t.read_part(p, geometry=False)["mesh_vol"] for p in (pos_id, neg_id))
+        want = sorted(cw.volume(h) for h in halves)
+        assert abs(got[0] - want[0]) < 1e-6 and abs(got[1] - want[1]) < 1e-6
+        n += 1
+    assert n >= 3
+
+
+# ---- hull: closed forms, invariants, brute force ----------------------------------------------
+def _hull(cw, oracle, pts):
+    t = tree(cw, oracle, width=3, width2=3, depth=1, n_concave_edges=0)
+    return t.hull_meshes([cw.Mesh(pts, np.zeros((0, 3), np.uint32))])[0]
+
+
+def test_hull_known_answers(cw, oracle):
+    # tests/cpu.test.ts:58-76
+    assert _hull(cw, oracle, cw.unit_cube().verts)[1] == 1.0
+    assert _hull(cw, oracle, cw.l_shape().verts)[1] == 1.75
+    tris, vol = _hull(cw, oracle, park_miller_cloud())
+    assert 6.5 < vol < 8
+    assert cw.is_closed(cw.Mesh(park_miller_cloud(), tris))
+
+
+def _quantize(v):
+    v = v.astype(np.float32)
+    lo, hi = v.min(0), v.max(0)
+    ext = (hi - lo).astype(np.float32)
+    scale = max(float(ext.max()), 1e-30)
+    qs = np.float32(2097152.0) / np.float32(scale)
+    return np.rint((v - lo).astype(np.float32) * qs).astype(np.int64)
+
+
+def _exact_hull_vertices(q):
+    """Vertices of the convex hull of an integer point set, exactly: enumerate supporting planes
+    through point triples (all points on one side, integer arithmetic), then a point is a vertex
+    iff the planes through it have three linearly independent normals."""
+    pts = [tuple(int(x) for x in p) for p in q]
+    n = len(pts)
+    planes = set()
+    for i, j, k in itertools.combinations(range(n), 3):
+        a, b, c = pts[i], pts[j], pts[k]
+        e1 = [b[t] - a[t] for t in range(3)]
+        e2 = [c[t] - a[t] for t in range(3)]
+        nrm = [e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]]
+        if nrm == [0, 0, 0]:
+            continue
+        s = [sum(nrm[t] * (p[t] - a[t]) for t in range(3)) for p in pts]
+        if all(x >= 0 for x in s):
+            nrm = [-x for x in nrm]
+        elif not all(x <= 0 for x in s):
+            continue
+        g = math.gcd(math.gcd(abs(nrm[0]), abs(nrm[1])), abs(nrm[2]))
+        nrm = tuple(x // g for x in nrm)
+        planes.add((nrm, sum(nrm[t] * a[t] for t in range(3))))
+    verts = set()
+    for m in range(n):
+        ns = [pl[0] for pl in planes if sum(pl[0][t] * pts[m][t] for t in range(3)) == pl[1]]
+        for u, v, w in itertools.combinations(ns, 3):
+            det = (u[0] * (v[1] * w[2] - v[2] * w[1]) - u[1] * (v[0] * w[2] - v[2] * w[0])
+                   + u[2] * (v[0] * w[1] - v[1] * w[0]))
+            if det != 0:
+                verts.add(m)
+                break
+    return verts
+
+
+@pytest.mark.parametrize("seed,n", [(1, 8), (2, 10), (3, 12), (4, 9)])
+def test_hull_vertices_match_brute_force(cw, oracle, seed, n):
+    rng = np.random.default_rng(seed)
+    pts = rng.standard_normal((n, 3)).astype(np.float32)
+    tris, vol = _hull(cw, oracle, pts)
+    used = set(int(x) for x in tris.reshape(-1))
+    q = _quantize(pts)
+    assert _exact_hull_vertices(q) == used  # general position: hull vertices are exactly the extreme points
+    assert used <= set(range(n))
+    # volume equals the exact polytope volume over the original coordinates (scipy qhull, FP64)
+    from scipy.spatial import ConvexHull
+    assert abs(vol - ConvexHull(pts.astype(np.float64)).volume) < 1e-9 * vol
+
+
+def test_hull_against_qhull_on_clouds(cw, oracle):
+    from scipy.spatial import ConvexHull
+    rng = np.random.default_rng(11)
+    for n in (50, 400, 3000):
+        pts = rng.standard_normal((n, 3)).astype(np.float32)
+        tris, vol = _hull(cw, oracle, pts)
+        h = ConvexHull(pts.astype(np.float64))
+        assert set(int(x) for x in tris.reshape(-1)) == set(int(x) for x in h.vertices)
+        assert abs(vol - h.volume) < 1e-9 * h.volume
+        m = cw.Mesh(pts, tris)
+        assert cw.is_closed(m) and cw.signed_volume(m) > 0
+
+
+def test_hull_degenerate_inputs(cw, oracle):
+    assert _hull(cw, oracle, np.zeros((3, 3), np.float32))[1] == 0.0
+    flat = np.concatenate([np.random.default_rng(0).random((40, 2)), np.zeros((40, 1))], axis=1).astype(np.float32)
+    tris, vol = _hull(cw, oracle, flat)
+    assert vol == 0.0 and len(tris) == 0
+    dup = np.repeat(cw.unit_cube().verts, 5, axis=0)
+    assert _hull(cw, oracle, dup)[1] == 1.0
+
+
+# ---- Hausdorff (tests/cpu.test.ts:78-87) ------------------------------------------------------
+def test_hausdorff_known_answers(cw, oracle):
+    t = tree(cw, oracle, width=3, width2=3, depth=1, n_concave_edges=0)
+    cube, L = cw.unit_cube(), cw.l_shape()
+    hc = t.hull_meshes([cube, L])
+    d = t.hausdorff([(cube, cw.Mesh(cube.verts, hc[0][0])), (L, cw.Mesh(L.verts, hc[1][0])),
+                     (cw.make_box((0, 0, 0), (1, 1, 1)), cw.make_box((0.1, 0, 0), (1.1, 1, 1)))])
+    assert d[0] < 1e-6
+    assert 0.3 < d[1] < 0.6
+    assert abs(d[2] - 0.1) < 0.02
+
+
+def test_hausdorff_invariants(cw, oracle):
+    """Symmetric in its arguments (max of both directions) and zero for a mesh against itself
+    up to f32 rounding of the sample positions."""
+    t = tree(cw, oracle, width=3, width2=3, depth=1, n_concave_edges=0)
+    a = cw.normalize(stairs(cw))[0]
+    b = cw.make_box((-0.9, -0.8, -0.7), (0.8, 0.9, 0.6))
+    d = t.hausdorff([(a, b), (b, a), (a, a)])
+    assert d[0] == d[1] and 0.0 < d[0] < 2.0
+    assert d[2] < 1e-6
+
+
+def test_wang_hash_known_values():
+    # src/cpu/hausdorff.ts:10-17 evaluated by hand for seed 0: 61 -> *9 = 549 -> ^>>4 = 519
+    # -> *0x27d4eb2d mod 2^32 = 3232286779 -> ^>>15 = 3232286779 ^ 98641
+    s = 519 * 0x27D4EB2D % (1 << 32)
+    assert s == 3232286779
+    assert (s ^ (s >> 15)) == 3232286779 ^ 98641
+
+
+# ---- rv cost (src/part.ts:24-26) ----------------------------------------------------------------
+def test_rv_cost_closed_form(cw, oracle):
+    # a sphere's worth of gap: 3/(4 pi) * (4/3 pi r^3) = r^3 -> rv = 0.3 r
+    for r in (0.01, 0.1, 0.5, 1.0):
+        gap = 4.0 / 3.0 * math.pi * r ** 3
+        assert abs(cw.rv_cost(1.0, 1.0 + gap) - 0.3 * r) < 1e-12
