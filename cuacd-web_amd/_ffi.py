@@ -71,6 +71,21 @@ class DecomposeStats(C.Structure):
                 ("norm_scale", f64), ("norm_center", f64 * 3)]
 
 
+NUM_KERNELS = 9
+KERNEL_NAMES = ["plan", "scan", "emit", "hull", "hull_big", "record", "evaluate", "hd_samples", "hd_dist"]
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("ms", f64 * NUM_KERNELS), ("launches", u64 * NUM_KERNELS), ("face_tests", u64),
+                ("emit_bytes", u64), ("hd_evals", u64), ("hull_jobs", u64), ("face_tests_big", u64)]
+
+    def as_dict(self):
+        return dict(ms={n: self.ms[i] for i, n in enumerate(KERNEL_NAMES)},
+                    launches={n: self.launches[i] for i, n in enumerate(KERNEL_NAMES)},
+                    face_tests=self.face_tests, face_tests_big=self.face_tests_big, emit_bytes=self.emit_bytes,
+                    hd_evals=self.hd_evals, hull_jobs=self.hull_jobs)
+
+
 class CutRecord(C.Structure):
     _fields_ = [("iter", u32), ("part", u32), ("best_cut", i32), ("applied", i32), ("best_cost", f64),
                 ("plane", f32 * 4)]
@@ -98,6 +113,8 @@ TREE_API = {
     "result_part": (C.c_int, [C.c_void_p, u32, P(PartDesc)]),
     "result_part_data": (C.c_int, [C.c_void_p, u32, P(f32), P(u32), P(f32), P(u32), P(f64)]),
     "result_stats": (C.c_int, [C.c_void_p, P(DecomposeStats)]),
+    "result_kernel_stats": (C.c_int, [C.c_void_p, P(KernelStats)]),
+    "tree_kernel_stats": (C.c_int, [C.c_void_p, P(KernelStats)]),
     "result_num_cuts": (u32, [C.c_void_p]),
     "result_cut": (C.c_int, [C.c_void_p, u32, P(CutRecord)]),
     "result_free": (None, [C.c_void_p]),

@@ -43,6 +43,10 @@ struct Counters {
   uint32_t vTop, tTop, eTop, sTop;  // pool bump pointers (floats, u32, edges, scratch words)
   uint32_t overflow, hullFail, npOverflow, pruned;
   unsigned long long hullsDone;
+  unsigned long long faceTests;     // point-face record evaluations, wave-per-part hull kernel
+  unsigned long long emitBytes;     // algorithmic bytes of emit: parent read + children written
+  unsigned long long hdEvals;       // point-triangle distance evaluations
+  unsigned long long faceTestsBig;  // point-face record evaluations, workgroup-per-part hull kernel
   uint32_t hd[1024];  // Hausdorff results (f32 bits of the squared distance)
 };
 

@@ -68,8 +68,15 @@ class GpuBackend : public Backend {
     ctr_.ensure(1);
   }
   ~GpuBackend() override {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (const Pending& pe : pending_) {
+      (void)hipEventDestroy(pe.a);
+      (void)hipEventDestroy(pe.b);
+    }
+    for (hipEvent_t e : evPool_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
+  KernelStats kernelStats() const override { return kst_; }
   const TreeConfig& config() const override { return cfg_; }
   uint32_t planeSlots() const override { return edgeStride_; }
 
@@ -82,7 +89,53 @@ class GpuBackend : public Backend {
   bool readPart(uint32_t partIdx, PartView* pv, Mesh* mesh) override;
 
  private:
-  void sync() { hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+  void sync() {
+    hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    for (const Pending& pe : pending_) {
+      float ms = 0.f;
+      hipCheck(hipEventElapsedTime(&ms, pe.a, pe.b), "hipEventElapsedTime");
+      kst_.ms[pe.k] += ms;
+      kst_.launches[pe.k]++;
+      evPool_.push_back(pe.a);
+      evPool_.push_back(pe.b);
+    }
+    pending_.clear();
+  }
+  // device time of one launch (or a group of launches) of kernel family k, on the launch stream
+  hipEvent_t event() {
+    if (evPool_.empty()) {
+      hipEvent_t e;
+      hipCheck(hipEventCreate(&e), "hipEventCreate");
+      return e;
+    }
+    hipEvent_t e = evPool_.back();
+    evPool_.pop_back();
+    return e;
+  }
+  hipEvent_t tBegin() {
+    hipEvent_t a = event();
+    hipCheck(hipEventRecord(a, stream_), "hipEventRecord");
+    return a;
+  }
+  void tEnd(int k, hipEvent_t a) {
+    hipEvent_t b = event();
+    hipCheck(hipEventRecord(b, stream_), "hipEventRecord");
+    pending_.push_back(Pending{k, a, b});
+  }
+  // work counters of the kernels (cleared at the start of every entry point, summed at its end)
+  void clearWork() {
+    const unsigned long long z[4] = {0, 0, 0, 0};
+    h2d(&ctr_.p->faceTests, z, sizeof(z), "clear work counters");
+  }
+  void collectWork() {
+    unsigned long long w[4];
+    hipCheck(hipMemcpyAsync(w, &ctr_.p->faceTests, sizeof(w), hipMemcpyDeviceToHost, stream_), "read work counters");
+    sync();
+    kst_.faceTests += w[0];
+    kst_.emitBytes += w[1];
+    kst_.hdEvals += w[2];
+    kst_.faceTestsBig += w[3];
+  }
   // host -> device copy ordered on the stream that returns once the host buffer may be reused
   void h2d(void* dst, const void* src, size_t bytes, const char* what) {
     if (bytes) hipCheck(hipMemcpyWithStream(dst, src, bytes, hipMemcpyHostToDevice, stream_), what);
@@ -117,6 +170,13 @@ class GpuBackend : public Backend {
   DevBuf<uint32_t> hdTris_, hdCounts_;
   DevBuf<dk::HdMesh> hdMeshes_;
   DevBuf<dk::HdTile> hdTiles_;
+  struct Pending {
+    int k;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> evPool_;
+  KernelStats kst_;
   // last run
   std::vector<uint32_t> lastLevelCounts_;
   uint32_t lastPartCount_ = 0;
@@ -172,9 +232,15 @@ void GpuBackend::ensurePools() {
   scratch_.ensure(std::min<uint64_t>(cfg_.scratchWords, 0xffffffffull));
 }
 
-void GpuBackend::runHulls(std::vector<dk::HullJob>& hj, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
+void GpuBackend::runHulls(std::vector<dk::HullJob>& hjIn, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
                           std::vector<dk::HullResultD>* results) {
-  if (hj.empty()) return;
+  if (hjIn.empty()) return;
+  // large parts first (a workgroup each), then small ones (a wavefront each)
+  std::vector<uint32_t> order(hjIn.size());
+  for (uint32_t k = 0; k < order.size(); k++) order[k] = k;
+  std::stable_partition(order.begin(), order.end(), [&](uint32_t k) { return hjIn[k].nv >= dk::HBIG_NV; });
+  std::vector<dk::HullJob> hj(hjIn.size());
+  for (size_t k = 0; k < order.size(); k++) hj[k] = hjIn[order[k]];
   const uint64_t budgetWords = std::max<uint64_t>(cfg_.hullBudgetBytes / 4, 1ull << 20);
   uint64_t maxSlab = 0;
   for (const dk::HullJob& j : hj) maxSlab = std::max(maxSlab, dk::hullSlabWords(j.nv));
@@ -193,19 +259,36 @@ void GpuBackend::runHulls(std::vector<dk::HullJob>& hj, dk::DPart* parts, uint32
       used += w;
       end++;
     }
-    const uint32_t n = (uint32_t)(end - start);
-    h2d(hjobs_.p + start, hj.data() + start, n * sizeof(dk::HullJob), "upload hull jobs");
-    dk::k_hull<<<(n + dk::HW - 1) / dk::HW, 64 * dk::HW, 0, stream_>>>(hjobs_.p + start, n, verts_.p, arena_.p, parts,
-                                                                        hres_.p + start, exportBuf, items_.p, jobs_.p, HL,
-                                                                        ctr_.p);
-    launched("k_hull");
+    size_t mid = start;
+    while (mid < end && hj[mid].nv >= dk::HBIG_NV) mid++;
+    h2d(hjobs_.p + start, hj.data() + start, (end - start) * sizeof(dk::HullJob), "upload hull jobs");
+    if (mid > start) {
+      const uint32_t n = (uint32_t)(mid - start);
+      hipEvent_t a = tBegin();
+      dk::k_hull_big<<<n, dk::HBIG, 0, stream_>>>(hjobs_.p + start, n, verts_.p, arena_.p, parts, hres_.p + start,
+                                                  exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+      launched("k_hull_big");
+      tEnd(K_HULL_BIG, a);
+    }
+    if (end > mid) {
+      const uint32_t n = (uint32_t)(end - mid);
+      hipEvent_t a = tBegin();
+      dk::k_hull<<<(n + dk::HW - 1) / dk::HW, 64 * dk::HW, 0, stream_>>>(hjobs_.p + mid, n, verts_.p, arena_.p, parts,
+                                                                          hres_.p + mid, exportBuf, items_.p, jobs_.p,
+                                                                          HL, ctr_.p);
+      launched("k_hull");
+      tEnd(K_HULL, a);
+    }
+    kst_.hullJobs += end - start;
     start = end;
   }
   if (results) {
-    results->resize(hj.size());
-    hipCheck(hipMemcpyAsync(results->data(), hres_.p, hj.size() * sizeof(dk::HullResultD), hipMemcpyDeviceToHost, stream_),
+    std::vector<dk::HullResultD> r(hj.size());
+    hipCheck(hipMemcpyAsync(r.data(), hres_.p, hj.size() * sizeof(dk::HullResultD), hipMemcpyDeviceToHost, stream_),
              "read hull results");
     sync();
+    results->resize(hj.size());
+    for (size_t k = 0; k < order.size(); k++) (*results)[order[k]] = r[k];
   }
 }
 
@@ -298,14 +381,20 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       const Level& L = levels[l];
       const dk::LevelParams P = levelParams(L, l, depth, nSeeds);
       if (L.nJobs == 0) continue;
+      hipEvent_t e0 = tBegin();
       dk::k_plan<<<L.nJobs, dk::BT, 0, stream_>>>(P, items_.p, parts_.p, jobs_.p, jobPlanes_.p, edgePlanes_.p, verts_.p,
                                                   tris_.p, edges_.p, scratch_.p, ctr_.p);
       launched("k_plan");
+      tEnd(K_PLAN, e0);
+      e0 = tBegin();
       dk::k_scan<<<1, dk::ST, 0, stream_>>>(P, items_.p, parts_.p, jobs_.p, ctr_.p);
       launched("k_scan");
+      tEnd(K_SCAN, e0);
+      e0 = tBegin();
       dk::k_emit<<<L.nJobs, dk::BT, 0, stream_>>>(P, parts_.p, jobs_.p, jobPlanes_.p, verts_.p, tris_.p, edges_.p,
-                                                  scratch_.p);
+                                                  scratch_.p, &ctr_.p->emitBytes);
       launched("k_emit");
+      tEnd(K_EMIT, e0);
       // hull jobs for every child with enough points
       const uint32_t nChild = 2 * L.nJobs;
       hostParts.resize(nChild);
@@ -335,8 +424,10 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       HL.depth = depth;
       HL.jobBase = L.jobBase;
       runHulls(hj, parts_.p, nullptr, HL, nullptr);
+      hipEvent_t e1 = tBegin();
       dk::k_record<<<(L.nJobs + 255) / 256, 256, 0, stream_>>>(P, items_.p, parts_.p);
       launched("k_record");
+      tEnd(K_RECORD, e1);
     }
     dk::EvalParams E;
     std::memset(&E, 0, sizeof(E));
@@ -351,8 +442,10 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       E.outBase[l] = levels[l].itemsOutBase;
       E.nJobs[l] = levels[l].nJobs;
     }
+    hipEvent_t e2 = tBegin();
     dk::k_evaluate<<<nSeeds, dk::BT, 0, stream_>>>(E, items_.p, jobPlanes_.p, edgePlanes_.p, state_.p);
     launched("k_evaluate");
+    tEnd(K_EVALUATE, e2);
     lastLevelCounts_.assign(1, nSeeds);
     for (const Level& L : levels) lastLevelCounts_.push_back(L.nJobs);
   }
@@ -378,6 +471,10 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
   R.hullsDone = c1.hullsDone;
   for (const Level& L : treeLevels) R.jobsPerLevel.push_back(L.nJobs);
   R.gpuMs = nowMs() - t0;
+  kst_.faceTests += c1.faceTests;
+  kst_.emitBytes += c1.emitBytes;
+  kst_.hdEvals += c1.hdEvals;
+  kst_.faceTestsBig += c1.faceTestsBig;
   return R;
 }
 
@@ -481,7 +578,9 @@ std::vector<HullResult> GpuBackend::hullMeshes(const std::vector<MeshRef>& meshe
   dk::HullLevel HL;
   std::memset(&HL, 0, sizeof(HL));
   std::vector<dk::HullResultD> res;
+  clearWork();
   runHulls(hj, nullptr, export_.p, HL, &res);
+  collectWork();
   std::vector<uint32_t> faces(exportTotal);
   hipCheck(hipMemcpyAsync(faces.data(), export_.p, exportTotal * 4, hipMemcpyDeviceToHost, stream_), "read hull faces");
   sync();
@@ -533,10 +632,14 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
   if (!pt.empty()) h2d(hdTris_.p, pt.data(), pt.size() * 4, "upload hd tris");
   h2d(hdCounts_.p, counts.data(), counts.size() * 4, "upload counts");
   h2d(hdMeshes_.p, hm.data(), hm.size() * sizeof(dk::HdMesh), "upload hd meshes");
-  if (tTotal)
+  clearWork();
+  if (tTotal) {
+    hipEvent_t a = tBegin();
     dk::k_hd_samples<<<(uint32_t)((tTotal + 255) / 256), 256, 0, stream_>>>(hdMeshes_.p, (uint32_t)hm.size(), (uint32_t)tTotal,
                                                                           hdVerts_.p, hdTris_.p, hdCounts_.p, hdSamples_.p);
-  launched("k_hd_samples");
+    launched("k_hd_samples");
+    tEnd(K_HD_SAMPLES, a);
+  }
   std::vector<dk::HdTile> tiles;
   for (size_t p = 0; p < n; p++) {
     if (a[p].nt == 0 || b[p].nt == 0) continue;  // src/cpu/hausdorff.ts:227
@@ -560,13 +663,15 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
   if (!tiles.empty()) {
     hdTiles_.ensure(tiles.size());
     h2d(hdTiles_.p, tiles.data(), tiles.size() * sizeof(dk::HdTile), "upload tiles");
+    hipEvent_t a = tBegin();
     dk::k_hd_dist<<<(uint32_t)tiles.size(), dk::HDT, 0, stream_>>>(hdTiles_.p, (uint32_t)tiles.size(), hdSamples_.p,
-                                                                  hdVerts_.p, hdTris_.p, ctr_.p->hd);
+                                                                  hdVerts_.p, hdTris_.p, ctr_.p->hd, &ctr_.p->hdEvals);
     launched("k_hd_dist");
+    tEnd(K_HD_DIST, a);
   }
   std::vector<uint32_t> bits(n);
   hipCheck(hipMemcpyAsync(bits.data(), ctr_.p->hd, n * 4, hipMemcpyDeviceToHost, stream_), "read hd results");
-  sync();
+  collectWork();
   for (size_t p = 0; p < n; p++) {
     float f;
     std::memcpy(&f, &bits[p], 4);

@@ -122,7 +122,8 @@ struct HdTile {  // 256 samples of one mesh against all triangles of the other m
 constexpr int HDT = 256;
 __global__ __launch_bounds__(HDT) void k_hd_dist(const HdTile* __restrict__ tiles, uint32_t nTiles,
                                                  const float* __restrict__ samples, const float* __restrict__ verts,
-                                                 const uint32_t* __restrict__ tris, uint32_t* __restrict__ result) {
+                                                 const uint32_t* __restrict__ tris, uint32_t* __restrict__ result,
+                                                 unsigned long long* evals) {
   const uint32_t b = blockIdx.x;
   if (b >= nTiles) return;
   const HdTile Tl = tiles[b];
@@ -164,6 +165,7 @@ __global__ __launch_bounds__(HDT) void k_hd_dist(const HdTile* __restrict__ tile
     float w = s_red[0];
     for (int k = 1; k < HDT / 64; k++) w = s_red[k] > w ? s_red[k] : w;
     atomicMax(result + Tl.pair, __float_as_uint(w));
+    atomicAdd(evals, (unsigned long long)Tl.sCount * Tl.nt);
   }
 }
 

@@ -20,7 +20,9 @@
 
 namespace dk {
 
-constexpr int HW = 4;  // waves per block in the hull kernel
+constexpr int HW = 4;         // parts per block in the wave-per-part hull kernel
+constexpr int HBIG = 1024;    // threads per part in the workgroup-per-part hull kernel
+constexpr uint32_t HBIG_NV = 2048;  // parts with at least this many points get a whole workgroup
 constexpr uint32_t ALIVE = 0x80000000u;
 constexpr uint32_t FW = 12;  // words per face record
 
@@ -54,6 +56,168 @@ __constant__ float KDIR[40][3] = {
     {-9.950323105e-01f, -9.221937507e-02f, 3.750000149e-02f}, {7.964972258e-01f, -6.045129895e-01f, 1.250000019e-02f},
 };
 
+// ---- thread groups: one wavefront, or one workgroup of G threads -----------------
+// Every operation returns the same value in every thread of the group, so the
+// hull's control flow is uniform whatever the group size, and the decisions do
+// not depend on it (ties are resolved by value, then by index).
+struct WaveGrp {
+  static constexpr uint32_t G = 64;
+  __device__ __forceinline__ uint32_t tid() const { return lane_id(); }
+  __device__ __forceinline__ void sync() const { wave_sync(); }
+  __device__ __forceinline__ uint32_t rank(bool f, uint32_t* total) const {
+    const uint64_t m = __ballot(f);
+    *total = (uint32_t)__popcll(m);
+    return prefix_below(m);
+  }
+  __device__ __forceinline__ float minf(float v) const { return wave_min_f(v); }
+  __device__ __forceinline__ float maxf(float v) const { return wave_max_f(v); }
+  __device__ __forceinline__ double mind(double v) const {
+    for (int m = 32; m >= 1; m >>= 1) {
+      const double o = __shfl_xor(v, m, 64);
+      v = o < v ? o : v;
+    }
+    return v;
+  }
+  __device__ __forceinline__ i128 sum(i128 v) const { return wave_sum_i128(v); }
+  __device__ __forceinline__ void argmax(float& v, uint32_t& i) const { wave_argmax(v, i); }
+  __device__ __forceinline__ void argmin(float& v, uint32_t& i) const { wave_argmin(v, i); }
+  // largest m, then lowest k (carrying s)
+  __device__ __forceinline__ void argmax128(i128& m, uint32_t& k, int& s) const {
+    for (int d = 32; d >= 1; d >>= 1) {
+      const i128 om = shfl_xor_i128(m, d);
+      const uint32_t ok = __shfl_xor(k, d, 64);
+      const int os = __shfl_xor(s, d, 64);
+      if (om > m || (om == m && ok < k)) {
+        m = om;
+        k = ok;
+        s = os;
+      }
+    }
+  }
+};
+
+template <int NT>
+struct BlockGrp {
+  static constexpr uint32_t G = NT;
+  static constexpr uint32_t NW = NT / 64;
+  uint32_t* su;            // >= NW
+  float* sf;               // >= NW
+  unsigned long long* sl;  // >= 3 * NW
+  __device__ __forceinline__ uint32_t tid() const { return threadIdx.x; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+  __device__ __forceinline__ uint32_t rank(bool f, uint32_t* total) const {
+    const uint64_t m = __ballot(f);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) su[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t k = 0; k < NW; k++) {
+      const uint32_t c = su[k];
+      before += k < w ? c : 0;
+      all += c;
+    }
+    __syncthreads();
+    *total = all;
+    return before + prefix_below(m);
+  }
+  __device__ __forceinline__ float minf(float v) const {
+    v = wave_min_f(v);
+    if ((threadIdx.x & 63) == 0) sf[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = sf[0];
+    for (uint32_t k = 1; k < NW; k++) r = sf[k] < r ? sf[k] : r;
+    __syncthreads();
+    return r;
+  }
+  __device__ __forceinline__ float maxf(float v) const {
+    v = wave_max_f(v);
+    if ((threadIdx.x & 63) == 0) sf[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float r = sf[0];
+    for (uint32_t k = 1; k < NW; k++) r = sf[k] > r ? sf[k] : r;
+    __syncthreads();
+    return r;
+  }
+  __device__ __forceinline__ double mind(double v) const {
+    for (int m = 32; m >= 1; m >>= 1) {
+      const double o = __shfl_xor(v, m, 64);
+      v = o < v ? o : v;
+    }
+    double* sd = reinterpret_cast<double*>(sl);
+    if ((threadIdx.x & 63) == 0) sd[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = sd[0];
+    for (uint32_t k = 1; k < NW; k++) r = sd[k] < r ? sd[k] : r;
+    __syncthreads();
+    return r;
+  }
+  __device__ __forceinline__ i128 sum(i128 v) const {
+    v = wave_sum_i128(v);
+    if ((threadIdx.x & 63) == 0) {
+      sl[2 * (threadIdx.x >> 6)] = (unsigned long long)(uint64_t)v;
+      sl[2 * (threadIdx.x >> 6) + 1] = (unsigned long long)(uint64_t)(v >> 64);
+    }
+    __syncthreads();
+    i128 r = 0;
+    for (uint32_t k = 0; k < NW; k++) r += (i128)(((unsigned __int128)sl[2 * k + 1] << 64) | sl[2 * k]);
+    __syncthreads();
+    return r;
+  }
+  __device__ __forceinline__ void argmax(float& v, uint32_t& i) const {
+    wave_argmax(v, i);
+    if ((threadIdx.x & 63) == 0) {
+      sf[threadIdx.x >> 6] = v;
+      su[threadIdx.x >> 6] = i;
+    }
+    __syncthreads();
+    v = sf[0];
+    i = su[0];
+    for (uint32_t k = 1; k < NW; k++)
+      if (sf[k] > v || (sf[k] == v && su[k] < i)) {
+        v = sf[k];
+        i = su[k];
+      }
+    __syncthreads();
+  }
+  __device__ __forceinline__ void argmin(float& v, uint32_t& i) const {
+    wave_argmin(v, i);
+    if ((threadIdx.x & 63) == 0) {
+      sf[threadIdx.x >> 6] = v;
+      su[threadIdx.x >> 6] = i;
+    }
+    __syncthreads();
+    v = sf[0];
+    i = su[0];
+    for (uint32_t k = 1; k < NW; k++)
+      if (sf[k] < v || (sf[k] == v && su[k] < i)) {
+        v = sf[k];
+        i = su[k];
+      }
+    __syncthreads();
+  }
+  __device__ __forceinline__ void argmax128(i128& m, uint32_t& k, int& s) const {
+    WaveGrp().argmax128(m, k, s);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      sl[3 * w] = (unsigned long long)(uint64_t)m;
+      sl[3 * w + 1] = (unsigned long long)(uint64_t)(m >> 64);
+      sl[3 * w + 2] = ((unsigned long long)k << 32) | (uint32_t)s;
+    }
+    __syncthreads();
+    for (uint32_t q = 0; q < NW; q++) {
+      const i128 om = (i128)(((unsigned __int128)sl[3 * q + 1] << 64) | sl[3 * q]);
+      const uint32_t ok = (uint32_t)(sl[3 * q + 2] >> 32);
+      const int os = (int)(uint32_t)sl[3 * q + 2];
+      if (om > m || (om == m && ok < k)) {
+        m = om;
+        k = ok;
+        s = os;
+      }
+    }
+    __syncthreads();
+  }
+};
+
 struct HullCtx {
   const float* V;  // original coordinates
   int32_t* Q;      // quantized
@@ -65,6 +229,7 @@ struct HullCtx {
   uint32_t* F;  // face records
   uint32_t nv, fmax;
   uint32_t nf, nalive, ncand, nIns;
+  unsigned long long tests;  // point-face record evaluations by this thread
 };
 
 __device__ __forceinline__ void face_write(HullCtx& C, uint32_t f, uint32_t a, uint32_t b, uint32_t c) {
@@ -81,19 +246,11 @@ __device__ __forceinline__ void face_write(HullCtx& C, uint32_t f, uint32_t a, u
   float nn = n0 * n0;
   nn = nn + n1 * n1;
   nn = nn + n2 * n2;
-  uint32_t* r = C.F + (size_t)FW * f;
-  r[0] = a | ALIVE;
-  r[1] = b;
-  r[2] = c;
-  r[3] = 0;
-  r[4] = __float_as_uint(n0);
-  r[5] = __float_as_uint(n1);
-  r[6] = __float_as_uint(n2);
-  r[7] = __float_as_uint(s1 * s2);
-  r[8] = __float_as_uint((float)qa[0]);
-  r[9] = __float_as_uint((float)qa[1]);
-  r[10] = __float_as_uint((float)qa[2]);
-  r[11] = __float_as_uint(sqrtf(nn));
+  uint4* r4 = reinterpret_cast<uint4*>(C.F + (size_t)FW * f);
+  r4[0] = make_uint4(a | ALIVE, b, c, 0u);
+  r4[1] = make_uint4(__float_as_uint(n0), __float_as_uint(n1), __float_as_uint(n2), __float_as_uint(s1 * s2));
+  r4[2] = make_uint4(__float_as_uint((float)qa[0]), __float_as_uint((float)qa[1]), __float_as_uint((float)qa[2]),
+                     __float_as_uint(sqrtf(nn)));
 }
 
 __device__ __forceinline__ int orient_exact(const HullCtx& C, uint32_t a, uint32_t b, uint32_t c, uint32_t p) {
@@ -138,6 +295,7 @@ __device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, uint32_
 
 // best face among [f0, f1) in creation order (first on ties), for one point
 __device__ __forceinline__ void assign_point(HullCtx& C, uint32_t i, uint32_t f0, uint32_t f1) {
+  C.tests += f1 - f0;
   uint32_t bestF = NONE;
   float bestD = -1.0f;
   for (uint32_t f = f0; f < f1; f++) {
@@ -151,25 +309,25 @@ __device__ __forceinline__ void assign_point(HullCtx& C, uint32_t i, uint32_t f0
   C.dist[i] = bestF != NONE ? bestD : 0.0f;
 }
 
-__device__ __forceinline__ double hull_volume(const HullCtx& C) {
-  const uint32_t lane = lane_id();
+template <typename Grp>
+__device__ double hull_volume(const Grp& g, const HullCtx& C) {
   i128 acc = 0;
-  for (uint32_t f = lane; f < C.nf; f += 64) {
+  for (uint32_t f = g.tid(); f < C.nf; f += Grp::G) {
     const uint32_t* r = C.F + (size_t)FW * f;
     if (!(r[0] & ALIVE)) continue;
     const uint32_t a = r[0] & ~ALIVE, b = r[1], c = r[2];
     acc_det(acc, C.V + 3 * a, C.V + 3 * b, C.V + 3 * c, false);
   }
-  acc = wave_sum_i128(acc);
+  acc = g.sum(acc);
   return fix_to_f64(fix_abs(acc)) / 6.0;
 }
 
 // stable in-place removal of dead face slots; owners of outstanding points follow
-__device__ void compact_faces(HullCtx& C) {
-  const uint32_t lane = lane_id();
+template <typename Grp>
+__device__ void compact_faces(const Grp& g, HullCtx& C) {
   uint32_t w = 0;
-  for (uint32_t base = 0; base < C.nf; base += 64) {
-    const uint32_t f = base + lane;
+  for (uint32_t base = 0; base < C.nf; base += Grp::G) {
+    const uint32_t f = base + g.tid();
     uint4 r[3];
     bool alive = false;
     if (f < C.nf) {
@@ -179,9 +337,9 @@ __device__ void compact_faces(HullCtx& C) {
       r[2] = r4[2];
       alive = (r[0].x & ALIVE) != 0;
     }
-    const uint64_t m = __ballot(alive);
-    const uint32_t dst = w + prefix_below(m);
-    wave_sync();  // every lane holds its record before any lane overwrites a slot
+    uint32_t cnt;
+    const uint32_t dst = w + g.rank(alive, &cnt);
+    g.sync();  // every thread holds its record before any thread overwrites a slot
     if (f < C.nf) C.map[f] = alive ? dst : NONE;
     if (alive) {
       uint4* d4 = reinterpret_cast<uint4*>(C.F + (size_t)FW * dst);
@@ -189,47 +347,50 @@ __device__ void compact_faces(HullCtx& C) {
       d4[1] = r[1];
       d4[2] = r[2];
     }
-    w += (uint32_t)__popcll(m);
-    wave_sync();
+    w += cnt;
+    g.sync();
   }
-  for (uint32_t e = lane; e < C.ncand; e += 64) {
+  for (uint32_t e = g.tid(); e < C.ncand; e += Grp::G) {
     const uint32_t i = C.cand[e];
     const uint32_t o = C.owner[i];
     if (o != NONE) C.owner[i] = C.map[o];
   }
   C.nf = w;
   C.nalive = w;
-  wave_sync();
+  g.sync();
 }
 
 // insert point p: visible faces, horizon, new faces, reassignment
-__device__ void hull_insert(HullCtx& C, uint32_t p) {
-  const uint32_t lane = lane_id();
-  if (C.nf > 2 * C.nalive + 128 || C.nf + C.nv + 8 > C.fmax) compact_faces(C);
+template <typename Grp>
+__device__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p) {
+  const uint32_t t = g.tid();
+  if (C.nf > 2 * C.nalive + 128 || C.nf + C.nv + 8 > C.fmax) compact_faces(g, C);
   // visible faces in creation order
   uint32_t nvis = 0;
-  for (uint32_t base = 0; base < C.nf; base += 64) {
-    const uint32_t f = base + lane;
+  for (uint32_t base = 0; base < C.nf; base += Grp::G) {
+    const uint32_t f = base + t;
     const bool v = f < C.nf && face_eval(C, f, p) > 0.0f;
-    const uint64_t m = __ballot(v);
-    if (v) C.vis[nvis + prefix_below(m)] = f;
-    nvis += (uint32_t)__popcll(m);
+    C.tests += f < C.nf ? 1 : 0;
+    uint32_t cnt;
+    const uint32_t r = g.rank(v, &cnt);
+    if (v) C.vis[nvis + r] = f;
+    nvis += cnt;
   }
-  wave_sync();
-  if (lane == 0) {
+  g.sync();
+  if (t == 0) {
     C.owner[p] = NONE;
     C.dist[p] = 0.0f;
   }
   C.nIns++;
   if (nvis == 0) {
-    wave_sync();
+    g.sync();
     return;
   }
   // horizon edges in (visible face, edge) order; each becomes a new face (a, b, p)
   const uint32_t f0 = C.nf;
   uint32_t nnew = 0;
-  for (uint32_t base = 0; base < 3 * nvis; base += 64) {
-    const uint32_t e = base + lane;
+  for (uint32_t base = 0; base < 3 * nvis; base += Grp::G) {
+    const uint32_t e = base + t;
     bool hor = false;
     uint32_t a = 0, b = 0;
     if (e < 3 * nvis) {
@@ -240,42 +401,43 @@ __device__ void hull_insert(HullCtx& C, uint32_t p) {
       b = k == 0 ? fb : (k == 1 ? fc : fa);
       hor = true;
       for (uint32_t j = 0; j < nvis; j++) {
-        const uint32_t* g = C.F + (size_t)FW * C.vis[j];
-        const uint32_t ga = g[0] & ~ALIVE, gb = g[1], gc = g[2];
+        const uint32_t* q = C.F + (size_t)FW * C.vis[j];
+        const uint32_t ga = q[0] & ~ALIVE, gb = q[1], gc = q[2];
         if ((ga == b && gb == a) || (gb == b && gc == a) || (gc == b && ga == a)) {
           hor = false;
           break;
         }
       }
     }
-    const uint64_t m = __ballot(hor);
-    if (hor) face_write(C, f0 + nnew + prefix_below(m), a, b, p);
-    nnew += (uint32_t)__popcll(m);
+    uint32_t cnt;
+    const uint32_t r = g.rank(hor, &cnt);
+    if (hor) face_write(C, f0 + nnew + r, a, b, p);
+    nnew += cnt;
   }
-  wave_sync();
-  for (uint32_t vi = lane; vi < nvis; vi += 64) C.F[(size_t)FW * C.vis[vi]] &= ~ALIVE;
+  g.sync();
+  for (uint32_t vi = t; vi < nvis; vi += Grp::G) C.F[(size_t)FW * C.vis[vi]] &= ~ALIVE;
   C.nf = f0 + nnew;
   C.nalive = C.nalive + nnew - nvis;
-  wave_sync();
+  g.sync();
   // points whose owner died see only the new faces
-  for (uint32_t e = lane; e < C.ncand; e += 64) {
+  for (uint32_t e = t; e < C.ncand; e += Grp::G) {
     const uint32_t i = C.cand[e];
     const uint32_t o = C.owner[i];
     if (o == NONE || (C.F[(size_t)FW * o] & ALIVE)) continue;
     assign_point(C, i, f0, C.nf);
   }
-  wave_sync();
+  g.sync();
 }
 
 // farthest outstanding candidate (largest distance, then lowest index) until none
-__device__ void hull_grow(HullCtx& C) {
-  const uint32_t lane = lane_id();
+template <typename Grp>
+__device__ void hull_grow(const Grp& g, HullCtx& C) {
   uint32_t since = 0;
   const uint32_t maxIter = C.nv + 8;
   for (uint32_t it = 0; it < maxIter; it++) {
     float best = -1.0f;
     uint32_t bi = NONE;
-    for (uint32_t e = lane; e < C.ncand; e += 64) {
+    for (uint32_t e = g.tid(); e < C.ncand; e += Grp::G) {
       const uint32_t i = C.cand[e];
       if (C.owner[i] == NONE) continue;
       const float d = C.dist[i];
@@ -284,25 +446,26 @@ __device__ void hull_grow(HullCtx& C) {
         bi = i;
       }
     }
-    wave_argmax(best, bi);
+    g.argmax(best, bi);
     if (bi == NONE) return;
-    hull_insert(C, bi);
+    hull_insert(g, C, bi);
     if (++since >= 16) {
       since = 0;
       uint32_t w = 0;
-      for (uint32_t base = 0; base < C.ncand; base += 64) {
-        const uint32_t e = base + lane;
+      for (uint32_t base = 0; base < C.ncand; base += Grp::G) {
+        const uint32_t e = base + g.tid();
         uint32_t i = 0;
         bool keep = false;
         if (e < C.ncand) {
           i = C.cand[e];
           keep = C.owner[i] != NONE;
         }
-        const uint64_t m = __ballot(keep);
-        wave_sync();
-        if (keep) C.cand[w + prefix_below(m)] = i;
-        w += (uint32_t)__popcll(m);
-        wave_sync();
+        uint32_t cnt;
+        const uint32_t r = g.rank(keep, &cnt);
+        g.sync();
+        if (keep) C.cand[w + r] = i;
+        w += cnt;
+        g.sync();
       }
       C.ncand = w;
     }
@@ -313,18 +476,13 @@ struct HullLevel {  // branch-and-bound inputs at the last tree level
   uint32_t prune, itemsInBase, nItemsIn, nSeeds, depth, jobBase;
 };
 
-__global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hjobs, uint32_t nJobsH,
-                                                  const float* __restrict__ verts, uint32_t* __restrict__ arena,
-                                                  DPart* __restrict__ parts, HullResultD* __restrict__ results,
-                                                  uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items,
-                                                  const DJob* __restrict__ jobs, HullLevel HL,
-                                                  Counters* __restrict__ ctr) {
-  const uint32_t wid = threadIdx.x >> 6;
-  const uint32_t lane = lane_id();
-  const uint32_t hj = blockIdx.x * HW + wid;
-  if (hj >= nJobsH) return;
-  __shared__ uint32_t s_seed[HW][80];
-  const HullJob J = hjobs[hj];
+// one hull job on one group; seed points in `seed` (80 entries of group-shared memory)
+template <typename Grp>
+__device__ void hull_job(const Grp& g, const dk::HullJob& J, uint32_t hj, uint32_t* seed, const float* __restrict__ verts,
+                         uint32_t* __restrict__ arena, DPart* __restrict__ parts, HullResultD* __restrict__ results,
+                         uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items, const DJob* __restrict__ jobs,
+                         const HullLevel& HL, Counters* __restrict__ ctr, unsigned long long* testsOut) {
+  const uint32_t t = g.tid();
   HullResultD res;
   res.volume = 0.0;
   res.nFaces = 0;
@@ -347,6 +505,7 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
   C.vis = C.cand + cn;
   C.map = C.vis + C.fmax;
   C.nf = C.nalive = C.ncand = C.nIns = 0;
+  C.tests = 0;
   bool done = false;
   if (J.nv < 4) done = true;
   else if (hullSlabWords(J.nv) > J.slabWords) {
@@ -356,33 +515,33 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
   if (!done) {
     // bbox -> 2^21 grid
     float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-    for (uint32_t i = lane; i < C.nv; i += 64)
+    for (uint32_t i = t; i < C.nv; i += Grp::G)
       for (int k = 0; k < 3; k++) {
         const float x = C.V[3 * i + k];
         lo[k] = x < lo[k] ? x : lo[k];
         hi[k] = x > hi[k] ? x : hi[k];
       }
     for (int k = 0; k < 3; k++) {
-      lo[k] = wave_min_f(lo[k]);
-      hi[k] = wave_max_f(hi[k]);
+      lo[k] = g.minf(lo[k]);
+      hi[k] = g.maxf(hi[k]);
     }
     const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
     float scale = ex > ey ? ex : ey;
     scale = scale > ez ? scale : ez;
     scale = scale > 1e-30f ? scale : 1e-30f;
     const float qs = 2097152.0f / scale;
-    for (uint32_t i = lane; i < C.nv; i += 64)
+    for (uint32_t i = t; i < C.nv; i += Grp::G)
       for (int k = 0; k < 3; k++) {
         const float d = C.V[3 * i + k] - lo[k];
         C.Q[3 * i + k] = (int32_t)rintf(d * qs);
       }
-    wave_sync();
+    g.sync();
     // seeds: maximum and minimum along each direction (lowest index on ties)
     for (int k = 0; k < 40; k++) {
       const float ax = KDIR[k][0], ay = KDIR[k][1], az = KDIR[k][2];
       float mx = -1e30f, mn = 1e30f;
-      uint32_t ix = 0, in = 0;
-      for (uint32_t i = lane; i < C.nv; i += 64) {
+      uint32_t ix = NONE, in = NONE;
+      for (uint32_t i = t; i < C.nv; i += Grp::G) {
         const int32_t* q = C.Q + 3 * i;
         float d = ax * (float)q[0];
         d = d + ay * (float)q[1];
@@ -396,19 +555,14 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
           in = i;
         }
       }
-      if (lane >= C.nv) {
-        ix = NONE;
-        in = NONE;
-      }
-      wave_argmax(mx, ix);
-      wave_argmin(mn, in);
-      if (lane == 0) {
-        s_seed[wid][2 * k] = ix;
-        s_seed[wid][2 * k + 1] = in;
+      g.argmax(mx, ix);
+      g.argmin(mn, in);
+      if (t == 0) {
+        seed[2 * k] = ix;
+        seed[2 * k + 1] = in;
       }
     }
-    wave_sync();
-    const uint32_t* seed = s_seed[wid];
+    g.sync();
     // simplex from the seeds (exact integer tests, first seed on ties)
     const uint32_t i0 = seed[1], i1 = seed[0];
     const int32_t* q0 = C.Q + 3 * i0;
@@ -416,7 +570,8 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
     const int64_t d01[3] = {(int64_t)q1[0] - q0[0], (int64_t)q1[1] - q0[1], (int64_t)q1[2] - q0[2]};
     i128 bm = 0;
     uint32_t bk = NONE;
-    for (uint32_t k = lane; k < 80; k += 64) {
+    int bsg = 0;
+    for (uint32_t k = t; k < 80; k += Grp::G) {
       const int32_t* qs2 = C.Q + 3 * seed[k];
       const int64_t d[3] = {(int64_t)qs2[0] - q0[0], (int64_t)qs2[1] - q0[1], (int64_t)qs2[2] - q0[2]};
       const int64_t cx = d01[1] * d[2] - d01[2] * d[1], cy = d01[2] * d[0] - d01[0] * d[2], cz = d01[0] * d[1] - d01[1] * d[0];
@@ -426,15 +581,8 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
         bk = k;
       }
     }
-    for (int m = 32; m >= 1; m >>= 1) {
-      const i128 om = shfl_xor_i128(bm, m);
-      const uint32_t ok = __shfl_xor(bk, m, 64);
-      if (om > bm || (om == bm && ok < bk)) {
-        bm = om;
-        bk = ok;
-      }
-    }
-    uint32_t i2 = bk == NONE ? NONE : seed[bk];
+    g.argmax128(bm, bk, bsg);
+    const uint32_t i2 = bk == NONE ? NONE : seed[bk];
     uint32_t i3 = NONE;
     int bs = 0;
     if (i2 != NONE) {
@@ -444,8 +592,8 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
                              d01[0] * d02[1] - d01[1] * d02[0]};
       bm = 0;
       bk = NONE;
-      int bsg = 0;
-      for (uint32_t k = lane; k < 80; k += 64) {
+      bsg = 0;
+      for (uint32_t k = t; k < 80; k += Grp::G) {
         const int32_t* qs2 = C.Q + 3 * seed[k];
         const i128 dt = (i128)nr[0] * ((int64_t)qs2[0] - q0[0]) + (i128)nr[1] * ((int64_t)qs2[1] - q0[1]) +
                         (i128)nr[2] * ((int64_t)qs2[2] - q0[2]);
@@ -456,16 +604,7 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
           bsg = dt > 0 ? 1 : -1;
         }
       }
-      for (int m = 32; m >= 1; m >>= 1) {
-        const i128 om = shfl_xor_i128(bm, m);
-        const uint32_t ok = __shfl_xor(bk, m, 64);
-        const int os = __shfl_xor(bsg, m, 64);
-        if (om > bm || (om == bm && ok < bk)) {
-          bm = om;
-          bk = ok;
-          bsg = os;
-        }
-      }
+      g.argmax128(bm, bk, bsg);
       if (bk != NONE) {
         i3 = seed[bk];
         bs = bsg;
@@ -479,42 +618,39 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
         b = i2;
         c = i1;
       }
-      if (lane == 0) face_write(C, 0, a, b, c);
-      if (lane == 1) face_write(C, 1, a, i3, b);
-      if (lane == 2) face_write(C, 2, b, i3, c);
-      if (lane == 3) face_write(C, 3, c, i3, a);
+      if (t == 0) face_write(C, 0, a, b, c);
+      if (t == 1) face_write(C, 1, a, i3, b);
+      if (t == 2) face_write(C, 2, b, i3, c);
+      if (t == 3) face_write(C, 3, c, i3, a);
       C.nf = C.nalive = 4;
-      wave_sync();
+      g.sync();
       // phase 1: the seeds (duplicates assign identically)
-      for (uint32_t k = lane; k < 80; k += 64) {
+      for (uint32_t k = t; k < 80; k += Grp::G) {
         C.cand[k] = seed[k];
         assign_point(C, seed[k], 0, 4);
       }
       C.ncand = 80;
-      wave_sync();
-      hull_grow(C);
+      g.sync();
+      hull_grow(g, C);
       res.status = 0;
       if (HL.prune) {
         // branch-and-bound: rough-hull rv lower bound vs the seed's best upper bound
-        const double vRough = hull_volume(C);
+        const double vRough = hull_volume(g, C);
         const DItem pit = items[HL.itemsInBase + jobs[J.job].parentItem];
         double partial = 0.0;
         for (uint32_t l = 0; l < pit.nLevels; l++) partial += pit.levelCosts[l];
         const uint32_t perSeed = HL.nItemsIn / HL.nSeeds;
         double ub = 1e30;
-        for (uint32_t i = lane; i < perSeed; i += 64) {
+        for (uint32_t i = t; i < perSeed; i += Grp::G) {
           const DItem& it = items[HL.itemsInBase + pit.srcPart * perSeed + i];
           if (it.flags == 1 && it.nLevels > 0) {
             double ps = 0.0;
             for (uint32_t l = 0; l < it.nLevels; l++) ps += it.levelCosts[l];
-            const double b = ps + it.levelCosts[it.nLevels - 1] * (double)(HL.depth - it.nLevels);
-            ub = b < ub ? b : ub;
+            const double bnd = ps + it.levelCosts[it.nLevels - 1] * (double)(HL.depth - it.nLevels);
+            ub = bnd < ub ? bnd : ub;
           }
         }
-        for (int m = 32; m >= 1; m >>= 1) {
-          const double o = __shfl_xor(ub, m, 64);
-          ub = o < ub ? o : ub;
-        }
+        ub = g.mind(ub);
         const double meshVol = parts ? parts[J.part].meshVol : 0.0;
         if (partial + cuacd::rv_cost(meshVol, vRough) > ub) {
           res.status = 2;
@@ -527,43 +663,47 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
         // phase 2: every point against the seed hull; outside points become candidates
         uint32_t nc = 0;
         const uint32_t nf = C.nf;
-        for (uint32_t base = 0; base < C.nv; base += 64) {
-          const uint32_t i = base + lane;
+        for (uint32_t base = 0; base < C.nv; base += Grp::G) {
+          const uint32_t i = base + t;
           bool out = false;
           if (i < C.nv) {
             assign_point(C, i, 0, nf);
             out = C.owner[i] != NONE;
           }
-          const uint64_t m = __ballot(out);
-          if (out) C.cand[nc + prefix_below(m)] = i;
-          nc += (uint32_t)__popcll(m);
+          uint32_t cnt;
+          const uint32_t r = g.rank(out, &cnt);
+          if (out) C.cand[nc + r] = i;
+          nc += cnt;
         }
         C.ncand = nc;
         res.nCand = nc;
-        wave_sync();
-        hull_grow(C);
-        res.volume = hull_volume(C);
+        g.sync();
+        hull_grow(g, C);
+        res.volume = hull_volume(g, C);
         // alive faces in creation order: count and optional export
         uint32_t w = 0;
-        for (uint32_t base = 0; base < C.nf; base += 64) {
-          const uint32_t f = base + lane;
+        for (uint32_t base = 0; base < C.nf; base += Grp::G) {
+          const uint32_t f = base + t;
           const uint32_t* r = C.F + (size_t)FW * f;
           const bool alive = f < C.nf && (r[0] & ALIVE);
-          const uint64_t m = __ballot(alive);
+          uint32_t cnt;
+          const uint32_t rk = g.rank(alive, &cnt);
           if (alive && J.exportOff != NONE) {
-            uint32_t* d = exportBuf + J.exportOff + 3 * (w + prefix_below(m));
+            uint32_t* d = exportBuf + J.exportOff + 3 * (w + rk);
             d[0] = r[0] & ~ALIVE;
             d[1] = r[1];
             d[2] = r[2];
           }
-          w += (uint32_t)__popcll(m);
+          w += cnt;
         }
         res.nFaces = w;
       }
     }
     res.nIns = C.nIns;
   }
-  if (lane == 0) {
+  const i128 tests = g.sum((i128)C.tests);
+  if (t == 0) {
+    atomicAdd(testsOut, (unsigned long long)tests);
     results[hj] = res;
     if (parts) {
       DPart& pd = parts[J.part];
@@ -576,6 +716,41 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
     if (res.status == 3) atomicAdd(&ctr->hullFail, 1u);
     atomicAdd(&ctr->hullsDone, 1ull);
   }
+}
+
+// small parts: one wavefront per part, HW parts per block
+__global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hjobs, uint32_t nJobsH,
+                                                  const float* __restrict__ verts, uint32_t* __restrict__ arena,
+                                                  DPart* __restrict__ parts, HullResultD* __restrict__ results,
+                                                  uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items,
+                                                  const DJob* __restrict__ jobs, HullLevel HL,
+                                                  Counters* __restrict__ ctr) {
+  const uint32_t wid = threadIdx.x >> 6;
+  const uint32_t hj = blockIdx.x * HW + wid;
+  if (hj >= nJobsH) return;
+  __shared__ uint32_t s_seed[HW][80];
+  const WaveGrp g;
+  hull_job(g, hjobs[hj], hj, s_seed[wid], verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTests);
+}
+
+// large parts: one workgroup of HBIG threads per part
+__global__ __launch_bounds__(HBIG) void k_hull_big(const HullJob* __restrict__ hjobs, uint32_t nJobsH,
+                                                   const float* __restrict__ verts, uint32_t* __restrict__ arena,
+                                                   DPart* __restrict__ parts, HullResultD* __restrict__ results,
+                                                   uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items,
+                                                   const DJob* __restrict__ jobs, HullLevel HL,
+                                                   Counters* __restrict__ ctr) {
+  const uint32_t hj = blockIdx.x;
+  if (hj >= nJobsH) return;
+  __shared__ uint32_t s_seed[80];
+  __shared__ uint32_t s_u[HBIG / 64];
+  __shared__ float s_f[HBIG / 64];
+  __shared__ unsigned long long s_l[3 * (HBIG / 64)];
+  BlockGrp<HBIG> g;
+  g.su = s_u;
+  g.sf = s_f;
+  g.sl = s_l;
+  hull_job(g, hjobs[hj], hj, s_seed, verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTestsBig);
 }
 
 }  // namespace dk

@@ -401,7 +401,7 @@ __device__ __forceinline__ void interp(const float* V, const uint32_t* tr, const
 __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ parts, const DJob* __restrict__ jobs,
                                              const float4* __restrict__ jobPlanes, float* __restrict__ verts,
                                              uint32_t* __restrict__ tris, DEdge* __restrict__ edges,
-                                             uint32_t* __restrict__ scratch) {
+                                             uint32_t* __restrict__ scratch, unsigned long long* ctrEmitBytes) {
   const uint32_t g = blockIdx.x;
   if (g >= P.nJobs) return;
   const uint32_t tid = threadIdx.x;
@@ -662,6 +662,10 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
   if (tid == 0) {
     parts[P.partBase + 2 * g].meshVol = fix_to_f64(fix_abs(fp)) / 6.0;
     parts[P.partBase + 2 * g + 1].meshVol = fix_to_f64(fix_abs(fn)) / 6.0;
+    // parent read once, both children written, job and part records
+    unsigned long long b = 12ull * part.nv + 12ull * part.nt + 32ull * part.ne + 12ull * (job.cnt[0] + job.cnt[3]) + 96;
+    if (keepTopo) b += 12ull * (job.cnt[1] + job.cnt[4]) + 32ull * (job.cnt[2] + job.cnt[5]);
+    atomicAdd(ctrEmitBytes, b);
   }
 }
 

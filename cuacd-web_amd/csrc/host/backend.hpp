@@ -114,8 +114,33 @@ struct HullResult {
   uint32_t failCode = 0;
 };
 
+// Device time and algorithmic work per kernel family, accumulated over the
+// life of a backend (GPU event timing on the launch stream).
+enum KernelId { K_PLAN = 0, K_SCAN, K_EMIT, K_HULL, K_HULL_BIG, K_RECORD, K_EVALUATE, K_HD_SAMPLES, K_HD_DIST, K_COUNT };
+struct KernelStats {
+  double ms[K_COUNT] = {0};
+  uint64_t launches[K_COUNT] = {0};
+  uint64_t faceTests = 0;   // wave-per-part hull: point-face record evaluations
+  uint64_t faceTestsBig = 0;  // workgroup-per-part hull: point-face record evaluations
+  uint64_t emitBytes = 0;   // emit: parent read + children written
+  uint64_t hdEvals = 0;     // Hausdorff: point-triangle evaluations
+  uint64_t hullJobs = 0;
+  void add(const KernelStats& o) {
+    for (int k = 0; k < K_COUNT; k++) {
+      ms[k] += o.ms[k];
+      launches[k] += o.launches[k];
+    }
+    faceTests += o.faceTests;
+    faceTestsBig += o.faceTestsBig;
+    emitBytes += o.emitBytes;
+    hdEvals += o.hdEvals;
+    hullJobs += o.hullJobs;
+  }
+};
+
 class Backend {
  public:
+  virtual KernelStats kernelStats() const { return KernelStats(); }
   virtual ~Backend() {}
   virtual const TreeConfig& config() const = 0;
   virtual uint32_t planeSlots() const = 0;

@@ -555,6 +555,7 @@ DecomposeResult Decomposer::decompose(const Mesh& input) {
   }
   R.parts = std::move(parts);
   st.totalMs = nowMs() - tStart;
+  R.kernels = be_.kernelStats();
   return R;
 }
 

@@ -102,6 +102,7 @@ struct DecomposeResult {
   bool normalized = false;
   Normalization norm{1.0, {0, 0, 0}};
   std::vector<CutRecord> cuts;
+  KernelStats kernels;
 };
 
 // src/merge.ts:25-52 MergeOptions

@@ -89,6 +89,7 @@ class DecomposeResult:
     merge: dict
     norm: tuple
     cuts: list
+    kernels: dict = None
 
 
 def _read_result(api, h):
@@ -119,8 +120,10 @@ def _read_result(api, h):
         check(api, api.result_cut(h, k, C.byref(c)))
         cuts.append(dict(iter=c.iter, part=c.part, best_cut=c.best_cut, applied=bool(c.applied),
                          best_cost=c.best_cost, plane=tuple(c.plane)))
+    ks = F.KernelStats()
+    check(api, api.result_kernel_stats(h, C.byref(ks)))
     api.result_free(h)
-    return DecomposeResult(parts, s.iterations, stats, merge, (s.norm_scale, tuple(s.norm_center)), cuts)
+    return DecomposeResult(parts, s.iterations, stats, merge, (s.norm_scale, tuple(s.norm_center)), cuts, ks.as_dict())
 
 
 def decompose(mesh, options=None, api=None, **overrides):

@@ -101,6 +101,12 @@ class GpuTree:
         except Exception:
             pass
 
+    def kernel_stats(self):
+        """Device time and algorithmic work per kernel family over the life of this tree."""
+        ks = F.KernelStats()
+        check(self.api, self.api.tree_kernel_stats(self._h, C.byref(ks)))
+        return ks.as_dict()
+
     @property
     def plane_slots(self):
         return self.api.plane_slots(self._h)

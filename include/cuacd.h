@@ -164,6 +164,22 @@ typedef struct cuacd_cut_record {
   float plane[4];
 } cuacd_cut_record;
 
+/* device time (HIP events on the launch stream) and algorithmic work per kernel family:
+ * 0 plan, 1 scan, 2 emit, 3 hull (wave per part), 4 hull (workgroup per part), 5 record,
+ * 6 evaluate, 7 Hausdorff samples, 8 Hausdorff distances */
+#define CUACD_NUM_KERNELS 9
+typedef struct cuacd_kernel_stats {
+  double ms[CUACD_NUM_KERNELS];
+  uint64_t launches[CUACD_NUM_KERNELS];
+  uint64_t face_tests; /* wave-per-part hull: point-face record evaluations */
+  uint64_t emit_bytes; /* emit: parent read once + children written */
+  uint64_t hd_evals;   /* Hausdorff: point-triangle evaluations */
+  uint64_t hull_jobs;
+  uint64_t face_tests_big; /* the workgroup-per-part hull kernel's share */
+} cuacd_kernel_stats;
+/* accumulated over the life of the handle */
+int cuacd_tree_kernel_stats(cuacd_tree* t, cuacd_kernel_stats* s);
+
 void cuacd_decompose_options_default(cuacd_decompose_options* o);
 /* decompose(mesh, options) (src/decompose.ts:407) */
 int cuacd_decompose(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
@@ -176,6 +192,7 @@ int cuacd_result_part(const cuacd_result* r, uint32_t k, cuacd_part_desc* d);
 int cuacd_result_part_data(const cuacd_result* r, uint32_t k, float* verts, uint32_t* tris, float* hull_verts,
                            uint32_t* hull_tris, double* seams);
 int cuacd_result_stats(const cuacd_result* r, cuacd_decompose_stats* s);
+int cuacd_result_kernel_stats(const cuacd_result* r, cuacd_kernel_stats* s);
 uint32_t cuacd_result_num_cuts(const cuacd_result* r);
 int cuacd_result_cut(const cuacd_result* r, uint32_t k, cuacd_cut_record* c);
 void cuacd_result_free(cuacd_result* r);

@@ -358,8 +358,13 @@ void OracleBackend::runLevel(const Level& L, uint32_t level, uint32_t depth, uin
     }
     for (int s = 0; s < 2; s++) ch[s]->meshVol = fix_to_f64(fix_abs(flux[s])) / 6.0;
   }
-  // ---- hulls of every child part
+  // ---- hulls of every child part (independent: spread over host threads)
   const bool prune = lastLevel && level > 0 && cfg_.pruneLastLevel && nSeeds > 0;
+  struct Task {
+    uint32_t part;
+    HullParams hp;
+  };
+  std::vector<Task> tasks;
   for (uint32_t g = 0; g < L.nJobs; g++) {
     double partial = 0, ub = 1e30;
     bool boundReady = false;
@@ -391,19 +396,24 @@ void OracleBackend::runLevel(const Level& L, uint32_t level, uint32_t depth, uin
         hp.partial = partial;
         hp.ub = ub;
       }
-      const HullOut h = exactHull(c.v.data(), c.nv, hp);
-      (*hullsDone)++;
-      c.nCand = h.nCand;
-      c.nIns = h.nIns;
-      if (h.status == 2) {
-        c.hullVol = h.volume;
-        c.flags |= 4u;
-        r->pruned++;
-      } else {
-        c.hullVol = h.volume;  // 0 for a degenerate point set
-      }
+      tasks.push_back(Task{L.partBase + 2 * g + s, hp});
     }
   }
+  uint32_t nPruned = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nPruned)
+  for (size_t k = 0; k < tasks.size(); k++) {
+    OPart& c = parts_[tasks[k].part];
+    const HullOut h = exactHull(c.v.data(), c.nv, tasks[k].hp);
+    c.nCand = h.nCand;
+    c.nIns = h.nIns;
+    c.hullVol = h.volume;  // 0 for a degenerate point set
+    if (h.status == 2) {
+      c.flags |= 4u;
+      nPruned++;
+    }
+  }
+  *hullsDone += tasks.size();
+  r->pruned += nPruned;
   // ---- record: rv per part, ascending sort, level cost = worst part
   for (uint32_t g = 0; g < L.nJobs; g++) {
     OItem& it = items_[L.itemsOutBase + g];
@@ -619,30 +629,31 @@ bool OracleBackend::readPart(uint32_t partIdx, PartView* pv, cuacd::Mesh* mesh) 
 }
 
 std::vector<HullResult> OracleBackend::hullMeshes(const std::vector<MeshRef>& meshes) {
-  std::vector<HullResult> out;
-  for (const MeshRef& m : meshes) {
-    const HullOut h = exactHull(m.verts, m.nv, HullParams{});
-    HullResult r;
-    r.ok = true;
-    r.volume = h.volume;
-    r.tris = h.faces;
-    out.push_back(std::move(r));
+  std::vector<HullResult> out(meshes.size());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t k = 0; k < meshes.size(); k++) {
+    const HullOut h = exactHull(meshes[k].verts, meshes[k].nv, HullParams{});
+    out[k].ok = true;
+    out[k].volume = h.volume;
+    out[k].tris = h.faces;
   }
   return out;
 }
 
 std::vector<double> OracleBackend::hausdorff(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b) {
-  std::vector<double> out;
-  for (size_t k = 0; k < a.size(); k++) out.push_back(hausdorffPair(a[k], b[k], cfg_.hausdorffSamples));
+  std::vector<double> out(a.size());
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t k = 0; k < a.size(); k++) out[k] = hausdorffPair(a[k], b[k], cfg_.hausdorffSamples);
   return out;
 }
 
 std::vector<HullResult> OracleBackend::hullsWithHausdorff(const std::vector<MeshRef>& meshes, std::vector<double>* hd) {
   std::vector<HullResult> out = hullMeshes(meshes);
-  hd->clear();
+  hd->assign(meshes.size(), 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
   for (size_t k = 0; k < meshes.size(); k++) {
     const MeshRef hm{meshes[k].verts, meshes[k].nv, out[k].tris.data(), (uint32_t)(out[k].tris.size() / 3)};
-    hd->push_back(hausdorffPair(meshes[k], hm, cfg_.hausdorffSamples));
+    (*hd)[k] = hausdorffPair(meshes[k], hm, cfg_.hausdorffSamples);
   }
   return out;
 }
