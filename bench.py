@@ -31,9 +31,85 @@ def _package():
     return mod
 
 
+# Algorithmic work per unit the kernels count themselves: (counter, amount per unit, unit kind).
+#  hull / hull_big: one point-face test reads one 16-byte face plane (a, b, c, offset);
+#  emit: the job's parent read plus the children it writes, summed in bytes by the kernel;
+#  hd_dist: one point-triangle distance evaluation = 70 FP32 operations.
+KERNEL_WORK = {
+    "hull_big": ("face_tests_big", 16.0, "bytes"),
+    "hull": ("face_tests", 16.0, "bytes"),
+    "emit": ("emit_bytes", 1.0, "bytes"),
+    "hd_dist": ("hd_evals", 70.0, "flops"),
+}
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3    # MI355X dense FP32
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def sum_kernels(results):
+    tot = None
+    for r in results:
+        k = r.kernels
+        if tot is None:
+            tot = dict(ms=dict(k["ms"]), launches=dict(k["launches"]),
+                       **{n: k[n] for n in ("face_tests", "face_tests_big", "emit_bytes", "hd_evals", "hull_jobs")})
+            continue
+        for n in k["ms"]:
+            tot["ms"][n] += k["ms"][n]
+            tot["launches"][n] += k["launches"][n]
+        for n in ("face_tests", "face_tests_big", "emit_bytes", "hd_evals", "hull_jobs"):
+            tot[n] += k[n]
+    return tot
+
+
+def roofline(kst):
+    """The kernel with the most device time over the timed steps, priced against its bound.
+    Times come from HIP events the library records on its own stream around each launch."""
+    name = max(KERNEL_WORK, key=lambda n: kst["ms"][n])
+    counter, per_unit, kind = KERNEL_WORK[name]
+    launches = max(kst["launches"][name], 1)
+    avg_ms = kst["ms"][name] / launches
+    if avg_ms <= 0:
+        return None
+    work_per_launch = kst[counter] * per_unit / launches
+    if kind == "bytes":
+        achieved, peak, unit, bound = work_per_launch / (avg_ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s", "hbm"
+    else:
+        achieved, peak, unit, bound = work_per_launch / (avg_ms * 1e-3) / 1e12, FP32_PEAK_TFLOPS, "TFLOP/s", "mfma"
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        if t.get("kernel") == name:
+            traffic = t["hbm_bytes_per_launch"]
+    return dict(bound=bound, achieved=achieved, peak=peak, unit=unit, frac=achieved / peak, traffic=traffic,
+                kernel=name, launches=kst["launches"][name], avg_launch_ms=avg_ms,
+                work_per_launch=work_per_launch, work_counter=counter, work_per_unit=per_unit,
+                kernel_ms={n: round(v, 3) for n, v in kst["ms"].items()},
+                kernel_launches=dict(kst["launches"]))
+
+
+def copy_probe(mib):
+    """One plain device copy of `mib` MiB (src -> dst), the calibration kernel for HBM counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    assert bool((dst[-4:] == src[-4:]).all())
+    print(f"copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+
+
+def oracle_threads():
+    v = os.environ.get("OMP_NUM_THREADS")
+    return int(v) if v and v.isdigit() else len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(cw, mesh, gpu_step_s, sample_iters):
-    """The oracle (CPU restatement, 1 thread) on the first `sample_iters` decomposition iterations;
-    meshes/s scaled by the GPU's own full-run / sample-run time ratio."""
+    """The oracle (the CPU restatement, its hull and Hausdorff loops over OpenMP host threads) on the
+    first `sample_iters` decomposition iterations; meshes/s scaled by the GPU's own full-run /
+    sample-run time ratio."""
     from cuacd_web_amd._ffi import Api
     oracle = Api(ctypes.CDLL(os.path.join(ROOT, "oracle", "build", "liboracle.so"), mode=ctypes.RTLD_LOCAL), "oracle_")
     opts = dict(OPTIONS, max_iters=sample_iters, merge_hulls=False)
@@ -44,8 +120,9 @@ def cpu_baseline(cw, mesh, gpu_step_s, sample_iters):
     cw.decompose(mesh, api=oracle, **opts)
     t_cpu_sample = time.perf_counter() - t0
     est_cpu_full = t_cpu_sample * (gpu_step_s / max(t_gpu_sample, 1e-9))
-    return dict(value=1.0 / est_cpu_full, unit="meshes/s", cores=1, kind="port",
-                sample=f"first {sample_iters} decompose iterations (no merge) of the same mesh on 1 host thread: "
+    cores = oracle_threads()
+    return dict(value=1.0 / est_cpu_full, unit="meshes/s", cores=cores, kind="port",
+                sample=f"first {sample_iters} decompose iterations (no merge) of the same mesh on {cores} host threads: "
                        f"{t_cpu_sample:.2f} s CPU vs {t_gpu_sample:.3f} s GPU; full-run estimate scales by the GPU "
                        f"full/sample ratio", cpu_sample_s=t_cpu_sample, gpu_sample_s=t_gpu_sample)
 
@@ -57,21 +134,26 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run one plain device copy of MIB MiB first (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU path "
               "(multi-GPU sharding is designed, not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     cw = _package()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
     mesh = cw.generate("blobs", WORKLOAD["seed"], WORKLOAD["params"])
     for _ in range(a.warmup):
         cw.decompose(mesh, **OPTIONS)
     times, results = [], []
-    for _ in range(a.steps):
+    for i in range(a.steps):
         t0 = time.perf_counter()
         r = cw.decompose(mesh, **OPTIONS)
         times.append(time.perf_counter() - t0)
         results.append(r)
+        print(f"step {i}: {times[-1]:.3f} s, {len(r.parts)} parts", file=sys.stderr, flush=True)
     total = sum(times)
     r = results[-1]
     step_s = total / a.steps
@@ -94,7 +176,7 @@ def main():
         "candidate_hulls_per_s": hulls / step_s,
         "candidate_hulls": hulls,
         "pruned": r.stats["pruned"],
-        "roofline": None,
+        "roofline": roofline(sum_kernels(results)),
         "cpu_baseline": None,
     }
     if not a.no_cpu:
