@@ -112,6 +112,134 @@ __device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
   }
   return v;
 }
+// ---- DPP reductions: four in-row steps (quad swaps, half-row and row mirrors)
+// leave every lane holding its row's result; the four row results are then
+// combined through readlane, so the value is wave-uniform. All 64 lanes must
+// be active.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+constexpr int DPP_QXOR1 = 0xB1, DPP_QXOR2 = 0x4E, DPP_HMIRROR = 0x141, DPP_MIRROR = 0x140;
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+  const uint32_t lo = dpp_u32<CTRL>((uint32_t)v), hi = dpp_u32<CTRL>((uint32_t)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ unsigned long long rdlane_u64(unsigned long long v, uint32_t lane) {
+  return ((unsigned long long)rdlane((uint32_t)(v >> 32), lane) << 32) | rdlane((uint32_t)v, lane);
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  uint32_t o;
+  o = dpp_u32<DPP_QXOR1>(v);
+  v = o > v ? o : v;
+  o = dpp_u32<DPP_QXOR2>(v);
+  v = o > v ? o : v;
+  o = dpp_u32<DPP_HMIRROR>(v);
+  v = o > v ? o : v;
+  o = dpp_u32<DPP_MIRROR>(v);
+  v = o > v ? o : v;
+  uint32_t r = rdlane(v, 0);
+  for (uint32_t row = 16; row < 64; row += 16) {
+    o = rdlane(v, row);
+    r = o > r ? o : r;
+  }
+  return r;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  unsigned long long o;
+  o = dpp_u64<DPP_QXOR1>(v);
+  v = o > v ? o : v;
+  o = dpp_u64<DPP_QXOR2>(v);
+  v = o > v ? o : v;
+  o = dpp_u64<DPP_HMIRROR>(v);
+  v = o > v ? o : v;
+  o = dpp_u64<DPP_MIRROR>(v);
+  v = o > v ? o : v;
+  unsigned long long r = rdlane_u64(v, 0);
+  for (uint32_t row = 16; row < 64; row += 16) {
+    o = rdlane_u64(v, row);
+    r = o > r ? o : r;
+  }
+  return r;
+}
+// largest 64-bit key of the wave: the high words first, the full keys only on a tie
+__device__ __forceinline__ unsigned long long wave_max_key(unsigned long long v) {
+  const uint32_t hi = (uint32_t)(v >> 32);
+  const uint32_t mhi = wave_max_u32(hi);
+  const uint64_t bal = __ballot(hi == mhi);
+  if (__popcll(bal) == 1) return rdlane_u64(v, (uint32_t)__ffsll((unsigned long long)bal) - 1);
+  return wave_max_u64(hi == mhi ? v : 0ull);
+}
+// (value, index) with the larger value, then the lower index
+__device__ __forceinline__ void pair_max(float& v, uint32_t& i, float ov, uint32_t oi) {
+  if (ov > v || (ov == v && oi < i)) {
+    v = ov;
+    i = oi;
+  }
+}
+__device__ __forceinline__ void pair_min(float& v, uint32_t& i, float ov, uint32_t oi) {
+  if (ov < v || (ov == v && oi < i)) {
+    v = ov;
+    i = oi;
+  }
+}
+template <bool MAX>
+__device__ __forceinline__ void wave_argext(float& v, uint32_t& i) {
+#define CUACD_ARG_STEP(CTRL)                                                        \
+  {                                                                                 \
+    const float ov = __uint_as_float(dpp_u32<CTRL>(__float_as_uint(v)));            \
+    const uint32_t oi = dpp_u32<CTRL>(i);                                           \
+    if (MAX) pair_max(v, i, ov, oi);                                                \
+    else pair_min(v, i, ov, oi);                                                    \
+  }
+  CUACD_ARG_STEP(DPP_QXOR1)
+  CUACD_ARG_STEP(DPP_QXOR2)
+  CUACD_ARG_STEP(DPP_HMIRROR)
+  CUACD_ARG_STEP(DPP_MIRROR)
+#undef CUACD_ARG_STEP
+  float rv = __uint_as_float(rdlane(__float_as_uint(v), 0));
+  uint32_t ri = rdlane(i, 0);
+  for (uint32_t row = 16; row < 64; row += 16) {
+    const float ov = __uint_as_float(rdlane(__float_as_uint(v), row));
+    const uint32_t oi = rdlane(i, row);
+    if (MAX) pair_max(rv, ri, ov, oi);
+    else pair_min(rv, ri, ov, oi);
+  }
+  v = rv;
+  i = ri;
+}
+// float min or max over a fully active wave (DPP)
+template <bool MAX>
+__device__ __forceinline__ float wave_ext_f(float v) {
+  float o;
+  o = __uint_as_float(dpp_u32<DPP_QXOR1>(__float_as_uint(v)));
+  v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+  o = __uint_as_float(dpp_u32<DPP_QXOR2>(__float_as_uint(v)));
+  v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+  o = __uint_as_float(dpp_u32<DPP_HMIRROR>(__float_as_uint(v)));
+  v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+  o = __uint_as_float(dpp_u32<DPP_MIRROR>(__float_as_uint(v)));
+  v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+  float r = __uint_as_float(rdlane(__float_as_uint(v), 0));
+  for (uint32_t row = 16; row < 64; row += 16) {
+    o = __uint_as_float(rdlane(__float_as_uint(v), row));
+    r = MAX ? (o > r ? o : r) : (o < r ? o : r);
+  }
+  return r;
+}
+// inclusive prefix sum over the wave
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = lane_id();
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(v, d, 64);
+    v += lane >= (uint32_t)d ? o : 0u;
+  }
+  return v;
+}
 // (value, index): largest value, then lowest index
 __device__ __forceinline__ void wave_argmax(float& v, uint32_t& i) {
   for (int m = 32; m >= 1; m >>= 1) {

@@ -1,19 +1,36 @@
-// k_hull.hpp — exact grid convex hull, one wavefront per part (HIP, gfx950).
+// k_hull.hpp — exact grid convex hull, batched farthest-first insertion (HIP, gfx950).
 //
 // Points are quantized to a 2^21 grid over their bbox; 80 extreme seeds along
 // 40 directions give a simplex and a tight inner hull; every point outside it
-// is inserted farthest-first. Orientation is decided by an f32 filter with an
-// exact int128 fallback (the reference's 96-bit fallback, src/gpu/shaders.ts
-// :813-921), so the predicate never contradicts itself. Ties are broken by
-// order-independent rules (largest distance then lowest point index; first
-// face in creation order), and new faces are appended by ordered ballot
-// compaction instead of atomics, so the face list of a part is reproducible
-// bit for bit. Branch-and-bound after the seed phase at the last tree level
-// (src/gpu/shaders.ts:1294-1331). Volumes over the original float
+// is inserted farthest-first (largest distance, then lowest point index).
+// Orientation is decided by an f32 filter with an exact int128 fallback (the
+// reference's 96-bit fallback, src/gpu/shaders.ts:813-921), so the predicate
+// never contradicts itself. Branch-and-bound after the seed phase at the last
+// tree level (src/gpu/shaders.ts:1294-1331). Volumes over the original float
 // coordinates with error-free products and exact fixed-point sums.
 //
-// Per-wave global slab (u32 words, 256-byte aligned): faces[12 fmax] | q[3 nv] |
-// owner[nv] | dist[nv] | cand[max(nv,80)] | vis[fmax] | map[fmax], fmax = 4 nv + 256.
+// Insertion runs in rounds. Each face keeps the key (distance, ~index) of its
+// farthest outside point, so the K largest face keys are the next K points of
+// the sequential order. A round inserts the longest prefix of them that the
+// sequential algorithm would insert identically:
+//   - their visible face sets are pairwise disjoint;
+//   - no point sees a face created earlier in the same round;
+//   - no point reassigned earlier in the round is farther than the next one.
+// Under these conditions every visible set, horizon and reassignment equals
+// the one-at-a-time result, and new faces are numbered in the same order
+// (candidate, visible face in creation order, edge), so the face list is the
+// sequential one bit for bit. Visible faces are collected in LDS; horizon
+// edges are found with an LDS hash of directed edges. A candidate whose
+// visible set does not fit in LDS is inserted by the one-at-a-time path.
+//
+// A face record (80 bytes) carries its plane for the f32 filter and the grid
+// coordinates of its three corners, and a candidate entry (16 bytes) carries
+// the point's grid coordinates, so exact tests, new faces and reassignments
+// need no lookups of point coordinates. Owner faces sit in a parallel array.
+//
+// Per-part global slab (u32 words, 256-byte aligned):
+//   faces[20 f] | fkey[2 f] | vmask[f] | vis[f] | map[f] | orph[8 c] | cand[4 c] |
+//   cand2[4 c] | q[3 nv] | cown[c] | cown2[c],   f = 4 nv + 256, c = max(nv, 80).
 #pragma once
 #include "dnum.hpp"
 #include "dtypes.hpp"
@@ -21,16 +38,21 @@
 namespace dk {
 
 constexpr int HW = 4;         // parts per block in the wave-per-part hull kernel
-constexpr int HBIG = 1024;    // threads per part in the workgroup-per-part hull kernel
+constexpr int HBIG = 512;     // threads per part in the workgroup-per-part hull kernel
 constexpr uint32_t HBIG_NV = 2048;  // parts with at least this many points get a whole workgroup
 constexpr uint32_t ALIVE = 0x80000000u;
-constexpr uint32_t FW = 12;  // words per face record
+constexpr uint32_t FW = 20;                 // words per face record
+constexpr uint32_t HULL_MAX_NV = 1u << 27;  // candidate slots share a word with the candidate number
+
+// batch shape: candidates per round, LDS visible-face capacity, edge-hash slots
+constexpr uint32_t KB_BIG = 16, LCAP_BIG = 512, HT_BIG = 4096, NCAP_BIG = 256;
+constexpr uint32_t KB_W = 8, LCAP_W = 128, HT_W = 1024, NCAP_W = 48;
 
 __host__ __device__ inline uint32_t hullFmax(uint32_t nv) { return 4 * nv + 256; }
 __host__ __device__ inline uint64_t hullSlabWords(uint32_t nv) {
   const uint64_t f = hullFmax(nv);
   const uint64_t c = nv > 80 ? nv : 80;
-  return ((5ull * nv + c + 2 * f + FW * f) + 63) & ~63ull;
+  return ((25ull * f + 3ull * nv + 18ull * c) + 63) & ~63ull;
 }
 
 __constant__ float KDIR[40][3] = {
@@ -59,9 +81,11 @@ __constant__ float KDIR[40][3] = {
 // ---- thread groups: one wavefront, or one workgroup of G threads -----------------
 // Every operation returns the same value in every thread of the group, so the
 // hull's control flow is uniform whatever the group size, and the decisions do
-// not depend on it (ties are resolved by value, then by index).
+// not depend on it (ties are resolved by value, then by index). Operations are
+// called by every thread of the group.
 struct WaveGrp {
   static constexpr uint32_t G = 64;
+  static constexpr uint32_t NW = 1;
   __device__ __forceinline__ uint32_t tid() const { return lane_id(); }
   __device__ __forceinline__ void sync() const { wave_sync(); }
   __device__ __forceinline__ uint32_t rank(bool f, uint32_t* total) const {
@@ -69,8 +93,13 @@ struct WaveGrp {
     *total = (uint32_t)__popcll(m);
     return prefix_below(m);
   }
-  __device__ __forceinline__ float minf(float v) const { return wave_min_f(v); }
-  __device__ __forceinline__ float maxf(float v) const { return wave_max_f(v); }
+  __device__ __forceinline__ uint32_t excl_scan(uint32_t v, uint32_t* total) const {
+    const uint32_t inc = wave_incl_scan(v);
+    *total = rdlane(inc, 63);
+    return inc - v;
+  }
+  __device__ __forceinline__ float minf(float v) const { return wave_ext_f<false>(v); }
+  __device__ __forceinline__ float maxf(float v) const { return wave_ext_f<true>(v); }
   __device__ __forceinline__ double mind(double v) const {
     for (int m = 32; m >= 1; m >>= 1) {
       const double o = __shfl_xor(v, m, 64);
@@ -79,8 +108,15 @@ struct WaveGrp {
     return v;
   }
   __device__ __forceinline__ i128 sum(i128 v) const { return wave_sum_i128(v); }
-  __device__ __forceinline__ void argmax(float& v, uint32_t& i) const { wave_argmax(v, i); }
-  __device__ __forceinline__ void argmin(float& v, uint32_t& i) const { wave_argmin(v, i); }
+  // NP maxima and NP minima of (value, index): larger/smaller value, then lower index
+  template <int NP>
+  __device__ __forceinline__ void argext(float* mx, uint32_t* ix, float* mn, uint32_t* in) const {
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      wave_argext<true>(mx[k], ix[k]);
+      wave_argext<false>(mn[k], in[k]);
+    }
+  }
   // largest m, then lowest k (carrying s)
   __device__ __forceinline__ void argmax128(i128& m, uint32_t& k, int& s) const {
     for (int d = 32; d >= 1; d >>= 1) {
@@ -103,6 +139,7 @@ struct BlockGrp {
   uint32_t* su;            // >= NW
   float* sf;               // >= NW
   unsigned long long* sl;  // >= 3 * NW
+  uint32_t* sm;            // >= 4 * 16 * NW (argext of up to 16 pairs)
   __device__ __forceinline__ uint32_t tid() const { return threadIdx.x; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
   __device__ __forceinline__ uint32_t rank(bool f, uint32_t* total) const {
@@ -120,24 +157,33 @@ struct BlockGrp {
     *total = all;
     return before + prefix_below(m);
   }
-  __device__ __forceinline__ float minf(float v) const {
-    v = wave_min_f(v);
+  __device__ __forceinline__ uint32_t excl_scan(uint32_t v, uint32_t* total) const {
+    const uint32_t inc = wave_incl_scan(v);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) su[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t k = 0; k < NW; k++) {
+      const uint32_t c = su[k];
+      before += k < w ? c : 0;
+      all += c;
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - v;
+  }
+  template <bool MAX>
+  __device__ __forceinline__ float ext(float v) const {
+    v = wave_ext_f<MAX>(v);
     if ((threadIdx.x & 63) == 0) sf[threadIdx.x >> 6] = v;
     __syncthreads();
     float r = sf[0];
-    for (uint32_t k = 1; k < NW; k++) r = sf[k] < r ? sf[k] : r;
+    for (uint32_t k = 1; k < NW; k++) r = MAX ? (sf[k] > r ? sf[k] : r) : (sf[k] < r ? sf[k] : r);
     __syncthreads();
     return r;
   }
-  __device__ __forceinline__ float maxf(float v) const {
-    v = wave_max_f(v);
-    if ((threadIdx.x & 63) == 0) sf[threadIdx.x >> 6] = v;
-    __syncthreads();
-    float r = sf[0];
-    for (uint32_t k = 1; k < NW; k++) r = sf[k] > r ? sf[k] : r;
-    __syncthreads();
-    return r;
-  }
+  __device__ __forceinline__ float minf(float v) const { return ext<false>(v); }
+  __device__ __forceinline__ float maxf(float v) const { return ext<true>(v); }
   __device__ __forceinline__ double mind(double v) const {
     for (int m = 32; m >= 1; m >>= 1) {
       const double o = __shfl_xor(v, m, 64);
@@ -163,36 +209,39 @@ struct BlockGrp {
     __syncthreads();
     return r;
   }
-  __device__ __forceinline__ void argmax(float& v, uint32_t& i) const {
-    wave_argmax(v, i);
-    if ((threadIdx.x & 63) == 0) {
-      sf[threadIdx.x >> 6] = v;
-      su[threadIdx.x >> 6] = i;
+  template <int NP>
+  __device__ __forceinline__ void argext(float* mx, uint32_t* ix, float* mn, uint32_t* in) const {
+    static_assert(NP <= 16, "argext width");
+    const uint32_t w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      wave_argext<true>(mx[k], ix[k]);
+      wave_argext<false>(mn[k], in[k]);
     }
-    __syncthreads();
-    v = sf[0];
-    i = su[0];
-    for (uint32_t k = 1; k < NW; k++)
-      if (sf[k] > v || (sf[k] == v && su[k] < i)) {
-        v = sf[k];
-        i = su[k];
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+      for (int k = 0; k < NP; k++) {
+        uint32_t* e = sm + 4 * (w * NP + k);
+        e[0] = __float_as_uint(mx[k]);
+        e[1] = ix[k];
+        e[2] = __float_as_uint(mn[k]);
+        e[3] = in[k];
       }
     __syncthreads();
-  }
-  __device__ __forceinline__ void argmin(float& v, uint32_t& i) const {
-    wave_argmin(v, i);
-    if ((threadIdx.x & 63) == 0) {
-      sf[threadIdx.x >> 6] = v;
-      su[threadIdx.x >> 6] = i;
-    }
-    __syncthreads();
-    v = sf[0];
-    i = su[0];
-    for (uint32_t k = 1; k < NW; k++)
-      if (sf[k] < v || (sf[k] == v && su[k] < i)) {
-        v = sf[k];
-        i = su[k];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      float v = __uint_as_float(sm[4 * k]), u = __uint_as_float(sm[4 * k + 2]);
+      uint32_t vi = sm[4 * k + 1], ui = sm[4 * k + 3];
+      for (uint32_t q = 1; q < NW; q++) {
+        const uint32_t* e = sm + 4 * (q * NP + k);
+        pair_max(v, vi, __uint_as_float(e[0]), e[1]);
+        pair_min(u, ui, __uint_as_float(e[2]), e[3]);
       }
+      mx[k] = v;
+      ix[k] = vi;
+      mn[k] = u;
+      in[k] = ui;
+    }
     __syncthreads();
   }
   __device__ __forceinline__ void argmax128(i128& m, uint32_t& k, int& s) const {
@@ -221,21 +270,33 @@ struct BlockGrp {
 struct HullCtx {
   const float* V;  // original coordinates
   int32_t* Q;      // quantized
-  uint32_t* owner;
-  float* dist;
-  uint32_t* cand;
+  uint4* cand;     // outside points: (index, grid x, y, z)
+  uint32_t* cown;  // their owner faces (NONE: no longer outside)
+  uint4* cand2;
+  uint32_t* cown2;
+  uint32_t* orph;
   uint32_t* vis;
   uint32_t* map;
-  uint32_t* F;  // face records
+  uint32_t* vmask;
+  unsigned long long* fkey;  // per face: key of its farthest outside point, 0 if none or dead
+  uint32_t* F;               // face records
   uint32_t nv, fmax;
   uint32_t nf, nalive, ncand, nIns;
   unsigned long long tests;  // point-face record evaluations by this thread
 };
 
-__device__ __forceinline__ void face_write(HullCtx& C, uint32_t f, uint32_t a, uint32_t b, uint32_t c) {
-  const int32_t* qa = C.Q + 3 * a;
-  const int32_t* qb = C.Q + 3 * b;
-  const int32_t* qc = C.Q + 3 * c;
+// order of outside points: larger distance first, then lower index
+__device__ __forceinline__ unsigned long long pkey(float d, uint32_t i) {
+  return ((unsigned long long)__float_as_uint(d) << 32) | (uint32_t)(~i);
+}
+
+struct FaceRec {
+  uint4 w[5];
+};
+
+// record of face (a, b, c) with corner grid coordinates qa, qb, qc
+__device__ __forceinline__ FaceRec face_make(uint32_t a, uint32_t b, uint32_t c, const int32_t* qa, const int32_t* qb,
+                                             const int32_t* qc) {
   const float e1[3] = {(float)qb[0] - (float)qa[0], (float)qb[1] - (float)qa[1], (float)qb[2] - (float)qa[2]};
   const float e2[3] = {(float)qc[0] - (float)qa[0], (float)qc[1] - (float)qa[1], (float)qc[2] - (float)qa[2]};
   const float n0 = e1[1] * e2[2] - e1[2] * e2[1];
@@ -246,18 +307,30 @@ __device__ __forceinline__ void face_write(HullCtx& C, uint32_t f, uint32_t a, u
   float nn = n0 * n0;
   nn = nn + n1 * n1;
   nn = nn + n2 * n2;
+  FaceRec R;
+  R.w[0] = make_uint4(a | ALIVE, b, c, 0u);
+  R.w[1] = make_uint4(__float_as_uint(n0), __float_as_uint(n1), __float_as_uint(n2), __float_as_uint(s1 * s2));
+  R.w[2] = make_uint4((uint32_t)qa[0], (uint32_t)qa[1], (uint32_t)qa[2], __float_as_uint(sqrtf(nn)));
+  R.w[3] = make_uint4((uint32_t)qb[0], (uint32_t)qb[1], (uint32_t)qb[2], 0u);
+  R.w[4] = make_uint4((uint32_t)qc[0], (uint32_t)qc[1], (uint32_t)qc[2], 0u);
+  return R;
+}
+__device__ __forceinline__ void face_store(HullCtx& C, uint32_t f, const FaceRec& R) {
   uint4* r4 = reinterpret_cast<uint4*>(C.F + (size_t)FW * f);
-  r4[0] = make_uint4(a | ALIVE, b, c, 0u);
-  r4[1] = make_uint4(__float_as_uint(n0), __float_as_uint(n1), __float_as_uint(n2), __float_as_uint(s1 * s2));
-  r4[2] = make_uint4(__float_as_uint((float)qa[0]), __float_as_uint((float)qa[1]), __float_as_uint((float)qa[2]),
-                     __float_as_uint(sqrtf(nn)));
+#pragma unroll
+  for (int k = 0; k < 5; k++) r4[k] = R.w[k];
+  C.fkey[f] = 0ull;
+}
+__device__ __forceinline__ void face_write_q(HullCtx& C, uint32_t f, uint32_t a, uint32_t b, uint32_t c,
+                                             const int32_t* qa, const int32_t* qb, const int32_t* qc) {
+  face_store(C, f, face_make(a, b, c, qa, qb, qc));
+}
+__device__ __forceinline__ void face_write(HullCtx& C, uint32_t f, uint32_t a, uint32_t b, uint32_t c) {
+  face_write_q(C, f, a, b, c, C.Q + 3 * a, C.Q + 3 * b, C.Q + 3 * c);
 }
 
-__device__ __forceinline__ int orient_exact(const HullCtx& C, uint32_t a, uint32_t b, uint32_t c, uint32_t p) {
-  const int32_t* qa = C.Q + 3 * a;
-  const int32_t* qb = C.Q + 3 * b;
-  const int32_t* qc = C.Q + 3 * c;
-  const int32_t* qp = C.Q + 3 * p;
+// exact sign of det[qb-qa; qc-qa; qp-qa]
+__device__ __forceinline__ int orient_q(const int32_t* qa, const int32_t* qb, const int32_t* qc, const int32_t* qp) {
   const int64_t e1[3] = {(int64_t)qb[0] - qa[0], (int64_t)qb[1] - qa[1], (int64_t)qb[2] - qa[2]};
   const int64_t e2[3] = {(int64_t)qc[0] - qa[0], (int64_t)qc[1] - qa[1], (int64_t)qc[2] - qa[2]};
   const int64_t d[3] = {(int64_t)qp[0] - qa[0], (int64_t)qp[1] - qa[1], (int64_t)qp[2] - qa[2]};
@@ -267,50 +340,86 @@ __device__ __forceinline__ int orient_exact(const HullCtx& C, uint32_t a, uint32
   const i128 det = (i128)cx * d[0] + (i128)cy * d[1] + (i128)cz * d[2];
   return det > 0 ? 1 : (det < 0 ? -1 : 0);
 }
+// exact orientation of qp against a face record held in registers
+__device__ __forceinline__ int orient_w(const uint4& w2, const uint4& w3, const uint4& w4, const int32_t* qp) {
+  const int32_t qa[3] = {(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z};
+  const int32_t qb[3] = {(int32_t)w3.x, (int32_t)w3.y, (int32_t)w3.z};
+  const int32_t qc[3] = {(int32_t)w4.x, (int32_t)w4.y, (int32_t)w4.z};
+  return orient_q(qa, qb, qc, qp);
+}
+// exact orientation of qp against the face whose record starts at r (corners in words 8.., 12.., 16..)
+__device__ __forceinline__ int orient_rec(const uint32_t* r, const uint4& w2, const int32_t* qp) {
+  const uint4 w3 = reinterpret_cast<const uint4*>(r)[3];
+  const uint4 w4 = reinterpret_cast<const uint4*>(r)[4];
+  const int32_t qa[3] = {(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z};
+  const int32_t qb[3] = {(int32_t)w3.x, (int32_t)w3.y, (int32_t)w3.z};
+  const int32_t qc[3] = {(int32_t)w4.x, (int32_t)w4.y, (int32_t)w4.z};
+  return orient_q(qa, qb, qc, qp);
+}
 
-// > 0: p strictly outside face f (f32 distance, 1e-30 when only the exact test
-// decided); <= 0: inside, on the plane, or a dead face
-__device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, uint32_t p) {
-  const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * f);
-  const uint4 w0 = r4[0];
-  if (!(w0.x & ALIVE)) return -1e30f;
-  const uint4 w1 = r4[1];
-  const uint4 w2 = r4[2];
-  const int32_t* qp = C.Q + 3 * p;
-  const float d0 = (float)qp[0] - __uint_as_float(w2.x);
-  const float d1 = (float)qp[1] - __uint_as_float(w2.y);
-  const float d2 = (float)qp[2] - __uint_as_float(w2.z);
+// f32 filter of a face plane against a quantized point: 1 outside, -1 inside
+// or on the plane, 0 undecided (the exact test decides)
+__device__ __forceinline__ int face_filter(const uint4& w1, const uint4& w2, const int32_t* qp, float* detOut) {
+  const float d0 = (float)qp[0] - (float)(int32_t)w2.x;
+  const float d1 = (float)qp[1] - (float)(int32_t)w2.y;
+  const float d2 = (float)qp[2] - (float)(int32_t)w2.z;
   float det = __uint_as_float(w1.x) * d0;
   det = det + __uint_as_float(w1.y) * d1;
   det = det + __uint_as_float(w1.z) * d2;
   const float sd = (fabsf(d0) + fabsf(d1)) + fabsf(d2);
   const float bound = (1e-6f * __uint_as_float(w1.w)) * sd;
-  if (det > bound) {
+  *detOut = det;
+  return det > bound ? 1 : (det < -bound ? -1 : 0);
+}
+
+// > 0: point qp strictly outside the face at r (f32 distance, 1e-30 when only
+// the exact test decided); <= 0: inside, on the plane, or a dead face
+__device__ __forceinline__ float face_eval_q(const uint32_t* r, const int32_t* qp) {
+  const uint4* r4 = reinterpret_cast<const uint4*>(r);
+  const uint4 w0 = r4[0], w1 = r4[1], w2 = r4[2];
+  if (!(w0.x & ALIVE)) return -1e30f;
+  float det;
+  const int s = face_filter(w1, w2, qp, &det);
+  if (s > 0) {
     const float nl = __uint_as_float(w2.w);
     return det / (nl > 1e-30f ? nl : 1e-30f);
   }
-  if (det < -bound) return -1.0f;
-  return orient_exact(C, w0.x & ~ALIVE, w0.y, w0.z, p) > 0 ? 1e-30f : -1.0f;
+  if (s < 0) return -1.0f;
+  return orient_rec(r, w2, qp) > 0 ? 1e-30f : -1.0f;
+}
+__device__ __forceinline__ float face_eval_rec(const FaceRec& R, const int32_t* qp) {
+  if (!(R.w[0].x & ALIVE)) return -1e30f;
+  float det;
+  const int s = face_filter(R.w[1], R.w[2], qp, &det);
+  if (s > 0) {
+    const float nl = __uint_as_float(R.w[2].w);
+    return det / (nl > 1e-30f ? nl : 1e-30f);
+  }
+  if (s < 0) return -1.0f;
+  return orient_w(R.w[2], R.w[3], R.w[4], qp) > 0 ? 1e-30f : -1.0f;
+}
+__device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, const int32_t* qp) {
+  return face_eval_q(C.F + (size_t)FW * f, qp);
 }
 
-// best face among [f0, f1) in creation order (first on ties), for one point
-__device__ __forceinline__ void assign_point(HullCtx& C, uint32_t i, uint32_t f0, uint32_t f1) {
+// best face among [f0, f1) in creation order (first on ties) for grid point qp
+__device__ __forceinline__ uint32_t best_face(HullCtx& C, const int32_t* qp, uint32_t f0, uint32_t f1, float* bestD) {
   C.tests += f1 - f0;
   uint32_t bestF = NONE;
-  float bestD = -1.0f;
+  float bd = -1.0f;
   for (uint32_t f = f0; f < f1; f++) {
-    const float dd = face_eval(C, f, i);
-    if (dd > 0.0f && dd > bestD) {
-      bestD = dd;
+    const float dd = face_eval(C, f, qp);
+    if (dd > 0.0f && dd > bd) {
+      bd = dd;
       bestF = f;
     }
   }
-  C.owner[i] = bestF;
-  C.dist[i] = bestF != NONE ? bestD : 0.0f;
+  *bestD = bd;
+  return bestF;
 }
 
 template <typename Grp>
-__device__ double hull_volume(const Grp& g, const HullCtx& C) {
+__device__ __forceinline__ double hull_volume(const Grp& g, const HullCtx& C) {
   i128 acc = 0;
   for (uint32_t f = g.tid(); f < C.nf; f += Grp::G) {
     const uint32_t* r = C.F + (size_t)FW * f;
@@ -322,19 +431,19 @@ __device__ double hull_volume(const Grp& g, const HullCtx& C) {
   return fix_to_f64(fix_abs(acc)) / 6.0;
 }
 
-// stable in-place removal of dead face slots; owners of outstanding points follow
+// stable in-place removal of dead face slots; face keys and candidate owners follow
 template <typename Grp>
-__device__ void compact_faces(const Grp& g, HullCtx& C) {
+__device__ __forceinline__ void compact_faces(const Grp& g, HullCtx& C) {
   uint32_t w = 0;
   for (uint32_t base = 0; base < C.nf; base += Grp::G) {
     const uint32_t f = base + g.tid();
-    uint4 r[3];
+    uint4 r[5];
+    unsigned long long key = 0;
     bool alive = false;
     if (f < C.nf) {
       const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * f);
-      r[0] = r4[0];
-      r[1] = r4[1];
-      r[2] = r4[2];
+      for (int k = 0; k < 5; k++) r[k] = r4[k];
+      key = C.fkey[f];
       alive = (r[0].x & ALIVE) != 0;
     }
     uint32_t cnt;
@@ -343,33 +452,61 @@ __device__ void compact_faces(const Grp& g, HullCtx& C) {
     if (f < C.nf) C.map[f] = alive ? dst : NONE;
     if (alive) {
       uint4* d4 = reinterpret_cast<uint4*>(C.F + (size_t)FW * dst);
-      d4[0] = r[0];
-      d4[1] = r[1];
-      d4[2] = r[2];
+      for (int k = 0; k < 5; k++) d4[k] = r[k];
+      C.fkey[dst] = key;
     }
     w += cnt;
     g.sync();
   }
   for (uint32_t e = g.tid(); e < C.ncand; e += Grp::G) {
-    const uint32_t i = C.cand[e];
-    const uint32_t o = C.owner[i];
-    if (o != NONE) C.owner[i] = C.map[o];
+    const uint32_t o = C.cown[e];
+    if (o != NONE) C.cown[e] = C.map[o];
   }
   C.nf = w;
   C.nalive = w;
   g.sync();
 }
 
-// insert point p: visible faces, horizon, new faces, reassignment
+// drop candidates that are no longer outside (double-buffered, order kept)
 template <typename Grp>
-__device__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p) {
+__device__ __forceinline__ void compact_cand(const Grp& g, HullCtx& C) {
+  const uint32_t n = C.ncand;
+  const uint32_t per = (n + Grp::G - 1) / Grp::G;
+  const uint32_t b0 = min(n, g.tid() * per), b1 = min(n, b0 + per);
+  uint32_t cnt = 0;
+  for (uint32_t e = b0; e < b1; e++) cnt += C.cown[e] != NONE ? 1u : 0u;
+  uint32_t total;
+  uint32_t w = g.excl_scan(cnt, &total);
+  for (uint32_t e = b0; e < b1; e++) {
+    const uint32_t o = C.cown[e];
+    if (o == NONE) continue;
+    C.cand2[w] = C.cand[e];
+    C.cown2[w] = o;
+    w++;
+  }
+  uint4* tc = C.cand;
+  C.cand = C.cand2;
+  C.cand2 = tc;
+  uint32_t* to = C.cown;
+  C.cown = C.cown2;
+  C.cown2 = to;
+  C.ncand = total;
+  g.sync();
+}
+
+// one-at-a-time insertion of point p over global memory: visible faces,
+// horizon, new faces, reassignment (used when a visible set exceeds LDS).
+// p lies on every new face, so its own reassignment finds no face.
+template <typename Grp>
+__device__ __forceinline__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p) {
   const uint32_t t = g.tid();
   if (C.nf > 2 * C.nalive + 128 || C.nf + C.nv + 8 > C.fmax) compact_faces(g, C);
+  const int32_t* qp = C.Q + 3 * p;
   // visible faces in creation order
   uint32_t nvis = 0;
   for (uint32_t base = 0; base < C.nf; base += Grp::G) {
     const uint32_t f = base + t;
-    const bool v = f < C.nf && face_eval(C, f, p) > 0.0f;
+    const bool v = f < C.nf && face_eval(C, f, qp) > 0.0f;
     C.tests += f < C.nf ? 1 : 0;
     uint32_t cnt;
     const uint32_t r = g.rank(v, &cnt);
@@ -377,15 +514,8 @@ __device__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p) {
     nvis += cnt;
   }
   g.sync();
-  if (t == 0) {
-    C.owner[p] = NONE;
-    C.dist[p] = 0.0f;
-  }
   C.nIns++;
-  if (nvis == 0) {
-    g.sync();
-    return;
-  }
+  if (nvis == 0) return;
   // horizon edges in (visible face, edge) order; each becomes a new face (a, b, p)
   const uint32_t f0 = C.nf;
   uint32_t nnew = 0;
@@ -415,60 +545,398 @@ __device__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p) {
     nnew += cnt;
   }
   g.sync();
-  for (uint32_t vi = t; vi < nvis; vi += Grp::G) C.F[(size_t)FW * C.vis[vi]] &= ~ALIVE;
+  for (uint32_t vi = t; vi < nvis; vi += Grp::G) {
+    C.F[(size_t)FW * C.vis[vi]] &= ~ALIVE;
+    C.fkey[C.vis[vi]] = 0ull;
+  }
   C.nf = f0 + nnew;
   C.nalive = C.nalive + nnew - nvis;
   g.sync();
   // points whose owner died see only the new faces
   for (uint32_t e = t; e < C.ncand; e += Grp::G) {
-    const uint32_t i = C.cand[e];
-    const uint32_t o = C.owner[i];
+    const uint32_t o = C.cown[e];
     if (o == NONE || (C.F[(size_t)FW * o] & ALIVE)) continue;
-    assign_point(C, i, f0, C.nf);
+    const uint4 ce = C.cand[e];
+    const int32_t qi[3] = {(int32_t)ce.y, (int32_t)ce.z, (int32_t)ce.w};
+    float bd;
+    const uint32_t bf = best_face(C, qi, f0, C.nf, &bd);
+    C.cown[e] = bf;
+    if (bf != NONE) atomicMax(&C.fkey[bf], pkey(bd, ce.x));
   }
   g.sync();
 }
 
-// farthest outstanding candidate (largest distance, then lowest index) until none
-template <typename Grp>
-__device__ void hull_grow(const Grp& g, HullCtx& C) {
-  uint32_t since = 0;
-  const uint32_t maxIter = C.nv + 8;
-  for (uint32_t it = 0; it < maxIter; it++) {
-    float best = -1.0f;
-    uint32_t bi = NONE;
-    for (uint32_t e = g.tid(); e < C.ncand; e += Grp::G) {
-      const uint32_t i = C.cand[e];
-      if (C.owner[i] == NONE) continue;
-      const float d = C.dist[i];
-      if (d > best || (d == best && i < bi)) {
-        best = d;
-        bi = i;
-      }
+// group-shared state of one insertion round
+template <uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+struct HullBatch {
+  FaceRec nrec[NCAP];  // the round's first new faces
+  unsigned long long ckey[K];  // candidate keys, descending
+  unsigned long long okey[K];  // largest reassigned key per candidate
+  uint32_t cpt[K];
+  int32_t cq[K][3];
+  uint32_t nvis[K], nh[K];
+  uint32_t lface[LCAP], la[LCAP], lb[LCAP], lc[LCAP];  // visible faces (any candidate)
+  uint8_t lj[LCAP], lflag[LCAP];                       // candidate, horizon edge bits
+  uint32_t lbase[LCAP];                                 // new faces of the same candidate before this entry
+  unsigned long long hkey[HT];                          // directed edges (a << 32 | b); seed-hull faces in phase 2
+  uint8_t hj[HT];
+  unsigned long long wkey[NW][K];
+  unsigned long long wB[NW];
+  uint32_t wcnt[NW];
+  uint32_t nList, nOrph, conflict, cutA, m;
+};
+
+__device__ __forceinline__ uint32_t edge_hash(unsigned long long key, uint32_t HT) {
+  return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & (HT - 1);
+}
+
+// inserts the next points of the farthest-first order; returns how many (0: none left)
+template <typename Grp, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+__device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBatch<K, LCAP, HT, NW, NCAP>& S,
+                                               uint32_t kEff) {
+  static_assert(K <= 32 && (HT & (HT - 1)) == 0 && HT >= 6 * LCAP && NW == Grp::NW && NW * K <= 128, "batch shape");
+  const uint32_t t = g.tid();
+  const uint32_t lane = lane_id();
+  const uint32_t w = NW == 1 ? 0u : (t >> 6);
+  if (C.nf > 2 * C.nalive + 128 || C.nf + C.nv + 8 > C.fmax) compact_faces(g, C);
+  for (uint32_t s = t; s < HT; s += Grp::G) S.hkey[s] = ~0ull;
+  if (t < K) {
+    S.nvis[t] = 0;
+    S.nh[t] = 0;
+    S.okey[t] = 0ull;
+  }
+  if (t == 0) {
+    S.nList = 0;
+    S.nOrph = 0;
+    S.conflict = K;
+    S.cutA = K;
+  }
+  // ---- the K largest face keys. Each lane offers its two largest; the third
+  // largest of every lane bounds everything not offered, so only keys above
+  // that bound are taken (they are then exactly the largest ones, in order).
+  // Candidate point numbers are the complemented low words of the keys.
+  unsigned long long k1 = 0, k2 = 0, k3 = 0;
+  for (uint32_t f = t; f < C.nf; f += Grp::G) {
+    const unsigned long long k = C.fkey[f];
+    if (k > k1) {
+      k3 = k2;
+      k2 = k1;
+      k1 = k;
+    } else if (k > k2) {
+      k3 = k2;
+      k2 = k;
+    } else if (k > k3) {
+      k3 = k;
     }
-    g.argmax(best, bi);
-    if (bi == NONE) return;
-    hull_insert(g, C, bi);
-    if (++since >= 16) {
-      since = 0;
-      uint32_t w = 0;
-      for (uint32_t base = 0; base < C.ncand; base += Grp::G) {
-        const uint32_t e = base + g.tid();
-        uint32_t i = 0;
-        bool keep = false;
-        if (e < C.ncand) {
-          i = C.cand[e];
-          keep = C.owner[i] != NONE;
+  }
+  {
+    uint32_t ptr = 0, cnt = 0;
+    for (uint32_t it = 0; it < kEff; it++) {
+      const unsigned long long head = ptr == 0 ? k1 : (ptr == 1 ? k2 : 0ull);
+      const unsigned long long mx = wave_max_key(head);
+      if (mx == 0ull) break;
+      if (lane == 0) S.wkey[w][it] = mx;
+      if (head == mx) ptr++;  // keys are distinct: one lane matches
+      cnt++;
+    }
+    const unsigned long long bw = wave_max_key(k3);
+    if (lane == 0) {
+      S.wB[w] = bw;
+      S.wcnt[w] = cnt;
+    }
+  }
+  g.sync();
+  if (NW == 1) {
+    // one wave: its list is the answer, cut at its bound
+    const unsigned long long B = S.wB[0];
+    const uint32_t cnt = S.wcnt[0];
+    const bool above = lane < cnt && S.wkey[0][lane < K ? lane : 0] > B;
+    const uint32_t m0 = (uint32_t)__popcll(__ballot(above));
+    if (lane < m0) S.ckey[lane] = S.wkey[0][lane];
+    if (lane == 0) S.m = m0;
+  } else if (t < 64) {
+    // the waves' lists, two entries per lane, merged in registers
+    unsigned long long B = 0;
+    for (uint32_t q = 0; q < NW; q++) B = S.wB[q] > B ? S.wB[q] : B;
+    unsigned long long e0 = 0, e1 = 0;
+    {
+      const uint32_t a0 = lane, a1 = lane + 64;
+      if (a0 < NW * K && (a0 % K) < S.wcnt[a0 / K]) e0 = S.wkey[a0 / K][a0 % K];
+      if (a1 < NW * K && (a1 % K) < S.wcnt[a1 / K]) e1 = S.wkey[a1 / K][a1 % K];
+    }
+    uint32_t m0 = 0;
+    for (uint32_t it = 0; it < kEff; it++) {
+      const unsigned long long head = e0 > e1 ? e0 : e1;
+      const unsigned long long mx = wave_max_key(head);
+      if (mx <= B) break;
+      if (lane == 0) S.ckey[it] = mx;
+      if (e0 == mx) e0 = 0;
+      if (e1 == mx) e1 = 0;
+      m0++;
+    }
+    if (lane == 0) S.m = m0;
+  }
+  g.sync();
+  uint32_t m = S.m;
+  if (m == 0) return 0;
+  if (t < m) {
+    const uint32_t pt = ~(uint32_t)S.ckey[t];
+    S.cpt[t] = pt;
+    S.cq[t][0] = C.Q[3 * pt];
+    S.cq[t][1] = C.Q[3 * pt + 1];
+    S.cq[t][2] = C.Q[3 * pt + 2];
+  }
+  g.sync();
+  // ---- visibility of every alive face from every candidate
+  for (int pass = 0;; pass++) {
+    for (uint32_t f = t; f < C.nf; f += Grp::G) {
+      const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * f);
+      const uint4 w0 = r4[0], w1 = r4[1], w2 = r4[2], w3 = r4[3], w4 = r4[4];
+      if (!(w0.x & ALIVE)) continue;
+      uint32_t mask = 0;
+      for (uint32_t j = 0; j < m; j++) {
+        float det;
+        int s = face_filter(w1, w2, S.cq[j], &det);
+        if (s == 0) s = orient_w(w2, w3, w4, S.cq[j]) > 0 ? 1 : -1;
+        mask |= s > 0 ? (1u << j) : 0u;
+      }
+      C.tests += m;
+      C.vmask[f] = mask;
+      if (mask) {
+        const uint32_t j0 = (uint32_t)__ffs(mask) - 1;
+        const uint32_t rest = mask & (mask - 1);
+        if (rest) atomicMin(&S.conflict, (uint32_t)__ffs(rest) - 1);
+        atomicAdd(&S.nvis[j0], 1u);
+        const uint32_t pos = atomicAdd(&S.nList, 1u);
+        if (pos < LCAP) {
+          S.lface[pos] = f;
+          S.la[pos] = w0.x & ~ALIVE;
+          S.lb[pos] = w0.y;
+          S.lc[pos] = w0.z;
+          S.lj[pos] = (uint8_t)j0;
         }
-        uint32_t cnt;
-        const uint32_t r = g.rank(keep, &cnt);
-        g.sync();
-        if (keep) C.cand[w + r] = i;
-        w += cnt;
-        g.sync();
       }
-      C.ncand = w;
     }
+    g.sync();
+    if (S.nList <= LCAP) break;
+    // the visible faces do not all fit: keep the longest prefix of candidates
+    // whose faces fit, and collect again
+    uint32_t cum = 0, mc = 0;
+    for (uint32_t j = 0; j < m; j++) {
+      if (cum + S.nvis[j] > LCAP) break;
+      cum += S.nvis[j];
+      mc = j + 1;
+    }
+    g.sync();
+    if (mc == 0 || pass > 0) {
+      hull_insert(g, C, S.cpt[0]);
+      return 1;
+    }
+    m = mc;
+    if (t < K) S.nvis[t] = 0;
+    if (t == 0) {
+      S.nList = 0;
+      S.conflict = K;
+    }
+    g.sync();
+  }
+  m = m < S.conflict ? m : S.conflict;
+  const uint32_t nL = S.nList;
+  // ---- horizon: directed edges of visible faces whose twin is not visible from the same candidate
+  for (uint32_t e = t; e < nL; e += Grp::G) {
+    if (S.lj[e] >= m) continue;
+    const uint32_t v3[3] = {S.la[e], S.lb[e], S.lc[e]};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const unsigned long long key = ((unsigned long long)v3[k] << 32) | v3[k == 2 ? 0 : k + 1];
+      uint32_t s = edge_hash(key, HT);
+      while (atomicCAS(&S.hkey[s], ~0ull, key) != ~0ull) s = (s + 1) & (HT - 1);
+      S.hj[s] = S.lj[e];
+    }
+  }
+  g.sync();
+  for (uint32_t e = t; e < nL; e += Grp::G) {
+    const uint32_t j = S.lj[e];
+    if (j >= m) continue;
+    const uint32_t v3[3] = {S.la[e], S.lb[e], S.lc[e]};
+    uint32_t flags = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const unsigned long long twin = ((unsigned long long)v3[k == 2 ? 0 : k + 1] << 32) | v3[k];
+      uint32_t s = edge_hash(twin, HT);
+      int tj = -1;
+      for (;;) {
+        const unsigned long long hk = S.hkey[s];
+        if (hk == twin) {
+          tj = S.hj[s];
+          break;
+        }
+        if (hk == ~0ull) break;
+        s = (s + 1) & (HT - 1);
+      }
+      if (tj != (int)j) flags |= 1u << k;
+    }
+    S.lflag[e] = (uint8_t)flags;
+    atomicAdd(&S.nh[j], (uint32_t)__popc(flags));
+  }
+  g.sync();
+  // capacity of the face array for the new faces of each prefix
+  {
+    uint32_t off = 0;
+    for (uint32_t j = 0; j < m; j++) {
+      if (C.nf + off + S.nh[j] > C.fmax) {
+        m = j;
+        break;
+      }
+      off += S.nh[j];
+    }
+  }
+  // new faces of candidate j, numbered (visible face in creation order, edge):
+  // each entry's base is the count of horizon edges of earlier entries of the same candidate
+  for (uint32_t e = t; e < nL; e += Grp::G) S.lbase[e] = 0;
+  g.sync();
+  for (uint32_t pr = t; pr < nL * nL; pr += Grp::G) {
+    const uint32_t e = pr / nL, e2 = pr - e * nL;
+    const uint32_t j = S.lj[e];
+    if (j < m && S.lj[e2] == j && S.lface[e2] < S.lface[e] && S.lflag[e2])
+      atomicAdd(&S.lbase[e], (uint32_t)__popc(S.lflag[e2]));
+  }
+  g.sync();
+  for (uint32_t e = t; e < nL; e += Grp::G) {
+    const uint32_t j = S.lj[e];
+    if (j >= m) continue;
+    const uint32_t flags = S.lflag[e];
+    if (!flags) continue;
+    const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * S.lface[e]);
+    const uint4 w2 = r4[2], w3 = r4[3], w4 = r4[4];
+    uint32_t off = 0;
+    for (uint32_t i = 0; i < j; i++) off += S.nh[i];
+    uint32_t rel = off + S.lbase[e];
+    const uint32_t v3[3] = {S.la[e], S.lb[e], S.lc[e]};
+    const int32_t q3[3][3] = {{(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z},
+                              {(int32_t)w3.x, (int32_t)w3.y, (int32_t)w3.z},
+                              {(int32_t)w4.x, (int32_t)w4.y, (int32_t)w4.z}};
+    const int32_t qp[3] = {S.cq[j][0], S.cq[j][1], S.cq[j][2]};
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      if (flags & (1u << k)) {
+        const int k1 = k == 2 ? 0 : k + 1;
+        const FaceRec R = face_make(v3[k], v3[k1], S.cpt[j], q3[k], q3[k1], qp);
+        face_store(C, C.nf + rel, R);
+        if (rel < NCAP) S.nrec[rel] = R;
+        rel++;
+      }
+  }
+  g.sync();
+  // a candidate that sees a face made earlier in the round ends the round before it
+  {
+    uint32_t total = 0, off = 0;
+    for (uint32_t j = 1; j < m; j++) {
+      off += S.nh[j - 1];
+      total += off;
+    }
+    for (uint32_t p = t; p < total; p += Grp::G) {
+      uint32_t j = 1, off2 = S.nh[0], start = 0;
+      while (p >= start + off2) {
+        start += off2;
+        off2 += S.nh[j];
+        j++;
+      }
+      C.tests++;
+      const uint32_t rel = p - start;
+      const float dd = rel < NCAP ? face_eval_rec(S.nrec[rel], S.cq[j]) : face_eval(C, C.nf + rel, S.cq[j]);
+      if (dd > 0.0f) atomicMin(&S.cutA, j);
+    }
+  }
+  g.sync();
+  m = m < S.cutA ? m : S.cutA;
+  // ---- reassignment of the points whose owner is visible, over the new faces
+  // of that candidate (the candidate itself lies on them and finds none).
+  // Collected first, then evaluated one point per thread.
+  for (uint32_t e = t; e < C.ncand; e += Grp::G) {
+    const uint32_t o = C.cown[e];
+    const uint4 ce = C.cand[e];
+    if (o == NONE) continue;
+    const uint32_t mask = C.vmask[o];
+    if (!mask) continue;
+    const uint32_t j = (uint32_t)__ffs(mask) - 1;
+    if (j >= m) continue;
+    const uint32_t pos = atomicAdd(&S.nOrph, 1u);
+    uint4* o4 = reinterpret_cast<uint4*>(C.orph) + 2 * pos;
+    o4[0] = make_uint4(e | (j << 27), ce.x, 0u, 0u);
+    o4[1] = ce;
+  }
+  g.sync();
+  const uint32_t no = S.nOrph;
+  for (uint32_t q = t; q < no; q += Grp::G) {
+    uint4* o4 = reinterpret_cast<uint4*>(C.orph) + 2 * q;
+    const uint4 ov = o4[0], ce = o4[1];
+    const uint32_t j = ov.x >> 27, i = ov.y;
+    const int32_t qi[3] = {(int32_t)ce.y, (int32_t)ce.z, (int32_t)ce.w};
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < j; k++) off += S.nh[k];
+    const uint32_t r1 = off + S.nh[j];
+    C.tests += S.nh[j];
+    uint32_t bf = NONE;
+    float bd = -1.0f;
+    for (uint32_t rel = off; rel < r1; rel++) {
+      const float dd = rel < NCAP ? face_eval_rec(S.nrec[rel], qi) : face_eval(C, C.nf + rel, qi);
+      if (dd > 0.0f && dd > bd) {
+        bd = dd;
+        bf = C.nf + rel;
+      }
+    }
+    o4->z = bf;
+    o4->w = __float_as_uint(bd);
+    if (bf != NONE) atomicMax(&S.okey[j], pkey(bd, i));
+  }
+  g.sync();
+  // a reassigned point farther than the next candidate ends the round before it
+  {
+    unsigned long long run = 0;
+    for (uint32_t j = 0; j < m; j++) {
+      if (j > 0 && S.ckey[j] < run) {
+        m = j;
+        break;
+      }
+      run = S.okey[j] > run ? S.okey[j] : run;
+    }
+  }
+  // ---- commit candidates [0, m)
+  uint32_t addF = 0, delF = 0;
+  for (uint32_t j = 0; j < m; j++) {
+    addF += S.nh[j];
+    delF += S.nvis[j];
+  }
+  for (uint32_t e = t; e < nL; e += Grp::G) {
+    if (S.lj[e] >= m) continue;
+    C.F[(size_t)FW * S.lface[e]] &= ~ALIVE;
+    C.fkey[S.lface[e]] = 0ull;
+  }
+  for (uint32_t q = t; q < no; q += Grp::G) {
+    const uint4 o4 = reinterpret_cast<const uint4*>(C.orph)[2 * q];
+    if ((o4.x >> 27) >= m) continue;
+    C.cown[o4.x & (HULL_MAX_NV - 1)] = o4.z;
+    if (o4.z != NONE) atomicMax(&C.fkey[o4.z], pkey(__uint_as_float(o4.w), o4.y));
+  }
+  C.nf += addF;
+  C.nalive = C.nalive + addF - delF;
+  C.nIns += m;
+  g.sync();
+  return m;
+}
+
+// farthest outstanding candidate (largest distance, then lowest index) until none
+template <typename Grp, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+__device__ __forceinline__ void hull_grow(const Grp& g, HullCtx& C, HullBatch<K, LCAP, HT, NW, NCAP>& S) {
+  const uint32_t maxRounds = C.nv + 8;
+  uint32_t kEff = K;  // candidates looked at: about twice what the last round inserted
+  for (uint32_t r = 0; r < maxRounds; r++) {
+    const uint32_t ins = hull_round(g, C, S, kEff);
+    if (ins == 0) return;
+    kEff = 2 * ins + 2;
+    kEff = kEff < 4 ? 4 : (kEff > K ? K : kEff);
+    if ((r & 3) == 3) compact_cand(g, C);
   }
 }
 
@@ -477,8 +945,9 @@ struct HullLevel {  // branch-and-bound inputs at the last tree level
 };
 
 // one hull job on one group; seed points in `seed` (80 entries of group-shared memory)
-template <typename Grp>
-__device__ void hull_job(const Grp& g, const dk::HullJob& J, uint32_t hj, uint32_t* seed, const float* __restrict__ verts,
+template <typename Grp, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+__device__ void hull_job(const Grp& g, HullBatch<K, LCAP, HT, NW, NCAP>& S, const dk::HullJob& J, uint32_t hj, uint32_t* seed,
+                         const float* __restrict__ verts,
                          uint32_t* __restrict__ arena, DPart* __restrict__ parts, HullResultD* __restrict__ results,
                          uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items, const DJob* __restrict__ jobs,
                          const HullLevel& HL, Counters* __restrict__ ctr, unsigned long long* testsOut) {
@@ -495,15 +964,19 @@ __device__ void hull_job(const Grp& g, const dk::HullJob& J, uint32_t hj, uint32
   C.nv = J.nv;
   C.fmax = hullFmax(J.nv);
   uint32_t* slab = arena + J.slabOff;
-  C.F = slab;
-  uint32_t* rest = slab + (size_t)FW * C.fmax;
-  C.Q = reinterpret_cast<int32_t*>(rest);
-  C.owner = rest + 3 * (size_t)C.nv;
-  C.dist = reinterpret_cast<float*>(rest + 4 * (size_t)C.nv);
-  C.cand = rest + 5 * (size_t)C.nv;
+  const size_t fm = C.fmax;
   const uint32_t cn = C.nv > 80 ? C.nv : 80;
-  C.vis = C.cand + cn;
-  C.map = C.vis + C.fmax;
+  C.F = slab;
+  C.fkey = reinterpret_cast<unsigned long long*>(slab + FW * fm);
+  C.vmask = slab + (FW + 2) * fm;
+  C.vis = C.vmask + fm;
+  C.map = C.vis + fm;
+  C.orph = C.map + fm;  // 16-byte aligned: 25 f is a multiple of 4
+  C.cand = reinterpret_cast<uint4*>(C.orph + 8 * (size_t)cn);
+  C.cand2 = C.cand + cn;
+  C.Q = reinterpret_cast<int32_t*>(C.cand2 + cn);
+  C.cown = reinterpret_cast<uint32_t*>(C.Q) + 3 * (size_t)C.nv;
+  C.cown2 = C.cown + cn;
   C.nf = C.nalive = C.ncand = C.nIns = 0;
   C.tests = 0;
   bool done = false;
@@ -536,31 +1009,42 @@ __device__ void hull_job(const Grp& g, const dk::HullJob& J, uint32_t hj, uint32
         C.Q[3 * i + k] = (int32_t)rintf(d * qs);
       }
     g.sync();
-    // seeds: maximum and minimum along each direction (lowest index on ties)
-    for (int k = 0; k < 40; k++) {
-      const float ax = KDIR[k][0], ay = KDIR[k][1], az = KDIR[k][2];
-      float mx = -1e30f, mn = 1e30f;
-      uint32_t ix = NONE, in = NONE;
+    // seeds: maximum and minimum along each direction (lowest index on ties), 8 directions per pass
+    for (int k0 = 0; k0 < 40; k0 += 4) {
+      float mx[4], mn[4];
+      uint32_t ix[4], in[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        mx[k] = -1e30f;
+        mn[k] = 1e30f;
+        ix[k] = NONE;
+        in[k] = NONE;
+      }
       for (uint32_t i = t; i < C.nv; i += Grp::G) {
         const int32_t* q = C.Q + 3 * i;
-        float d = ax * (float)q[0];
-        d = d + ay * (float)q[1];
-        d = d + az * (float)q[2];
-        if (d > mx) {
-          mx = d;
-          ix = i;
-        }
-        if (d < mn) {
-          mn = d;
-          in = i;
+        const float x = (float)q[0], y = (float)q[1], z = (float)q[2];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          float d = KDIR[k0 + k][0] * x;
+          d = d + KDIR[k0 + k][1] * y;
+          d = d + KDIR[k0 + k][2] * z;
+          if (d > mx[k]) {
+            mx[k] = d;
+            ix[k] = i;
+          }
+          if (d < mn[k]) {
+            mn[k] = d;
+            in[k] = i;
+          }
         }
       }
-      g.argmax(mx, ix);
-      g.argmin(mn, in);
-      if (t == 0) {
-        seed[2 * k] = ix;
-        seed[2 * k + 1] = in;
-      }
+      g.template argext<4>(mx, ix, mn, in);
+      if (t == 0)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          seed[2 * (k0 + k)] = ix[k];
+          seed[2 * (k0 + k) + 1] = in[k];
+        }
     }
     g.sync();
     // simplex from the seeds (exact integer tests, first seed on ties)
@@ -626,12 +1110,18 @@ __device__ void hull_job(const Grp& g, const dk::HullJob& J, uint32_t hj, uint32
       g.sync();
       // phase 1: the seeds (duplicates assign identically)
       for (uint32_t k = t; k < 80; k += Grp::G) {
-        C.cand[k] = seed[k];
-        assign_point(C, seed[k], 0, 4);
+        float bd;
+        const int32_t* qs3 = C.Q + 3 * seed[k];
+        const uint32_t bf = best_face(C, qs3, 0, 4, &bd);
+        C.cand[k] = make_uint4(seed[k], (uint32_t)qs3[0], (uint32_t)qs3[1], (uint32_t)qs3[2]);
+        C.cown[k] = bf;
+        if (bf != NONE) atomicMax(&C.fkey[bf], pkey(bd, seed[k]));
       }
       C.ncand = 80;
       g.sync();
-      hull_grow(g, C);
+      for (int phase = 1;; phase++) {
+      hull_grow(g, C, S);
+      if (phase == 2) break;
       res.status = 0;
       if (HL.prune) {
         // branch-and-bound: rough-hull rv lower bound vs the seed's best upper bound
@@ -660,25 +1150,81 @@ __device__ void hull_job(const Grp& g, const dk::HullJob& J, uint32_t hj, uint32
         }
       }
       if (!done) {
-        // phase 2: every point against the seed hull; outside points become candidates
-        uint32_t nc = 0;
+        // phase 2: every point against the seed hull; outside points become candidates.
+        // The seed hull's alive faces are staged in LDS (in the edge-hash space of
+        // the batch state) when they fit; dead faces never win, so skipping them
+        // keeps the first-in-creation-order choice.
+        for (uint32_t f = t; f < C.nf; f += Grp::G) C.fkey[f] = 0ull;
+        uint4* stage = reinterpret_cast<uint4*>(S.hkey);
+        constexpr uint32_t SCAP = (uint32_t)(sizeof(S.hkey) / 48);
+        uint32_t ns = 0;
+        for (uint32_t base = 0; base < C.nf; base += Grp::G) {
+          const uint32_t f = base + t;
+          const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * f);
+          const bool alive = f < C.nf && (r4[0].x & ALIVE);
+          uint32_t cnt;
+          const uint32_t rk = g.rank(alive, &cnt);
+          if (alive && ns + rk < SCAP) {
+            stage[3 * (ns + rk)] = r4[1];
+            stage[3 * (ns + rk) + 1] = r4[2];
+            stage[3 * (ns + rk) + 2] = make_uint4(f, 0u, 0u, 0u);
+          }
+          ns += cnt;
+        }
+        g.sync();
+        const bool staged = ns <= SCAP;
         const uint32_t nf = C.nf;
+        uint32_t nc = 0;
         for (uint32_t base = 0; base < C.nv; base += Grp::G) {
           const uint32_t i = base + t;
-          bool out = false;
+          uint32_t bf = NONE;
+          float bd = -1.0f;
+          int32_t qi[3] = {0, 0, 0};
           if (i < C.nv) {
-            assign_point(C, i, 0, nf);
-            out = C.owner[i] != NONE;
+            qi[0] = C.Q[3 * i];
+            qi[1] = C.Q[3 * i + 1];
+            qi[2] = C.Q[3 * i + 2];
+            if (staged) {
+              C.tests += ns;
+              for (uint32_t s = 0; s < ns; s++) {
+                const uint4 w1 = stage[3 * s], w2 = stage[3 * s + 1];
+                float det;
+                const int sg = face_filter(w1, w2, qi, &det);
+                float dd;
+                if (sg > 0) {
+                  const float nl = __uint_as_float(w2.w);
+                  dd = det / (nl > 1e-30f ? nl : 1e-30f);
+                } else if (sg < 0) {
+                  dd = -1.0f;
+                } else {
+                  dd = orient_rec(C.F + (size_t)FW * stage[3 * s + 2].x, w2, qi) > 0 ? 1e-30f : -1.0f;
+                }
+                if (dd > 0.0f && dd > bd) {
+                  bd = dd;
+                  bf = stage[3 * s + 2].x;
+                }
+              }
+            } else {
+              bf = best_face(C, qi, 0, nf, &bd);
+            }
           }
+          const bool out = bf != NONE;
           uint32_t cnt;
           const uint32_t r = g.rank(out, &cnt);
-          if (out) C.cand[nc + r] = i;
+          if (out) {
+            C.cand[nc + r] = make_uint4(i, (uint32_t)qi[0], (uint32_t)qi[1], (uint32_t)qi[2]);
+            C.cown[nc + r] = bf;
+            atomicMax(&C.fkey[bf], pkey(bd, i));
+          }
           nc += cnt;
         }
         C.ncand = nc;
         res.nCand = nc;
         g.sync();
-        hull_grow(g, C);
+      }
+      if (done) break;
+      }
+      if (!done) {
         res.volume = hull_volume(g, C);
         // alive faces in creation order: count and optional export
         uint32_t w = 0;
@@ -729,8 +1275,9 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
   const uint32_t hj = blockIdx.x * HW + wid;
   if (hj >= nJobsH) return;
   __shared__ uint32_t s_seed[HW][80];
+  __shared__ HullBatch<KB_W, LCAP_W, HT_W, 1, NCAP_W> s_batch[HW];
   const WaveGrp g;
-  hull_job(g, hjobs[hj], hj, s_seed[wid], verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTests);
+  hull_job(g, s_batch[wid], hjobs[hj], hj, s_seed[wid], verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTests);
 }
 
 // large parts: one workgroup of HBIG threads per part
@@ -746,11 +1293,14 @@ __global__ __launch_bounds__(HBIG) void k_hull_big(const HullJob* __restrict__ h
   __shared__ uint32_t s_u[HBIG / 64];
   __shared__ float s_f[HBIG / 64];
   __shared__ unsigned long long s_l[3 * (HBIG / 64)];
+  __shared__ uint32_t s_m[4 * 16 * (HBIG / 64)];
+  __shared__ HullBatch<KB_BIG, LCAP_BIG, HT_BIG, HBIG / 64, NCAP_BIG> s_batch;
   BlockGrp<HBIG> g;
   g.su = s_u;
   g.sf = s_f;
   g.sl = s_l;
-  hull_job(g, hjobs[hj], hj, s_seed, verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTestsBig);
+  g.sm = s_m;
+  hull_job(g, s_batch, hjobs[hj], hj, s_seed, verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTestsBig);
 }
 
 }  // namespace dk
