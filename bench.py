@@ -31,16 +31,20 @@ def _package():
     return mod
 
 
-# Algorithmic work per unit the kernels count themselves: (counter, amount per unit, unit kind).
-#  hull / hull_big: one point-face test reads one 16-byte face plane (a, b, c, offset);
+# Algorithmic work the kernels count themselves: (counter, amount per unit, unit kind).
+#  hull / hull_big: HBM bytes a part's hull needs at least, 12 nv (points read once) + 16 (result)
+#    + 12 per exported face, summed by the kernel;
 #  emit: the job's parent read plus the children it writes, summed in bytes by the kernel;
 #  hd_dist: one point-triangle distance evaluation = 70 FP32 operations.
 KERNEL_WORK = {
-    "hull_big": ("face_tests_big", 16.0, "bytes"),
-    "hull": ("face_tests", 16.0, "bytes"),
+    "hull_big": ("hull_bytes_big", 1.0, "bytes"),
+    "hull": ("hull_bytes", 1.0, "bytes"),
     "emit": ("emit_bytes", 1.0, "bytes"),
     "hd_dist": ("hd_evals", 70.0, "flops"),
 }
+# point-face evaluations per hull kernel, reported as effort beside the roofline
+HULL_TESTS = {"hull_big": "face_tests_big", "hull": "face_tests"}
+COUNTERS = ("face_tests", "face_tests_big", "emit_bytes", "hd_evals", "hull_jobs", "hull_bytes", "hull_bytes_big")
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3    # MI355X dense FP32
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
@@ -52,12 +56,12 @@ def sum_kernels(results):
         k = r.kernels
         if tot is None:
             tot = dict(ms=dict(k["ms"]), launches=dict(k["launches"]),
-                       **{n: k[n] for n in ("face_tests", "face_tests_big", "emit_bytes", "hd_evals", "hull_jobs")})
+                       **{n: k[n] for n in COUNTERS})
             continue
         for n in k["ms"]:
             tot["ms"][n] += k["ms"][n]
             tot["launches"][n] += k["launches"][n]
-        for n in ("face_tests", "face_tests_big", "emit_bytes", "hd_evals", "hull_jobs"):
+        for n in COUNTERS:
             tot[n] += k[n]
     return tot
 
@@ -82,9 +86,13 @@ def roofline(kst):
             t = json.load(f)
         if t.get("kernel") == name:
             traffic = t["hbm_bytes_per_launch"]
+    effort = None
+    if name in HULL_TESTS:
+        effort = dict(point_face_tests_per_s=kst[HULL_TESTS[name]] / (kst["ms"][name] * 1e-3),
+                      tests_per_launch=kst[HULL_TESTS[name]] / launches)
     return dict(bound=bound, achieved=achieved, peak=peak, unit=unit, frac=achieved / peak, traffic=traffic,
                 kernel=name, launches=kst["launches"][name], avg_launch_ms=avg_ms,
-                work_per_launch=work_per_launch, work_counter=counter, work_per_unit=per_unit,
+                work_per_launch=work_per_launch, work_counter=counter, work_per_unit=per_unit, effort=effort,
                 kernel_ms={n: round(v, 3) for n, v in kst["ms"].items()},
                 kernel_launches=dict(kst["launches"]))
 

@@ -77,13 +77,15 @@ KERNEL_NAMES = ["plan", "scan", "emit", "hull", "hull_big", "record", "evaluate"
 
 class KernelStats(C.Structure):
     _fields_ = [("ms", f64 * NUM_KERNELS), ("launches", u64 * NUM_KERNELS), ("face_tests", u64),
-                ("emit_bytes", u64), ("hd_evals", u64), ("hull_jobs", u64), ("face_tests_big", u64)]
+                ("emit_bytes", u64), ("hd_evals", u64), ("hull_jobs", u64), ("face_tests_big", u64),
+                ("hull_bytes", u64), ("hull_bytes_big", u64)]
 
     def as_dict(self):
         return dict(ms={n: self.ms[i] for i, n in enumerate(KERNEL_NAMES)},
                     launches={n: self.launches[i] for i, n in enumerate(KERNEL_NAMES)},
                     face_tests=self.face_tests, face_tests_big=self.face_tests_big, emit_bytes=self.emit_bytes,
-                    hd_evals=self.hd_evals, hull_jobs=self.hull_jobs)
+                    hd_evals=self.hd_evals, hull_jobs=self.hull_jobs, hull_bytes=self.hull_bytes,
+                    hull_bytes_big=self.hull_bytes_big)
 
 
 class CutRecord(C.Structure):

@@ -42,6 +42,37 @@ __device__ __forceinline__ double fix_to_f64(i128 a) {
 }
 __device__ __forceinline__ i128 fix_abs(i128 a) { return a < 0 ? -a : a; }
 
+// a 128-bit two's-complement accumulator in two 64-bit words; adding the fixed
+// point value of a float (x * 2^64 truncated toward zero, as fix_f32) needs only
+// 64-bit shifts, so the sum equals the i128 sum of fix_f32 terms exactly
+struct Acc128 {
+  unsigned long long lo = 0, hi = 0;
+  __device__ __forceinline__ void add_f32(float x, bool neg) {
+    const uint32_t b = __float_as_uint(x);
+    const uint32_t eb = (b >> 23) & 0xffu;
+    const uint32_t m = eb == 0 ? (b & 0x7fffffu) : ((b & 0x7fffffu) | 0x800000u);
+    const int sh = (int)(eb == 0 ? 1u : eb) - 86;
+    unsigned long long vlo, vhi;
+    if (sh < 0) {
+      vlo = sh > -24 ? (unsigned long long)(m >> (uint32_t)(-sh)) : 0ull;
+      vhi = 0;
+    } else if (sh < 64) {
+      vlo = (unsigned long long)m << sh;
+      vhi = sh == 0 ? 0ull : ((unsigned long long)m >> (64 - sh));
+    } else {
+      vlo = 0;
+      vhi = (unsigned long long)m << (sh - 64);
+    }
+    if (((b >> 31) != 0) != neg) {
+      vlo = ~vlo + 1ull;
+      vhi = ~vhi + (vlo == 0ull ? 1ull : 0ull);
+    }
+    lo += vlo;
+    hi += vhi + (lo < vlo ? 1ull : 0ull);
+  }
+  __device__ __forceinline__ i128 value() const { return (i128)(((unsigned __int128)hi << 64) | lo); }
+};
+
 __device__ __forceinline__ void acc_triple(i128& acc, float x, float y, float z, bool neg) {
   const float p = x * y;
   const float e = __builtin_fmaf(x, y, -p);
@@ -51,7 +82,18 @@ __device__ __forceinline__ void acc_triple(i128& acc, float x, float y, float z,
   const i128 s = fix_f32(p2) + fix_f32(e2) + fix_f32(e3);
   acc += neg ? -s : s;
 }
-__device__ __forceinline__ void acc_det(i128& acc, const float* a, const float* b, const float* c, bool neg) {
+__device__ __forceinline__ void acc_triple(Acc128& acc, float x, float y, float z, bool neg) {
+  const float p = x * y;
+  const float e = __builtin_fmaf(x, y, -p);
+  const float p2 = p * z;
+  const float e2 = __builtin_fmaf(p, z, -p2);
+  const float e3 = e * z;
+  acc.add_f32(p2, neg);
+  acc.add_f32(e2, neg);
+  acc.add_f32(e3, neg);
+}
+template <typename A>
+__device__ __forceinline__ void acc_det(A& acc, const float* a, const float* b, const float* c, bool neg) {
   acc_triple(acc, a[0], b[1], c[2], neg);
   acc_triple(acc, a[0], b[2], c[1], !neg);
   acc_triple(acc, a[1], b[2], c[0], neg);
@@ -59,7 +101,8 @@ __device__ __forceinline__ void acc_det(i128& acc, const float* a, const float* 
   acc_triple(acc, a[2], b[0], c[1], neg);
   acc_triple(acc, a[2], b[1], c[0], !neg);
 }
-__device__ __forceinline__ void acc_det_anchor(i128& acc, const float* a, const float* b, const float* c, const float* o) {
+template <typename A>
+__device__ __forceinline__ void acc_det_anchor(A& acc, const float* a, const float* b, const float* c, const float* o) {
   acc_det(acc, a, b, c, false);
   acc_det(acc, o, b, c, true);
   acc_det(acc, a, o, c, true);
@@ -322,22 +365,51 @@ __device__ inline void anchor_for(const float pls[4][4], uint32_t n, float* o) {
   plane_point(b, o);
 }
 
-// src/gpu/shaders.ts:171-181 clip_tri: entries 0..2 = corner, 4+k = crossing on edge k->k+1
+// src/gpu/shaders.ts:171-181 clip_tri: entries 0..2 = corner, 4+k = crossing on edge k->k+1.
+// Entries are packed 4 bits each so that every index stays in registers.
 struct Poly {
   uint32_t n;
-  uint32_t src[4];
+  uint32_t src;
 };
+__device__ __forceinline__ uint32_t psrc(const Poly& p, uint32_t i) { return (p.src >> (4 * i)) & 0xFu; }
+// a[k] for k in 0..2 without indexing memory
+template <typename T>
+__device__ __forceinline__ T sel3(const T* a, uint32_t k) {
+  return k == 0 ? a[0] : (k == 1 ? a[1] : a[2]);
+}
 __device__ __forceinline__ Poly clip_tri(const int* sg, int side) {
   Poly p;
   p.n = 0;
+  p.src = 0;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     const int q = k == 2 ? 0 : k + 1;
     const int sp = sg[k] * side, sq = sg[q] * side;
-    if (sp >= 0) p.src[p.n++] = (uint32_t)k;
-    if (sg[k] != 0 && sg[q] != 0 && sp * sq < 0) p.src[p.n++] = 4u + (uint32_t)k;
+    if (sp >= 0) {
+      p.src |= (uint32_t)k << (4 * p.n);
+      p.n++;
+    }
+    if (sg[k] != 0 && sg[q] != 0 && sp * sq < 0) {
+      p.src |= (4u + (uint32_t)k) << (4 * p.n);
+      p.n++;
+    }
   }
   return p;
+}
+// new vertices and on-plane edges of a clipped polygon
+__device__ __forceinline__ void poly_stats(const Poly& pp, const int* sg, uint32_t* nnew, uint32_t* nedge) {
+  uint32_t a = 0, b = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < 4; i++) {
+    if (i < pp.n) {
+      const uint32_t u = psrc(pp, i), w = psrc(pp, i + 1 == pp.n ? 0 : i + 1);
+      if (u >= 4) a++;
+      const bool uOn = u >= 4 || sel3(sg, u) == 0, wOn = w >= 4 || sel3(sg, w) == 0;
+      if (uOn && wOn) b++;
+    }
+  }
+  *nnew = a;
+  *nedge = b;
 }
 
 }  // namespace dk

@@ -47,6 +47,8 @@ struct Counters {
   unsigned long long emitBytes;     // algorithmic bytes of emit: parent read + children written
   unsigned long long hdEvals;       // point-triangle distance evaluations
   unsigned long long faceTestsBig;  // point-face record evaluations, workgroup-per-part hull kernel
+  unsigned long long hullBytes;     // algorithmic HBM bytes, wave-per-part hull: 12 nv + 16 (+ 12 per exported face)
+  unsigned long long hullBytesBig;  // the same for the workgroup-per-part hull kernel
   uint32_t hd[1024];  // Hausdorff results (f32 bits of the squared distance)
 };
 

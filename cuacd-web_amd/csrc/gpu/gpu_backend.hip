@@ -124,17 +124,19 @@ class GpuBackend : public Backend {
   }
   // work counters of the kernels (cleared at the start of every entry point, summed at its end)
   void clearWork() {
-    const unsigned long long z[4] = {0, 0, 0, 0};
+    const unsigned long long z[6] = {0, 0, 0, 0, 0, 0};
     h2d(&ctr_.p->faceTests, z, sizeof(z), "clear work counters");
   }
   void collectWork() {
-    unsigned long long w[4];
+    unsigned long long w[6];
     hipCheck(hipMemcpyAsync(w, &ctr_.p->faceTests, sizeof(w), hipMemcpyDeviceToHost, stream_), "read work counters");
     sync();
     kst_.faceTests += w[0];
     kst_.emitBytes += w[1];
     kst_.hdEvals += w[2];
     kst_.faceTestsBig += w[3];
+    kst_.hullBytes += w[4];
+    kst_.hullBytesBig += w[5];
   }
   // host -> device copy ordered on the stream that returns once the host buffer may be reused
   void h2d(void* dst, const void* src, size_t bytes, const char* what) {
@@ -475,6 +477,8 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
   kst_.emitBytes += c1.emitBytes;
   kst_.hdEvals += c1.hdEvals;
   kst_.faceTestsBig += c1.faceTestsBig;
+  kst_.hullBytes += c1.hullBytes;
+  kst_.hullBytesBig += c1.hullBytesBig;
   return R;
 }
 

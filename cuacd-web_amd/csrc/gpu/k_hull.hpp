@@ -950,7 +950,8 @@ __device__ void hull_job(const Grp& g, HullBatch<K, LCAP, HT, NW, NCAP>& S, cons
                          const float* __restrict__ verts,
                          uint32_t* __restrict__ arena, DPart* __restrict__ parts, HullResultD* __restrict__ results,
                          uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items, const DJob* __restrict__ jobs,
-                         const HullLevel& HL, Counters* __restrict__ ctr, unsigned long long* testsOut) {
+                         const HullLevel& HL, Counters* __restrict__ ctr, unsigned long long* testsOut,
+                         unsigned long long* bytesOut) {
   const uint32_t t = g.tid();
   HullResultD res;
   res.volume = 0.0;
@@ -1250,6 +1251,10 @@ __device__ void hull_job(const Grp& g, HullBatch<K, LCAP, HT, NW, NCAP>& S, cons
   const i128 tests = g.sum((i128)C.tests);
   if (t == 0) {
     atomicAdd(testsOut, (unsigned long long)tests);
+    // algorithmic HBM bytes: the points read once, the result record, the exported faces
+    unsigned long long hb = 12ull * J.nv + 16ull;
+    if (J.exportOff != NONE) hb += 12ull * res.nFaces;
+    atomicAdd(bytesOut, hb);
     results[hj] = res;
     if (parts) {
       DPart& pd = parts[J.part];
@@ -1277,7 +1282,7 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
   __shared__ uint32_t s_seed[HW][80];
   __shared__ HullBatch<KB_W, LCAP_W, HT_W, 1, NCAP_W> s_batch[HW];
   const WaveGrp g;
-  hull_job(g, s_batch[wid], hjobs[hj], hj, s_seed[wid], verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTests);
+  hull_job(g, s_batch[wid], hjobs[hj], hj, s_seed[wid], verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTests, &ctr->hullBytes);
 }
 
 // large parts: one workgroup of HBIG threads per part
@@ -1300,7 +1305,7 @@ __global__ __launch_bounds__(HBIG) void k_hull_big(const HullJob* __restrict__ h
   g.sf = s_f;
   g.sl = s_l;
   g.sm = s_m;
-  hull_job(g, s_batch, hjobs[hj], hj, s_seed, verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTestsBig);
+  hull_job(g, s_batch, hjobs[hj], hj, s_seed, verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTestsBig, &ctr->hullBytesBig);
 }
 
 }  // namespace dk

@@ -43,6 +43,32 @@ __device__ __forceinline__ T block_excl_scan(T v, T* sh, T* total) {
   *total = tot;
   return off + inc - v;
 }
+// two exclusive scans over the block at once (one barrier pair); sh holds >= 32 entries
+template <typename T>
+__device__ __forceinline__ void block_excl_scan2(T a, T b, T* sh, T* pa, T* pb, T* ta, T* tb) {
+  const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const T ia = wave_incl_scan(a), ib = wave_incl_scan(b);
+  if (lane == 63) {
+    sh[2 * wid] = ia;
+    sh[2 * wid + 1] = ib;
+  }
+  __syncthreads();
+  T oa = 0, ob = 0, sa = 0, sb = 0;
+  for (uint32_t w = 0; w < nw; w++) {
+    const T xa = sh[2 * w], xb = sh[2 * w + 1];
+    if (w < wid) {
+      oa += xa;
+      ob += xb;
+    }
+    sa += xa;
+    sb += xb;
+  }
+  __syncthreads();
+  *pa = oa + ia - a;
+  *pb = ob + ib - b;
+  *ta = sa;
+  *tb = sb;
+}
 __device__ __forceinline__ uint32_t block_sum_u(uint32_t v, uint32_t* sh) {
   v = wave_sum_u(v);
   const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -194,13 +220,13 @@ __global__ __launch_bounds__(BT) void k_plan(LevelParams P, DItem* __restrict__ 
     for (int side = 0; side < 2; side++) {
       const Poly pp = clip_tri(sg, side == 0 ? 1 : -1);
       if (pp.n < 3) continue;
-      uint32_t nnew = 0, nedge = 0;
-      for (uint32_t i = 0; i < pp.n; i++) {
-        const uint32_t u = pp.src[i], w = pp.src[i + 1 == pp.n ? 0 : i + 1];
-        if (u >= 4) nnew++;
-        else S[2 * tr[u] + side] = 1;
-        const bool uOn = u >= 4 || sg[u] == 0, wOn = w >= 4 || sg[w] == 0;
-        if (uOn && wOn) nedge++;
+      uint32_t nnew, nedge;
+      poly_stats(pp, sg, &nnew, &nedge);
+#pragma unroll
+      for (uint32_t i = 0; i < 4; i++) {
+        if (i >= pp.n) break;
+        const uint32_t u = psrc(pp, i);
+        if (u < 4) S[2 * sel3(tr, u) + side] = 1;
       }
       if (side == 0) {
         posT += pp.n - 2;
@@ -392,11 +418,30 @@ __global__ __launch_bounds__(ST) void k_scan(LevelParams P, DItem* __restrict__ 
 // emit: one block per valid job — both children and their volumes
 __device__ __forceinline__ void interp(const float* V, const uint32_t* tr, const float* d, uint32_t k, float* x) {
   const uint32_t q = k == 2 ? 0 : k + 1;
-  const uint32_t ia = tr[k] < tr[q] ? tr[k] : tr[q], ib = tr[k] < tr[q] ? tr[q] : tr[k];
-  const float da = tr[k] < tr[q] ? d[k] : d[q], db = tr[k] < tr[q] ? d[q] : d[k];
+  const uint32_t tk = sel3(tr, k), tq = sel3(tr, q);
+  const float dk = sel3(d, k), dq = sel3(d, q);
+  const uint32_t ia = tk < tq ? tk : tq, ib = tk < tq ? tq : tk;
+  const float da = tk < tq ? dk : dq, db = tk < tq ? dq : dk;
   const float tt = da / (da - db);
   for (int c = 0; c < 3; c++) x[c] = V[3 * ia + c] + tt * (V[3 * ib + c] - V[3 * ia + c]);
 }
+
+// clip counts of one triangle on both sides: (tris | new verts << 21 | on-plane edges << 42)
+__device__ __forceinline__ void emit_counts(const int* sg, unsigned long long* st) {
+  st[0] = st[1] = 0;
+  if (sg[0] == 0 && sg[1] == 0 && sg[2] == 0) return;
+  for (int s = 0; s < 2; s++) {
+    const Poly pp = clip_tri(sg, s == 0 ? 1 : -1);
+    if (pp.n < 3) continue;
+    uint32_t nnew, nedge;
+    poly_stats(pp, sg, &nnew, &nedge);
+    st[s] = (unsigned long long)(pp.n - 2) | ((unsigned long long)nnew << 21) | ((unsigned long long)nedge << 42);
+  }
+}
+
+// Each thread takes EPT consecutive items per chunk, so one block scan per
+// chunk gives offsets in parent order; running totals stay in registers.
+constexpr uint32_t EPT = 1;
 
 __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ parts, const DJob* __restrict__ jobs,
                                              const float4* __restrict__ jobPlanes, float* __restrict__ verts,
@@ -408,11 +453,10 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
   const uint32_t jobIdx = P.jobBase + g;
   const DJob job = jobs[jobIdx];
   if (job.valid != 1) return;
-  __shared__ uint32_t shs[16];
-  __shared__ unsigned long long shl[16];
+  __shared__ uint32_t shs[32];
+  __shared__ unsigned long long shl[32];
   __shared__ unsigned long long shw[16];
   __shared__ float s_o[3], s_cp[3];
-  __shared__ uint32_t s_run[6];
   const DPart part = parts[job.parentPart];
   const float pl[4] = {job.plane[0], job.plane[1], job.plane[2], job.plane[3]};
   if (tid == 0) {
@@ -441,7 +485,6 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
       s_o[q] = o[q];
       s_cp[q] = cp[q];
     }
-    for (int q = 0; q < 6; q++) s_run[q] = 0;
   }
   __syncthreads();
   const float o[3] = {s_o[0], s_o[1], s_o[2]}, cP[3] = {s_cp[0], s_cp[1], s_cp[2]};
@@ -455,210 +498,233 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
   const uint32_t eOut[2] = {job.off[2], job.off[5]};
   const uint32_t kept[2] = {job.cnt[6], job.cnt[7]};
   // 1. vertex remap: kept vertices in parent order
-  for (uint32_t c = 0; c * BT < part.nv; c++) {
-    const uint32_t i = c * BT + tid;
-    uint32_t f[2] = {0, 0};
-    if (i < part.nv) {
-      f[0] = S[2 * i];
-      f[1] = S[2 * i + 1];
-    }
-    uint32_t tot[2];
-    const uint32_t pre0 = block_excl_scan(f[0], shs, &tot[0]);
-    const uint32_t pre1 = block_excl_scan(f[1], shs, &tot[1]);
-    if (i < part.nv) {
-      const uint32_t pre[2] = {pre0, pre1};
-      for (int s = 0; s < 2; s++) {
-        uint32_t idx = NONE;
-        if (f[s]) {
-          idx = s_run[s] + pre[s];
-          float* dst = verts + vOut[s] + 3 * idx;
-          dst[0] = V[3 * i];
-          dst[1] = V[3 * i + 1];
-          dst[2] = V[3 * i + 2];
-        }
-        S[2 * i + s] = idx;
+  {
+    uint32_t run0 = 0, run1 = 0;
+    for (uint32_t base = 0; base < part.nv; base += BT * EPT) {
+      const uint32_t i0 = base + tid * EPT;
+      uint32_t f0[EPT], f1[EPT], c0 = 0, c1 = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < EPT; k++) {
+        const uint32_t i = i0 + k;
+        f0[k] = i < part.nv ? S[2 * i] : 0u;
+        f1[k] = i < part.nv ? S[2 * i + 1] : 0u;
+        c0 += f0[k];
+        c1 += f1[k];
       }
+      uint32_t p0, p1, t0, t1;
+      block_excl_scan2(c0, c1, shs, &p0, &p1, &t0, &t1);
+      p0 += run0;
+      p1 += run1;
+#pragma unroll
+      for (uint32_t k = 0; k < EPT; k++) {
+        const uint32_t i = i0 + k;
+        if (i >= part.nv) break;
+        uint32_t idx0 = NONE, idx1 = NONE;
+        if (f0[k] | f1[k]) {
+          const float x = V[3 * i], y = V[3 * i + 1], z = V[3 * i + 2];
+          if (f0[k]) {
+            idx0 = p0++;
+            float* dst = verts + vOut[0] + 3 * idx0;
+            dst[0] = x;
+            dst[1] = y;
+            dst[2] = z;
+          }
+          if (f1[k]) {
+            idx1 = p1++;
+            float* dst = verts + vOut[1] + 3 * idx1;
+            dst[0] = x;
+            dst[1] = y;
+            dst[2] = z;
+          }
+        }
+        S[2 * i] = idx0;
+        S[2 * i + 1] = idx1;
+      }
+      run0 += t0;
+      run1 += t1;
     }
-    __syncthreads();
-    if (tid == 0) {
-      s_run[0] += tot[0];
-      s_run[1] += tot[1];
-    }
-    __syncthreads();
   }
-  if (tid == 0)
-    for (int q = 0; q < 6; q++) s_run[q] = 0;
-  __syncthreads();
-  i128 flux[2] = {0, 0};
+  __syncthreads();  // every remap is in place before triangles read it
+  Acc128 flux[2];
   // 2. triangles: both clipped polygons, fans, on-plane open edges
-  for (uint32_t c = 0; c * BT < part.nt; c++) {
-    const uint32_t t = c * BT + tid;
-    uint32_t tr[3] = {0, 0, 0};
-    float d[3] = {0.f, 0.f, 0.f};
-    int sg[3] = {0, 0, 0};
-    Poly pp[2];
-    pp[0].n = pp[1].n = 0;
-    unsigned long long st[2] = {0, 0};
-    bool live = false;
-    if (t < part.nt) {
-      tr[0] = T[3 * t];
-      tr[1] = T[3 * t + 1];
-      tr[2] = T[3 * t + 2];
-      for (int k = 0; k < 3; k++) {
-        d[k] = plane_eval(pl, V[3 * tr[k]], V[3 * tr[k] + 1], V[3 * tr[k] + 2]);
-        sg[k] = vsign(d[k]);
-      }
-      live = !(sg[0] == 0 && sg[1] == 0 && sg[2] == 0);
-      if (live) {
-        for (int s = 0; s < 2; s++) {
-          pp[s] = clip_tri(sg, s == 0 ? 1 : -1);
-          if (pp[s].n < 3) continue;
-          unsigned long long nnew = 0, nedge = 0;
-          for (uint32_t i = 0; i < pp[s].n; i++) {
-            const uint32_t u = pp[s].src[i], w = pp[s].src[i + 1 == pp[s].n ? 0 : i + 1];
-            if (u >= 4) nnew++;
-            const bool uOn = u >= 4 || sg[u] == 0, wOn = w >= 4 || sg[w] == 0;
-            if (uOn && wOn) nedge++;
-          }
-          st[s] = (unsigned long long)(pp[s].n - 2) | (nnew << 21) | (nedge << 42);
-        }
-      }
-    }
-    unsigned long long tot[2];
-    const unsigned long long pre0 = block_excl_scan(st[0], shl, &tot[0]);
-    const unsigned long long pre1 = block_excl_scan(st[1], shl, &tot[1]);
-    if (live) {
-      const unsigned long long pre[2] = {pre0, pre1};
-      for (int s = 0; s < 2; s++) {
-        const Poly& py = pp[s];
-        if (py.n < 3) continue;
-        uint32_t tBase = s_run[s] + (uint32_t)(pre[s] & 0x1fffff);
-        uint32_t vBase = kept[s] + s_run[2 + s] + (uint32_t)((pre[s] >> 21) & 0x1fffff);
-        uint32_t eBase = s_run[4 + s] + (uint32_t)(pre[s] >> 42);
-        uint32_t idx[4];
-        float pts[4][3];
-        bool onp[4];
-        for (uint32_t i = 0; i < py.n; i++) {
-          const uint32_t u = py.src[i];
-          if (u < 4) {
-            idx[i] = S[2 * tr[u] + s];
-            for (int q = 0; q < 3; q++) pts[i][q] = V[3 * tr[u] + q];
-            onp[i] = sg[u] == 0;
-          } else {
-            interp(V, tr, d, u - 4, pts[i]);
-            float* dst = verts + vOut[s] + 3 * vBase;
-            dst[0] = pts[i][0];
-            dst[1] = pts[i][1];
-            dst[2] = pts[i][2];
-            idx[i] = vBase++;
-            onp[i] = true;
-          }
-        }
-        for (uint32_t i = 1; i + 1 < py.n; i++) {
-          if (keepTopo) {
-            uint32_t* dst = tris + tOut[s] + 3 * tBase;
-            dst[0] = idx[0];
-            dst[1] = idx[i];
-            dst[2] = idx[i + 1];
-          }
-          tBase++;
-          acc_det_anchor(flux[s], pts[0], pts[i], pts[i + 1], o);
-        }
-        for (uint32_t i = 0; i < py.n; i++) {
-          const uint32_t jn = i + 1 == py.n ? 0 : i + 1;
-          if (onp[i] && onp[jn]) {
-            if (keepTopo) {
-              DEdge e;
-              for (int q = 0; q < 3; q++) {
-                e.p[q] = pts[i][q];
-                e.q[q] = pts[jn][q];
-              }
-              e.plane = jobIdx;
-              e.pad = 0;
-              edges[eOut[s] + eBase] = e;
-            }
-            eBase++;
-            acc_det_anchor(flux[s], cP, pts[jn], pts[i], o);
-          }
-        }
-      }
-    }
-    __syncthreads();
-    if (tid == 0) {
-      s_run[0] += (uint32_t)(tot[0] & 0x1fffff);
-      s_run[1] += (uint32_t)(tot[1] & 0x1fffff);
-      s_run[2] += (uint32_t)((tot[0] >> 21) & 0x1fffff);
-      s_run[3] += (uint32_t)((tot[1] >> 21) & 0x1fffff);
-      s_run[4] += (uint32_t)(tot[0] >> 42);
-      s_run[5] += (uint32_t)(tot[1] >> 42);
-    }
-    __syncthreads();
-  }
-  // 3. inherited open edges of earlier planes, clipped by this one
-  for (uint32_t c = 0; c * BT < part.ne; c++) {
-    const uint32_t e = c * BT + tid;
-    uint32_t cnt[2] = {0, 0};
-    DEdge ed;
-    float dp = 0.f, dq = 0.f;
-    if (e < part.ne) {
-      ed = E[e];
-      dp = plane_eval(pl, ed.p[0], ed.p[1], ed.p[2]);
-      dq = plane_eval(pl, ed.q[0], ed.q[1], ed.q[2]);
-      const bool sp = dp >= 0.0f, sq = dq >= 0.0f;
-      if (sp == sq) {
-        cnt[0] = sp ? 1u : 0u;
-        cnt[1] = sp ? 0u : 1u;
-      } else {
-        cnt[0] = cnt[1] = 1;
-      }
-    }
-    uint32_t tot[2];
-    const uint32_t pre0 = block_excl_scan(cnt[0], shs, &tot[0]);
-    const uint32_t pre1 = block_excl_scan(cnt[1], shs, &tot[1]);
-    if (e < part.ne) {
-      const float4 q4 = jobPlanes[ed.plane];
-      const float epl[4] = {q4.x, q4.y, q4.z, q4.w};
-      float cE[3];
-      plane_point(epl, cE);
-      const uint32_t base[2] = {s_run[4] + pre0, s_run[5] + pre1};
-      const bool sp = dp >= 0.0f, sq = dq >= 0.0f;
-      auto put = [&](int s, const float* a, const float* b) {
-        if (keepTopo) {
-          DEdge ne;
+  {
+    unsigned long long run[2] = {0, 0};  // packed (tris | new verts << 21 | edges << 42), tris/verts/edges
+    uint32_t runT[2] = {0, 0}, runN[2] = {0, 0}, runE[2] = {0, 0};
+    for (uint32_t base = 0; base < part.nt; base += BT * EPT) {
+      const uint32_t t0 = base + tid * EPT;
+      unsigned long long st[EPT][2];
+      unsigned long long a0 = 0, a1 = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < EPT; k++) {
+        const uint32_t t = t0 + k;
+        st[k][0] = st[k][1] = 0;
+        if (t < part.nt) {
+          int sg[3];
           for (int q = 0; q < 3; q++) {
-            ne.p[q] = a[q];
-            ne.q[q] = b[q];
+            const uint32_t v = T[3 * t + q];
+            sg[q] = vsign(plane_eval(pl, V[3 * v], V[3 * v + 1], V[3 * v + 2]));
           }
-          ne.plane = ed.plane;
-          ne.pad = 0;
-          edges[eOut[s] + base[s]] = ne;
+          emit_counts(sg, st[k]);
         }
-        acc_det_anchor(flux[s], cE, b, a, o);
-      };
-      if (sp == sq) {
-        put(sp ? 0 : 1, ed.p, ed.q);
-      } else {
-        const float tt = dp / (dp - dq);
-        float x[3];
-        for (int q = 0; q < 3; q++) x[q] = ed.p[q] + tt * (ed.q[q] - ed.p[q]);
-        if (sp) {
-          put(0, ed.p, x);
-          put(1, x, ed.q);
-        } else {
-          put(1, ed.p, x);
-          put(0, x, ed.q);
+        a0 += st[k][0];
+        a1 += st[k][1];
+      }
+      unsigned long long pre[2], tot[2];
+      block_excl_scan2(a0, a1, shl, &pre[0], &pre[1], &tot[0], &tot[1]);
+      (void)run;
+      for (uint32_t k = 0; k < EPT; k++) {
+        const uint32_t t = t0 + k;
+        if (t >= part.nt) break;
+        if (!(st[k][0] | st[k][1])) continue;
+        uint32_t tr[3];
+        float d[3];
+        int sg[3];
+        for (int q = 0; q < 3; q++) {
+          tr[q] = T[3 * t + q];
+          d[q] = plane_eval(pl, V[3 * tr[q]], V[3 * tr[q] + 1], V[3 * tr[q] + 2]);
+          sg[q] = vsign(d[q]);
+        }
+        for (int s = 0; s < 2; s++) {
+          if (!st[k][s]) continue;
+          const Poly py = clip_tri(sg, s == 0 ? 1 : -1);
+          uint32_t tBase = runT[s] + (uint32_t)(pre[s] & 0x1fffff);
+          uint32_t vBase = kept[s] + runN[s] + (uint32_t)((pre[s] >> 21) & 0x1fffff);
+          uint32_t eBase = runE[s] + (uint32_t)(pre[s] >> 42);
+          pre[s] += st[k][s];
+          const uint32_t n = py.n;
+          uint32_t idx[4];
+          float pts[4][3];
+          bool onp[4];
+#pragma unroll
+          for (uint32_t i = 0; i < 4; i++) {
+            idx[i] = 0;
+            onp[i] = false;
+            pts[i][0] = pts[i][1] = pts[i][2] = 0.0f;
+            if (i < n) {
+              const uint32_t u = psrc(py, i);
+              if (u < 4) {
+                const uint32_t vu = sel3(tr, u);
+                idx[i] = S[2 * vu + s];
+                for (int q = 0; q < 3; q++) pts[i][q] = V[3 * vu + q];
+                onp[i] = sel3(sg, u) == 0;
+              } else {
+                interp(V, tr, d, u - 4, pts[i]);
+                float* dst = verts + vOut[s] + 3 * vBase;
+                dst[0] = pts[i][0];
+                dst[1] = pts[i][1];
+                dst[2] = pts[i][2];
+                idx[i] = vBase++;
+                onp[i] = true;
+              }
+            }
+          }
+#pragma unroll
+          for (uint32_t i = 1; i < 3; i++) {
+            if (i + 1 >= n) break;
+            if (keepTopo) {
+              uint32_t* dst = tris + tOut[s] + 3 * tBase;
+              dst[0] = idx[0];
+              dst[1] = idx[i];
+              dst[2] = idx[i + 1];
+            }
+            tBase++;
+            acc_det_anchor(flux[s], pts[0], pts[i], pts[i + 1], o);
+          }
+#pragma unroll
+          for (uint32_t i = 0; i < 4; i++) {
+            if (i >= n) break;
+            const bool last = i + 1 == n;
+            const uint32_t i1 = (i + 1) & 3;
+            const float nx[3] = {last ? pts[0][0] : pts[i1][0], last ? pts[0][1] : pts[i1][1],
+                                 last ? pts[0][2] : pts[i1][2]};
+            const bool onn = last ? onp[0] : onp[i1];
+            if (onp[i] && onn) {
+              if (keepTopo) {
+                DEdge e;
+                for (int q = 0; q < 3; q++) {
+                  e.p[q] = pts[i][q];
+                  e.q[q] = nx[q];
+                }
+                e.plane = jobIdx;
+                e.pad = 0;
+                edges[eOut[s] + eBase] = e;
+              }
+              eBase++;
+              acc_det_anchor(flux[s], cP, nx, pts[i], o);
+            }
+          }
         }
       }
+      for (int s = 0; s < 2; s++) {
+        runT[s] += (uint32_t)(tot[s] & 0x1fffff);
+        runN[s] += (uint32_t)((tot[s] >> 21) & 0x1fffff);
+        runE[s] += (uint32_t)(tot[s] >> 42);
+      }
     }
-    __syncthreads();
-    if (tid == 0) {
-      s_run[4] += tot[0];
-      s_run[5] += tot[1];
+    // 3. inherited open edges of earlier planes, clipped by this one
+    for (uint32_t base = 0; base < part.ne; base += BT * EPT) {
+      const uint32_t e0 = base + tid * EPT;
+      uint32_t c0 = 0, c1 = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < EPT; k++) {
+        const uint32_t e = e0 + k;
+        if (e < part.ne) {
+          const DEdge& ed = E[e];
+          const bool sp = plane_eval(pl, ed.p[0], ed.p[1], ed.p[2]) >= 0.0f;
+          const bool sq = plane_eval(pl, ed.q[0], ed.q[1], ed.q[2]) >= 0.0f;
+          c0 += (sp || sq) ? 1u : 0u;
+          c1 += (!sp || !sq) ? 1u : 0u;
+        }
+      }
+      uint32_t p0, p1, t0, t1;
+      block_excl_scan2(c0, c1, shs, &p0, &p1, &t0, &t1);
+      uint32_t bs[2] = {runE[0] + p0, runE[1] + p1};
+      for (uint32_t k = 0; k < EPT; k++) {
+        const uint32_t e = e0 + k;
+        if (e >= part.ne) break;
+        const DEdge ed = E[e];
+        const float dp = plane_eval(pl, ed.p[0], ed.p[1], ed.p[2]);
+        const float dq = plane_eval(pl, ed.q[0], ed.q[1], ed.q[2]);
+        const float4 q4 = jobPlanes[ed.plane];
+        const float epl[4] = {q4.x, q4.y, q4.z, q4.w};
+        float cE[3];
+        plane_point(epl, cE);
+        const bool sp = dp >= 0.0f, sq = dq >= 0.0f;
+        auto put = [&](int s, const float* a, const float* b) {
+          if (keepTopo) {
+            DEdge ne;
+            for (int q = 0; q < 3; q++) {
+              ne.p[q] = a[q];
+              ne.q[q] = b[q];
+            }
+            ne.plane = ed.plane;
+            ne.pad = 0;
+            edges[eOut[s] + bs[s]] = ne;
+          }
+          bs[s]++;
+          acc_det_anchor(flux[s], cE, b, a, o);
+        };
+        if (sp == sq) {
+          put(sp ? 0 : 1, ed.p, ed.q);
+        } else {
+          const float tt = dp / (dp - dq);
+          float x[3];
+          for (int q = 0; q < 3; q++) x[q] = ed.p[q] + tt * (ed.q[q] - ed.p[q]);
+          if (sp) {
+            put(0, ed.p, x);
+            put(1, x, ed.q);
+          } else {
+            put(1, ed.p, x);
+            put(0, x, ed.q);
+          }
+        }
+      }
+      runE[0] += t0;
+      runE[1] += t1;
     }
-    __syncthreads();
   }
-  const i128 fp = block_sum_i128(flux[0], shw);
-  const i128 fn = block_sum_i128(flux[1], shw);
+  const i128 fp = block_sum_i128(flux[0].value(), shw);
+  const i128 fn = block_sum_i128(flux[1].value(), shw);
   if (tid == 0) {
     parts[P.partBase + 2 * g].meshVol = fix_to_f64(fix_abs(fp)) / 6.0;
     parts[P.partBase + 2 * g + 1].meshVol = fix_to_f64(fix_abs(fn)) / 6.0;

@@ -125,6 +125,8 @@ struct KernelStats {
   uint64_t emitBytes = 0;   // emit: parent read + children written
   uint64_t hdEvals = 0;     // Hausdorff: point-triangle evaluations
   uint64_t hullJobs = 0;
+  uint64_t hullBytes = 0;     // wave-per-part hull: algorithmic HBM bytes (12 nv + 16 + 12 per exported face)
+  uint64_t hullBytesBig = 0;  // workgroup-per-part hull: the same
   void add(const KernelStats& o) {
     for (int k = 0; k < K_COUNT; k++) {
       ms[k] += o.ms[k];
@@ -135,6 +137,8 @@ struct KernelStats {
     emitBytes += o.emitBytes;
     hdEvals += o.hdEvals;
     hullJobs += o.hullJobs;
+    hullBytes += o.hullBytes;
+    hullBytesBig += o.hullBytesBig;
   }
 };
 

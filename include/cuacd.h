@@ -176,6 +176,8 @@ typedef struct cuacd_kernel_stats {
   uint64_t hd_evals;   /* Hausdorff: point-triangle evaluations */
   uint64_t hull_jobs;
   uint64_t face_tests_big; /* the workgroup-per-part hull kernel's share */
+  uint64_t hull_bytes;     /* wave-per-part hull: algorithmic HBM bytes, 12 nv + 16 (+ 12 per exported face) */
+  uint64_t hull_bytes_big; /* workgroup-per-part hull: the same */
 } cuacd_kernel_stats;
 /* accumulated over the life of the handle */
 int cuacd_tree_kernel_stats(cuacd_tree* t, cuacd_kernel_stats* s);
