@@ -16,7 +16,7 @@ HOST_SRCS = ["mesh.cpp", "cut.cpp", "concave.cpp", "sampling.cpp", "decompose.cp
 HIP_SRCS = ["gpu_backend.hip"]
 
 # -ffp-contract=off everywhere: every float expression rounds exactly as written.
-CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fvisibility=hidden", "-Wall",
+CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-pthread", "-ffp-contract=off", "-fno-fast-math", "-fvisibility=hidden", "-Wall",
             "-Wno-unused-function"]
 HIPFLAGS = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fvisibility=hidden",
             "--offload-arch=gfx950", "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
@@ -48,7 +48,7 @@ def build(jobs=8, verbose=False):
                 sys.stderr.write(r.stderr)
     lib = os.path.join(OUT, "libcuacd.so")
     tmp = lib + ".tmp"
-    _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", tmp, *objs])
+    _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-Wl,-Bsymbolic", "-o", tmp, *objs])
     os.replace(tmp, lib)
     return lib
 

@@ -125,6 +125,8 @@ CUACD_EXPORT int cuacd_generate(uint32_t kind, uint64_t seed, const double* p, u
       m->m.push_back(cuacd::genBlobs(b, seed));
     } else if (kind == 2) {
       m->m.push_back(cuacd::genTwistedTorus(P(0, 0.3), P(1, 0.1), P(2, 0.0), (uint32_t)P(3, 64), seed));
+    } else if (kind == 3) {
+      m->m.push_back(cuacd::genMixMesh((uint32_t)P(0, 0), seed));
     } else {
       return CUACD_ERR_ARG;
     }

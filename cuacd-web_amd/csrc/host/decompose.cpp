@@ -3,6 +3,11 @@
 #include "decompose.hpp"
 
 #include <algorithm>
+#include <cstdlib>
+#include <exception>
+#include <mutex>
+#include <atomic>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -337,48 +342,98 @@ Decomposer::Decomposer(Backend& be, const DecomposeOptions& o) : be_(be), o_(o) 
   if (o.width % 3 != 0 || o.width2 % 3 != 0) throw std::invalid_argument("width and width2 must be multiples of 3");
 }
 
-void Decomposer::fillHausdorff(std::vector<Part>& parts) {
-  std::vector<size_t> todo;
-  for (size_t i = 0; i < parts.size(); i++)
-    if (!parts[i].hasHausdorff) todo.push_back(i);
-  const size_t batch = 64;
-  for (size_t s = 0; s < todo.size(); s += batch) {
-    const size_t e = std::min(todo.size(), s + batch);
-    std::vector<MeshRef> a, b;
-    for (size_t k = s; k < e; k++) {
-      a.push_back(refOf(parts[todo[k]].mesh));
-      b.push_back(refOf(parts[todo[k]].hull));
+unsigned hostThreads() {
+  unsigned n = std::thread::hardware_concurrency();
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const int v = std::atoi(e);
+    if (v > 0) n = (unsigned)v;
+  }
+  return std::max(1u, std::min(n, 64u));
+}
+
+namespace {
+// f(i) for i in [0, n) on the host worker threads; the first exception is rethrown
+template <typename F>
+void parallelFor(size_t n, F f) {
+  const unsigned nt = (unsigned)std::min<size_t>(hostThreads(), n);
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::exception_ptr err;
+  std::mutex mu;
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; t++)
+    th.emplace_back([&] {
+      for (size_t i; (i = next++) < n;) {
+        try {
+          f(i);
+        } catch (...) {
+          std::lock_guard<std::mutex> g(mu);
+          if (!err) err = std::current_exception();
+        }
+      }
+    });
+  for (std::thread& x : th) x.join();
+  if (err) std::rethrow_exception(err);
+}
+}  // namespace
+
+// one mesh of a batch: its parts, result and uid counter
+struct Decomposer::State {
+  DecomposeResult R;
+  std::vector<Part> parts;
+  uint64_t nextUid = 1;
+  bool active = true;
+};
+
+void Decomposer::fillHausdorff(std::vector<State>& S) {
+  std::vector<std::pair<size_t, size_t>> todo;
+  for (size_t s = 0; s < S.size(); s++)
+    for (size_t i = 0; i < S[s].parts.size(); i++)
+      if (!S[s].parts[i].hasHausdorff) todo.push_back({s, i});
+  const size_t batch = 256;
+  for (size_t b = 0; b < todo.size(); b += batch) {
+    const size_t e = std::min(todo.size(), b + batch);
+    std::vector<MeshRef> a, c;
+    for (size_t k = b; k < e; k++) {
+      const Part& p = S[todo[k].first].parts[todo[k].second];
+      a.push_back(refOf(p.mesh));
+      c.push_back(refOf(p.hull));
     }
-    const std::vector<double> hd = be_.hausdorff(a, b);
-    for (size_t k = s; k < e; k++) {
-      parts[todo[k]].hausdorff = hd[k - s];
-      parts[todo[k]].hasHausdorff = std::isfinite(hd[k - s]);
+    const std::vector<double> hd = be_.hausdorff(a, c);
+    for (size_t k = b; k < e; k++) {
+      Part& p = S[todo[k].first].parts[todo[k].second];
+      p.hausdorff = hd[k - b];
+      p.hasHausdorff = std::isfinite(hd[k - b]);
     }
   }
 }
 
-std::vector<Part> Decomposer::makeParts(const std::vector<Mesh>& meshes) {
-  std::vector<Part> out;
+std::vector<Part> Decomposer::makeParts(const std::vector<const Mesh*>& meshes) {
+  std::vector<Part> out(meshes.size());
   const size_t batch = 256;
   for (size_t s = 0; s < meshes.size(); s += batch) {
     const size_t e = std::min(meshes.size(), s + batch);
     std::vector<MeshRef> refs;
-    for (size_t k = s; k < e; k++) refs.push_back(refOf(meshes[k]));
+    for (size_t k = s; k < e; k++) refs.push_back(refOf(*meshes[k]));
     std::vector<double> hd;
     const std::vector<HullResult> hulls = be_.hullsWithHausdorff(refs, &hd);
-    for (size_t k = s; k < e; k++) {
-      const HullResult& h = hulls[k - s];
-      if (!h.ok) throw std::runtime_error("hull kernel failed for a part (code " + std::to_string(h.failCode) + ")");
-      Part p;
-      p.mesh = meshes[k];
-      p.hull = compactHull(meshes[k].verts.data(), h.tris, nullptr);
-      p.meshVol = volume(meshes[k]);
+    for (size_t k = s; k < e; k++)
+      if (!hulls[k - s].ok)
+        throw std::runtime_error("hull kernel failed for a part (code " + std::to_string(hulls[k - s].failCode) + ")");
+    parallelFor(e - s, [&](size_t q) {
+      const size_t k = s + q;
+      const HullResult& h = hulls[q];
+      Part& p = out[k];
+      p.mesh = *meshes[k];
+      p.hull = compactHull(meshes[k]->verts.data(), h.tris, nullptr);
+      p.meshVol = volume(*meshes[k]);
       p.hullVol = h.volume;
-      p.hasHausdorff = std::isfinite(hd[k - s]);
-      p.hausdorff = p.hasHausdorff ? hd[k - s] : 0;
-      p.uid = nextUid_++;
-      out.push_back(std::move(p));
-    }
+      p.hasHausdorff = std::isfinite(hd[q]);
+      p.hausdorff = p.hasHausdorff ? hd[q] : 0;
+    });
   }
   return out;
 }
@@ -399,66 +454,94 @@ std::vector<float> Decomposer::refineSpacing(const std::vector<const Mesh*>& mes
   return out;
 }
 
-DecomposeResult Decomposer::decompose(const Mesh& input) {
+DecomposeResult Decomposer::decompose(const Mesh& input) { return std::move(decomposeBatch({&input})[0]); }
+
+std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const Mesh*>& inputs) {
   const DecomposeOptions& o = o_;
-  DecomposeResult R;
   const double tStart = nowMs();
-  if (o.requireClosed && !isClosed(input))
-    throw std::invalid_argument(
-        "input mesh is not closed (open or non-manifold edges); repair or remesh it first, or set requireClosed: false");
-  Mesh mesh = input;
-  if (o.normalize) {
-    mesh = normalize(input, &R.norm);
-    R.normalized = true;
+  std::vector<State> S(inputs.size());
+  DecomposeStats st;  // the batch's counters and times
+  std::vector<Mesh> starts(inputs.size());
+  for (size_t m = 0; m < inputs.size(); m++) {
+    if (o.requireClosed && !isClosed(*inputs[m]))
+      throw std::invalid_argument(
+          "input mesh is not closed (open or non-manifold edges); repair or remesh it first, or set requireClosed: false");
+    starts[m] = *inputs[m];
+    if (o.normalize) {
+      starts[m] = normalize(*inputs[m], &S[m].R.norm);
+      S[m].R.normalized = true;
+    }
   }
-  std::vector<Part> parts = makeParts({mesh});
-  DecomposeStats& st = R.stats;
+  {
+    std::vector<const Mesh*> ms;
+    for (const Mesh& m : starts) ms.push_back(&m);
+    std::vector<Part> made = makeParts(ms);
+    for (size_t m = 0; m < S.size(); m++) {
+      made[m].uid = S[m].nextUid++;
+      S[m].parts.push_back(std::move(made[m]));
+    }
+  }
+  // a seed of the combined tree run: its mesh and part
+  struct Seed {
+    uint32_t m, pi;
+    std::vector<float> edge;
+    SeedResult res;
+  };
   for (uint32_t iter = 0; iter < o.maxIters; iter++) {
-    R.iterations = iter + 1;
     const double tc0 = nowMs();
-    fillHausdorff(parts);
+    fillHausdorff(S);
     st.hausdorffMs += nowMs() - tc0;
-    // candidates ascending by full cost; the last maxNCutting above the threshold
-    struct OC {
-      uint32_t i;
-      double c;
-    };
-    std::vector<OC> order;
-    for (uint32_t i = 0; i < parts.size(); i++) order.push_back({i, partCost(parts[i])});
-    std::stable_sort(order.begin(), order.end(), [](const OC& a, const OC& b) { return a.c < b.c; });
-    std::vector<uint32_t> above;
-    for (const OC& e : order)
-      if (e.c >= o.threshold && !parts[e.i].stuck) above.push_back(e.i);
-    std::vector<uint32_t> cutting(above.begin() + (above.size() > o.maxNCutting ? above.size() - o.maxNCutting : 0),
-                                  above.end());
-    const double worst = order.empty() ? 0 : order.back().c;
-    if (cutting.empty()) break;
+    std::vector<Seed> seeds;
+    std::vector<double> worst(S.size(), 0.0);
+    for (uint32_t m = 0; m < S.size(); m++) {
+      State& s = S[m];
+      if (!s.active) continue;
+      s.R.iterations = iter + 1;
+      // candidates ascending by full cost; the last maxNCutting above the threshold
+      struct OC {
+        uint32_t i;
+        double c;
+      };
+      std::vector<OC> order;
+      for (uint32_t i = 0; i < s.parts.size(); i++) order.push_back({i, partCost(s.parts[i])});
+      std::stable_sort(order.begin(), order.end(), [](const OC& a, const OC& b) { return a.c < b.c; });
+      std::vector<uint32_t> above;
+      for (const OC& e : order)
+        if (e.c >= o.threshold && !s.parts[e.i].stuck) above.push_back(e.i);
+      const size_t skip = above.size() > o.maxNCutting ? above.size() - o.maxNCutting : 0;
+      worst[m] = order.empty() ? 0 : order.back().c;
+      if (above.size() == skip) {  // nothing left to cut: this mesh is done (iterations counts this one)
+        s.active = false;
+        continue;
+      }
+      for (size_t k = skip; k < above.size(); k++) seeds.push_back(Seed{m, above[k], {}, {}});
+    }
+    if (seeds.empty()) break;
     const double tcc = nowMs();
-    std::vector<std::vector<float>> edgePlanes;
-    for (uint32_t pi : cutting) {
-      if (iter < o.concaveIters && o.nConcaveEdges > 0) {
+    if (iter < o.concaveIters && o.nConcaveEdges > 0) {
+      parallelFor(seeds.size(), [&](size_t k) {
         ConcaveOptions co;
         co.nConcaveEdges = o.nConcaveEdges;
         co.concaveEps = o.concaveEps;
         co.concaveThreshold = o.concaveThreshold;
-        co.seed = iter * 7919u + pi;
-        edgePlanes.push_back(concaveEdgePlanes(parts[pi].mesh, co));
-      } else {
-        edgePlanes.emplace_back();
-      }
+        co.seed = iter * 7919u + seeds[k].pi;
+        seeds[k].edge = concaveEdgePlanes(S[seeds[k].m].parts[seeds[k].pi].mesh, co);
+      });
     }
     st.concaveMs += nowMs() - tcc;
-    // tree search, halving the batch on pool overflow
-    std::vector<SeedResult> results(cutting.size());
-    size_t batchN = cutting.size(), start = 0;
-    while (start < cutting.size()) {
-      const size_t e = std::min(cutting.size(), start + batchN);
+    // tree search over every mesh's seeds, halving the batch on pool overflow
+    size_t batchN = treeBatch_ ? std::min(treeBatch_, seeds.size()) : seeds.size(), start = 0;
+    while (start < seeds.size()) {
+      const size_t e = std::min(seeds.size(), start + batchN);
       const double tt0 = nowMs();
-      std::vector<MeshRef> seeds;
+      std::vector<MeshRef> refs;
       std::vector<const Mesh*> seedMeshes;
+      std::vector<std::vector<float>> ep;
       for (size_t k = start; k < e; k++) {
-        seeds.push_back(refOf(parts[cutting[k]].mesh));
-        seedMeshes.push_back(&parts[cutting[k]].mesh);
+        const Mesh& pm = S[seeds[k].m].parts[seeds[k].pi].mesh;
+        refs.push_back(refOf(pm));
+        seedMeshes.push_back(&pm);
+        ep.push_back(seeds[k].edge);
       }
       const uint32_t nProbes = std::min(o.refineProbes, be_.planeSlots());
       RefineRequest rr;
@@ -469,42 +552,47 @@ DecomposeResult Decomposer::decompose(const Mesh& input) {
         rr.spacing = refineSpacing(seedMeshes);
         ro.refine = &rr;
       }
-      std::vector<std::vector<float>> ep(edgePlanes.begin() + start, edgePlanes.begin() + e);
-      const TreeRunResult res = be_.run(seeds, ep, ro);
+      const TreeRunResult res = be_.run(refs, ep, ro);
       st.treeMs += nowMs() - tt0;
       st.treeRuns++;
       st.treeGpuMs += res.gpuMs;
       st.candidateHulls += res.hullsDone;
       st.pruned += res.pruned;
       if (o.verbose >= 2)
-        std::fprintf(stderr, "[iter %u] tree run: %zu seeds, hulls %llu, pruned %u, %.1f ms\n", iter, seeds.size(),
+        std::fprintf(stderr, "[iter %u] tree run: %zu seeds, hulls %llu, pruned %u, %.1f ms\n", iter, refs.size(),
                      (unsigned long long)res.hullsDone, res.pruned, nowMs() - tt0);
       if (res.overflow) {
         if (batchN == 1) throw std::runtime_error("GPU pool overflow with a single part; increase tree pool sizes");
         batchN = std::max<size_t>(1, batchN >> 1);
+        treeBatch_ = batchN;
         continue;
       }
       if (res.npOverflow) throw std::runtime_error("tree depth exceeds LA_MAX_PARTS");
       if (res.hullFail) st.hullFailRuns++;
-      for (size_t k = start; k < e; k++) results[k] = res.seeds[k - start];
+      for (size_t k = start; k < e; k++) seeds[k].res = res.seeds[k - start];
       start = e;
     }
-    // apply the winning cuts on the host, then hull all new halves in one batch
+    // apply the winning cuts on the host (worker threads), then hull all new halves in shared batches
     const double ta0 = nowMs();
-    std::vector<Mesh> halves;
-    std::vector<uint32_t> replaced;
-    for (size_t k = 0; k < cutting.size(); k++) {
-      const uint32_t pi = cutting[k];
-      Part& p = parts[pi];
-      const SeedResult& sr = results[k];
-      CutRecord rec{iter, pi, sr.bestCut, sr.bestCost, {sr.plane[0], sr.plane[1], sr.plane[2], sr.plane[3]}, false};
+    std::vector<CutResult> cuts(seeds.size());
+    parallelFor(seeds.size(), [&](size_t k) {
+      if (seeds[k].res.valid) cuts[k] = planeCut(S[seeds[k].m].parts[seeds[k].pi].mesh, seeds[k].res.plane);
+    });
+    std::vector<const Mesh*> halves;
+    std::vector<size_t> applied;  // seed index per applied cut
+    for (size_t k = 0; k < seeds.size(); k++) {
+      State& s = S[seeds[k].m];
+      Part& p = s.parts[seeds[k].pi];
+      const SeedResult& sr = seeds[k].res;
+      CutRecord rec{iter, seeds[k].pi, sr.bestCut, sr.bestCost, {sr.plane[0], sr.plane[1], sr.plane[2], sr.plane[3]},
+                    false};
       if (!sr.valid) {
         p.stuck = true;
         p.stuckReason = "no-plane";
-        R.cuts.push_back(rec);
+        s.R.cuts.push_back(rec);
         continue;
       }
-      CutResult r = planeCut(p.mesh, sr.plane);
+      const CutResult& r = cuts[k];
       if (!r.ok) {
         p.stuck = true;
         p.stuckReason = "cut-failed";
@@ -516,47 +604,62 @@ DecomposeResult Decomposer::decompose(const Mesh& input) {
         p.stuckReason = "not-closed";
       } else {
         rec.applied = true;
-        replaced.push_back(pi);
-        halves.push_back(std::move(r.pos));
-        halves.push_back(std::move(r.neg));
+        applied.push_back(k);
+        halves.push_back(&r.pos);
+        halves.push_back(&r.neg);
       }
-      R.cuts.push_back(rec);
+      s.R.cuts.push_back(rec);
     }
     st.cutMs += nowMs() - ta0;
     const double th0 = nowMs();
     std::vector<Part> made = makeParts(halves);
     st.hullMs += nowMs() - th0;
-    for (size_t k = 0; k < replaced.size(); k++) {
-      parts[replaced[k]] = std::move(made[2 * k]);
-      parts.push_back(std::move(made[2 * k + 1]));
+    // per mesh, in seed order: the positive half replaces the part, the negative half is appended
+    for (size_t q = 0; q < applied.size(); q++) {
+      State& s = S[seeds[applied[q]].m];
+      made[2 * q].uid = s.nextUid++;
+      made[2 * q + 1].uid = s.nextUid++;
+      s.parts[seeds[applied[q]].pi] = std::move(made[2 * q]);
+      s.parts.push_back(std::move(made[2 * q + 1]));
     }
     if (o.verbose)
-      std::fprintf(stderr, "[iter %u] parts=%zu cutting=%zu worst=%.4f\n", iter, parts.size(), cutting.size(), worst);
+      for (uint32_t m = 0; m < S.size(); m++)
+        if (S[m].active)
+          std::fprintf(stderr, "[iter %u] mesh %u parts=%zu worst=%.4f\n", iter, m, S[m].parts.size(), worst[m]);
   }
-  fillHausdorff(parts);
-  if (o.mergeHulls && parts.size() > 1) {
-    const double tm0 = nowMs();
-    MergeOptions mo;
-    mo.threshold = o.threshold;
-    mo.maxRounds = o.mergeRounds;
-    mo.planeTol = o.mergePlaneTol;
-    mo.verbose = o.verbose;
-    parts = mergeHulls(be_, parts, mo, &R.merge);
-    R.merged = true;
-    st.mergeMs = nowMs() - tm0;
-  }
-  if (R.normalized) {
-    for (Part& p : parts) {
-      p.mesh = denormalize(p.mesh, R.norm);
-      p.hull = denormalize(p.hull, R.norm);
-      for (Plane& q : p.seams)
-        q[3] = q[3] * R.norm.scale - (q[0] * R.norm.center[0] + q[1] * R.norm.center[1] + q[2] * R.norm.center[2]);
+  fillHausdorff(S);
+  for (State& s : S) {
+    if (o.mergeHulls && s.parts.size() > 1) {
+      const double tm0 = nowMs();
+      MergeOptions mo;
+      mo.threshold = o.threshold;
+      mo.maxRounds = o.mergeRounds;
+      mo.planeTol = o.mergePlaneTol;
+      mo.verbose = o.verbose;
+      s.parts = mergeHulls(be_, s.parts, mo, &s.R.merge);
+      s.R.merged = true;
+      st.mergeMs += nowMs() - tm0;
     }
+    if (s.R.normalized) {
+      const Normalization& nz = s.R.norm;
+      for (Part& p : s.parts) {
+        p.mesh = denormalize(p.mesh, nz);
+        p.hull = denormalize(p.hull, nz);
+        for (Plane& q : p.seams)
+          q[3] = q[3] * nz.scale - (q[0] * nz.center[0] + q[1] * nz.center[1] + q[2] * nz.center[2]);
+      }
+    }
+    s.R.parts = std::move(s.parts);
   }
-  R.parts = std::move(parts);
   st.totalMs = nowMs() - tStart;
-  R.kernels = be_.kernelStats();
-  return R;
+  const KernelStats ks = be_.kernelStats();
+  std::vector<DecomposeResult> out;
+  for (State& s : S) {
+    s.R.stats = st;
+    s.R.kernels = ks;
+    out.push_back(std::move(s.R));
+  }
+  return out;
 }
 
 }  // namespace cuacd

@@ -128,14 +128,24 @@ class Decomposer {
  public:
   Decomposer(Backend& be, const DecomposeOptions& o);
   DecomposeResult decompose(const Mesh& input);
+  // Several meshes through the same iterations. The GPU work of all meshes shares
+  // launches (one tree run per iteration over every mesh's seeds, shared hull and
+  // Hausdorff batches); host cuts run on worker threads. Parts, cuts and merges of
+  // each result equal decompose() of that mesh alone; stats and kernel times are the
+  // batch's. decompose(m) is the batch of one.
+  std::vector<DecomposeResult> decomposeBatch(const std::vector<const Mesh*>& inputs);
 
  private:
-  void fillHausdorff(std::vector<Part>& parts);
-  std::vector<Part> makeParts(const std::vector<Mesh>& meshes);
+  struct State;
+  void fillHausdorff(std::vector<State>& S);
+  std::vector<Part> makeParts(const std::vector<const Mesh*>& meshes);
   std::vector<float> refineSpacing(const std::vector<const Mesh*>& meshes) const;
   Backend& be_;
   DecomposeOptions o_;
-  uint64_t nextUid_ = 1;
+  size_t treeBatch_ = 0;  // largest seed count that last fit the tree pools (0: unknown)
 };
+
+// number of host worker threads for per-part host work (cuts, concave planes)
+unsigned hostThreads();
 
 }  // namespace cuacd

@@ -1,6 +1,8 @@
 // gen.cpp — seeded synthetic closed meshes (see gen.hpp).
 #include "gen.hpp"
 
+#include <map>
+
 #include <algorithm>
 #include <cmath>
 #include <functional>
@@ -235,6 +237,63 @@ Mesh genBlobs(const BlobParams& p, uint64_t seed) {
   Mesh m = marchTets(f, p.grid, p.cmin - pad, p.cmax + pad, p.iso + 1e-7);
   normalizeUnitCube(m);
   return m;
+}
+
+Mesh subdivide(const Mesh& m, uint32_t levels) {
+  Mesh cur = m;
+  for (uint32_t l = 0; l < levels; l++) {
+    Mesh nx;
+    nx.verts = cur.verts;
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> mid;
+    auto midpoint = [&](uint32_t a, uint32_t b) {
+      const std::pair<uint32_t, uint32_t> k(std::min(a, b), std::max(a, b));
+      auto it = mid.find(k);
+      if (it != mid.end()) return it->second;
+      const uint32_t idx = (uint32_t)(nx.verts.size() / 3);
+      for (int c = 0; c < 3; c++) nx.verts.push_back(0.5f * (cur.verts[3 * a + c] + cur.verts[3 * b + c]));
+      mid.emplace(k, idx);
+      return idx;
+    };
+    for (size_t t = 0; t + 2 < cur.tris.size(); t += 3) {
+      const uint32_t a = cur.tris[t], b = cur.tris[t + 1], c = cur.tris[t + 2];
+      const uint32_t ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+      const uint32_t q[12] = {a, ab, ca, ab, b, bc, ca, bc, c, ab, bc, ca};
+      nx.tris.insert(nx.tris.end(), q, q + 12);
+    }
+    cur = std::move(nx);
+  }
+  return cur;
+}
+
+Mesh genMixMesh(uint32_t k, uint64_t seedBase) {
+  SplitMix64 rng(seedBase + k);
+  const double target = std::exp(rng.uniform(std::log(500.0), std::log(60000.0)));
+  const uint64_t seed = rng.next();
+  if (k % 3 == 1) {
+    const uint32_t steps = 3 + (uint32_t)(rng.uniform() * 10.0);
+    Mesh m = genStaircase(std::min(steps, 12u), 1.0, 0.1, 0.4, seed);
+    normalizeUnitCube(m);
+    const double nt = (double)(m.tris.size() / 3);
+    const double lv = std::log(std::max(target / nt, 1.0)) / std::log(4.0);
+    return subdivide(m, (uint32_t)std::floor(lv + 0.5));
+  }
+  // grid scaled from a coarse calibration mesh: triangles grow with the grid squared
+  auto make = [&](uint32_t grid) {
+    if (k % 3 == 0) {
+      BlobParams b;
+      b.nBlobs = 4 + (uint32_t)(SplitMix64(seed).uniform() * 13.0);
+      b.grid = grid;
+      return genBlobs(b, seed);
+    }
+    SplitMix64 r2(seed);
+    const double twist = r2.uniform(0.0, 3.14159265358979323846);
+    const double tube = r2.uniform(0.05, 0.2);
+    return genTwistedTorus(0.3, tube, twist, grid, seed);
+  };
+  const uint32_t g0 = 24;
+  const double n0 = (double)(make(g0).tris.size() / 3);
+  const double g = g0 * std::sqrt(target / std::max(n0, 1.0));
+  return make((uint32_t)std::max(8.0, std::min(400.0, std::floor(g + 0.5))));
 }
 
 Mesh genTwistedTorus(double R, double r, double twist, uint32_t grid, uint64_t seed) {

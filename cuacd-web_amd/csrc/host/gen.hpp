@@ -49,4 +49,14 @@ Mesh genTwistedTorus(double R, double r, double twist, uint32_t grid, uint64_t s
 // uniform scale + shift into [0,1]^3 (largest extent 1)
 void normalizeUnitCube(Mesh& m);
 
+// every triangle split into four at shared edge midpoints, `levels` times (closed stays closed)
+Mesh subdivide(const Mesh& m, uint32_t levels);
+
+// mesh k of the uneven batch mix (seed seedBase + k): a triangle budget drawn
+// log-uniformly from [500, 60000], then by k mod 3 a blob surface (4-16 blobs),
+// a staircase (3-12 steps of U[0.1, 0.4]) subdivided to the budget, or a twisted
+// torus (twist U[0, pi], tube U[0.05, 0.2]); grids are scaled to the budget from a
+// coarse calibration mesh
+Mesh genMixMesh(uint32_t k, uint64_t seedBase);
+
 }  // namespace cuacd

@@ -139,6 +139,20 @@ def decompose(mesh, options=None, api=None, **overrides):
     return _read_result(api, h)
 
 
+def decompose_batch(meshes, options=None, api=None, **overrides):
+    """Several meshes decomposed together (shared tree runs and hull batches); element k
+    equals decompose(meshes[k]). Stats and kernel times are the batch's."""
+    api = api or load_product()
+    o = options or DecomposeOptions()
+    for k, v in overrides.items():
+        setattr(o, k, v)
+    raw = o.raw()
+    b = _batch(meshes)
+    hs = (C.c_void_p * max(len(meshes), 1))()
+    check(api, api.decompose_batch(C.byref(b), C.byref(raw), hs))
+    return [_read_result(api, C.c_void_p(hs[k])) for k in range(len(meshes))]
+
+
 def merge_hulls(meshes, options=None, api=None, **overrides):
     """mergeHulls (src/merge.ts:170) over parts made from `meshes` (hull + Hausdorff on the GPU)."""
     api = api or load_product()

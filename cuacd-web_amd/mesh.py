@@ -122,9 +122,9 @@ def parse_stl(data: bytes):
 def generate(kind, seed, params=()):
     """Seeded synthetic closed mesh in [0,1]^3: kind 'staircase' [steps, width, step_min, step_max],
     'blobs' [n_blobs, grid, cmin, cmax, smin, smax, iso, n_tori, torus_R, torus_r],
-    'torus' [R, r, twist, grid]."""
+    'torus' [R, r, twist, grid], 'mix' [index] (mesh `index` of the uneven batch mix, seed base `seed`)."""
     api = load_product()
-    k = {"staircase": 0, "blobs": 1, "torus": 2}[kind]
+    k = {"staircase": 0, "blobs": 1, "torus": 2, "mix": 3}[kind]
     p = (f64 * max(len(params), 1))(*[float(x) for x in params])
     h = C.c_void_p()
     check(api, api.generate(k, seed, p, len(params), C.byref(h)))

@@ -183,6 +183,15 @@ typedef struct cuacd_kernel_stats {
 int cuacd_tree_kernel_stats(cuacd_tree* t, cuacd_kernel_stats* s);
 
 void cuacd_decompose_options_default(cuacd_decompose_options* o);
+/* Several meshes decomposed together. Every iteration runs one tree search over the
+ * seeds of all meshes and shares the hull and Hausdorff batches; host cuts run on
+ * worker threads. results[k] (caller array of meshes->n) equals cuacd_decompose of
+ * mesh k: same cuts, parts and merges. Stats and kernel times of every result are the
+ * batch's. Free each result with cuacd_result_free. The reference decomposes one mesh
+ * per call (src/decompose.ts:407, Decomposer src/decompose.ts:128-398); this is the
+ * batched form of that driver. */
+int cuacd_decompose_batch(const cuacd_mesh_batch* meshes, const cuacd_decompose_options* o, cuacd_result** results);
+
 /* decompose(mesh, options) (src/decompose.ts:407) */
 int cuacd_decompose(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
                     const cuacd_decompose_options* o, cuacd_result** out);
@@ -215,7 +224,8 @@ int32_t cuacd_is_closed(const float* verts, uint32_t nv, const uint32_t* tris, u
 double cuacd_signed_volume(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt);
 /* binary STL, welded by exact position (src/mesh.ts:166-212) */
 int cuacd_parse_stl(const uint8_t* buf, uint64_t len, cuacd_meshes** out);
-/* synthetic meshes: kind 0 staircase (steps), 1 blobs (n_blobs, grid), 2 torus (major, minor,
+/* synthetic meshes: kind 3 = mesh params[0] of the uneven batch mix with seed base `seed`;
+ * kind 0 staircase (steps), 1 blobs (n_blobs, grid), 2 torus (major, minor,
  * twist), all closed, float32, normalized into the unit cube [0,1]^3 */
 int cuacd_generate(uint32_t kind, uint64_t seed, const double* params, uint32_t n_params, cuacd_meshes** out);
 
