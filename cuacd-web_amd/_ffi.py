@@ -71,8 +71,8 @@ class DecomposeStats(C.Structure):
                 ("norm_scale", f64), ("norm_center", f64 * 3)]
 
 
-NUM_KERNELS = 9
-KERNEL_NAMES = ["plan", "scan", "emit", "hull", "hull_big", "record", "evaluate", "hd_samples", "hd_dist"]
+NUM_KERNELS = 10
+KERNEL_NAMES = ["plan", "scan", "emit", "hull", "hull_big", "record", "evaluate", "hd_samples", "hd_dist", "volume"]
 
 
 class KernelStats(C.Structure):

@@ -101,6 +101,29 @@ __device__ __forceinline__ void acc_det(A& acc, const float* a, const float* b, 
   acc_triple(acc, a[2], b[0], c[1], neg);
   acc_triple(acc, a[2], b[1], c[0], !neg);
 }
+// det(a-o, b-o, c-o) as 4 x 6 triple products in rolled loops (operands chosen by
+// selects): the same integer sum as acc_det_anchor with few live registers
+__device__ __forceinline__ void acc_tet_rolled(Acc128& acc, const float* a, const float* b, const float* c,
+                                               const float* o) {
+#pragma unroll 1
+  for (int t = 0; t < 4; t++) {
+    const bool neg0 = t != 0;
+    float A[3], B[3], C[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      A[q] = t == 1 ? o[q] : a[q];
+      B[q] = t == 2 ? o[q] : b[q];
+      C[q] = t == 3 ? o[q] : c[q];
+    }
+#pragma unroll 1
+    for (int k = 0; k < 6; k++) {
+      const float x = k < 2 ? A[0] : (k < 4 ? A[1] : A[2]);
+      const float y = (k == 0 || k == 5) ? B[1] : ((k == 1 || k == 2) ? B[2] : B[0]);
+      const float z = (k == 0 || k == 3) ? C[2] : ((k == 1 || k == 4) ? C[1] : C[0]);
+      acc_triple(acc, x, y, z, (k & 1) ? !neg0 : neg0);
+    }
+  }
+}
 template <typename A>
 __device__ __forceinline__ void acc_det_anchor(A& acc, const float* a, const float* b, const float* c, const float* o) {
   acc_det(acc, a, b, c, false);

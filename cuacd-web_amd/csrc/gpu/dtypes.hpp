@@ -49,6 +49,7 @@ struct Counters {
   unsigned long long faceTestsBig;  // point-face record evaluations, workgroup-per-part hull kernel
   unsigned long long hullBytes;     // algorithmic HBM bytes, wave-per-part hull: 12 nv + 16 (+ 12 per exported face)
   unsigned long long hullBytesBig;  // the same for the workgroup-per-part hull kernel
+  unsigned long long tetTop;        // tetrahedra written by emit in the current level
   uint32_t hd[1024];  // Hausdorff results (f32 bits of the squared distance)
 };
 

@@ -153,6 +153,7 @@ class GpuBackend : public Backend {
 
   TreeConfig cfg_;
   uint32_t edgeStride_;
+  unsigned long long tetCap_ = 0;
   hipStream_t stream_ = nullptr;
   DevBuf<float> verts_;
   DevBuf<uint32_t> tris_;
@@ -163,6 +164,8 @@ class GpuBackend : public Backend {
   DevBuf<float4> jobPlanes_, edgePlanes_;
   DevBuf<dk::SeedState> state_;
   DevBuf<uint32_t> scratch_;
+  DevBuf<float4> tets_, jobAnchor_;  // emit's tetrahedron records (3 float4 each), anchor per job
+  DevBuf<unsigned long long> vacc_;  // volume limbs, 3 per child of a level
   DevBuf<dk::Counters> ctr_;
   DevBuf<uint32_t> arena_;
   DevBuf<dk::HullJob> hjobs_;
@@ -232,6 +235,9 @@ void GpuBackend::ensurePools() {
   tris_.ensure(std::min<uint64_t>(cfg_.triCap, 0xffffffffull));
   edges_.ensure(std::min<uint64_t>(cfg_.edgeCap, 0xffffffffull));
   scratch_.ensure(std::min<uint64_t>(cfg_.scratchWords, 0xffffffffull));
+  // tetrahedra of one level: at most the child triangles and cap edges it writes
+  tetCap_ = std::min<uint64_t>(cfg_.triCap / 3 + cfg_.edgeCap, 128ull << 20);
+  tets_.ensure(3 * tetCap_);
 }
 
 void GpuBackend::runHulls(std::vector<dk::HullJob>& hjIn, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
@@ -318,6 +324,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
   parts_.ensure(nSeeds + 2 * totalJobs);
   items_.ensure(nSeeds + totalJobs);
   jobs_.ensure(totalJobs);
+  jobAnchor_.ensure(totalJobs);
   jobPlanes_.ensure(totalJobs);
   edgePlanes_.ensure((size_t)nSeeds * edgeStride_);
   state_.ensure(nSeeds);
@@ -393,10 +400,20 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       launched("k_scan");
       tEnd(K_SCAN, e0);
       e0 = tBegin();
+      vacc_.ensure(6ull * L.nJobs);
+      dk::k_vol_clear<<<std::min<uint32_t>((6 * L.nJobs + 255) / 256, 1024), 256, 0, stream_>>>(vacc_.p, 6 * L.nJobs,
+                                                                                               ctr_.p);
+      launched("k_vol_clear");
       dk::k_emit<<<L.nJobs, dk::BT, 0, stream_>>>(P, parts_.p, jobs_.p, jobPlanes_.p, verts_.p, tris_.p, edges_.p,
-                                                  scratch_.p, &ctr_.p->emitBytes);
+                                                  scratch_.p, tets_.p, tetCap_, jobAnchor_.p, ctr_.p);
       launched("k_emit");
       tEnd(K_EMIT, e0);
+      e0 = tBegin();
+      dk::k_volume<<<2048, dk::VT, 0, stream_>>>(P, tets_.p, jobAnchor_.p, vacc_.p, ctr_.p);
+      launched("k_volume");
+      dk::k_volume_fin<<<(2 * L.nJobs + 255) / 256, 256, 0, stream_>>>(P, jobs_.p, vacc_.p, parts_.p);
+      launched("k_volume_fin");
+      tEnd(K_VOLUME, e0);
       // hull jobs for every child with enough points
       const uint32_t nChild = 2 * L.nJobs;
       hostParts.resize(nChild);

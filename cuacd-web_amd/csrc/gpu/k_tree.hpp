@@ -443,10 +443,23 @@ __device__ __forceinline__ void emit_counts(const int* sg, unsigned long long* s
 // chunk gives offsets in parent order; running totals stay in registers.
 constexpr uint32_t EPT = 1;
 
+// A child's volume is the sum over its tetrahedra (fan triangle or cap edge, with
+// the job's anchor) of det(a-o, b-o, c-o). Emit writes one 48-byte record per
+// tetrahedron (a, b, c and the child part); k_volume sums them exactly.
+__device__ __forceinline__ void put_tet(float4* __restrict__ tets, uint64_t i, const float* a, const float* b,
+                                        const float* c, uint32_t child) {
+  float4* r = tets + 3 * i;
+  r[0] = make_float4(a[0], a[1], a[2], __uint_as_float(child));
+  r[1] = make_float4(b[0], b[1], b[2], 0.0f);
+  r[2] = make_float4(c[0], c[1], c[2], 0.0f);
+}
+
 __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ parts, const DJob* __restrict__ jobs,
                                              const float4* __restrict__ jobPlanes, float* __restrict__ verts,
                                              uint32_t* __restrict__ tris, DEdge* __restrict__ edges,
-                                             uint32_t* __restrict__ scratch, unsigned long long* ctrEmitBytes) {
+                                             uint32_t* __restrict__ scratch, float4* __restrict__ tets,
+                                             unsigned long long tetCap, float4* __restrict__ jobAnchor,
+                                             Counters* __restrict__ ctr) {
   const uint32_t g = blockIdx.x;
   if (g >= P.nJobs) return;
   const uint32_t tid = threadIdx.x;
@@ -455,8 +468,8 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
   if (job.valid != 1) return;
   __shared__ uint32_t shs[32];
   __shared__ unsigned long long shl[32];
-  __shared__ unsigned long long shw[16];
   __shared__ float s_o[3], s_cp[3];
+  __shared__ unsigned long long s_tet;
   const DPart part = parts[job.parentPart];
   const float pl[4] = {job.plane[0], job.plane[1], job.plane[2], job.plane[3]};
   if (tid == 0) {
@@ -485,9 +498,24 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
       s_o[q] = o[q];
       s_cp[q] = cp[q];
     }
+    jobAnchor[jobIdx] = make_float4(o[0], o[1], o[2], 0.0f);
+    // tetrahedra of both children: fan triangles and cap edges of each side
+    const unsigned long long n = (unsigned long long)job.cnt[1] + job.cnt[2] + job.cnt[4] + job.cnt[5];
+    unsigned long long b = atomicAdd(&ctr->tetTop, n);
+    if (b + n > tetCap) {
+      atomicExch(&ctr->overflow, 1u);
+      b = ~0ull;
+    }
+    s_tet = b;
   }
   __syncthreads();
-  const float o[3] = {s_o[0], s_o[1], s_o[2]}, cP[3] = {s_cp[0], s_cp[1], s_cp[2]};
+  const float cP[3] = {s_cp[0], s_cp[1], s_cp[2]};
+  const unsigned long long tetBase = s_tet;
+  const bool tetOk = tetBase != ~0ull;
+  // first tetrahedron of each side; cap edges follow the fan triangles
+  const unsigned long long sideT[2] = {tetBase, tetBase + job.cnt[1] + job.cnt[2]};
+  const uint32_t sideE[2] = {job.cnt[1], job.cnt[4]};
+  const uint32_t childOf[2] = {P.partBase + 2 * g, P.partBase + 2 * g + 1};
   const float* V = verts + part.vOff;
   const uint32_t* T = tris + part.tOff;
   const DEdge* E = edges + part.eOff;
@@ -545,7 +573,6 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
     }
   }
   __syncthreads();  // every remap is in place before triangles read it
-  Acc128 flux[2];
   // 2. triangles: both clipped polygons, fans, on-plane open edges
   {
     unsigned long long run[2] = {0, 0};  // packed (tris | new verts << 21 | edges << 42), tris/verts/edges
@@ -627,8 +654,8 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
               dst[1] = idx[i];
               dst[2] = idx[i + 1];
             }
+            if (tetOk) put_tet(tets, sideT[s] + tBase, pts[0], pts[i], pts[i + 1], childOf[s]);
             tBase++;
-            acc_det_anchor(flux[s], pts[0], pts[i], pts[i + 1], o);
           }
 #pragma unroll
           for (uint32_t i = 0; i < 4; i++) {
@@ -649,8 +676,8 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
                 e.pad = 0;
                 edges[eOut[s] + eBase] = e;
               }
+              if (tetOk) put_tet(tets, sideT[s] + sideE[s] + eBase, cP, nx, pts[i], childOf[s]);
               eBase++;
-              acc_det_anchor(flux[s], cP, nx, pts[i], o);
             }
           }
         }
@@ -701,8 +728,8 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
             ne.pad = 0;
             edges[eOut[s] + bs[s]] = ne;
           }
+          if (tetOk) put_tet(tets, sideT[s] + sideE[s] + bs[s], cE, b, a, childOf[s]);
           bs[s]++;
-          acc_det_anchor(flux[s], cE, b, a, o);
         };
         if (sp == sq) {
           put(sp ? 0 : 1, ed.p, ed.q);
@@ -723,16 +750,75 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
       runE[1] += t1;
     }
   }
-  const i128 fp = block_sum_i128(flux[0].value(), shw);
-  const i128 fn = block_sum_i128(flux[1].value(), shw);
   if (tid == 0) {
-    parts[P.partBase + 2 * g].meshVol = fix_to_f64(fix_abs(fp)) / 6.0;
-    parts[P.partBase + 2 * g + 1].meshVol = fix_to_f64(fix_abs(fn)) / 6.0;
     // parent read once, both children written, job and part records
     unsigned long long b = 12ull * part.nv + 12ull * part.nt + 32ull * part.ne + 12ull * (job.cnt[0] + job.cnt[3]) + 96;
     if (keepTopo) b += 12ull * (job.cnt[1] + job.cnt[4]) + 32ull * (job.cnt[2] + job.cnt[5]);
-    atomicAdd(ctrEmitBytes, b);
+    atomicAdd(&ctr->emitBytes, b);
   }
+}
+
+// ---- exact child volumes from emit's tetrahedron records. Each 128-bit term is
+// split into 42-bit limbs (two unsigned, the top one signed); the limbs add with
+// 64-bit atomics without overflow for up to 2^21 terms per child, and integer
+// sums do not depend on order, so the total equals the oracle's sum.
+constexpr uint32_t VT = 256;
+__global__ void k_vol_clear(unsigned long long* __restrict__ acc, uint32_t n, Counters* __restrict__ ctr) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr->tetTop = 0;
+}
+
+__global__ __launch_bounds__(VT) void k_volume(LevelParams P, const float4* __restrict__ tets,
+                                               const float4* __restrict__ jobAnchor,
+                                               unsigned long long* __restrict__ acc, const Counters* __restrict__ ctr) {
+  const unsigned long long n = ctr->tetTop;
+  const unsigned long long stride = (unsigned long long)gridDim.x * VT;
+  for (unsigned long long base = (unsigned long long)blockIdx.x * VT; base < n; base += stride) {
+    const unsigned long long i = base + threadIdx.x;
+    uint32_t child = NONE;
+    Acc128 t;
+    if (i < n) {
+      const float4 ra = tets[3 * i], rb = tets[3 * i + 1], rc = tets[3 * i + 2];
+      child = __float_as_uint(ra.w);
+      const float4 o4 = jobAnchor[P.jobBase + ((child - P.partBase) >> 1)];
+      const float a[3] = {ra.x, ra.y, ra.z}, b[3] = {rb.x, rb.y, rb.z}, c[3] = {rc.x, rc.y, rc.z};
+      const float o[3] = {o4.x, o4.y, o4.z};
+      acc_tet_rolled(t, a, b, c, o);
+    }
+    const i128 v = t.value();
+    unsigned long long l0 = (unsigned long long)v & ((1ull << 42) - 1);
+    unsigned long long l1 = (unsigned long long)(v >> 42) & ((1ull << 42) - 1);
+    unsigned long long l2 = (unsigned long long)(long long)(v >> 84);
+    // records of one child are contiguous: one set of atomics per run of equal children in the wave
+    const uint32_t c0 = __shfl(child, 0, 64);
+    if (__all(child == c0)) {
+      for (int m = 32; m >= 1; m >>= 1) {
+        l0 += __shfl_xor(l0, m, 64);
+        l1 += __shfl_xor(l1, m, 64);
+        l2 += __shfl_xor(l2, m, 64);
+      }
+      if ((threadIdx.x & 63) == 0 && c0 != NONE) {
+        unsigned long long* a3 = acc + 3ull * (c0 - P.partBase);
+        atomicAdd(a3, l0);
+        atomicAdd(a3 + 1, l1);
+        atomicAdd(a3 + 2, l2);
+      }
+    } else if (child != NONE) {
+      unsigned long long* a3 = acc + 3ull * (child - P.partBase);
+      atomicAdd(a3, l0);
+      atomicAdd(a3 + 1, l1);
+      atomicAdd(a3 + 2, l2);
+    }
+  }
+}
+
+__global__ void k_volume_fin(LevelParams P, const DJob* __restrict__ jobs, const unsigned long long* __restrict__ acc,
+                             DPart* __restrict__ parts) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= 2 * P.nJobs || jobs[P.jobBase + c / 2].valid != 1) return;
+  const unsigned long long* a3 = acc + 3ull * c;
+  const i128 v = ((i128)(long long)a3[2] << 84) + ((i128)a3[1] << 42) + (i128)a3[0];
+  parts[P.partBase + c].meshVol = fix_to_f64(fix_abs(v)) / 6.0;
 }
 
 // ---------------------------------------------------------------------------

@@ -116,7 +116,19 @@ struct HullResult {
 
 // Device time and algorithmic work per kernel family, accumulated over the
 // life of a backend (GPU event timing on the launch stream).
-enum KernelId { K_PLAN = 0, K_SCAN, K_EMIT, K_HULL, K_HULL_BIG, K_RECORD, K_EVALUATE, K_HD_SAMPLES, K_HD_DIST, K_COUNT };
+enum KernelId {
+  K_PLAN = 0,
+  K_SCAN,
+  K_EMIT,
+  K_HULL,
+  K_HULL_BIG,
+  K_RECORD,
+  K_EVALUATE,
+  K_HD_SAMPLES,
+  K_HD_DIST,
+  K_VOLUME,
+  K_COUNT
+};
 struct KernelStats {
   double ms[K_COUNT] = {0};
   uint64_t launches[K_COUNT] = {0};

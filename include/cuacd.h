@@ -166,8 +166,8 @@ typedef struct cuacd_cut_record {
 
 /* device time (HIP events on the launch stream) and algorithmic work per kernel family:
  * 0 plan, 1 scan, 2 emit, 3 hull (wave per part), 4 hull (workgroup per part), 5 record,
- * 6 evaluate, 7 Hausdorff samples, 8 Hausdorff distances */
-#define CUACD_NUM_KERNELS 9
+ * 6 evaluate, 7 Hausdorff samples, 8 Hausdorff distances, 9 child volumes */
+#define CUACD_NUM_KERNELS 10
 typedef struct cuacd_kernel_stats {
   double ms[CUACD_NUM_KERNELS];
   uint64_t launches[CUACD_NUM_KERNELS];
