@@ -578,12 +578,13 @@ struct HullBatch {
   uint32_t lface[LCAP], la[LCAP], lb[LCAP], lc[LCAP];  // visible faces (any candidate)
   uint8_t lj[LCAP], lflag[LCAP];                       // candidate, horizon edge bits
   uint32_t lbase[LCAP];                                 // new faces of the same candidate before this entry
+  uint32_t pface[LCAP / 2], pmask[LCAP / 2], pund[LCAP / 2];  // faces the f32 filter left undecided
   unsigned long long hkey[HT];                          // directed edges (a << 32 | b); seed-hull faces in phase 2
   uint8_t hj[HT];
   unsigned long long wkey[NW][K];
   unsigned long long wB[NW];
   uint32_t wcnt[NW];
-  uint32_t nList, nOrph, conflict, cutA, m;
+  uint32_t nList, nOrph, conflict, cutA, m, nPend;
 };
 
 __device__ __forceinline__ uint32_t edge_hash(unsigned long long key, uint32_t HT) {
@@ -608,6 +609,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   if (t == 0) {
     S.nList = 0;
     S.nOrph = 0;
+    S.nPend = 0;
     S.conflict = K;
     S.cutA = K;
   }
@@ -689,32 +691,83 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   g.sync();
   // ---- visibility of every alive face from every candidate
   for (int pass = 0;; pass++) {
-    for (uint32_t f = t; f < C.nf; f += Grp::G) {
-      const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * f);
-      const uint4 w0 = r4[0], w1 = r4[1], w2 = r4[2], w3 = r4[3], w4 = r4[4];
-      if (!(w0.x & ALIVE)) continue;
-      uint32_t mask = 0;
-      for (uint32_t j = 0; j < m; j++) {
-        float det;
-        int s = face_filter(w1, w2, S.cq[j], &det);
-        if (s == 0) s = orient_w(w2, w3, w4, S.cq[j]) > 0 ? 1 : -1;
-        mask |= s > 0 ? (1u << j) : 0u;
-      }
-      C.tests += m;
+    // a face's visibility mask is final: record it, count conflicts, list it
+    auto finish = [&](uint32_t f, uint32_t mask, uint32_t a, uint32_t b, uint32_t c) {
       C.vmask[f] = mask;
-      if (mask) {
-        const uint32_t j0 = (uint32_t)__ffs(mask) - 1;
-        const uint32_t rest = mask & (mask - 1);
-        if (rest) atomicMin(&S.conflict, (uint32_t)__ffs(rest) - 1);
-        atomicAdd(&S.nvis[j0], 1u);
-        const uint32_t pos = atomicAdd(&S.nList, 1u);
-        if (pos < LCAP) {
-          S.lface[pos] = f;
-          S.la[pos] = w0.x & ~ALIVE;
-          S.lb[pos] = w0.y;
-          S.lc[pos] = w0.z;
-          S.lj[pos] = (uint8_t)j0;
+      if (!mask) return;
+      const uint32_t j0 = (uint32_t)__ffs(mask) - 1;
+      const uint32_t rest = mask & (mask - 1);
+      if (rest) atomicMin(&S.conflict, (uint32_t)__ffs(rest) - 1);
+      atomicAdd(&S.nvis[j0], 1u);
+      const uint32_t pos = atomicAdd(&S.nList, 1u);
+      if (pos < LCAP) {
+        S.lface[pos] = f;
+        S.la[pos] = a;
+        S.lb[pos] = b;
+        S.lc[pos] = c;
+        S.lj[pos] = (uint8_t)j0;
+      }
+    };
+    // faces in groups of VB per thread (every record of a group in flight at once),
+    // tested with the f32 filter; faces it leaves undecided for some candidate wait
+    // in a pending list and get their exact tests after the loop, one face per thread
+    constexpr uint32_t VB = 4;
+    for (uint32_t f0 = t; f0 < C.nf; f0 += VB * Grp::G) {
+      uint4 h0[VB], h1[VB], h2[VB];
+#pragma unroll
+      for (uint32_t b = 0; b < VB; b++) {
+        const uint32_t f = f0 + b * Grp::G;
+        if (f < C.nf) {
+          const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * f);
+          h0[b] = r4[0];
+          h1[b] = r4[1];
+          h2[b] = r4[2];
+        } else {
+          h0[b] = make_uint4(0u, 0u, 0u, 0u);
         }
+      }
+#pragma unroll
+      for (uint32_t b = 0; b < VB; b++) {
+        const uint32_t f = f0 + b * Grp::G;
+        if (!(h0[b].x & ALIVE)) continue;
+        uint32_t mask = 0, und = 0;
+        for (uint32_t j = 0; j < m; j++) {
+          float det;
+          const int sgn = face_filter(h1[b], h2[b], S.cq[j], &det);
+          mask |= sgn > 0 ? (1u << j) : 0u;
+          und |= sgn == 0 ? (1u << j) : 0u;
+        }
+        C.tests += m;
+        if (und) {
+          const uint32_t pp = atomicAdd(&S.nPend, 1u);
+          if (pp < LCAP / 2) {
+            S.pface[pp] = f;
+            S.pmask[pp] = mask;
+            S.pund[pp] = und;
+            continue;
+          }
+          const uint32_t* r = C.F + (size_t)FW * f;
+          for (uint32_t u = und; u; u &= u - 1) {
+            const uint32_t j = (uint32_t)__ffs(u) - 1;
+            if (orient_rec(r, h2[b], S.cq[j]) > 0) mask |= 1u << j;
+          }
+        }
+        finish(f, mask, h0[b].x & ~ALIVE, h0[b].y, h0[b].z);
+      }
+    }
+    g.sync();
+    {
+      const uint32_t np = S.nPend < LCAP / 2 ? S.nPend : LCAP / 2;
+      for (uint32_t p = t; p < np; p += Grp::G) {
+        const uint32_t f = S.pface[p];
+        const uint32_t* r = C.F + (size_t)FW * f;
+        const uint4 w0 = reinterpret_cast<const uint4*>(r)[0], w2 = reinterpret_cast<const uint4*>(r)[2];
+        uint32_t mask = S.pmask[p];
+        for (uint32_t u = S.pund[p]; u; u &= u - 1) {
+          const uint32_t j = (uint32_t)__ffs(u) - 1;
+          if (orient_rec(r, w2, S.cq[j]) > 0) mask |= 1u << j;
+        }
+        finish(f, mask, w0.x & ~ALIVE, w0.y, w0.z);
       }
     }
     g.sync();
@@ -736,6 +789,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     if (t < K) S.nvis[t] = 0;
     if (t == 0) {
       S.nList = 0;
+      S.nPend = 0;
       S.conflict = K;
     }
     g.sync();
@@ -853,18 +907,31 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   // ---- reassignment of the points whose owner is visible, over the new faces
   // of that candidate (the candidate itself lies on them and finds none).
   // Collected first, then evaluated one point per thread.
-  for (uint32_t e = t; e < C.ncand; e += Grp::G) {
-    const uint32_t o = C.cown[e];
-    const uint4 ce = C.cand[e];
-    if (o == NONE) continue;
-    const uint32_t mask = C.vmask[o];
-    if (!mask) continue;
-    const uint32_t j = (uint32_t)__ffs(mask) - 1;
-    if (j >= m) continue;
-    const uint32_t pos = atomicAdd(&S.nOrph, 1u);
-    uint4* o4 = reinterpret_cast<uint4*>(C.orph) + 2 * pos;
-    o4[0] = make_uint4(e | (j << 27), ce.x, 0u, 0u);
-    o4[1] = ce;
+  {
+    // candidates in groups of OB per thread: owners, then their masks, all in flight together
+    constexpr uint32_t OB = 4;
+    for (uint32_t e0 = t; e0 < C.ncand; e0 += OB * Grp::G) {
+      uint32_t ow[OB], mk[OB];
+#pragma unroll
+      for (uint32_t b = 0; b < OB; b++) {
+        const uint32_t e = e0 + b * Grp::G;
+        ow[b] = e < C.ncand ? C.cown[e] : NONE;
+      }
+#pragma unroll
+      for (uint32_t b = 0; b < OB; b++) mk[b] = ow[b] != NONE ? C.vmask[ow[b]] : 0u;
+#pragma unroll
+      for (uint32_t b = 0; b < OB; b++) {
+        if (!mk[b]) continue;
+        const uint32_t j = (uint32_t)__ffs(mk[b]) - 1;
+        if (j >= m) continue;
+        const uint32_t e = e0 + b * Grp::G;
+        const uint32_t pos = atomicAdd(&S.nOrph, 1u);
+        const uint4 ce = C.cand[e];
+        uint4* o4 = reinterpret_cast<uint4*>(C.orph) + 2 * pos;
+        o4[0] = make_uint4(e | (j << 27), ce.x, 0u, 0u);
+        o4[1] = ce;
+      }
+    }
   }
   g.sync();
   const uint32_t no = S.nOrph;
@@ -910,7 +977,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   }
   for (uint32_t e = t; e < nL; e += Grp::G) {
     if (S.lj[e] >= m) continue;
-    C.F[(size_t)FW * S.lface[e]] &= ~ALIVE;
+    C.F[(size_t)FW * S.lface[e]] = S.la[e];  // first corner without the alive bit
     C.fkey[S.lface[e]] = 0ull;
   }
   for (uint32_t q = t; q < no; q += Grp::G) {
