@@ -65,15 +65,19 @@ class GpuBackend : public Backend {
     if (cfg.device < 0 || cfg.device >= n) throw std::invalid_argument("device ordinal out of range");
     hipCheck(hipSetDevice(cfg.device), "hipSetDevice");
     hipCheck(hipStreamCreate(&stream_), "hipStreamCreate");
+    hipCheck(hipStreamCreate(&side_), "hipStreamCreate");
     ctr_.ensure(1);
   }
   ~GpuBackend() override {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (side_) (void)hipStreamSynchronize(side_);
     for (const Pending& pe : pending_) {
       (void)hipEventDestroy(pe.a);
       (void)hipEventDestroy(pe.b);
     }
     for (hipEvent_t e : evPool_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : joinEv_) (void)hipEventDestroy(e);
+    if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
   KernelStats kernelStats() const override { return kst_; }
@@ -100,6 +104,8 @@ class GpuBackend : public Backend {
       evPool_.push_back(pe.b);
     }
     pending_.clear();
+    evPool_.insert(evPool_.end(), joinEv_.begin(), joinEv_.end());
+    joinEv_.clear();
   }
   // device time of one launch (or a group of launches) of kernel family k, on the launch stream
   hipEvent_t event() {
@@ -112,15 +118,22 @@ class GpuBackend : public Backend {
     evPool_.pop_back();
     return e;
   }
-  hipEvent_t tBegin() {
+  hipEvent_t tBegin(hipStream_t s = nullptr) {
     hipEvent_t a = event();
-    hipCheck(hipEventRecord(a, stream_), "hipEventRecord");
+    hipCheck(hipEventRecord(a, s ? s : stream_), "hipEventRecord");
     return a;
   }
-  void tEnd(int k, hipEvent_t a) {
+  void tEnd(int k, hipEvent_t a, hipStream_t s = nullptr) {
     hipEvent_t b = event();
-    hipCheck(hipEventRecord(b, stream_), "hipEventRecord");
+    hipCheck(hipEventRecord(b, s ? s : stream_), "hipEventRecord");
     pending_.push_back(Pending{k, a, b});
+  }
+  // work queued on `from` so far must finish before later work on `to`
+  void join(hipStream_t from, hipStream_t to) {
+    hipEvent_t e = event();
+    hipCheck(hipEventRecord(e, from), "hipEventRecord");
+    hipCheck(hipStreamWaitEvent(to, e, 0), "hipStreamWaitEvent");
+    joinEv_.push_back(e);
   }
   // work counters of the kernels (cleared at the start of every entry point, summed at its end)
   void clearWork() {
@@ -155,6 +168,7 @@ class GpuBackend : public Backend {
   uint32_t edgeStride_;
   unsigned long long tetCap_ = 0;
   hipStream_t stream_ = nullptr;
+  hipStream_t side_ = nullptr;  // small-part hulls, concurrent with the large-part kernel
   DevBuf<float> verts_;
   DevBuf<uint32_t> tris_;
   DevBuf<dk::DEdge> edges_;
@@ -181,6 +195,7 @@ class GpuBackend : public Backend {
   };
   std::vector<Pending> pending_;
   std::vector<hipEvent_t> evPool_;
+  std::vector<hipEvent_t> joinEv_;  // stream joins since the last sync
   KernelStats kst_;
   // last run
   std::vector<uint32_t> lastLevelCounts_;
@@ -243,6 +258,10 @@ void GpuBackend::ensurePools() {
 void GpuBackend::runHulls(std::vector<dk::HullJob>& hjIn, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
                           std::vector<dk::HullResultD>* results) {
   if (hjIn.empty()) return;
+  // Tree candidates (no face export) need only the hull volume, which is exact
+  // on the grid and so the same for every insertion order: they insert in any
+  // valid order. Exported hulls keep the farthest-first face list.
+  const bool freeOrder = exportBuf == nullptr;
   // large parts first (a workgroup each), then small ones (a wavefront each)
   std::vector<uint32_t> order(hjIn.size());
   for (uint32_t k = 0; k < order.size(); k++) order[k] = k;
@@ -270,22 +289,35 @@ void GpuBackend::runHulls(std::vector<dk::HullJob>& hjIn, dk::DPart* parts, uint
     size_t mid = start;
     while (mid < end && hj[mid].nv >= dk::HBIG_NV) mid++;
     h2d(hjobs_.p + start, hj.data() + start, (end - start) * sizeof(dk::HullJob), "upload hull jobs");
+    // The large-part kernel holds one workgroup per part, usually far fewer than the
+    // chip's CUs, for its whole duration: the small parts run beside it on the side
+    // stream, and the main stream waits for both before anything else.
+    if (end > mid) join(stream_, side_);
     if (mid > start) {
       const uint32_t n = (uint32_t)(mid - start);
       hipEvent_t a = tBegin();
-      dk::k_hull_big<<<n, dk::HBIG, 0, stream_>>>(hjobs_.p + start, n, verts_.p, arena_.p, parts, hres_.p + start,
-                                                  exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+      if (freeOrder)
+        dk::k_hull_big<true><<<n, dk::HBIG, 0, stream_>>>(hjobs_.p + start, n, verts_.p, arena_.p, parts,
+                                                          hres_.p + start, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+      else
+        dk::k_hull_big<false><<<n, dk::HBIG, 0, stream_>>>(hjobs_.p + start, n, verts_.p, arena_.p, parts,
+                                                           hres_.p + start, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
       launched("k_hull_big");
       tEnd(K_HULL_BIG, a);
     }
     if (end > mid) {
       const uint32_t n = (uint32_t)(end - mid);
-      hipEvent_t a = tBegin();
-      dk::k_hull<<<(n + dk::HW - 1) / dk::HW, 64 * dk::HW, 0, stream_>>>(hjobs_.p + mid, n, verts_.p, arena_.p, parts,
-                                                                          hres_.p + mid, exportBuf, items_.p, jobs_.p,
-                                                                          HL, ctr_.p);
+      hipEvent_t a = tBegin(side_);
+      const dim3 grid((n + dk::HW - 1) / dk::HW), block(64 * dk::HW);
+      if (freeOrder)
+        dk::k_hull<true><<<grid, block, 0, side_>>>(hjobs_.p + mid, n, verts_.p, arena_.p, parts, hres_.p + mid,
+                                                    exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+      else
+        dk::k_hull<false><<<grid, block, 0, side_>>>(hjobs_.p + mid, n, verts_.p, arena_.p, parts, hres_.p + mid,
+                                                     exportBuf, items_.p, jobs_.p, HL, ctr_.p);
       launched("k_hull");
-      tEnd(K_HULL, a);
+      tEnd(K_HULL, a, side_);
+      join(side_, stream_);
     }
     kst_.hullJobs += end - start;
     start = end;

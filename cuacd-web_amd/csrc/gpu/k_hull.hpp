@@ -6,8 +6,8 @@
 // Orientation is decided by an f32 filter with an exact int128 fallback (the
 // reference's 96-bit fallback, src/gpu/shaders.ts:813-921), so the predicate
 // never contradicts itself. Branch-and-bound after the seed phase at the last
-// tree level (src/gpu/shaders.ts:1294-1331). Volumes over the original float
-// coordinates with error-free products and exact fixed-point sums.
+// tree level (src/gpu/shaders.ts:1294-1331). The volume is that of the grid
+// polytope, exact (integer determinants, int128 sum), times the cell volume.
 //
 // Insertion runs in rounds. Each face keeps the key (distance, ~index) of its
 // farthest outside point, so the K largest face keys are the next K points of
@@ -29,9 +29,11 @@
 // need no lookups of point coordinates. Owner faces sit in a parallel array.
 //
 // Per-part global slab (u32 words, 256-byte aligned):
-//   faces[20 f] | fkey[2 f] | vmask[f] | vis[f] | map[f] | orph[8 c] | cand[4 c] |
+//   faces[20 f] | fkey[2 f] | vmask[2 f] | vis[f] | map[f] | orph[8 c] | cand[4 c] |
 //   cand2[4 c] | q[3 nv] | cown[c] | cown2[c],   f = 4 nv + 256, c = max(nv, 80).
 #pragma once
+#include <type_traits>
+
 #include "dnum.hpp"
 #include "dtypes.hpp"
 
@@ -42,7 +44,6 @@ constexpr int HBIG = 512;     // threads per part in the workgroup-per-part hull
 constexpr uint32_t HBIG_NV = 2048;  // parts with at least this many points get a whole workgroup
 constexpr uint32_t ALIVE = 0x80000000u;
 constexpr uint32_t FW = 20;                 // words per face record
-constexpr uint32_t HULL_MAX_NV = 1u << 27;  // candidate slots share a word with the candidate number
 
 // batch shape: candidates per round, LDS visible-face capacity, edge-hash slots
 constexpr uint32_t KB_BIG = 16, LCAP_BIG = 512, HT_BIG = 4096, NCAP_BIG = 256;
@@ -52,7 +53,7 @@ __host__ __device__ inline uint32_t hullFmax(uint32_t nv) { return 4 * nv + 256;
 __host__ __device__ inline uint64_t hullSlabWords(uint32_t nv) {
   const uint64_t f = hullFmax(nv);
   const uint64_t c = nv > 80 ? nv : 80;
-  return ((25ull * f + 3ull * nv + 18ull * c) + 63) & ~63ull;
+  return ((26ull * f + 3ull * nv + 18ull * c) + 63) & ~63ull;
 }
 
 __constant__ float KDIR[40][3] = {
@@ -277,12 +278,13 @@ struct HullCtx {
   uint32_t* orph;
   uint32_t* vis;
   uint32_t* map;
-  uint32_t* vmask;
+  unsigned long long* vmask;  // per face: candidates of the round that see it
   unsigned long long* fkey;  // per face: key of its farthest outside point, 0 if none or dead
   uint32_t* F;               // face records
   uint32_t nv, fmax;
   uint32_t nf, nalive, ncand, nIns;
   unsigned long long tests;  // point-face record evaluations by this thread
+  double cell3;              // volume of one grid cell
 };
 
 // order of outside points: larger distance first, then lower index
@@ -418,17 +420,32 @@ __device__ __forceinline__ uint32_t best_face(HullCtx& C, const int32_t* qp, uin
   return bestF;
 }
 
+// det[a; b; c] of grid corners relative to the grid centre (2^20 per axis):
+// coordinates within 2^20 of it, a 42-bit cross product, a 63-bit determinant
+__device__ __forceinline__ int64_t grid_det(const uint4& a, const uint4& b, const uint4& c) {
+  constexpr int64_t H = 1 << 20;
+  const int64_t ax = (int64_t)(int32_t)a.x - H, ay = (int64_t)(int32_t)a.y - H, az = (int64_t)(int32_t)a.z - H;
+  const int64_t bx = (int64_t)(int32_t)b.x - H, by = (int64_t)(int32_t)b.y - H, bz = (int64_t)(int32_t)b.z - H;
+  const int64_t cx = (int64_t)(int32_t)c.x - H, cy = (int64_t)(int32_t)c.y - H, cz = (int64_t)(int32_t)c.z - H;
+  const int64_t x = by * cz - bz * cy, y = bz * cx - bx * cz, z = bx * cy - by * cx;
+  return ax * x + ay * y + az * z;
+}
+
+// exact volume of the grid polytope (integer determinants of the alive faces'
+// corners, summed exactly: the same value for every triangulation of the same
+// polytope), times the cell volume
 template <typename Grp>
 __device__ __forceinline__ double hull_volume(const Grp& g, const HullCtx& C) {
   i128 acc = 0;
   for (uint32_t f = g.tid(); f < C.nf; f += Grp::G) {
-    const uint32_t* r = C.F + (size_t)FW * f;
-    if (!(r[0] & ALIVE)) continue;
-    const uint32_t a = r[0] & ~ALIVE, b = r[1], c = r[2];
-    acc_det(acc, C.V + 3 * a, C.V + 3 * b, C.V + 3 * c, false);
+    const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * f);
+    if (!(r4[0].x & ALIVE)) continue;
+    acc += (i128)grid_det(r4[2], r4[3], r4[4]);
   }
   acc = g.sum(acc);
-  return fix_to_f64(fix_abs(acc)) / 6.0;
+  const i128 a = acc < 0 ? -acc : acc;
+  const double d = (double)(int64_t)(a >> 64) * 0x1p64 + (double)(uint64_t)a;
+  return d / 6.0 * C.cell3;
 }
 
 // stable in-place removal of dead face slots; face keys and candidate owners follow
@@ -566,36 +583,65 @@ __device__ __forceinline__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p
   g.sync();
 }
 
-// group-shared state of one insertion round
-template <uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+// group-shared state of one insertion round (FREE: free-order mode, see hull_round)
+template <bool FREE_, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
 struct HullBatch {
+  static constexpr bool FREE = FREE_;
+  using Mask = typename std::conditional<(K > 32), unsigned long long, uint32_t>::type;  // one bit per candidate
   FaceRec nrec[NCAP];  // the round's first new faces
-  unsigned long long ckey[K];  // candidate keys, descending
+  unsigned long long ckey[K];  // candidate keys (descending in the ordered mode)
   unsigned long long okey[K];  // largest reassigned key per candidate
   uint32_t cpt[K];
   int32_t cq[K][3];
-  uint32_t nvis[K], nh[K];
+  uint32_t nvis[K], nh[K];                              // list entries and horizon edges per candidate
+  uint32_t boff[K], bcnt[K];                            // per-candidate buckets of list entries
+  uint32_t noff[K];                                     // first new face of each candidate
   uint32_t lface[LCAP], la[LCAP], lb[LCAP], lc[LCAP];  // visible faces (any candidate)
   uint8_t lj[LCAP], lflag[LCAP];                       // candidate, horizon edge bits
-  uint32_t lbase[LCAP];                                 // new faces of the same candidate before this entry
-  uint32_t pface[LCAP / 2], pmask[LCAP / 2], pund[LCAP / 2];  // faces the f32 filter left undecided
+  uint16_t bkt[LCAP];                                   // list entries grouped by candidate
+  uint32_t pface[LCAP / 2];
+  Mask pmask[LCAP / 2], pund[LCAP / 2];  // faces the f32 filter left undecided
   unsigned long long hkey[HT];                          // directed edges (a << 32 | b); seed-hull faces in phase 2
   uint8_t hj[HT];
-  unsigned long long wkey[NW][K];
+  unsigned long long wkey[NW][FREE ? K / NW : K];
   unsigned long long wB[NW];
   uint32_t wcnt[NW];
-  uint32_t nList, nOrph, conflict, cutA, m, nPend;
+  Mask conflict, cutA;
+  uint32_t nList, nOrph, m, nPend;
 };
 
 __device__ __forceinline__ uint32_t edge_hash(unsigned long long key, uint32_t HT) {
   return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 40) & (HT - 1);
 }
 
-// inserts the next points of the farthest-first order; returns how many (0: none left)
-template <typename Grp, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
-__device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBatch<K, LCAP, HT, NW, NCAP>& S,
+template <typename M>
+__device__ __forceinline__ M low_mask(uint32_t m) {
+  return m >= 8 * sizeof(M) ? ~(M)0 : (((M)1 << m) - 1);
+}
+__device__ __forceinline__ uint32_t mask_ffs(uint32_t m) { return (uint32_t)__ffs(m); }
+__device__ __forceinline__ uint32_t mask_ffs(unsigned long long m) { return (uint32_t)__ffsll(m); }
+__device__ __forceinline__ uint32_t mask_popc(uint32_t m) { return (uint32_t)__popc(m); }
+__device__ __forceinline__ uint32_t mask_popc(unsigned long long m) { return (uint32_t)__popcll(m); }
+
+// One insertion round; returns how many points it inserted (0: none left).
+//
+// Ordered mode (FREE = false): the next points of the farthest-first order,
+// the longest prefix of the K largest face keys that the one-at-a-time
+// algorithm would insert identically, so the face list is the sequential one.
+//
+// Free mode (FREE = true), for hulls of which only the volume is used: any
+// set of outside points whose insertion one after another (in candidate
+// order) changes only faces seen from the current hull. Candidates are the
+// largest keys of each wave; a candidate is dropped when it shares a visible
+// face with an earlier one or sees a face an earlier one creates. The hull
+// reached is the same polytope whatever the order, and its grid volume is
+// exact, so the volume is the ordered one bit for bit.
+template <typename Grp, bool FREE, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+__device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S,
                                                uint32_t kEff) {
-  static_assert(K <= 32 && (HT & (HT - 1)) == 0 && HT >= 6 * LCAP && NW == Grp::NW && NW * K <= 128, "batch shape");
+  static_assert(K <= 64 && (HT & (HT - 1)) == 0 && HT >= 6 * LCAP && NW == Grp::NW && LCAP <= 65536, "batch shape");
+  static_assert(FREE ? (K % NW == 0 && K / NW <= 64) : (NW * K <= 128 && K <= 32), "batch shape");
+  using Mask = typename HullBatch<FREE, K, LCAP, HT, NW, NCAP>::Mask;
   const uint32_t t = g.tid();
   const uint32_t lane = lane_id();
   const uint32_t w = NW == 1 ? 0u : (t >> 6);
@@ -610,13 +656,15 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     S.nList = 0;
     S.nOrph = 0;
     S.nPend = 0;
-    S.conflict = K;
-    S.cutA = K;
+    S.conflict = FREE ? (Mask)0 : (Mask)K;
+    S.cutA = FREE ? (Mask)0 : (Mask)K;
   }
-  // ---- the K largest face keys. Each lane offers its two largest; the third
-  // largest of every lane bounds everything not offered, so only keys above
-  // that bound are taken (they are then exactly the largest ones, in order).
+  // ---- candidates. Each lane offers its two largest face keys. Ordered: the
+  // third largest of every lane bounds everything not offered, so only keys
+  // above that bound are taken (they are then exactly the largest ones, in
+  // order). Free: each wave takes its largest kEff / NW offered keys.
   // Candidate point numbers are the complemented low words of the keys.
+  const uint32_t kw = FREE ? (kEff + NW - 1) / NW : kEff;
   unsigned long long k1 = 0, k2 = 0, k3 = 0;
   for (uint32_t f = t; f < C.nf; f += Grp::G) {
     const unsigned long long k = C.fkey[f];
@@ -633,7 +681,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   }
   {
     uint32_t ptr = 0, cnt = 0;
-    for (uint32_t it = 0; it < kEff; it++) {
+    for (uint32_t it = 0; it < kw; it++) {
       const unsigned long long head = ptr == 0 ? k1 : (ptr == 1 ? k2 : 0ull);
       const unsigned long long mx = wave_max_key(head);
       if (mx == 0ull) break;
@@ -641,14 +689,23 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       if (head == mx) ptr++;  // keys are distinct: one lane matches
       cnt++;
     }
-    const unsigned long long bw = wave_max_key(k3);
+    const unsigned long long bw = FREE ? 0ull : wave_max_key(k3);
     if (lane == 0) {
       S.wB[w] = bw;
       S.wcnt[w] = cnt;
     }
   }
   g.sync();
-  if (NW == 1) {
+  if (FREE) {
+    if (t < 64) {
+      // the waves' lists side by side (at most K entries)
+      const uint32_t q = lane / kw, i = lane - q * kw;
+      const bool v = q < NW && i < S.wcnt[q < NW ? q : 0];
+      const uint64_t bal = __ballot(v);
+      if (v) S.ckey[prefix_below(bal)] = S.wkey[q][i];
+      if (lane == 0) S.m = (uint32_t)__popcll(bal);
+    }
+  } else if (NW == 1) {
     // one wave: its list is the answer, cut at its bound
     const unsigned long long B = S.wB[0];
     const uint32_t cnt = S.wcnt[0];
@@ -691,13 +748,17 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   g.sync();
   // ---- visibility of every alive face from every candidate
   for (int pass = 0;; pass++) {
-    // a face's visibility mask is final: record it, count conflicts, list it
-    auto finish = [&](uint32_t f, uint32_t mask, uint32_t a, uint32_t b, uint32_t c) {
+    // a face's visibility mask is final: record it, note conflicts, list it
+    // under its first candidate
+    auto finish = [&](uint32_t f, Mask mask, uint32_t a, uint32_t b, uint32_t c) {
       C.vmask[f] = mask;
       if (!mask) return;
-      const uint32_t j0 = (uint32_t)__ffs(mask) - 1;
-      const uint32_t rest = mask & (mask - 1);
-      if (rest) atomicMin(&S.conflict, (uint32_t)__ffs(rest) - 1);
+      const uint32_t j0 = mask_ffs(mask) - 1;
+      const Mask rest = mask & (mask - 1);
+      if (rest) {
+        if (FREE) atomicOr(&S.conflict, rest);
+        else atomicMin(&S.conflict, (Mask)(mask_ffs(rest) - 1));
+      }
       atomicAdd(&S.nvis[j0], 1u);
       const uint32_t pos = atomicAdd(&S.nList, 1u);
       if (pos < LCAP) {
@@ -730,12 +791,12 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       for (uint32_t b = 0; b < VB; b++) {
         const uint32_t f = f0 + b * Grp::G;
         if (!(h0[b].x & ALIVE)) continue;
-        uint32_t mask = 0, und = 0;
+        Mask mask = 0, und = 0;
         for (uint32_t j = 0; j < m; j++) {
           float det;
           const int sgn = face_filter(h1[b], h2[b], S.cq[j], &det);
-          mask |= sgn > 0 ? (1u << j) : 0u;
-          und |= sgn == 0 ? (1u << j) : 0u;
+          mask |= sgn > 0 ? ((Mask)1 << j) : (Mask)0;
+          und |= sgn == 0 ? ((Mask)1 << j) : (Mask)0;
         }
         C.tests += m;
         if (und) {
@@ -747,9 +808,9 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
             continue;
           }
           const uint32_t* r = C.F + (size_t)FW * f;
-          for (uint32_t u = und; u; u &= u - 1) {
-            const uint32_t j = (uint32_t)__ffs(u) - 1;
-            if (orient_rec(r, h2[b], S.cq[j]) > 0) mask |= 1u << j;
+          for (Mask u = und; u; u &= u - 1) {
+            const uint32_t j = mask_ffs(u) - 1;
+            if (orient_rec(r, h2[b], S.cq[j]) > 0) mask |= (Mask)1 << j;
           }
         }
         finish(f, mask, h0[b].x & ~ALIVE, h0[b].y, h0[b].z);
@@ -762,10 +823,10 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
         const uint32_t f = S.pface[p];
         const uint32_t* r = C.F + (size_t)FW * f;
         const uint4 w0 = reinterpret_cast<const uint4*>(r)[0], w2 = reinterpret_cast<const uint4*>(r)[2];
-        uint32_t mask = S.pmask[p];
-        for (uint32_t u = S.pund[p]; u; u &= u - 1) {
-          const uint32_t j = (uint32_t)__ffs(u) - 1;
-          if (orient_rec(r, w2, S.cq[j]) > 0) mask |= 1u << j;
+        Mask mask = S.pmask[p];
+        for (Mask u = S.pund[p]; u; u &= u - 1) {
+          const uint32_t j = mask_ffs(u) - 1;
+          if (orient_rec(r, w2, S.cq[j]) > 0) mask |= (Mask)1 << j;
         }
         finish(f, mask, w0.x & ~ALIVE, w0.y, w0.z);
       }
@@ -790,15 +851,23 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     if (t == 0) {
       S.nList = 0;
       S.nPend = 0;
-      S.conflict = K;
+      S.conflict = FREE ? (Mask)0 : (Mask)K;
     }
     g.sync();
   }
-  m = m < S.conflict ? m : S.conflict;
+  // candidates taking part: ordered, the prefix before the first shared face;
+  // free, those sharing no visible face with an earlier candidate
+  Mask act;
+  if (FREE) {
+    act = low_mask<Mask>(m) & ~S.conflict;
+  } else {
+    m = m < (uint32_t)S.conflict ? m : (uint32_t)S.conflict;
+    act = low_mask<Mask>(m);
+  }
   const uint32_t nL = S.nList;
   // ---- horizon: directed edges of visible faces whose twin is not visible from the same candidate
   for (uint32_t e = t; e < nL; e += Grp::G) {
-    if (S.lj[e] >= m) continue;
+    if (!((act >> S.lj[e]) & 1u)) continue;
     const uint32_t v3[3] = {S.la[e], S.lb[e], S.lc[e]};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -808,10 +877,16 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       S.hj[s] = S.lj[e];
     }
   }
+  if (t < K) {
+    uint32_t b = 0;
+    for (uint32_t i = 0; i < t; i++) b += S.nvis[i];
+    S.boff[t] = b;
+    S.bcnt[t] = 0;
+  }
   g.sync();
   for (uint32_t e = t; e < nL; e += Grp::G) {
     const uint32_t j = S.lj[e];
-    if (j >= m) continue;
+    if (!((act >> j) & 1u)) continue;
     const uint32_t v3[3] = {S.la[e], S.lb[e], S.lc[e]};
     uint32_t flags = 0;
 #pragma unroll
@@ -832,40 +907,48 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     }
     S.lflag[e] = (uint8_t)flags;
     atomicAdd(&S.nh[j], (uint32_t)__popc(flags));
+    S.bkt[S.boff[j] + atomicAdd(&S.bcnt[j], 1u)] = (uint16_t)e;
   }
   g.sync();
-  // capacity of the face array for the new faces of each prefix
+  // capacity of the face array for the new faces (candidates in order)
   {
     uint32_t off = 0;
     for (uint32_t j = 0; j < m; j++) {
+      if (!((act >> j) & 1u)) continue;
       if (C.nf + off + S.nh[j] > C.fmax) {
-        m = j;
+        act &= low_mask<Mask>(j);
+        if (!FREE) m = j;
         break;
       }
       off += S.nh[j];
     }
   }
-  // new faces of candidate j, numbered (visible face in creation order, edge):
-  // each entry's base is the count of horizon edges of earlier entries of the same candidate
-  for (uint32_t e = t; e < nL; e += Grp::G) S.lbase[e] = 0;
-  g.sync();
-  for (uint32_t pr = t; pr < nL * nL; pr += Grp::G) {
-    const uint32_t e = pr / nL, e2 = pr - e * nL;
-    const uint32_t j = S.lj[e];
-    if (j < m && S.lj[e2] == j && S.lface[e2] < S.lface[e] && S.lflag[e2])
-      atomicAdd(&S.lbase[e], (uint32_t)__popc(S.lflag[e2]));
+  // first new face of each candidate: the faces of all earlier candidates that have any
+  if (t < K) {
+    uint32_t o = 0;
+    for (uint32_t i = 0; i < t; i++) o += S.nh[i];
+    S.noff[t] = o;
   }
   g.sync();
+  // new faces of candidate j, numbered (visible face in creation order, edge):
+  // an entry's base counts the horizon edges of the candidate's entries on
+  // earlier-created faces
   for (uint32_t e = t; e < nL; e += Grp::G) {
     const uint32_t j = S.lj[e];
-    if (j >= m) continue;
+    if (!((act >> j) & 1u)) continue;
     const uint32_t flags = S.lflag[e];
     if (!flags) continue;
     const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * S.lface[e]);
     const uint4 w2 = r4[2], w3 = r4[3], w4 = r4[4];
-    uint32_t off = 0;
-    for (uint32_t i = 0; i < j; i++) off += S.nh[i];
-    uint32_t rel = off + S.lbase[e];
+    uint32_t off = S.noff[j];
+    {
+      const uint32_t fe = S.lface[e], b0 = S.boff[j], nb = S.nvis[j];
+      for (uint32_t k = 0; k < nb; k++) {
+        const uint32_t e2 = S.bkt[b0 + k];
+        if (S.lface[e2] < fe) off += (uint32_t)__popc(S.lflag[e2]);
+      }
+    }
+    uint32_t rel = off;
     const uint32_t v3[3] = {S.la[e], S.lb[e], S.lc[e]};
     const int32_t q3[3][3] = {{(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z},
                               {(int32_t)w3.x, (int32_t)w3.y, (int32_t)w3.z},
@@ -882,28 +965,34 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       }
   }
   g.sync();
-  // a candidate that sees a face made earlier in the round ends the round before it
-  {
-    uint32_t total = 0, off = 0;
-    for (uint32_t j = 1; j < m; j++) {
-      off += S.nh[j - 1];
-      total += off;
-    }
-    for (uint32_t p = t; p < total; p += Grp::G) {
-      uint32_t j = 1, off2 = S.nh[0], start = 0;
-      while (p >= start + off2) {
-        start += off2;
-        off2 += S.nh[j];
-        j++;
-      }
+  // a candidate that sees a face made earlier in the round: ordered, the round
+  // ends before it; free, it is dropped (its new faces die)
+  for (uint32_t j = 1; j < m; j++) {
+    if (!((act >> j) & 1u)) continue;
+    const uint32_t nEarlier = S.noff[j];
+    for (uint32_t rel = t; rel < nEarlier; rel += Grp::G) {
       C.tests++;
-      const uint32_t rel = p - start;
       const float dd = rel < NCAP ? face_eval_rec(S.nrec[rel], S.cq[j]) : face_eval(C, C.nf + rel, S.cq[j]);
-      if (dd > 0.0f) atomicMin(&S.cutA, j);
+      if (dd > 0.0f) {
+        if (FREE) atomicOr(&S.cutA, (Mask)1 << j);
+        else atomicMin(&S.cutA, (Mask)j);
+      }
     }
   }
   g.sync();
-  m = m < S.cutA ? m : S.cutA;
+  Mask acc;
+  if (FREE) {
+    const Mask drop = act & S.cutA;
+    acc = act & ~drop;
+    for (Mask d = drop; d; d &= d - 1) {
+      const uint32_t j = mask_ffs(d) - 1;
+      for (uint32_t rel = S.noff[j] + t; rel < S.noff[j] + S.nh[j]; rel += Grp::G)
+        C.F[(size_t)FW * (C.nf + rel)] &= ~ALIVE;
+    }
+  } else {
+    m = m < (uint32_t)S.cutA ? m : (uint32_t)S.cutA;
+    acc = low_mask<Mask>(m);
+  }
   // ---- reassignment of the points whose owner is visible, over the new faces
   // of that candidate (the candidate itself lies on them and finds none).
   // Collected first, then evaluated one point per thread.
@@ -911,24 +1000,25 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     // candidates in groups of OB per thread: owners, then their masks, all in flight together
     constexpr uint32_t OB = 4;
     for (uint32_t e0 = t; e0 < C.ncand; e0 += OB * Grp::G) {
-      uint32_t ow[OB], mk[OB];
+      uint32_t ow[OB];
+      Mask mk[OB];
 #pragma unroll
       for (uint32_t b = 0; b < OB; b++) {
         const uint32_t e = e0 + b * Grp::G;
         ow[b] = e < C.ncand ? C.cown[e] : NONE;
       }
 #pragma unroll
-      for (uint32_t b = 0; b < OB; b++) mk[b] = ow[b] != NONE ? C.vmask[ow[b]] : 0u;
+      for (uint32_t b = 0; b < OB; b++) mk[b] = ow[b] != NONE ? (Mask)C.vmask[ow[b]] : (Mask)0;
 #pragma unroll
       for (uint32_t b = 0; b < OB; b++) {
         if (!mk[b]) continue;
-        const uint32_t j = (uint32_t)__ffs(mk[b]) - 1;
-        if (j >= m) continue;
+        const uint32_t j = mask_ffs(mk[b]) - 1;
+        if (!((acc >> j) & 1u)) continue;
         const uint32_t e = e0 + b * Grp::G;
         const uint32_t pos = atomicAdd(&S.nOrph, 1u);
         const uint4 ce = C.cand[e];
         uint4* o4 = reinterpret_cast<uint4*>(C.orph) + 2 * pos;
-        o4[0] = make_uint4(e | (j << 27), ce.x, 0u, 0u);
+        o4[0] = make_uint4(e, j, 0u, 0u);  // candidate slot, inserting candidate, new owner, distance
         o4[1] = ce;
       }
     }
@@ -938,10 +1028,9 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   for (uint32_t q = t; q < no; q += Grp::G) {
     uint4* o4 = reinterpret_cast<uint4*>(C.orph) + 2 * q;
     const uint4 ov = o4[0], ce = o4[1];
-    const uint32_t j = ov.x >> 27, i = ov.y;
+    const uint32_t j = ov.y, i = ce.x;
     const int32_t qi[3] = {(int32_t)ce.y, (int32_t)ce.z, (int32_t)ce.w};
-    uint32_t off = 0;
-    for (uint32_t k = 0; k < j; k++) off += S.nh[k];
+    const uint32_t off = S.noff[j];
     const uint32_t r1 = off + S.nh[j];
     C.tests += S.nh[j];
     uint32_t bf = NONE;
@@ -955,11 +1044,11 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     }
     o4->z = bf;
     o4->w = __float_as_uint(bd);
-    if (bf != NONE) atomicMax(&S.okey[j], pkey(bd, i));
+    if (!FREE && bf != NONE) atomicMax(&S.okey[j], pkey(bd, i));
   }
   g.sync();
-  // a reassigned point farther than the next candidate ends the round before it
-  {
+  if (!FREE) {
+    // a reassigned point farther than the next candidate ends the round before it
     unsigned long long run = 0;
     for (uint32_t j = 0; j < m; j++) {
       if (j > 0 && S.ckey[j] < run) {
@@ -968,34 +1057,41 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       }
       run = S.okey[j] > run ? S.okey[j] : run;
     }
+    acc = low_mask<Mask>(m);
   }
-  // ---- commit candidates [0, m)
-  uint32_t addF = 0, delF = 0;
+  // ---- commit the accepted candidates
+  uint32_t addF = 0, delF = 0, used = 0;
   for (uint32_t j = 0; j < m; j++) {
-    addF += S.nh[j];
-    delF += S.nvis[j];
+    used += (act >> j) & 1u ? S.nh[j] : 0u;
+    if ((acc >> j) & 1u) {
+      addF += S.nh[j];
+      delF += S.nvis[j];
+    }
   }
   for (uint32_t e = t; e < nL; e += Grp::G) {
-    if (S.lj[e] >= m) continue;
+    if (!((acc >> S.lj[e]) & 1u)) continue;
     C.F[(size_t)FW * S.lface[e]] = S.la[e];  // first corner without the alive bit
     C.fkey[S.lface[e]] = 0ull;
   }
   for (uint32_t q = t; q < no; q += Grp::G) {
     const uint4 o4 = reinterpret_cast<const uint4*>(C.orph)[2 * q];
-    if ((o4.x >> 27) >= m) continue;
-    C.cown[o4.x & (HULL_MAX_NV - 1)] = o4.z;
-    if (o4.z != NONE) atomicMax(&C.fkey[o4.z], pkey(__uint_as_float(o4.w), o4.y));
+    if (!((acc >> o4.y) & 1u)) continue;
+    C.cown[o4.x] = o4.z;
+    if (o4.z != NONE) atomicMax(&C.fkey[o4.z], pkey(__uint_as_float(o4.w), reinterpret_cast<const uint4*>(C.orph)[2 * q + 1].x));
   }
-  C.nf += addF;
+  // ordered: the accepted prefix's faces come first, later ones are overwritten;
+  // free: every written face keeps its slot (dropped candidates' faces are dead)
+  C.nf += FREE ? used : addF;
   C.nalive = C.nalive + addF - delF;
-  C.nIns += m;
+  const uint32_t ins = mask_popc(acc);
+  C.nIns += ins;
   g.sync();
-  return m;
+  return ins;
 }
 
-// farthest outstanding candidate (largest distance, then lowest index) until none
-template <typename Grp, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
-__device__ __forceinline__ void hull_grow(const Grp& g, HullCtx& C, HullBatch<K, LCAP, HT, NW, NCAP>& S) {
+// insertion rounds until no point is outside (ordered: farthest first)
+template <typename Grp, bool FREE, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+__device__ __forceinline__ void hull_grow(const Grp& g, HullCtx& C, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S) {
   const uint32_t maxRounds = C.nv + 8;
   uint32_t kEff = K;  // candidates looked at: about twice what the last round inserted
   for (uint32_t r = 0; r < maxRounds; r++) {
@@ -1011,9 +1107,11 @@ struct HullLevel {  // branch-and-bound inputs at the last tree level
   uint32_t prune, itemsInBase, nItemsIn, nSeeds, depth, jobBase;
 };
 
-// one hull job on one group; seed points in `seed` (80 entries of group-shared memory)
-template <typename Grp, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
-__device__ void hull_job(const Grp& g, HullBatch<K, LCAP, HT, NW, NCAP>& S, const dk::HullJob& J, uint32_t hj, uint32_t* seed,
+// one hull job on one group; seed points in `seed` (80 entries of group-shared memory).
+// FREE: insertion in any valid order (the volume is exact either way; the face
+// list and insertion count are those of some valid order, not the sequential one)
+template <typename Grp, bool FREE, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+__device__ void hull_job(const Grp& g, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S, const dk::HullJob& J, uint32_t hj, uint32_t* seed,
                          const float* __restrict__ verts,
                          uint32_t* __restrict__ arena, DPart* __restrict__ parts, HullResultD* __restrict__ results,
                          uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items, const DJob* __restrict__ jobs,
@@ -1036,10 +1134,10 @@ __device__ void hull_job(const Grp& g, HullBatch<K, LCAP, HT, NW, NCAP>& S, cons
   const uint32_t cn = C.nv > 80 ? C.nv : 80;
   C.F = slab;
   C.fkey = reinterpret_cast<unsigned long long*>(slab + FW * fm);
-  C.vmask = slab + (FW + 2) * fm;
-  C.vis = C.vmask + fm;
+  C.vmask = reinterpret_cast<unsigned long long*>(slab + (FW + 2) * fm);
+  C.vis = slab + (FW + 4) * fm;
   C.map = C.vis + fm;
-  C.orph = C.map + fm;  // 16-byte aligned: 25 f is a multiple of 4
+  C.orph = C.map + fm;  // 16-byte aligned: 26 f is a multiple of 4
   C.cand = reinterpret_cast<uint4*>(C.orph + 8 * (size_t)cn);
   C.cand2 = C.cand + cn;
   C.Q = reinterpret_cast<int32_t*>(C.cand2 + cn);
@@ -1047,6 +1145,7 @@ __device__ void hull_job(const Grp& g, HullBatch<K, LCAP, HT, NW, NCAP>& S, cons
   C.cown2 = C.cown + cn;
   C.nf = C.nalive = C.ncand = C.nIns = 0;
   C.tests = 0;
+  C.cell3 = 0.0;
   bool done = false;
   if (J.nv < 4) done = true;
   else if (hullSlabWords(J.nv) > J.slabWords) {
@@ -1071,6 +1170,10 @@ __device__ void hull_job(const Grp& g, HullBatch<K, LCAP, HT, NW, NCAP>& S, cons
     scale = scale > ez ? scale : ez;
     scale = scale > 1e-30f ? scale : 1e-30f;
     const float qs = 2097152.0f / scale;
+    {
+      const double cell = (double)scale * 0x1p-21;
+      C.cell3 = cell * cell * cell;
+    }
     for (uint32_t i = t; i < C.nv; i += Grp::G)
       for (int k = 0; k < 3; k++) {
         const float d = C.V[3 * i + k] - lo[k];
@@ -1336,7 +1439,11 @@ __device__ void hull_job(const Grp& g, HullBatch<K, LCAP, HT, NW, NCAP>& S, cons
   }
 }
 
+// candidates per round in the free-order mode
+constexpr uint32_t KF_BIG = 32, KF_W = 16;
+
 // small parts: one wavefront per part, HW parts per block
+template <bool FREE>
 __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hjobs, uint32_t nJobsH,
                                                   const float* __restrict__ verts, uint32_t* __restrict__ arena,
                                                   DPart* __restrict__ parts, HullResultD* __restrict__ results,
@@ -1346,13 +1453,16 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
   const uint32_t wid = threadIdx.x >> 6;
   const uint32_t hj = blockIdx.x * HW + wid;
   if (hj >= nJobsH) return;
+  constexpr uint32_t KW = FREE ? KF_W : KB_W;
   __shared__ uint32_t s_seed[HW][80];
-  __shared__ HullBatch<KB_W, LCAP_W, HT_W, 1, NCAP_W> s_batch[HW];
+  __shared__ HullBatch<FREE, KW, LCAP_W, HT_W, 1, NCAP_W> s_batch[HW];
   const WaveGrp g;
-  hull_job(g, s_batch[wid], hjobs[hj], hj, s_seed[wid], verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTests, &ctr->hullBytes);
+  hull_job(g, s_batch[wid], hjobs[hj], hj, s_seed[wid], verts, arena, parts, results, exportBuf, items, jobs, HL,
+                 ctr, &ctr->faceTests, &ctr->hullBytes);
 }
 
 // large parts: one workgroup of HBIG threads per part
+template <bool FREE>
 __global__ __launch_bounds__(HBIG) void k_hull_big(const HullJob* __restrict__ hjobs, uint32_t nJobsH,
                                                    const float* __restrict__ verts, uint32_t* __restrict__ arena,
                                                    DPart* __restrict__ parts, HullResultD* __restrict__ results,
@@ -1361,18 +1471,20 @@ __global__ __launch_bounds__(HBIG) void k_hull_big(const HullJob* __restrict__ h
                                                    Counters* __restrict__ ctr) {
   const uint32_t hj = blockIdx.x;
   if (hj >= nJobsH) return;
+  constexpr uint32_t KB = FREE ? KF_BIG : KB_BIG;
   __shared__ uint32_t s_seed[80];
   __shared__ uint32_t s_u[HBIG / 64];
   __shared__ float s_f[HBIG / 64];
   __shared__ unsigned long long s_l[3 * (HBIG / 64)];
   __shared__ uint32_t s_m[4 * 16 * (HBIG / 64)];
-  __shared__ HullBatch<KB_BIG, LCAP_BIG, HT_BIG, HBIG / 64, NCAP_BIG> s_batch;
+  __shared__ HullBatch<FREE, KB, LCAP_BIG, HT_BIG, HBIG / 64, NCAP_BIG> s_batch;
   BlockGrp<HBIG> g;
   g.su = s_u;
   g.sf = s_f;
   g.sl = s_l;
   g.sm = s_m;
-  hull_job(g, s_batch, hjobs[hj], hj, s_seed, verts, arena, parts, results, exportBuf, items, jobs, HL, ctr, &ctr->faceTestsBig, &ctr->hullBytesBig);
+  hull_job(g, s_batch, hjobs[hj], hj, s_seed, verts, arena, parts, results, exportBuf, items, jobs, HL, ctr,
+                 &ctr->faceTestsBig, &ctr->hullBytesBig);
 }
 
 }  // namespace dk

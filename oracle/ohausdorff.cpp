@@ -85,10 +85,28 @@ float distPtTriSq(const float* p, const float* a, const float* b, const float* c
 }
 
 namespace {
+// Rounding weight of a triangle for the box test: the f32 distance above
+// departs from the true one by a few ulps of |p - a| (times the sliver factor
+// k = |ab||ac|/|n| for the plane distance, and k^2 for the inside/outside
+// decision), so w = k^2 + k + 2 scales the margin a box needs. Degenerate
+// triangles (vertex distances only) have w = 2.
+double roundingWeight(const float* a, const float* b, const float* c) {
+  const float ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const float ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  const float n[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+  const float n2 = dot3(n, n);
+  if (n2 < 1e-20f) return 2.0;
+  const double lab = std::sqrt((double)ab[0] * ab[0] + (double)ab[1] * ab[1] + (double)ab[2] * ab[2]);
+  const double lac = std::sqrt((double)ac[0] * ac[0] + (double)ac[1] * ac[1] + (double)ac[2] * ac[2]);
+  const double k = lab * lac / std::sqrt((double)n2);
+  return k * k + k + 2.0;
+}
+
 // median-split BVH (src/cpu/hausdorff.ts:118-178) with a conservative box test
 struct BVH {
   const MeshRef* m;
   std::vector<float> bmin, bmax;
+  std::vector<double> wmax;  // largest rounding weight of the node's triangles
   std::vector<int> left, right, start, count;
   std::vector<uint32_t> order;
   std::vector<float> cent;
@@ -102,6 +120,7 @@ struct BVH {
       bmin.push_back(INFINITY);
       bmax.push_back(-INFINITY);
     }
+    wmax.push_back(0.0);
     float c0[3] = {INFINITY, INFINITY, INFINITY}, c1[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int i = s; i < e; i++) {
       const uint32_t t = order[i];
@@ -116,6 +135,7 @@ struct BVH {
         c0[a] = std::min(c0[a], cent[3 * t + a]);
         c1[a] = std::max(c1[a], cent[3 * t + a]);
       }
+      wmax[node] = std::max(wmax[node], weight[t]);
     }
     if (e - s <= 4) {
       start[node] = s;
@@ -143,8 +163,13 @@ struct BVH {
         for (int k = 0; k < 3; k++) s += mm.verts[3 * (size_t)mm.tris[3 * t + k] + a];
         cent[3 * t + a] = (float)(s / 3);
       }
+    weight.resize(mm.nt);
+    for (uint32_t t = 0; t < mm.nt; t++)
+      weight[t] = roundingWeight(mm.verts + 3 * (size_t)mm.tris[3 * t], mm.verts + 3 * (size_t)mm.tris[3 * t + 1],
+                                 mm.verts + 3 * (size_t)mm.tris[3 * t + 2]);
     if (mm.nt > 0) build(0, (int)mm.nt);
   }
+  std::vector<double> weight;
   double boxSq(int node, const float* p) const {
     double s = 0;
     for (int a = 0; a < 3; a++) {
@@ -153,6 +178,22 @@ struct BVH {
     }
     return s;
   }
+  // squared distance from p to the farthest point of the node's box
+  double farSq(int node, const float* p) const {
+    double s = 0;
+    for (int a = 0; a < 3; a++) {
+      const double d = std::max(std::fabs((double)p[a] - bmin[3 * node + a]), std::fabs((double)p[a] - bmax[3 * node + a]));
+      s += d * d;
+    }
+    return s;
+  }
+  // the node can hold no triangle whose f32 distance is below best: its box is
+  // farther than sqrt(best) by a margin of 1e-4 w (about 1700 ulps per unit of
+  // weight) of the box's far distance
+  bool skip(int node, const float* p, float best) const {
+    const double lb = std::sqrt(boxSq(node, p)) - 1e-4 * wmax[node] * std::sqrt(farSq(node, p));
+    return lb > 0.0 && lb * lb > (double)best;
+  }
   float nearestSq(const float* p) const {
     float best = INFINITY;
     int stack[128];
@@ -160,8 +201,7 @@ struct BVH {
     stack[sp++] = 0;
     while (sp > 0) {
       const int node = stack[--sp];
-      // skip only boxes that are farther by a margin well above f32 rounding of the triangle distance
-      if (boxSq(node, p) > (double)best * (1.0 + 1e-5) + 1e-30) continue;
+      if (skip(node, p, best)) continue;
       if (left[node] < 0) {
         for (int i = start[node]; i < start[node] + count[node]; i++) {
           const uint32_t t = order[i];

@@ -52,6 +52,7 @@ struct Ctx {
   std::vector<float> dist;
   std::vector<std::vector<uint32_t>> members;  // conflict lists by owner face
   uint32_t nIns = 0;
+  double cell3 = 0;
 
   inline float qf(uint32_t i, int k) const { return (float)q[3 * i + k]; }
 
@@ -184,11 +185,13 @@ struct Ctx {
     }
   }
 
+  // exact volume of the grid polytope, scaled back to the original units
+  // (the reference's grid-unit volume times the cell volume, src/gpu/shaders.ts:1132-1142, :1368)
   double volume() const {
     i128 acc = 0;
     for (const Face& f : faces)
-      if (f.alive) acc_det(acc, v + 3 * (size_t)f.a, v + 3 * (size_t)f.b, v + 3 * (size_t)f.c, false);
-    return fix_to_f64(fix_abs(acc)) / 6.0;
+      if (f.alive) acc += grid_det(&q[3 * (size_t)f.a], &q[3 * (size_t)f.b], &q[3 * (size_t)f.c]);
+    return grid_volume(acc, cell3);
   }
 };
 
@@ -208,6 +211,7 @@ Grid quantize(const float* v, uint32_t nv) {
   const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
   const float scale = std::max(std::max(std::max(ex, ey), ez), 1e-30f);
   const float qs = 2097152.0f / scale;
+  g.scale = scale;
   for (uint32_t i = 0; i < nv; i++)
     for (int k = 0; k < 3; k++) {
       const float d = v[3 * i + k] - lo[k];
@@ -226,7 +230,9 @@ HullOut exactHull(const float* v, uint32_t nv, const HullParams& hp) {
   Ctx C;
   C.v = v;
   C.nv = nv;
-  C.q = quantize(v, nv).q;
+  Grid G = quantize(v, nv);
+  C.q.swap(G.q);
+  C.cell3 = grid_cell3(G.scale);
   // seeds: max and min along each direction (first index on ties)
   uint32_t seed[80];
   for (int k = 0; k < 40; k++) {

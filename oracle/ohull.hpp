@@ -9,8 +9,9 @@
 // Every tie is broken by an order-independent rule (largest distance, then
 // lowest point index; owner face = first face in creation order), so a
 // parallel implementation that follows the same rules reproduces the same
-// face list exactly. The volume is taken over the ORIGINAL float coordinates
-// of the hull triangles with error-free products and exact accumulation.
+// face list exactly. The volume is the exact volume of the grid polytope
+// (integer determinants, int128 sum) times the cell volume: it depends only
+// on the point set, not on the order of insertion or the triangulation.
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -38,6 +39,7 @@ HullOut exactHull(const float* v, uint32_t nv, const HullParams& hp);
 // sign of det[b-a; c-a; p-a] on the quantization grid of the point set
 struct Grid {
   std::vector<int32_t> q;  // 3 per point
+  float scale = 0;         // largest bbox extent: the grid has 2^21 cells across it
   bool ok;
 };
 Grid quantize(const float* v, uint32_t nv);

@@ -57,6 +57,9 @@ def test_l_shape_best_plane(cw, oracle):
 
 
 def test_offcentre_box_hull_equals_mesh(cw, oracle):
+    """tests/gpu_tree.test.ts:81-106: every candidate part of an off-centre box is a box, so its hull
+    volume is its mesh volume up to the 2^21 grid (the reference allows 1e-4); here it also equals the
+    grid box's volume exactly: the product of the quantized extents times the cell volume."""
     box = cw.make_box((0.2, -0.7, 0.1), (0.9, 0.3, 0.8))
     total = cw.volume(box)  # of the float32 corners
     t = tree(cw, oracle, width=6, width2=3, depth=2, n_concave_edges=0, min_edge_dist=0.01)
@@ -68,7 +71,11 @@ def test_offcentre_box_hull_equals_mesh(cw, oracle):
             s = 0.0
             for p in range(it["np"]):
                 d = t.read_part(it["part_ids"][p])
-                assert abs(d["hull_vol"] - d["mesh_vol"]) < 1e-9
+                q, cell = _quantize(d["verts"], with_cell=True)
+                ext = [int(x) for x in q.max(0) - q.min(0)]
+                want = float(ext[0] * ext[1] * ext[2]) * cell ** 3
+                assert abs(d["hull_vol"] - want) <= 1e-12 * want
+                assert abs(d["hull_vol"] - d["mesh_vol"]) < 1e-5
                 s += d["mesh_vol"]
             assert abs(s - total) < 1e-9
 
@@ -132,13 +139,23 @@ def test_hull_known_answers(cw, oracle):
     assert cw.is_closed(cw.Mesh(park_miller_cloud(), tris))
 
 
-def _quantize(v):
+def _quantize(v, with_cell=False):
+    """The hull's 2^21 grid over the bbox (f32 arithmetic), and the cell edge scale / 2^21."""
     v = v.astype(np.float32)
     lo, hi = v.min(0), v.max(0)
     ext = (hi - lo).astype(np.float32)
     scale = max(float(ext.max()), 1e-30)
     qs = np.float32(2097152.0) / np.float32(scale)
-    return np.rint((v - lo).astype(np.float32) * qs).astype(np.int64)
+    q = np.rint((v - lo).astype(np.float32) * qs).astype(np.int64)
+    return (q, scale / 2097152.0) if with_cell else q
+
+
+def _grid_hull_volume(pts):
+    """Volume of the convex hull of the quantized points (scipy qhull, FP64 over exact integers)
+    times the cell volume."""
+    from scipy.spatial import ConvexHull
+    q, cell = _quantize(pts, with_cell=True)
+    return ConvexHull(q.astype(np.float64)).volume * cell ** 3
 
 
 def _exact_hull_vertices(q):
@@ -184,9 +201,10 @@ def test_hull_vertices_match_brute_force(cw, oracle, seed, n):
     q = _quantize(pts)
     assert _exact_hull_vertices(q) == used  # general position: hull vertices are exactly the extreme points
     assert used <= set(range(n))
-    # volume equals the exact polytope volume over the original coordinates (scipy qhull, FP64)
+    # volume equals the grid polytope's (scipy qhull, FP64), and the original one's up to the grid
     from scipy.spatial import ConvexHull
-    assert abs(vol - ConvexHull(pts.astype(np.float64)).volume) < 1e-9 * vol
+    assert abs(vol - _grid_hull_volume(pts)) < 1e-12 * vol
+    assert abs(vol - ConvexHull(pts.astype(np.float64)).volume) < 1e-5 * vol
 
 
 def test_hull_against_qhull_on_clouds(cw, oracle):
@@ -197,9 +215,28 @@ def test_hull_against_qhull_on_clouds(cw, oracle):
         tris, vol = _hull(cw, oracle, pts)
         h = ConvexHull(pts.astype(np.float64))
         assert set(int(x) for x in tris.reshape(-1)) == set(int(x) for x in h.vertices)
-        assert abs(vol - h.volume) < 1e-9 * h.volume
+        assert abs(vol - _grid_hull_volume(pts)) < 1e-12 * vol
+        assert abs(vol - h.volume) < 1e-5 * h.volume
         m = cw.Mesh(pts, tris)
         assert cw.is_closed(m) and cw.signed_volume(m) > 0
+
+
+def test_hull_volume_independent_of_order(cw, oracle):
+    """The grid volume is exact, so it does not depend on which triangulation of the coplanar facets the
+    insertion order produced: a lattice cube (many coplanar points per facet) is exactly 1, and every
+    permutation of a cloud with coplanar clusters gives the same bits."""
+    g = np.linspace(0.0, 1.0, 5, dtype=np.float32)
+    lattice = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    assert _hull(cw, oracle, lattice)[1] == 1.0
+    rng = np.random.default_rng(5)
+    pts = np.concatenate([lattice * np.float32(0.7), rng.random((200, 3)).astype(np.float32) * np.float32(0.7)])
+    vols = set()
+    for k in range(4):
+        perm = rng.permutation(len(pts)) if k else np.arange(len(pts))
+        tris, vol = _hull(cw, oracle, pts[perm])
+        vols.add(vol)
+    assert len(vols) == 1
+    assert abs(vols.pop() - _grid_hull_volume(pts)) < 1e-12
 
 
 def test_hull_degenerate_inputs(cw, oracle):
@@ -232,6 +269,46 @@ def test_hausdorff_invariants(cw, oracle):
     d = t.hausdorff([(a, b), (b, a), (a, a)])
     assert d[0] == d[1] and 0.0 < d[0] < 2.0
     assert d[2] < 1e-6
+
+
+def _seam_filtered(cw, part, tol=1e-3):
+    """The merged part's mesh without the triangles on its seam planes (what the merge step measures
+    against the merged hull, src/merge.ts filtered concatenation)."""
+    V = part.mesh.verts.astype(np.float64)
+    keep = [t for t in part.mesh.tris
+            if not any(all(abs(pl[0] * V[i][0] + pl[1] * V[i][1] + pl[2] * V[i][2] + pl[3]) <= tol for i in t)
+                       for pl in part.seams)]
+    return cw.Mesh(part.mesh.verts, np.array(keep, dtype=np.uint32).reshape(-1, 3))
+
+
+def test_hausdorff_bvh_equals_brute_force(cw, oracle):
+    """The oracle's BVH only skips boxes that cannot hold the nearest triangle after f32 rounding: its
+    distances equal a brute-force evaluation (every sample against every triangle) bit for bit, also
+    for a sample set lying within 1e-3 of large faces (the merged torus part against its hull) and
+    for sliver triangles."""
+    import hd_brute
+    t = tree(cw, oracle, width=3, width2=3, depth=1, n_concave_edges=0)
+    opts = dict(max_iters=6, threshold=0.05, width=12, width2=6, depth=2, n_concave_edges=8, vertex_cap=4 << 20,
+                tri_cap=4 << 20, edge_cap=1 << 18, scratch_words=4 << 20, hull_budget_bytes=256 << 20)
+    pairs = [(cw.normalize(stairs(cw))[0], cw.make_box((-0.9, -0.8, -0.7), (0.8, 0.9, 0.6)))]
+    r = cw.decompose(square_torus(cw), api=oracle, **opts)
+    merged = [p for p in r.parts if p.merged > 1 and p.seams]
+    assert merged
+    for p in r.parts:
+        pairs.append((p.mesh, p.hull))
+    for p in merged:
+        pairs.append((_seam_filtered(cw, p), p.hull))
+    # slivers: a strip of needle triangles 1e-3 above a flat plate
+    n = 40
+    xs = np.linspace(0, 1, n + 1, dtype=np.float32)
+    strip_v = np.concatenate([np.stack([xs, np.zeros_like(xs), np.full_like(xs, 1e-3)], 1),
+                              np.stack([xs, np.full_like(xs, 2e-3), np.full_like(xs, 1.3e-3)], 1)]).astype(np.float32)
+    strip_t = np.array([[i, i + 1, n + 1 + i] for i in range(n)] + [[i + 1, n + 2 + i, n + 1 + i] for i in range(n)],
+                       np.uint32)
+    plate = cw.make_box((-0.1, -0.1, -0.2), (1.1, 0.1, 0.0))
+    pairs.append((cw.Mesh(strip_v, strip_t), plate))
+    for a, b in pairs:
+        assert t.hausdorff([(a, b)])[0] == hd_brute.hausdorff(a, b)
 
 
 def test_wang_hash_known_values():
