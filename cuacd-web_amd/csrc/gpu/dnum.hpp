@@ -167,6 +167,10 @@ __device__ __forceinline__ float wave_max_f(float v) {
   }
   return v;
 }
+__device__ __forceinline__ float wave_sum_f(float v) {
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
 __device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) {
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
   return v;

@@ -189,6 +189,7 @@ class GpuBackend : public Backend {
   DevBuf<uint32_t> hdTris_, hdCounts_;
   DevBuf<dk::HdMesh> hdMeshes_;
   DevBuf<dk::HdTile> hdTiles_;
+  DevBuf<float4> hdRec_, hdChunks_;  // per-triangle distance records, per-chunk boxes
   struct Pending {
     int k;
     hipEvent_t a, b;
@@ -658,7 +659,7 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
   std::vector<float> pv;
   std::vector<uint32_t> pt, counts;
   std::vector<dk::HdMesh> hm(all.size());
-  uint64_t sTotal = 0, tTotal = 0;
+  uint64_t sTotal = 0, tTotal = 0, chTotal = 0;
   for (size_t k = 0; k < all.size(); k++) {
     const MeshRef& m = all[k];
     dk::HdMesh& M = hm[k];
@@ -668,6 +669,8 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
     M.gTri = (uint32_t)tTotal;
     M.sOff = (uint32_t)sTotal;
     M.cOff = (uint32_t)counts.size();
+    M.chOff = (uint32_t)chTotal;
+    chTotal += (m.nt + dk::HDC - 1) / dk::HDC;
     pv.insert(pv.end(), m.verts, m.verts + 3ull * m.nv);
     if (m.nt) pt.insert(pt.end(), m.tris, m.tris + 3ull * m.nt);
     const std::vector<uint32_t> off = m.nt ? sampleOffsets(m, cfg_.hausdorffSamples) : std::vector<uint32_t>(1, 0);
@@ -681,6 +684,8 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
   hdCounts_.ensure(counts.size());
   hdMeshes_.ensure(hm.size());
   hdSamples_.ensure(3 * sTotal);
+  hdRec_.ensure(8 * std::max<uint64_t>(tTotal, 1));
+  hdChunks_.ensure(2 * std::max<uint64_t>(chTotal, 1));
   h2d(hdVerts_.p, pv.data(), pv.size() * 4, "upload hd verts");
   if (!pt.empty()) h2d(hdTris_.p, pt.data(), pt.size() * 4, "upload hd tris");
   h2d(hdCounts_.p, counts.data(), counts.size() * 4, "upload counts");
@@ -691,6 +696,12 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
     dk::k_hd_samples<<<(uint32_t)((tTotal + 255) / 256), 256, 0, stream_>>>(hdMeshes_.p, (uint32_t)hm.size(), (uint32_t)tTotal,
                                                                           hdVerts_.p, hdTris_.p, hdCounts_.p, hdSamples_.p);
     launched("k_hd_samples");
+    dk::k_hd_prep<<<(uint32_t)((tTotal + 255) / 256), 256, 0, stream_>>>(hdMeshes_.p, (uint32_t)hm.size(), (uint32_t)tTotal,
+                                                                       hdVerts_.p, hdTris_.p, hdRec_.p);
+    launched("k_hd_prep");
+    dk::k_hd_chunks<<<(uint32_t)((chTotal + 3) / 4), 256, 0, stream_>>>(hdMeshes_.p, (uint32_t)hm.size(), (uint32_t)chTotal,
+                                                                      hdRec_.p, hdChunks_.p);
+    launched("k_hd_chunks");
     tEnd(K_HD_SAMPLES, a);
   }
   std::vector<dk::HdTile> tiles;
@@ -700,14 +711,14 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
       const dk::HdMesh& S = hm[dir == 0 ? p : n + p];
       const dk::HdMesh& O = hm[dir == 0 ? n + p : p];
       const uint32_t ns = counts[S.cOff + S.nt];
-      for (uint32_t s0 = 0; s0 < ns; s0 += dk::HDT) {
+      for (uint32_t s0 = 0; s0 < ns; s0 += dk::HDS) {
         dk::HdTile t;
         t.pair = (uint32_t)p;
         t.sBegin = S.sOff + s0;
-        t.sCount = std::min<uint32_t>(dk::HDT, ns - s0);
-        t.vOff = O.vOff;
-        t.tOff = O.tOff;
+        t.sCount = std::min<uint32_t>(dk::HDS, ns - s0);
+        t.gTri = O.gTri;
         t.nt = O.nt;
+        t.chOff = O.chOff;
         tiles.push_back(t);
       }
     }
@@ -718,7 +729,7 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
     h2d(hdTiles_.p, tiles.data(), tiles.size() * sizeof(dk::HdTile), "upload tiles");
     hipEvent_t a = tBegin();
     dk::k_hd_dist<<<(uint32_t)tiles.size(), dk::HDT, 0, stream_>>>(hdTiles_.p, (uint32_t)tiles.size(), hdSamples_.p,
-                                                                  hdVerts_.p, hdTris_.p, ctr_.p->hd, &ctr_.p->hdEvals);
+                                                                  hdRec_.p, hdChunks_.p, ctr_.p->hd, &ctr_.p->hdEvals);
     launched("k_hd_dist");
     tEnd(K_HD_DIST, a);
   }

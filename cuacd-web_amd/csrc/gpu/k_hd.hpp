@@ -2,8 +2,10 @@
 //
 // Samples: area-proportional counts per triangle (computed once on the host),
 // Wang-hash barycentrics (src/gpu/hausdorff_shader.ts:16-23, :88-99).
-// Distances: each block streams 256 samples of one mesh against the other
-// mesh's triangles, staged through LDS 256 triangles at a time; the directed
+// Distances: per-triangle quantities are computed once (k_hd_prep), chunks of
+// 64 triangles get a bounding box (k_hd_chunks), and a wave measures one
+// sample at a time against a chunk's 64 triangles (one per lane), visiting
+// only the chunks that could hold the sample's nearest triangle; the directed
 // maximum of the per-sample minimum squared distance goes to an atomicMax on
 // the pair (non-negative f32 bits order like their values).
 #pragma once
@@ -28,6 +30,7 @@ struct HdMesh {     // one mesh of a Hausdorff batch
   uint32_t gTri;    // first triangle in the batch-global triangle numbering
   uint32_t sOff;    // first sample (batch-global)
   uint32_t cOff;    // first entry of this mesh's count prefix (nt + 1 entries)
+  uint32_t chOff;   // first triangle chunk (batch-global)
 };
 
 // one thread per triangle of the batch: write its samples
@@ -75,97 +78,252 @@ __device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, f
 }
 __device__ __forceinline__ float fminx(float a, float b) { return b < a ? b : a; }
 
-__device__ __forceinline__ float seg_sq(float px, float py, float pz, float ax, float ay, float az, float bx, float by,
-                                        float bz) {
-  const float abx = bx - ax, aby = by - ay, abz = bz - az;
-  const float apx = px - ax, apy = py - ay, apz = pz - az;
-  const float ab2 = dot3f(abx, aby, abz, abx, aby, abz);
+// ---- per-triangle quantities, evaluated once with exactly the expressions of
+// the per-pair distance (the oracle evaluates them per pair), so the distance
+// is the same bit for bit.
+// Record layout (32 floats): a[0..2] b[3..5] c[6..8] ab[9..11] ac[12..14]
+// n[15..17] n2[18] d00[19] d01[20] d11[21] inv[22] bc[23..25] ca[26..28]
+// |bc|^2[29] |ca|^2[30] w[31] (the rounding weight used by the chunk test).
+constexpr int HDC = 64;  // triangles per chunk (one per lane when a chunk is staged)
+
+__device__ __forceinline__ float seg_sq_pre(float apx, float apy, float apz, float abx, float aby, float abz, float ab2) {
   float t = ab2 > 1e-20f ? dot3f(apx, apy, apz, abx, aby, abz) / ab2 : 0.0f;
   t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
   const float dx = apx - t * abx, dy = apy - t * aby, dz = apz - t * abz;
   return dot3f(dx, dy, dz, dx, dy, dz);
 }
 
-// src/gpu/hausdorff_shader.ts:33-51 / src/cpu/hausdorff.ts:79-115 point-triangle squared distance
-__device__ __forceinline__ float tri_sq(float px, float py, float pz, const float* t) {
-  const float ax = t[0], ay = t[1], az = t[2], bx = t[3], by = t[4], bz = t[5], cx = t[6], cy = t[7], cz = t[8];
-  const float abx = bx - ax, aby = by - ay, abz = bz - az;
-  const float acx = cx - ax, acy = cy - ay, acz = cz - az;
-  const float apx = px - ax, apy = py - ay, apz = pz - az;
-  const float nx = aby * acz - abz * acy, ny = abz * acx - abx * acz, nz = abx * acy - aby * acx;
-  const float n2 = dot3f(nx, ny, nz, nx, ny, nz);
+// point-triangle squared distance (src/gpu/hausdorff_shader.ts:33-51 /
+// src/cpu/hausdorff.ts:79-115) over a precomputed record: the plane
+// projection when it falls inside the triangle, else the nearest edge
+__device__ __forceinline__ float tri_sq_pre(float px, float py, float pz, const float* T) {
+  const float apx = px - T[0], apy = py - T[1], apz = pz - T[2];
+  const float n2 = T[18];
   if (n2 < 1e-20f) {
-    const float bpx = px - bx, bpy = py - by, bpz = pz - bz;
-    const float cpx = px - cx, cpy = py - cy, cpz = pz - cz;
+    const float bpx = px - T[3], bpy = py - T[4], bpz = pz - T[5];
+    const float cpx = px - T[6], cpy = py - T[7], cpz = pz - T[8];
     return fminx(dot3f(apx, apy, apz, apx, apy, apz),
                  fminx(dot3f(bpx, bpy, bpz, bpx, bpy, bpz), dot3f(cpx, cpy, cpz, cpx, cpy, cpz)));
   }
-  const float d00 = dot3f(abx, aby, abz, abx, aby, abz), d01 = dot3f(abx, aby, abz, acx, acy, acz),
-              d02 = dot3f(abx, aby, abz, apx, apy, apz);
-  const float d11 = dot3f(acx, acy, acz, acx, acy, acz), d12 = dot3f(acx, acy, acz, apx, apy, apz);
-  const float inv = 1.0f / (d00 * d11 - d01 * d01);
-  const float u = (d11 * d02 - d01 * d12) * inv;
-  const float v = (d00 * d12 - d01 * d02) * inv;
+  const float d02 = dot3f(T[9], T[10], T[11], apx, apy, apz);
+  const float d12 = dot3f(T[12], T[13], T[14], apx, apy, apz);
+  const float inv = T[22];
+  const float u = (T[21] * d02 - T[20] * d12) * inv;
+  const float v = (T[19] * d12 - T[20] * d02) * inv;
   if (u >= 0.0f && v >= 0.0f && u + v <= 1.0f) {
-    const float dist = dot3f(nx, ny, nz, apx, apy, apz);
+    const float dist = dot3f(T[15], T[16], T[17], apx, apy, apz);
     return (dist * dist) / n2;
   }
-  return fminx(seg_sq(px, py, pz, ax, ay, az, bx, by, bz),
-               fminx(seg_sq(px, py, pz, bx, by, bz, cx, cy, cz), seg_sq(px, py, pz, cx, cy, cz, ax, ay, az)));
+  const float bpx = px - T[3], bpy = py - T[4], bpz = pz - T[5];
+  const float cpx = px - T[6], cpy = py - T[7], cpz = pz - T[8];
+  return fminx(seg_sq_pre(apx, apy, apz, T[9], T[10], T[11], T[19]),
+               fminx(seg_sq_pre(bpx, bpy, bpz, T[23], T[24], T[25], T[29]),
+                     seg_sq_pre(cpx, cpy, cpz, T[26], T[27], T[28], T[30])));
 }
 
-struct HdTile {  // 256 samples of one mesh against all triangles of the other mesh of a pair
+// last mesh whose first entry (by `field`) is <= g
+template <typename F>
+__device__ __forceinline__ uint32_t hd_mesh_of(const HdMesh* __restrict__ meshes, uint32_t nMeshes, uint32_t g, F field) {
+  uint32_t lo = 0, hi = nMeshes;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (field(meshes[mid]) <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// one thread per triangle of the batch: its distance record
+__global__ void k_hd_prep(const HdMesh* __restrict__ meshes, uint32_t nMeshes, uint32_t nTriTotal,
+                          const float* __restrict__ verts, const uint32_t* __restrict__ tris, float4* __restrict__ rec) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nTriTotal) return;
+  const HdMesh M = meshes[hd_mesh_of(meshes, nMeshes, g, [](const HdMesh& m) { return m.gTri; })];
+  const uint32_t t = g - M.gTri;
+  if (t >= M.nt) return;
+  const float* V = verts + M.vOff;
+  const uint32_t* T = tris + M.tOff;
+  const float* A = V + 3 * T[3 * t];
+  const float* B = V + 3 * T[3 * t + 1];
+  const float* Cc = V + 3 * T[3 * t + 2];
+  const float ax = A[0], ay = A[1], az = A[2], bx = B[0], by = B[1], bz = B[2], cx = Cc[0], cy = Cc[1], cz = Cc[2];
+  const float abx = bx - ax, aby = by - ay, abz = bz - az;
+  const float acx = cx - ax, acy = cy - ay, acz = cz - az;
+  const float nx = aby * acz - abz * acy, ny = abz * acx - abx * acz, nz = abx * acy - aby * acx;
+  const float n2 = dot3f(nx, ny, nz, nx, ny, nz);
+  const float d00 = dot3f(abx, aby, abz, abx, aby, abz), d01 = dot3f(abx, aby, abz, acx, acy, acz);
+  const float d11 = dot3f(acx, acy, acz, acx, acy, acz);
+  const float inv = 1.0f / (d00 * d11 - d01 * d01);
+  const float bcx = cx - bx, bcy = cy - by, bcz = cz - bz;
+  const float cax = ax - cx, cay = ay - cy, caz = az - cz;
+  // rounding weight k^2 + k + 2 with k = |ab||ac|/|n| (2 for degenerate triangles)
+  double wgt = 2.0;
+  if (!(n2 < 1e-20f)) {
+    const double k = sqrt((double)d00) * sqrt((double)d11) / sqrt((double)n2);
+    wgt = k * k + k + 2.0;
+  }
+  float4* r = rec + 8 * (size_t)g;
+  r[0] = make_float4(ax, ay, az, bx);
+  r[1] = make_float4(by, bz, cx, cy);
+  r[2] = make_float4(cz, abx, aby, abz);
+  r[3] = make_float4(acx, acy, acz, nx);
+  r[4] = make_float4(ny, nz, n2, d00);
+  r[5] = make_float4(d01, d11, inv, bcx);
+  r[6] = make_float4(bcy, bcz, cax, cay);
+  r[7] = make_float4(caz, dot3f(bcx, bcy, bcz, bcx, bcy, bcz), dot3f(cax, cay, caz, cax, cay, caz), (float)(wgt * 1.0001));
+}
+
+// one wave per chunk of HDC consecutive triangles of one mesh: bounding box and largest weight
+__global__ void k_hd_chunks(const HdMesh* __restrict__ meshes, uint32_t nMeshes, uint32_t nChunks,
+                            const float4* __restrict__ rec, float4* __restrict__ chunks) {
+  const uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (c >= nChunks) return;  // whole waves
+  const HdMesh M = meshes[hd_mesh_of(meshes, nMeshes, c, [](const HdMesh& m) { return m.chOff; })];
+  const uint32_t t = (c - M.chOff) * HDC + lane_id();
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float wmax = 0.0f;
+  if (t < M.nt) {
+    const float4* r = rec + 8 * ((size_t)M.gTri + t);
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r7 = r[7];
+    const float x[3] = {r0.x, r0.w, r1.z}, y[3] = {r0.y, r1.x, r1.w}, z[3] = {r0.z, r1.y, r2.x};
+    for (int k = 0; k < 3; k++) {
+      lo[0] = fminf(lo[0], x[k]);
+      hi[0] = fmaxf(hi[0], x[k]);
+      lo[1] = fminf(lo[1], y[k]);
+      hi[1] = fmaxf(hi[1], y[k]);
+      lo[2] = fminf(lo[2], z[k]);
+      hi[2] = fmaxf(hi[2], z[k]);
+    }
+    wmax = r7.w;
+  }
+  for (int k = 0; k < 3; k++) {
+    lo[k] = wave_min_f(lo[k]);
+    hi[k] = wave_max_f(hi[k]);
+  }
+  wmax = wave_max_f(wmax);
+  if (lane_id() == 0) {
+    chunks[2 * c] = make_float4(lo[0], lo[1], lo[2], wmax);
+    chunks[2 * c + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+  }
+}
+
+struct HdTile {  // HDS samples of one mesh against all triangles of the other mesh of a pair
   uint32_t pair, sBegin, sCount;  // samples [sBegin, sBegin + sCount) of the batch sample array
-  uint32_t vOff, tOff, nt;        // the other mesh
+  uint32_t gTri, nt, chOff;       // the other mesh: first triangle record, triangles, first chunk
 };
 
-constexpr int HDT = 256;
+// A chunk can hold no triangle nearer to p than sqrt(best) after f32 rounding:
+// its box is farther than sqrt(best) by 1e-4 w of the box's far distance. The
+// f32 distance departs from the true one by a few ulps of |p - a| times
+// k^2 + k + 2 (k the sliver factor), so the margin holds with a wide factor.
+__device__ __forceinline__ bool hd_chunk_far(float px, float py, float pz, const float4& lo, const float4& hi, float best) {
+  const double x = px, y = py, z = pz;
+  const double nx = fmax(0.0, fmax((double)lo.x - x, x - (double)hi.x));
+  const double ny = fmax(0.0, fmax((double)lo.y - y, y - (double)hi.y));
+  const double nz = fmax(0.0, fmax((double)lo.z - z, z - (double)hi.z));
+  const double fx = fmax(fabs(x - (double)lo.x), fabs(x - (double)hi.x));
+  const double fy = fmax(fabs(y - (double)lo.y), fabs(y - (double)hi.y));
+  const double fz = fmax(fabs(z - (double)lo.z), fabs(z - (double)hi.z));
+  const double lb = sqrt(nx * nx + ny * ny + nz * nz) - 1e-4 * (double)lo.w * sqrt(fx * fx + fy * fy + fz * fz);
+  return lb > 0.0 && lb * lb > (double)best;
+}
+
+constexpr int HDT = 256;  // threads per block
+constexpr int HDS = 64;   // samples per tile (HDS / 4 per wave: many waves in flight hide the loads)
+// Each wave takes HDS / 4 samples of its tile one at a time. For a sample, the 64
+// lanes hold the 64 triangles of a chunk (each lane reads its own distance
+// record); the sample's best distance is the wave minimum. The chunk whose
+// box is nearest the sample is taken first, then every chunk that the box
+// test cannot rule out against the best so far (re-tested after each chunk).
 __global__ __launch_bounds__(HDT) void k_hd_dist(const HdTile* __restrict__ tiles, uint32_t nTiles,
-                                                 const float* __restrict__ samples, const float* __restrict__ verts,
-                                                 const uint32_t* __restrict__ tris, uint32_t* __restrict__ result,
+                                                 const float* __restrict__ samples, const float4* __restrict__ rec,
+                                                 const float4* __restrict__ chunks, uint32_t* __restrict__ result,
                                                  unsigned long long* evals) {
   const uint32_t b = blockIdx.x;
   if (b >= nTiles) return;
   const HdTile Tl = tiles[b];
-  __shared__ float s_tri[HDT * 9];
   __shared__ float s_red[HDT / 64];
-  const uint32_t tid = threadIdx.x;
-  const bool have = tid < Tl.sCount;
-  float px = 0.f, py = 0.f, pz = 0.f;
-  if (have) {
-    const float* s = samples + 3 * ((size_t)Tl.sBegin + tid);
-    px = s[0];
-    py = s[1];
-    pz = s[2];
-  }
-  const float* V = verts + Tl.vOff;
-  const uint32_t* T = tris + Tl.tOff;
-  float best = 1e30f;
-  for (uint32_t base = 0; base < Tl.nt; base += HDT) {
-    const uint32_t t = base + tid;
-    if (t < Tl.nt) {
-      for (int k = 0; k < 3; k++) {
-        const float* p = V + 3 * T[3 * t + k];
-        s_tri[9 * tid + 3 * k] = p[0];
-        s_tri[9 * tid + 3 * k + 1] = p[1];
-        s_tri[9 * tid + 3 * k + 2] = p[2];
+  __shared__ unsigned long long s_ev[HDT / 64];
+  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  const uint32_t nch = (Tl.nt + HDC - 1) / HDC;
+  const float4* C4 = chunks + 2 * (size_t)Tl.chOff;
+  const float4* R4 = rec + 8 * (size_t)Tl.gTri;
+  float worst = 0.0f;
+  unsigned long long ev = 0;
+  constexpr uint32_t SPW = HDS / (HDT / 64);
+  const uint32_t s1 = (w + 1) * SPW < Tl.sCount ? (w + 1) * SPW : Tl.sCount;
+  for (uint32_t si = w * SPW; si < s1; si++) {
+    const float* sp = samples + 3 * ((size_t)Tl.sBegin + si);
+    const float px = sp[0], py = sp[1], pz = sp[2];
+    // squared distance from the sample to the box of chunk c (f32: ordering only)
+    auto boxd = [&](uint32_t c) {
+      const float4 lo = C4[2 * c], hi = C4[2 * c + 1];
+      const float dx = fmaxf(0.f, fmaxf(lo.x - px, px - hi.x)), dy = fmaxf(0.f, fmaxf(lo.y - py, py - hi.y));
+      const float dz = fmaxf(0.f, fmaxf(lo.z - pz, pz - hi.z));
+      return dx * dx + dy * dy + dz * dz;
+    };
+    // the distances from the sample to the triangles of chunk c, one per lane
+    auto chunk_min = [&](uint32_t c) {
+      const uint32_t t = c * HDC + lane;
+      float d = INFINITY;
+      if (t < Tl.nt) {
+        float T[32];
+        const float4* r = R4 + 8 * (size_t)t;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const float4 v = r[q];
+          T[4 * q] = v.x;
+          T[4 * q + 1] = v.y;
+          T[4 * q + 2] = v.z;
+          T[4 * q + 3] = v.w;
+        }
+        d = tri_sq_pre(px, py, pz, T);
+      }
+      ev += t < Tl.nt ? 1u : 0u;
+      return d;
+    };
+    float dmin = INFINITY;
+    uint32_t cmin = 0xffffffffu;
+    for (uint32_t c = lane; c < nch; c += 64) {
+      const float d = boxd(c);
+      if (d < dmin) {
+        dmin = d;
+        cmin = c;
       }
     }
-    __syncthreads();
-    const uint32_t n = Tl.nt - base < (uint32_t)HDT ? Tl.nt - base : (uint32_t)HDT;
-    if (have)
-      for (uint32_t j = 0; j < n; j++) best = fminx(best, tri_sq(px, py, pz, s_tri + 9 * j));
-    __syncthreads();
+    wave_argext<false>(dmin, cmin);
+    const uint32_t c0 = cmin < nch ? cmin : 0u;
+    float best = wave_ext_f<false>(chunk_min(c0));
+    for (uint32_t g0 = 0; g0 < nch; g0 += 64) {
+      const uint32_t c = g0 + lane;
+      uint64_t need = __ballot(c < nch && c != c0 && !hd_chunk_far(px, py, pz, C4[2 * c], C4[2 * c + 1], best));
+      while (need) {
+        const uint32_t k = (uint32_t)__ffsll((unsigned long long)need) - 1;
+        need &= need - 1;
+        const float d = wave_ext_f<false>(chunk_min(g0 + k));
+        if (d < best) {
+          best = d;
+          // a nearer triangle: drop the chunks it now rules out
+          need &= __ballot(c < nch && !hd_chunk_far(px, py, pz, C4[2 * c], C4[2 * c + 1], best));
+        }
+      }
+    }
+    worst = best > worst ? best : worst;
   }
-  float worst = have ? best : 0.0f;
-  worst = wave_max_f(worst);
-  if ((tid & 63) == 0) s_red[tid >> 6] = worst;
+  if (lane == 0) {
+    s_red[w] = worst;
+    s_ev[w] = ev;
+  }
   __syncthreads();
-  if (tid == 0) {
-    float w = s_red[0];
-    for (int k = 1; k < HDT / 64; k++) w = s_red[k] > w ? s_red[k] : w;
-    atomicMax(result + Tl.pair, __float_as_uint(w));
-    atomicAdd(evals, (unsigned long long)Tl.sCount * Tl.nt);
+  if (threadIdx.x == 0) {
+    float wv = s_red[0];
+    unsigned long long e = s_ev[0];
+    for (int k = 1; k < HDT / 64; k++) {
+      wv = s_red[k] > wv ? s_red[k] : wv;
+      e += s_ev[k];
+    }
+    atomicMax(result + Tl.pair, __float_as_uint(wv));
+    atomicAdd(evals, e);
   }
 }
 

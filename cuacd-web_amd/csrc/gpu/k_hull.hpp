@@ -41,7 +41,7 @@ namespace dk {
 
 constexpr int HW = 4;         // parts per block in the wave-per-part hull kernel
 constexpr int HBIG = 512;     // threads per part in the workgroup-per-part hull kernel
-constexpr uint32_t HBIG_NV = 2048;  // parts with at least this many points get a whole workgroup
+constexpr uint32_t HBIG_NV = 1024;  // parts with at least this many points get a whole workgroup
 constexpr uint32_t ALIVE = 0x80000000u;
 constexpr uint32_t FW = 20;                 // words per face record
 
@@ -772,7 +772,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     // faces in groups of VB per thread (every record of a group in flight at once),
     // tested with the f32 filter; faces it leaves undecided for some candidate wait
     // in a pending list and get their exact tests after the loop, one face per thread
-    constexpr uint32_t VB = 4;
+    constexpr uint32_t VB = Grp::G > 64 ? 2 : 4;  // fewer records in flight where registers are capped
     for (uint32_t f0 = t; f0 < C.nf; f0 += VB * Grp::G) {
       uint4 h0[VB], h1[VB], h2[VB];
 #pragma unroll
@@ -1461,9 +1461,11 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
                  ctr, &ctr->faceTests, &ctr->hullBytes);
 }
 
-// large parts: one workgroup of HBIG threads per part
+// large parts: one workgroup of HBIG threads per part. Rounds are latency-bound,
+// so two workgroups share a CU (4 waves per SIMD: at most 128 VGPRs; the LDS
+// batch is under half of the CU's 160 KB).
 template <bool FREE>
-__global__ __launch_bounds__(HBIG) void k_hull_big(const HullJob* __restrict__ hjobs, uint32_t nJobsH,
+__global__ __launch_bounds__(HBIG, 4) void k_hull_big(const HullJob* __restrict__ hjobs, uint32_t nJobsH,
                                                    const float* __restrict__ verts, uint32_t* __restrict__ arena,
                                                    DPart* __restrict__ parts, HullResultD* __restrict__ results,
                                                    uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items,
