@@ -888,7 +888,8 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     const uint32_t j = S.lj[e];
     if (!((act >> j) & 1u)) continue;
     const uint32_t v3[3] = {S.la[e], S.lb[e], S.lc[e]};
-    uint32_t flags = 0;
+    uint32_t flags = 0, cross = 0;
+    int tjs[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const unsigned long long twin = ((unsigned long long)v3[k == 2 ? 0 : k + 1] << 32) | v3[k];
@@ -903,13 +904,41 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
         if (hk == ~0ull) break;
         s = (s + 1) & (HT - 1);
       }
+      tjs[k] = tj;
       if (tj != (int)j) flags |= 1u << k;
+      if (tj > (int)j) cross |= 1u << k;
+    }
+    if (FREE && cross) {
+      // The face across this horizon edge is seen by a later candidate tj. A
+      // later candidate can see a new face (a, b, p_j) only if it sees one of
+      // the two old faces on edge ab (the new face's plane keeps every point
+      // below both of them below itself): a shared face is already a
+      // conflict, so these edges are the only places to check.
+      const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * S.lface[e]);
+      const uint4 w2 = r4[2], w3 = r4[3], w4 = r4[4];
+      const int32_t q3[3][3] = {{(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z},
+                                {(int32_t)w3.x, (int32_t)w3.y, (int32_t)w3.z},
+                                {(int32_t)w4.x, (int32_t)w4.y, (int32_t)w4.z}};
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        if (cross & (1u << k)) {
+          const int k1 = k == 2 ? 0 : k + 1;
+          const FaceRec R = face_make(v3[k], v3[k1], S.cpt[j], q3[k], q3[k1], S.cq[j]);
+          C.tests++;
+          if (face_eval_rec(R, S.cq[tjs[k]]) > 0.0f) atomicOr(&S.cutA, (Mask)1 << tjs[k]);
+        }
     }
     S.lflag[e] = (uint8_t)flags;
     atomicAdd(&S.nh[j], (uint32_t)__popc(flags));
     S.bkt[S.boff[j] + atomicAdd(&S.bcnt[j], 1u)] = (uint16_t)e;
   }
   g.sync();
+  // free mode: a candidate that sees a new face of an earlier one is dropped here,
+  // before any face is made; its horizon edges count for nothing
+  if (FREE) {
+    act &= ~S.cutA;
+    if (t < K && !((act >> t) & 1u)) S.nh[t] = 0;  // read after the barrier below
+  }
   // capacity of the face array for the new faces (candidates in order)
   {
     uint32_t off = 0;
@@ -965,31 +994,20 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       }
   }
   g.sync();
-  // a candidate that sees a face made earlier in the round: ordered, the round
-  // ends before it; free, it is dropped (its new faces die)
-  for (uint32_t j = 1; j < m; j++) {
-    if (!((act >> j) & 1u)) continue;
-    const uint32_t nEarlier = S.noff[j];
-    for (uint32_t rel = t; rel < nEarlier; rel += Grp::G) {
-      C.tests++;
-      const float dd = rel < NCAP ? face_eval_rec(S.nrec[rel], S.cq[j]) : face_eval(C, C.nf + rel, S.cq[j]);
-      if (dd > 0.0f) {
-        if (FREE) atomicOr(&S.cutA, (Mask)1 << j);
-        else atomicMin(&S.cutA, (Mask)j);
+  // ordered mode: a candidate that sees a face made earlier in the round ends the
+  // round before it (free mode made this check at the horizon)
+  Mask acc = act;
+  if (!FREE) {
+    for (uint32_t j = 1; j < m; j++) {
+      if (!((act >> j) & 1u)) continue;
+      const uint32_t nEarlier = S.noff[j];
+      for (uint32_t rel = t; rel < nEarlier; rel += Grp::G) {
+        C.tests++;
+        const float dd = rel < NCAP ? face_eval_rec(S.nrec[rel], S.cq[j]) : face_eval(C, C.nf + rel, S.cq[j]);
+        if (dd > 0.0f) atomicMin(&S.cutA, (Mask)j);
       }
     }
-  }
-  g.sync();
-  Mask acc;
-  if (FREE) {
-    const Mask drop = act & S.cutA;
-    acc = act & ~drop;
-    for (Mask d = drop; d; d &= d - 1) {
-      const uint32_t j = mask_ffs(d) - 1;
-      for (uint32_t rel = S.noff[j] + t; rel < S.noff[j] + S.nh[j]; rel += Grp::G)
-        C.F[(size_t)FW * (C.nf + rel)] &= ~ALIVE;
-    }
-  } else {
+    g.sync();
     m = m < (uint32_t)S.cutA ? m : (uint32_t)S.cutA;
     acc = low_mask<Mask>(m);
   }
