@@ -908,12 +908,13 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       if (tj != (int)j) flags |= 1u << k;
       if (tj > (int)j) cross |= 1u << k;
     }
-    if (FREE && cross) {
+    if (cross) {
       // The face across this horizon edge is seen by a later candidate tj. A
       // later candidate can see a new face (a, b, p_j) only if it sees one of
       // the two old faces on edge ab (the new face's plane keeps every point
       // below both of them below itself): a shared face is already a
-      // conflict, so these edges are the only places to check.
+      // conflict, so these edges are the only places to check. Free mode
+      // drops tj; ordered mode ends the round before the first such tj.
       const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * S.lface[e]);
       const uint4 w2 = r4[2], w3 = r4[3], w4 = r4[4];
       const int32_t q3[3][3] = {{(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z},
@@ -925,7 +926,10 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
           const int k1 = k == 2 ? 0 : k + 1;
           const FaceRec R = face_make(v3[k], v3[k1], S.cpt[j], q3[k], q3[k1], S.cq[j]);
           C.tests++;
-          if (face_eval_rec(R, S.cq[tjs[k]]) > 0.0f) atomicOr(&S.cutA, (Mask)1 << tjs[k]);
+          if (face_eval_rec(R, S.cq[tjs[k]]) > 0.0f) {
+            if (FREE) atomicOr(&S.cutA, (Mask)1 << tjs[k]);
+            else atomicMin(&S.cutA, (Mask)tjs[k]);
+          }
         }
     }
     S.lflag[e] = (uint8_t)flags;
@@ -933,11 +937,14 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     S.bkt[S.boff[j] + atomicAdd(&S.bcnt[j], 1u)] = (uint16_t)e;
   }
   g.sync();
-  // free mode: a candidate that sees a new face of an earlier one is dropped here,
-  // before any face is made; its horizon edges count for nothing
+  // a candidate that sees a new face of an earlier one: free mode drops it, ordered
+  // mode ends the round before it; this is settled before any face is made
   if (FREE) {
     act &= ~S.cutA;
     if (t < K && !((act >> t) & 1u)) S.nh[t] = 0;  // read after the barrier below
+  } else {
+    m = m < (uint32_t)S.cutA ? m : (uint32_t)S.cutA;
+    act = low_mask<Mask>(m);
   }
   // capacity of the face array for the new faces (candidates in order)
   {
@@ -994,23 +1001,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       }
   }
   g.sync();
-  // ordered mode: a candidate that sees a face made earlier in the round ends the
-  // round before it (free mode made this check at the horizon)
   Mask acc = act;
-  if (!FREE) {
-    for (uint32_t j = 1; j < m; j++) {
-      if (!((act >> j) & 1u)) continue;
-      const uint32_t nEarlier = S.noff[j];
-      for (uint32_t rel = t; rel < nEarlier; rel += Grp::G) {
-        C.tests++;
-        const float dd = rel < NCAP ? face_eval_rec(S.nrec[rel], S.cq[j]) : face_eval(C, C.nf + rel, S.cq[j]);
-        if (dd > 0.0f) atomicMin(&S.cutA, (Mask)j);
-      }
-    }
-    g.sync();
-    m = m < (uint32_t)S.cutA ? m : (uint32_t)S.cutA;
-    acc = low_mask<Mask>(m);
-  }
   // ---- reassignment of the points whose owner is visible, over the new faces
   // of that candidate (the candidate itself lies on them and finds none).
   // Collected first, then evaluated one point per thread.
