@@ -3,6 +3,8 @@ on one MI355X — the plain per-mesh GPU method.
 
 One step = one decompose(mesh, threshold 0.05, width 12, width2 6, depth 3, nConcaveEdges 16,
 refineRounds 2 x 8 probes, mergeHulls on, 3000 Hausdorff samples per part). Prints one JSON line.
+Beside it: the batched multi-mesh method against the plain one on the first meshes of the mixed
+workload (`batched_vs_plain`), and the CPU oracle on a bounded sample (`cpu_baseline`).
 """
 import argparse
 import ctypes
@@ -135,6 +137,34 @@ def cpu_baseline(cw, mesh, gpu_step_s, sample_iters):
                        f"full/sample ratio", cpu_sample_s=t_cpu_sample, gpu_sample_s=t_gpu_sample)
 
 
+MIX = dict(seed=20260400, options=dict(threshold=0.1, width=12, depth=3, merge_hulls=True))
+
+
+def batched_vs_plain(cw, n):
+    """The first n meshes of the 256-mesh mix (log-uniform 500-60000 triangles: blobs, staircases,
+    twisted tori), decomposed together (one tree run per iteration over every mesh's seeds, shared
+    hull and Hausdorff batches) and one by one; both wall times, and whether the results agree."""
+    meshes = [cw.generate("mix", MIX["seed"], [k]) for k in range(n)]
+    cw.decompose_batch(meshes[:4], **MIX["options"])  # warm
+    t0 = time.perf_counter()
+    rb = cw.decompose_batch(meshes, **MIX["options"])
+    tb = time.perf_counter() - t0
+    print(f"batched: {n} meshes in {tb:.2f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    rp = [cw.decompose(m, **MIX["options"]) for m in meshes]
+    tp = time.perf_counter() - t0
+    print(f"plain: {n} meshes in {tp:.2f} s", file=sys.stderr, flush=True)
+    same = all(a.cuts == b.cuts and len(a.parts) == len(b.parts) and
+               all(x.hull_vol == y.hull_vol and x.mesh_vol == y.mesh_vol and x.hausdorff == y.hausdorff
+                   for x, y in zip(a.parts, b.parts)) for a, b in zip(rb, rp))
+    return dict(workload=f"first {n} meshes of the 256-mesh mix (seed {MIX['seed']}), "
+                         f"{sum(m.nt for m in meshes)} tris", options=MIX["options"],
+                batched_meshes_per_s=n / tb, plain_meshes_per_s=n / tp, ratio=tp / tb,
+                candidate_hulls=sum(r.stats["candidate_hulls"] for r in rp),
+                batched_candidate_hulls_per_s=sum(r.stats["candidate_hulls"] for r in rp) / tb,
+                identical=same)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +172,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched", type=int, default=32, metavar="N",
+                    help="also time the first N meshes of the mix batched and plain (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (HBM counter calibration)")
     a = ap.parse_args()
@@ -187,6 +219,8 @@ def main():
         "roofline": roofline(sum_kernels(results)),
         "cpu_baseline": None,
     }
+    if a.batched:
+        line["batched_vs_plain"] = batched_vs_plain(cw, a.batched)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cw, mesh, step_s, a.cpu_sample_iters)
     print(json.dumps(line))

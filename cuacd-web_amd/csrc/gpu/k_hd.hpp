@@ -229,7 +229,7 @@ __device__ __forceinline__ bool hd_chunk_far(float px, float py, float pz, const
 }
 
 constexpr int HDT = 256;  // threads per block
-constexpr int HDS = 64;   // samples per tile (HDS / 4 per wave: many waves in flight hide the loads)
+constexpr int HDS = 4;    // samples per tile, one per wave: many waves in flight hide the record loads
 // Each wave takes HDS / 4 samples of its tile one at a time. For a sample, the 64
 // lanes hold the 64 triangles of a chunk (each lane reads its own distance
 // record); the sample's best distance is the wave minimum. The chunk whose
