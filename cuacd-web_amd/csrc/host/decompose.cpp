@@ -178,170 +178,6 @@ Mesh filteredConcat(const Mesh& a, const Mesh& b, const std::vector<Plane>& plan
   return m;
 }
 
-std::vector<Part> mergeHulls(Backend& be, const std::vector<Part>& input, const MergeOptions& o, MergeStats* statsOut) {
-  const double t0 = nowMs();
-  MergeStats stats;
-  std::vector<Part> parts = input;
-  std::unordered_map<uint64_t, Box3> boxes;
-  uint64_t nextId = 1u << 30;
-  for (Part& p : parts)
-    if (p.uid == 0) p.uid = nextId++;
-  auto boxOf = [&](const Part& p) -> const Box3& {
-    auto it = boxes.find(p.uid);
-    if (it == boxes.end()) it = boxes.emplace(p.uid, bbox(p.mesh)).first;
-    return it->second;
-  };
-  std::map<std::pair<uint64_t, uint64_t>, PairEval> cache;
-  auto key = [](const Part& a, const Part& b) {
-    return a.uid < b.uid ? std::make_pair(a.uid, b.uid) : std::make_pair(b.uid, a.uid);
-  };
-  const double halfT2 = (o.threshold / 2) * (o.threshold / 2);
-  for (uint32_t round = 0; round < o.maxRounds && parts.size() > 1; round++) {
-    stats.rounds = round + 1;
-    std::vector<std::pair<uint32_t, uint32_t>> todo, candidates;
-    for (uint32_t i = 0; i < parts.size(); i++) {
-      for (uint32_t j = i + 1; j < parts.size(); j++) {
-        const Part& A = parts[i];
-        const Part& B = parts[j];
-        if (cache.count(key(A, B))) {
-          candidates.push_back({i, j});
-          continue;
-        }
-        if (rv_cost(A.meshVol + B.meshVol, A.hullVol + B.hullVol) >= o.threshold) continue;
-        if (bboxGap2(boxOf(A), boxOf(B)) >= halfT2) continue;
-        todo.push_back({i, j});
-        candidates.push_back({i, j});
-      }
-    }
-    // merged hulls of the two hulls' vertex sets, batched
-    const double th0 = nowMs();
-    std::vector<PairEval> evals(todo.size());
-    std::vector<std::vector<float>> pointSets(todo.size());
-    for (size_t k = 0; k < todo.size(); k++) {
-      const Part& A = parts[todo[k].first];
-      const Part& B = parts[todo[k].second];
-      pointSets[k] = A.hull.verts;
-      pointSets[k].insert(pointSets[k].end(), B.hull.verts.begin(), B.hull.verts.end());
-    }
-    const size_t batch = 1024;
-    for (size_t s = 0; s < todo.size(); s += batch) {
-      const size_t e = std::min(todo.size(), s + batch);
-      std::vector<MeshRef> refs;
-      for (size_t k = s; k < e; k++) refs.push_back(MeshRef{pointSets[k].data(), (uint32_t)(pointSets[k].size() / 3), nullptr, 0});
-      const std::vector<HullResult> hr = be.hullMeshes(refs);
-      for (size_t k = s; k < e; k++) {
-        const HullResult& h = hr[k - s];
-        const Part& A = parts[todo[k].first];
-        const Part& B = parts[todo[k].second];
-        PairEval& ev = evals[k];
-        if (!h.ok) {
-          // a failed merged hull leaves the pair unmerged
-          stats.hullFallbacks++;
-          ev.cost = INFINITY;
-          continue;
-        }
-        ev.hullVol = h.volume;
-        const double rv = rv_cost(A.meshVol + B.meshVol, h.volume);
-        ev.cost = rv;
-        if (rv < o.threshold) {
-          ev.hull = compactHull(pointSets[k].data(), h.tris, nullptr);
-          ev.hasHull = true;
-        }
-      }
-    }
-    stats.hullMs += nowMs() - th0;
-    stats.pairsHulled += (uint32_t)todo.size();
-    // Hausdorff(filtered union, merged hull) for pairs still under the threshold
-    const double thd = nowMs();
-    std::vector<size_t> hdIdx;
-    std::vector<Mesh> hdA;
-    for (size_t k = 0; k < todo.size(); k++) {
-      PairEval& ev = evals[k];
-      if (!ev.hasHull) continue;
-      const Part& A = parts[todo[k].first];
-      const Part& B = parts[todo[k].second];
-      ev.hasSeam = seamPlane(A, B, o.planeTol, &ev.seam);
-      std::vector<Plane> planes;
-      if (ev.hasSeam) planes.push_back(ev.seam);
-      planes.insert(planes.end(), A.seams.begin(), A.seams.end());
-      planes.insert(planes.end(), B.seams.begin(), B.seams.end());
-      Mesh m = filteredConcat(A.mesh, B.mesh, planes, o.planeTol);
-      if (m.tris.empty()) continue;
-      hdIdx.push_back(k);
-      hdA.push_back(std::move(m));
-    }
-    for (size_t s = 0; s < hdIdx.size(); s += batch) {
-      const size_t e = std::min(hdIdx.size(), s + batch);
-      std::vector<MeshRef> ra, rb;
-      for (size_t q = s; q < e; q++) {
-        ra.push_back(refOf(hdA[q]));
-        rb.push_back(refOf(evals[hdIdx[q]].hull));
-      }
-      const std::vector<double> hd = be.hausdorff(ra, rb);
-      for (size_t q = s; q < e; q++) {
-        PairEval& ev = evals[hdIdx[q]];
-        ev.hd = hd[q - s];
-        ev.cost = std::max(ev.cost, ev.hd);
-        if (ev.cost >= o.threshold) ev.hasHull = false;
-      }
-    }
-    stats.hausdorffMs += nowMs() - thd;
-    stats.pairsHausdorff += (uint32_t)hdIdx.size();
-    for (size_t k = 0; k < todo.size(); k++) cache[key(parts[todo[k].first], parts[todo[k].second])] = std::move(evals[k]);
-    // greedy min-cost matching (stable by candidate order on equal costs)
-    struct Ranked {
-      uint32_t i, j;
-      const PairEval* ev;
-    };
-    std::vector<Ranked> ranked;
-    for (const auto& c : candidates) {
-      const PairEval& ev = cache[key(parts[c.first], parts[c.second])];
-      if (ev.cost < o.threshold && ev.hasHull) ranked.push_back({c.first, c.second, &ev});
-    }
-    std::stable_sort(ranked.begin(), ranked.end(), [](const Ranked& x, const Ranked& y) { return x.ev->cost < y.ev->cost; });
-    std::vector<uint8_t> used(parts.size(), 0);
-    std::vector<Part> next;
-    uint32_t merges = 0;
-    for (const Ranked& c : ranked) {
-      if (used[c.i] || used[c.j]) continue;
-      used[c.i] = used[c.j] = 1;
-      const Part& A = parts[c.i];
-      const Part& B = parts[c.j];
-      Part m;
-      m.mesh = mergeMeshes(A.mesh, B.mesh);
-      m.hull = c.ev->hull;
-      m.meshVol = A.meshVol + B.meshVol;
-      m.hullVol = c.ev->hullVol;
-      m.hasHausdorff = true;
-      m.hausdorff = c.ev->hd;
-      m.seams = A.seams;
-      m.seams.insert(m.seams.end(), B.seams.begin(), B.seams.end());
-      if (c.ev->hasSeam) m.seams.push_back(c.ev->seam);
-      m.merged = A.merged + B.merged;
-      m.uid = nextId++;
-      next.push_back(std::move(m));
-      merges++;
-    }
-    if (o.verbose)
-      std::fprintf(stderr, "[merge round %u] parts=%zu candidates=%zu hulled=%zu hausdorff=%zu merged=%u\n", round,
-                   parts.size(), candidates.size(), todo.size(), hdIdx.size(), merges);
-    if (merges == 0) break;
-    for (size_t i = 0; i < parts.size(); i++)
-      if (!used[i]) next.push_back(std::move(parts[i]));
-    parts.swap(next);
-    stats.merges += merges;
-  }
-  stats.totalMs = nowMs() - t0;
-  if (statsOut) *statsOut = stats;
-  return parts;
-}
-
-// ---- decomposer --------------------------------------------------------------
-
-Decomposer::Decomposer(Backend& be, const DecomposeOptions& o) : be_(be), o_(o) {
-  if (o.width % 3 != 0 || o.width2 % 3 != 0) throw std::invalid_argument("width and width2 must be multiples of 3");
-}
-
 unsigned hostThreads() {
   unsigned n = std::thread::hardware_concurrency();
   if (const char* e = std::getenv("OMP_NUM_THREADS")) {
@@ -379,6 +215,261 @@ void parallelFor(size_t n, F f) {
   if (err) std::rethrow_exception(err);
 }
 }  // namespace
+
+namespace {
+
+// one parts list going through the merge rounds
+struct MergeRun {
+  std::vector<Part> parts;
+  MergeStats stats;
+  std::unordered_map<uint64_t, Box3> boxes;
+  std::map<std::pair<uint64_t, uint64_t>, PairEval> cache;
+  uint64_t nextId = 1u << 30;
+  bool done = false;
+  // this round
+  std::vector<std::pair<uint32_t, uint32_t>> todo, candidates;
+  std::vector<PairEval> evals;
+  std::vector<std::vector<float>> pointSets;
+  std::vector<size_t> hdIdx;
+  std::vector<Mesh> hdA;
+
+  const Box3& boxOf(const Part& p) {
+    auto it = boxes.find(p.uid);
+    if (it == boxes.end()) it = boxes.emplace(p.uid, bbox(p.mesh)).first;
+    return it->second;
+  }
+  static std::pair<uint64_t, uint64_t> key(const Part& a, const Part& b) {
+    return a.uid < b.uid ? std::make_pair(a.uid, b.uid) : std::make_pair(b.uid, a.uid);
+  }
+};
+
+}  // namespace
+
+// Merge rounds over several parts lists at once: every round's merged hulls of all
+// lists go to the device in shared batches, then all their Hausdorff pairs; each
+// list's decisions are its own (src/merge.ts:170-303), so every list ends exactly
+// as it would alone.
+std::vector<std::vector<Part>> mergeHullsMany(Backend& be, std::vector<std::vector<Part>> inputs, const MergeOptions& o,
+                                              std::vector<MergeStats>* statsOut) {
+  const double t0 = nowMs();
+  std::vector<MergeRun> R(inputs.size());
+  for (size_t r = 0; r < R.size(); r++) {
+    R[r].parts = std::move(inputs[r]);
+    for (Part& p : R[r].parts)
+      if (p.uid == 0) p.uid = R[r].nextId++;
+  }
+  const double halfT2 = (o.threshold / 2) * (o.threshold / 2);
+  const size_t batch = 1024;
+  double hullMs = 0, hdMs = 0;
+  for (uint32_t round = 0; round < o.maxRounds; round++) {
+    std::vector<MergeRun*> act;
+    for (MergeRun& m : R)
+      if (!m.done && m.parts.size() > 1) act.push_back(&m);
+      else m.done = true;
+    if (act.empty()) break;
+    // candidate pairs of each list; the merged hull of each new pair's two hull vertex sets
+    for (MergeRun* m : act) {
+      m->stats.rounds = round + 1;
+      m->todo.clear();
+      m->candidates.clear();
+      const std::vector<Part>& parts = m->parts;
+      for (uint32_t i = 0; i < parts.size(); i++) {
+        for (uint32_t j = i + 1; j < parts.size(); j++) {
+          const Part& A = parts[i];
+          const Part& B = parts[j];
+          if (m->cache.count(MergeRun::key(A, B))) {
+            m->candidates.push_back({i, j});
+            continue;
+          }
+          if (rv_cost(A.meshVol + B.meshVol, A.hullVol + B.hullVol) >= o.threshold) continue;
+          if (bboxGap2(m->boxOf(A), m->boxOf(B)) >= halfT2) continue;
+          m->todo.push_back({i, j});
+          m->candidates.push_back({i, j});
+        }
+      }
+      m->evals.assign(m->todo.size(), PairEval());
+      m->pointSets.assign(m->todo.size(), {});
+      for (size_t k = 0; k < m->todo.size(); k++) {
+        const Part& A = parts[m->todo[k].first];
+        const Part& B = parts[m->todo[k].second];
+        m->pointSets[k] = A.hull.verts;
+        m->pointSets[k].insert(m->pointSets[k].end(), B.hull.verts.begin(), B.hull.verts.end());
+      }
+    }
+    const double th0 = nowMs();
+    {
+      std::vector<std::pair<MergeRun*, size_t>> items;
+      for (MergeRun* m : act)
+        for (size_t k = 0; k < m->todo.size(); k++) items.push_back({m, k});
+      for (size_t s0 = 0; s0 < items.size(); s0 += batch) {
+        const size_t e = std::min(items.size(), s0 + batch);
+        std::vector<MeshRef> refs;
+        for (size_t q = s0; q < e; q++) {
+          const std::vector<float>& ps = items[q].first->pointSets[items[q].second];
+          refs.push_back(MeshRef{ps.data(), (uint32_t)(ps.size() / 3), nullptr, 0});
+        }
+        const std::vector<HullResult> hr = be.hullMeshes(refs);
+        parallelFor(e - s0, [&](size_t qq) {
+          const size_t q = s0 + qq;
+          MergeRun* m = items[q].first;
+          const size_t k = items[q].second;
+          const HullResult& h = hr[q - s0];
+          const Part& A = m->parts[m->todo[k].first];
+          const Part& B = m->parts[m->todo[k].second];
+          PairEval& ev = m->evals[k];
+          if (!h.ok) {
+            // a failed merged hull leaves the pair unmerged
+            ev.cost = INFINITY;
+            return;
+          }
+          ev.hullVol = h.volume;
+          const double rv = rv_cost(A.meshVol + B.meshVol, h.volume);
+          ev.cost = rv;
+          if (rv < o.threshold) {
+            ev.hull = compactHull(m->pointSets[k].data(), h.tris, nullptr);
+            ev.hasHull = true;
+          }
+        });
+        for (size_t q = s0; q < e; q++)
+          if (!hr[q - s0].ok) items[q].first->stats.hullFallbacks++;
+      }
+    }
+    hullMs += nowMs() - th0;
+    // Hausdorff(filtered union, merged hull) for pairs still under the threshold
+    const double thd = nowMs();
+    {
+      std::vector<std::pair<MergeRun*, size_t>> items;
+      // seam planes and filtered unions of every pair under the threshold, on the host threads
+      std::vector<std::pair<MergeRun*, size_t>> under;
+      for (MergeRun* m : act) {
+        m->stats.pairsHulled += (uint32_t)m->todo.size();
+        for (size_t k = 0; k < m->todo.size(); k++)
+          if (m->evals[k].hasHull) under.push_back({m, k});
+      }
+      std::vector<Mesh> filtered(under.size());
+      parallelFor(under.size(), [&](size_t u) {
+        MergeRun* m = under[u].first;
+        const size_t k = under[u].second;
+        PairEval& ev = m->evals[k];
+        const Part& A = m->parts[m->todo[k].first];
+        const Part& B = m->parts[m->todo[k].second];
+        ev.hasSeam = seamPlane(A, B, o.planeTol, &ev.seam);
+        std::vector<Plane> planes;
+        if (ev.hasSeam) planes.push_back(ev.seam);
+        planes.insert(planes.end(), A.seams.begin(), A.seams.end());
+        planes.insert(planes.end(), B.seams.begin(), B.seams.end());
+        filtered[u] = filteredConcat(A.mesh, B.mesh, planes, o.planeTol);
+      });
+      for (MergeRun* m : act) {
+        m->hdIdx.clear();
+        m->hdA.clear();
+      }
+      for (size_t u = 0; u < under.size(); u++) {  // pair order within each list is kept
+        if (filtered[u].tris.empty()) continue;
+        MergeRun* m = under[u].first;
+        m->hdIdx.push_back(under[u].second);
+        m->hdA.push_back(std::move(filtered[u]));
+      }
+      for (MergeRun* m : act)
+        for (size_t q = 0; q < m->hdIdx.size(); q++) items.push_back({m, q});
+      for (size_t s0 = 0; s0 < items.size(); s0 += batch) {
+        const size_t e = std::min(items.size(), s0 + batch);
+        std::vector<MeshRef> ra, rb;
+        for (size_t q = s0; q < e; q++) {
+          MergeRun* m = items[q].first;
+          ra.push_back(refOf(m->hdA[items[q].second]));
+          rb.push_back(refOf(m->evals[m->hdIdx[items[q].second]].hull));
+        }
+        const std::vector<double> hd = be.hausdorff(ra, rb);
+        for (size_t q = s0; q < e; q++) {
+          MergeRun* m = items[q].first;
+          PairEval& ev = m->evals[m->hdIdx[items[q].second]];
+          ev.hd = hd[q - s0];
+          ev.cost = std::max(ev.cost, ev.hd);
+          if (ev.cost >= o.threshold) ev.hasHull = false;
+        }
+      }
+    }
+    hdMs += nowMs() - thd;
+    // each list: cache the new pairs, then greedy min-cost matching (stable by candidate order)
+    for (MergeRun* m : act) {
+      m->stats.pairsHausdorff += (uint32_t)m->hdIdx.size();
+      std::vector<Part>& parts = m->parts;
+      for (size_t k = 0; k < m->todo.size(); k++)
+        m->cache[MergeRun::key(parts[m->todo[k].first], parts[m->todo[k].second])] = std::move(m->evals[k]);
+      struct Ranked {
+        uint32_t i, j;
+        const PairEval* ev;
+      };
+      std::vector<Ranked> ranked;
+      for (const auto& c : m->candidates) {
+        const PairEval& ev = m->cache[MergeRun::key(parts[c.first], parts[c.second])];
+        if (ev.cost < o.threshold && ev.hasHull) ranked.push_back({c.first, c.second, &ev});
+      }
+      std::stable_sort(ranked.begin(), ranked.end(),
+                       [](const Ranked& x, const Ranked& y) { return x.ev->cost < y.ev->cost; });
+      std::vector<uint8_t> used(parts.size(), 0);
+      std::vector<Part> next;
+      uint32_t merges = 0;
+      for (const Ranked& c : ranked) {
+        if (used[c.i] || used[c.j]) continue;
+        used[c.i] = used[c.j] = 1;
+        const Part& A = parts[c.i];
+        const Part& B = parts[c.j];
+        Part mp;
+        mp.mesh = mergeMeshes(A.mesh, B.mesh);
+        mp.hull = c.ev->hull;
+        mp.meshVol = A.meshVol + B.meshVol;
+        mp.hullVol = c.ev->hullVol;
+        mp.hasHausdorff = true;
+        mp.hausdorff = c.ev->hd;
+        mp.seams = A.seams;
+        mp.seams.insert(mp.seams.end(), B.seams.begin(), B.seams.end());
+        if (c.ev->hasSeam) mp.seams.push_back(c.ev->seam);
+        mp.merged = A.merged + B.merged;
+        mp.uid = m->nextId++;
+        next.push_back(std::move(mp));
+        merges++;
+      }
+      if (o.verbose)
+        std::fprintf(stderr, "[merge round %u] parts=%zu candidates=%zu hulled=%zu hausdorff=%zu merged=%u\n", round,
+                     parts.size(), m->candidates.size(), m->todo.size(), m->hdIdx.size(), merges);
+      if (merges == 0) {
+        m->done = true;
+        continue;
+      }
+      for (size_t i = 0; i < parts.size(); i++)
+        if (!used[i]) next.push_back(std::move(parts[i]));
+      parts.swap(next);
+      m->stats.merges += merges;
+    }
+  }
+  std::vector<std::vector<Part>> out(R.size());
+  const double total = nowMs() - t0;
+  if (statsOut) statsOut->resize(R.size());
+  for (size_t r = 0; r < R.size(); r++) {
+    R[r].stats.hullMs = hullMs;
+    R[r].stats.hausdorffMs = hdMs;
+    R[r].stats.totalMs = total;
+    if (statsOut) (*statsOut)[r] = R[r].stats;
+    out[r] = std::move(R[r].parts);
+  }
+  return out;
+}
+
+std::vector<Part> mergeHulls(Backend& be, const std::vector<Part>& input, const MergeOptions& o, MergeStats* statsOut) {
+  std::vector<MergeStats> st;
+  std::vector<std::vector<Part>> out = mergeHullsMany(be, {input}, o, &st);
+  if (statsOut) *statsOut = st[0];
+  return std::move(out[0]);
+}
+
+// ---- decomposer --------------------------------------------------------------
+
+Decomposer::Decomposer(Backend& be, const DecomposeOptions& o) : be_(be), o_(o) {
+  if (o.width % 3 != 0 || o.width2 % 3 != 0) throw std::invalid_argument("width and width2 must be multiples of 3");
+}
+
 
 // one mesh of a batch: its parts, result and uid counter
 struct Decomposer::State {
@@ -628,18 +719,31 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
           std::fprintf(stderr, "[iter %u] mesh %u parts=%zu worst=%.4f\n", iter, m, S[m].parts.size(), worst[m]);
   }
   fillHausdorff(S);
-  for (State& s : S) {
-    if (o.mergeHulls && s.parts.size() > 1) {
-      const double tm0 = nowMs();
-      MergeOptions mo;
-      mo.threshold = o.threshold;
-      mo.maxRounds = o.mergeRounds;
-      mo.planeTol = o.mergePlaneTol;
-      mo.verbose = o.verbose;
-      s.parts = mergeHulls(be_, s.parts, mo, &s.R.merge);
-      s.R.merged = true;
-      st.mergeMs += nowMs() - tm0;
+  if (o.mergeHulls) {
+    // every mesh with more than one part merges, the lists sharing device batches
+    const double tm0 = nowMs();
+    MergeOptions mo;
+    mo.threshold = o.threshold;
+    mo.maxRounds = o.mergeRounds;
+    mo.planeTol = o.mergePlaneTol;
+    mo.verbose = o.verbose;
+    std::vector<uint32_t> who;
+    std::vector<std::vector<Part>> lists;
+    for (uint32_t m = 0; m < S.size(); m++)
+      if (S[m].parts.size() > 1) {
+        who.push_back(m);
+        lists.push_back(std::move(S[m].parts));
+      }
+    std::vector<MergeStats> mst;
+    std::vector<std::vector<Part>> merged = mergeHullsMany(be_, std::move(lists), mo, &mst);
+    for (size_t q = 0; q < who.size(); q++) {
+      S[who[q]].parts = std::move(merged[q]);
+      S[who[q]].R.merge = mst[q];
+      S[who[q]].R.merged = true;
     }
+    st.mergeMs += nowMs() - tm0;
+  }
+  for (State& s : S) {
     if (s.R.normalized) {
       const Normalization& nz = s.R.norm;
       for (Part& p : s.parts) {

@@ -122,6 +122,9 @@ bool seamPlane(const Part& a, const Part& b, double tol, Plane* out);
 Mesh filteredConcat(const Mesh& a, const Mesh& b, const std::vector<Plane>& planes, double tol);
 // src/merge.ts:170-303 mergeHulls
 std::vector<Part> mergeHulls(Backend& be, const std::vector<Part>& input, const MergeOptions& o, MergeStats* stats);
+// the same over several independent parts lists at once, their device work in shared batches
+std::vector<std::vector<Part>> mergeHullsMany(Backend& be, std::vector<std::vector<Part>> inputs, const MergeOptions& o,
+                                              std::vector<MergeStats>* stats);
 
 // src/decompose.ts:128-398 Decomposer
 class Decomposer {
