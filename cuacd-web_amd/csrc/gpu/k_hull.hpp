@@ -23,13 +23,14 @@
 // edges are found with an LDS hash of directed edges. A candidate whose
 // visible set does not fit in LDS is inserted by the one-at-a-time path.
 //
-// A face record (80 bytes) carries its plane for the f32 filter and the grid
+// A face record (five 16-byte words, each in an array of its own so scans read
+// contiguous memory) carries its plane for the f32 filter and the grid
 // coordinates of its three corners, and a candidate entry (16 bytes) carries
 // the point's grid coordinates, so exact tests, new faces and reassignments
 // need no lookups of point coordinates. Owner faces sit in a parallel array.
 //
 // Per-part global slab (u32 words, 256-byte aligned):
-//   faces[20 f] | fkey[2 f] | vmask[2 f] | vis[f] | map[f] | orph[8 c] | cand[4 c] |
+//   faces[5 x 4 f] | fkey[2 f] | vmask[2 f] | vis[f] | map[f] | orph[8 c] | cand[4 c] |
 //   cand2[4 c] | q[3 nv] | cown[c] | cown2[c],   f = 4 nv + 256, c = max(nv, 80).
 #pragma once
 #include <type_traits>
@@ -43,7 +44,7 @@ constexpr int HW = 4;         // parts per block in the wave-per-part hull kerne
 constexpr int HBIG = 512;     // threads per part in the workgroup-per-part hull kernel
 constexpr uint32_t HBIG_NV = 1024;  // parts with at least this many points get a whole workgroup
 constexpr uint32_t ALIVE = 0x80000000u;
-constexpr uint32_t FW = 20;                 // words per face record
+constexpr uint32_t FW = 20;                 // words per face record (five uint4 words, one array each)
 
 // batch shape: candidates per round, LDS visible-face capacity, edge-hash slots
 constexpr uint32_t KB_BIG = 16, LCAP_BIG = 512, HT_BIG = 4096, NCAP_BIG = 256;
@@ -287,6 +288,15 @@ struct HullCtx {
   double cell3;              // volume of one grid cell
 };
 
+// Face records live in five arrays of uint4 (word k of face f at F4[k * fmax + f]):
+//   0: (a | ALIVE, b, c, 0)   1: (n0, n1, n2, bound factor)   2: (qa, |n|)   3: (qb, 0)   4: (qc, 0)
+// so a wave scanning consecutive faces reads contiguous 16-byte words.
+__device__ __forceinline__ uint4* face_row(const HullCtx& C, int k) {
+  return reinterpret_cast<uint4*>(C.F) + (size_t)k * C.fmax;
+}
+__device__ __forceinline__ uint4 face_word(const HullCtx& C, uint32_t f, int k) { return face_row(C, k)[f]; }
+__device__ __forceinline__ uint32_t& face_a(const HullCtx& C, uint32_t f) { return C.F[4 * (size_t)f]; }  // a | ALIVE
+
 // order of outside points: larger distance first, then lower index
 __device__ __forceinline__ unsigned long long pkey(float d, uint32_t i) {
   return ((unsigned long long)__float_as_uint(d) << 32) | (uint32_t)(~i);
@@ -318,9 +328,8 @@ __device__ __forceinline__ FaceRec face_make(uint32_t a, uint32_t b, uint32_t c,
   return R;
 }
 __device__ __forceinline__ void face_store(HullCtx& C, uint32_t f, const FaceRec& R) {
-  uint4* r4 = reinterpret_cast<uint4*>(C.F + (size_t)FW * f);
 #pragma unroll
-  for (int k = 0; k < 5; k++) r4[k] = R.w[k];
+  for (int k = 0; k < 5; k++) face_row(C, k)[f] = R.w[k];
   C.fkey[f] = 0ull;
 }
 __device__ __forceinline__ void face_write_q(HullCtx& C, uint32_t f, uint32_t a, uint32_t b, uint32_t c,
@@ -349,10 +358,10 @@ __device__ __forceinline__ int orient_w(const uint4& w2, const uint4& w3, const 
   const int32_t qc[3] = {(int32_t)w4.x, (int32_t)w4.y, (int32_t)w4.z};
   return orient_q(qa, qb, qc, qp);
 }
-// exact orientation of qp against the face whose record starts at r (corners in words 8.., 12.., 16..)
-__device__ __forceinline__ int orient_rec(const uint32_t* r, const uint4& w2, const int32_t* qp) {
-  const uint4 w3 = reinterpret_cast<const uint4*>(r)[3];
-  const uint4 w4 = reinterpret_cast<const uint4*>(r)[4];
+// exact orientation of qp against face f (corner a given in w2, b and c read from words 3 and 4)
+__device__ __forceinline__ int orient_rec(const HullCtx& C, uint32_t f, const uint4& w2, const int32_t* qp) {
+  const uint4 w3 = face_word(C, f, 3);
+  const uint4 w4 = face_word(C, f, 4);
   const int32_t qa[3] = {(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z};
   const int32_t qb[3] = {(int32_t)w3.x, (int32_t)w3.y, (int32_t)w3.z};
   const int32_t qc[3] = {(int32_t)w4.x, (int32_t)w4.y, (int32_t)w4.z};
@@ -374,11 +383,10 @@ __device__ __forceinline__ int face_filter(const uint4& w1, const uint4& w2, con
   return det > bound ? 1 : (det < -bound ? -1 : 0);
 }
 
-// > 0: point qp strictly outside the face at r (f32 distance, 1e-30 when only
-// the exact test decided); <= 0: inside, on the plane, or a dead face
-__device__ __forceinline__ float face_eval_q(const uint32_t* r, const int32_t* qp) {
-  const uint4* r4 = reinterpret_cast<const uint4*>(r);
-  const uint4 w0 = r4[0], w1 = r4[1], w2 = r4[2];
+// > 0: point qp strictly outside face f (f32 distance, 1e-30 when only the
+// exact test decided); <= 0: inside, on the plane, or a dead face
+__device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, const int32_t* qp) {
+  const uint4 w0 = face_word(C, f, 0), w1 = face_word(C, f, 1), w2 = face_word(C, f, 2);
   if (!(w0.x & ALIVE)) return -1e30f;
   float det;
   const int s = face_filter(w1, w2, qp, &det);
@@ -387,7 +395,7 @@ __device__ __forceinline__ float face_eval_q(const uint32_t* r, const int32_t* q
     return det / (nl > 1e-30f ? nl : 1e-30f);
   }
   if (s < 0) return -1.0f;
-  return orient_rec(r, w2, qp) > 0 ? 1e-30f : -1.0f;
+  return orient_rec(C, f, w2, qp) > 0 ? 1e-30f : -1.0f;
 }
 __device__ __forceinline__ float face_eval_rec(const FaceRec& R, const int32_t* qp) {
   if (!(R.w[0].x & ALIVE)) return -1e30f;
@@ -399,9 +407,6 @@ __device__ __forceinline__ float face_eval_rec(const FaceRec& R, const int32_t* 
   }
   if (s < 0) return -1.0f;
   return orient_w(R.w[2], R.w[3], R.w[4], qp) > 0 ? 1e-30f : -1.0f;
-}
-__device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, const int32_t* qp) {
-  return face_eval_q(C.F + (size_t)FW * f, qp);
 }
 
 // best face among [f0, f1) in creation order (first on ties) for grid point qp
@@ -438,9 +443,8 @@ template <typename Grp>
 __device__ __forceinline__ double hull_volume(const Grp& g, const HullCtx& C) {
   i128 acc = 0;
   for (uint32_t f = g.tid(); f < C.nf; f += Grp::G) {
-    const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * f);
-    if (!(r4[0].x & ALIVE)) continue;
-    acc += (i128)grid_det(r4[2], r4[3], r4[4]);
+    if (!(face_a(C, f) & ALIVE)) continue;
+    acc += (i128)grid_det(face_word(C, f, 2), face_word(C, f, 3), face_word(C, f, 4));
   }
   acc = g.sum(acc);
   const i128 a = acc < 0 ? -acc : acc;
@@ -458,8 +462,7 @@ __device__ __forceinline__ void compact_faces(const Grp& g, HullCtx& C) {
     unsigned long long key = 0;
     bool alive = false;
     if (f < C.nf) {
-      const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * f);
-      for (int k = 0; k < 5; k++) r[k] = r4[k];
+      for (int k = 0; k < 5; k++) r[k] = face_word(C, f, k);
       key = C.fkey[f];
       alive = (r[0].x & ALIVE) != 0;
     }
@@ -468,8 +471,7 @@ __device__ __forceinline__ void compact_faces(const Grp& g, HullCtx& C) {
     g.sync();  // every thread holds its record before any thread overwrites a slot
     if (f < C.nf) C.map[f] = alive ? dst : NONE;
     if (alive) {
-      uint4* d4 = reinterpret_cast<uint4*>(C.F + (size_t)FW * dst);
-      for (int k = 0; k < 5; k++) d4[k] = r[k];
+      for (int k = 0; k < 5; k++) face_row(C, k)[dst] = r[k];
       C.fkey[dst] = key;
     }
     w += cnt;
@@ -542,14 +544,14 @@ __device__ __forceinline__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p
     uint32_t a = 0, b = 0;
     if (e < 3 * nvis) {
       const uint32_t vi = e / 3, k = e - 3 * vi;
-      const uint32_t* r = C.F + (size_t)FW * C.vis[vi];
-      const uint32_t fa = r[0] & ~ALIVE, fb = r[1], fc = r[2];
+      const uint4 r = face_word(C, C.vis[vi], 0);
+      const uint32_t fa = r.x & ~ALIVE, fb = r.y, fc = r.z;
       a = k == 0 ? fa : (k == 1 ? fb : fc);
       b = k == 0 ? fb : (k == 1 ? fc : fa);
       hor = true;
       for (uint32_t j = 0; j < nvis; j++) {
-        const uint32_t* q = C.F + (size_t)FW * C.vis[j];
-        const uint32_t ga = q[0] & ~ALIVE, gb = q[1], gc = q[2];
+        const uint4 q = face_word(C, C.vis[j], 0);
+        const uint32_t ga = q.x & ~ALIVE, gb = q.y, gc = q.z;
         if ((ga == b && gb == a) || (gb == b && gc == a) || (gc == b && ga == a)) {
           hor = false;
           break;
@@ -563,7 +565,7 @@ __device__ __forceinline__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p
   }
   g.sync();
   for (uint32_t vi = t; vi < nvis; vi += Grp::G) {
-    C.F[(size_t)FW * C.vis[vi]] &= ~ALIVE;
+    face_a(C, C.vis[vi]) &= ~ALIVE;
     C.fkey[C.vis[vi]] = 0ull;
   }
   C.nf = f0 + nnew;
@@ -572,7 +574,7 @@ __device__ __forceinline__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p
   // points whose owner died see only the new faces
   for (uint32_t e = t; e < C.ncand; e += Grp::G) {
     const uint32_t o = C.cown[e];
-    if (o == NONE || (C.F[(size_t)FW * o] & ALIVE)) continue;
+    if (o == NONE || (face_a(C, o) & ALIVE)) continue;
     const uint4 ce = C.cand[e];
     const int32_t qi[3] = {(int32_t)ce.y, (int32_t)ce.z, (int32_t)ce.w};
     float bd;
@@ -779,10 +781,9 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       for (uint32_t b = 0; b < VB; b++) {
         const uint32_t f = f0 + b * Grp::G;
         if (f < C.nf) {
-          const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * f);
-          h0[b] = r4[0];
-          h1[b] = r4[1];
-          h2[b] = r4[2];
+          h0[b] = face_word(C, f, 0);
+          h1[b] = face_word(C, f, 1);
+          h2[b] = face_word(C, f, 2);
         } else {
           h0[b] = make_uint4(0u, 0u, 0u, 0u);
         }
@@ -807,10 +808,9 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
             S.pund[pp] = und;
             continue;
           }
-          const uint32_t* r = C.F + (size_t)FW * f;
           for (Mask u = und; u; u &= u - 1) {
             const uint32_t j = mask_ffs(u) - 1;
-            if (orient_rec(r, h2[b], S.cq[j]) > 0) mask |= (Mask)1 << j;
+            if (orient_rec(C, f, h2[b], S.cq[j]) > 0) mask |= (Mask)1 << j;
           }
         }
         finish(f, mask, h0[b].x & ~ALIVE, h0[b].y, h0[b].z);
@@ -821,12 +821,11 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       const uint32_t np = S.nPend < LCAP / 2 ? S.nPend : LCAP / 2;
       for (uint32_t p = t; p < np; p += Grp::G) {
         const uint32_t f = S.pface[p];
-        const uint32_t* r = C.F + (size_t)FW * f;
-        const uint4 w0 = reinterpret_cast<const uint4*>(r)[0], w2 = reinterpret_cast<const uint4*>(r)[2];
+        const uint4 w0 = face_word(C, f, 0), w2 = face_word(C, f, 2);
         Mask mask = S.pmask[p];
         for (Mask u = S.pund[p]; u; u &= u - 1) {
           const uint32_t j = mask_ffs(u) - 1;
-          if (orient_rec(r, w2, S.cq[j]) > 0) mask |= (Mask)1 << j;
+          if (orient_rec(C, f, w2, S.cq[j]) > 0) mask |= (Mask)1 << j;
         }
         finish(f, mask, w0.x & ~ALIVE, w0.y, w0.z);
       }
@@ -915,8 +914,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       // below both of them below itself): a shared face is already a
       // conflict, so these edges are the only places to check. Free mode
       // drops tj; ordered mode ends the round before the first such tj.
-      const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * S.lface[e]);
-      const uint4 w2 = r4[2], w3 = r4[3], w4 = r4[4];
+      const uint4 w2 = face_word(C, S.lface[e], 2), w3 = face_word(C, S.lface[e], 3), w4 = face_word(C, S.lface[e], 4);
       const int32_t q3[3][3] = {{(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z},
                                 {(int32_t)w3.x, (int32_t)w3.y, (int32_t)w3.z},
                                 {(int32_t)w4.x, (int32_t)w4.y, (int32_t)w4.z}};
@@ -974,8 +972,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     if (!((act >> j) & 1u)) continue;
     const uint32_t flags = S.lflag[e];
     if (!flags) continue;
-    const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * S.lface[e]);
-    const uint4 w2 = r4[2], w3 = r4[3], w4 = r4[4];
+    const uint4 w2 = face_word(C, S.lface[e], 2), w3 = face_word(C, S.lface[e], 3), w4 = face_word(C, S.lface[e], 4);
     uint32_t off = S.noff[j];
     {
       const uint32_t fe = S.lface[e], b0 = S.boff[j], nb = S.nvis[j];
@@ -1079,7 +1076,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   }
   for (uint32_t e = t; e < nL; e += Grp::G) {
     if (!((acc >> S.lj[e]) & 1u)) continue;
-    C.F[(size_t)FW * S.lface[e]] = S.la[e];  // first corner without the alive bit
+    face_a(C, S.lface[e]) = S.la[e];  // first corner without the alive bit
     C.fkey[S.lface[e]] = 0ull;
   }
   for (uint32_t q = t; q < no; q += Grp::G) {
@@ -1340,13 +1337,12 @@ __device__ void hull_job(const Grp& g, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S
         uint32_t ns = 0;
         for (uint32_t base = 0; base < C.nf; base += Grp::G) {
           const uint32_t f = base + t;
-          const uint4* r4 = reinterpret_cast<const uint4*>(C.F + (size_t)FW * f);
-          const bool alive = f < C.nf && (r4[0].x & ALIVE);
+          const bool alive = f < C.nf && (face_a(C, f) & ALIVE);
           uint32_t cnt;
           const uint32_t rk = g.rank(alive, &cnt);
           if (alive && ns + rk < SCAP) {
-            stage[3 * (ns + rk)] = r4[1];
-            stage[3 * (ns + rk) + 1] = r4[2];
+            stage[3 * (ns + rk)] = face_word(C, f, 1);
+            stage[3 * (ns + rk) + 1] = face_word(C, f, 2);
             stage[3 * (ns + rk) + 2] = make_uint4(f, 0u, 0u, 0u);
           }
           ns += cnt;
@@ -1377,7 +1373,7 @@ __device__ void hull_job(const Grp& g, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S
                 } else if (sg < 0) {
                   dd = -1.0f;
                 } else {
-                  dd = orient_rec(C.F + (size_t)FW * stage[3 * s + 2].x, w2, qi) > 0 ? 1e-30f : -1.0f;
+                  dd = orient_rec(C, stage[3 * s + 2].x, w2, qi) > 0 ? 1e-30f : -1.0f;
                 }
                 if (dd > 0.0f && dd > bd) {
                   bd = dd;
@@ -1410,15 +1406,15 @@ __device__ void hull_job(const Grp& g, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S
         uint32_t w = 0;
         for (uint32_t base = 0; base < C.nf; base += Grp::G) {
           const uint32_t f = base + t;
-          const uint32_t* r = C.F + (size_t)FW * f;
-          const bool alive = f < C.nf && (r[0] & ALIVE);
+          const bool alive = f < C.nf && (face_a(C, f) & ALIVE);
           uint32_t cnt;
           const uint32_t rk = g.rank(alive, &cnt);
           if (alive && J.exportOff != NONE) {
+            const uint4 r = face_word(C, f, 0);
             uint32_t* d = exportBuf + J.exportOff + 3 * (w + rk);
-            d[0] = r[0] & ~ALIVE;
-            d[1] = r[1];
-            d[2] = r[2];
+            d[0] = r.x & ~ALIVE;
+            d[1] = r.y;
+            d[2] = r.z;
           }
           w += cnt;
         }
