@@ -1099,11 +1099,13 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
 template <typename Grp, bool FREE, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
 __device__ __forceinline__ void hull_grow(const Grp& g, HullCtx& C, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S) {
   const uint32_t maxRounds = C.nv + 8;
-  uint32_t kEff = K;  // candidates looked at: about twice what the last round inserted
+  // candidates looked at: ordered, about twice what the last round inserted (later
+  // ones rarely make the prefix); free, always K (a round's cost is mostly fixed)
+  uint32_t kEff = K;
   for (uint32_t r = 0; r < maxRounds; r++) {
     const uint32_t ins = hull_round(g, C, S, kEff);
     if (ins == 0) return;
-    kEff = 2 * ins + 2;
+    kEff = FREE ? K : 2 * ins + 2;
     kEff = kEff < 4 ? 4 : (kEff > K ? K : kEff);
     if ((r & 3) == 3) compact_cand(g, C);
   }
@@ -1360,10 +1362,33 @@ __device__ void hull_job(const Grp& g, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S
             qi[0] = C.Q[3 * i];
             qi[1] = C.Q[3 * i + 1];
             qi[2] = C.Q[3 * i + 2];
+          }
+          // the grid box of this wave's points (integers, exact in f32)
+          float lo[3], hi[3];
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            lo[k] = wave_ext_f<false>(i < C.nv ? (float)qi[k] : 3e7f);
+            hi[k] = wave_ext_f<true>(i < C.nv ? (float)qi[k] : -3e7f);
+          }
+          if (i < C.nv) {
             if (staged) {
-              C.tests += ns;
               for (uint32_t s = 0; s < ns; s++) {
                 const uint4 w1 = stage[3 * s], w2 = stage[3 * s + 1];
+                {
+                  // every point of the box is strictly inside this face when its corner
+                  // farthest along the f32 normal is, by the filter's margin widened by the
+                  // box extent (the exact normal may pick another corner, at most that far off)
+                  const float n0 = __uint_as_float(w1.x), n1 = __uint_as_float(w1.y), n2 = __uint_as_float(w1.z);
+                  const float d0 = (n0 > 0.0f ? hi[0] : lo[0]) - (float)(int32_t)w2.x;
+                  const float d1 = (n1 > 0.0f ? hi[1] : lo[1]) - (float)(int32_t)w2.y;
+                  const float d2 = (n2 > 0.0f ? hi[2] : lo[2]) - (float)(int32_t)w2.z;
+                  float det = n0 * d0;
+                  det = det + n1 * d1;
+                  det = det + n2 * d2;
+                  const float sd = ((fabsf(d0) + fabsf(d1)) + fabsf(d2)) + ((hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]));
+                  if (det < -((1e-6f * __uint_as_float(w1.w)) * sd)) continue;
+                }
+                C.tests++;
                 float det;
                 const int sg = face_filter(w1, w2, qi, &det);
                 float dd;
