@@ -331,6 +331,7 @@ __device__ __forceinline__ void face_store(HullCtx& C, uint32_t f, const FaceRec
 #pragma unroll
   for (int k = 0; k < 5; k++) face_row(C, k)[f] = R.w[k];
   C.fkey[f] = 0ull;
+  C.vmask[f] = 0ull;
 }
 __device__ __forceinline__ void face_write_q(HullCtx& C, uint32_t f, uint32_t a, uint32_t b, uint32_t c,
                                              const int32_t* qa, const int32_t* qb, const int32_t* qc) {
@@ -752,8 +753,11 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   for (int pass = 0;; pass++) {
     // a face's visibility mask is final: record it, note conflicts, list it
     // under its first candidate
+    // Masks are zero outside a round: a round writes the faces some candidate sees
+    // and clears them at its commit (a second pass, after a list overflow, writes
+    // every face, since the first pass may have marked faces it no longer lists).
     auto finish = [&](uint32_t f, Mask mask, uint32_t a, uint32_t b, uint32_t c) {
-      C.vmask[f] = mask;
+      if (mask || pass > 0) C.vmask[f] = mask;
       if (!mask) return;
       const uint32_t j0 = mask_ffs(mask) - 1;
       const Mask rest = mask & (mask - 1);
@@ -842,6 +846,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     }
     g.sync();
     if (mc == 0 || pass > 0) {
+      for (uint32_t f = t; f < C.nf; f += Grp::G) C.vmask[f] = 0ull;
       hull_insert(g, C, S.cpt[0]);
       return 1;
     }
@@ -876,11 +881,13 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       S.hj[s] = S.lj[e];
     }
   }
-  if (t < K) {
-    uint32_t b = 0;
-    for (uint32_t i = 0; i < t; i++) b += S.nvis[i];
-    S.boff[t] = b;
-    S.bcnt[t] = 0;
+  if (t < 64) {  // bucket starts: exclusive prefix of the per-candidate list counts
+    const uint32_t v = lane < K ? S.nvis[lane] : 0u;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane < K) {
+      S.boff[lane] = incl - v;
+      S.bcnt[lane] = 0;
+    }
   }
   g.sync();
   for (uint32_t e = t; e < nL; e += Grp::G) {
@@ -939,29 +946,24 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   // mode ends the round before it; this is settled before any face is made
   if (FREE) {
     act &= ~S.cutA;
-    if (t < K && !((act >> t) & 1u)) S.nh[t] = 0;  // read after the barrier below
   } else {
     m = m < (uint32_t)S.cutA ? m : (uint32_t)S.cutA;
     act = low_mask<Mask>(m);
   }
-  // capacity of the face array for the new faces (candidates in order)
+  // new faces per taking-part candidate, prefix-summed across the wave (each wave
+  // computes the same): the first new face of each candidate, and the capacity
+  // of the face array (candidates in order)
   {
-    uint32_t off = 0;
-    for (uint32_t j = 0; j < m; j++) {
-      if (!((act >> j) & 1u)) continue;
-      if (C.nf + off + S.nh[j] > C.fmax) {
-        act &= low_mask<Mask>(j);
-        if (!FREE) m = j;
-        break;
-      }
-      off += S.nh[j];
+    const bool on = lane < K && lane < m && (((unsigned long long)act >> lane) & 1ull);
+    const uint32_t v = on ? S.nh[lane] : 0u;
+    const uint32_t incl = wave_incl_scan(v);
+    const uint64_t over = __ballot(on && C.nf + incl > C.fmax);
+    if (over) {
+      const uint32_t j = (uint32_t)__ffsll((unsigned long long)over) - 1;
+      act &= low_mask<Mask>(j);
+      if (!FREE) m = j;
     }
-  }
-  // first new face of each candidate: the faces of all earlier candidates that have any
-  if (t < K) {
-    uint32_t o = 0;
-    for (uint32_t i = 0; i < t; i++) o += S.nh[i];
-    S.noff[t] = o;
+    if (t < K) S.noff[t] = incl - v;  // read after the barrier below
   }
   g.sync();
   // new faces of candidate j, numbered (visible face in creation order, edge):
@@ -1066,15 +1068,17 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     acc = low_mask<Mask>(m);
   }
   // ---- commit the accepted candidates
-  uint32_t addF = 0, delF = 0, used = 0;
-  for (uint32_t j = 0; j < m; j++) {
-    used += (act >> j) & 1u ? S.nh[j] : 0u;
-    if ((acc >> j) & 1u) {
-      addF += S.nh[j];
-      delF += S.nvis[j];
-    }
+  uint32_t addF, delF, used;
+  {
+    const bool in = lane < K && lane < m;
+    const uint32_t nh = in ? S.nh[lane] : 0u, nv = in ? S.nvis[lane] : 0u;
+    const bool ac = in && (((unsigned long long)acc >> lane) & 1ull);
+    used = wave_sum_u(in && (((unsigned long long)act >> lane) & 1ull) ? nh : 0u);
+    addF = wave_sum_u(ac ? nh : 0u);
+    delF = wave_sum_u(ac ? nv : 0u);
   }
   for (uint32_t e = t; e < nL; e += Grp::G) {
+    C.vmask[S.lface[e]] = 0ull;  // every face with a mask is listed
     if (!((acc >> S.lj[e]) & 1u)) continue;
     face_a(C, S.lface[e]) = S.la[e];  // first corner without the alive bit
     C.fkey[S.lface[e]] = 0ull;
