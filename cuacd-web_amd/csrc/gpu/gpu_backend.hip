@@ -259,10 +259,10 @@ void GpuBackend::ensurePools() {
 void GpuBackend::runHulls(std::vector<dk::HullJob>& hjIn, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
                           std::vector<dk::HullResultD>* results) {
   if (hjIn.empty()) return;
-  // Tree candidates (no face export) need only the hull volume, which is exact
-  // on the grid and so the same for every insertion order: they insert in any
-  // valid order. Exported hulls keep the farthest-first face list.
-  const bool freeOrder = exportBuf == nullptr;
+  // Exported hulls take exactly the largest keys each round (their face list is
+  // the same for every group size); tree candidates need only the exact volume
+  // and take each wave's largest keys.
+  const bool exact = exportBuf != nullptr;
   // large parts first (a workgroup each), then small ones (a wavefront each)
   std::vector<uint32_t> order(hjIn.size());
   for (uint32_t k = 0; k < order.size(); k++) order[k] = k;
@@ -297,7 +297,7 @@ void GpuBackend::runHulls(std::vector<dk::HullJob>& hjIn, dk::DPart* parts, uint
     if (mid > start) {
       const uint32_t n = (uint32_t)(mid - start);
       hipEvent_t a = tBegin();
-      if (freeOrder)
+      if (exact)
         dk::k_hull_big<true><<<n, dk::HBIG, 0, stream_>>>(hjobs_.p + start, n, verts_.p, arena_.p, parts,
                                                           hres_.p + start, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
       else
@@ -310,7 +310,7 @@ void GpuBackend::runHulls(std::vector<dk::HullJob>& hjIn, dk::DPart* parts, uint
       const uint32_t n = (uint32_t)(end - mid);
       hipEvent_t a = tBegin(side_);
       const dim3 grid((n + dk::HW - 1) / dk::HW), block(64 * dk::HW);
-      if (freeOrder)
+      if (exact)
         dk::k_hull<true><<<grid, block, 0, side_>>>(hjobs_.p + mid, n, verts_.p, arena_.p, parts, hres_.p + mid,
                                                     exportBuf, items_.p, jobs_.p, HL, ctr_.p);
       else

@@ -1,27 +1,30 @@
-// k_hull.hpp — exact grid convex hull, batched farthest-first insertion (HIP, gfx950).
+// k_hull.hpp — exact grid convex hull, batched insertion rounds (HIP, gfx950).
 //
 // Points are quantized to a 2^21 grid over their bbox; 80 extreme seeds along
-// 40 directions give a simplex and a tight inner hull; every point outside it
-// is inserted farthest-first (largest distance, then lowest point index).
-// Orientation is decided by an f32 filter with an exact int128 fallback (the
-// reference's 96-bit fallback, src/gpu/shaders.ts:813-921), so the predicate
-// never contradicts itself. Branch-and-bound after the seed phase at the last
-// tree level (src/gpu/shaders.ts:1294-1331). The volume is that of the grid
-// polytope, exact (integer determinants, int128 sum), times the cell volume.
+// 40 directions give a simplex and a tight inner hull; the points outside it
+// are inserted in rounds. Orientation is decided by an f32 filter with an
+// exact int128 fallback (the reference's 96-bit fallback,
+// src/gpu/shaders.ts:813-921), so the predicate never contradicts itself.
+// Branch-and-bound after the seed phase at the last tree level
+// (src/gpu/shaders.ts:1294-1331). The volume is that of the grid polytope,
+// exact (integer determinants, int128 sum), times the cell volume, so it does
+// not depend on the order of insertion.
 //
-// Insertion runs in rounds. Each face keeps the key (distance, ~index) of its
-// farthest outside point, so the K largest face keys are the next K points of
-// the sequential order. A round inserts the longest prefix of them that the
-// sequential algorithm would insert identically:
-//   - their visible face sets are pairwise disjoint;
-//   - no point sees a face created earlier in the same round;
-//   - no point reassigned earlier in the round is farther than the next one.
-// Under these conditions every visible set, horizon and reassignment equals
-// the one-at-a-time result, and new faces are numbered in the same order
-// (candidate, visible face in creation order, edge), so the face list is the
-// sequential one bit for bit. Visible faces are collected in LDS; horizon
-// edges are found with an LDS hash of directed edges. A candidate whose
-// visible set does not fit in LDS is inserted by the one-at-a-time path.
+// Each face keeps the key (distance, ~index) of its farthest outside point. A
+// round takes the farthest points of the faces with the largest keys as
+// candidates and inserts those that can go in together, one after another in
+// candidate order, as if alone:
+//   - a candidate sharing a visible face with an earlier one waits;
+//   - a candidate seeing a face an earlier one creates waits.
+// New faces are numbered (candidate, visible face in creation order, edge),
+// and a point whose owner face is removed moves to the best new face of the
+// candidate that removed it. Exported hulls take exactly the 16 largest keys
+// and cap the visible-face list at 128, so their face list is the same for
+// both group sizes (and the oracle's); tree candidates, of which only the
+// volume is used, take the largest keys of each wave. Visible faces are
+// collected in LDS; horizon edges are found with an LDS hash of directed
+// edges. A candidate whose visible set does not fit in LDS is inserted by the
+// one-at-a-time path.
 //
 // A face record (five 16-byte words, each in an array of its own so scans read
 // contiguous memory) carries its plane for the f32 filter and the grid
@@ -46,9 +49,9 @@ constexpr uint32_t HBIG_NV = 1024;  // parts with at least this many points get 
 constexpr uint32_t ALIVE = 0x80000000u;
 constexpr uint32_t FW = 20;                 // words per face record (five uint4 words, one array each)
 
-// batch shape: candidates per round, LDS visible-face capacity, edge-hash slots
-constexpr uint32_t KB_BIG = 16, LCAP_BIG = 512, HT_BIG = 4096, NCAP_BIG = 256;
-constexpr uint32_t KB_W = 8, LCAP_W = 128, HT_W = 1024, NCAP_W = 48;
+// batch shape: LDS visible-face capacity, edge-hash slots, staged new faces
+constexpr uint32_t LCAP_BIG = 512, HT_BIG = 4096, NCAP_BIG = 256;
+constexpr uint32_t LCAP_W = 128, HT_W = 1024, NCAP_W = 48;
 
 __host__ __device__ inline uint32_t hullFmax(uint32_t nv) { return 4 * nv + 256; }
 __host__ __device__ inline uint64_t hullSlabWords(uint32_t nv) {
@@ -586,14 +589,13 @@ __device__ __forceinline__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p
   g.sync();
 }
 
-// group-shared state of one insertion round (FREE: free-order mode, see hull_round)
-template <bool FREE_, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+// group-shared state of one insertion round (EXACT: exported hulls, see hull_round)
+template <bool EXACT_, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
 struct HullBatch {
-  static constexpr bool FREE = FREE_;
+  static constexpr bool EXACT = EXACT_;
   using Mask = typename std::conditional<(K > 32), unsigned long long, uint32_t>::type;  // one bit per candidate
   FaceRec nrec[NCAP];  // the round's first new faces
-  unsigned long long ckey[K];  // candidate keys (descending in the ordered mode)
-  unsigned long long okey[K];  // largest reassigned key per candidate
+  unsigned long long ckey[K];  // candidate keys (descending when EXACT)
   uint32_t cpt[K];
   int32_t cq[K][3];
   uint32_t nvis[K], nh[K];                              // list entries and horizon edges per candidate
@@ -606,7 +608,7 @@ struct HullBatch {
   Mask pmask[LCAP / 2], pund[LCAP / 2];  // faces the f32 filter left undecided
   unsigned long long hkey[HT];                          // directed edges (a << 32 | b); seed-hull faces in phase 2
   uint8_t hj[HT];
-  unsigned long long wkey[NW][FREE ? K / NW : K];
+  unsigned long long wkey[NW][EXACT ? K : K / NW];
   unsigned long long wB[NW];
   uint32_t wcnt[NW];
   Mask conflict, cutA;
@@ -626,48 +628,57 @@ __device__ __forceinline__ uint32_t mask_ffs(unsigned long long m) { return (uin
 __device__ __forceinline__ uint32_t mask_popc(uint32_t m) { return (uint32_t)__popc(m); }
 __device__ __forceinline__ uint32_t mask_popc(unsigned long long m) { return (uint32_t)__popcll(m); }
 
+// list cap of exported hulls, the same for every group size (the rounds, and so
+// the face list, depend on it)
+constexpr uint32_t HULL_LIST_CAP = 128;
+
 // One insertion round; returns how many points it inserted (0: none left).
+// The round inserts a set of outside points whose insertion one after another,
+// in candidate order, changes only faces seen from the current hull: a
+// candidate that shares a visible face with an earlier one, or sees a face an
+// earlier one creates, waits for a later round; the others are inserted
+// together. New faces are numbered (candidate, visible face in creation order,
+// edge); a point whose owner face is removed moves to the best new face of the
+// candidate that removed it.
 //
-// Ordered mode (FREE = false): the next points of the farthest-first order,
-// the longest prefix of the K largest face keys that the one-at-a-time
-// algorithm would insert identically, so the face list is the sequential one.
-//
-// Free mode (FREE = true), for hulls of which only the volume is used: any
-// set of outside points whose insertion one after another (in candidate
-// order) changes only faces seen from the current hull. Candidates are the
-// largest keys of each wave; a candidate is dropped when it shares a visible
-// face with an earlier one or sees a face an earlier one creates. The hull
-// reached is the same polytope whatever the order, and its grid volume is
-// exact, so the volume is the ordered one bit for bit.
-template <typename Grp, bool FREE, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
-__device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S,
+// EXACT (exported hulls): candidates are the K largest face keys, and the
+// visible-face list is capped at HULL_LIST_CAP, so the rounds and the face
+// list are the same for every group size (the oracle restates them).
+// Otherwise (tree candidates, which use only the exact grid volume, the same
+// for every insertion order): candidates are the largest keys of each wave.
+template <typename Grp, bool EXACT, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+__device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& S,
                                                uint32_t kEff) {
   static_assert(K <= 64 && (HT & (HT - 1)) == 0 && HT >= 6 * LCAP && NW == Grp::NW && LCAP <= 65536, "batch shape");
-  static_assert(FREE ? (K % NW == 0 && K / NW <= 64) : (NW * K <= 128 && K <= 32), "batch shape");
-  using Mask = typename HullBatch<FREE, K, LCAP, HT, NW, NCAP>::Mask;
+  static_assert(EXACT ? (NW * K <= 128 && K <= 32 && LCAP >= HULL_LIST_CAP) : (K % NW == 0 && K / NW <= 64),
+                "batch shape");
+  using Mask = typename HullBatch<EXACT, K, LCAP, HT, NW, NCAP>::Mask;
+  constexpr uint32_t LC = EXACT ? HULL_LIST_CAP : LCAP;
   const uint32_t t = g.tid();
   const uint32_t lane = lane_id();
   const uint32_t w = NW == 1 ? 0u : (t >> 6);
-  if (C.nf > 2 * C.nalive + 128 || C.nf + C.nv + 8 > C.fmax) compact_faces(g, C);
+  // room for a round's new faces: each candidate's visible faces form a disk, whose
+  // boundary has at most two edges more than it has faces, so a round makes at most
+  // HULL_LIST_CAP + 2 K faces; compacted, the array has 2 nv + 256 slots free
+  if (C.nf > 2 * C.nalive + 128 || C.nf + C.nv + 168 > C.fmax) compact_faces(g, C);
   for (uint32_t s = t; s < HT; s += Grp::G) S.hkey[s] = ~0ull;
   if (t < K) {
     S.nvis[t] = 0;
     S.nh[t] = 0;
-    S.okey[t] = 0ull;
   }
   if (t == 0) {
     S.nList = 0;
     S.nOrph = 0;
     S.nPend = 0;
-    S.conflict = FREE ? (Mask)0 : (Mask)K;
-    S.cutA = FREE ? (Mask)0 : (Mask)K;
+    S.conflict = 0;
+    S.cutA = 0;
   }
-  // ---- candidates. Each lane offers its two largest face keys. Ordered: the
+  // ---- candidates. Each lane offers its two largest face keys. EXACT: the
   // third largest of every lane bounds everything not offered, so only keys
   // above that bound are taken (they are then exactly the largest ones, in
-  // order). Free: each wave takes its largest kEff / NW offered keys.
+  // order). Otherwise each wave takes its largest kEff / NW offered keys.
   // Candidate point numbers are the complemented low words of the keys.
-  const uint32_t kw = FREE ? (kEff + NW - 1) / NW : kEff;
+  const uint32_t kw = EXACT ? kEff : (kEff + NW - 1) / NW;
   unsigned long long k1 = 0, k2 = 0, k3 = 0;
   for (uint32_t f = t; f < C.nf; f += Grp::G) {
     const unsigned long long k = C.fkey[f];
@@ -692,14 +703,14 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       if (head == mx) ptr++;  // keys are distinct: one lane matches
       cnt++;
     }
-    const unsigned long long bw = FREE ? 0ull : wave_max_key(k3);
+    const unsigned long long bw = EXACT ? wave_max_key(k3) : 0ull;
     if (lane == 0) {
       S.wB[w] = bw;
       S.wcnt[w] = cnt;
     }
   }
   g.sync();
-  if (FREE) {
+  if (!EXACT) {
     if (t < 64) {
       // the waves' lists side by side (at most K entries)
       const uint32_t q = lane / kw, i = lane - q * kw;
@@ -761,10 +772,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       if (!mask) return;
       const uint32_t j0 = mask_ffs(mask) - 1;
       const Mask rest = mask & (mask - 1);
-      if (rest) {
-        if (FREE) atomicOr(&S.conflict, rest);
-        else atomicMin(&S.conflict, (Mask)(mask_ffs(rest) - 1));
-      }
+      if (rest) atomicOr(&S.conflict, rest);
       atomicAdd(&S.nvis[j0], 1u);
       const uint32_t pos = atomicAdd(&S.nList, 1u);
       if (pos < LCAP) {
@@ -835,12 +843,12 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       }
     }
     g.sync();
-    if (S.nList <= LCAP) break;
+    if (S.nList <= LC) break;
     // the visible faces do not all fit: keep the longest prefix of candidates
     // whose faces fit, and collect again
     uint32_t cum = 0, mc = 0;
     for (uint32_t j = 0; j < m; j++) {
-      if (cum + S.nvis[j] > LCAP) break;
+      if (cum + S.nvis[j] > LC) break;
       cum += S.nvis[j];
       mc = j + 1;
     }
@@ -855,19 +863,12 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     if (t == 0) {
       S.nList = 0;
       S.nPend = 0;
-      S.conflict = FREE ? (Mask)0 : (Mask)K;
+      S.conflict = 0;
     }
     g.sync();
   }
-  // candidates taking part: ordered, the prefix before the first shared face;
-  // free, those sharing no visible face with an earlier candidate
-  Mask act;
-  if (FREE) {
-    act = low_mask<Mask>(m) & ~S.conflict;
-  } else {
-    m = m < (uint32_t)S.conflict ? m : (uint32_t)S.conflict;
-    act = low_mask<Mask>(m);
-  }
+  // candidates taking part: those sharing no visible face with an earlier candidate
+  Mask act = low_mask<Mask>(m) & ~S.conflict;
   const uint32_t nL = S.nList;
   // ---- horizon: directed edges of visible faces whose twin is not visible from the same candidate
   for (uint32_t e = t; e < nL; e += Grp::G) {
@@ -919,8 +920,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       // later candidate can see a new face (a, b, p_j) only if it sees one of
       // the two old faces on edge ab (the new face's plane keeps every point
       // below both of them below itself): a shared face is already a
-      // conflict, so these edges are the only places to check. Free mode
-      // drops tj; ordered mode ends the round before the first such tj.
+      // conflict, so these edges are the only places to check.
       const uint4 w2 = face_word(C, S.lface[e], 2), w3 = face_word(C, S.lface[e], 3), w4 = face_word(C, S.lface[e], 4);
       const int32_t q3[3][3] = {{(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z},
                                 {(int32_t)w3.x, (int32_t)w3.y, (int32_t)w3.z},
@@ -931,10 +931,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
           const int k1 = k == 2 ? 0 : k + 1;
           const FaceRec R = face_make(v3[k], v3[k1], S.cpt[j], q3[k], q3[k1], S.cq[j]);
           C.tests++;
-          if (face_eval_rec(R, S.cq[tjs[k]]) > 0.0f) {
-            if (FREE) atomicOr(&S.cutA, (Mask)1 << tjs[k]);
-            else atomicMin(&S.cutA, (Mask)tjs[k]);
-          }
+          if (face_eval_rec(R, S.cq[tjs[k]]) > 0.0f) atomicOr(&S.cutA, (Mask)1 << tjs[k]);
         }
     }
     S.lflag[e] = (uint8_t)flags;
@@ -942,14 +939,8 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     S.bkt[S.boff[j] + atomicAdd(&S.bcnt[j], 1u)] = (uint16_t)e;
   }
   g.sync();
-  // a candidate that sees a new face of an earlier one: free mode drops it, ordered
-  // mode ends the round before it; this is settled before any face is made
-  if (FREE) {
-    act &= ~S.cutA;
-  } else {
-    m = m < (uint32_t)S.cutA ? m : (uint32_t)S.cutA;
-    act = low_mask<Mask>(m);
-  }
+  // a candidate that sees a new face of an earlier one waits (settled before any face is made)
+  act &= ~S.cutA;
   // new faces per taking-part candidate, prefix-summed across the wave (each wave
   // computes the same): the first new face of each candidate, and the capacity
   // of the face array (candidates in order)
@@ -958,11 +949,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     const uint32_t v = on ? S.nh[lane] : 0u;
     const uint32_t incl = wave_incl_scan(v);
     const uint64_t over = __ballot(on && C.nf + incl > C.fmax);
-    if (over) {
-      const uint32_t j = (uint32_t)__ffsll((unsigned long long)over) - 1;
-      act &= low_mask<Mask>(j);
-      if (!FREE) m = j;
-    }
+    if (over) act &= low_mask<Mask>((uint32_t)__ffsll((unsigned long long)over) - 1);  // never in EXACT rounds
     if (t < K) S.noff[t] = incl - v;  // read after the barrier below
   }
   g.sync();
@@ -1000,7 +987,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       }
   }
   g.sync();
-  Mask acc = act;
+  const Mask acc = act;
   // ---- reassignment of the points whose owner is visible, over the new faces
   // of that candidate (the candidate itself lies on them and finds none).
   // Collected first, then evaluated one point per thread.
@@ -1036,7 +1023,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   for (uint32_t q = t; q < no; q += Grp::G) {
     uint4* o4 = reinterpret_cast<uint4*>(C.orph) + 2 * q;
     const uint4 ov = o4[0], ce = o4[1];
-    const uint32_t j = ov.y, i = ce.x;
+    const uint32_t j = ov.y;
     const int32_t qi[3] = {(int32_t)ce.y, (int32_t)ce.z, (int32_t)ce.w};
     const uint32_t off = S.noff[j];
     const uint32_t r1 = off + S.nh[j];
@@ -1052,30 +1039,14 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     }
     o4->z = bf;
     o4->w = __float_as_uint(bd);
-    if (!FREE && bf != NONE) atomicMax(&S.okey[j], pkey(bd, i));
   }
   g.sync();
-  if (!FREE) {
-    // a reassigned point farther than the next candidate ends the round before it
-    unsigned long long run = 0;
-    for (uint32_t j = 0; j < m; j++) {
-      if (j > 0 && S.ckey[j] < run) {
-        m = j;
-        break;
-      }
-      run = S.okey[j] > run ? S.okey[j] : run;
-    }
-    acc = low_mask<Mask>(m);
-  }
   // ---- commit the accepted candidates
-  uint32_t addF, delF, used;
+  uint32_t addF, delF;
   {
-    const bool in = lane < K && lane < m;
-    const uint32_t nh = in ? S.nh[lane] : 0u, nv = in ? S.nvis[lane] : 0u;
-    const bool ac = in && (((unsigned long long)acc >> lane) & 1ull);
-    used = wave_sum_u(in && (((unsigned long long)act >> lane) & 1ull) ? nh : 0u);
-    addF = wave_sum_u(ac ? nh : 0u);
-    delF = wave_sum_u(ac ? nv : 0u);
+    const bool ac = lane < K && lane < m && (((unsigned long long)acc >> lane) & 1ull);
+    addF = wave_sum_u(ac ? S.nh[lane] : 0u);
+    delF = wave_sum_u(ac ? S.nvis[lane] : 0u);
   }
   for (uint32_t e = t; e < nL; e += Grp::G) {
     C.vmask[S.lface[e]] = 0ull;  // every face with a mask is listed
@@ -1089,9 +1060,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     C.cown[o4.x] = o4.z;
     if (o4.z != NONE) atomicMax(&C.fkey[o4.z], pkey(__uint_as_float(o4.w), reinterpret_cast<const uint4*>(C.orph)[2 * q + 1].x));
   }
-  // ordered: the accepted prefix's faces come first, later ones are overwritten;
-  // free: every written face keeps its slot (dropped candidates' faces are dead)
-  C.nf += FREE ? used : addF;
+  C.nf += addF;
   C.nalive = C.nalive + addF - delF;
   const uint32_t ins = mask_popc(acc);
   C.nIns += ins;
@@ -1099,18 +1068,12 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   return ins;
 }
 
-// insertion rounds until no point is outside (ordered: farthest first)
-template <typename Grp, bool FREE, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
-__device__ __forceinline__ void hull_grow(const Grp& g, HullCtx& C, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S) {
+// insertion rounds until no point is outside (K candidates every round)
+template <typename Grp, bool EXACT, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+__device__ __forceinline__ void hull_grow(const Grp& g, HullCtx& C, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& S) {
   const uint32_t maxRounds = C.nv + 8;
-  // candidates looked at: ordered, about twice what the last round inserted (later
-  // ones rarely make the prefix); free, always K (a round's cost is mostly fixed)
-  uint32_t kEff = K;
   for (uint32_t r = 0; r < maxRounds; r++) {
-    const uint32_t ins = hull_round(g, C, S, kEff);
-    if (ins == 0) return;
-    kEff = FREE ? K : 2 * ins + 2;
-    kEff = kEff < 4 ? 4 : (kEff > K ? K : kEff);
+    if (hull_round(g, C, S, K) == 0) return;
     if ((r & 3) == 3) compact_cand(g, C);
   }
 }
@@ -1120,10 +1083,9 @@ struct HullLevel {  // branch-and-bound inputs at the last tree level
 };
 
 // one hull job on one group; seed points in `seed` (80 entries of group-shared memory).
-// FREE: insertion in any valid order (the volume is exact either way; the face
-// list and insertion count are those of some valid order, not the sequential one)
-template <typename Grp, bool FREE, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
-__device__ void hull_job(const Grp& g, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S, const dk::HullJob& J, uint32_t hj, uint32_t* seed,
+// EXACT: exported hulls, whose face list is the same for every group size
+template <typename Grp, bool EXACT, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+__device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& S, const dk::HullJob& J, uint32_t hj, uint32_t* seed,
                          const float* __restrict__ verts,
                          uint32_t* __restrict__ arena, DPart* __restrict__ parts, HullResultD* __restrict__ results,
                          uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items, const DJob* __restrict__ jobs,
@@ -1473,11 +1435,11 @@ __device__ void hull_job(const Grp& g, HullBatch<FREE, K, LCAP, HT, NW, NCAP>& S
   }
 }
 
-// candidates per round in the free-order mode
-constexpr uint32_t KF_BIG = 32, KF_W = 16;
+// candidates per round: exported hulls (the same for both group sizes), tree candidates
+constexpr uint32_t KE = 16, KF_BIG = 32, KF_W = 16;
 
 // small parts: one wavefront per part, HW parts per block
-template <bool FREE>
+template <bool EXACT>
 __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hjobs, uint32_t nJobsH,
                                                   const float* __restrict__ verts, uint32_t* __restrict__ arena,
                                                   DPart* __restrict__ parts, HullResultD* __restrict__ results,
@@ -1487,9 +1449,9 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
   const uint32_t wid = threadIdx.x >> 6;
   const uint32_t hj = blockIdx.x * HW + wid;
   if (hj >= nJobsH) return;
-  constexpr uint32_t KW = FREE ? KF_W : KB_W;
+  constexpr uint32_t KW = EXACT ? KE : KF_W;
   __shared__ uint32_t s_seed[HW][80];
-  __shared__ HullBatch<FREE, KW, LCAP_W, HT_W, 1, NCAP_W> s_batch[HW];
+  __shared__ HullBatch<EXACT, KW, LCAP_W, HT_W, 1, NCAP_W> s_batch[HW];
   const WaveGrp g;
   hull_job(g, s_batch[wid], hjobs[hj], hj, s_seed[wid], verts, arena, parts, results, exportBuf, items, jobs, HL,
                  ctr, &ctr->faceTests, &ctr->hullBytes);
@@ -1498,7 +1460,7 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
 // large parts: one workgroup of HBIG threads per part. Rounds are latency-bound,
 // so two workgroups share a CU (4 waves per SIMD: at most 128 VGPRs; the LDS
 // batch is under half of the CU's 160 KB).
-template <bool FREE>
+template <bool EXACT>
 __global__ __launch_bounds__(HBIG, 4) void k_hull_big(const HullJob* __restrict__ hjobs, uint32_t nJobsH,
                                                    const float* __restrict__ verts, uint32_t* __restrict__ arena,
                                                    DPart* __restrict__ parts, HullResultD* __restrict__ results,
@@ -1507,13 +1469,13 @@ __global__ __launch_bounds__(HBIG, 4) void k_hull_big(const HullJob* __restrict_
                                                    Counters* __restrict__ ctr) {
   const uint32_t hj = blockIdx.x;
   if (hj >= nJobsH) return;
-  constexpr uint32_t KB = FREE ? KF_BIG : KB_BIG;
+  constexpr uint32_t KB = EXACT ? KE : KF_BIG;
   __shared__ uint32_t s_seed[80];
   __shared__ uint32_t s_u[HBIG / 64];
   __shared__ float s_f[HBIG / 64];
   __shared__ unsigned long long s_l[3 * (HBIG / 64)];
   __shared__ uint32_t s_m[4 * 16 * (HBIG / 64)];
-  __shared__ HullBatch<FREE, KB, LCAP_BIG, HT_BIG, HBIG / 64, NCAP_BIG> s_batch;
+  __shared__ HullBatch<EXACT, KB, LCAP_BIG, HT_BIG, HBIG / 64, NCAP_BIG> s_batch;
   BlockGrp<HBIG> g;
   g.su = s_u;
   g.sf = s_f;

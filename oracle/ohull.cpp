@@ -2,6 +2,8 @@
 #include "ohull.hpp"
 
 #include <algorithm>
+#include <array>
+#include <cstring>
 #include <cmath>
 
 #include "onum.hpp"
@@ -159,22 +161,147 @@ struct Ctx {
     }
   }
 
-  // farthest outstanding point (largest distance, then lowest index) until none
+  static uint64_t pkey(float d, uint32_t i) {
+    uint32_t b;
+    std::memcpy(&b, &d, 4);
+    return ((uint64_t)b << 32) | (uint32_t)(~i);
+  }
+  // > 0: p strictly outside the face (a, b, c) made as addFace would make it
+  float evalNew(uint32_t a, uint32_t b, uint32_t c, uint32_t p) {
+    addFace(a, b, c);
+    const float d = faceEval((uint32_t)faces.size() - 1, p);
+    faces.pop_back();
+    members.pop_back();
+    return d;
+  }
+
+  // The device's insertion rounds, restated one round at a time. A round's
+  // candidates are the farthest outside points of the K faces with the largest
+  // keys (key: larger distance, then lower point index), in key order. A
+  // candidate that shares a visible face with an earlier one, or that sees a new
+  // face of an earlier one that shares none, waits for a later round; the others
+  // are inserted together, in candidate order: new faces by (candidate, visible
+  // face in creation order, edge), each owned point of a removed face moving to
+  // the best new face of the candidate that removed it. When the visible faces
+  // of all candidates exceed LCAP, the round keeps the longest prefix of
+  // candidates whose first-seen faces fit, once; failing that it inserts the
+  // first candidate alone.
+  static constexpr uint32_t K = 16, LCAP = 128;
   void grow(const std::vector<uint32_t>& cand) {
     std::vector<uint32_t> c(cand);
     std::sort(c.begin(), c.end());
     c.erase(std::unique(c.begin(), c.end()), c.end());
     size_t sinceCompact = 0;
     for (;;) {
-      uint32_t pick = NONE;
-      float best = -1.0f;
+      const uint32_t nf = (uint32_t)faces.size();
+      std::vector<uint64_t> key(nf, 0);
       for (uint32_t i : c)
-        if (owner[i] != NONE && dist[i] > best) {  // ascending index: first max is the lowest index
-          best = dist[i];
-          pick = i;
+        if (owner[i] != NONE) key[owner[i]] = std::max(key[owner[i]], pkey(dist[i], i));
+      std::vector<uint64_t> top;
+      for (uint32_t f = 0; f < nf; f++)
+        if (key[f]) top.push_back(key[f]);
+      if (top.empty()) break;
+      std::sort(top.begin(), top.end(), std::greater<uint64_t>());
+      uint32_t m = (uint32_t)std::min<size_t>(K, top.size());
+      std::vector<uint32_t> cp(m);
+      for (uint32_t j = 0; j < m; j++) cp[j] = ~(uint32_t)top[j];
+      std::vector<uint32_t> mask(nf, 0);
+      bool single = false;
+      for (int pass = 0;; pass++) {
+        std::vector<uint32_t> first(m, 0);
+        uint32_t nList = 0;
+        for (uint32_t f = 0; f < nf; f++) {
+          uint32_t mk = 0;
+          if (faces[f].alive)
+            for (uint32_t j = 0; j < m; j++)
+              if (faceEval(f, cp[j]) > 0.0f) mk |= 1u << j;
+          mask[f] = mk;
+          if (mk) {
+            first[__builtin_ctz(mk)]++;
+            nList++;
+          }
         }
-      if (pick == NONE) break;
-      insert(pick);
+        if (nList <= LCAP) break;
+        uint32_t cum = 0, mc = 0;
+        for (uint32_t j = 0; j < m; j++) {
+          if (cum + first[j] > LCAP) break;
+          cum += first[j];
+          mc = j + 1;
+        }
+        if (mc == 0 || pass > 0) {
+          single = true;
+          break;
+        }
+        m = mc;
+      }
+      if (single) {
+        insert(cp[0]);
+      } else {
+        const uint32_t all = m >= 32 ? ~0u : ((1u << m) - 1);
+        uint32_t conflict = 0;
+        for (uint32_t f = 0; f < nf; f++) {
+          const uint32_t mk = mask[f] & all;
+          if (mk & (mk - 1)) conflict |= mk & (mk - 1);
+        }
+        const uint32_t act = all & ~conflict;
+        // visible faces and new faces of each candidate that takes part
+        std::vector<std::vector<uint32_t>> V(m);
+        std::vector<std::vector<std::array<uint32_t, 3>>> N(m);
+        for (uint32_t j = 0; j < m; j++) {
+          if (!((act >> j) & 1u)) continue;
+          for (uint32_t f = 0; f < nf; f++)
+            if ((mask[f] >> j) & 1u) V[j].push_back(f);
+          for (uint32_t fi : V[j]) {
+            const uint32_t vv[3] = {faces[fi].a, faces[fi].b, faces[fi].c};
+            for (int k = 0; k < 3; k++) {
+              const uint32_t ea = vv[k], eb = vv[(k + 1) % 3];
+              bool twin = false;
+              for (uint32_t gi : V[j]) {
+                const Face& g = faces[gi];
+                if ((g.a == eb && g.b == ea) || (g.b == eb && g.c == ea) || (g.c == eb && g.a == ea)) {
+                  twin = true;
+                  break;
+                }
+              }
+              if (!twin) N[j].push_back({ea, eb, cp[j]});
+            }
+          }
+        }
+        uint32_t drop = 0;
+        for (uint32_t j = 1; j < m; j++) {
+          if (!((act >> j) & 1u)) continue;
+          for (uint32_t i = 0; i < j && !((drop >> j) & 1u); i++) {
+            if (!((act >> i) & 1u)) continue;
+            for (const auto& t : N[i])
+              if (evalNew(t[0], t[1], t[2], cp[j]) > 0.0f) {
+                drop |= 1u << j;
+                break;
+              }
+          }
+        }
+        const uint32_t acc = act & ~drop;
+        std::vector<uint32_t> f0(m), f1(m);
+        for (uint32_t j = 0; j < m; j++) {
+          if (!((acc >> j) & 1u)) continue;
+          f0[j] = (uint32_t)faces.size();
+          for (const auto& t : N[j]) addFace(t[0], t[1], t[2]);
+          f1[j] = (uint32_t)faces.size();
+        }
+        for (uint32_t j = 0; j < m; j++)
+          if ((acc >> j) & 1u)
+            for (uint32_t fi : V[j]) faces[fi].alive = false;
+        for (uint32_t j = 0; j < m; j++) {
+          if (!((acc >> j) & 1u)) continue;
+          owner[cp[j]] = NONE;
+          for (uint32_t fi : V[j]) {
+            std::vector<uint32_t> mem;
+            mem.swap(members[fi]);
+            for (uint32_t i : mem)
+              if (owner[i] == fi) assign(i, f0[j], f1[j]);
+          }
+          nIns++;
+        }
+      }
       if (++sinceCompact >= 16) {
         sinceCompact = 0;
         size_t w = 0;

@@ -1,17 +1,20 @@
 // ohull.hpp — ORACLE exact grid hull (test infrastructure only).
 //
-// Sequential restatement of the hull kernel (src/gpu/shaders.ts:752-1401):
-// points quantized to a 2^21 grid over their bbox, 80 extreme seeds along 40
-// directions, a simplex from the seeds, farthest-first insertion (seeds first,
-// then every point outside the seed hull), orientation by an f32 filter with
-// an exact integer fallback, optional last-level branch-and-bound.
+// Restatement of the hull kernel (src/gpu/shaders.ts:752-1401): points
+// quantized to a 2^21 grid over their bbox, 80 extreme seeds along 40
+// directions, a simplex from the seeds, insertion rounds (seeds first, then
+// every point outside the seed hull), orientation by an f32 filter with an
+// exact integer fallback, optional last-level branch-and-bound.
 //
-// Every tie is broken by an order-independent rule (largest distance, then
-// lowest point index; owner face = first face in creation order), so a
-// parallel implementation that follows the same rules reproduces the same
-// face list exactly. The volume is the exact volume of the grid polytope
-// (integer determinants, int128 sum) times the cell volume: it depends only
-// on the point set, not on the order of insertion or the triangulation.
+// The rounds are the device's exported-hull rounds, one after another: the
+// farthest outside points of the 16 faces with the largest keys (larger
+// distance, then lower point index) are the candidates; those that share no
+// visible face with an earlier candidate and see no face an earlier one
+// creates are inserted together. Every tie is broken by an order-independent
+// rule, so the device reproduces the face list exactly. The volume is the
+// exact volume of the grid polytope (integer determinants, int128 sum) times
+// the cell volume: it depends only on the point set, not on the order of
+// insertion or the triangulation.
 #pragma once
 #include <cstdint>
 #include <vector>
