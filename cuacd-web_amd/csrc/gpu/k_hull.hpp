@@ -105,6 +105,14 @@ struct WaveGrp {
   }
   __device__ __forceinline__ float minf(float v) const { return wave_ext_f<false>(v); }
   __device__ __forceinline__ float maxf(float v) const { return wave_ext_f<true>(v); }
+  // three minima and three maxima at once
+  __device__ __forceinline__ void minmax3(float* lo, float* hi) const {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      lo[k] = wave_ext_f<false>(lo[k]);
+      hi[k] = wave_ext_f<true>(hi[k]);
+    }
+  }
   __device__ __forceinline__ double mind(double v) const {
     for (int m = 32; m >= 1; m >>= 1) {
       const double o = __shfl_xor(v, m, 64);
@@ -189,6 +197,34 @@ struct BlockGrp {
   }
   __device__ __forceinline__ float minf(float v) const { return ext<false>(v); }
   __device__ __forceinline__ float maxf(float v) const { return ext<true>(v); }
+  // three minima and three maxima with one exchange through LDS
+  __device__ __forceinline__ void minmax3(float* lo, float* hi) const {
+    float* e = reinterpret_cast<float*>(sm);
+    const uint32_t w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      lo[k] = wave_ext_f<false>(lo[k]);
+      hi[k] = wave_ext_f<true>(hi[k]);
+    }
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        e[6 * w + k] = lo[k];
+        e[6 * w + 3 + k] = hi[k];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      float a = e[k], b = e[3 + k];
+      for (uint32_t q = 1; q < NW; q++) {
+        a = e[6 * q + k] < a ? e[6 * q + k] : a;
+        b = e[6 * q + 3 + k] > b ? e[6 * q + 3 + k] : b;
+      }
+      lo[k] = a;
+      hi[k] = b;
+    }
+    __syncthreads();
+  }
   __device__ __forceinline__ double mind(double v) const {
     for (int m = 32; m >= 1; m >>= 1) {
       const double o = __shfl_xor(v, m, 64);
@@ -1135,10 +1171,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
         lo[k] = x < lo[k] ? x : lo[k];
         hi[k] = x > hi[k] ? x : hi[k];
       }
-    for (int k = 0; k < 3; k++) {
-      lo[k] = g.minf(lo[k]);
-      hi[k] = g.maxf(hi[k]);
-    }
+    g.minmax3(lo, hi);
     const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
     float scale = ex > ey ? ex : ey;
     scale = scale > ez ? scale : ez;
@@ -1300,6 +1333,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
         // the batch state) when they fit; dead faces never win, so skipping them
         // keeps the first-in-creation-order choice.
         for (uint32_t f = t; f < C.nf; f += Grp::G) C.fkey[f] = 0ull;
+        if (t == 0) S.nOrph = 0;  // outside points so far (read after the barriers below)
         uint4* stage = reinterpret_cast<uint4*>(S.hkey);
         constexpr uint32_t SCAP = (uint32_t)(sizeof(S.hkey) / 48);
         uint32_t ns = 0;
@@ -1375,16 +1409,24 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
               bf = best_face(C, qi, 0, nf, &bd);
             }
           }
+          // each wave appends its outside points (the list's order changes no
+          // result: keys are maxima, and the rounds read the list as a set)
           const bool out = bf != NONE;
-          uint32_t cnt;
-          const uint32_t r = g.rank(out, &cnt);
-          if (out) {
-            C.cand[nc + r] = make_uint4(i, (uint32_t)qi[0], (uint32_t)qi[1], (uint32_t)qi[2]);
-            C.cown[nc + r] = bf;
-            atomicMax(&C.fkey[bf], pkey(bd, i));
+          const uint64_t bal = __ballot(out);
+          if (bal) {
+            uint32_t wb = 0;
+            if (lane_id() == 0) wb = atomicAdd(&S.nOrph, (uint32_t)__popcll(bal));
+            wb = __shfl(wb, 0, 64);
+            if (out) {
+              const uint32_t pos = wb + prefix_below(bal);
+              C.cand[pos] = make_uint4(i, (uint32_t)qi[0], (uint32_t)qi[1], (uint32_t)qi[2]);
+              C.cown[pos] = bf;
+              atomicMax(&C.fkey[bf], pkey(bd, i));
+            }
           }
-          nc += cnt;
         }
+        g.sync();
+        nc = S.nOrph;
         C.ncand = nc;
         res.nCand = nc;
         g.sync();
