@@ -39,9 +39,12 @@ struct DEdge {  // open boundary edge p -> q on plane `plane` (a job index)
   uint32_t plane, pad;
 };
 
+// pools a tree level can run out of (bits of Counters::overflow)
+constexpr uint32_t OVF_SCRATCH = 1, OVF_VERTS = 2, OVF_TRIS = 4, OVF_EDGES = 8, OVF_TETS = 16;
+
 struct Counters {
   uint32_t vTop, tTop, eTop, sTop;  // pool bump pointers (floats, u32, edges, scratch words)
-  uint32_t overflow, hullFail, npOverflow, pruned;
+  uint32_t overflow, hullFail, npOverflow, pruned;  // overflow: OVF_* bits of the pools that ran out
   unsigned long long hullsDone;
   unsigned long long faceTests;     // point-face record evaluations, wave-per-part hull kernel
   unsigned long long emitBytes;     // algorithmic bytes of emit: parent read + children written

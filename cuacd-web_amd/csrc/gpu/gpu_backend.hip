@@ -415,7 +415,8 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
   // ---- stages: the tree, then each refinement round over the probe layout
   const uint32_t stages = 1 + (refine ? opts.refine->rounds : 0);
   std::vector<dk::DPart> hostParts;
-  for (uint32_t s = 0; s < stages; s++) {
+  bool aborted = false;
+  for (uint32_t s = 0; s < stages && !aborted; s++) {
     const std::vector<Level>& levels = s == 0 ? treeLevels : probeLevels;
     if (s > 0)  // pool pointers back to the seeds; flags and counts accumulate
       h2d(ctr_.p, &c0, 4 * sizeof(uint32_t), "reset counters");
@@ -442,7 +443,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       launched("k_emit");
       tEnd(K_EMIT, e0);
       e0 = tBegin();
-      dk::k_volume<<<2048, dk::VT, 0, stream_>>>(P, tets_.p, jobAnchor_.p, vacc_.p, ctr_.p);
+      dk::k_volume<<<2048, dk::VT, 0, stream_>>>(P, tets_.p, tetCap_, jobAnchor_.p, vacc_.p, ctr_.p);
       launched("k_volume");
       dk::k_volume_fin<<<(2 * L.nJobs + 255) / 256, 256, 0, stream_>>>(P, jobs_.p, vacc_.p, parts_.p);
       launched("k_volume_fin");
@@ -453,7 +454,13 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       hipCheck(hipMemcpyAsync(hostParts.data(), parts_.p + L.partBase, nChild * sizeof(dk::DPart), hipMemcpyDeviceToHost,
                               stream_),
                "read child parts");
+      uint32_t ovf = 0;
+      hipCheck(hipMemcpyAsync(&ovf, &ctr_.p->overflow, sizeof(ovf), hipMemcpyDeviceToHost, stream_), "read overflow");
       sync();
+      if (ovf) {  // a pool ran out: the caller runs these seeds again in smaller batches
+        aborted = true;
+        break;
+      }
       std::vector<dk::HullJob> hj;
       for (uint32_t c = 0; c < nChild; c++) {
         const dk::DPart& p = hostParts[c];
@@ -481,6 +488,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       launched("k_record");
       tEnd(K_RECORD, e1);
     }
+    if (aborted) break;
     dk::EvalParams E;
     std::memset(&E, 0, sizeof(E));
     E.depth = depth;
@@ -516,6 +524,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
     R.seeds.push_back(r);
   }
   R.overflow = c1.overflow != 0;
+  R.overflowPools = c1.overflow;
   R.hullFail = c1.hullFail != 0;
   R.hullFailCount = c1.hullFail;
   R.npOverflow = c1.npOverflow != 0;

@@ -193,10 +193,22 @@ __global__ __launch_bounds__(BT) void k_plan(LevelParams P, DItem* __restrict__ 
     }
     uint32_t sOff = 0;
     if (ok) {
-      sOff = atomicAdd(&ctr->sTop, 2u * part.nv);
-      if ((uint64_t)sOff + 2ull * part.nv > P.sCap) {
-        ok = 0;
-        atomicExch(&ctr->overflow, 1u);
+      // Reserve only what fits, so the 32-bit bump pointer never passes sCap and
+      // cannot wrap into ranges that other blocks hold.
+      const uint64_t need = 2ull * part.nv;
+      uint32_t cur = ctr->sTop;
+      for (;;) {
+        if ((uint64_t)cur + need > P.sCap) {
+          ok = 0;
+          atomicOr(&ctr->overflow, OVF_SCRATCH);
+          break;
+        }
+        const uint32_t prev = atomicCAS(&ctr->sTop, cur, cur + (uint32_t)need);
+        if (prev == cur) {
+          sOff = cur;
+          break;
+        }
+        cur = prev;
       }
     }
     for (int k = 0; k < 4; k++) s_pl[k] = pl[k];
@@ -334,10 +346,13 @@ __global__ __launch_bounds__(ST) void k_scan(LevelParams P, DItem* __restrict__ 
     const uint32_t pt = block_excl_scan(tNeed, shs, &tt);
     const uint32_t pe = block_excl_scan(eNeed, shs, &te);
     const unsigned long long vOff = run[0] + pv, tOff = run[1] + pt, eOff = run[2] + pe;
-    if (ok && (vOff + vNeed > P.vCap || tOff + tNeed > P.tCap || eOff + eNeed > P.eCap)) {
+    const uint32_t ovf = !ok ? 0u
+                             : (vOff + vNeed > P.vCap ? OVF_VERTS : 0u) | (tOff + tNeed > P.tCap ? OVF_TRIS : 0u) |
+                                   (eOff + eNeed > P.eCap ? OVF_EDGES : 0u);
+    if (ovf) {
       ok = false;
       job.valid = 4;
-      atomicExch(&ctr->overflow, 1u);
+      atomicOr(&ctr->overflow, ovf);
       for (int k = 0; k < 8; k++) out.partIds[k] = 0;
       out.np = out.srcPart = out.initialCut = out.nLevels = out.flags = out.parentItem = out.cutIdx = 0;
       for (int k = 0; k < 4; k++) out.levelCosts[k] = 0.0;
@@ -503,7 +518,7 @@ __global__ __launch_bounds__(BT) void k_emit(LevelParams P, DPart* __restrict__ 
     const unsigned long long n = (unsigned long long)job.cnt[1] + job.cnt[2] + job.cnt[4] + job.cnt[5];
     unsigned long long b = atomicAdd(&ctr->tetTop, n);
     if (b + n > tetCap) {
-      atomicExch(&ctr->overflow, 1u);
+      atomicOr(&ctr->overflow, OVF_TETS);
       b = ~0ull;
     }
     s_tet = b;
@@ -768,10 +783,14 @@ __global__ void k_vol_clear(unsigned long long* __restrict__ acc, uint32_t n, Co
   if (blockIdx.x == 0 && threadIdx.x == 0) ctr->tetTop = 0;
 }
 
-__global__ __launch_bounds__(VT) void k_volume(LevelParams P, const float4* __restrict__ tets,
+__global__ __launch_bounds__(VT) void k_volume(LevelParams P, const float4* __restrict__ tets, unsigned long long tetCap,
                                                const float4* __restrict__ jobAnchor,
                                                unsigned long long* __restrict__ acc, const Counters* __restrict__ ctr) {
-  const unsigned long long n = ctr->tetTop;
+  // An overflowing level is run again with fewer seeds: its records are incomplete
+  // (a job whose tetrahedra did not fit wrote none, yet counted them in tetTop),
+  // so nothing is read. Otherwise every record below tetTop (<= tetCap) is written.
+  if (ctr->overflow) return;
+  const unsigned long long n = ctr->tetTop < tetCap ? ctr->tetTop : tetCap;
   const unsigned long long stride = (unsigned long long)gridDim.x * VT;
   for (unsigned long long base = (unsigned long long)blockIdx.x * VT; base < n; base += stride) {
     const unsigned long long i = base + threadIdx.x;

@@ -98,6 +98,7 @@ struct PartView {
 struct TreeRunResult {
   std::vector<SeedResult> seeds;
   bool overflow = false;
+  uint32_t overflowPools = 0;  // which pools ran out: 1 scratch, 2 vertices, 4 triangles, 8 edges, 16 tetrahedra
   bool hullFail = false;
   uint32_t hullFailCount = 0;
   bool npOverflow = false;

@@ -53,3 +53,42 @@ def test_batch_gpu_equals_single_oracle(cw, oracle):
     batch = cw.decompose_batch(meshes, **OPTS)
     for m, rb in zip(meshes, batch):
         _same(cw.decompose(m, api=oracle, **OPTS), rb)
+
+
+def _run_or_none(cw, meshes, caps):
+    """decompose_batch with the given pool sizes; None when one part alone does not fit."""
+    try:
+        return cw.decompose_batch(meshes, **{**OPTS, **caps})
+    except RuntimeError as e:
+        assert "single part" in str(e), e
+        return None
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pool", ["vertex_cap", "tri_cap"])
+def test_batch_gpu_pool_overflow_recovers(cw, oracle, pool):
+    """Pools too small for the whole batch: a tree run that overflows a pool is dropped
+    and run again in halves, and the results still equal the oracle's single runs.
+
+    The smallest size at which the batch completes is found by bisection: there one part
+    only just fits, so the first iteration's tree run (all meshes together) overflows.
+    vertex_cap bounds the child vertices of every level, tri_cap the child triangles
+    of the levels that keep topology."""
+    meshes = _meshes(cw)
+    base = cw.decompose_batch(meshes, **OPTS)[0].stats["tree_runs"]
+    extra = {}
+    lo = 1 << 11 if pool == "vertex_cap" else 1 << 9  # one part does not fit (the seeds do)
+    hi = 1 << 17  # the batch fits
+    assert _run_or_none(cw, meshes, {pool: lo, **extra}) is None
+    assert _run_or_none(cw, meshes, {pool: hi, **extra}) is not None
+    while hi - lo > 64:
+        mid = (lo + hi) // 2
+        if _run_or_none(cw, meshes, {pool: mid, **extra}) is None:
+            lo = mid
+        else:
+            hi = mid
+    batch = _run_or_none(cw, meshes, {pool: hi, **extra})
+    print(pool, "smallest fit", hi, "tree runs", batch[0].stats["tree_runs"], "with default pools", base, flush=True)
+    assert batch[0].stats["tree_runs"] > base
+    for m, rb in zip(meshes, batch):
+        _same(cw.decompose(m, api=oracle, **OPTS), rb)

@@ -147,3 +147,44 @@ def test_hausdorff_bit_exact(cw, oracle):
     hw_g, hw_o = g.hulls_with_hausdorff(meshes), o.hulls_with_hausdorff(meshes)
     for a, b in zip(hw_g, hw_o):
         assert a[1] == b[1] and a[2] == b[2] and np.array_equal(a[0], b[0])
+
+
+def test_pool_overflow_names_the_pool(cw):
+    """A run whose pools are too small reports which pools ran out (1 scratch,
+    2 vertices, 4 triangles, 8 edges, 16 tetrahedra) and stops there; with pools that
+    just fit, the result equals the run with the default pools. The tetrahedra of
+    one level (child triangles plus cap edges, the volume integrand) are bounded by
+    tri_cap / 3 + edge_cap; the last level writes the most of them, since it keeps no
+    topology but computes every child's volume."""
+    meshes = [stairs(cw), square_torus(cw), l_prism(cw)]
+    cfg = dict(width=12, width2=6, depth=2, scratch_words=4 << 20, hull_budget_bytes=256 << 20)
+
+    def run(**caps):
+        t = cw.GpuTree(**cfg, **caps)
+        r = t.run(meshes, read_levels=False)
+        t.destroy()
+        return r
+
+    ref = run()
+    assert not ref.overflow and ref.overflow_pools == 0
+
+    def smallest(key, lo, hi, **fixed):
+        masks = set()
+        while hi - lo > 16:
+            mid = (lo + hi) // 2
+            r = run(**{key: mid}, **fixed)
+            assert bool(r.overflow) == (r.overflow_pools != 0)
+            masks.add(r.overflow_pools)
+            lo, hi = (mid, hi) if r.overflow_pools else (lo, mid)
+        return hi, masks
+
+    e, masks = smallest("edge_cap", 0, 1 << 16)
+    print("edge_cap fit", e, "masks", sorted(masks), flush=True)
+    assert 8 in masks
+    t, masks = smallest("tri_cap", 1 << 10, 1 << 20, edge_cap=e)
+    print("tri_cap fit", t, "masks", sorted(masks), flush=True)
+    assert any(m & 16 for m in masks)
+    r = run(tri_cap=t, edge_cap=e)
+    assert r.overflow_pools == 0 and r.hulls_done == ref.hulls_done
+    for a, b in zip(r.seeds, ref.seeds):
+        assert (a.best_cut, a.best_cost, a.plane) == (b.best_cut, b.best_cost, b.plane)
