@@ -43,7 +43,7 @@ struct DEdge {  // open boundary edge p -> q on plane `plane` (a job index)
 constexpr uint32_t OVF_SCRATCH = 1, OVF_VERTS = 2, OVF_TRIS = 4, OVF_EDGES = 8, OVF_TETS = 16;
 
 struct Counters {
-  uint32_t vTop, tTop, eTop, sTop;  // pool bump pointers (floats, u32, edges, scratch words)
+  uint32_t vTop, tTop, eTop, pad0;  // pool bump pointers (floats, u32, edges)
   uint32_t overflow, hullFail, npOverflow, pruned;  // overflow: OVF_* bits of the pools that ran out
   unsigned long long hullsDone;
   unsigned long long faceTests;     // point-face record evaluations, wave-per-part hull kernel
@@ -53,6 +53,7 @@ struct Counters {
   unsigned long long hullBytes;     // algorithmic HBM bytes, wave-per-part hull: 12 nv + 16 (+ 12 per exported face)
   unsigned long long hullBytesBig;  // the same for the workgroup-per-part hull kernel
   unsigned long long tetTop;        // tetrahedra written by emit in the current level
+  unsigned long long sTop;          // scratch words reserved by plan in the current level (64-bit: cannot wrap)
   uint32_t hd[1024];  // Hausdorff results (f32 bits of the squared distance)
 };
 

@@ -193,22 +193,14 @@ __global__ __launch_bounds__(BT) void k_plan(LevelParams P, DItem* __restrict__ 
     }
     uint32_t sOff = 0;
     if (ok) {
-      // Reserve only what fits, so the 32-bit bump pointer never passes sCap and
-      // cannot wrap into ranges that other blocks hold.
-      const uint64_t need = 2ull * part.nv;
-      uint32_t cur = ctr->sTop;
-      for (;;) {
-        if ((uint64_t)cur + need > P.sCap) {
-          ok = 0;
-          atomicOr(&ctr->overflow, OVF_SCRATCH);
-          break;
-        }
-        const uint32_t prev = atomicCAS(&ctr->sTop, cur, cur + (uint32_t)need);
-        if (prev == cur) {
-          sOff = cur;
-          break;
-        }
-        cur = prev;
+      // 64-bit bump pointer: a level's requests sum to far below 2^64, so it never wraps
+      const unsigned long long need = 2ull * part.nv;
+      const unsigned long long at = atomicAdd(&ctr->sTop, need);
+      if (at + need > P.sCap) {
+        ok = 0;
+        atomicOr(&ctr->overflow, OVF_SCRATCH);
+      } else {
+        sOff = (uint32_t)at;
       }
     }
     for (int k = 0; k < 4; k++) s_pl[k] = pl[k];
