@@ -328,7 +328,7 @@ struct HullCtx {
 };
 
 // Face records live in five arrays of uint4 (word k of face f at F4[k * fmax + f]):
-//   0: (a | ALIVE, b, c, 0)   1: (n0, n1, n2, bound factor)   2: (qa, |n|)   3: (qb, 0)   4: (qc, 0)
+//   0: (a | ALIVE, b, c, n.qa)   1: (n0, n1, n2, bound factor)   2: (qa, |n|)   3: (qb, 0)   4: (qc, 0)
 // so a wave scanning consecutive faces reads contiguous 16-byte words.
 __device__ __forceinline__ uint4* face_row(const HullCtx& C, int k) {
   return reinterpret_cast<uint4*>(C.F) + (size_t)k * C.fmax;
@@ -340,6 +340,23 @@ __device__ __forceinline__ uint32_t& face_a(const HullCtx& C, uint32_t f) { retu
 __device__ __forceinline__ unsigned long long pkey(float d, uint32_t i) {
   return ((unsigned long long)__float_as_uint(d) << 32) | (uint32_t)(~i);
 }
+
+// Cheap rejection against a face plane. With grid coordinates in [0, 2^21],
+// c = n.p - n.qa evaluated with fused multiply-adds (d = n.qa the same way)
+// differs from the exact det N.(p - qa) by at most 3 eps |n|inf (|p|1 + 2 |qa|1)
+// for the rounding of the two dot products plus |n - N|inf |p - qa|1 <= 2 eps s1 s2 3 2^21
+// for the f32 normal; with |n|inf <= s1 s2 that is below 4.3 s1 s2 (eps = 2^-24).
+// So c < -8 s1 s2 proves the point strictly inside, and the f32 filter and the
+// exact test would say the same; any other point needs those tests.
+__device__ __forceinline__ float plane_offset(const uint4& w1, const uint4& w2) {
+  return fmaf(__uint_as_float(w1.x), (float)(int32_t)w2.x,
+              fmaf(__uint_as_float(w1.y), (float)(int32_t)w2.y, __uint_as_float(w1.z) * (float)(int32_t)w2.z));
+}
+__device__ __forceinline__ bool maybe_outside(const uint4& w1, float off, float lim, const float4& p) {
+  const float c = fmaf(__uint_as_float(w1.x), p.x, fmaf(__uint_as_float(w1.y), p.y, fmaf(__uint_as_float(w1.z), p.z, -off)));
+  return c >= lim;
+}
+__device__ __forceinline__ float reject_limit(const uint4& w1) { return -8.0f * __uint_as_float(w1.w); }
 
 struct FaceRec {
   uint4 w[5];
@@ -359,9 +376,9 @@ __device__ __forceinline__ FaceRec face_make(uint32_t a, uint32_t b, uint32_t c,
   nn = nn + n1 * n1;
   nn = nn + n2 * n2;
   FaceRec R;
-  R.w[0] = make_uint4(a | ALIVE, b, c, 0u);
   R.w[1] = make_uint4(__float_as_uint(n0), __float_as_uint(n1), __float_as_uint(n2), __float_as_uint(s1 * s2));
   R.w[2] = make_uint4((uint32_t)qa[0], (uint32_t)qa[1], (uint32_t)qa[2], __float_as_uint(sqrtf(nn)));
+  R.w[0] = make_uint4(a | ALIVE, b, c, __float_as_uint(plane_offset(R.w[1], R.w[2])));
   R.w[3] = make_uint4((uint32_t)qb[0], (uint32_t)qb[1], (uint32_t)qb[2], 0u);
   R.w[4] = make_uint4((uint32_t)qc[0], (uint32_t)qc[1], (uint32_t)qc[2], 0u);
   return R;
@@ -428,6 +445,8 @@ __device__ __forceinline__ int face_filter(const uint4& w1, const uint4& w2, con
 __device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, const int32_t* qp) {
   const uint4 w0 = face_word(C, f, 0), w1 = face_word(C, f, 1), w2 = face_word(C, f, 2);
   if (!(w0.x & ALIVE)) return -1e30f;
+  if (!maybe_outside(w1, __uint_as_float(w0.w), reject_limit(w1), make_float4((float)qp[0], (float)qp[1], (float)qp[2], 0.0f)))
+    return -1.0f;
   float det;
   const int s = face_filter(w1, w2, qp, &det);
   if (s > 0) {
@@ -439,6 +458,9 @@ __device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, const i
 }
 __device__ __forceinline__ float face_eval_rec(const FaceRec& R, const int32_t* qp) {
   if (!(R.w[0].x & ALIVE)) return -1e30f;
+  if (!maybe_outside(R.w[1], __uint_as_float(R.w[0].w), reject_limit(R.w[1]),
+                     make_float4((float)qp[0], (float)qp[1], (float)qp[2], 0.0f)))
+    return -1.0f;
   float det;
   const int s = face_filter(R.w[1], R.w[2], qp, &det);
   if (s > 0) {
@@ -634,6 +656,7 @@ struct HullBatch {
   unsigned long long ckey[K];  // candidate keys (descending when EXACT)
   uint32_t cpt[K];
   int32_t cq[K][3];
+  float4 cqf[K];  // the same coordinates as floats (exact: below 2^24)
   uint32_t nvis[K], nh[K];                              // list entries and horizon edges per candidate
   uint32_t boff[K], bcnt[K];                            // per-candidate buckets of list entries
   uint32_t noff[K];                                     // first new face of each candidate
@@ -794,6 +817,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     S.cq[t][0] = C.Q[3 * pt];
     S.cq[t][1] = C.Q[3 * pt + 1];
     S.cq[t][2] = C.Q[3 * pt + 2];
+    S.cqf[t] = make_float4((float)S.cq[t][0], (float)S.cq[t][1], (float)S.cq[t][2], 0.0f);
   }
   g.sync();
   // ---- visibility of every alive face from every candidate
@@ -833,15 +857,31 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
           h1[b] = face_word(C, f, 1);
           h2[b] = face_word(C, f, 2);
         } else {
-          h0[b] = make_uint4(0u, 0u, 0u, 0u);
+          h0[b] = h1[b] = h2[b] = make_uint4(0u, 0u, 0u, 0u);
         }
+      }
+      // every candidate against the faces of the group by the cheap rejection;
+      // the f32 filter (and the exact test) only for the pairs it leaves
+      float off[VB], lim[VB];
+      Mask maybe[VB];
+#pragma unroll
+      for (uint32_t b = 0; b < VB; b++) {
+        off[b] = __uint_as_float(h0[b].w);
+        lim[b] = reject_limit(h1[b]);
+        maybe[b] = 0;
+      }
+      for (uint32_t j = 0; j < m; j++) {
+        const float4 pj = S.cqf[j];
+#pragma unroll
+        for (uint32_t b = 0; b < VB; b++) maybe[b] |= maybe_outside(h1[b], off[b], lim[b], pj) ? ((Mask)1 << j) : (Mask)0;
       }
 #pragma unroll
       for (uint32_t b = 0; b < VB; b++) {
         const uint32_t f = f0 + b * Grp::G;
         if (!(h0[b].x & ALIVE)) continue;
         Mask mask = 0, und = 0;
-        for (uint32_t j = 0; j < m; j++) {
+        for (Mask u = maybe[b]; u; u &= u - 1) {
+          const uint32_t j = mask_ffs(u) - 1;
           float det;
           const int sgn = face_filter(h1[b], h2[b], S.cq[j], &det);
           mask |= sgn > 0 ? ((Mask)1 << j) : (Mask)0;
@@ -1345,7 +1385,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
           if (alive && ns + rk < SCAP) {
             stage[3 * (ns + rk)] = face_word(C, f, 1);
             stage[3 * (ns + rk) + 1] = face_word(C, f, 2);
-            stage[3 * (ns + rk) + 2] = make_uint4(f, 0u, 0u, 0u);
+            stage[3 * (ns + rk) + 2] = make_uint4(f, face_word(C, f, 0).w, 0u, 0u);
           }
           ns += cnt;
         }
@@ -1371,6 +1411,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
             hi[k] = wave_ext_f<true>(i < C.nv ? (float)qi[k] : -3e7f);
           }
           if (i < C.nv) {
+            const float4 pf = make_float4((float)qi[0], (float)qi[1], (float)qi[2], 0.0f);
             if (staged) {
               for (uint32_t s = 0; s < ns; s++) {
                 const uint4 w1 = stage[3 * s], w2 = stage[3 * s + 1];
@@ -1389,6 +1430,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
                   if (det < -((1e-6f * __uint_as_float(w1.w)) * sd)) continue;
                 }
                 C.tests++;
+                if (!maybe_outside(w1, __uint_as_float(stage[3 * s + 2].y), reject_limit(w1), pf)) continue;
                 float det;
                 const int sg = face_filter(w1, w2, qi, &det);
                 float dd;
