@@ -1410,25 +1410,32 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
             lo[k] = wave_ext_f<false>(i < C.nv ? (float)qi[k] : 3e7f);
             hi[k] = wave_ext_f<true>(i < C.nv ? (float)qi[k] : -3e7f);
           }
-          if (i < C.nv) {
+          if (staged) {
             const float4 pf = make_float4((float)qi[0], (float)qi[1], (float)qi[2], 0.0f);
-            if (staged) {
-              for (uint32_t s = 0; s < ns; s++) {
+            // Faces in chunks of 64, one per lane, against the wave's box: every point
+            // of the box is strictly inside a face when its corner farthest along the
+            // f32 normal is, by the filter's margin widened by the box extent (the
+            // exact normal may pick another corner, at most that far off). The points
+            // then visit the faces the box leaves, in creation order.
+            for (uint32_t s0 = 0; s0 < ns; s0 += 64) {
+              const uint32_t sl = s0 + lane_id();
+              bool keep = false;
+              if (sl < ns) {
+                const uint4 w1 = stage[3 * sl], w2 = stage[3 * sl + 1];
+                const float n0 = __uint_as_float(w1.x), n1 = __uint_as_float(w1.y), n2 = __uint_as_float(w1.z);
+                const float d0 = (n0 > 0.0f ? hi[0] : lo[0]) - (float)(int32_t)w2.x;
+                const float d1 = (n1 > 0.0f ? hi[1] : lo[1]) - (float)(int32_t)w2.y;
+                const float d2 = (n2 > 0.0f ? hi[2] : lo[2]) - (float)(int32_t)w2.z;
+                float det = n0 * d0;
+                det = det + n1 * d1;
+                det = det + n2 * d2;
+                const float sd = ((fabsf(d0) + fabsf(d1)) + fabsf(d2)) + ((hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]));
+                keep = !(det < -((1e-6f * __uint_as_float(w1.w)) * sd));
+              }
+              for (uint64_t km = __ballot(keep); km; km &= km - 1) {
+                if (i >= C.nv) continue;
+                const uint32_t s = s0 + (uint32_t)__ffsll((unsigned long long)km) - 1;
                 const uint4 w1 = stage[3 * s], w2 = stage[3 * s + 1];
-                {
-                  // every point of the box is strictly inside this face when its corner
-                  // farthest along the f32 normal is, by the filter's margin widened by the
-                  // box extent (the exact normal may pick another corner, at most that far off)
-                  const float n0 = __uint_as_float(w1.x), n1 = __uint_as_float(w1.y), n2 = __uint_as_float(w1.z);
-                  const float d0 = (n0 > 0.0f ? hi[0] : lo[0]) - (float)(int32_t)w2.x;
-                  const float d1 = (n1 > 0.0f ? hi[1] : lo[1]) - (float)(int32_t)w2.y;
-                  const float d2 = (n2 > 0.0f ? hi[2] : lo[2]) - (float)(int32_t)w2.z;
-                  float det = n0 * d0;
-                  det = det + n1 * d1;
-                  det = det + n2 * d2;
-                  const float sd = ((fabsf(d0) + fabsf(d1)) + fabsf(d2)) + ((hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]));
-                  if (det < -((1e-6f * __uint_as_float(w1.w)) * sd)) continue;
-                }
                 C.tests++;
                 if (!maybe_outside(w1, __uint_as_float(stage[3 * s + 2].y), reject_limit(w1), pf)) continue;
                 float det;
@@ -1447,9 +1454,9 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
                   bf = stage[3 * s + 2].x;
                 }
               }
-            } else {
-              bf = best_face(C, qi, 0, nf, &bd);
             }
+          } else if (i < C.nv) {
+            bf = best_face(C, qi, 0, nf, &bd);
           }
           // each wave appends its outside points (the list's order changes no
           // result: keys are maxima, and the rounds read the list as a set)
@@ -1520,7 +1527,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
 }
 
 // candidates per round: exported hulls (the same for both group sizes), tree candidates
-constexpr uint32_t KE = 16, KF_BIG = 32, KF_W = 16;
+constexpr uint32_t KE = 16, KF_BIG = 64, KF_W = 16;
 
 // small parts: one wavefront per part, HW parts per block
 template <bool EXACT>
