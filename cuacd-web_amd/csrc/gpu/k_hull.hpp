@@ -1227,25 +1227,29 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
         C.Q[3 * i + k] = (int32_t)rintf(d * qs);
       }
     g.sync();
-    // seeds: maximum and minimum along each direction (lowest index on ties), 8 directions per pass
-    for (int k0 = 0; k0 < 40; k0 += 4) {
-      float mx[4], mn[4];
-      uint32_t ix[4], in[4];
+    // seeds: maximum and minimum along each direction (lowest index on ties)
+    if constexpr (Grp::NW > 1) {
+      // a workgroup: each wave owns 40 / NW directions and scans every point once
+      static_assert(40 % Grp::NW == 0, "directions per wave");
+      constexpr int DPW = 40 / (int)Grp::NW;
+      const uint32_t lane = lane_id(), d0 = (t >> 6) * DPW;
+      float mx[DPW], mn[DPW];
+      uint32_t ix[DPW], in[DPW];
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
+      for (int k = 0; k < DPW; k++) {
         mx[k] = -1e30f;
         mn[k] = 1e30f;
         ix[k] = NONE;
         in[k] = NONE;
       }
-      for (uint32_t i = t; i < C.nv; i += Grp::G) {
+      for (uint32_t i = lane; i < C.nv; i += 64) {
         const int32_t* q = C.Q + 3 * i;
         const float x = (float)q[0], y = (float)q[1], z = (float)q[2];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          float d = KDIR[k0 + k][0] * x;
-          d = d + KDIR[k0 + k][1] * y;
-          d = d + KDIR[k0 + k][2] * z;
+        for (int k = 0; k < DPW; k++) {
+          float d = KDIR[d0 + k][0] * x;
+          d = d + KDIR[d0 + k][1] * y;
+          d = d + KDIR[d0 + k][2] * z;
           if (d > mx[k]) {
             mx[k] = d;
             ix[k] = i;
@@ -1256,13 +1260,54 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
           }
         }
       }
-      g.template argext<4>(mx, ix, mn, in);
-      if (t == 0)
+#pragma unroll
+      for (int k = 0; k < DPW; k++) {
+        wave_argext<true>(mx[k], ix[k]);
+        wave_argext<false>(mn[k], in[k]);
+      }
+      if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < DPW; k++) {
+          seed[2 * (d0 + k)] = ix[k];
+          seed[2 * (d0 + k) + 1] = in[k];
+        }
+    } else {
+      for (int k0 = 0; k0 < 40; k0 += 4) {
+        float mx[4], mn[4];
+        uint32_t ix[4], in[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          seed[2 * (k0 + k)] = ix[k];
-          seed[2 * (k0 + k) + 1] = in[k];
+          mx[k] = -1e30f;
+          mn[k] = 1e30f;
+          ix[k] = NONE;
+          in[k] = NONE;
         }
+        for (uint32_t i = t; i < C.nv; i += Grp::G) {
+          const int32_t* q = C.Q + 3 * i;
+          const float x = (float)q[0], y = (float)q[1], z = (float)q[2];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            float d = KDIR[k0 + k][0] * x;
+            d = d + KDIR[k0 + k][1] * y;
+            d = d + KDIR[k0 + k][2] * z;
+            if (d > mx[k]) {
+              mx[k] = d;
+              ix[k] = i;
+            }
+            if (d < mn[k]) {
+              mn[k] = d;
+              in[k] = i;
+            }
+          }
+        }
+        g.template argext<4>(mx, ix, mn, in);
+        if (t == 0)
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            seed[2 * (k0 + k)] = ix[k];
+            seed[2 * (k0 + k) + 1] = in[k];
+          }
+      }
     }
     g.sync();
     // simplex from the seeds (exact integer tests, first seed on ties)
