@@ -3,6 +3,7 @@
 brute force on small inputs. CPU only."""
 import itertools
 import math
+import os
 
 import numpy as np
 import pytest
@@ -219,6 +220,50 @@ def test_hull_against_qhull_on_clouds(cw, oracle):
         assert abs(vol - h.volume) < 1e-5 * h.volume
         m = cw.Mesh(pts, tris)
         assert cw.is_closed(m) and cw.signed_volume(m) > 0
+
+
+REF_QH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "libref_qh.so")
+
+
+def _ref_hull(pts):
+    """The reference's own CPU hull (wasm/quickhull/QuickHull.cpp via oracle/ref_qh_driver.cpp, called as
+    src/cpu/hull.ts:47-50 calls it, eps 1e-7): (hull vertices as f32 rows, volume)."""
+    import ctypes
+    lib = ctypes.CDLL(REF_QH)
+    lib.ref_qh_hull.restype = ctypes.c_int
+    p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1)
+    n = len(p) // 3
+    vol = ctypes.c_double()
+    nv = ctypes.c_uint32()
+    out = np.zeros(3 * n, np.float32)
+    lib.ref_qh_hull(p.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, ctypes.c_double(1e-7), ctypes.byref(vol),
+                    ctypes.byref(nv), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n)
+    return out[: 3 * nv.value].reshape(-1, 3), vol.value
+
+
+@pytest.mark.skipif(not os.path.exists(REF_QH), reason="oracle/_ref not built (needs the reference tree)")
+def test_hull_against_reference_quickhull(cw, oracle):
+    """The oracle's exact grid hull against the reference's own CPU hull: the same known answers, the
+    same hull vertices on clouds in general position, and volumes equal up to the 2^21 grid (1e-5)."""
+    for pts, want in ((cw.unit_cube().verts, 1.0), (cw.l_shape().verts, 1.75)):
+        rv, rvol = _ref_hull(pts)
+        assert abs(rvol - want) < 1e-12 and _hull(cw, oracle, pts)[1] == want
+    pm = park_miller_cloud()
+    rv, rvol = _ref_hull(pm)
+    tris, vol = _hull(cw, oracle, pm)
+    assert 6.5 < rvol < 8 and abs(vol - rvol) < 1e-5 * rvol  # tests/cpu.test.ts:65-75
+    assert {tuple(x) for x in pm[np.unique(tris)]} == {tuple(x) for x in rv}
+    rng = np.random.default_rng(23)
+    for n in (60, 500, 4000):
+        pts = rng.standard_normal((n, 3)).astype(np.float32)
+        rv, rvol = _ref_hull(pts)
+        tris, vol = _hull(cw, oracle, pts)
+        assert {tuple(x) for x in pts[np.unique(tris)]} == {tuple(x) for x in rv}
+        assert abs(vol - rvol) < 1e-5 * rvol
+    # a smooth closed surface (many nearly coplanar points): the volumes agree up to the grid
+    blob = cw.generate("blobs", 20260200, [3, 32, -0.6, 0.6, 0.12, 0.30, 0.5])
+    rv, rvol = _ref_hull(blob.verts)
+    assert abs(_hull(cw, oracle, blob.verts)[1] - rvol) < 1e-5 * rvol
 
 
 def test_hull_volume_independent_of_order(cw, oracle):
