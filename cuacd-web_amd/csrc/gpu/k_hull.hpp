@@ -862,19 +862,32 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       }
       // every candidate against the faces of the group by the cheap rejection;
       // the f32 filter (and the exact test) only for the pairs it leaves
+      // (candidates in descending order, each 32-bit half of the mask built by shift
+      // and or, so candidate j lands on bit j; the count is the group's, read once)
       float off[VB], lim[VB];
+      uint32_t mlo[VB], mhi[VB];
       Mask maybe[VB];
 #pragma unroll
       for (uint32_t b = 0; b < VB; b++) {
         off[b] = __uint_as_float(h0[b].w);
         lim[b] = reject_limit(h1[b]);
-        maybe[b] = 0;
+        mlo[b] = mhi[b] = 0u;
       }
-      for (uint32_t j = 0; j < m; j++) {
+      const int mu = (int)__builtin_amdgcn_readfirstlane(m);
+#pragma unroll 4
+      for (int j = mu - 1; j >= 32; j--) {
         const float4 pj = S.cqf[j];
 #pragma unroll
-        for (uint32_t b = 0; b < VB; b++) maybe[b] |= maybe_outside(h1[b], off[b], lim[b], pj) ? ((Mask)1 << j) : (Mask)0;
+        for (uint32_t b = 0; b < VB; b++) mhi[b] = (mhi[b] << 1) | (maybe_outside(h1[b], off[b], lim[b], pj) ? 1u : 0u);
       }
+#pragma unroll 4
+      for (int j = (mu < 32 ? mu : 32) - 1; j >= 0; j--) {
+        const float4 pj = S.cqf[j];
+#pragma unroll
+        for (uint32_t b = 0; b < VB; b++) mlo[b] = (mlo[b] << 1) | (maybe_outside(h1[b], off[b], lim[b], pj) ? 1u : 0u);
+      }
+#pragma unroll
+      for (uint32_t b = 0; b < VB; b++) maybe[b] = (Mask)(((unsigned long long)mhi[b] << 32) | mlo[b]);
 #pragma unroll
       for (uint32_t b = 0; b < VB; b++) {
         const uint32_t f = f0 + b * Grp::G;
