@@ -665,6 +665,7 @@ struct HullBatch {
   uint16_t bkt[LCAP];                                   // list entries grouped by candidate
   uint32_t pface[LCAP / 2];
   Mask pmask[LCAP / 2], pund[LCAP / 2];  // faces the f32 filter left undecided
+  uint16_t ppre[LCAP / 2];                // their undecided candidates, prefix counts
   unsigned long long hkey[HT];                          // directed edges (a << 32 | b); seed-hull faces in phase 2
   uint8_t hj[HT];
   unsigned long long wkey[NW][EXACT ? K : K / NW];
@@ -919,16 +920,31 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     }
     g.sync();
     {
+      // exact tests of the undecided (face, candidate) pairs, one pair per thread
+      // (coplanar points, as on a cut's cap, leave many undecided on a few faces)
+      static_assert(LCAP / 2 <= Grp::G, "one pending face per thread");
       const uint32_t np = S.nPend < LCAP / 2 ? S.nPend : LCAP / 2;
+      uint32_t npairs;
+      const uint32_t pre = g.excl_scan(t < np ? mask_popc(S.pund[t]) : 0u, &npairs);
+      if (t < np) S.ppre[t] = (uint16_t)pre;
+      g.sync();
+      for (uint32_t q = t; q < npairs; q += Grp::G) {
+        uint32_t lo = 0, hi = np - 1;  // the last pending face whose pairs start at or before q
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1) >> 1;
+          if (S.ppre[mid] <= q) lo = mid;
+          else hi = mid - 1;
+        }
+        Mask u = S.pund[lo];
+        for (uint32_t r = q - S.ppre[lo]; r > 0; r--) u &= u - 1;
+        const uint32_t j = mask_ffs(u) - 1, f = S.pface[lo];
+        if (orient_rec(C, f, face_word(C, f, 2), S.cq[j]) > 0) atomicOr(&S.pmask[lo], (Mask)1 << j);
+      }
+      g.sync();
       for (uint32_t p = t; p < np; p += Grp::G) {
         const uint32_t f = S.pface[p];
-        const uint4 w0 = face_word(C, f, 0), w2 = face_word(C, f, 2);
-        Mask mask = S.pmask[p];
-        for (Mask u = S.pund[p]; u; u &= u - 1) {
-          const uint32_t j = mask_ffs(u) - 1;
-          if (orient_rec(C, f, w2, S.cq[j]) > 0) mask |= (Mask)1 << j;
-        }
-        finish(f, mask, w0.x & ~ALIVE, w0.y, w0.z);
+        const uint4 w0 = face_word(C, f, 0);
+        finish(f, S.pmask[p], w0.x & ~ALIVE, w0.y, w0.z);
       }
     }
     g.sync();
