@@ -208,6 +208,9 @@ struct CapCtx {
   std::vector<uint32_t>* tris;
   double n[3];
   double capDir;
+  const std::vector<int8_t>* sign;  // side of each input vertex (0: on the plane)
+  uint32_t nvIn;                    // input vertex count: higher indices are crossing points
+  bool closedIn;                    // the input is closed (every directed edge has its twin)
 };
 
 // src/cpu/cut.ts:243-254 fanLoop: centroid vertex, triangles (c, q, p)
@@ -280,13 +283,21 @@ bool pointInPoly(double px, double py, const std::vector<double>& poly) {
 bool cap(CapCtx& C, std::string* reason) {
   std::vector<uint32_t>& tris = *C.tris;
   const std::vector<double>& xyz = *C.xyz;
-  // directed edges in first-appearance order (the reference iterates an insertion-ordered Map)
+  // directed edges in first-appearance order (the reference iterates an insertion-ordered
+  // Map). When the input is closed, an edge between two input vertices off the plane
+  // keeps its twin on the same side (the neighbour is kept whole or clipped around it):
+  // only edges with a crossing point or an on-plane vertex can be open, and their twins
+  // are such edges too. Those alone are collected, in the same relative order, which
+  // leaves the open edges and their order as they are.
+  const std::vector<int8_t>& sg = *C.sign;
+  auto special = [&](uint32_t v) { return !C.closedIn || v >= C.nvIn || sg[v] == 0; };
   std::unordered_set<uint64_t> edges;
   std::vector<uint64_t> order;
-  edges.reserve(tris.size() * 2);
   for (size_t i = 0; i + 2 < tris.size(); i += 3)
     for (int e = 0; e < 3; e++) {
-      const uint64_t k = ekey(tris[i + e], tris[i + (e + 1) % 3]);
+      const uint32_t a = tris[i + e], b = tris[i + (e + 1) % 3];
+      if (!special(a) && !special(b)) continue;
+      const uint64_t k = ekey(a, b);
       if (edges.insert(k).second) order.push_back(k);
     }
   std::unordered_map<uint32_t, size_t> nextIdx;  // start vertex -> slot in `next`
@@ -462,7 +473,7 @@ Mesh compactMesh(const std::vector<double>& xyz, const std::vector<uint32_t>& tr
 
 }  // namespace
 
-CutResult planeCut(const Mesh& mesh, const float plane[4], double eps) {
+CutResult planeCut(const Mesh& mesh, const float plane[4], double eps, bool closed) {
   CutResult res;
   const std::vector<float>& V = mesh.verts;
   const std::vector<uint32_t>& T = mesh.tris;
@@ -535,13 +546,13 @@ CutResult planeCut(const Mesh& mesh, const float plane[4], double eps) {
       clip(neg, -1, i0, i1, i2);
     }
   }
-  CapCtx cp{&xyz, &pos, {n[0], n[1], n[2]}, -1.0};
+  CapCtx cp{&xyz, &pos, {n[0], n[1], n[2]}, -1.0, &sign, nv, closed};
   std::string why;
   if (!cap(cp, &why)) {
     res.reason = "pos cap: " + why;
     return res;
   }
-  CapCtx cn{&xyz, &neg, {n[0], n[1], n[2]}, 1.0};
+  CapCtx cn{&xyz, &neg, {n[0], n[1], n[2]}, 1.0, &sign, nv, closed};
   if (!cap(cn, &why)) {
     res.reason = "neg cap: " + why;
     return res;

@@ -17,7 +17,9 @@ struct CutResult {
 // src/cpu/cut.ts:33-107 planeCut: three-state signs at +-eps, Sutherland-Hodgman
 // clip with deduplicated edge intersections, boundary loops capped with holes
 // nested by parity (our own ear clipping; centroid fans as the fallback).
-CutResult planeCut(const Mesh& mesh, const float plane[4], double eps = PC_EPS);
+// closed: the caller knows the mesh is closed, which lets the cap look only at the
+// edges along the cut; the result is the same as without it.
+CutResult planeCut(const Mesh& mesh, const float plane[4], double eps = PC_EPS, bool closed = false);
 
 // Ear-clipping triangulation of a polygon with holes in 2D. `xy` holds the
 // outer ring followed by the holes (2 doubles per point); holeStart[k] is the

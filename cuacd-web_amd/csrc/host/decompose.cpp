@@ -667,7 +667,10 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
     const double ta0 = nowMs();
     std::vector<CutResult> cuts(seeds.size());
     parallelFor(seeds.size(), [&](size_t k) {
-      if (seeds[k].res.valid) cuts[k] = planeCut(S[seeds[k].m].parts[seeds[k].pi].mesh, seeds[k].res.plane);
+      // parts are closed when the input was checked and every applied cut was validated
+      if (seeds[k].res.valid)
+        cuts[k] = planeCut(S[seeds[k].m].parts[seeds[k].pi].mesh, seeds[k].res.plane, PC_EPS,
+                           o.requireClosed && o.validateCuts);
     });
     std::vector<const Mesh*> halves;
     std::vector<size_t> applied;  // seed index per applied cut

@@ -84,6 +84,24 @@ def test_cut_random_planes_conserve_volume(cw):
             assert cw.volume(pos) + cw.volume(neg) == pytest.approx(total, rel=1e-5)
 
 
+def test_cut_of_closed_parts_uses_only_edges_at_the_cut(cw, oracle):
+    """The decomposer tells the cut its parts are closed (the input was checked and every
+    applied cut validated), and the cap then looks only at edges with a crossing point or
+    an on-plane vertex. Without cut validation the cut scans every edge, as for any mesh;
+    on closed inputs both must give the same cuts, parts and volumes, bit for bit."""
+    import numpy as np
+    opts = dict(max_iters=5, threshold=0.05, width=12, width2=6, depth=2, n_concave_edges=8,
+                hausdorff_samples=300, merge_hulls=False)
+    for m in (stairs(cw), square_torus(cw), cw.generate("mix", 20260400, [3]),
+              cw.generate("blobs", 20260200, [3, 24, -0.6, 0.6, 0.12, 0.30, 0.5])):
+        a = cw.decompose(m, api=oracle, **opts)
+        b = cw.decompose(m, api=oracle, validate_cuts=False, **opts)
+        assert a.cuts == b.cuts and len(a.parts) == len(b.parts)
+        for x, y in zip(a.parts, b.parts):
+            assert np.array_equal(x.mesh.verts, y.mesh.verts) and np.array_equal(x.mesh.tris, y.mesh.tris)
+            assert x.mesh_vol == y.mesh_vol and x.hull_vol == y.hull_vol
+
+
 def test_concave_edge_planes(cw):
     """tests/cpu.test.ts:89-95: the L prism's junction edge gives 4 planes (16 floats);
     the unwelded L-shape and the cube give none."""
