@@ -21,7 +21,7 @@
 // candidate that removed it. Exported hulls take exactly the 16 largest keys
 // and cap the visible-face list at 128, so their face list is the same for
 // both group sizes (and the oracle's); tree candidates, of which only the
-// volume is used, take the largest keys of each wave. Visible faces are
+// volume is used, take the largest key of each group of lanes. Visible faces are
 // collected in LDS; horizon edges are found with an LDS hash of directed
 // edges. A candidate whose visible set does not fit in LDS is inserted by the
 // one-at-a-time path.
@@ -705,7 +705,8 @@ constexpr uint32_t HULL_LIST_CAP = 128;
 // visible-face list is capped at HULL_LIST_CAP, so the rounds and the face
 // list are the same for every group size (the oracle restates them).
 // Otherwise (tree candidates, which use only the exact grid volume, the same
-// for every insertion order): candidates are the largest keys of each wave.
+// for every insertion order): candidates are the largest key of each group of
+// G / K lanes.
 template <typename Grp, bool EXACT, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
 __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& S,
                                                uint32_t kEff) {
@@ -733,81 +734,95 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     S.conflict = 0;
     S.cutA = 0;
   }
-  // ---- candidates. Each lane offers its two largest face keys. EXACT: the
-  // third largest of every lane bounds everything not offered, so only keys
-  // above that bound are taken (they are then exactly the largest ones, in
-  // order). Otherwise each wave takes its largest kEff / NW offered keys.
-  // Candidate point numbers are the complemented low words of the keys.
-  const uint32_t kw = EXACT ? kEff : (kEff + NW - 1) / NW;
-  unsigned long long k1 = 0, k2 = 0, k3 = 0;
-  for (uint32_t f = t; f < C.nf; f += Grp::G) {
-    const unsigned long long k = C.fkey[f];
-    if (k > k1) {
-      k3 = k2;
-      k2 = k1;
-      k1 = k;
-    } else if (k > k2) {
-      k3 = k2;
-      k2 = k;
-    } else if (k > k3) {
-      k3 = k;
+  // ---- candidates; a candidate's point number is the complemented low word of its key
+  if constexpr (!EXACT) {
+    // Tree candidates: each group of G / K consecutive lanes offers its largest key,
+    // the farthest outside point of the faces it scans. Any outside points are valid
+    // candidates (the grid volume is the same for every insertion order); the groups'
+    // order, by lane, keeps the rounds deterministic.
+    static_assert(Grp::G % K == 0 && ((Grp::G / K) & (Grp::G / K - 1)) == 0 && Grp::G / K <= 64, "lane groups");
+    constexpr uint32_t GS = Grp::G / K;
+    unsigned long long k1 = 0;
+    for (uint32_t f = t; f < C.nf; f += Grp::G) {
+      const unsigned long long k = C.fkey[f];
+      k1 = k > k1 ? k : k1;
     }
-  }
-  {
-    uint32_t ptr = 0, cnt = 0;
-    for (uint32_t it = 0; it < kw; it++) {
-      const unsigned long long head = ptr == 0 ? k1 : (ptr == 1 ? k2 : 0ull);
-      const unsigned long long mx = wave_max_key(head);
-      if (mx == 0ull) break;
-      if (lane == 0) S.wkey[w][it] = mx;
-      if (head == mx) ptr++;  // keys are distinct: one lane matches
-      cnt++;
+#pragma unroll
+    for (uint32_t d = 1; d < GS; d <<= 1) {
+      const unsigned long long o = __shfl_xor(k1, (int)d, 64);
+      k1 = o > k1 ? o : k1;
     }
-    const unsigned long long bw = EXACT ? wave_max_key(k3) : 0ull;
-    if (lane == 0) {
-      S.wB[w] = bw;
-      S.wcnt[w] = cnt;
+    const bool v = (t & (GS - 1)) == 0 && k1 != 0ull;
+    uint32_t cnt;
+    const uint32_t r = g.rank(v, &cnt);
+    if (v) S.ckey[r] = k1;
+    if (t == 0) S.m = cnt;
+  } else {
+    // Exported hulls: each lane offers its two largest face keys; the third largest
+    // of every lane bounds everything not offered, so only keys above that bound
+    // are taken (they are then exactly the largest ones, in order).
+    const uint32_t kw = kEff;
+    unsigned long long k1 = 0, k2 = 0, k3 = 0;
+    for (uint32_t f = t; f < C.nf; f += Grp::G) {
+      const unsigned long long k = C.fkey[f];
+      if (k > k1) {
+        k3 = k2;
+        k2 = k1;
+        k1 = k;
+      } else if (k > k2) {
+        k3 = k2;
+        k2 = k;
+      } else if (k > k3) {
+        k3 = k;
+      }
     }
-  }
-  g.sync();
-  if (!EXACT) {
-    if (t < 64) {
-      // the waves' lists side by side (at most K entries)
-      const uint32_t q = lane / kw, i = lane - q * kw;
-      const bool v = q < NW && i < S.wcnt[q < NW ? q : 0];
-      const uint64_t bal = __ballot(v);
-      if (v) S.ckey[prefix_below(bal)] = S.wkey[q][i];
-      if (lane == 0) S.m = (uint32_t)__popcll(bal);
-    }
-  } else if (NW == 1) {
-    // one wave: its list is the answer, cut at its bound
-    const unsigned long long B = S.wB[0];
-    const uint32_t cnt = S.wcnt[0];
-    const bool above = lane < cnt && S.wkey[0][lane < K ? lane : 0] > B;
-    const uint32_t m0 = (uint32_t)__popcll(__ballot(above));
-    if (lane < m0) S.ckey[lane] = S.wkey[0][lane];
-    if (lane == 0) S.m = m0;
-  } else if (t < 64) {
-    // the waves' lists, two entries per lane, merged in registers
-    unsigned long long B = 0;
-    for (uint32_t q = 0; q < NW; q++) B = S.wB[q] > B ? S.wB[q] : B;
-    unsigned long long e0 = 0, e1 = 0;
     {
-      const uint32_t a0 = lane, a1 = lane + 64;
-      if (a0 < NW * K && (a0 % K) < S.wcnt[a0 / K]) e0 = S.wkey[a0 / K][a0 % K];
-      if (a1 < NW * K && (a1 % K) < S.wcnt[a1 / K]) e1 = S.wkey[a1 / K][a1 % K];
+      uint32_t ptr = 0, cnt = 0;
+      for (uint32_t it = 0; it < kw; it++) {
+        const unsigned long long head = ptr == 0 ? k1 : (ptr == 1 ? k2 : 0ull);
+        const unsigned long long mx = wave_max_key(head);
+        if (mx == 0ull) break;
+        if (lane == 0) S.wkey[w][it] = mx;
+        if (head == mx) ptr++;  // keys are distinct: one lane matches
+        cnt++;
+      }
+      const unsigned long long bw = EXACT ? wave_max_key(k3) : 0ull;
+      if (lane == 0) {
+        S.wB[w] = bw;
+        S.wcnt[w] = cnt;
+      }
     }
-    uint32_t m0 = 0;
-    for (uint32_t it = 0; it < kEff; it++) {
-      const unsigned long long head = e0 > e1 ? e0 : e1;
-      const unsigned long long mx = wave_max_key(head);
-      if (mx <= B) break;
-      if (lane == 0) S.ckey[it] = mx;
-      if (e0 == mx) e0 = 0;
-      if (e1 == mx) e1 = 0;
-      m0++;
+    g.sync();
+    if (NW == 1) {
+      // one wave: its list is the answer, cut at its bound
+      const unsigned long long B = S.wB[0];
+      const uint32_t cnt = S.wcnt[0];
+      const bool above = lane < cnt && S.wkey[0][lane < K ? lane : 0] > B;
+      const uint32_t m0 = (uint32_t)__popcll(__ballot(above));
+      if (lane < m0) S.ckey[lane] = S.wkey[0][lane];
+      if (lane == 0) S.m = m0;
+    } else if (t < 64) {
+      // the waves' lists, two entries per lane, merged in registers
+      unsigned long long B = 0;
+      for (uint32_t q = 0; q < NW; q++) B = S.wB[q] > B ? S.wB[q] : B;
+      unsigned long long e0 = 0, e1 = 0;
+      {
+        const uint32_t a0 = lane, a1 = lane + 64;
+        if (a0 < NW * K && (a0 % K) < S.wcnt[a0 / K]) e0 = S.wkey[a0 / K][a0 % K];
+        if (a1 < NW * K && (a1 % K) < S.wcnt[a1 / K]) e1 = S.wkey[a1 / K][a1 % K];
+      }
+      uint32_t m0 = 0;
+      for (uint32_t it = 0; it < kEff; it++) {
+        const unsigned long long head = e0 > e1 ? e0 : e1;
+        const unsigned long long mx = wave_max_key(head);
+        if (mx <= B) break;
+        if (lane == 0) S.ckey[it] = mx;
+        if (e0 == mx) e0 = 0;
+        if (e1 == mx) e1 = 0;
+        m0++;
+      }
+      if (lane == 0) S.m = m0;
     }
-    if (lane == 0) S.m = m0;
   }
   g.sync();
   uint32_t m = S.m;
