@@ -18,7 +18,7 @@
 //   - a candidate seeing a face an earlier one creates waits.
 // New faces are numbered (candidate, visible face in creation order, edge),
 // and a point whose owner face is removed moves to the best new face of the
-// candidate that removed it. Exported hulls take exactly the 16 largest keys
+// candidate that removed it. Exported hulls take exactly the 32 largest keys
 // and cap the visible-face list at 128, so their face list is the same for
 // both group sizes (and the oracle's); tree candidates, of which only the
 // volume is used, take the largest key of each group of lanes. Visible faces are
@@ -671,6 +671,8 @@ struct HullBatch {
   unsigned long long wkey[NW][EXACT ? K : K / NW];
   unsigned long long wB[NW];
   uint32_t wcnt[NW];
+  unsigned long long selT;  // exported-hull selection: the last key taken by a pass
+  uint32_t selMore;         // another pass is needed
   Mask conflict, cutA;
   uint32_t nList, nOrph, m, nPend;
 };
@@ -711,8 +713,7 @@ template <typename Grp, bool EXACT, uint32_t K, uint32_t LCAP, uint32_t HT, uint
 __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& S,
                                                uint32_t kEff) {
   static_assert(K <= 64 && (HT & (HT - 1)) == 0 && HT >= 6 * LCAP && NW == Grp::NW && LCAP <= 65536, "batch shape");
-  static_assert(EXACT ? (NW * K <= 128 && K <= 32 && LCAP >= HULL_LIST_CAP) : (K % NW == 0 && K / NW <= 64),
-                "batch shape");
+  static_assert(EXACT ? (K <= 32 && LCAP >= HULL_LIST_CAP) : (K % NW == 0 && K / NW <= 64), "batch shape");
   using Mask = typename HullBatch<EXACT, K, LCAP, HT, NW, NCAP>::Mask;
   constexpr uint32_t LC = EXACT ? HULL_LIST_CAP : LCAP;
   const uint32_t t = g.tid();
@@ -758,70 +759,85 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     if (v) S.ckey[r] = k1;
     if (t == 0) S.m = cnt;
   } else {
-    // Exported hulls: each lane offers its two largest face keys; the third largest
-    // of every lane bounds everything not offered, so only keys above that bound
-    // are taken (they are then exactly the largest ones, in order).
-    const uint32_t kw = kEff;
-    unsigned long long k1 = 0, k2 = 0, k3 = 0;
-    for (uint32_t f = t; f < C.nf; f += Grp::G) {
-      const unsigned long long k = C.fkey[f];
-      if (k > k1) {
-        k3 = k2;
-        k2 = k1;
-        k1 = k;
-      } else if (k > k2) {
-        k3 = k2;
-        k2 = k;
-      } else if (k > k3) {
-        k3 = k;
+    // Exported hulls: exactly the kEff largest keys, in order. Each lane offers its
+    // two largest keys below T; the largest third key B of any lane bounds what was
+    // not offered, so the offered keys above B are exactly the largest keys below T.
+    // When they are fewer than needed while keys remain at or below B, the next pass
+    // continues below the last key taken (every pass takes at least one key: the
+    // largest key below T is some lane's first, above that lane's third).
+    unsigned long long T = ~0ull;
+    uint32_t mtot = 0;
+    for (;;) {
+      const uint32_t need = kEff - mtot;
+      unsigned long long k1 = 0, k2 = 0, k3 = 0;
+      for (uint32_t f = t; f < C.nf; f += Grp::G) {
+        const unsigned long long k = C.fkey[f];
+        if (k >= T) continue;
+        if (k > k1) {
+          k3 = k2;
+          k2 = k1;
+          k1 = k;
+        } else if (k > k2) {
+          k3 = k2;
+          k2 = k;
+        } else if (k > k3) {
+          k3 = k;
+        }
       }
-    }
-    {
-      uint32_t ptr = 0, cnt = 0;
-      for (uint32_t it = 0; it < kw; it++) {
-        const unsigned long long head = ptr == 0 ? k1 : (ptr == 1 ? k2 : 0ull);
-        const unsigned long long mx = wave_max_key(head);
-        if (mx == 0ull) break;
-        if (lane == 0) S.wkey[w][it] = mx;
-        if (head == mx) ptr++;  // keys are distinct: one lane matches
-        cnt++;
-      }
-      const unsigned long long bw = EXACT ? wave_max_key(k3) : 0ull;
-      if (lane == 0) {
-        S.wB[w] = bw;
-        S.wcnt[w] = cnt;
-      }
-    }
-    g.sync();
-    if (NW == 1) {
-      // one wave: its list is the answer, cut at its bound
-      const unsigned long long B = S.wB[0];
-      const uint32_t cnt = S.wcnt[0];
-      const bool above = lane < cnt && S.wkey[0][lane < K ? lane : 0] > B;
-      const uint32_t m0 = (uint32_t)__popcll(__ballot(above));
-      if (lane < m0) S.ckey[lane] = S.wkey[0][lane];
-      if (lane == 0) S.m = m0;
-    } else if (t < 64) {
-      // the waves' lists, two entries per lane, merged in registers
-      unsigned long long B = 0;
-      for (uint32_t q = 0; q < NW; q++) B = S.wB[q] > B ? S.wB[q] : B;
-      unsigned long long e0 = 0, e1 = 0;
       {
-        const uint32_t a0 = lane, a1 = lane + 64;
-        if (a0 < NW * K && (a0 % K) < S.wcnt[a0 / K]) e0 = S.wkey[a0 / K][a0 % K];
-        if (a1 < NW * K && (a1 % K) < S.wcnt[a1 / K]) e1 = S.wkey[a1 / K][a1 % K];
+        uint32_t ptr = 0, cnt = 0;
+        for (uint32_t it = 0; it < need; it++) {
+          const unsigned long long head = ptr == 0 ? k1 : (ptr == 1 ? k2 : 0ull);
+          const unsigned long long mx = wave_max_key(head);
+          if (mx == 0ull) break;
+          if (lane == 0) S.wkey[w][it] = mx;
+          if (head == mx) ptr++;  // keys are distinct: one lane matches
+          cnt++;
+        }
+        const unsigned long long bw = wave_max_key(k3);
+        if (lane == 0) {
+          S.wB[w] = bw;
+          S.wcnt[w] = cnt;
+        }
       }
-      uint32_t m0 = 0;
-      for (uint32_t it = 0; it < kEff; it++) {
-        const unsigned long long head = e0 > e1 ? e0 : e1;
-        const unsigned long long mx = wave_max_key(head);
-        if (mx <= B) break;
-        if (lane == 0) S.ckey[it] = mx;
-        if (e0 == mx) e0 = 0;
-        if (e1 == mx) e1 = 0;
-        m0++;
+      g.sync();
+      if (t < 64) {
+        // the waves' lists (EPL entries per lane), merged in registers: the keys above B
+        constexpr uint32_t EPL = (NW * K + 63) / 64;
+        unsigned long long B = 0;
+        for (uint32_t q = 0; q < NW; q++) B = S.wB[q] > B ? S.wB[q] : B;
+        unsigned long long e[EPL];
+#pragma unroll
+        for (uint32_t i = 0; i < EPL; i++) {
+          const uint32_t ai = lane + 64 * i;
+          e[i] = (ai < NW * K && (ai % K) < S.wcnt[ai / K]) ? S.wkey[ai / K][ai % K] : 0ull;
+        }
+        uint32_t m0 = 0;
+        unsigned long long last = 0;
+        for (uint32_t it = 0; it < need; it++) {
+          unsigned long long head = 0;
+#pragma unroll
+          for (uint32_t i = 0; i < EPL; i++) head = e[i] > head ? e[i] : head;
+          const unsigned long long mx = wave_max_key(head);
+          if (mx <= B) break;
+          if (lane == 0) S.ckey[mtot + it] = mx;
+#pragma unroll
+          for (uint32_t i = 0; i < EPL; i++)
+            if (e[i] == mx) e[i] = 0;
+          last = mx;
+          m0++;
+        }
+        if (lane == 0) {
+          S.m = mtot + m0;
+          S.selMore = (m0 < need && B != 0ull) ? 1u : 0u;
+          S.selT = last;
+        }
       }
-      if (lane == 0) S.m = m0;
+      g.sync();
+      mtot = S.m;
+      if (!S.selMore) break;
+      T = S.selT;
+      g.sync();  // every thread has read the pass's results before the next writes them
     }
   }
   g.sync();
@@ -1616,7 +1632,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
 }
 
 // candidates per round: exported hulls (the same for both group sizes), tree candidates
-constexpr uint32_t KE = 16, KF_BIG = 64, KF_W = 16;
+constexpr uint32_t KE = 32, KF_BIG = 64, KF_W = 16;
 
 // small parts: one wavefront per part, HW parts per block
 template <bool EXACT>

@@ -186,7 +186,7 @@ struct Ctx {
   // of all candidates exceed LCAP, the round keeps the longest prefix of
   // candidates whose first-seen faces fit, once; failing that it inserts the
   // first candidate alone.
-  static constexpr uint32_t K = 16, LCAP = 128;
+  static constexpr uint32_t K = 32, LCAP = 128;
   void grow(const std::vector<uint32_t>& cand) {
     std::vector<uint32_t> c(cand);
     std::sort(c.begin(), c.end());

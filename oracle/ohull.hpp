@@ -7,7 +7,7 @@
 // exact integer fallback, optional last-level branch-and-bound.
 //
 // The rounds are the device's exported-hull rounds, one after another: the
-// farthest outside points of the 16 faces with the largest keys (larger
+// farthest outside points of the 32 faces with the largest keys (larger
 // distance, then lower point index) are the candidates; those that share no
 // visible face with an earlier candidate and see no face an earlier one
 // creates are inserted together. Every tie is broken by an order-independent
