@@ -252,7 +252,7 @@ void GpuBackend::ensurePools() {
   edges_.ensure(std::min<uint64_t>(cfg_.edgeCap, 0xffffffffull));
   scratch_.ensure(std::min<uint64_t>(cfg_.scratchWords, 0xffffffffull));
   // tetrahedra of one level: at most the child triangles and cap edges it writes
-  tetCap_ = std::min<uint64_t>(cfg_.triCap / 3 + cfg_.edgeCap, 128ull << 20);
+  tetCap_ = std::min<uint64_t>(cfg_.triCap / 3 + cfg_.edgeCap, 512ull << 20);
   tets_.ensure(3 * tetCap_);
 }
 

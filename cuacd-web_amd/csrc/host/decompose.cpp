@@ -43,6 +43,19 @@ TreeConfig treeConfigFor(const DecomposeOptions& o) {
   return c;
 }
 
+TreeConfig treeConfigForBatch(const DecomposeOptions& o, uint64_t totalTris) {
+  TreeConfig c = treeConfigFor(o);
+  uint64_t scale = 1;
+  while (scale < 4 && totalTris > scale * 400000ull) scale *= 2;
+  // the last level keeps only child vertices (every child's), so that pool binds first
+  if (!o.vertexCap) c.vertexCap = std::min<uint64_t>(c.vertexCap * 2 * scale, 0xffffffffull);
+  if (!o.triCap) c.triCap *= scale;
+  if (!o.edgeCap) c.edgeCap *= scale;
+  if (!o.scratchWords) c.scratchWords *= scale;
+  if (!o.hullBudgetBytes) c.hullBudgetBytes *= scale;
+  return c;
+}
+
 Mesh compactHull(const float* hv, const std::vector<uint32_t>& tris, double* volume) {
   std::unordered_map<uint32_t, uint32_t> remap;
   Mesh m;
@@ -650,8 +663,8 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
       st.candidateHulls += res.hullsDone;
       st.pruned += res.pruned;
       if (o.verbose >= 2)
-        std::fprintf(stderr, "[iter %u] tree run: %zu seeds, hulls %llu, pruned %u, %.1f ms\n", iter, refs.size(),
-                     (unsigned long long)res.hullsDone, res.pruned, nowMs() - tt0);
+        std::fprintf(stderr, "[iter %u] tree run: %zu seeds, hulls %llu, pruned %u, overflow pools %u, %.1f ms\n", iter,
+                     refs.size(), (unsigned long long)res.hullsDone, res.pruned, res.overflowPools, nowMs() - tt0);
       if (res.overflow) {
         if (batchN == 1) throw std::runtime_error("GPU pool overflow with a single part; increase tree pool sizes");
         batchN = std::max<size_t>(1, batchN >> 1);

@@ -68,6 +68,11 @@ struct DecomposeOptions {
 
 // TreeConfig a Decomposer runs with (src/decompose.ts:141-148)
 TreeConfig treeConfigFor(const DecomposeOptions& o);
+// The tree pools and the hull arena for a batch of meshes: the single-mesh defaults
+// scaled with the batch's total triangle count (x1 to x4 per 400k triangles; the
+// vertex pool twice that), where the options leave them at their defaults. One tree run then holds more seeds, so
+// fewer runs overflow a pool and are run again in halves.
+TreeConfig treeConfigForBatch(const DecomposeOptions& o, uint64_t totalTris);
 
 // src/merge.ts:54-66 MergeStats
 struct MergeStats {
