@@ -3,8 +3,9 @@ on one MI355X — the plain per-mesh GPU method.
 
 One step = one decompose(mesh, threshold 0.05, width 12, width2 6, depth 3, nConcaveEdges 16,
 refineRounds 2 x 8 probes, mergeHulls on, 3000 Hausdorff samples per part). Prints one JSON line.
-Beside it: the batched multi-mesh method against the plain one on the first meshes of the mixed
-workload (`batched_vs_plain`), and the CPU oracle on a bounded sample (`cpu_baseline`).
+Beside it: the batched multi-mesh method against the plain one on the 256-mesh mixed workload
+(`batched_vs_plain`; `--batched N` takes its first N meshes, 0 skips it), and the CPU oracle on a
+bounded sample (`cpu_baseline`).
 """
 import argparse
 import ctypes
@@ -151,7 +152,11 @@ def batched_vs_plain(cw, n):
     tb = time.perf_counter() - t0
     print(f"batched: {n} meshes in {tb:.2f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    rp = [cw.decompose(m, **MIX["options"]) for m in meshes]
+    rp = []
+    for k, m in enumerate(meshes):
+        rp.append(cw.decompose(m, **MIX["options"]))
+        if k % 32 == 31:
+            print(f"plain: {k + 1} meshes, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     tp = time.perf_counter() - t0
     print(f"plain: {n} meshes in {tp:.2f} s", file=sys.stderr, flush=True)
     same = all(a.cuts == b.cuts and len(a.parts) == len(b.parts) and
@@ -172,8 +177,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample-iters", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batched", type=int, default=32, metavar="N",
-                    help="also time the first N meshes of the mix batched and plain (0: skip)")
+    ap.add_argument("--batched", type=int, default=256, metavar="N",
+                    help="also time the first N meshes of the 256-mesh mix batched and plain (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (HBM counter calibration)")
     a = ap.parse_args()
