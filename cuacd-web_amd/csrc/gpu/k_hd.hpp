@@ -93,9 +93,56 @@ __device__ __forceinline__ float seg_sq_pre(float apx, float apy, float apz, flo
   return dot3f(dx, dy, dz, dx, dy, dz);
 }
 
+// point-triangle squared distance of src/cpu/hausdorff.ts:79-115 in FP64,
+// from the f32 sample and corners, rounded to f32 at the end: the path for
+// sliver triangles (below)
+__device__ __noinline__ float tri_sq_f64(float px, float py, float pz, float fax, float fay, float faz, float fbx,
+                                         float fby, float fbz, float fcx, float fcy, float fcz) {
+  const double ax = fax, ay = fay, az = faz, bx = fbx, by = fby, bz = fbz, cx = fcx, cy = fcy, cz = fcz;
+  const double x = px, y = py, z = pz;
+  auto seg = [](double qx, double qy, double qz, double sx, double sy, double sz, double ex, double ey, double ez) {
+    const double abx = ex - sx, aby = ey - sy, abz = ez - sz;
+    const double apx = qx - sx, apy = qy - sy, apz = qz - sz;
+    const double ab2 = abx * abx + aby * aby + abz * abz;
+    double t = ab2 > 1e-20 ? (apx * abx + apy * aby + apz * abz) / ab2 : 0.0;
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    const double dx = apx - t * abx, dy = apy - t * aby, dz = apz - t * abz;
+    return dx * dx + dy * dy + dz * dz;
+  };
+  const double abx = bx - ax, aby = by - ay, abz = bz - az;
+  const double acx = cx - ax, acy = cy - ay, acz = cz - az;
+  const double apx = x - ax, apy = y - ay, apz = z - az;
+  const double nx = aby * acz - abz * acy, ny = abz * acx - abx * acz, nz = abx * acy - aby * acx;
+  const double n2 = nx * nx + ny * ny + nz * nz;
+  const double d00 = abx * abx + aby * aby + abz * abz;
+  const double d01 = abx * acx + aby * acy + abz * acz;
+  const double d02 = abx * apx + aby * apy + abz * apz;
+  const double d11 = acx * acx + acy * acy + acz * acz;
+  const double d12 = acx * apx + acy * apy + acz * apz;
+  const double inv = 1.0 / (d00 * d11 - d01 * d01);
+  const double u = (d11 * d02 - d01 * d12) * inv;
+  const double v = (d00 * d12 - d01 * d02) * inv;
+  double r;
+  if (u >= 0.0 && v >= 0.0 && u + v <= 1.0) {
+    const double dist = nx * apx + ny * apy + nz * apz;
+    r = (dist * dist) / n2;
+  } else {
+    const double e0 = seg(x, y, z, ax, ay, az, bx, by, bz), e1 = seg(x, y, z, bx, by, bz, cx, cy, cz);
+    const double e2 = seg(x, y, z, cx, cy, cz, ax, ay, az);
+    r = e1 < e0 ? e1 : e0;
+    r = e2 < r ? e2 : r;
+  }
+  return (float)r;
+}
+
 // point-triangle squared distance (src/gpu/hausdorff_shader.ts:33-51 /
 // src/cpu/hausdorff.ts:79-115) over a precomputed record: the plane
-// projection when it falls inside the triangle, else the nearest edge
+// projection when it falls inside the triangle, else the nearest edge.
+// Slivers (sin^2 of the angle at a below 1e-2: n2 < 1e-2 d00 d11) take the
+// same formula in FP64. In f32 the inside test divides by d00 d11 - d01^2,
+// which cancels there, so a point far outside a sliver can pass the test and
+// get its small distance to the sliver's plane; in FP64 the test holds down to
+// sin^2 ~ 1e-14. Elsewhere the f32 test's error is at most ~1e2 ulps.
 __device__ __forceinline__ float tri_sq_pre(float px, float py, float pz, const float* T) {
   const float apx = px - T[0], apy = py - T[1], apz = pz - T[2];
   const float n2 = T[18];
@@ -105,6 +152,8 @@ __device__ __forceinline__ float tri_sq_pre(float px, float py, float pz, const 
     return fminx(dot3f(apx, apy, apz, apx, apy, apz),
                  fminx(dot3f(bpx, bpy, bpz, bpx, bpy, bpz), dot3f(cpx, cpy, cpz, cpx, cpy, cpz)));
   }
+  if (n2 < 1e-2f * (T[19] * T[21]))
+    return tri_sq_f64(px, py, pz, T[0], T[1], T[2], T[3], T[4], T[5], T[6], T[7], T[8]);
   const float d02 = dot3f(T[9], T[10], T[11], apx, apy, apz);
   const float d12 = dot3f(T[12], T[13], T[14], apx, apy, apz);
   const float inv = T[22];
@@ -156,11 +205,12 @@ __global__ void k_hd_prep(const HdMesh* __restrict__ meshes, uint32_t nMeshes, u
   const float inv = 1.0f / (d00 * d11 - d01 * d01);
   const float bcx = cx - bx, bcy = cy - by, bcz = cz - bz;
   const float cax = ax - cx, cay = ay - cy, caz = az - cz;
-  // rounding weight k^2 + k + 2 with k = |ab||ac|/|n| (2 for degenerate triangles)
+  // rounding weight k^2 + k + 2 with k = |ab||ac|/|n| (2 for degenerate
+  // triangles); slivers, whose distance is FP64, scale k^2 by 1e-8 > 2^-29
   double wgt = 2.0;
   if (!(n2 < 1e-20f)) {
     const double k = sqrt((double)d00) * sqrt((double)d11) / sqrt((double)n2);
-    wgt = k * k + k + 2.0;
+    wgt = n2 < 1e-2f * (d00 * d11) ? 2.0 + k * k * 1e-8 : k * k + k + 2.0;
   }
   float4* r = rec + 8 * (size_t)g;
   r[0] = make_float4(ax, ay, az, bx);
@@ -215,7 +265,8 @@ struct HdTile {  // HDS samples of one mesh against all triangles of the other m
 // A chunk can hold no triangle nearer to p than sqrt(best) after f32 rounding:
 // its box is farther than sqrt(best) by 1e-4 w of the box's far distance. The
 // f32 distance departs from the true one by a few ulps of |p - a| times
-// k^2 + k + 2 (k the sliver factor), so the margin holds with a wide factor.
+// k^2 + k + 2 (k the sliver factor, at most 10 on the f32 path; the FP64
+// path's departure is 2^-29 of that), so the margin holds with a wide factor.
 __device__ __forceinline__ bool hd_chunk_far(float px, float py, float pz, const float4& lo, const float4& hi, float best) {
   const double x = px, y = py, z = pz;
   const double nx = fmax(0.0, fmax((double)lo.x - x, x - (double)hi.x));

@@ -8,7 +8,8 @@
 // Branch-and-bound after the seed phase at the last tree level
 // (src/gpu/shaders.ts:1294-1331). The volume is that of the grid polytope,
 // exact (integer determinants, int128 sum), times the cell volume, so it does
-// not depend on the order of insertion.
+// not depend on the order of insertion. Exported hulls, whose face list is
+// canonical, report the volume of that surface over the original coordinates.
 //
 // Each face keeps the key (distance, ~index) of its farthest outside point. A
 // round takes the farthest points of the faces with the largest keys as
@@ -512,6 +513,29 @@ __device__ __forceinline__ double hull_volume(const Grp& g, const HullCtx& C) {
   const i128 a = acc < 0 ? -acc : acc;
   const double d = (double)(int64_t)(a >> 64) * 0x1p64 + (double)(uint64_t)a;
   return d / 6.0 * C.cell3;
+}
+
+// volume of the alive faces' surface over the ORIGINAL f32 coordinates of their
+// corners: each det[a; b; c] as error-free f32 product terms in the int128
+// fixed-point sum (order-independent), rounded once to FP64. Exported hulls
+// report it: their face list is canonical (the same as the oracle's), so this
+// surface is too, and it carries none of the grid's quantization (a cell is
+// 2^-21 of the extent, which moves a thin hull's volume by ~1e-6 relative).
+// Tree candidates keep the grid volume, which is the same for every insertion
+// order and triangulation.
+template <typename Grp>
+__device__ __forceinline__ double hull_volume_f32(const Grp& g, const HullCtx& C) {
+  i128 acc = 0;
+  for (uint32_t f = g.tid(); f < C.nf; f += Grp::G) {
+    const uint4 r = face_word(C, f, 0);
+    if (!(r.x & ALIVE)) continue;
+    const float* a = C.V + 3 * (size_t)(r.x & ~ALIVE);
+    const float* b = C.V + 3 * (size_t)r.y;
+    const float* c = C.V + 3 * (size_t)r.z;
+    acc_det(acc, a, b, c, false);
+  }
+  acc = g.sum(acc);
+  return fix_to_f64(fix_abs(acc)) / 6.0;
 }
 
 // stable in-place removal of dead face slots; face keys and candidate owners follow
@@ -1588,7 +1612,8 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
       if (done) break;
       }
       if (!done) {
-        res.volume = hull_volume(g, C);
+        if constexpr (EXACT) res.volume = hull_volume_f32(g, C);
+        else res.volume = hull_volume(g, C);
         // alive faces in creation order: count and optional export
         uint32_t w = 0;
         for (uint32_t base = 0; base < C.nf; base += Grp::G) {

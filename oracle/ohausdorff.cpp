@@ -6,6 +6,11 @@
 // evaluates them (src/gpu/hausdorff_shader.ts:24-51, :88-99), so that the
 // per-pair maxima agree bit for bit. The per-triangle counts come from the
 // shared host rule in sampling.cpp.
+// One departure from the shader: sliver triangles (n2 < 1e-2 d00 d11, sin^2 of
+// the angle at a below 1e-2) take the formula in FP64, as src/cpu/hausdorff.ts
+// evaluates it. In f32 the inside test's denominator d00 d11 - d01^2 cancels on
+// slivers, a far point can pass it, and the plane distance then undershoots by
+// orders of magnitude.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -59,6 +64,45 @@ float segSq(const float* p, const float* a, const float* b) {
   const float d[3] = {ap[0] - t * ab[0], ap[1] - t * ab[1], ap[2] - t * ab[2]};
   return dot3(d, d);
 }
+// src/cpu/hausdorff.ts:69-77 in FP64
+double segSq64(const double* p, const double* a, const double* b) {
+  const double abx = b[0] - a[0], aby = b[1] - a[1], abz = b[2] - a[2];
+  const double apx = p[0] - a[0], apy = p[1] - a[1], apz = p[2] - a[2];
+  const double ab2 = abx * abx + aby * aby + abz * abz;
+  double t = ab2 > 1e-20 ? (apx * abx + apy * aby + apz * abz) / ab2 : 0.0;
+  t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+  const double dx = apx - t * abx, dy = apy - t * aby, dz = apz - t * abz;
+  return dx * dx + dy * dy + dz * dz;
+}
+// src/cpu/hausdorff.ts:79-115 in FP64 from the f32 point and corners (the
+// degenerate case is decided in f32 before this is called), rounded to f32
+float distPtTriSq64(const float* fp, const float* fa, const float* fb, const float* fc) {
+  const double p[3] = {fp[0], fp[1], fp[2]}, a[3] = {fa[0], fa[1], fa[2]};
+  const double b[3] = {fb[0], fb[1], fb[2]}, c[3] = {fc[0], fc[1], fc[2]};
+  const double abx = b[0] - a[0], aby = b[1] - a[1], abz = b[2] - a[2];
+  const double acx = c[0] - a[0], acy = c[1] - a[1], acz = c[2] - a[2];
+  const double apx = p[0] - a[0], apy = p[1] - a[1], apz = p[2] - a[2];
+  const double nx = aby * acz - abz * acy, ny = abz * acx - abx * acz, nz = abx * acy - aby * acx;
+  const double n2 = nx * nx + ny * ny + nz * nz;
+  const double d00 = abx * abx + aby * aby + abz * abz;
+  const double d01 = abx * acx + aby * acy + abz * acz;
+  const double d02 = abx * apx + aby * apy + abz * apz;
+  const double d11 = acx * acx + acy * acy + acz * acz;
+  const double d12 = acx * apx + acy * apy + acz * apz;
+  const double inv = 1.0 / (d00 * d11 - d01 * d01);
+  const double u = (d11 * d02 - d01 * d12) * inv;
+  const double v = (d00 * d12 - d01 * d02) * inv;
+  double r;
+  if (u >= 0.0 && v >= 0.0 && u + v <= 1.0) {
+    const double dist = nx * apx + ny * apy + nz * apz;
+    r = (dist * dist) / n2;
+  } else {
+    const double e0 = segSq64(p, a, b), e1 = segSq64(p, b, c), e2 = segSq64(p, c, a);
+    r = e1 < e0 ? e1 : e0;
+    r = e2 < r ? e2 : r;
+  }
+  return (float)r;
+}
 }  // namespace
 
 float distPtTriSq(const float* p, const float* a, const float* b, const float* c) {
@@ -74,6 +118,7 @@ float distPtTriSq(const float* p, const float* a, const float* b, const float* c
   }
   const float d00 = dot3(ab, ab), d01 = dot3(ab, ac), d02 = dot3(ab, ap);
   const float d11 = dot3(ac, ac), d12 = dot3(ac, ap);
+  if (n2 < 1e-2f * (d00 * d11)) return distPtTriSq64(p, a, b, c);
   const float inv = 1.0f / (d00 * d11 - d01 * d01);
   const float u = (d11 * d02 - d01 * d12) * inv;
   const float v = (d00 * d12 - d01 * d02) * inv;
@@ -88,8 +133,9 @@ namespace {
 // Rounding weight of a triangle for the box test: the f32 distance above
 // departs from the true one by a few ulps of |p - a| (times the sliver factor
 // k = |ab||ac|/|n| for the plane distance, and k^2 for the inside/outside
-// decision), so w = k^2 + k + 2 scales the margin a box needs. Degenerate
-// triangles (vertex distances only) have w = 2.
+// decision), so w = k^2 + k + 2 scales the margin a box needs; k <= 10 where
+// the f32 formula runs. Slivers, whose distance is FP64, scale k^2 by
+// 1e-8 > 2^-29. Degenerate triangles (vertex distances only) have w = 2.
 double roundingWeight(const float* a, const float* b, const float* c) {
   const float ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
   const float ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
@@ -99,6 +145,7 @@ double roundingWeight(const float* a, const float* b, const float* c) {
   const double lab = std::sqrt((double)ab[0] * ab[0] + (double)ab[1] * ab[1] + (double)ab[2] * ab[2]);
   const double lac = std::sqrt((double)ac[0] * ac[0] + (double)ac[1] * ac[1] + (double)ac[2] * ac[2]);
   const double k = lab * lac / std::sqrt((double)n2);
+  if (n2 < 1e-2f * (dot3(ab, ab) * dot3(ac, ac))) return 2.0 + k * k * 1e-8;
   return k * k + k + 2.0;
 }
 

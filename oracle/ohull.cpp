@@ -320,6 +320,14 @@ struct Ctx {
       if (f.alive) acc += grid_det(&q[3 * (size_t)f.a], &q[3 * (size_t)f.b], &q[3 * (size_t)f.c]);
     return grid_volume(acc, cell3);
   }
+  // volume of the alive faces' surface over the original f32 coordinates: the
+  // exported hulls' volume (their face list is canonical; see ohull.hpp)
+  double volumeF32() const {
+    i128 acc = 0;
+    for (const Face& f : faces)
+      if (f.alive) acc_det(acc, &v[3 * (size_t)f.a], &v[3 * (size_t)f.b], &v[3 * (size_t)f.c], false);
+    return fix_to_f64(fix_abs(acc)) / 6.0;
+  }
 };
 
 }  // namespace
@@ -459,6 +467,7 @@ HullOut exactHull(const float* v, uint32_t nv, const HullParams& hp) {
   out.nCand = (uint32_t)all.size();
   C.grow(all);
   out.volume = C.volume();
+  out.volumeF32 = C.volumeF32();
   out.nIns = C.nIns;
   for (const Face& f : C.faces)
     if (f.alive) {

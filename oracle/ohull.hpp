@@ -14,7 +14,10 @@
 // rule, so the device reproduces the face list exactly. The volume is the
 // exact volume of the grid polytope (integer determinants, int128 sum) times
 // the cell volume: it depends only on the point set, not on the order of
-// insertion or the triangulation.
+// insertion or the triangulation. Exported hulls report instead the volume of
+// their face list's surface over the original f32 coordinates (error-free
+// product terms in the same kind of int128 sum), free of the grid's
+// quantization; it depends on the triangulation, which is canonical for them.
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -30,7 +33,8 @@ struct HullParams {
 
 struct HullOut {
   int status = 0;  // 0 exact hull, 1 degenerate (volume 0, no faces), 2 pruned (rough volume)
-  double volume = 0;
+  double volume = 0;             // grid polytope (tree candidates)
+  double volumeF32 = 0;          // face surface over the original coordinates (exported hulls)
   std::vector<uint32_t> faces;  // alive faces in creation order, outward
   uint32_t nCand = 0;           // points outside the seed hull
   uint32_t nIns = 0;            // insertions

@@ -634,7 +634,7 @@ std::vector<HullResult> OracleBackend::hullMeshes(const std::vector<MeshRef>& me
   for (size_t k = 0; k < meshes.size(); k++) {
     const HullOut h = exactHull(meshes[k].verts, meshes[k].nv, HullParams{});
     out[k].ok = true;
-    out[k].volume = h.volume;
+    out[k].volume = h.volumeF32;
     out[k].tris = h.faces;
   }
   return out;

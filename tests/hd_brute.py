@@ -1,8 +1,9 @@
 """Brute-force restatement of the sampled Hausdorff distance in numpy float32, for checking the
 oracle's BVH search: the same sample counts (src/cpu/hausdorff.ts:27-48), Wang-hash barycentrics
-(src/gpu/hausdorff_shader.ts:88-99) and point-triangle distance (src/gpu/hausdorff_shader.ts:33-51),
-every float32 operation in the order the kernels use, and the minimum over ALL triangles (no
-pruning). numpy rounds each float32 operation on its own (no fused multiply-add), so the result is
+(src/gpu/hausdorff_shader.ts:88-99) and point-triangle distance (src/gpu/hausdorff_shader.ts:33-51;
+sliver triangles, sin^2 of the angle at a below 1e-2, take src/cpu/hausdorff.ts:79-115 in float64,
+where the float32 inside test fails), every float32 operation in the order the kernels use, and the
+minimum over ALL triangles (no pruning). numpy rounds each float32 operation on its own (no fused multiply-add), so the result is
 bit-comparable. Test infrastructure only."""
 import numpy as np
 
@@ -121,7 +122,54 @@ def tri_sq(px, py, pz, T):
                  fminx(seg_sq(px, py, pz, bx, by, bz, cx, cy, cz), seg_sq(px, py, pz, cx, cy, cz, ax, ay, az)))
     with np.errstate(all="ignore"):
         ins = (u >= F(0.0)) & (v >= F(0.0)) & (u + v <= F(1.0))
-    return np.where(n2 < F(1e-20), degen, np.where(ins, inside, edge)).astype(F)
+    out = np.where(n2 < F(1e-20), degen, np.where(ins, inside, edge)).astype(F)
+    sliver = ~(n2 < F(1e-20)) & (n2 < F(1e-2) * (d00 * d11))
+    if sliver.any():
+        out = np.where(sliver, tri_sq64(px, py, pz, T), out).astype(F)
+    return out
+
+
+def seg_sq64(px, py, pz, ax, ay, az, bx, by, bz):
+    """src/cpu/hausdorff.ts:69-77 in float64."""
+    abx, aby, abz = bx - ax, by - ay, bz - az
+    apx, apy, apz = px - ax, py - ay, pz - az
+    ab2 = abx * abx + aby * aby + abz * abz
+    with np.errstate(all="ignore"):
+        t = np.where(ab2 > 1e-20, (apx * abx + apy * aby + apz * abz) / ab2, 0.0)
+    t = np.where(t < 0.0, 0.0, np.where(t > 1.0, 1.0, t))
+    dx, dy, dz = apx - t * abx, apy - t * aby, apz - t * abz
+    return dx * dx + dy * dy + dz * dz
+
+
+def tri_sq64(px, py, pz, T):
+    """src/cpu/hausdorff.ts:79-115 (past its degenerate test) in float64 from the float32 point and
+    corners, rounded to float32: the distance of sliver triangles."""
+    D = np.float64
+    px, py, pz = px.astype(D), py.astype(D), pz.astype(D)
+    ax, ay, az, bx, by, bz, cx, cy, cz = (T[:, i][None, :].astype(D) for i in range(9))
+    abx, aby, abz = bx - ax, by - ay, bz - az
+    acx, acy, acz = cx - ax, cy - ay, cz - az
+    apx, apy, apz = px - ax, py - ay, pz - az
+    nx, ny, nz = aby * acz - abz * acy, abz * acx - abx * acz, abx * acy - aby * acx
+    n2 = nx * nx + ny * ny + nz * nz
+    d00 = abx * abx + aby * aby + abz * abz
+    d01 = abx * acx + aby * acy + abz * acz
+    d02 = abx * apx + aby * apy + abz * apz
+    d11 = acx * acx + acy * acy + acz * acz
+    d12 = acx * apx + acy * apy + acz * apz
+    with np.errstate(all="ignore"):
+        inv = 1.0 / (d00 * d11 - d01 * d01)
+        u = (d11 * d02 - d01 * d12) * inv
+        v = (d00 * d12 - d01 * d02) * inv
+        dist = nx * apx + ny * apy + nz * apz
+        inside = (dist * dist) / n2
+        ins = (u >= 0.0) & (v >= 0.0) & (u + v <= 1.0)
+    e0 = seg_sq64(px, py, pz, ax, ay, az, bx, by, bz)
+    e1 = seg_sq64(px, py, pz, bx, by, bz, cx, cy, cz)
+    e2 = seg_sq64(px, py, pz, cx, cy, cz, ax, ay, az)
+    r = np.where(e1 < e0, e1, e0)
+    r = np.where(e2 < r, e2, r)
+    return np.where(ins, inside, r).astype(F)
 
 
 def directed(S, verts, tris):

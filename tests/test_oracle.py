@@ -202,10 +202,10 @@ def test_hull_vertices_match_brute_force(cw, oracle, seed, n):
     q = _quantize(pts)
     assert _exact_hull_vertices(q) == used  # general position: hull vertices are exactly the extreme points
     assert used <= set(range(n))
-    # volume equals the grid polytope's (scipy qhull, FP64), and the original one's up to the grid
+    # the exported volume is that of the face surface over the original coordinates: with the same
+    # vertex set as the original points' hull (general position), scipy qhull's FP64 volume up to rounding
     from scipy.spatial import ConvexHull
-    assert abs(vol - _grid_hull_volume(pts)) < 1e-12 * vol
-    assert abs(vol - ConvexHull(pts.astype(np.float64)).volume) < 1e-5 * vol
+    assert abs(vol - ConvexHull(pts.astype(np.float64)).volume) < 1e-12 * vol
 
 
 def test_hull_against_qhull_on_clouds(cw, oracle):
@@ -216,8 +216,8 @@ def test_hull_against_qhull_on_clouds(cw, oracle):
         tris, vol = _hull(cw, oracle, pts)
         h = ConvexHull(pts.astype(np.float64))
         assert set(int(x) for x in tris.reshape(-1)) == set(int(x) for x in h.vertices)
-        assert abs(vol - _grid_hull_volume(pts)) < 1e-12 * vol
-        assert abs(vol - h.volume) < 1e-5 * h.volume
+        assert abs(vol - h.volume) < 1e-12 * h.volume  # same vertices: FP64 volume up to rounding
+        assert abs(vol - _grid_hull_volume(pts)) < 1e-5 * vol  # the grid polytope differs by its cells
         m = cw.Mesh(pts, tris)
         assert cw.is_closed(m) and cw.signed_volume(m) > 0
 
@@ -267,9 +267,10 @@ def test_hull_against_reference_quickhull(cw, oracle):
 
 
 def test_hull_volume_independent_of_order(cw, oracle):
-    """The grid volume is exact, so it does not depend on which triangulation of the coplanar facets the
-    insertion order produced: a lattice cube (many coplanar points per facet) is exactly 1, and every
-    permutation of a cloud with coplanar clusters gives the same bits."""
+    """The exported volume is an exact sum (error-free products in fixed point) over the face surface, and
+    facets whose points are coplanar in f32 give the same surface for every triangulation: a lattice cube
+    (many coplanar points per facet) is exactly 1, and every permutation of a cloud with coplanar
+    clusters gives the same bits."""
     g = np.linspace(0.0, 1.0, 5, dtype=np.float32)
     lattice = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
     assert _hull(cw, oracle, lattice)[1] == 1.0
