@@ -326,6 +326,8 @@ struct HullCtx {
   uint32_t nf, nalive, ncand, nIns;
   unsigned long long tests;  // point-face record evaluations by this thread
   double cell3;              // volume of one grid cell
+  int64_t gridH;             // grid centre 2^(G-1) on the 2^G grid
+  float rejK;                // cheap-rejection factor 8 2^(G-21): the rounding grows with the coordinates
 };
 
 // Face records live in five arrays of uint4 (word k of face f at F4[k * fmax + f]):
@@ -342,13 +344,14 @@ __device__ __forceinline__ unsigned long long pkey(float d, uint32_t i) {
   return ((unsigned long long)__float_as_uint(d) << 32) | (uint32_t)(~i);
 }
 
-// Cheap rejection against a face plane. With grid coordinates in [0, 2^21],
+// Cheap rejection against a face plane. With grid coordinates in [0, 2^G],
 // c = n.p - n.qa evaluated with fused multiply-adds (d = n.qa the same way)
 // differs from the exact det N.(p - qa) by at most 3 eps |n|inf (|p|1 + 2 |qa|1)
-// for the rounding of the two dot products plus |n - N|inf |p - qa|1 <= 2 eps s1 s2 3 2^21
-// for the f32 normal; with |n|inf <= s1 s2 that is below 4.3 s1 s2 (eps = 2^-24).
-// So c < -8 s1 s2 proves the point strictly inside, and the f32 filter and the
-// exact test would say the same; any other point needs those tests.
+// for the rounding of the two dot products plus |n - N|inf |p - qa|1 <= 2 eps s1 s2 3 2^G
+// for the f32 normal; with |n|inf <= s1 s2 that is below 33 2^(G-24) s1 s2
+// (eps = 2^-24): 4.2 s1 s2 on the 2^21 grid, 33 s1 s2 on the 2^24 grid.
+// So c < -8 2^(G-21) s1 s2 proves the point strictly inside, and the f32 filter
+// and the exact test would say the same; any other point needs those tests.
 __device__ __forceinline__ float plane_offset(const uint4& w1, const uint4& w2) {
   return fmaf(__uint_as_float(w1.x), (float)(int32_t)w2.x,
               fmaf(__uint_as_float(w1.y), (float)(int32_t)w2.y, __uint_as_float(w1.z) * (float)(int32_t)w2.z));
@@ -357,7 +360,7 @@ __device__ __forceinline__ bool maybe_outside(const uint4& w1, float off, float 
   const float c = fmaf(__uint_as_float(w1.x), p.x, fmaf(__uint_as_float(w1.y), p.y, fmaf(__uint_as_float(w1.z), p.z, -off)));
   return c >= lim;
 }
-__device__ __forceinline__ float reject_limit(const uint4& w1) { return -8.0f * __uint_as_float(w1.w); }
+__device__ __forceinline__ float reject_limit(const uint4& w1, float k) { return -k * __uint_as_float(w1.w); }
 
 struct FaceRec {
   uint4 w[5];
@@ -446,7 +449,7 @@ __device__ __forceinline__ int face_filter(const uint4& w1, const uint4& w2, con
 __device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, const int32_t* qp) {
   const uint4 w0 = face_word(C, f, 0), w1 = face_word(C, f, 1), w2 = face_word(C, f, 2);
   if (!(w0.x & ALIVE)) return -1e30f;
-  if (!maybe_outside(w1, __uint_as_float(w0.w), reject_limit(w1), make_float4((float)qp[0], (float)qp[1], (float)qp[2], 0.0f)))
+  if (!maybe_outside(w1, __uint_as_float(w0.w), reject_limit(w1, C.rejK), make_float4((float)qp[0], (float)qp[1], (float)qp[2], 0.0f)))
     return -1.0f;
   float det;
   const int s = face_filter(w1, w2, qp, &det);
@@ -457,9 +460,9 @@ __device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, const i
   if (s < 0) return -1.0f;
   return orient_rec(C, f, w2, qp) > 0 ? 1e-30f : -1.0f;
 }
-__device__ __forceinline__ float face_eval_rec(const FaceRec& R, const int32_t* qp) {
+__device__ __forceinline__ float face_eval_rec(const FaceRec& R, const int32_t* qp, float rejK) {
   if (!(R.w[0].x & ALIVE)) return -1e30f;
-  if (!maybe_outside(R.w[1], __uint_as_float(R.w[0].w), reject_limit(R.w[1]),
+  if (!maybe_outside(R.w[1], __uint_as_float(R.w[0].w), reject_limit(R.w[1], rejK),
                      make_float4((float)qp[0], (float)qp[1], (float)qp[2], 0.0f)))
     return -1.0f;
   float det;
@@ -488,15 +491,14 @@ __device__ __forceinline__ uint32_t best_face(HullCtx& C, const int32_t* qp, uin
   return bestF;
 }
 
-// det[a; b; c] of grid corners relative to the grid centre (2^20 per axis):
-// coordinates within 2^20 of it, a 42-bit cross product, a 63-bit determinant
-__device__ __forceinline__ int64_t grid_det(const uint4& a, const uint4& b, const uint4& c) {
-  constexpr int64_t H = 1 << 20;
+// det[a; b; c] of grid corners relative to the grid centre H = 2^(G-1) per axis:
+// coordinates within 2^(G-1) of it, a 2G-bit cross product, a (3G+2)-bit determinant
+__device__ __forceinline__ i128 grid_det(const uint4& a, const uint4& b, const uint4& c, int64_t H) {
   const int64_t ax = (int64_t)(int32_t)a.x - H, ay = (int64_t)(int32_t)a.y - H, az = (int64_t)(int32_t)a.z - H;
   const int64_t bx = (int64_t)(int32_t)b.x - H, by = (int64_t)(int32_t)b.y - H, bz = (int64_t)(int32_t)b.z - H;
   const int64_t cx = (int64_t)(int32_t)c.x - H, cy = (int64_t)(int32_t)c.y - H, cz = (int64_t)(int32_t)c.z - H;
   const int64_t x = by * cz - bz * cy, y = bz * cx - bx * cz, z = bx * cy - by * cx;
-  return ax * x + ay * y + az * z;
+  return (i128)ax * x + (i128)ay * y + (i128)az * z;
 }
 
 // exact volume of the grid polytope (integer determinants of the alive faces'
@@ -507,7 +509,7 @@ __device__ __forceinline__ double hull_volume(const Grp& g, const HullCtx& C) {
   i128 acc = 0;
   for (uint32_t f = g.tid(); f < C.nf; f += Grp::G) {
     if (!(face_a(C, f) & ALIVE)) continue;
-    acc += (i128)grid_det(face_word(C, f, 2), face_word(C, f, 3), face_word(C, f, 4));
+    acc += grid_det(face_word(C, f, 2), face_word(C, f, 3), face_word(C, f, 4), C.gridH);
   }
   acc = g.sum(acc);
   const i128 a = acc < 0 ? -acc : acc;
@@ -926,7 +928,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
 #pragma unroll
       for (uint32_t b = 0; b < VB; b++) {
         off[b] = __uint_as_float(h0[b].w);
-        lim[b] = reject_limit(h1[b]);
+        lim[b] = reject_limit(h1[b], C.rejK);
         mlo[b] = mhi[b] = 0u;
       }
       const int mu = (int)__builtin_amdgcn_readfirstlane(m);
@@ -1091,7 +1093,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
           const int k1 = k == 2 ? 0 : k + 1;
           const FaceRec R = face_make(v3[k], v3[k1], S.cpt[j], q3[k], q3[k1], S.cq[j]);
           C.tests++;
-          if (face_eval_rec(R, S.cq[tjs[k]]) > 0.0f) atomicOr(&S.cutA, (Mask)1 << tjs[k]);
+          if (face_eval_rec(R, S.cq[tjs[k]], C.rejK) > 0.0f) atomicOr(&S.cutA, (Mask)1 << tjs[k]);
         }
     }
     S.lflag[e] = (uint8_t)flags;
@@ -1191,7 +1193,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     uint32_t bf = NONE;
     float bd = -1.0f;
     for (uint32_t rel = off; rel < r1; rel++) {
-      const float dd = rel < NCAP ? face_eval_rec(S.nrec[rel], qi) : face_eval(C, C.nf + rel, qi);
+      const float dd = rel < NCAP ? face_eval_rec(S.nrec[rel], qi, C.rejK) : face_eval(C, C.nf + rel, qi);
       if (dd > 0.0f && dd > bd) {
         bd = dd;
         bf = C.nf + rel;
@@ -1280,6 +1282,10 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
   C.nf = C.nalive = C.ncand = C.nIns = 0;
   C.tests = 0;
   C.cell3 = 0.0;
+  // exported hulls quantize to 2^24 (integers exact in f32), tree candidates to 2^21
+  constexpr int GB = EXACT ? 24 : 21;
+  C.gridH = (int64_t)1 << (GB - 1);
+  C.rejK = 8.0f * (float)(1 << (GB - 21));
   bool done = false;
   if (J.nv < 4) done = true;
   else if (hullSlabWords(J.nv) > J.slabWords) {
@@ -1287,7 +1293,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
     done = true;
   }
   if (!done) {
-    // bbox -> 2^21 grid
+    // bbox -> 2^GB grid
     float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
     for (uint32_t i = t; i < C.nv; i += Grp::G)
       for (int k = 0; k < 3; k++) {
@@ -1300,9 +1306,9 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
     float scale = ex > ey ? ex : ey;
     scale = scale > ez ? scale : ez;
     scale = scale > 1e-30f ? scale : 1e-30f;
-    const float qs = 2097152.0f / scale;
+    const float qs = (float)(1 << GB) / scale;
     {
-      const double cell = (double)scale * 0x1p-21;
+      const double cell = (double)scale * (EXACT ? 0x1p-24 : 0x1p-21);
       C.cell3 = cell * cell * cell;
     }
     for (uint32_t i = t; i < C.nv; i += Grp::G)
@@ -1566,7 +1572,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
                 const uint32_t s = s0 + (uint32_t)__ffsll((unsigned long long)km) - 1;
                 const uint4 w1 = stage[3 * s], w2 = stage[3 * s + 1];
                 C.tests++;
-                if (!maybe_outside(w1, __uint_as_float(stage[3 * s + 2].y), reject_limit(w1), pf)) continue;
+                if (!maybe_outside(w1, __uint_as_float(stage[3 * s + 2].y), reject_limit(w1, C.rejK), pf)) continue;
                 float det;
                 const int sg = face_filter(w1, w2, qi, &det);
                 float dd;

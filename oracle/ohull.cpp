@@ -55,6 +55,7 @@ struct Ctx {
   std::vector<std::vector<uint32_t>> members;  // conflict lists by owner face
   uint32_t nIns = 0;
   double cell3 = 0;
+  int bits = 21;  // grid size 2^bits
 
   inline float qf(uint32_t i, int k) const { return (float)q[3 * i + k]; }
 
@@ -317,7 +318,7 @@ struct Ctx {
   double volume() const {
     i128 acc = 0;
     for (const Face& f : faces)
-      if (f.alive) acc += grid_det(&q[3 * (size_t)f.a], &q[3 * (size_t)f.b], &q[3 * (size_t)f.c]);
+      if (f.alive) acc += grid_det(&q[3 * (size_t)f.a], &q[3 * (size_t)f.b], &q[3 * (size_t)f.c], bits);
     return grid_volume(acc, cell3);
   }
   // volume of the alive faces' surface over the original f32 coordinates: the
@@ -332,7 +333,7 @@ struct Ctx {
 
 }  // namespace
 
-Grid quantize(const float* v, uint32_t nv) {
+Grid quantize(const float* v, uint32_t nv, int bits) {
   Grid g;
   g.ok = nv > 0;
   g.q.assign(3 * (size_t)nv, 0);
@@ -345,7 +346,7 @@ Grid quantize(const float* v, uint32_t nv) {
     }
   const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
   const float scale = std::max(std::max(std::max(ex, ey), ez), 1e-30f);
-  const float qs = 2097152.0f / scale;
+  const float qs = std::ldexp(1.0f, bits) / scale;
   g.scale = scale;
   for (uint32_t i = 0; i < nv; i++)
     for (int k = 0; k < 3; k++) {
@@ -365,9 +366,10 @@ HullOut exactHull(const float* v, uint32_t nv, const HullParams& hp) {
   Ctx C;
   C.v = v;
   C.nv = nv;
-  Grid G = quantize(v, nv);
+  Grid G = quantize(v, nv, hp.gridBits);
+  C.bits = hp.gridBits;
   C.q.swap(G.q);
-  C.cell3 = grid_cell3(G.scale);
+  C.cell3 = grid_cell3(G.scale, hp.gridBits);
   // seeds: max and min along each direction (first index on ties)
   uint32_t seed[80];
   for (int k = 0; k < 40; k++) {

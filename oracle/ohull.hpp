@@ -1,7 +1,8 @@
 // ohull.hpp — ORACLE exact grid hull (test infrastructure only).
 //
 // Restatement of the hull kernel (src/gpu/shaders.ts:752-1401): points
-// quantized to a 2^21 grid over their bbox, 80 extreme seeds along 40
+// quantized to a 2^21 grid over their bbox (2^24 for exported hulls, whose
+// vertex choice is then 8x finer), 80 extreme seeds along 40
 // directions, a simplex from the seeds, insertion rounds (seeds first, then
 // every point outside the seed hull), orientation by an f32 filter with an
 // exact integer fallback, optional last-level branch-and-bound.
@@ -25,6 +26,7 @@
 namespace orc {
 
 struct HullParams {
+  int gridBits = 21;     // quantization grid 2^gridBits: 21 for tree candidates, 24 for exported hulls
   bool prune = false;    // branch-and-bound after the seed phase
   double meshVol = 0;    // part volume (for the bound)
   double partial = 0;    // parent item's level-cost sum
@@ -46,9 +48,9 @@ HullOut exactHull(const float* v, uint32_t nv, const HullParams& hp);
 // sign of det[b-a; c-a; p-a] on the quantization grid of the point set
 struct Grid {
   std::vector<int32_t> q;  // 3 per point
-  float scale = 0;         // largest bbox extent: the grid has 2^21 cells across it
+  float scale = 0;         // largest bbox extent: the grid has 2^bits cells across it
   bool ok;
 };
-Grid quantize(const float* v, uint32_t nv);
+Grid quantize(const float* v, uint32_t nv, int bits = 21);
 
 }  // namespace orc

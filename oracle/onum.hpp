@@ -69,23 +69,23 @@ inline void acc_det_anchor(i128& acc, const float* a, const float* b, const floa
   acc_det(acc, a, b, o, true);
 }
 
-// ---- exact hull volume on the 2^21 quantization grid
-// det[a; b; c] of grid corners taken relative to the grid centre (2^20 per
-// axis): every coordinate is within 2^20 of it, so the cross product fits 42
-// bits and the determinant 63 bits. Summed over a closed triangulated surface
-// it gives six times the enclosed volume in grid cells, exactly, whatever the
-// triangulation.
-inline int64_t grid_det(const int32_t* a, const int32_t* b, const int32_t* c) {
-  const int64_t H = 1 << 20;
+// ---- exact hull volume on a 2^G quantization grid (G = 21 or 24)
+// det[a; b; c] of grid corners taken relative to the grid centre (2^(G-1) per
+// axis): every coordinate is within 2^(G-1) of it, so the cross product fits
+// 2G bits and the determinant 3G + 2 (int128). Summed over a closed
+// triangulated surface it gives six times the enclosed volume in grid cells,
+// exactly, whatever the triangulation.
+inline i128 grid_det(const int32_t* a, const int32_t* b, const int32_t* c, int bits) {
+  const int64_t H = (int64_t)1 << (bits - 1);
   const int64_t ax = a[0] - H, ay = a[1] - H, az = a[2] - H;
   const int64_t bx = b[0] - H, by = b[1] - H, bz = b[2] - H;
   const int64_t cx = c[0] - H, cy = c[1] - H, cz = c[2] - H;
   const int64_t x = by * cz - bz * cy, y = bz * cx - bx * cz, z = bx * cy - by * cx;
-  return ax * x + ay * y + az * z;
+  return (i128)ax * x + (i128)ay * y + (i128)az * z;
 }
-// cube of the grid cell, (scale / 2^21)^3 in FP64
-inline double grid_cell3(float scale) {
-  const double c = (double)scale * 0x1p-21;
+// cube of the grid cell, (scale / 2^G)^3 in FP64
+inline double grid_cell3(float scale, int bits) {
+  const double c = std::ldexp((double)scale, -bits);
   return c * c * c;
 }
 // |six| / 6 grid cells in the units of the original coordinates

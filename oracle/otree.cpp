@@ -632,7 +632,9 @@ std::vector<HullResult> OracleBackend::hullMeshes(const std::vector<MeshRef>& me
   std::vector<HullResult> out(meshes.size());
 #pragma omp parallel for schedule(dynamic, 1)
   for (size_t k = 0; k < meshes.size(); k++) {
-    const HullOut h = exactHull(meshes[k].verts, meshes[k].nv, HullParams{});
+    HullParams hp;
+    hp.gridBits = 24;  // exported hulls: the finer grid
+    const HullOut h = exactHull(meshes[k].verts, meshes[k].nv, hp);
     out[k].ok = true;
     out[k].volume = h.volumeF32;
     out[k].tris = h.faces;

@@ -5,7 +5,8 @@
  *
  * Kernel formulas: normal n = e1 x e2 in f32 from integer corner differences,
  * s = (|e1|1) (|e2|1), off = fma(n0, qa0, fma(n1, qa1, n2 qa2)),
- * c = fma(n0, p0, fma(n1, p1, fma(n2, p2, -off))), rejected when c < -8 s. */
+ * c = fma(n0, p0, fma(n1, p1, fma(n2, p2, -off))), rejected when c < -8 2^(G-21) s
+ * on a 2^G grid (G = 21 for tree candidates, 24 for exported hulls). */
 #include <math.h>
 #include <stdint.h>
 
@@ -18,12 +19,15 @@ static uint64_t rng(void) {
   rng_state ^= rng_state << 17;
   return rng_state;
 }
-static int32_t coord(void) { return (int32_t)(rng() % ((1u << 21) + 1)); }
+static int G = 21;
+static int32_t coord(void) { return (int32_t)(rng() % ((1u << G) + 1)); }
 
 /* out[0] rejections, out[1] violations (rejected but det >= 0), out[2] pairs
  * with det < 0 that were not rejected, out[3] worst (c - det) / s scaled by 1e6 */
-void reject_check(uint64_t seed, int trials, int64_t* out) {
+void reject_check(uint64_t seed, int trials, int bits, int64_t* out) {
   rng_state = seed | 1;
+  G = bits;
+  const float lim = -8.0f * (float)(1 << (bits - 21));
   out[0] = out[1] = out[2] = out[3] = 0;
   double worst = 0;
   for (int t = 0; t < trials; t++) {
@@ -43,7 +47,7 @@ void reject_check(uint64_t seed, int trials, int64_t* out) {
       for (int k = 0; k < 3; k++) {
         double x = qa[k] + al * (qb[k] - qa[k]) + be * (qc[k] - qa[k]) + (double)((int)(rng() % 9) - 4);
         if (kind == 3) x = qa[k] + 3.0 * al * (qb[k] - qa[k]) - 2.0 * be * (qc[k] - qa[k]);
-        x = x < 0 ? 0 : (x > (1 << 21) ? (1 << 21) : x);
+        x = x < 0 ? 0 : (x > (1 << G) ? (1 << G) : x);
         p[k] = (int32_t)llround(x);
       }
     } else {
@@ -53,7 +57,7 @@ void reject_check(uint64_t seed, int trials, int64_t* out) {
       if (qa[k] < 0) qa[k] = 0;
     for (int k = 0; k < 3; k++) {
       if (qc[k] < 0) qc[k] = 0;
-      if (qc[k] > (1 << 21)) qc[k] = 1 << 21;
+      if (qc[k] > (1 << G)) qc[k] = 1 << G;
     }
     const float e1[3] = {(float)qb[0] - (float)qa[0], (float)qb[1] - (float)qa[1], (float)qb[2] - (float)qa[2]};
     const float e2[3] = {(float)qc[0] - (float)qa[0], (float)qc[1] - (float)qa[1], (float)qc[2] - (float)qa[2]};
@@ -70,7 +74,7 @@ void reject_check(uint64_t seed, int trials, int64_t* out) {
     const int64_t D[3] = {(int64_t)p[0] - qa[0], (int64_t)p[1] - qa[1], (int64_t)p[2] - qa[2]};
     const int64_t N[3] = {E1[1] * E2[2] - E1[2] * E2[1], E1[2] * E2[0] - E1[0] * E2[2], E1[0] * E2[1] - E1[1] * E2[0]};
     const i128 det = (i128)N[0] * D[0] + (i128)N[1] * D[1] + (i128)N[2] * D[2];
-    const int rejected = c < -8.0f * s;
+    const int rejected = c < lim * s;
     if (rejected) out[0]++;
     if (rejected && det >= 0) out[1]++;
     if (!rejected && det < 0) out[2]++;

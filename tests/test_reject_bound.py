@@ -1,9 +1,12 @@
 """The hull kernel's cheap plane rejection (n.p - n.qa in f32 with fused
-multiply-adds, rejected below -8 (|e1|1 |e2|1)) must only reject points that
-are strictly inside the face's plane by the exact integer determinant. The
-kernel's float formulas are restated in C (tests/reject_bound.c) and checked on
-random and near-coplanar triangles and points of the 2^21 grid; the largest
-observed error must also stay within the derived bound of 4.3 |e1|1 |e2|1."""
+multiply-adds, rejected below -8 2^(G-21) (|e1|1 |e2|1) on a 2^G grid) must only
+reject points that are strictly inside the face's plane by the exact integer
+determinant. The kernel's float formulas are restated in C (tests/reject_bound.c)
+and checked on random and near-coplanar triangles and points of the 2^21 grid
+(tree candidates) and the 2^24 grid (exported hulls). The rounding grows with the
+coordinates: 3 eps s (|p|1 + 2 |qa|1) + 2 eps s 3 2^G <= 33 2^(G-24) s, that is
+4.125 s at 21 bits and 33 s at 24; the largest observed error must stay within
+4.3 2^(G-21) s."""
 import ctypes
 import os
 import shutil
@@ -22,17 +25,18 @@ def reject_lib(tmp_path_factory):
     subprocess.run(["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", "-o", str(out),
                     os.path.join(HERE, "reject_bound.c"), "-lm"], check=True)
     lib = ctypes.CDLL(str(out))
-    lib.reject_check.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    lib.reject_check.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     return lib
 
 
+@pytest.mark.parametrize("bits", [21, 24])
 @pytest.mark.parametrize("seed", [1, 20260401, 987654321])
-def test_rejection_never_rejects_a_point_outside_or_on(reject_lib, seed):
+def test_rejection_never_rejects_a_point_outside_or_on(reject_lib, seed, bits):
     out = (ctypes.c_int64 * 4)()
-    reject_lib.reject_check(seed, 400000, out)
+    reject_lib.reject_check(seed, 400000, bits, out)
     rejected, violations, missed, worst = out[0], out[1], out[2], out[3] / 1e6
     assert violations == 0
     print("worst error per |e1|1 |e2|1:", worst)
-    assert worst <= 4.3, worst
+    assert worst <= 4.3 * 2 ** (bits - 21), worst
     # the test is not vacuous: most random pairs are rejected, near-coplanar ones are not
     assert rejected > 50000 and missed > 1000, (rejected, missed)
