@@ -5,7 +5,8 @@ One step = one decompose(mesh, threshold 0.05, width 12, width2 6, depth 3, nCon
 refineRounds 2 x 8 probes, mergeHulls on, 3000 Hausdorff samples per part). Prints one JSON line.
 Beside it: the batched multi-mesh method against the plain one on the 256-mesh mixed workload
 (`batched_vs_plain`; `--batched N` takes its first N meshes, 0 skips it), and the CPU oracle on a
-bounded sample (`cpu_baseline`).
+bounded sample (`cpu_baseline`). Each workload runs with the hull kernel split that
+cuacd-web_amd/autotune.py measured for it (cuacd-web_amd/tuned.json), when that file exists.
 """
 import argparse
 import ctypes
@@ -51,6 +52,16 @@ COUNTERS = ("face_tests", "face_tests_big", "emit_bytes", "hd_evals", "hull_jobs
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3    # MI355X dense FP32
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+TUNED_FILE = os.path.join(PKG_DIR, "tuned.json")  # written by cuacd-web_amd/autotune.py
+
+
+def tuned_split(workload):
+    """The hull kernel split (hull_wg_min_verts) the autotuner measured best for a workload; 0 (the
+    library default) when it has not been tuned. The split changes timing only, never results."""
+    if not os.path.exists(TUNED_FILE):
+        return 0
+    with open(TUNED_FILE) as f:
+        return int(json.load(f).get(workload, {}).get("hull_wg_min_verts", 0))
 
 
 def sum_kernels(results):
@@ -146,15 +157,16 @@ def batched_vs_plain(cw, n):
     twisted tori), decomposed together (one tree run per iteration over every mesh's seeds, shared
     hull and Hausdorff batches) and one by one; both wall times, and whether the results agree."""
     meshes = [cw.generate("mix", MIX["seed"], [k]) for k in range(n)]
-    cw.decompose_batch(meshes[:4], **MIX["options"])  # warm
+    opts = dict(MIX["options"], hull_wg_min_verts=tuned_split("mix"))
+    cw.decompose_batch(meshes[:4], **opts)  # warm
     t0 = time.perf_counter()
-    rb = cw.decompose_batch(meshes, **MIX["options"])
+    rb = cw.decompose_batch(meshes, **opts)
     tb = time.perf_counter() - t0
     print(f"batched: {n} meshes in {tb:.2f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     rp = []
     for k, m in enumerate(meshes):
-        rp.append(cw.decompose(m, **MIX["options"]))
+        rp.append(cw.decompose(m, **opts))
         if k % 32 == 31:
             print(f"plain: {k + 1} meshes, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     tp = time.perf_counter() - t0
@@ -163,7 +175,7 @@ def batched_vs_plain(cw, n):
                all(x.hull_vol == y.hull_vol and x.mesh_vol == y.mesh_vol and x.hausdorff == y.hausdorff
                    for x, y in zip(a.parts, b.parts)) for a, b in zip(rb, rp))
     return dict(workload=f"first {n} meshes of the 256-mesh mix (seed {MIX['seed']}), "
-                         f"{sum(m.nt for m in meshes)} tris", options=MIX["options"],
+                         f"{sum(m.nt for m in meshes)} tris", options=opts,
                 batched_meshes_per_s=n / tb, plain_meshes_per_s=n / tp, ratio=tp / tb,
                 candidate_hulls=sum(r.stats["candidate_hulls"] for r in rp),
                 batched_candidate_hulls_per_s=sum(r.stats["candidate_hulls"] for r in rp) / tb,
@@ -190,12 +202,13 @@ def main():
     if a.copy_probe:
         copy_probe(a.copy_probe)
     mesh = cw.generate("blobs", WORKLOAD["seed"], WORKLOAD["params"])
+    opts = dict(OPTIONS, hull_wg_min_verts=tuned_split("blob"))
     for _ in range(a.warmup):
-        cw.decompose(mesh, **OPTIONS)
+        cw.decompose(mesh, **opts)
     times, results = [], []
     for i in range(a.steps):
         t0 = time.perf_counter()
-        r = cw.decompose(mesh, **OPTIONS)
+        r = cw.decompose(mesh, **opts)
         times.append(time.perf_counter() - t0)
         results.append(r)
         print(f"step {i}: {times[-1]:.3f} s, {len(r.parts)} parts", file=sys.stderr, flush=True)
@@ -217,7 +230,7 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic (seeded sum-of-Gaussians mesh)",
         "config": {"workload": f"plain decompose, {WORKLOAD['name']}, {mesh.nt} tris", "method": "plain",
-                   "options": OPTIONS, "parts": len(r.parts), "iterations": r.iterations},
+                   "options": opts, "parts": len(r.parts), "iterations": r.iterations},
         "candidate_hulls_per_s": hulls / step_s,
         "candidate_hulls": hulls,
         "pruned": r.stats["pruned"],

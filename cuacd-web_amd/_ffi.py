@@ -14,7 +14,7 @@ class TreeConfig(C.Structure):
                 ("min_edge_dist", f32), ("hull_max_verts", u32),
                 ("vertex_cap", u64), ("tri_cap", u64), ("edge_cap", u64), ("scratch_words", u64),
                 ("hull_budget_bytes", u64), ("prune_last_level", i32), ("hausdorff_samples", u32),
-                ("device", i32)]
+                ("device", i32), ("hull_wg_min_verts", u32)]
 
 
 class MeshBatch(C.Structure):
@@ -53,7 +53,8 @@ class DecomposeOptions(C.Structure):
                 ("refine_rounds", u32), ("refine_probes", u32), ("merge_hulls", i32), ("merge_rounds", u32),
                 ("merge_plane_tol", f64), ("verbose", u32), ("hausdorff_samples", u32),
                 ("vertex_cap", u64), ("tri_cap", u64), ("edge_cap", u64), ("scratch_words", u64),
-                ("hull_budget_bytes", u64), ("hull_max_verts", u32), ("prune_last_level", i32), ("device", i32)]
+                ("hull_budget_bytes", u64), ("hull_max_verts", u32), ("prune_last_level", i32), ("device", i32),
+                ("hull_wg_min_verts", u32)]
 
 
 class PartDesc(C.Structure):

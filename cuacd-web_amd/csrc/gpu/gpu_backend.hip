@@ -266,7 +266,7 @@ void GpuBackend::runHulls(std::vector<dk::HullJob>& hjIn, dk::DPart* parts, uint
   // large parts first (a workgroup each), then small ones (a wavefront each)
   std::vector<uint32_t> order(hjIn.size());
   for (uint32_t k = 0; k < order.size(); k++) order[k] = k;
-  std::stable_partition(order.begin(), order.end(), [&](uint32_t k) { return hjIn[k].nv >= dk::HBIG_NV; });
+  std::stable_partition(order.begin(), order.end(), [&](uint32_t k) { return hjIn[k].nv >= cfg_.hullWgMinVerts; });
   std::vector<dk::HullJob> hj(hjIn.size());
   for (size_t k = 0; k < order.size(); k++) hj[k] = hjIn[order[k]];
   const uint64_t budgetWords = std::max<uint64_t>(cfg_.hullBudgetBytes / 4, 1ull << 20);
@@ -288,7 +288,7 @@ void GpuBackend::runHulls(std::vector<dk::HullJob>& hjIn, dk::DPart* parts, uint
       end++;
     }
     size_t mid = start;
-    while (mid < end && hj[mid].nv >= dk::HBIG_NV) mid++;
+    while (mid < end && hj[mid].nv >= cfg_.hullWgMinVerts) mid++;
     h2d(hjobs_.p + start, hj.data() + start, (end - start) * sizeof(dk::HullJob), "upload hull jobs");
     // The large-part kernel holds one workgroup per part, usually far fewer than the
     // chip's CUs, for its whole duration: the small parts run beside it on the side

@@ -46,7 +46,6 @@ namespace dk {
 
 constexpr int HW = 4;         // parts per block in the wave-per-part hull kernel
 constexpr int HBIG = 512;     // threads per part in the workgroup-per-part hull kernel
-constexpr uint32_t HBIG_NV = 1024;  // parts with at least this many points get a whole workgroup
 constexpr uint32_t ALIVE = 0x80000000u;
 constexpr uint32_t FW = 20;                 // words per face record (five uint4 words, one array each)
 

@@ -33,6 +33,9 @@ struct TreeConfig {
   // N > 0 = exactly the same distribution rule with res = N.
   uint32_t hausdorffSamples = 0;
   int device = 0;  // HIP device ordinal
+  // Parts with at least this many points get a workgroup in the hull kernel, the
+  // rest a wavefront. Tree volumes and exported face lists are the same either way.
+  uint32_t hullWgMinVerts = 1024;
 };
 
 // A HIP runtime failure (reported as CUACD_ERR_DEVICE at the C ABI).

@@ -39,6 +39,7 @@ TreeConfig treeConfigFor(const DecomposeOptions& o) {
   if (o.hullBudgetBytes) c.hullBudgetBytes = o.hullBudgetBytes;
   if (o.hullMaxVerts) c.hullMaxVerts = o.hullMaxVerts;
   if (o.pruneLastLevel >= 0) c.pruneLastLevel = o.pruneLastLevel != 0;
+  if (o.hullWgMinVerts) c.hullWgMinVerts = o.hullWgMinVerts;
   c.hausdorffSamples = o.hausdorffSamples;
   return c;
 }

@@ -64,6 +64,7 @@ struct DecomposeOptions {
   uint64_t vertexCap = 0, triCap = 0, edgeCap = 0, scratchWords = 0, hullBudgetBytes = 0;
   uint32_t hullMaxVerts = 0;
   int pruneLastLevel = -1;  // -1 = default (on)
+  uint32_t hullWgMinVerts = 0;  // 0 = default (TreeConfig::hullWgMinVerts)
 };
 
 // TreeConfig a Decomposer runs with (src/decompose.ts:141-148)

@@ -69,6 +69,7 @@ class DecomposeOptions:
     hull_max_verts: int = 0
     prune_last_level: int = -1
     device: int = 0
+    hull_wg_min_verts: int = 0  # hull kernel split (0: 1024); a performance knob, results unchanged
 
     def raw(self):
         r = F.DecomposeOptions()

@@ -35,6 +35,10 @@ typedef struct cuacd_tree_config {
   int32_t prune_last_level;
   uint32_t hausdorff_samples; /* 0 = max(1000, floor(2000*area)) */
   int32_t device;
+  /* Parts with at least this many points get a workgroup in the hull kernel, the
+   * rest a wavefront; 0 = 1024. A tuning knob with no reference counterpart:
+   * results do not depend on it. */
+  uint32_t hull_wg_min_verts;
 } cuacd_tree_config;
 
 /* A batch of meshes packed as the reference uploads them (src/gpu/tree.ts:240-262):
@@ -140,6 +144,7 @@ typedef struct cuacd_decompose_options {
   uint32_t hull_max_verts;
   int32_t prune_last_level;
   int32_t device;
+  uint32_t hull_wg_min_verts; /* as in cuacd_tree_config; 0 = 1024 */
 } cuacd_decompose_options;
 
 /* src/part.ts:9-22 Part, flattened */

@@ -101,3 +101,17 @@ def test_stl_49160_runs(cw, oracle):
     r = _both(cw, oracle, mesh, max_iters=3, threshold=0.05, width=30, merge_hulls=False, **SMALL)
     total = sum(cw.volume(p.mesh) for p in r.parts)
     assert abs(total - cw.volume(mesh)) < cw.volume(mesh) * 1e-3
+
+
+def test_hull_kernel_split_does_not_change_results(cw):
+    """Parts go to the workgroup or the wavefront hull kernel by their point count (hull_wg_min_verts).
+    Tree candidates' volumes do not depend on the insertion order and exported hulls' face lists do not
+    depend on the group size, so the decomposition is the same with every part in either kernel."""
+    mesh = cw.generate("blobs", 20260210, [6, 40, -0.6, 0.6, 0.12, 0.30, 0.5])
+    opts = dict(threshold=0.05, width=12, width2=6, depth=3, n_concave_edges=8, refine_rounds=1, refine_probes=4,
+                merge_hulls=True, hausdorff_samples=500)
+    print(f"\nblob: {mesh.nt} triangles", flush=True)
+    ref = cw.decompose(mesh, **opts)
+    for split in (1, 1 << 30):  # every part in the workgroup kernel, then every part in the wave kernel
+        print(f"hull_wg_min_verts={split}", flush=True)
+        _same(ref, cw.decompose(mesh, hull_wg_min_verts=split, **opts))
