@@ -5,7 +5,8 @@ One step = one decompose(mesh, threshold 0.05, width 12, width2 6, depth 3, nCon
 refineRounds 2 x 8 probes, mergeHulls on, 3000 Hausdorff samples per part). Prints one JSON line.
 Beside it: the batched multi-mesh method against the plain one on the 256-mesh mixed workload
 (`batched_vs_plain`; `--batched N` takes its first N meshes, 0 skips it), and the CPU oracle on a
-bounded sample (`cpu_baseline`). Each workload runs with the hull kernel split that
+bounded sample (`cpu_baseline`) and on the first meshes of the mix (`batched_vs_plain.cpu`,
+`--cpu-mix K`). Each workload runs with the hull kernel split that
 cuacd-web_amd/autotune.py measured for it (cuacd-web_amd/tuned.json), when that file exists.
 """
 import argparse
@@ -132,8 +133,7 @@ def cpu_baseline(cw, mesh, gpu_step_s, sample_iters):
     """The oracle (the CPU restatement, its hull and Hausdorff loops over OpenMP host threads) on the
     first `sample_iters` decomposition iterations; meshes/s scaled by the GPU's own full-run /
     sample-run time ratio."""
-    from cuacd_web_amd._ffi import Api
-    oracle = Api(ctypes.CDLL(os.path.join(ROOT, "oracle", "build", "liboracle.so"), mode=ctypes.RTLD_LOCAL), "oracle_")
+    oracle = _oracle(cw)
     opts = dict(OPTIONS, max_iters=sample_iters, merge_hulls=False)
     t0 = time.perf_counter()
     cw.decompose(mesh, **opts)
@@ -152,10 +152,23 @@ def cpu_baseline(cw, mesh, gpu_step_s, sample_iters):
 MIX = dict(seed=20260400, options=dict(threshold=0.1, width=12, depth=3, merge_hulls=True))
 
 
-def batched_vs_plain(cw, n):
+def _oracle(cw):
+    from cuacd_web_amd._ffi import Api
+    return Api(ctypes.CDLL(os.path.join(ROOT, "oracle", "build", "liboracle.so"), mode=ctypes.RTLD_LOCAL), "oracle_")
+
+
+def _same_result(a, b):
+    return a.cuts == b.cuts and len(a.parts) == len(b.parts) and all(
+        x.hull_vol == y.hull_vol and x.mesh_vol == y.mesh_vol and x.hausdorff == y.hausdorff
+        for x, y in zip(a.parts, b.parts))
+
+
+def batched_vs_plain(cw, n, cpu_meshes):
     """The first n meshes of the 256-mesh mix (log-uniform 500-60000 triangles: blobs, staircases,
     twisted tori), decomposed together (one tree run per iteration over every mesh's seeds, shared
-    hull and Hausdorff batches) and one by one; both wall times, and whether the results agree."""
+    hull and Hausdorff batches) and one by one; both wall times, and whether the results agree.
+    Then the CPU oracle decomposes the first `cpu_meshes` of them in full (merge included) on the
+    host's threads, as the CPU rate on the same mix, and its results are checked against the GPU's."""
     meshes = [cw.generate("mix", MIX["seed"], [k]) for k in range(n)]
     opts = dict(MIX["options"], hull_wg_min_verts=tuned_split("mix"))
     cw.decompose_batch(meshes[:4], **opts)  # warm
@@ -171,15 +184,29 @@ def batched_vs_plain(cw, n):
             print(f"plain: {k + 1} meshes, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     tp = time.perf_counter() - t0
     print(f"plain: {n} meshes in {tp:.2f} s", file=sys.stderr, flush=True)
-    same = all(a.cuts == b.cuts and len(a.parts) == len(b.parts) and
-               all(x.hull_vol == y.hull_vol and x.mesh_vol == y.mesh_vol and x.hausdorff == y.hausdorff
-                   for x, y in zip(a.parts, b.parts)) for a, b in zip(rb, rp))
-    return dict(workload=f"first {n} meshes of the 256-mesh mix (seed {MIX['seed']}), "
-                         f"{sum(m.nt for m in meshes)} tris", options=opts,
-                batched_meshes_per_s=n / tb, plain_meshes_per_s=n / tp, ratio=tp / tb,
-                candidate_hulls=sum(r.stats["candidate_hulls"] for r in rp),
-                batched_candidate_hulls_per_s=sum(r.stats["candidate_hulls"] for r in rp) / tb,
-                identical=same)
+    same = all(_same_result(a, b) for a, b in zip(rb, rp))
+    out = dict(workload=f"first {n} meshes of the 256-mesh mix (seed {MIX['seed']}), "
+                        f"{sum(m.nt for m in meshes)} tris", options=opts,
+               batched_meshes_per_s=n / tb, plain_meshes_per_s=n / tp, ratio=tp / tb,
+               candidate_hulls=sum(r.stats["candidate_hulls"] for r in rp),
+               batched_candidate_hulls_per_s=sum(r.stats["candidate_hulls"] for r in rp) / tb,
+               identical=same)
+    k = min(cpu_meshes, n)
+    if k:
+        oracle = _oracle(cw)
+        t0 = time.perf_counter()
+        rc = []
+        for m in meshes[:k]:
+            rc.append(cw.decompose(m, api=oracle, **MIX["options"]))
+            print(f"cpu: {len(rc)} meshes, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        tc = time.perf_counter() - t0
+        cores = oracle_threads()
+        out["cpu"] = dict(value=k / tc, unit="meshes/s", cores=cores, kind="port",
+                          sample=f"the first {k} meshes of the mix ({sum(m.nt for m in meshes[:k])} tris), whole "
+                                 f"decompose with merge, CPU oracle on {cores} host threads: {tc:.2f} s",
+                          identical_to_gpu=all(_same_result(a, b) for a, b in zip(rc, rp)))
+        out["batched_vs_cpu"] = (n / tb) / (k / tc)
+    return out
 
 
 def main():
@@ -191,6 +218,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched", type=int, default=256, metavar="N",
                     help="also time the first N meshes of the 256-mesh mix batched and plain (0: skip)")
+    ap.add_argument("--cpu-mix", type=int, default=4, metavar="K",
+                    help="also time the CPU oracle on the first K meshes of the mix (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (HBM counter calibration)")
     a = ap.parse_args()
@@ -238,7 +267,7 @@ def main():
         "cpu_baseline": None,
     }
     if a.batched:
-        line["batched_vs_plain"] = batched_vs_plain(cw, a.batched)
+        line["batched_vs_plain"] = batched_vs_plain(cw, a.batched, 0 if a.no_cpu else a.cpu_mix)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cw, mesh, step_s, a.cpu_sample_iters)
     print(json.dumps(line))
