@@ -50,7 +50,25 @@ def build(jobs=8, verbose=False):
     tmp = lib + ".tmp"
     _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-Wl,-Bsymbolic", "-o", tmp, *objs])
     os.replace(tmp, lib)
+    build_node()
     return lib
+
+
+NODE_INCLUDE = "/usr/include/node"  # Node's own headers (node_api.h)
+
+
+def build_node():
+    """The Node-API addon (node/cuacd_napi.c) into node/build/, when Node's headers are installed;
+    it opens libcuacd.so at run time and links nothing else. Returns its path or None."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    out_dir = os.path.join(HERE, "node", "build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "cuacd_napi.node")
+    _run(["gcc", "-O2", "-std=c11", "-Wall", "-shared", "-fPIC", "-DNODE_GYP_MODULE_NAME=cuacd_napi",
+          "-I" + NODE_INCLUDE, os.path.join(HERE, "node", "cuacd_napi.c"), "-o", out + ".tmp", "-ldl"])
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":
