@@ -71,7 +71,7 @@ CUACD_EXPORT int cuacd_concave_edge_planes(const float* verts, uint32_t nv, cons
     o.seed = seed;
     const std::vector<float> p = cuacd::concaveEdgePlanes(meshOf(verts, nv, tris, nt), o);
     *n = (uint32_t)p.size();
-    std::memcpy(out, p.data(), std::min<size_t>(p.size(), cap) * 4);
+    if (out && !p.empty()) std::memcpy(out, p.data(), std::min<size_t>(p.size(), cap) * 4);
     return CUACD_OK;
   } catch (const std::exception&) {
     return CUACD_ERR_ARG;
