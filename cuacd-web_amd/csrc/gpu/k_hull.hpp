@@ -1318,9 +1318,9 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
     g.sync();
     // seeds: maximum and minimum along each direction (lowest index on ties)
     if constexpr (Grp::NW > 1) {
-      // a workgroup: each wave owns 40 / NW directions and scans every point once
-      static_assert(40 % Grp::NW == 0, "directions per wave");
-      constexpr int DPW = 40 / (int)Grp::NW;
+      // a workgroup: each wave owns ceil(40 / NW) directions (the last waves fewer
+      // when NW does not divide 40) and scans every point once
+      constexpr int DPW = (40 + (int)Grp::NW - 1) / (int)Grp::NW;
       const uint32_t lane = lane_id(), d0 = (t >> 6) * DPW;
       float mx[DPW], mn[DPW];
       uint32_t ix[DPW], in[DPW];
@@ -1336,9 +1336,10 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
         const float x = (float)q[0], y = (float)q[1], z = (float)q[2];
 #pragma unroll
         for (int k = 0; k < DPW; k++) {
-          float d = KDIR[d0 + k][0] * x;
-          d = d + KDIR[d0 + k][1] * y;
-          d = d + KDIR[d0 + k][2] * z;
+          const uint32_t dk = d0 + k < 40 ? d0 + k : 39;  // past 39: computed, never stored
+          float d = KDIR[dk][0] * x;
+          d = d + KDIR[dk][1] * y;
+          d = d + KDIR[dk][2] * z;
           if (d > mx[k]) {
             mx[k] = d;
             ix[k] = i;
@@ -1357,6 +1358,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
       if (lane == 0)
 #pragma unroll
         for (int k = 0; k < DPW; k++) {
+          if (d0 + k >= 40) break;
           seed[2 * (d0 + k)] = ix[k];
           seed[2 * (d0 + k) + 1] = in[k];
         }
