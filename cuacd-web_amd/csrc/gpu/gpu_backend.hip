@@ -251,10 +251,10 @@ void GpuBackend::ensurePools() {
   tris_.ensure(std::min<uint64_t>(cfg_.triCap, 0xffffffffull));
   edges_.ensure(std::min<uint64_t>(cfg_.edgeCap, 0xffffffffull));
   scratch_.ensure(std::min<uint64_t>(cfg_.scratchWords, 0xffffffffull));
-  // tetrahedra of one level: its child triangles and cap edges. The last level
-  // keeps child vertices only, so its triangles are bounded by the vertex pool,
-  // not the triangle pool. 48 bytes each; the 1.5G clamp is 72 GB of HBM.
-  tetCap_ = std::min<uint64_t>(cfg_.triCap / 3 + cfg_.edgeCap + cfg_.vertexCap / 3, 1536ull << 20);
+  // tetrahedra of one level: at most the child triangles and cap edges it writes
+  // (48 bytes each; the 1.5G clamp is 72 GB of HBM, which batches of hundreds of
+  // meshes reach: their triangle pool is scaled for it)
+  tetCap_ = std::min<uint64_t>(cfg_.triCap / 3 + cfg_.edgeCap, 1536ull << 20);
   tets_.ensure(3 * tetCap_);
 }
 

@@ -50,7 +50,9 @@ TreeConfig treeConfigForBatch(const DecomposeOptions& o, uint64_t totalTris) {
   while (scale < 4 && totalTris > scale * 400000ull) scale *= 2;
   // the last level keeps only child vertices (every child's), so that pool binds first
   if (!o.vertexCap) c.vertexCap = std::min<uint64_t>(c.vertexCap * 2 * scale, 0xffffffffull);
-  if (!o.triCap) c.triCap *= scale;
+  // the last level writes a volume tetrahedron per child triangle without keeping
+  // the triangles, so the tetrahedron pool (sized from this one) needs the headroom
+  if (!o.triCap) c.triCap = std::min<uint64_t>(c.triCap * 2 * scale, 0xffffffffull);
   if (!o.edgeCap) c.edgeCap *= scale;
   if (!o.scratchWords) c.scratchWords *= scale;
   if (!o.hullBudgetBytes) c.hullBudgetBytes *= scale;
