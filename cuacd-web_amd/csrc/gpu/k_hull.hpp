@@ -1305,15 +1305,25 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
     float scale = ex > ey ? ex : ey;
     scale = scale > ez ? scale : ez;
     scale = scale > 1e-30f ? scale : 1e-30f;
-    const float qs = (float)(1 << GB) / scale;
+    // Exported hulls quantize each axis over its own extent (at most 1024x finer
+    // than the largest): the hull commutes with per-axis scaling, and a thin part's
+    // vertex choice gets the thin axis's resolution. Tree candidates keep one scale,
+    // whose cell volume their grid volume uses.
+    const float ext[3] = {ex, ey, ez};
+    float qs[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float e = EXACT ? (ext[k] < scale * 0x1p-10f ? scale * 0x1p-10f : ext[k]) : scale;
+      qs[k] = (float)(1 << GB) / e;
+    }
     {
       const double cell = (double)scale * (EXACT ? 0x1p-24 : 0x1p-21);
-      C.cell3 = cell * cell * cell;
+      C.cell3 = cell * cell * cell;  // the grid volume: tree candidates only
     }
     for (uint32_t i = t; i < C.nv; i += Grp::G)
       for (int k = 0; k < 3; k++) {
         const float d = C.V[3 * i + k] - lo[k];
-        C.Q[3 * i + k] = (int32_t)rintf(d * qs);
+        C.Q[3 * i + k] = (int32_t)rintf(d * qs[k]);
       }
     g.sync();
     // seeds: maximum and minimum along each direction (lowest index on ties)

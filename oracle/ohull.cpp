@@ -333,7 +333,7 @@ struct Ctx {
 
 }  // namespace
 
-Grid quantize(const float* v, uint32_t nv, int bits) {
+Grid quantize(const float* v, uint32_t nv, int bits, bool perAxis) {
   Grid g;
   g.ok = nv > 0;
   g.q.assign(3 * (size_t)nv, 0);
@@ -346,12 +346,18 @@ Grid quantize(const float* v, uint32_t nv, int bits) {
     }
   const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
   const float scale = std::max(std::max(std::max(ex, ey), ez), 1e-30f);
-  const float qs = std::ldexp(1.0f, bits) / scale;
+  const float ext[3] = {ex, ey, ez};
+  float qs[3];
+  for (int k = 0; k < 3; k++) {
+    // per axis: each axis over its own extent, at most 1024x finer than the largest
+    const float e = perAxis ? std::max(ext[k], scale * 0x1p-10f) : scale;
+    qs[k] = std::ldexp(1.0f, bits) / e;
+  }
   g.scale = scale;
   for (uint32_t i = 0; i < nv; i++)
     for (int k = 0; k < 3; k++) {
       const float d = v[3 * i + k] - lo[k];
-      const float t = d * qs;
+      const float t = d * qs[k];
       g.q[3 * i + k] = (int32_t)std::nearbyint(t);
     }
   return g;
@@ -366,7 +372,7 @@ HullOut exactHull(const float* v, uint32_t nv, const HullParams& hp) {
   Ctx C;
   C.v = v;
   C.nv = nv;
-  Grid G = quantize(v, nv, hp.gridBits);
+  Grid G = quantize(v, nv, hp.gridBits, hp.gridBits > 21);
   C.bits = hp.gridBits;
   C.q.swap(G.q);
   C.cell3 = grid_cell3(G.scale, hp.gridBits);

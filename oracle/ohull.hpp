@@ -51,6 +51,6 @@ struct Grid {
   float scale = 0;         // largest bbox extent: the grid has 2^bits cells across it
   bool ok;
 };
-Grid quantize(const float* v, uint32_t nv, int bits = 21);
+Grid quantize(const float* v, uint32_t nv, int bits = 21, bool perAxis = false);
 
 }  // namespace orc

@@ -4,16 +4,16 @@ near-convex parts. For every part of the GPU decomposition:
 
 - mesh volume: the part's signed volume in FP64 over its float32 vertices;
 - hull volume: scipy qhull in FP64 over the part's original float32 vertices (the GPU's exported hull
-  volume is exact over its faces in the original coordinates; a 2^24 grid decides which points are
-  vertices, and a point within a cell of the surface can be left out);
+  volume is exact over its faces in the original coordinates; a 2^24 grid over each axis's own extent
+  decides which points are vertices, and a point within a cell of the surface can be left out);
 - rv: 0.3 cbrt(3/(4 pi) max(hull - mesh, 0)) from those two;
 - Hausdorff: the same sample points (the Wang-hash samples, float32), each point's distance to the
   nearest triangle of the other mesh in FP64, both directions (the GPU measures in float32).
 
 The relative errors are printed per quantity. The mesh volume is FP64 on both sides; the hull
-volume's error comes from the vertex choice on the grid (cell = largest extent / 2^24), largest on thin
-parts; rv inherits it, amplified where hull - mesh cancels; the distance error is float32 rounding. The workload is a
-smaller grid (96^3, about 60k triangles) and 500 samples per part than the full form (192^3, 10k
+volume's error comes from the vertex choice on the grid (cell = the axis's extent / 2^24); rv
+inherits it, amplified where hull - mesh cancels; the distance error is float32 rounding. The
+workload is a smaller grid (96^3, about 60k triangles) and 500 samples per part than the full form (192^3, 10k
 samples), so the FP64 brute force stays within seconds."""
 import math
 
@@ -108,7 +108,7 @@ def test_precision_against_fp64(cw):
         if v:
             print(f"  {k:12s} parts {len(v):3d}  max rel err {max(v):.3e}  median {float(np.median(v)):.3e}", flush=True)
     assert max(err["mesh volume"]) < 1e-12
-    # the exported hull's volume is exact over its faces; its vertices are chosen on the 2^24 grid
+    # the exported hull's volume is exact over its faces; its vertices are chosen on a 2^24 grid per axis
     assert max(err["hull volume"]) < 1e-6
-    assert max(err["rv"]) < 1e-5  # the hull's error over hull - mesh, which cancels on near-convex parts
+    assert max(err["rv"]) < 1e-6  # the hull's error over hull - mesh, which cancels on near-convex parts
     assert err["Hausdorff"] and max(err["Hausdorff"]) < 1e-6  # float32 distances, FP64 on slivers

@@ -140,16 +140,18 @@ def test_hull_known_answers(cw, oracle):
     assert cw.is_closed(cw.Mesh(park_miller_cloud(), tris))
 
 
-def _quantize(v, with_cell=False, bits=21):
+def _quantize(v, with_cell=False, bits=21, per_axis=False):
     """The hull's 2^bits grid over the bbox (f32 arithmetic), and the cell edge scale / 2^bits: 21 bits
-    for tree candidates, 24 for exported hulls."""
+    and one scale for tree candidates; 24 bits for exported hulls, each axis over its own extent (at
+    most 1024x finer than the largest)."""
     v = v.astype(np.float32)
     lo, hi = v.min(0), v.max(0)
     ext = (hi - lo).astype(np.float32)
-    scale = max(float(ext.max()), 1e-30)
-    qs = np.float32(2.0 ** bits) / np.float32(scale)
+    scale = np.float32(max(float(ext.max()), 1e-30))
+    e = np.maximum(ext, scale * np.float32(2.0 ** -10)) if per_axis else np.full(3, scale, np.float32)
+    qs = (np.float32(2.0 ** bits) / e.astype(np.float32)).astype(np.float32)
     q = np.rint((v - lo).astype(np.float32) * qs).astype(np.int64)
-    return (q, scale / 2.0 ** bits) if with_cell else q
+    return (q, float(scale) / 2.0 ** bits) if with_cell else q
 
 
 def _grid_hull_volume(pts, bits=24):
@@ -200,7 +202,7 @@ def test_hull_vertices_match_brute_force(cw, oracle, seed, n):
     pts = rng.standard_normal((n, 3)).astype(np.float32)
     tris, vol = _hull(cw, oracle, pts)
     used = set(int(x) for x in tris.reshape(-1))
-    q = _quantize(pts, bits=24)  # the exported hull's grid
+    q = _quantize(pts, bits=24, per_axis=True)  # the exported hull's grid
     assert _exact_hull_vertices(q) == used  # general position: hull vertices are exactly the extreme points
     assert used <= set(range(n))
     # the exported volume is that of the face surface over the original coordinates: with the same
