@@ -158,6 +158,7 @@ class GpuBackend : public Backend {
   void launched(const char* what) { hipCheck(hipGetLastError(), what); }
   std::vector<Level> layout(uint32_t nSeeds, uint32_t depth, uint32_t w0, uint32_t ne0) const;
   dk::LevelParams levelParams(const Level& L, uint32_t level, uint32_t depth, uint32_t nSeeds) const;
+  bool growPools(uint32_t overflowBits) override;
   void ensurePools();
   // hull the given jobs (slab offsets are assigned here, in arena-sized chunks)
   void runHulls(std::vector<dk::HullJob>& hj, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
@@ -244,6 +245,23 @@ dk::LevelParams GpuBackend::levelParams(const Level& L, uint32_t level, uint32_t
   P.eCap = std::min<uint64_t>(cfg_.edgeCap, 0xffffffffull);
   P.sCap = std::min<uint64_t>(cfg_.scratchWords, 0xffffffffull);
   return P;
+}
+
+bool GpuBackend::growPools(uint32_t bits) {
+  constexpr uint64_t LIM = 0xffffffffull;  // element counts stay 32-bit indexable
+  bool grew = false;
+  auto dbl = [&](uint64_t& cap) {
+    if (cap >= LIM) return false;
+    cap = std::min<uint64_t>(cap * 2, LIM);
+    grew = true;
+    return true;
+  };
+  if (bits & dk::OVF_SCRATCH) dbl(cfg_.scratchWords);
+  if (bits & dk::OVF_VERTS) dbl(cfg_.vertexCap);
+  if (bits & dk::OVF_TRIS) dbl(cfg_.triCap);
+  if (bits & dk::OVF_EDGES) dbl(cfg_.edgeCap);
+  if ((bits & dk::OVF_TETS) && !(bits & dk::OVF_TRIS) && !dbl(cfg_.triCap)) dbl(cfg_.edgeCap);
+  return grew;  // the next run's ensurePools() allocates the larger pools
 }
 
 void GpuBackend::ensurePools() {

@@ -161,6 +161,14 @@ struct KernelStats {
 class Backend {
  public:
   virtual KernelStats kernelStats() const { return KernelStats(); }
+  // Doubles the tree pools named by overflow bits (1 scratch, 2 vertices, 4 triangles,
+  // 8 edges, 16 tetrahedra, which are sized from the triangle and edge pools) up to
+  // their element limits; false when none of them could grow. A backend without
+  // pools never overflows and grows nothing.
+  virtual bool growPools(uint32_t overflowBits) {
+    (void)overflowBits;
+    return false;
+  }
   virtual ~Backend() {}
   virtual const TreeConfig& config() const = 0;
   virtual uint32_t planeSlots() const = 0;

@@ -669,7 +669,20 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
         std::fprintf(stderr, "[iter %u] tree run: %zu seeds, hulls %llu, pruned %u, overflow pools %u, %.1f ms\n", iter,
                      refs.size(), (unsigned long long)res.hullsDone, res.pruned, res.overflowPools, nowMs() - tt0);
       if (res.overflow) {
-        if (batchN == 1) throw std::runtime_error("GPU pool overflow with a single part; increase tree pool sizes");
+        if (batchN == 1) {
+          // one part alone does not fit: double the pools that ran out and run it again,
+          // when the caller left them at their defaults (explicit sizes are limits)
+          const uint32_t growable = (o.scratchWords ? 0u : 1u) | (o.vertexCap ? 0u : 2u) | (o.triCap ? 0u : 4u | 16u) |
+                                    (o.edgeCap ? 0u : 8u);
+          if ((res.overflowPools & ~growable) == 0 && poolGrowths_ < 4 && be_.growPools(res.overflowPools)) {
+            poolGrowths_++;
+            if (o.verbose)
+              std::fprintf(stderr, "[iter %u] pools %u grown for a single part (%u)\n", iter, res.overflowPools,
+                           poolGrowths_);
+            continue;
+          }
+          throw std::runtime_error("GPU pool overflow with a single part; increase tree pool sizes");
+        }
         batchN = std::max<size_t>(1, batchN >> 1);
         treeBatch_ = batchN;
         continue;

@@ -152,6 +152,7 @@ class Decomposer {
   Backend& be_;
   DecomposeOptions o_;
   size_t treeBatch_ = 0;  // largest seed count that last fit the tree pools (0: unknown)
+  uint32_t poolGrowths_ = 0;  // pool doublings after a single part overflowed (at most 4)
 };
 
 // number of host worker threads for per-part host work (cuts, concave planes)

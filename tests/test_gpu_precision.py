@@ -112,3 +112,37 @@ def test_precision_against_fp64(cw):
     assert max(err["hull volume"]) < 1e-6
     assert max(err["rv"]) < 1e-6  # the hull's error over hull - mesh, which cancels on near-convex parts
     assert err["Hausdorff"] and max(err["Hausdorff"]) < 1e-6  # float32 distances, FP64 on slivers
+
+
+def test_precision_full_form_volumes(cw):
+    """The full precision form (192^3 grid, about 230k triangles, four thin tori): mesh volume, hull
+    volume and rv of every part against FP64, to the same bounds. The Hausdorff distance is measured on
+    the reduced form above, whose FP64 brute force stays within seconds."""
+    from scipy.spatial import ConvexHull
+    params = list(WORKLOAD["params"])
+    params[1] = 192
+    mesh = cw.generate("blobs", WORKLOAD["seed"], params)
+    # One 230k-triangle part overflows the default tree pools at depth 3 (scratch, vertices and
+    # tetrahedra): the decomposer doubles those pools and runs the part again. The result equals a run
+    # with explicit 4x pools (about 75 GB of HBM).
+    r = cw.decompose(mesh, **OPTS)
+    pools = dict(vertex_cap=2 << 30, tri_cap=2 << 30, edge_cap=64 << 20, scratch_words=1 << 30,
+                 hull_budget_bytes=16 << 30)
+    big = cw.decompose(mesh, **OPTS, **pools)
+    assert r.cuts == big.cuts and len(r.parts) == len(big.parts)
+    assert all(a.mesh_vol == b.mesh_vol and a.hull_vol == b.hull_vol and a.hausdorff == b.hausdorff
+               for a, b in zip(r.parts, big.parts))
+    print(f"\nprecision form (full): {mesh.nt} triangles, {len(r.parts)} parts", flush=True)
+    err = {"mesh volume": [], "hull volume": [], "rv": []}
+    for p in r.parts:
+        m64 = abs(_signed_volume64(p.mesh.verts, p.mesh.tris.astype(np.int64)))
+        h64 = ConvexHull(p.mesh.verts.astype(np.float64)).volume
+        err["mesh volume"].append(abs(p.mesh_vol - m64) / m64)
+        err["hull volume"].append(abs(p.hull_vol - h64) / h64)
+        rv64 = _rv(m64, h64)
+        err["rv"].append(abs(_rv(p.mesh_vol, p.hull_vol) - rv64) / max(rv64, 1e-12))
+    for k, v in err.items():
+        print(f"  {k:12s} parts {len(v):3d}  max rel err {max(v):.3e}  median {float(np.median(v)):.3e}", flush=True)
+    assert max(err["mesh volume"]) < 1e-12
+    assert max(err["hull volume"]) < 1e-6
+    assert max(err["rv"]) < 1e-6
