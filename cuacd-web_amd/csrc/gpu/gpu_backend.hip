@@ -89,6 +89,7 @@ class GpuBackend : public Backend {
   std::vector<HullResult> hullMeshes(const std::vector<MeshRef>& meshes) override;
   std::vector<double> hausdorff(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b) override;
   std::vector<HullResult> hullsWithHausdorff(const std::vector<MeshRef>& meshes, std::vector<double>* hd) override;
+  std::vector<float> sampleMesh(const MeshRef& m) override;
   std::vector<ItemView> levelItems(uint32_t level) override;
   bool readPart(uint32_t partIdx, PartView* pv, Mesh* mesh) override;
 
@@ -770,6 +771,35 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
     std::memcpy(&f, &bits[p], 4);
     out[p] = std::sqrt((double)f);
   }
+  return out;
+}
+
+// the sample set k_hd_samples writes for one mesh, read back (a debugging and test view of the
+// Hausdorff kernels' input)
+std::vector<float> GpuBackend::sampleMesh(const MeshRef& m) {
+  if (m.nt == 0) return {};
+  const std::vector<uint32_t> counts = sampleOffsets(m, cfg_.hausdorffSamples);
+  const uint64_t ns = counts.back();
+  if (ns > 0x7fffffffull) throw std::length_error("hausdorff sample scratch exceeds capacity");
+  dk::HdMesh M;
+  std::memset(&M, 0, sizeof(M));
+  M.nt = m.nt;
+  hdVerts_.ensure(3ull * m.nv);
+  hdTris_.ensure(3ull * m.nt);
+  hdCounts_.ensure(counts.size());
+  hdMeshes_.ensure(1);
+  hdSamples_.ensure(std::max<uint64_t>(3 * ns, 1));
+  h2d(hdVerts_.p, m.verts, 12ull * m.nv, "upload hd verts");
+  h2d(hdTris_.p, m.tris, 12ull * m.nt, "upload hd tris");
+  h2d(hdCounts_.p, counts.data(), counts.size() * 4, "upload counts");
+  h2d(hdMeshes_.p, &M, sizeof(M), "upload hd meshes");
+  dk::k_hd_samples<<<(m.nt + 255) / 256, 256, 0, stream_>>>(hdMeshes_.p, 1, m.nt, hdVerts_.p, hdTris_.p, hdCounts_.p,
+                                                             hdSamples_.p);
+  launched("k_hd_samples");
+  std::vector<float> out(3 * ns);
+  if (ns)
+    hipCheck(hipMemcpyAsync(out.data(), hdSamples_.p, 12 * ns, hipMemcpyDeviceToHost, stream_), "read samples");
+  sync();
   return out;
 }
 

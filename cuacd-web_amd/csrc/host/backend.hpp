@@ -181,6 +181,9 @@ class Backend {
   virtual std::vector<double> hausdorff(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b) = 0;
   // src/gpu/tree.ts:671 hullsWithHausdorff: hull of each mesh and Hausdorff(mesh, hull)
   virtual std::vector<HullResult> hullsWithHausdorff(const std::vector<MeshRef>& meshes, std::vector<double>* hd) = 0;
+  // the Hausdorff sample set of one mesh (xyz interleaved), as the distance kernels see it
+  // (src/cpu/hausdorff.ts:27-67 sampleMesh, src/gpu/hausdorff_shader.ts:58-86)
+  virtual std::vector<float> sampleMesh(const MeshRef& m) = 0;
   // last run's items of one level (0 = seeds) and part descriptors
   virtual std::vector<ItemView> levelItems(uint32_t level) = 0;
   virtual bool readPart(uint32_t partIdx, PartView* pv, Mesh* mesh) = 0;

@@ -21,10 +21,17 @@ namespace cuacd {
 constexpr double K_RV = 0.3;                               // src/part.ts:7
 constexpr double THREE_OVER_4PI = 0.23873241463784300365;  // 3 / (4 pi)
 
-// Cube root of x >= 0 (0 for x <= 0 or tiny x): start at 2^floor(e/3) <= cbrt(x)
+// Cube root of x >= 0 (0 for x <= 0 or NaN): start at 2^floor(e/3) <= cbrt(x)
 // and run 12 Newton steps; the result is within an ulp or two of cbrt(x).
+// Below 2^-900 (subnormals included) x is scaled by 2^900 and the root by
+// 2^-300, both exact, so every positive double gets its own root.
 CUACD_HD inline double det_cbrt(double x) {
-  if (!(x > 1e-300)) return 0.0;
+  if (!(x > 0.0)) return 0.0;
+  double post = 1.0;
+  if (x < 0x1p-900) {
+    x *= 0x1p900;
+    post = 0x1p-300;
+  }
   uint64_t b;
   __builtin_memcpy(&b, &x, 8);
   const int e = (int)((b >> 52) & 0x7ff) - 1023;
@@ -36,7 +43,7 @@ CUACD_HD inline double det_cbrt(double x) {
     const double y2 = y * y;
     y = y - (y2 * y - x) / (3.0 * y2);
   }
-  return y;
+  return y * post;
 }
 
 // src/part.ts:24-26 rvCost

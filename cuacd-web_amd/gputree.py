@@ -214,6 +214,17 @@ class GpuTree:
                                                       hd.ctypes.data_as(C.POINTER(F.f64))))
         return [None if vol[k] < 0 else (self._faces(k), float(vol[k]), float(hd[k])) for k in range(n)]
 
+    def sample_mesh(self, mesh):
+        """The Hausdorff sample set of one mesh (src/cpu/hausdorff.ts:27-67 sampleMesh), float32 (n, 3),
+        as the distance kernels of this backend generate it."""
+        n = F.u64()
+        vp, tp = mesh.verts.ctypes.data_as(C.POINTER(F.f32)), mesh.tris.ctypes.data_as(C.POINTER(F.u32))
+        check(self.api, self.api.sample_mesh(self._h, vp, mesh.nv, tp, mesh.nt, None, 0, C.byref(n)))
+        out = np.zeros(max(3 * n.value, 1), dtype=np.float32)
+        check(self.api, self.api.sample_mesh(self._h, vp, mesh.nv, tp, mesh.nt, out.ctypes.data_as(C.POINTER(F.f32)),
+                                             n.value, C.byref(n)))
+        return out[: 3 * n.value].reshape(-1, 3)
+
     def hausdorff(self, pairs):
         """hausdorff(pairs) (src/gpu/tree.ts:318): float64 array, one per (a, b) pair."""
         n = len(pairs)

@@ -127,6 +127,13 @@ int cuacd_hulls_with_hausdorff(cuacd_tree* t, const cuacd_mesh_batch* meshes, do
 const uint32_t* cuacd_last_faces(cuacd_tree* t, uint32_t k, uint32_t* nfaces);
 /* hausdorff(pairs) (src/gpu/tree.ts:318): a[k] vs b[k], <= 1024 pairs */
 int cuacd_hausdorff(cuacd_tree* t, const cuacd_mesh_batch* a, const cuacd_mesh_batch* b, double* out);
+/* sampleMesh(mesh) (src/cpu/hausdorff.ts:27-67; the GPU shader's samples, src/gpu/hausdorff_shader.ts:58-86):
+ * the Hausdorff sample set of one mesh as the device generates it, *n samples (xyz) written to out when
+ * n <= cap; res follows the tree's hausdorff_samples setting */
+int cuacd_sample_mesh(cuacd_tree* t, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, float* out,
+                      uint64_t cap, uint64_t* n);
+/* rvCost(meshVol, hullVol) (src/part.ts:24-26), the FP64 cost the tree's record kernel evaluates */
+double cuacd_rv_cost(double mesh_vol, double hull_vol);
 
 /* src/decompose.ts:26-108 DecomposeOptions (GPU backends only) */
 typedef struct cuacd_decompose_options {

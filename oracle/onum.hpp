@@ -160,7 +160,12 @@ inline void anchor_for(const float pls[4][4], uint32_t n, float* o) {
 
 // ---- deterministic cube root and rv cost (src/part.ts:24-26)
 inline double det_cbrt(double x) {
-  if (!(x > 1e-300)) return 0.0;
+  if (!(x > 0.0)) return 0.0;
+  double post = 1.0;
+  if (x < 0x1p-900) {  // tiny and subnormal x: exact scalings by 2^900 and 2^-300
+    x *= 0x1p900;
+    post = 0x1p-300;
+  }
   uint64_t b;
   std::memcpy(&b, &x, 8);
   const int e = (int)((b >> 52) & 0x7ff) - 1023;
@@ -172,7 +177,7 @@ inline double det_cbrt(double x) {
     const double y2 = y * y;
     y = y - (y2 * y - x) / (3.0 * y2);
   }
-  return y;
+  return y * post;
 }
 inline double rv_cost(double meshVol, double hullVol) {
   double gap = hullVol - meshVol;

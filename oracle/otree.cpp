@@ -660,4 +660,8 @@ std::vector<HullResult> OracleBackend::hullsWithHausdorff(const std::vector<Mesh
   return out;
 }
 
+std::vector<float> OracleBackend::sampleMesh(const MeshRef& m) {
+  return m.nt ? orc::sampleMesh(m, cfg_.hausdorffSamples) : std::vector<float>();
+}
+
 }  // namespace orc

@@ -57,6 +57,7 @@ class OracleBackend : public cuacd::Backend {
   std::vector<double> hausdorff(const std::vector<cuacd::MeshRef>& a, const std::vector<cuacd::MeshRef>& b) override;
   std::vector<cuacd::HullResult> hullsWithHausdorff(const std::vector<cuacd::MeshRef>& meshes,
                                                     std::vector<double>* hd) override;
+  std::vector<float> sampleMesh(const cuacd::MeshRef& m) override;
   std::vector<cuacd::ItemView> levelItems(uint32_t level) override;
   bool readPart(uint32_t partIdx, cuacd::PartView* pv, cuacd::Mesh* mesh) override;
 
