@@ -1,28 +1,47 @@
-"""Benchmark: full decompose() of the 12-blob, 96^3 marching-tetrahedra mesh (~56k triangles)
-on one MI355X — the plain per-mesh GPU method.
+"""Benchmark: the batched decomposition of the 256-mesh uneven mix on one MI355X, against the plain
+per-mesh GPU path and the CPU oracle.
 
-One step = one decompose(mesh, threshold 0.05, width 12, width2 6, depth 3, nConcaveEdges 16,
-refineRounds 2 x 8 probes, mergeHulls on, 3000 Hausdorff samples per part). Prints one JSON line.
-Beside it: the batched multi-mesh method against the plain one on the 256-mesh mixed workload
-(`batched_vs_plain`; `--batched N` takes its first N meshes, 0 skips it), and the CPU oracle on a
-bounded sample (`cpu_baseline`) and on the first meshes of the mix (`batched_vs_plain.cpu`,
-`--cpu-mix K`). Each workload runs with the hull kernel split that
-cuacd-web_amd/autotune.py measured for it (cuacd-web_amd/tuned.json), when that file exists.
+One step = decompose_batch of all 256 meshes of the mix (seed 20260400: triangle counts log-uniform
+in [500, 60000], one third sum-of-Gaussian blob surfaces, one third box staircases, one third twisted
+tori; 3.31M triangles in all) with threshold 0.1, width 12, depth 3, merge on, the other options at
+their defaults: one tree search per iteration over every mesh's seeds, shared hull and Hausdorff
+batches. The engine (device pools sized for the batch) is created once, before the warmup, as the
+reference's Decomposer is (src/decompose.ts:135). `value` is meshes/s over the timed steps.
+
+Beside it, in the same JSON line:
+  plain        the same 256 meshes decomposed one at a time on one engine with single-mesh pools (the
+               plain per-mesh GPU path), its meshes/s, the batched/plain ratio, and whether every
+               result (cuts, parts, volumes, distances, hull faces) is identical to the batched one;
+  blob         the 12-blob 96^3 mesh (56,464 triangles, threshold 0.05, width 12, width2 6, depth 3,
+               16 concave-edge planes, refinement 2 x 8, merge, 3000 samples per part) decomposed
+               plain, the single-mesh workload;
+  roofline     the kernel with the most device time over the timed batched steps, priced against its
+               bound from the work the kernels count themselves;
+  cpu_baseline the CPU oracle (the C++ restatement of the path, OpenMP over the host's cores) decomposing
+               a sample of the mix in full, merge included, with every result checked equal to the GPU's;
+               all cores, and one thread in a child process on a smaller sample.
+Each workload runs with the hull kernel split that cuacd-web_amd/autotune.py measured for it
+(cuacd-web_amd/tuned.json), when that file exists; results do not depend on it.
 """
 import argparse
 import ctypes
 import importlib.util
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "cuacd-web_amd")
 
-WORKLOAD = dict(name="blobs12-96^3", seed=20260200, params=[12, 96, -0.6, 0.6, 0.12, 0.30, 0.5])
-OPTIONS = dict(threshold=0.05, width=12, width2=6, depth=3, n_concave_edges=16, refine_rounds=2, refine_probes=8,
-               merge_hulls=True, hausdorff_samples=3000)
+MIX = dict(seed=20260400, n=256, options=dict(threshold=0.1, width=12, depth=3, merge_hulls=True))
+BLOB = dict(name="blobs12-96^3", seed=20260200, params=[12, 96, -0.6, 0.6, 0.12, 0.30, 0.5],
+            options=dict(threshold=0.05, width=12, width2=6, depth=3, n_concave_edges=16, refine_rounds=2,
+                         refine_probes=8, merge_hulls=True, hausdorff_samples=3000))
+CPU_ALLCORE = [0, 1, 2, 3]  # mix meshes for the all-core CPU figure (44,028 triangles)
+CPU_ONE = [1, 14]           # mix meshes for the one-thread figure and its all-core twin (1,940 triangles)
 
 
 def _package():
@@ -36,22 +55,26 @@ def _package():
     return mod
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 # Algorithmic work the kernels count themselves: (counter, amount per unit, unit kind).
 #  hull / hull_big: HBM bytes a part's hull needs at least, 12 nv (points read once) + 16 (result)
 #    + 12 per exported face, summed by the kernel;
 #  emit: the job's parent read plus the children it writes, summed in bytes by the kernel;
-#  hd_dist: one point-triangle distance evaluation = 70 FP32 operations.
+#  hd_dist: one point-triangle distance evaluation = 70 FP32 operations (VALU, no matrix cores).
 KERNEL_WORK = {
     "hull_big": ("hull_bytes_big", 1.0, "bytes"),
     "hull": ("hull_bytes", 1.0, "bytes"),
     "emit": ("emit_bytes", 1.0, "bytes"),
     "hd_dist": ("hd_evals", 70.0, "flops"),
 }
-# point-face evaluations per hull kernel, reported as effort beside the roofline
+# point-face evaluations per hull kernel: 8 FP32 operations each (the plane test), the kernel's VALU effort
 HULL_TESTS = {"hull_big": "face_tests_big", "hull": "face_tests"}
 COUNTERS = ("face_tests", "face_tests_big", "emit_bytes", "hd_evals", "hull_jobs", "hull_bytes", "hull_bytes_big")
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
-FP32_PEAK_TFLOPS = 157.3    # MI355X dense FP32
+FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 TUNED_FILE = os.path.join(PKG_DIR, "tuned.json")  # written by cuacd-web_amd/autotune.py
 
@@ -65,13 +88,11 @@ def tuned_split(workload):
         return int(json.load(f).get(workload, {}).get("hull_wg_min_verts", 0))
 
 
-def sum_kernels(results):
+def sum_kernels(kernel_dicts):
     tot = None
-    for r in results:
-        k = r.kernels
+    for k in kernel_dicts:
         if tot is None:
-            tot = dict(ms=dict(k["ms"]), launches=dict(k["launches"]),
-                       **{n: k[n] for n in COUNTERS})
+            tot = dict(ms=dict(k["ms"]), launches=dict(k["launches"]), **{n: k[n] for n in COUNTERS})
             continue
         for n in k["ms"]:
             tot["ms"][n] += k["ms"][n]
@@ -81,9 +102,9 @@ def sum_kernels(results):
     return tot
 
 
-def roofline(kst):
+def roofline(kst, workload):
     """The kernel with the most device time over the timed steps, priced against its bound.
-    Times come from HIP events the library records on its own stream around each launch."""
+    Times come from HIP events the library records around each launch on that launch's stream."""
     name = max(KERNEL_WORK, key=lambda n: kst["ms"][n])
     counter, per_unit, kind = KERNEL_WORK[name]
     launches = max(kst["launches"][name], 1)
@@ -94,17 +115,19 @@ def roofline(kst):
     if kind == "bytes":
         achieved, peak, unit, bound = work_per_launch / (avg_ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s", "hbm"
     else:
-        achieved, peak, unit, bound = work_per_launch / (avg_ms * 1e-3) / 1e12, FP32_PEAK_TFLOPS, "TFLOP/s", "mfma"
+        achieved, peak, unit, bound = work_per_launch / (avg_ms * 1e-3) / 1e12, FP32_PEAK_TFLOPS, "TFLOP/s", "valu"
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
-        if t.get("kernel") == name:
+        if t.get("kernel") == name and t.get("workload") == workload:
             traffic = t["hbm_bytes_per_launch"]
     effort = None
     if name in HULL_TESTS:
-        effort = dict(point_face_tests_per_s=kst[HULL_TESTS[name]] / (kst["ms"][name] * 1e-3),
-                      tests_per_launch=kst[HULL_TESTS[name]] / launches)
+        tests = kst[HULL_TESTS[name]]
+        flops = 8.0 * tests / (kst["ms"][name] * 1e-3)
+        effort = dict(point_face_tests_per_s=tests / (kst["ms"][name] * 1e-3), tests_per_launch=tests / launches,
+                      valu_tflops=flops / 1e12, valu_frac=flops / 1e12 / FP32_PEAK_TFLOPS)
     return dict(bound=bound, achieved=achieved, peak=peak, unit=unit, frac=achieved / peak, traffic=traffic,
                 kernel=name, launches=kst["launches"][name], avg_launch_ms=avg_ms,
                 work_per_launch=work_per_launch, work_counter=counter, work_per_unit=per_unit, effort=effort,
@@ -121,7 +144,33 @@ def copy_probe(mib):
     dst.copy_(src)
     torch.cuda.synchronize()
     assert bool((dst[-4:] == src[-4:]).all())
-    print(f"copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+    log(f"copy probe: {mib} MiB read + {mib} MiB written")
+
+
+def _same(a, b):
+    """Equal decompositions: cuts, parts, volumes, distances, seams, part and hull geometry."""
+    import numpy as np
+    if a.cuts != b.cuts or len(a.parts) != len(b.parts) or a.merge != b.merge:
+        return False
+    for x, y in zip(a.parts, b.parts):
+        if (x.mesh_vol, x.hull_vol, x.hausdorff, x.merged, x.seams) != (y.mesh_vol, y.hull_vol, y.hausdorff, y.merged,
+                                                                        y.seams):
+            return False
+        if not (np.array_equal(x.mesh.verts, y.mesh.verts) and np.array_equal(x.mesh.tris, y.mesh.tris)
+                and np.array_equal(x.hull.tris, y.hull.tris)):
+            return False
+    return True
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
 
 
 def oracle_threads():
@@ -129,84 +178,51 @@ def oracle_threads():
     return int(v) if v and v.isdigit() else len(os.sched_getaffinity(0))
 
 
-def cpu_baseline(cw, mesh, gpu_step_s, sample_iters):
-    """The oracle (the CPU restatement, its hull and Hausdorff loops over OpenMP host threads) on the
-    first `sample_iters` decomposition iterations; meshes/s scaled by the GPU's own full-run /
-    sample-run time ratio."""
-    oracle = _oracle(cw)
-    opts = dict(OPTIONS, max_iters=sample_iters, merge_hulls=False)
-    t0 = time.perf_counter()
-    cw.decompose(mesh, **opts)
-    t_gpu_sample = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    cw.decompose(mesh, api=oracle, **opts)
-    t_cpu_sample = time.perf_counter() - t0
-    est_cpu_full = t_cpu_sample * (gpu_step_s / max(t_gpu_sample, 1e-9))
-    cores = oracle_threads()
-    return dict(value=1.0 / est_cpu_full, unit="meshes/s", cores=cores, kind="port",
-                sample=f"first {sample_iters} decompose iterations (no merge) of the same mesh on {cores} host threads: "
-                       f"{t_cpu_sample:.2f} s CPU vs {t_gpu_sample:.3f} s GPU; full-run estimate scales by the GPU "
-                       f"full/sample ratio", cpu_sample_s=t_cpu_sample, gpu_sample_s=t_gpu_sample)
-
-
-MIX = dict(seed=20260400, options=dict(threshold=0.1, width=12, depth=3, merge_hulls=True))
-
-
-def _oracle(cw):
+def _oracle():
     from cuacd_web_amd._ffi import Api
     return Api(ctypes.CDLL(os.path.join(ROOT, "oracle", "build", "liboracle.so"), mode=ctypes.RTLD_LOCAL), "oracle_")
 
 
-def _same_result(a, b):
-    return a.cuts == b.cuts and len(a.parts) == len(b.parts) and all(
-        x.hull_vol == y.hull_vol and x.mesh_vol == y.mesh_vol and x.hausdorff == y.hausdorff
-        for x, y in zip(a.parts, b.parts))
+def cpu_run(cw, idx):
+    """The CPU oracle decomposing mix meshes `idx` in full; (seconds, results)."""
+    oracle = _oracle()
+    meshes = [cw.generate("mix", MIX["seed"], [k]) for k in idx]
+    t0 = time.perf_counter()
+    res = [cw.decompose(m, api=oracle, **MIX["options"]) for m in meshes]
+    return time.perf_counter() - t0, res, sum(m.nt for m in meshes)
 
 
-def batched_vs_plain(cw, n, cpu_meshes):
-    """The first n meshes of the 256-mesh mix (log-uniform 500-60000 triangles: blobs, staircases,
-    twisted tori), decomposed together (one tree run per iteration over every mesh's seeds, shared
-    hull and Hausdorff batches) and one by one; both wall times, and whether the results agree.
-    Then the CPU oracle decomposes the first `cpu_meshes` of them in full (merge included) on the
-    host's threads, as the CPU rate on the same mix, and its results are checked against the GPU's."""
-    meshes = [cw.generate("mix", MIX["seed"], [k]) for k in range(n)]
-    opts = dict(MIX["options"], hull_wg_min_verts=tuned_split("mix"))
-    cw.decompose_batch(meshes[:4], **opts)  # warm
-    t0 = time.perf_counter()
-    rb = cw.decompose_batch(meshes, **opts)
-    tb = time.perf_counter() - t0
-    print(f"batched: {n} meshes in {tb:.2f} s", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    rp = []
-    for k, m in enumerate(meshes):
-        rp.append(cw.decompose(m, **opts))
-        if k % 32 == 31:
-            print(f"plain: {k + 1} meshes, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    tp = time.perf_counter() - t0
-    print(f"plain: {n} meshes in {tp:.2f} s", file=sys.stderr, flush=True)
-    same = all(_same_result(a, b) for a, b in zip(rb, rp))
-    out = dict(workload=f"first {n} meshes of the 256-mesh mix (seed {MIX['seed']}), "
-                        f"{sum(m.nt for m in meshes)} tris", options=opts,
-               batched_meshes_per_s=n / tb, plain_meshes_per_s=n / tp, ratio=tp / tb,
-               candidate_hulls=sum(r.stats["candidate_hulls"] for r in rp),
-               batched_candidate_hulls_per_s=sum(r.stats["candidate_hulls"] for r in rp) / tb,
-               identical=same)
-    k = min(cpu_meshes, n)
-    if k:
-        oracle = _oracle(cw)
-        t0 = time.perf_counter()
-        rc = []
-        for m in meshes[:k]:
-            rc.append(cw.decompose(m, api=oracle, **MIX["options"]))
-            print(f"cpu: {len(rc)} meshes, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-        tc = time.perf_counter() - t0
-        cores = oracle_threads()
-        out["cpu"] = dict(value=k / tc, unit="meshes/s", cores=cores, kind="port",
-                          sample=f"the first {k} meshes of the mix ({sum(m.nt for m in meshes[:k])} tris), whole "
-                                 f"decompose with merge, CPU oracle on {cores} host threads: {tc:.2f} s",
-                          identical_to_gpu=all(_same_result(a, b) for a, b in zip(rc, rp)))
-        out["batched_vs_cpu"] = (n / tb) / (k / tc)
-    return out
+def cpu_child(idx):
+    """Entry point of the one-thread child process: prints its time as JSON."""
+    cw = _package()
+    t, res, tris = cpu_run(cw, idx)
+    print(json.dumps(dict(seconds=t, tris=tris, parts=[len(r.parts) for r in res])))
+
+
+def cpu_baseline(cw, batched):
+    """The CPU oracle (kind "port": the C++ restatement of the path, its hull and Hausdorff loops over
+    OpenMP host threads) on bounded samples of the same mix, every result checked equal to the GPU's."""
+    cores = oracle_threads()
+    t_all, res_all, tris_all = cpu_run(cw, CPU_ALLCORE)
+    same = all(_same(a, batched[k]) for a, k in zip(res_all, CPU_ALLCORE))
+    log(f"cpu: {len(CPU_ALLCORE)} meshes on {cores} threads in {t_all:.1f} s")
+    t_few, res_few, tris_few = cpu_run(cw, CPU_ONE)
+    same = same and all(_same(a, batched[k]) for a, k in zip(res_few, CPU_ONE))
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", ",".join(map(str, CPU_ONE))],
+                         env=env, capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        raise RuntimeError("one-thread CPU run failed:\n" + out.stderr[-2000:])
+    one = json.loads(out.stdout.strip().splitlines()[-1])
+    log(f"cpu: {len(CPU_ONE)} meshes on 1 thread in {one['seconds']:.1f} s ({cores} threads: {t_few:.1f} s)")
+    return dict(value=len(CPU_ALLCORE) / t_all, unit="meshes/s", cores=cores, kind="port",
+                sample=f"mix meshes {CPU_ALLCORE} ({tris_all} triangles), whole decompose with merge, CPU oracle on "
+                       f"{cores} host threads: {t_all:.2f} s; results identical to the GPU's: {same}",
+                cpu_model=cpu_model(), identical_to_gpu=same,
+                one_thread=dict(value=len(CPU_ONE) / one["seconds"], unit="meshes/s", cores=1,
+                                sample=f"mix meshes {CPU_ONE} ({tris_few} triangles): {one['seconds']:.2f} s"),
+                same_sample_all_cores=dict(value=len(CPU_ONE) / t_few, unit="meshes/s", cores=cores,
+                                           seconds=t_few))
 
 
 def main():
@@ -214,15 +230,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample-iters", type=int, default=1)
+    ap.add_argument("--meshes", type=int, default=MIX["n"], help="first N meshes of the mix (default all 256)")
+    ap.add_argument("--plain", type=int, default=1, metavar="K",
+                    help="time the plain per-mesh path on every K-th mesh of the mix (1: all; 0: skip)")
+    ap.add_argument("--blob-steps", type=int, default=3, help="plain decompositions of the 56k-triangle blob (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batched", type=int, default=256, metavar="N",
-                    help="also time the first N meshes of the 256-mesh mix batched and plain (0: skip)")
-    ap.add_argument("--cpu-mix", type=int, default=4, metavar="K",
-                    help="also time the CPU oracle on the first K meshes of the mix (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (HBM counter calibration)")
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     a = ap.parse_args()
+    if a.cpu_child is not None:
+        cpu_child([int(x) for x in a.cpu_child.split(",")])
+        return
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU path "
               "(multi-GPU sharding is designed, not built); use --gpus 1", file=sys.stderr)
@@ -230,24 +249,31 @@ def main():
     cw = _package()
     if a.copy_probe:
         copy_probe(a.copy_probe)
-    mesh = cw.generate("blobs", WORKLOAD["seed"], WORKLOAD["params"])
-    opts = dict(OPTIONS, hull_wg_min_verts=tuned_split("blob"))
-    for _ in range(a.warmup):
-        cw.decompose(mesh, **opts)
-    times, results = [], []
+    meshes = [cw.generate("mix", MIX["seed"], [k]) for k in range(a.meshes)]
+    tris = sum(m.nt for m in meshes)
+    opts = dict(MIX["options"], hull_wg_min_verts=tuned_split("mix"))
+    eng = cw.Decomposer(batch_tris=tris, **opts)
+    for w in range(a.warmup):
+        t0 = time.perf_counter()
+        eng.decompose_batch(meshes)
+        log(f"warmup {w}: {time.perf_counter() - t0:.3f} s")
+    times, kernels, batched = [], [], None
     for i in range(a.steps):
         t0 = time.perf_counter()
-        r = cw.decompose(mesh, **opts)
+        batched = eng.decompose_batch(meshes)
         times.append(time.perf_counter() - t0)
-        results.append(r)
-        print(f"step {i}: {times[-1]:.3f} s, {len(r.parts)} parts", file=sys.stderr, flush=True)
+        kernels.append(batched[0].kernels)  # per call: the batch's kernels
+        log(f"step {i}: {times[-1]:.3f} s, {sum(len(r.parts) for r in batched)} parts")
+    eng.destroy()
     total = sum(times)
-    r = results[-1]
     step_s = total / a.steps
-    hulls = r.stats["candidate_hulls"]
+    st = batched[0].stats
+    kst = sum_kernels(kernels)
+    hull_ms = (kst["ms"]["hull"] + kst["ms"]["hull_big"]) / a.steps
+    workload = f"batched decompose of the first {a.meshes} meshes of the 256-mesh mix"
     line = {
         "metric": "meshes/sec + candidate hulls/sec, batched FP32 vs plain, equal cuts, 1e-6 rel",
-        "value": a.steps / total,
+        "value": a.meshes * a.steps / total,
         "unit": "meshes/s",
         "n_gpus": 1,
         "steps": a.steps,
@@ -257,20 +283,54 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
-        "data": "synthetic (seeded sum-of-Gaussians mesh)",
-        "config": {"workload": f"plain decompose, {WORKLOAD['name']}, {mesh.nt} tris", "method": "plain",
-                   "options": opts, "parts": len(r.parts), "iterations": r.iterations},
-        "candidate_hulls_per_s": hulls / step_s,
-        "candidate_hulls": hulls,
-        "pruned": r.stats["pruned"],
-        "roofline": roofline(sum_kernels(results)),
+        "data": "synthetic (seeded mix: blob surfaces, box staircases, twisted tori)",
+        "config": {"workload": workload, "meshes": a.meshes, "triangles": tris, "seed": MIX["seed"],
+                   "method": "batched", "options": opts, "parts": sum(len(r.parts) for r in batched),
+                   "iterations": max(r.iterations for r in batched)},
+        "candidate_hulls": st["candidate_hulls"],
+        "pruned": st["pruned"],
+        "candidate_hulls_per_s": st["candidate_hulls"] / step_s,
+        "candidate_hulls_per_hull_kernel_s": st["candidate_hulls"] / (hull_ms * 1e-3) if hull_ms else None,
+        "step_breakdown_ms": {k: round(st[k], 1) for k in ("tree_ms", "hull_ms", "hausdorff_ms", "cut_ms",
+                                                             "concave_ms", "merge_ms", "total_ms")},
+        "roofline": roofline(kst, workload),
         "cpu_baseline": None,
     }
-    if a.batched:
-        line["batched_vs_plain"] = batched_vs_plain(cw, a.batched, 0 if a.no_cpu else a.cpu_mix)
+    if a.plain:
+        sel = list(range(0, a.meshes, a.plain))
+        peng = cw.Decomposer(**opts)
+        t0 = time.perf_counter()
+        plain = []
+        for n, k in enumerate(sel):
+            plain.append(peng.decompose(meshes[k]))
+            if n % 32 == 31:
+                log(f"plain: {n + 1} meshes, {time.perf_counter() - t0:.1f} s")
+        tp = time.perf_counter() - t0
+        peng.destroy()
+        same = all(_same(p, batched[k]) for p, k in zip(plain, sel))
+        log(f"plain: {len(sel)} meshes in {tp:.2f} s, identical: {same}")
+        line["plain"] = dict(meshes=len(sel), every=a.plain, meshes_per_s=len(sel) / tp,
+                             ratio_batched_vs_plain=line["value"] / (len(sel) / tp), identical=same)
+    if a.blob_steps:
+        mesh = cw.generate("blobs", BLOB["seed"], BLOB["params"])
+        bopts = dict(BLOB["options"], hull_wg_min_verts=tuned_split("blob"))
+        beng = cw.Decomposer(**bopts)
+        beng.decompose(mesh)  # warm
+        bt, br = [], None
+        for i in range(a.blob_steps):
+            t0 = time.perf_counter()
+            br = beng.decompose(mesh)
+            bt.append(time.perf_counter() - t0)
+        beng.destroy()
+        log(f"blob: {a.blob_steps} decompositions, {sum(bt) / len(bt):.3f} s each, {len(br.parts)} parts")
+        line["blob"] = dict(workload=f"plain decompose, {BLOB['name']}, {mesh.nt} tris", options=bopts,
+                            meshes_per_s=len(bt) / sum(bt), ms_per_mesh=1000 * sum(bt) / len(bt),
+                            parts=len(br.parts), candidate_hulls=br.stats["candidate_hulls"],
+                            candidate_hulls_per_s=br.stats["candidate_hulls"] / (sum(bt) / len(bt)))
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(cw, mesh, step_s, a.cpu_sample_iters)
-    print(json.dumps(line))
+        line["cpu_baseline"] = cpu_baseline(cw, batched)
+        line["batched_vs_cpu"] = line["value"] / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

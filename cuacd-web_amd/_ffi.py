@@ -115,6 +115,10 @@ TREE_API = {
     "decompose": (C.c_int, [P(f32), u32, P(u32), u32, P(DecomposeOptions), P(C.c_void_p)]),
     "merge_meshes": (C.c_int, [P(MeshBatch), P(DecomposeOptions), P(C.c_void_p)]),
     "decompose_batch": (C.c_int, [P(MeshBatch), P(DecomposeOptions), P(C.c_void_p)]),
+    "decomposer_create": (C.c_int, [P(DecomposeOptions), u64, P(C.c_void_p)]),
+    "decomposer_destroy": (None, [C.c_void_p]),
+    "decomposer_decompose": (C.c_int, [C.c_void_p, P(f32), u32, P(u32), u32, P(C.c_void_p)]),
+    "decomposer_decompose_batch": (C.c_int, [C.c_void_p, P(MeshBatch), P(C.c_void_p)]),
     "result_num_parts": (u32, [C.c_void_p]),
     "result_part": (C.c_int, [C.c_void_p, u32, P(PartDesc)]),
     "result_part_data": (C.c_int, [C.c_void_p, u32, P(f32), P(u32), P(f32), P(u32), P(f64)]),

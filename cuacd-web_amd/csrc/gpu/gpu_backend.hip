@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -678,69 +679,52 @@ std::vector<HullResult> GpuBackend::hullMeshes(const std::vector<MeshRef>& meshe
   return out;
 }
 
-std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b) {
-  const size_t n = a.size();
-  std::vector<double> out(n, 0.0);
-  if (n == 0) return out;
-  if (n > 1024) throw std::invalid_argument("hausdorff: at most 1024 pairs per call");
-  // meshes 0..n-1 = A, n..2n-1 = B; counts and sample offsets from the host rule
-  std::vector<MeshRef> all(a);
-  all.insert(all.end(), b.begin(), b.end());
+// The host-side layout of one Hausdorff batch: meshes 0..n-1 are the A sides, n..2n-1 the B sides;
+// per-triangle sample counts from the host rule (src/cpu/hausdorff.ts:27-48), one tile per HDS
+// samples of one direction of one pair (src/cpu/hausdorff.ts:226-231: a pair with an empty side
+// measures 0).
+struct HdPlan {
   std::vector<float> pv;
   std::vector<uint32_t> pt, counts;
-  std::vector<dk::HdMesh> hm(all.size());
+  std::vector<dk::HdMesh> hm;
+  std::vector<dk::HdTile> tiles;
   uint64_t sTotal = 0, tTotal = 0, chTotal = 0;
+  size_t nPairs = 0;
+};
+
+static HdPlan hdPlan(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b, uint32_t fixedRes) {
+  HdPlan P;
+  const size_t n = a.size();
+  P.nPairs = n;
+  std::vector<MeshRef> all(a);
+  all.insert(all.end(), b.begin(), b.end());
+  P.hm.resize(all.size());
   for (size_t k = 0; k < all.size(); k++) {
     const MeshRef& m = all[k];
-    dk::HdMesh& M = hm[k];
-    M.vOff = (uint32_t)pv.size();
-    M.tOff = (uint32_t)pt.size();
+    dk::HdMesh& M = P.hm[k];
+    M.vOff = (uint32_t)P.pv.size();
+    M.tOff = (uint32_t)P.pt.size();
     M.nt = m.nt;
-    M.gTri = (uint32_t)tTotal;
-    M.sOff = (uint32_t)sTotal;
-    M.cOff = (uint32_t)counts.size();
-    M.chOff = (uint32_t)chTotal;
-    chTotal += (m.nt + dk::HDC - 1) / dk::HDC;
-    pv.insert(pv.end(), m.verts, m.verts + 3ull * m.nv);
-    if (m.nt) pt.insert(pt.end(), m.tris, m.tris + 3ull * m.nt);
-    const std::vector<uint32_t> off = m.nt ? sampleOffsets(m, cfg_.hausdorffSamples) : std::vector<uint32_t>(1, 0);
-    counts.insert(counts.end(), off.begin(), off.end());
-    sTotal += off.back();
-    tTotal += m.nt;
+    M.gTri = (uint32_t)P.tTotal;
+    M.sOff = (uint32_t)P.sTotal;
+    M.cOff = (uint32_t)P.counts.size();
+    M.chOff = (uint32_t)P.chTotal;
+    P.chTotal += (m.nt + dk::HDC - 1) / dk::HDC;
+    P.pv.insert(P.pv.end(), m.verts, m.verts + 3ull * m.nv);
+    if (m.nt) P.pt.insert(P.pt.end(), m.tris, m.tris + 3ull * m.nt);
+    const std::vector<uint32_t> off = m.nt ? sampleOffsets(m, fixedRes) : std::vector<uint32_t>(1, 0);
+    P.counts.insert(P.counts.end(), off.begin(), off.end());
+    P.sTotal += off.back();
+    P.tTotal += m.nt;
   }
-  if (sTotal > 0x7fffffffull) throw std::length_error("hausdorff sample scratch exceeds capacity");
-  hdVerts_.ensure(pv.size());
-  hdTris_.ensure(pt.size());
-  hdCounts_.ensure(counts.size());
-  hdMeshes_.ensure(hm.size());
-  hdSamples_.ensure(3 * sTotal);
-  hdRec_.ensure(8 * std::max<uint64_t>(tTotal, 1));
-  hdChunks_.ensure(2 * std::max<uint64_t>(chTotal, 1));
-  h2d(hdVerts_.p, pv.data(), pv.size() * 4, "upload hd verts");
-  if (!pt.empty()) h2d(hdTris_.p, pt.data(), pt.size() * 4, "upload hd tris");
-  h2d(hdCounts_.p, counts.data(), counts.size() * 4, "upload counts");
-  h2d(hdMeshes_.p, hm.data(), hm.size() * sizeof(dk::HdMesh), "upload hd meshes");
-  clearWork();
-  if (tTotal) {
-    hipEvent_t a = tBegin();
-    dk::k_hd_samples<<<(uint32_t)((tTotal + 255) / 256), 256, 0, stream_>>>(hdMeshes_.p, (uint32_t)hm.size(), (uint32_t)tTotal,
-                                                                          hdVerts_.p, hdTris_.p, hdCounts_.p, hdSamples_.p);
-    launched("k_hd_samples");
-    dk::k_hd_prep<<<(uint32_t)((tTotal + 255) / 256), 256, 0, stream_>>>(hdMeshes_.p, (uint32_t)hm.size(), (uint32_t)tTotal,
-                                                                       hdVerts_.p, hdTris_.p, hdRec_.p);
-    launched("k_hd_prep");
-    dk::k_hd_chunks<<<(uint32_t)((chTotal + 3) / 4), 256, 0, stream_>>>(hdMeshes_.p, (uint32_t)hm.size(), (uint32_t)chTotal,
-                                                                      hdRec_.p, hdChunks_.p);
-    launched("k_hd_chunks");
-    tEnd(K_HD_SAMPLES, a);
-  }
-  std::vector<dk::HdTile> tiles;
+  if (P.sTotal > 0x7fffffffull || P.pv.size() > 0xffffffffull || P.pt.size() > 0xffffffffull)
+    throw std::length_error("hausdorff sample scratch exceeds capacity");
   for (size_t p = 0; p < n; p++) {
-    if (a[p].nt == 0 || b[p].nt == 0) continue;  // src/cpu/hausdorff.ts:227
+    if (a[p].nt == 0 || b[p].nt == 0) continue;
     for (int dir = 0; dir < 2; dir++) {
-      const dk::HdMesh& S = hm[dir == 0 ? p : n + p];
-      const dk::HdMesh& O = hm[dir == 0 ? n + p : p];
-      const uint32_t ns = counts[S.cOff + S.nt];
+      const dk::HdMesh& S = P.hm[dir == 0 ? p : n + p];
+      const dk::HdMesh& O = P.hm[dir == 0 ? n + p : p];
+      const uint32_t ns = P.counts[S.cOff + S.nt];
       for (uint32_t s0 = 0; s0 < ns; s0 += dk::HDS) {
         dk::HdTile t;
         t.pair = (uint32_t)p;
@@ -749,29 +733,96 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
         t.gTri = O.gTri;
         t.nt = O.nt;
         t.chOff = O.chOff;
-        tiles.push_back(t);
+        P.tiles.push_back(t);
       }
     }
   }
-  hipCheck(hipMemsetAsync(ctr_.p->hd, 0, n * 4, stream_), "clear hd results");
-  if (!tiles.empty()) {
-    hdTiles_.ensure(tiles.size());
-    h2d(hdTiles_.p, tiles.data(), tiles.size() * sizeof(dk::HdTile), "upload tiles");
-    hipEvent_t a = tBegin();
-    dk::k_hd_dist<<<(uint32_t)tiles.size(), dk::HDT, 0, stream_>>>(hdTiles_.p, (uint32_t)tiles.size(), hdSamples_.p,
-                                                                  hdRec_.p, hdChunks_.p, ctr_.p->hd, &ctr_.p->hdEvals);
-    launched("k_hd_dist");
-    tEnd(K_HD_DIST, a);
+  return P;
+}
+
+// Device arrays of one Hausdorff batch (sized by its HdPlan).
+struct HdDev {
+  float* verts;
+  uint32_t* tris;
+  uint32_t* counts;
+  dk::HdMesh* meshes;
+  float* samples;
+  float4* rec;
+  float4* chunks;
+  dk::HdTile* tiles;
+  uint32_t* bits;               // nPairs results: f32 bits of the squared distance
+  unsigned long long* evals;    // point-triangle evaluations (added to)
+};
+
+// Uploads a planned batch and queues its kernels on `s`; `timed(kernel family, begin)` brackets
+// launches when given (begin = true before, false after).
+template <typename Timer>
+static void hdLaunch(hipStream_t s, const HdPlan& P, const HdDev& D, Timer&& timed) {
+  auto up = [&](void* dst, const void* src, size_t bytes, const char* what) {
+    if (bytes) hipCheck(hipMemcpyWithStream(dst, src, bytes, hipMemcpyHostToDevice, s), what);
+  };
+  up(D.verts, P.pv.data(), P.pv.size() * 4, "upload hd verts");
+  up(D.tris, P.pt.data(), P.pt.size() * 4, "upload hd tris");
+  up(D.counts, P.counts.data(), P.counts.size() * 4, "upload counts");
+  up(D.meshes, P.hm.data(), P.hm.size() * sizeof(dk::HdMesh), "upload hd meshes");
+  const uint32_t nm = (uint32_t)P.hm.size(), nt = (uint32_t)P.tTotal, nch = (uint32_t)P.chTotal;
+  if (nt) {
+    timed(K_HD_SAMPLES, true);
+    dk::k_hd_samples<<<(nt + 255) / 256, 256, 0, s>>>(D.meshes, nm, nt, D.verts, D.tris, D.counts, D.samples);
+    hipCheck(hipGetLastError(), "k_hd_samples");
+    dk::k_hd_prep<<<(nt + 255) / 256, 256, 0, s>>>(D.meshes, nm, nt, D.verts, D.tris, D.rec);
+    hipCheck(hipGetLastError(), "k_hd_prep");
+    dk::k_hd_chunks<<<(nch + 3) / 4, 256, 0, s>>>(D.meshes, nm, nch, D.rec, D.chunks);
+    hipCheck(hipGetLastError(), "k_hd_chunks");
+    timed(K_HD_SAMPLES, false);
   }
-  std::vector<uint32_t> bits(n);
-  hipCheck(hipMemcpyAsync(bits.data(), ctr_.p->hd, n * 4, hipMemcpyDeviceToHost, stream_), "read hd results");
-  collectWork();
-  for (size_t p = 0; p < n; p++) {
+  dk::k_fill_u32<<<1, 256, 0, s>>>(D.bits, (uint32_t)P.nPairs, 0u);
+  hipCheck(hipGetLastError(), "k_fill_u32");
+  if (!P.tiles.empty()) {
+    up(D.tiles, P.tiles.data(), P.tiles.size() * sizeof(dk::HdTile), "upload tiles");
+    timed(K_HD_DIST, true);
+    dk::k_hd_dist<<<(uint32_t)P.tiles.size(), dk::HDT, 0, s>>>(D.tiles, (uint32_t)P.tiles.size(), D.samples, D.rec,
+                                                               D.chunks, D.bits, D.evals);
+    hipCheck(hipGetLastError(), "k_hd_dist");
+    timed(K_HD_DIST, false);
+  }
+}
+
+static std::vector<double> hdDistances(const std::vector<uint32_t>& bits) {
+  std::vector<double> out(bits.size());
+  for (size_t p = 0; p < bits.size(); p++) {
     float f;
     std::memcpy(&f, &bits[p], 4);
     out[p] = std::sqrt((double)f);
   }
   return out;
+}
+
+std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b) {
+  const size_t n = a.size();
+  if (n == 0) return {};
+  if (n > 1024) throw std::invalid_argument("hausdorff: at most 1024 pairs per call");
+  const HdPlan P = hdPlan(a, b, cfg_.hausdorffSamples);
+  hdVerts_.ensure(P.pv.size());
+  hdTris_.ensure(P.pt.size());
+  hdCounts_.ensure(P.counts.size());
+  hdMeshes_.ensure(P.hm.size());
+  hdSamples_.ensure(3 * P.sTotal);
+  hdRec_.ensure(8 * std::max<uint64_t>(P.tTotal, 1));
+  hdChunks_.ensure(2 * std::max<uint64_t>(P.chTotal, 1));
+  hdTiles_.ensure(P.tiles.size());
+  const HdDev D{hdVerts_.p, hdTris_.p, hdCounts_.p, hdMeshes_.p, hdSamples_.p, hdRec_.p, hdChunks_.p, hdTiles_.p,
+                ctr_.p->hd, &ctr_.p->hdEvals};
+  clearWork();
+  hipEvent_t ev = nullptr;
+  hdLaunch(stream_, P, D, [&](int k, bool begin) {
+    if (begin) ev = tBegin();
+    else tEnd(k, ev);
+  });
+  std::vector<uint32_t> bits(n);
+  hipCheck(hipMemcpyAsync(bits.data(), ctr_.p->hd, n * 4, hipMemcpyDeviceToHost, stream_), "read hd results");
+  collectWork();
+  return hdDistances(bits);
 }
 
 // the sample set k_hd_samples writes for one mesh, read back (a debugging and test view of the
@@ -823,3 +874,4 @@ std::vector<HullResult> GpuBackend::hullsWithHausdorff(const std::vector<MeshRef
 #define CUACD_API(name) cuacd_##name
 #define CUACD_MAKE_BACKEND(cfg) (new cuacd::GpuBackend(cfg))
 #include "../host/capi_body.inc"
+

@@ -33,6 +33,11 @@ struct HdMesh {     // one mesh of a Hausdorff batch
   uint32_t chOff;   // first triangle chunk (batch-global)
 };
 
+// n words set to v (a kernel, not a memset, so that the clear can sit in a captured graph)
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
 // one thread per triangle of the batch: write its samples
 __global__ void k_hd_samples(const HdMesh* __restrict__ meshes, uint32_t nMeshes, uint32_t nTriTotal,
                              const float* __restrict__ verts, const uint32_t* __restrict__ tris,

@@ -156,6 +156,22 @@ struct KernelStats {
     hullBytes += o.hullBytes;
     hullBytesBig += o.hullBytesBig;
   }
+  // the work since an earlier snapshot of the same counters
+  KernelStats since(const KernelStats& before) const {
+    KernelStats d = *this;
+    for (int k = 0; k < K_COUNT; k++) {
+      d.ms[k] -= before.ms[k];
+      d.launches[k] -= before.launches[k];
+    }
+    d.faceTests -= before.faceTests;
+    d.faceTestsBig -= before.faceTestsBig;
+    d.emitBytes -= before.emitBytes;
+    d.hdEvals -= before.hdEvals;
+    d.hullJobs -= before.hullJobs;
+    d.hullBytes -= before.hullBytes;
+    d.hullBytesBig -= before.hullBytesBig;
+    return d;
+  }
 };
 
 class Backend {

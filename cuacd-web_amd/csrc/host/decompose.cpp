@@ -566,6 +566,8 @@ DecomposeResult Decomposer::decompose(const Mesh& input) { return std::move(deco
 std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const Mesh*>& inputs) {
   const DecomposeOptions& o = o_;
   const double tStart = nowMs();
+  const KernelStats ks0 = be_.kernelStats();  // a reused backend: report this call's kernels only
+  poolGrowths_ = 0;
   std::vector<State> S(inputs.size());
   DecomposeStats st;  // the batch's counters and times
   std::vector<Mesh> starts(inputs.size());
@@ -788,7 +790,7 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
     s.R.parts = std::move(s.parts);
   }
   st.totalMs = nowMs() - tStart;
-  const KernelStats ks = be_.kernelStats();
+  const KernelStats ks = be_.kernelStats().since(ks0);
   std::vector<DecomposeResult> out;
   for (State& s : S) {
     s.R.stats = st;

@@ -154,6 +154,45 @@ def decompose_batch(meshes, options=None, api=None, **overrides):
     return [_read_result(api, C.c_void_p(hs[k])) for k in range(len(meshes))]
 
 
+class Decomposer:
+    """src/decompose.ts:128-398 Decomposer: one engine whose device pools persist across calls.
+    batch_tris > 0 sizes the pools for batches of about that many triangles."""
+
+    def __init__(self, options=None, api=None, batch_tris=0, **overrides):
+        self.api = api or load_product()
+        o = options or DecomposeOptions()
+        for k, v in overrides.items():
+            setattr(o, k, v)
+        self.options = o
+        raw = o.raw()
+        h = C.c_void_p()
+        check(self.api, self.api.decomposer_create(C.byref(raw), batch_tris, C.byref(h)))
+        self._h = h
+
+    def decompose(self, mesh):
+        h = C.c_void_p()
+        check(self.api, self.api.decomposer_decompose(self._h, mesh.verts.ctypes.data_as(C.POINTER(F.f32)), mesh.nv,
+                                                      mesh.tris.ctypes.data_as(C.POINTER(F.u32)), mesh.nt, C.byref(h)))
+        return _read_result(self.api, h)
+
+    def decompose_batch(self, meshes):
+        b = _batch(meshes)
+        hs = (C.c_void_p * max(len(meshes), 1))()
+        check(self.api, self.api.decomposer_decompose_batch(self._h, C.byref(b), hs))
+        return [_read_result(self.api, C.c_void_p(hs[k])) for k in range(len(meshes))]
+
+    def destroy(self):
+        if self._h:
+            self.api.decomposer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 def merge_hulls(meshes, options=None, api=None, **overrides):
     """mergeHulls (src/merge.ts:170) over parts made from `meshes` (hull + Hausdorff on the GPU)."""
     api = api or load_product()

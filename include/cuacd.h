@@ -25,6 +25,7 @@ extern "C" {
 typedef struct cuacd_tree cuacd_tree;
 typedef struct cuacd_result cuacd_result;
 typedef struct cuacd_meshes cuacd_meshes;
+typedef struct cuacd_decomposer cuacd_decomposer;
 
 /* src/gpu/tree.ts:14-46 TreeConfig (+ device ordinal, Hausdorff sample count) */
 typedef struct cuacd_tree_config {
@@ -206,7 +207,18 @@ void cuacd_decompose_options_default(cuacd_decompose_options* o);
  * batched form of that driver. */
 int cuacd_decompose_batch(const cuacd_mesh_batch* meshes, const cuacd_decompose_options* o, cuacd_result** results);
 
-/* decompose(mesh, options) (src/decompose.ts:407) */
+/* new Decomposer(device, opts) (src/decompose.ts:135-150) / destroy() (:152): one engine whose device
+ * pools persist across its decompose calls. batch_tris > 0 sizes the pools for batches of about that many
+ * triangles (as cuacd_decompose_batch sizes them per call); 0 sizes them for single meshes. */
+int cuacd_decomposer_create(const cuacd_decompose_options* o, uint64_t batch_tris, cuacd_decomposer** out);
+void cuacd_decomposer_destroy(cuacd_decomposer* d);
+/* Decomposer.decompose(mesh) (src/decompose.ts:214-398); kernel stats of the result cover this call */
+int cuacd_decomposer_decompose(cuacd_decomposer* d, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
+                               cuacd_result** out);
+/* the batched form on the same engine: results[k] equals cuacd_decomposer_decompose of mesh k */
+int cuacd_decomposer_decompose_batch(cuacd_decomposer* d, const cuacd_mesh_batch* meshes, cuacd_result** results);
+
+/* decompose(mesh, options) (src/decompose.ts:407): a Decomposer created and destroyed around one call */
 int cuacd_decompose(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
                     const cuacd_decompose_options* o, cuacd_result** out);
 /* mergeHulls over parts built from the given meshes (src/merge.ts:170) */

@@ -92,3 +92,28 @@ def test_batch_gpu_pool_overflow_recovers(cw, oracle, pool):
     assert batch[0].stats["tree_runs"] > base
     for m, rb in zip(meshes, batch):
         _same(cw.decompose(m, api=oracle, **OPTS), rb)
+
+
+def _engine_cases(cw, api, oracle):
+    """A Decomposer engine reused across calls (src/decompose.ts:128-398) gives what decompose() gives,
+    call after call, single or batched, whatever ran on it before."""
+    meshes = _meshes(cw)
+    eng = cw.Decomposer(api=api, batch_tris=sum(m.nt for m in meshes), **OPTS)
+    batch = eng.decompose_batch(meshes)
+    for m, rb in zip(meshes, batch):
+        _same(cw.decompose(m, api=oracle, **OPTS), rb)
+    for m in (meshes[1], meshes[3], meshes[1]):
+        _same(cw.decompose(m, api=oracle, **OPTS), eng.decompose(m))
+    again = eng.decompose_batch(meshes[::-1])
+    for m, rb in zip(meshes[::-1], again):
+        _same(cw.decompose(m, api=oracle, **OPTS), rb)
+    eng.destroy()
+
+
+def test_engine_reuse_oracle(cw, oracle):
+    _engine_cases(cw, oracle, oracle)
+
+
+@pytest.mark.gpu
+def test_engine_reuse_gpu(cw, oracle):
+    _engine_cases(cw, cw.load_product(), oracle)
