@@ -141,6 +141,7 @@ HOST_API = {
     "signed_volume": (f64, [P(f32), u32, P(u32), u32]),
     "parse_stl": (C.c_int, [P(C.c_uint8), u64, P(C.c_void_p)]),
     "generate": (C.c_int, [u32, u64, P(f64), u32, P(C.c_void_p)]),
+    "hausdorff_noalloc": (C.c_int, [P(MeshBatch), P(MeshBatch), u32, P(f64)]),
 }
 
 # error codes (include/cuacd.h)

@@ -875,3 +875,66 @@ std::vector<HullResult> GpuBackend::hullsWithHausdorff(const std::vector<MeshRef
 #define CUACD_MAKE_BACKEND(cfg) (new cuacd::GpuBackend(cfg))
 #include "../host/capi_body.inc"
 
+// ---- the allocation-free Hausdorff entry point ---------------------------------------------------
+// Every device array it uses lives in `scratch`, one static array per GPU that the runtime maps with
+// the code object and that stays until the process ends. Host code never passes the array's name to
+// a kernel or a copy: its address on the current device comes from hipGetSymbolAddress, taken once
+// per device and kept. Calls that overlap on one GPU share the array and race; calls are blocking.
+static constexpr uint64_t NOALLOC_WORDS = 16ull << 20;  // 64 MiB per GPU
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+namespace {
+std::mutex g_noallocMu;
+std::vector<uint32_t*> g_noallocPtr;  // by device ordinal
+
+uint32_t* noallocScratch(int* device) {
+  cuacd::hipCheck(hipGetDevice(device), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_noallocMu);
+  if ((size_t)*device >= g_noallocPtr.size()) g_noallocPtr.resize(*device + 1, nullptr);
+  if (!g_noallocPtr[*device]) {
+    void* p = nullptr;
+    cuacd::hipCheck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_noallocPtr[*device] = static_cast<uint32_t*>(p);
+  }
+  return g_noallocPtr[*device];
+}
+
+// bump layout of one batch inside the scratch array, each array 256-byte aligned
+struct Carve {
+  uint32_t* base;
+  uint64_t used = 0;
+  template <typename T>
+  T* take(uint64_t count) {
+    const uint64_t words = (count * sizeof(T) + 3) / 4;
+    T* p = reinterpret_cast<T*>(base + used);
+    used += (words + 63) & ~63ull;
+    return p;
+  }
+};
+}  // namespace
+
+CUACD_EXPORT int cuacd_hausdorff_noalloc(const cuacd_mesh_batch* a, const cuacd_mesh_batch* b,
+                                         uint32_t hausdorffSamples, double* out) {
+  return guarded([&] {
+    if (a->n != b->n) throw std::invalid_argument("hausdorff: batches differ in size");
+    if (a->n > 1024) throw std::invalid_argument("hausdorff: at most 1024 pairs per call");
+    if (a->n == 0) return;
+    const cuacd::HdPlan P = cuacd::hdPlan(toRefs(a), toRefs(b), hausdorffSamples);
+    int device = 0;
+    Carve c{noallocScratch(&device)};
+    const cuacd::HdDev D{c.take<float>(P.pv.size()), c.take<uint32_t>(P.pt.size()), c.take<uint32_t>(P.counts.size()),
+                         c.take<dk::HdMesh>(P.hm.size()), c.take<float>(3 * P.sTotal),
+                         c.take<float4>(8 * P.tTotal), c.take<float4>(2 * P.chTotal), c.take<dk::HdTile>(P.tiles.size()),
+                         c.take<uint32_t>(P.nPairs), c.take<unsigned long long>(1)};
+    if (c.used > NOALLOC_WORDS)
+      throw std::length_error("hausdorff_noalloc: the batch needs " + std::to_string(c.used * 4 >> 20) +
+                              " MiB of scratch, above the static " + std::to_string(NOALLOC_WORDS * 4 >> 20) + " MiB");
+    hipStream_t s = nullptr;  // the device's default stream; the call blocks
+    cuacd::hdLaunch(s, P, D, [](int, bool) {});
+    std::vector<uint32_t> bits(P.nPairs);
+    cuacd::hipCheck(hipMemcpyAsync(bits.data(), D.bits, P.nPairs * 4, hipMemcpyDeviceToHost, s), "read hd results");
+    cuacd::hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+    const std::vector<double> d = cuacd::hdDistances(bits);
+    for (size_t k = 0; k < d.size(); k++) out[k] = d[k];
+  });
+}

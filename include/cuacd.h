@@ -133,6 +133,15 @@ int cuacd_hausdorff(cuacd_tree* t, const cuacd_mesh_batch* a, const cuacd_mesh_b
  * n <= cap; res follows the tree's hausdorff_samples setting */
 int cuacd_sample_mesh(cuacd_tree* t, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, float* out,
                       uint64_t cap, uint64_t* n);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): cuacd_hausdorff without a tree handle
+ * and without any device allocation. It runs on the calling thread's current HIP device and its default
+ * stream and blocks. Every device array it needs (the uploaded meshes, samples, per-triangle records,
+ * tiles, results) lives in one static 64 MiB array per GPU, held until the process ends; a batch whose
+ * working set exceeds it fails with CUACD_ERR_SCRATCH. Calls that overlap on one GPU share that array
+ * and race. Same arguments and results as cuacd_hausdorff, with hausdorff_samples as in the tree config. */
+int cuacd_hausdorff_noalloc(const cuacd_mesh_batch* a, const cuacd_mesh_batch* b, uint32_t hausdorff_samples,
+                            double* out);
 /* rvCost(meshVol, hullVol) (src/part.ts:24-26), the FP64 cost the tree's record kernel evaluates */
 double cuacd_rv_cost(double mesh_vol, double hull_vol);
 
