@@ -61,7 +61,7 @@ struct LevelParams {
   uint32_t itemsInBase, nItemsIn, itemsOutBase, jobBase;
   uint32_t partBase, nJobs, width, nEdgeCuts;
   uint32_t totalWidth, level, depth, lastLevel;
-  uint32_t nSeeds, edgeStride, prune, pad;
+  uint32_t nSeeds, edgeStride, prune, itemChunk0;  // itemChunk0: first input item of a partial level
   float minEdgeDist, padf[3];
   uint64_t vCap, tCap, eCap, sCap;
 };

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -16,7 +17,7 @@
 #include "../host/sampling.hpp"
 #include "k_hd.hpp"
 #include "k_hull.hpp"
-#include "k_tree.hpp"
+#include "k_level.hpp"
 
 namespace cuacd {
 
@@ -68,6 +69,9 @@ class GpuBackend : public Backend {
     hipCheck(hipStreamCreate(&stream_), "hipStreamCreate");
     hipCheck(hipStreamCreate(&side_), "hipStreamCreate");
     ctr_.ensure(1);
+    scanCtl_.ensure(1);
+    lvlCounts_.ensure(1);
+    hipCheck(hipMemset(scanCtl_.p, 0, sizeof(dk::ScanCtl)), "clear scan control");
   }
   ~GpuBackend() override {
     if (stream_) (void)hipStreamSynchronize(stream_);
@@ -162,6 +166,10 @@ class GpuBackend : public Backend {
   dk::LevelParams levelParams(const Level& L, uint32_t level, uint32_t depth, uint32_t nSeeds) const;
   bool growPools(uint32_t overflowBits) override;
   void ensurePools();
+  // one tree level: plans, counts, child pools, geometry and volumes (kernels of k_level.hpp)
+  void runLevel(const Level& L, const dk::LevelParams& P);
+  template <typename S, typename T>
+  void scan(S s, uint32_t maxElems, T* vals);
   // hull the given jobs (slab offsets are assigned here, in arena-sized chunks)
   void runHulls(std::vector<dk::HullJob>& hj, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
                 std::vector<dk::HullResultD>* results);
@@ -169,7 +177,6 @@ class GpuBackend : public Backend {
 
   TreeConfig cfg_;
   uint32_t edgeStride_;
-  unsigned long long tetCap_ = 0;
   hipStream_t stream_ = nullptr;
   hipStream_t side_ = nullptr;  // small-part hulls, concurrent with the large-part kernel
   DevBuf<float> verts_;
@@ -181,8 +188,13 @@ class GpuBackend : public Backend {
   DevBuf<float4> jobPlanes_, edgePlanes_;
   DevBuf<dk::SeedState> state_;
   DevBuf<uint32_t> scratch_;
-  DevBuf<float4> tets_, jobAnchor_;  // emit's tetrahedron records (3 float4 each), anchor per job
+  DevBuf<float4> jobAnchor_;          // volume anchor per job
   DevBuf<unsigned long long> vacc_;  // volume limbs, 3 per child of a level
+  // per-level tile tables and the device-wide scans' state
+  DevBuf<uint32_t> jobTiles_, jobStart_, tileCnt_, tilePre_, scanStatus_;
+  DevBuf<unsigned long long> scanVals_;  // aggregates and inclusive prefixes of the scan chunks
+  DevBuf<dk::LevelCounts> lvlCounts_;
+  DevBuf<dk::ScanCtl> scanCtl_;
   DevBuf<dk::Counters> ctr_;
   DevBuf<uint32_t> arena_;
   DevBuf<dk::HullJob> hjobs_;
@@ -271,11 +283,86 @@ void GpuBackend::ensurePools() {
   tris_.ensure(std::min<uint64_t>(cfg_.triCap, 0xffffffffull));
   edges_.ensure(std::min<uint64_t>(cfg_.edgeCap, 0xffffffffull));
   scratch_.ensure(std::min<uint64_t>(cfg_.scratchWords, 0xffffffffull));
-  // tetrahedra of one level: at most the child triangles and cap edges it writes
-  // (48 bytes each; the 1.5G clamp is 72 GB of HBM, which batches of hundreds of
-  // meshes reach: their triangle pool is scaled for it)
-  tetCap_ = std::min<uint64_t>(cfg_.triCap / 3 + cfg_.edgeCap, 1536ull << 20);
-  tets_.ensure(3 * tetCap_);
+}
+
+// a device-wide scan (dk::scan_chunks) of at most maxElems elements with the scan state cleared first
+template <typename S, typename T>
+void GpuBackend::scan(S s, uint32_t maxElems, T* vals) {
+  const uint32_t nChunks = std::max<uint32_t>(1, (maxElems + dk::SCAN_CH - 1) / dk::SCAN_CH);
+  dk::k_scan_reset<<<std::min<uint32_t>((nChunks + 255) / 256, 1024), 256, 0, stream_>>>(scanStatus_.p, nChunks,
+                                                                                          scanCtl_.p);
+  launched("k_scan_reset");
+  const uint32_t grid = std::min<uint32_t>(nChunks, 1024);
+  T* agg = vals;
+  T* inc = vals + (size_t)nChunks * S::NF;
+  if constexpr (std::is_same<S, dk::TileStartScan>::value)
+    dk::k_scan_tile_starts<<<grid, dk::SCAN_T, 0, stream_>>>(s, scanStatus_.p, agg, inc, scanCtl_.p);
+  else if constexpr (std::is_same<S, dk::TileCountScan>::value)
+    dk::k_scan_tile_counts<<<grid, dk::SCAN_T, 0, stream_>>>(s, scanStatus_.p, agg, inc, scanCtl_.p);
+  else
+    dk::k_lvl_alloc<<<grid, dk::SCAN_T, 0, stream_>>>(s, scanStatus_.p, agg, inc, scanCtl_.p);
+  launched("device scan");
+}
+
+void GpuBackend::runLevel(const Level& L, const dk::LevelParams& P) {
+  // tiles per job: up to 64, fewer when the level has many jobs, so a level has about
+  // TILE_CAP tiles at most (or one per job)
+  const uint32_t maxTiles = std::max<uint32_t>(1, std::min<uint32_t>(64, dk::TILE_CAP / std::max<uint32_t>(L.nJobs, 1)));
+  const uint64_t tileBound = (uint64_t)L.nJobs * maxTiles;
+  if (tileBound >= 0xffffffffull / dk::TC_N) throw std::length_error("tree level exceeds the tile scratch");
+  const uint64_t maxElems = std::max<uint64_t>(tileBound, L.nJobs);
+  const uint64_t nChunks = (maxElems + dk::SCAN_CH - 1) / dk::SCAN_CH + 1;
+  jobTiles_.ensure(L.nJobs);
+  jobStart_.ensure(L.nJobs + 1ull);
+  tileCnt_.ensure(tileBound * dk::TC_N);
+  tilePre_.ensure((tileBound + 1) * dk::TC_N);
+  scanStatus_.ensure(nChunks);
+  scanVals_.ensure(2 * nChunks * dk::TC_N);
+  vacc_.ensure(6ull * L.nJobs);
+  constexpr uint32_t GRID = 2048;  // tile kernels: 8192 waves over the level's tiles
+  hipEvent_t e0 = tBegin();
+  dk::k_lvl_items<<<L.nItemsIn, dk::LT, 0, stream_>>>(P, items_.p, parts_.p, jobs_.p, jobPlanes_.p, edgePlanes_.p,
+                                                      verts_.p, jobTiles_.p, maxTiles, ctr_.p);
+  launched("k_lvl_items");
+  scan(dk::TileStartScan{jobTiles_.p, jobStart_.p, L.nJobs, lvlCounts_.p}, L.nJobs,
+       reinterpret_cast<uint32_t*>(scanVals_.p));
+  dk::k_lvl_clear_flags<<<GRID, 256, 0, stream_>>>(scratch_.p, ctr_.p, P.sCap);
+  launched("k_lvl_clear_flags");
+  dk::k_lvl_count<<<GRID, dk::LT, 0, stream_>>>(P, jobs_.p, parts_.p, verts_.p, tris_.p, edges_.p, jobStart_.p,
+                                                lvlCounts_.p, scratch_.p, tileCnt_.p);
+  launched("k_lvl_count");
+  dk::k_lvl_kept<<<GRID, dk::LT, 0, stream_>>>(P, jobs_.p, parts_.p, jobStart_.p, lvlCounts_.p, scratch_.p,
+                                               tileCnt_.p);
+  launched("k_lvl_kept");
+  tEnd(K_PLAN, e0);
+  e0 = tBegin();
+  scan(dk::TileCountScan{tileCnt_.p, tilePre_.p, lvlCounts_.p}, (uint32_t)tileBound,
+       reinterpret_cast<uint32_t*>(scanVals_.p));
+  dk::AllocScan as;
+  as.P = P;
+  as.items = items_.p;
+  as.parts = parts_.p;
+  as.jobs = jobs_.p;
+  as.start = jobStart_.p;
+  as.pre = tilePre_.p;
+  as.ctr = ctr_.p;
+  as.vacc = vacc_.p;
+  as.vTop = as.tTop = as.eTop = 0;
+  scan(as, L.nJobs, scanVals_.p);
+  tEnd(K_SCAN, e0);
+  e0 = tBegin();
+  dk::k_lvl_verts<<<GRID, dk::LT, 0, stream_>>>(P, jobs_.p, parts_.p, jobPlanes_.p, verts_.p, jobStart_.p,
+                                                lvlCounts_.p, tilePre_.p, scratch_.p, jobAnchor_.p);
+  launched("k_lvl_verts");
+  dk::k_lvl_emit<<<GRID, dk::LT, 0, stream_>>>(P, jobs_.p, parts_.p, jobPlanes_.p, verts_.p, tris_.p, edges_.p,
+                                               jobStart_.p, lvlCounts_.p, tilePre_.p, scratch_.p, jobAnchor_.p,
+                                               vacc_.p);
+  launched("k_lvl_emit");
+  tEnd(K_EMIT, e0);
+  e0 = tBegin();
+  dk::k_lvl_volfin<<<(2 * L.nJobs + 255) / 256, 256, 0, stream_>>>(P, jobs_.p, vacc_.p, parts_.p);
+  launched("k_lvl_volfin");
+  tEnd(K_VOLUME, e0);
 }
 
 void GpuBackend::runHulls(std::vector<dk::HullJob>& hjIn, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
@@ -446,30 +533,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       const Level& L = levels[l];
       const dk::LevelParams P = levelParams(L, l, depth, nSeeds);
       if (L.nJobs == 0) continue;
-      hipEvent_t e0 = tBegin();
-      dk::k_plan<<<L.nJobs, dk::BT, 0, stream_>>>(P, items_.p, parts_.p, jobs_.p, jobPlanes_.p, edgePlanes_.p, verts_.p,
-                                                  tris_.p, edges_.p, scratch_.p, ctr_.p);
-      launched("k_plan");
-      tEnd(K_PLAN, e0);
-      e0 = tBegin();
-      dk::k_scan<<<1, dk::ST, 0, stream_>>>(P, items_.p, parts_.p, jobs_.p, ctr_.p);
-      launched("k_scan");
-      tEnd(K_SCAN, e0);
-      e0 = tBegin();
-      vacc_.ensure(6ull * L.nJobs);
-      dk::k_vol_clear<<<std::min<uint32_t>((6 * L.nJobs + 255) / 256, 1024), 256, 0, stream_>>>(vacc_.p, 6 * L.nJobs,
-                                                                                               ctr_.p);
-      launched("k_vol_clear");
-      dk::k_emit<<<L.nJobs, dk::BT, 0, stream_>>>(P, parts_.p, jobs_.p, jobPlanes_.p, verts_.p, tris_.p, edges_.p,
-                                                  scratch_.p, tets_.p, tetCap_, jobAnchor_.p, ctr_.p);
-      launched("k_emit");
-      tEnd(K_EMIT, e0);
-      e0 = tBegin();
-      dk::k_volume<<<2048, dk::VT, 0, stream_>>>(P, tets_.p, tetCap_, jobAnchor_.p, vacc_.p, ctr_.p);
-      launched("k_volume");
-      dk::k_volume_fin<<<(2 * L.nJobs + 255) / 256, 256, 0, stream_>>>(P, jobs_.p, vacc_.p, parts_.p);
-      launched("k_volume_fin");
-      tEnd(K_VOLUME, e0);
+      runLevel(L, P);
       // hull jobs for every child with enough points
       const uint32_t nChild = 2 * L.nJobs;
       hostParts.resize(nChild);
@@ -525,7 +589,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       E.nJobs[l] = levels[l].nJobs;
     }
     hipEvent_t e2 = tBegin();
-    dk::k_evaluate<<<nSeeds, dk::BT, 0, stream_>>>(E, items_.p, jobPlanes_.p, edgePlanes_.p, state_.p);
+    dk::k_evaluate<<<nSeeds, dk::LT, 0, stream_>>>(E, items_.p, jobPlanes_.p, edgePlanes_.p, state_.p);
     launched("k_evaluate");
     tEnd(K_EVALUATE, e2);
     lastLevelCounts_.assign(1, nSeeds);
@@ -536,7 +600,13 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
            "read seed state");
   dk::Counters c1;
   hipCheck(hipMemcpyAsync(&c1, ctr_.p, offsetof(dk::Counters, hd), hipMemcpyDeviceToHost, stream_), "read counters");
+  dk::ScanCtl sc;
+  hipCheck(hipMemcpyAsync(&sc, scanCtl_.p, sizeof(sc), hipMemcpyDeviceToHost, stream_), "read scan state");
   sync();
+  if (sc.error | sc.sticky) {
+    hipCheck(hipMemset(scanCtl_.p, 0, sizeof(dk::ScanCtl)), "clear scan control");
+    throw DeviceError("device scan: a chunk's look-back waited past its limit");
+  }
   for (uint32_t k = 0; k < nSeeds; k++) {
     SeedResult r;
     r.valid = st[k].valid != 0;

@@ -150,17 +150,15 @@ def test_hausdorff_bit_exact(cw, oracle):
 
 
 def test_pool_overflow_names_the_pool(cw):
-    """A run whose pools are too small reports which pools ran out (1 scratch,
-    2 vertices, 4 triangles, 8 edges, 16 tetrahedra) and stops there; with pools that
-    just fit, the result equals the run with the default pools. The tetrahedra of
-    one level (child triangles plus cap edges, the volume integrand) are bounded by
-    tri_cap / 3 + edge_cap; the last level writes the most of them, since it keeps no
-    topology but computes every child's volume."""
+    """A run whose pools are too small reports which pools ran out (1 scratch, 2 vertices, 4 triangles,
+    8 edges) and stops there; with pools that just fit, the result equals the run with the default
+    pools. Child volumes are summed inside the emitting kernel, so no pool of volume terms exists to
+    run out (the bit for it, 16, is never set)."""
     meshes = [stairs(cw), square_torus(cw), l_prism(cw)]
     cfg = dict(width=12, width2=6, depth=2, scratch_words=4 << 20, hull_budget_bytes=256 << 20)
 
     def run(**caps):
-        t = cw.GpuTree(**cfg, **caps)
+        t = cw.GpuTree(**{**cfg, **caps})
         r = t.run(meshes, read_levels=False)
         t.destroy()
         return r
@@ -183,8 +181,14 @@ def test_pool_overflow_names_the_pool(cw):
     assert 8 in masks
     t, masks = smallest("tri_cap", 1 << 10, 1 << 20, edge_cap=e)
     print("tri_cap fit", t, "masks", sorted(masks), flush=True)
-    assert any(m & 16 for m in masks)
-    r = run(tri_cap=t, edge_cap=e)
+    assert any(m & 4 for m in masks) and not any(m & 16 for m in masks)
+    v, masks = smallest("vertex_cap", 1 << 10, 1 << 20, edge_cap=e, tri_cap=t)
+    print("vertex_cap fit", v, "masks", sorted(masks), flush=True)
+    assert any(m & 2 for m in masks)
+    s_, masks = smallest("scratch_words", 1 << 8, 1 << 22, edge_cap=e, tri_cap=t, vertex_cap=v)
+    print("scratch_words fit", s_, "masks", sorted(masks), flush=True)
+    assert any(m & 1 for m in masks)
+    r = run(tri_cap=t, edge_cap=e, vertex_cap=v, scratch_words=s_)
     assert r.overflow_pools == 0 and r.hulls_done == ref.hulls_done
     for a, b in zip(r.seeds, ref.seeds):
         assert (a.best_cut, a.best_cost, a.plane) == (b.best_cut, b.best_cost, b.plane)
