@@ -101,35 +101,90 @@ __device__ __forceinline__ void acc_det(A& acc, const float* a, const float* b, 
   acc_triple(acc, a[2], b[0], c[1], neg);
   acc_triple(acc, a[2], b[1], c[0], !neg);
 }
-// det(a-o, b-o, c-o) as 4 x 6 triple products in rolled loops (operands chosen by
-// selects): the same integer sum as acc_det_anchor with few live registers
-__device__ __forceinline__ void acc_tet_rolled(Acc128& acc, const float* a, const float* b, const float* c,
-                                               const float* o) {
+// x*y as two f32 terms whose sum is exact (TwoProduct by fma), added with the given sign
+__device__ __forceinline__ void acc_prod(Acc128& acc, float x, float y, bool neg) {
+  const float p = x * y;
+  const float e = __builtin_fmaf(x, y, -p);
+  acc.add_f32(p, neg);
+  acc.add_f32(e, neg);
+}
+// A child's volume terms about an anchor o applied once, at the end:
+//   sum det(a-o, b-o, c-o) = sum det(a, b, c) - o . sum (b x c + c x a + a x b)
+// (multilinearity; the terms with o twice vanish), both sums exact in fixed point.
+struct VolAcc {
+  Acc128 d, n[3];
+};
+__device__ __forceinline__ void acc_cross(Acc128* n, const float* u, const float* w) {
+  acc_prod(n[0], u[1], w[2], false);
+  acc_prod(n[0], u[2], w[1], true);
+  acc_prod(n[1], u[2], w[0], false);
+  acc_prod(n[1], u[0], w[2], true);
+  acc_prod(n[2], u[0], w[1], false);
+  acc_prod(n[2], u[1], w[0], true);
+}
+__device__ __forceinline__ void acc_vol(VolAcc& v, const float* a, const float* b, const float* c) {
+  acc_det(v.d, a, b, c, false);
+  acc_cross(v.n, b, c);
+  acc_cross(v.n, c, a);
+  acc_cross(v.n, a, b);
+}
+// the same sums in rolled loops (operands chosen by selects), so few registers stay live
+__device__ __forceinline__ void acc_vol_rolled(VolAcc& v, const float* a, const float* b, const float* c) {
 #pragma unroll 1
-  for (int t = 0; t < 4; t++) {
-    const bool neg0 = t != 0;
-    float A[3], B[3], C[3];
+  for (int k = 0; k < 6; k++) {  // det(a, b, c): six signed triple products, acc_det's order
+    const float x = k < 2 ? a[0] : (k < 4 ? a[1] : a[2]);
+    const float y = (k == 0 || k == 5) ? b[1] : ((k == 1 || k == 2) ? b[2] : b[0]);
+    const float z = (k == 0 || k == 3) ? c[2] : ((k == 1 || k == 4) ? c[1] : c[0]);
+    acc_triple(v.d, x, y, z, (k & 1) != 0);
+  }
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-      A[q] = t == 1 ? o[q] : a[q];
-      B[q] = t == 2 ? o[q] : b[q];
-      C[q] = t == 3 ? o[q] : c[q];
-    }
+  for (int q = 0; q < 3; q++) {  // component q of b x c + c x a + a x b
+    const int i1 = q == 0 ? 1 : (q == 1 ? 2 : 0), i2 = q == 0 ? 2 : (q == 1 ? 0 : 1);
 #pragma unroll 1
-    for (int k = 0; k < 6; k++) {
-      const float x = k < 2 ? A[0] : (k < 4 ? A[1] : A[2]);
-      const float y = (k == 0 || k == 5) ? B[1] : ((k == 1 || k == 2) ? B[2] : B[0]);
-      const float z = (k == 0 || k == 3) ? C[2] : ((k == 1 || k == 4) ? C[1] : C[0]);
-      acc_triple(acc, x, y, z, (k & 1) ? !neg0 : neg0);
+    for (int p = 0; p < 3; p++) {
+      const float* u = p == 0 ? b : (p == 1 ? c : a);
+      const float* w = p == 0 ? c : (p == 1 ? a : b);
+      acc_prod(v.n[q], u[i1], w[i2], false);
+      acc_prod(v.n[q], u[i2], w[i1], true);
     }
   }
 }
-template <typename A>
-__device__ __forceinline__ void acc_det_anchor(A& acc, const float* a, const float* b, const float* c, const float* o) {
-  acc_det(acc, a, b, c, false);
-  acc_det(acc, o, b, c, true);
-  acc_det(acc, a, o, c, true);
-  acc_det(acc, a, b, o, true);
+__device__ __forceinline__ void acc128_add(Acc128& a, const Acc128& b) {
+  a.lo += b.lo;
+  a.hi += b.hi + (a.lo < b.lo ? 1ull : 0ull);
+}
+
+// o * N exactly (N in fixed point), truncated toward zero to the fixed point's 2^-64
+__device__ inline i128 fix_mul_f32(i128 N, float o) {
+  typedef unsigned __int128 u128;
+  const uint32_t b = __float_as_uint(o);
+  uint32_t e = (b >> 23) & 0xffu, m = b & 0x7fffffu;
+  if (e == 0) {
+    if (m == 0) return 0;
+    e = 1;
+  } else {
+    m |= 0x800000u;
+  }
+  if (N == 0) return 0;
+  const int E = (int)e - 150;  // o = m * 2^E
+  const bool neg = ((b >> 31) != 0) != (N < 0);
+  const u128 U = N < 0 ? (u128)0 - (u128)N : (u128)N;
+  const u128 lm = (u128)(uint64_t)U * m, hm = (u128)(uint64_t)(U >> 64) * m;
+  const uint64_t w0 = (uint64_t)lm;
+  const u128 t = (lm >> 64) + (u128)(uint64_t)hm;
+  const uint64_t w1 = (uint64_t)t, w2 = (uint64_t)(hm >> 64) + (uint64_t)(t >> 64);
+  u128 r;
+  if (E >= 0) {
+    r = (((u128)w1 << 64) | w0) << E;
+  } else {
+    const int sh = -E;  // the 192-bit product shifted right by sh: its low 128 bits
+    const int q = sh / 64, k = sh % 64;
+    auto limb = [&](int i) -> uint64_t { return i == 0 ? w0 : (i == 1 ? w1 : (i == 2 ? w2 : 0ull)); };
+    const uint64_t o0 = (limb(q) >> k) | (k ? limb(q + 1) << (64 - k) : 0ull);
+    const uint64_t o1 = (limb(q + 1) >> k) | (k ? limb(q + 2) << (64 - k) : 0ull);
+    r = q >= 3 ? (u128)0 : (((u128)o1 << 64) | o0);
+  }
+  return neg ? -(i128)r : (i128)r;
 }
 
 // ---- wave (64-lane) primitives

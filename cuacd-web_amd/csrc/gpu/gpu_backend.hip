@@ -318,7 +318,7 @@ void GpuBackend::runLevel(const Level& L, const dk::LevelParams& P) {
   tilePre_.ensure((tileBound + 1) * dk::TC_N);
   scanStatus_.ensure(nChunks);
   scanVals_.ensure(2 * nChunks * dk::TC_N);
-  vacc_.ensure(6ull * L.nJobs);
+  vacc_.ensure(2ull * dk::VOL_LIMBS * L.nJobs);
   constexpr uint32_t GRID = 2048;  // tile kernels: 8192 waves over the level's tiles
   hipEvent_t e0 = tBegin();
   dk::k_lvl_items<<<L.nItemsIn, dk::LT, 0, stream_>>>(P, items_.p, parts_.p, jobs_.p, jobPlanes_.p, edgePlanes_.p,
@@ -360,7 +360,7 @@ void GpuBackend::runLevel(const Level& L, const dk::LevelParams& P) {
   launched("k_lvl_emit");
   tEnd(K_EMIT, e0);
   e0 = tBegin();
-  dk::k_lvl_volfin<<<(2 * L.nJobs + 255) / 256, 256, 0, stream_>>>(P, jobs_.p, vacc_.p, parts_.p);
+  dk::k_lvl_volfin<<<(2 * L.nJobs + 255) / 256, 256, 0, stream_>>>(P, jobs_.p, vacc_.p, jobAnchor_.p, parts_.p);
   launched("k_lvl_volfin");
   tEnd(K_VOLUME, e0);
 }

@@ -41,7 +41,7 @@ namespace dk {
 
 constexpr uint32_t LT = 256;          // threads per block of the tile kernels (four tiles in flight)
 constexpr uint32_t TILE_MIN = 256;    // smallest tile (triangles, vertices or edges)
-constexpr uint32_t TILE_CAP = 1u << 18;  // tiles per level are kept near this (bounds the tile tables)
+constexpr uint32_t TILE_CAP = 1u << 22;  // tiles per level are kept below this (bounds the tile tables)
 
 // per-tile counts, in this order
 enum : uint32_t {
@@ -58,10 +58,22 @@ enum : uint32_t {
   TC_N
 };
 
-struct LevelCounts {  // device counters of one level (cleared at the level's start)
+// per child: the exact sums D and N[3] of its volume terms, 3 limbs each
+constexpr uint32_t VOL_LIMBS = 12;
+
+struct LevelCounts {  // device counters of one level (set by the tile-start scan)
   uint32_t nTiles;    // tiles of the level (the tile-start scan's total)
+  uint32_t next[4];   // work counters of the four tile kernels (count, kept, verts, emit)
   uint32_t pad[3];
 };
+
+// the next tile for this wave, taken from a level-wide counter (wave-uniform): waves that
+// drew small tiles come back for more while a large one is still being cut
+__device__ __forceinline__ uint32_t next_tile(uint32_t* counter) {
+  uint32_t t = 0;
+  if (lane_id() == 0) t = atomicAdd(counter, 1u);
+  return rdlane(t, 0);
+}
 
 // ---------------------------------------------------------------------------
 // block helpers (256 threads)
@@ -227,6 +239,7 @@ struct TileStartScan {
   __device__ void total(const T* t) const {
     start[n] = t[0];
     lc->nTiles = t[0];
+    for (int q = 0; q < 4; q++) lc->next[q] = 0;
   }
 };
 __global__ __launch_bounds__(SCAN_T) void k_scan_tile_starts(TileStartScan s, uint32_t* status, uint32_t* agg,
@@ -325,7 +338,7 @@ __global__ void k_lvl_clear_flags(uint32_t* __restrict__ scratch, const Counters
 __global__ __launch_bounds__(LT) void k_lvl_count(LevelParams P, const DJob* __restrict__ jobs,
                                                   const DPart* __restrict__ parts, const float* __restrict__ verts,
                                                   const uint32_t* __restrict__ tris, const DEdge* __restrict__ edges,
-                                                  const uint32_t* __restrict__ start, const LevelCounts* __restrict__ lc,
+                                                  const uint32_t* __restrict__ start, LevelCounts* __restrict__ lc,
                                                   uint32_t* __restrict__ scratch, uint32_t* __restrict__ tcnt) {
   const uint32_t lane = lane_id();
   const uint32_t nTiles = lc->nTiles;
@@ -386,7 +399,7 @@ __global__ __launch_bounds__(LT) void k_lvl_count(LevelParams P, const DJob* __r
 // k_lvl_kept: per tile, the parent vertices each side keeps (after every side flag is set)
 __global__ __launch_bounds__(LT) void k_lvl_kept(LevelParams P, const DJob* __restrict__ jobs,
                                                  const DPart* __restrict__ parts, const uint32_t* __restrict__ start,
-                                                 const LevelCounts* __restrict__ lc, const uint32_t* __restrict__ scratch,
+                                                 LevelCounts* __restrict__ lc, const uint32_t* __restrict__ scratch,
                                                  uint32_t* __restrict__ tcnt) {
   const uint32_t lane = lane_id();
   const uint32_t nTiles = lc->nTiles;
@@ -425,7 +438,7 @@ struct AllocScan {
   const uint32_t* start;
   const uint32_t* pre;  // per-tile prefixes
   Counters* ctr;
-  unsigned long long* vacc;  // 3 limbs per child of the level, cleared here
+  unsigned long long* vacc;  // VOL_LIMBS per child of the level, cleared here
   unsigned long long vTop, tTop, eTop;
   // the job's totals: cnt[0..7] as posNv posNt posNe negNv negNt negNe posKept negKept
   __device__ void totals(uint32_t g, uint32_t* cnt) const {
@@ -560,7 +573,7 @@ struct AllocScan {
     parts[negP] = neg;
     items[P.itemsOutBase + g] = out;
     if (job.valid != 0) jobs[P.jobBase + g] = job;
-    for (int q = 0; q < 6; q++) vacc[6ull * g + q] = 0;
+    for (uint32_t q = 0; q < 2 * VOL_LIMBS; q++) vacc[2ull * VOL_LIMBS * g + q] = 0;
   }
   __device__ void total(const T* t) const {
     const unsigned long long v = vTop + t[0], tt = tTop + t[1], e = eTop + t[2];
@@ -586,7 +599,7 @@ __global__ __launch_bounds__(SCAN_T) void k_lvl_alloc(AllocScan s, uint32_t* sta
 __global__ __launch_bounds__(LT) void k_lvl_verts(LevelParams P, const DJob* __restrict__ jobs,
                                                   const DPart* __restrict__ parts, const float4* __restrict__ jobPlanes,
                                                   float* __restrict__ verts, const uint32_t* __restrict__ start,
-                                                  const LevelCounts* __restrict__ lc, const uint32_t* __restrict__ pre,
+                                                  LevelCounts* __restrict__ lc, const uint32_t* __restrict__ pre,
                                                   uint32_t* __restrict__ scratch, float4* __restrict__ jobAnchor) {
   const uint32_t lane = lane_id();
   const uint32_t nTiles = lc->nTiles;
@@ -684,18 +697,74 @@ __device__ __forceinline__ void add_limbs(unsigned long long* a3, i128 v) {
   }
 }
 
+// Volume terms go through a per-wave LDS stack: emission pushes each term's three points
+// (per side, a clipped triangle makes up to two fan terms and one on-plane edge term), and whenever 64
+// or more are stacked every lane evaluates one, so the 72 exact products of a term run on full
+// waves instead of on the few lanes whose triangle crosses. The sum is exact, so the order in
+// which terms are taken does not matter.
+constexpr uint32_t TQ_CAP = 64 + 3 * 64;  // at most 63 left over plus one side's terms of one step
+struct TetQ {
+  float a[3], b[3], c[3];
+  uint32_t side;
+};
+
+__device__ __forceinline__ void tq_push(TetQ* q, uint32_t at, const float* a, const float* b, const float* c,
+                                        uint32_t side) {
+  TetQ& e = q[at];
+  for (int k = 0; k < 3; k++) {
+    e.a[k] = a[k];
+    e.b[k] = b[k];
+    e.c[k] = c[k];
+  }
+  e.side = side;
+}
+// evaluates the stack's top entries 64 at a time while 64 or more are stacked (all = true: every
+// entry); *qn is wave-uniform
+__device__ __forceinline__ void tq_drain(TetQ* q, uint32_t* qn, bool all, VolAcc* acc) {
+  wave_sync();
+  const uint32_t lane = lane_id();
+  while (*qn >= 64 || (all && *qn > 0)) {
+    const uint32_t take = *qn >= 64 ? 64 : *qn;
+    const uint32_t base = *qn - take;
+    if (lane < take) {
+      const TetQ& e = q[base + lane];
+      float a[3], b[3], c[3];
+      for (int k = 0; k < 3; k++) {
+        a[k] = e.a[k];
+        b[k] = e.b[k];
+        c[k] = e.c[k];
+      }
+      VolAcc t;
+      acc_vol_rolled(t, a, b, c);
+      const Acc128 z;
+      const bool s1 = e.side != 0;
+      acc128_add(acc[0].d, s1 ? z : t.d);
+      acc128_add(acc[1].d, s1 ? t.d : z);
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        acc128_add(acc[0].n[q], s1 ? z : t.n[q]);
+        acc128_add(acc[1].n[q], s1 ? t.n[q] : z);
+      }
+    }
+    *qn = base;
+  }
+  wave_sync();
+}
+
 __global__ __launch_bounds__(LT) void k_lvl_emit(LevelParams P, const DJob* __restrict__ jobs,
                                                  const DPart* __restrict__ parts, const float4* __restrict__ jobPlanes,
                                                  float* __restrict__ verts, uint32_t* __restrict__ tris,
                                                  DEdge* __restrict__ edges, const uint32_t* __restrict__ start,
-                                                 const LevelCounts* __restrict__ lc, const uint32_t* __restrict__ pre,
+                                                 LevelCounts* __restrict__ lc, const uint32_t* __restrict__ pre,
                                                  const uint32_t* __restrict__ scratch,
                                                  const float4* __restrict__ jobAnchor,
                                                  unsigned long long* __restrict__ vacc) {
+  __shared__ TetQ s_q[LT / 64][TQ_CAP];
+  TetQ* Q = s_q[threadIdx.x >> 6];
   const uint32_t lane = lane_id();
   const uint32_t nTiles = lc->nTiles;
   const bool keepTopo = P.lastLevel == 0;
-  for (uint32_t w = (blockIdx.x * LT + threadIdx.x) >> 6; w < nTiles; w += (gridDim.x * LT) >> 6) {
+  for (uint32_t w = next_tile(&lc->next[3]); w < nTiles; w = next_tile(&lc->next[3])) {
     const uint32_t g = job_of_tile(start, P.nJobs, w);
     const DJob job = jobs[P.jobBase + g];
     if (job.valid != 1) continue;
@@ -704,8 +773,6 @@ __global__ __launch_bounds__(LT) void k_lvl_emit(LevelParams P, const DJob* __re
     const DPart part = parts[job.parentPart];
     const uint32_t len = tile_len(part, nT);
     const float pl[4] = {job.plane[0], job.plane[1], job.plane[2], job.plane[3]};
-    const float4 o4 = jobAnchor[jobIdx];
-    const float o[3] = {o4.x, o4.y, o4.z};
     float cP[3];
     plane_point(pl, cP);
     const float* V = verts + part.vOff;
@@ -719,7 +786,8 @@ __global__ __launch_bounds__(LT) void k_lvl_emit(LevelParams P, const DJob* __re
     uint32_t rT[2] = {pw[TC_PT] - p0[TC_PT], pw[TC_NT] - p0[TC_NT]};
     uint32_t rN[2] = {kept[0] + pw[TC_PN] - p0[TC_PN], kept[1] + pw[TC_NN] - p0[TC_NN]};
     uint32_t rE[2] = {pw[TC_PE] - p0[TC_PE], pw[TC_NE] - p0[TC_NE]};
-    Acc128 acc[2];
+    VolAcc acc[2];
+    uint32_t qn = 0;
     uint32_t tb, te;
     tile_range(k, len, part.nt, &tb, &te);
     for (uint32_t t0 = tb; t0 < te; t0 += 64) {
@@ -740,7 +808,12 @@ __global__ __launch_bounds__(LT) void k_lvl_emit(LevelParams P, const DJob* __re
       const unsigned long long ex[2] = {wave_excl_u64(st[0], &tot[0]), wave_excl_u64(st[1], &tot[1])};
 #pragma unroll
       for (int s = 0; s < 2; s++) {
-        if (!st[s]) continue;
+        // this side's volume terms of this lane (fan triangles, on-plane edges) and their stack slots
+        const uint32_t nq = (uint32_t)(st[s] & 0x1fffff) + (uint32_t)(st[s] >> 42);
+        const uint32_t qi = wave_incl_scan(nq);
+        uint32_t qa = qn + qi - nq;
+        qn += rdlane(qi, 63);
+        if (st[s]) {
         const Poly py = clip_tri(sg, s == 0 ? 1 : -1);
         uint32_t tBase = rT[s] + (uint32_t)(ex[s] & 0x1fffff);
         uint32_t vBase = rN[s] + (uint32_t)((ex[s] >> 21) & 0x1fffff);
@@ -781,7 +854,7 @@ __global__ __launch_bounds__(LT) void k_lvl_emit(LevelParams P, const DJob* __re
             dst[1] = idx[i];
             dst[2] = idx[i + 1];
           }
-          acc_tet_rolled(acc[s], pts[0], pts[i], pts[i + 1], o);
+          tq_push(Q, qa++, pts[0], pts[i], pts[i + 1], s);
           tBase++;
         }
 #pragma unroll
@@ -803,10 +876,12 @@ __global__ __launch_bounds__(LT) void k_lvl_emit(LevelParams P, const DJob* __re
               e.pad = 0;
               edges[eOut[s] + eBase] = e;
             }
-            acc_tet_rolled(acc[s], cP, nx, pts[i], o);
+            tq_push(Q, qa++, cP, nx, pts[i], s);
             eBase++;
           }
         }
+        }
+        tq_drain(Q, &qn, false, acc);
       }
 #pragma unroll
       for (int s = 0; s < 2; s++) {
@@ -834,6 +909,9 @@ __global__ __launch_bounds__(LT) void k_lvl_emit(LevelParams P, const DJob* __re
       const uint32_t c0 = in && (sp || sq) ? 1u : 0u, c1 = in && (!sp || !sq) ? 1u : 0u;
       const uint64_t m0 = __ballot(c0 != 0), m1 = __ballot(c1 != 0);
       uint32_t bs[2] = {rI[0] + prefix_below(m0), rI[1] + prefix_below(m1)};
+      const uint32_t qi = wave_incl_scan(c0 + c1);
+      uint32_t qa = qn + qi - (c0 + c1);
+      qn += rdlane(qi, 63);
       if (in) {
         const float4 q4 = jobPlanes[ed.plane];
         const float epl[4] = {q4.x, q4.y, q4.z, q4.w};
@@ -850,7 +928,7 @@ __global__ __launch_bounds__(LT) void k_lvl_emit(LevelParams P, const DJob* __re
             ne.pad = 0;
             edges[eOut[s] + bs[s]] = ne;
           }
-          acc_tet_rolled(acc[s], cE, b, a, o);
+          tq_push(Q, qa++, cE, b, a, s);
           bs[s]++;
         };
         if (sp == sq) {
@@ -870,19 +948,30 @@ __global__ __launch_bounds__(LT) void k_lvl_emit(LevelParams P, const DJob* __re
       }
       rI[0] += (uint32_t)__popcll(m0);
       rI[1] += (uint32_t)__popcll(m1);
+      tq_drain(Q, &qn, false, acc);
     }
-    add_limbs(vacc + 6ull * g, acc[0].value());
-    add_limbs(vacc + 6ull * g + 3, acc[1].value());
+    tq_drain(Q, &qn, true, acc);
+#pragma unroll
+    for (int sd = 0; sd < 2; sd++) {
+      unsigned long long* a = vacc + VOL_LIMBS * (2ull * g + sd);
+      add_limbs(a, acc[sd].d.value());
+      for (int q = 0; q < 3; q++) add_limbs(a + 3 * (q + 1), acc[sd].n[q].value());
+    }
   }
 }
 
-// k_lvl_volfin: per child of the level, |sum of its volume terms| / 6
+// k_lvl_volfin: per child of the level, |D - o . N| / 6 with the job's anchor o
+__device__ __forceinline__ i128 from_limbs(const unsigned long long* a3) {
+  return ((i128)(long long)a3[2] << 84) + ((i128)a3[1] << 42) + (i128)a3[0];
+}
 __global__ void k_lvl_volfin(LevelParams P, const DJob* __restrict__ jobs, const unsigned long long* __restrict__ vacc,
-                             DPart* __restrict__ parts) {
+                             const float4* __restrict__ jobAnchor, DPart* __restrict__ parts) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= 2 * P.nJobs || jobs[P.jobBase + c / 2].valid != 1) return;
-  const unsigned long long* a3 = vacc + 3ull * c;
-  const i128 v = ((i128)(long long)a3[2] << 84) + ((i128)a3[1] << 42) + (i128)a3[0];
+  const unsigned long long* a = vacc + VOL_LIMBS * (size_t)c;
+  const float4 o = jobAnchor[P.jobBase + c / 2];
+  const i128 v = from_limbs(a) - fix_mul_f32(from_limbs(a + 3), o.x) - fix_mul_f32(from_limbs(a + 6), o.y) -
+                 fix_mul_f32(from_limbs(a + 9), o.z);
   parts[P.partBase + c].meshVol = fix_to_f64(fix_abs(v)) / 6.0;
 }
 

@@ -61,12 +61,72 @@ inline void acc_det(i128& acc, const float* a, const float* b, const float* c, b
   acc_triple(acc, a[2], b[0], c[1], neg);
   acc_triple(acc, a[2], b[1], c[0], !neg);
 }
-// det[a-o; b-o; c-o] by multilinearity (no rounded subtraction)
-inline void acc_det_anchor(i128& acc, const float* a, const float* b, const float* c, const float* o) {
-  acc_det(acc, a, b, c, false);
-  acc_det(acc, o, b, c, true);
-  acc_det(acc, a, o, c, true);
-  acc_det(acc, a, b, o, true);
+// x*y as two f32 terms whose sum is exact (TwoProduct by fmaf), accumulated with the given sign
+inline void acc_prod(i128& acc, float x, float y, bool neg) {
+  const float p = x * y;
+  const float e = std::fmaf(x, y, -p);
+  const i128 s = fix_f32(p) + fix_f32(e);
+  acc += neg ? -s : s;
+}
+// acc += u x w, componentwise
+inline void acc_cross(i128* n, const float* u, const float* w) {
+  acc_prod(n[0], u[1], w[2], false);
+  acc_prod(n[0], u[2], w[1], true);
+  acc_prod(n[1], u[2], w[0], false);
+  acc_prod(n[1], u[0], w[2], true);
+  acc_prod(n[2], u[0], w[1], false);
+  acc_prod(n[2], u[1], w[0], true);
+}
+// A child's volume terms about an anchor o that is applied once, at the end:
+//   sum det[a-o; b-o; c-o] = sum det[a; b; c] - o . sum (b x c + c x a + a x b)
+// (multilinearity; the terms with o twice vanish). d and n are exact fixed-point sums.
+struct VolSum {
+  i128 d = 0;
+  i128 n[3] = {0, 0, 0};
+};
+inline void acc_vol(VolSum& v, const float* a, const float* b, const float* c) {
+  acc_det(v.d, a, b, c, false);
+  acc_cross(v.n, b, c);
+  acc_cross(v.n, c, a);
+  acc_cross(v.n, a, b);
+}
+// o * N exactly (N in fixed point), truncated toward zero to the fixed point's 2^-64
+inline i128 fix_mul_f32(i128 N, float o) {
+  typedef unsigned __int128 u128;
+  uint32_t b;
+  std::memcpy(&b, &o, 4);
+  uint32_t e = (b >> 23) & 0xffu, m = b & 0x7fffffu;
+  if (e == 0) {
+    if (m == 0) return 0;
+    e = 1;
+  } else {
+    m |= 0x800000u;
+  }
+  if (N == 0) return 0;
+  const int E = (int)e - 150;  // o = m * 2^E
+  const bool neg = ((b >> 31) != 0) != (N < 0);
+  const u128 U = N < 0 ? (u128)0 - (u128)N : (u128)N;
+  const u128 lm = (u128)(uint64_t)U * m, hm = (u128)(uint64_t)(U >> 64) * m;
+  const uint64_t w0 = (uint64_t)lm;
+  const u128 t = (lm >> 64) + (u128)(uint64_t)hm;
+  const uint64_t w1 = (uint64_t)t, w2 = (uint64_t)(hm >> 64) + (uint64_t)(t >> 64);
+  u128 r;
+  if (E >= 0) {
+    r = (((u128)w1 << 64) | w0) << E;
+  } else {
+    const int sh = -E;  // the 192-bit product shifted right by sh: its low 128 bits
+    const uint64_t L[3] = {w0, w1, w2};
+    const int q = sh / 64, k = sh % 64;
+    auto limb = [&](int i) -> uint64_t { return i < 3 ? L[i] : 0ull; };
+    const uint64_t o0 = (limb(q) >> k) | (k ? limb(q + 1) << (64 - k) : 0ull);
+    const uint64_t o1 = (limb(q + 1) >> k) | (k ? limb(q + 2) << (64 - k) : 0ull);
+    r = q >= 3 ? (u128)0 : (((u128)o1 << 64) | o0);
+  }
+  return neg ? -(i128)r : (i128)r;
+}
+// six times the signed volume of a child: its sums with the anchor applied
+inline i128 vol_value(const VolSum& v, const float* o) {
+  return v.d - fix_mul_f32(v.n[0], o[0]) - fix_mul_f32(v.n[1], o[1]) - fix_mul_f32(v.n[2], o[2]);
 }
 
 // ---- exact hull volume on a 2^G quantization grid (G = 21 or 24)

@@ -270,7 +270,7 @@ void OracleBackend::runLevel(const Level& L, uint32_t level, uint32_t depth, uin
           c.v.insert(c.v.end(), &P.v[3 * i], &P.v[3 * i] + 3);
         }
     }
-    i128 flux[2] = {0, 0};
+    VolSum flux[2];
     for (uint32_t t = 0; t < nt; t++) {
       const uint32_t* tr = &P.t[3 * t];
       const int s3[3] = {sg[tr[0]], sg[tr[1]], sg[tr[2]]};
@@ -304,7 +304,7 @@ void OracleBackend::runLevel(const Level& L, uint32_t level, uint32_t depth, uin
             c.t.push_back(idx[i]);
             c.t.push_back(idx[i + 1]);
           }
-          acc_det_anchor(flux[s], pts[0], pts[i], pts[i + 1], o);
+          acc_vol(flux[s], pts[0], pts[i], pts[i + 1]);
         }
         for (uint32_t i = 0; i < pp.n; i++) {
           const uint32_t j = (i + 1) % pp.n;
@@ -318,7 +318,7 @@ void OracleBackend::runLevel(const Level& L, uint32_t level, uint32_t depth, uin
               e.plane = jobIdx;
               c.e.push_back(e);
             }
-            acc_det_anchor(flux[s], cP, pts[j], pts[i], o);
+            acc_vol(flux[s], cP, pts[j], pts[i]);
           }
         }
       }
@@ -339,7 +339,7 @@ void OracleBackend::runLevel(const Level& L, uint32_t level, uint32_t depth, uin
           ne.plane = e.plane;
           ch[s]->e.push_back(ne);
         }
-        acc_det_anchor(flux[s], cE, b, a, o);
+        acc_vol(flux[s], cE, b, a);
       };
       if (sp == sq) {
         put(sp ? 0 : 1, e.p, e.q);
@@ -356,7 +356,7 @@ void OracleBackend::runLevel(const Level& L, uint32_t level, uint32_t depth, uin
         }
       }
     }
-    for (int s = 0; s < 2; s++) ch[s]->meshVol = fix_to_f64(fix_abs(flux[s])) / 6.0;
+    for (int s = 0; s < 2; s++) ch[s]->meshVol = fix_to_f64(fix_abs(vol_value(flux[s], o))) / 6.0;
   }
   // ---- hulls of every child part (independent: spread over host threads)
   const bool prune = lastLevel && level > 0 && cfg_.pruneLastLevel && nSeeds > 0;
