@@ -39,8 +39,8 @@ struct DEdge {  // open boundary edge p -> q on plane `plane` (a job index)
   uint32_t plane, pad;
 };
 
-// pools a tree level can run out of (bits of Counters::overflow)
-constexpr uint32_t OVF_SCRATCH = 1, OVF_VERTS = 2, OVF_TRIS = 4, OVF_EDGES = 8, OVF_TETS = 16;
+// pools a tree run can run out of (bits of Counters::overflow); 16 is unused
+constexpr uint32_t OVF_SCRATCH = 1, OVF_VERTS = 2, OVF_TRIS = 4, OVF_EDGES = 8, OVF_ARENA = 32;
 
 struct Counters {
   uint32_t vTop, tTop, eTop, pad0;  // pool bump pointers (floats, u32, edges)
@@ -52,7 +52,7 @@ struct Counters {
   unsigned long long faceTestsBig;  // point-face record evaluations, workgroup-per-part hull kernel
   unsigned long long hullBytes;     // algorithmic HBM bytes, wave-per-part hull: 12 nv + 16 (+ 12 per exported face)
   unsigned long long hullBytesBig;  // the same for the workgroup-per-part hull kernel
-  unsigned long long tetTop;        // tetrahedra written by emit in the current level
+  unsigned long long pad1;
   unsigned long long sTop;          // scratch words reserved by plan in the current level (64-bit: cannot wrap)
   uint32_t hd[1024];  // Hausdorff results (f32 bits of the squared distance)
 };
@@ -73,6 +73,15 @@ struct HullJob {
   uint32_t exportOff;  // u32 offset of the face export (NONE = no export)
   uint64_t slabOff;   // u32 words into the slab arena
   uint32_t slabWords, job;  // job index of the part's parent job (tree levels) or NONE
+};
+
+// a work list of the persistent hull kernels: jobs [0, count) handed out from head
+struct HullQueue {
+  uint32_t count, head, pad[2];
+};
+// the hull slab arena: a bump pointer and its capacity (u32 words)
+struct HullArena {
+  unsigned long long top, cap;
 };
 
 struct HullResultD {

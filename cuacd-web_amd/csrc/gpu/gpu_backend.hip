@@ -171,8 +171,14 @@ class GpuBackend : public Backend {
   template <typename S, typename T>
   void scan(S s, uint32_t maxElems, T* vals);
   // hull the given jobs (slab offsets are assigned here, in arena-sized chunks)
-  void runHulls(std::vector<dk::HullJob>& hj, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
-                std::vector<dk::HullResultD>* results);
+  // exported hulls of host meshes: the jobs in queue order, results in the same order
+  void runHulls(std::vector<dk::HullJob>& hj, uint32_t* exportBuf, std::vector<dk::HullResultD>* results);
+  // every child of a tree level with enough points: queues built on the device
+  void levelHulls(const Level& L, const dk::HullLevel& HL);
+  // the persistent hull kernels over the queues in hq_ (wavefront queue on the side stream)
+  void launchHulls(bool exact, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL, uint32_t gridBig,
+                   uint32_t gridSmallWaves);
+  void ensureArena(uint64_t minWords);
   void uploadMeshes(const std::vector<MeshRef>& meshes, std::vector<uint32_t>* vOff, std::vector<uint32_t>* tOff);
 
   TreeConfig cfg_;
@@ -197,8 +203,11 @@ class GpuBackend : public Backend {
   DevBuf<dk::ScanCtl> scanCtl_;
   DevBuf<dk::Counters> ctr_;
   DevBuf<uint32_t> arena_;
-  DevBuf<dk::HullJob> hjobs_;
-  DevBuf<dk::HullResultD> hres_;
+  DevBuf<dk::HullArena> arenaCtl_;
+  DevBuf<dk::HullQueues> hq_;
+  DevBuf<dk::HullJob> hqSmall_, hqBig_;
+  DevBuf<dk::HullResultD> hresSmall_, hresBig_;
+  DevBuf<uint32_t> hqTag_, hqRank_;
   DevBuf<uint32_t> export_;
   DevBuf<float> hdVerts_, hdSamples_;
   DevBuf<uint32_t> hdTris_, hdCounts_;
@@ -262,24 +271,28 @@ dk::LevelParams GpuBackend::levelParams(const Level& L, uint32_t level, uint32_t
 }
 
 bool GpuBackend::growPools(uint32_t bits) {
+  // true only when every pool that ran out got larger: a pool at its limit makes a rerun futile
   constexpr uint64_t LIM = 0xffffffffull;  // element counts stay 32-bit indexable
-  bool grew = false;
-  auto dbl = [&](uint64_t& cap) {
-    if (cap >= LIM) return false;
-    cap = std::min<uint64_t>(cap * 2, LIM);
-    grew = true;
-    return true;
+  bool all = bits != 0;
+  auto dbl = [&](uint64_t& cap, uint64_t lim) {
+    if (cap >= lim) {
+      all = false;
+      return;
+    }
+    cap = std::min<uint64_t>(cap * 2, lim);
   };
-  if (bits & dk::OVF_SCRATCH) dbl(cfg_.scratchWords);
-  if (bits & dk::OVF_VERTS) dbl(cfg_.vertexCap);
-  if (bits & dk::OVF_TRIS) dbl(cfg_.triCap);
-  if (bits & dk::OVF_EDGES) dbl(cfg_.edgeCap);
-  if ((bits & dk::OVF_TETS) && !(bits & dk::OVF_TRIS) && !dbl(cfg_.triCap)) dbl(cfg_.edgeCap);
-  return grew;  // the next run's ensurePools() allocates the larger pools
+  if (bits & dk::OVF_SCRATCH) dbl(cfg_.scratchWords, LIM);
+  if (bits & dk::OVF_VERTS) dbl(cfg_.vertexCap, LIM);
+  if (bits & dk::OVF_TRIS) dbl(cfg_.triCap, LIM);
+  if (bits & dk::OVF_EDGES) dbl(cfg_.edgeCap, LIM);
+  if (bits & dk::OVF_ARENA) dbl(cfg_.hullBudgetBytes, 128ull << 30);
+  if (bits & ~(dk::OVF_SCRATCH | dk::OVF_VERTS | dk::OVF_TRIS | dk::OVF_EDGES | dk::OVF_ARENA)) all = false;
+  return all;  // the next run's ensurePools() / ensureArena() allocate the larger pools
 }
 
 void GpuBackend::ensurePools() {
   verts_.ensure(std::min<uint64_t>(cfg_.vertexCap, 0xffffffffull));
+  ensureArena(0);
   tris_.ensure(std::min<uint64_t>(cfg_.triCap, 0xffffffffull));
   edges_.ensure(std::min<uint64_t>(cfg_.edgeCap, 0xffffffffull));
   scratch_.ensure(std::min<uint64_t>(cfg_.scratchWords, 0xffffffffull));
@@ -365,81 +378,131 @@ void GpuBackend::runLevel(const Level& L, const dk::LevelParams& P) {
   tEnd(K_VOLUME, e0);
 }
 
-void GpuBackend::runHulls(std::vector<dk::HullJob>& hjIn, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
-                          std::vector<dk::HullResultD>* results) {
-  if (hjIn.empty()) return;
-  // Exported hulls take exactly the largest keys each round (their face list is
-  // the same for every group size); tree candidates need only the exact volume
-  // and take each wave's largest keys.
-  const bool exact = exportBuf != nullptr;
-  // large parts first (a workgroup each), then small ones (a wavefront each)
-  std::vector<uint32_t> order(hjIn.size());
-  for (uint32_t k = 0; k < order.size(); k++) order[k] = k;
-  std::stable_partition(order.begin(), order.end(), [&](uint32_t k) { return hjIn[k].nv >= cfg_.hullWgMinVerts; });
-  std::vector<dk::HullJob> hj(hjIn.size());
-  for (size_t k = 0; k < order.size(); k++) hj[k] = hjIn[order[k]];
+void GpuBackend::ensureArena(uint64_t minWords) {
   const uint64_t budgetWords = std::max<uint64_t>(cfg_.hullBudgetBytes / 4, 1ull << 20);
-  uint64_t maxSlab = 0;
-  for (const dk::HullJob& j : hj) maxSlab = std::max(maxSlab, dk::hullSlabWords(j.nv));
-  arena_.ensure(std::max(budgetWords, maxSlab));
-  hjobs_.ensure(hj.size());
-  hres_.ensure(hj.size());
-  size_t start = 0;
-  while (start < hj.size()) {
-    uint64_t used = 0;
-    size_t end = start;
-    while (end < hj.size()) {
-      const uint64_t w = dk::hullSlabWords(hj[end].nv);
-      if (used + w > arena_.n && end > start) break;
-      hj[end].slabOff = used;
-      hj[end].slabWords = (uint32_t)std::min<uint64_t>(w, 0xffffffffull);
-      used += w;
-      end++;
-    }
-    size_t mid = start;
-    while (mid < end && hj[mid].nv >= cfg_.hullWgMinVerts) mid++;
-    h2d(hjobs_.p + start, hj.data() + start, (end - start) * sizeof(dk::HullJob), "upload hull jobs");
-    // The large-part kernel holds one workgroup per part, usually far fewer than the
-    // chip's CUs, for its whole duration: the small parts run beside it on the side
-    // stream, and the main stream waits for both before anything else.
-    if (end > mid) join(stream_, side_);
-    if (mid > start) {
-      const uint32_t n = (uint32_t)(mid - start);
-      hipEvent_t a = tBegin();
-      if (exact)
-        dk::k_hull_big<true><<<n, dk::HBIG, 0, stream_>>>(hjobs_.p + start, n, verts_.p, arena_.p, parts,
-                                                          hres_.p + start, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
-      else
-        dk::k_hull_big<false><<<n, dk::HBIG, 0, stream_>>>(hjobs_.p + start, n, verts_.p, arena_.p, parts,
-                                                           hres_.p + start, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
-      launched("k_hull_big");
-      tEnd(K_HULL_BIG, a);
-    }
-    if (end > mid) {
-      const uint32_t n = (uint32_t)(end - mid);
-      hipEvent_t a = tBegin(side_);
-      const dim3 grid((n + dk::HW - 1) / dk::HW), block(64 * dk::HW);
-      if (exact)
-        dk::k_hull<true><<<grid, block, 0, side_>>>(hjobs_.p + mid, n, verts_.p, arena_.p, parts, hres_.p + mid,
-                                                    exportBuf, items_.p, jobs_.p, HL, ctr_.p);
-      else
-        dk::k_hull<false><<<grid, block, 0, side_>>>(hjobs_.p + mid, n, verts_.p, arena_.p, parts, hres_.p + mid,
-                                                     exportBuf, items_.p, jobs_.p, HL, ctr_.p);
-      launched("k_hull");
-      tEnd(K_HULL, a, side_);
-      join(side_, stream_);
-    }
-    kst_.hullJobs += end - start;
-    start = end;
+  const uint64_t want = std::max(budgetWords, minWords);
+  if (arena_.n < want || !arenaCtl_.p) {
+    arena_.ensure(want);
+    arenaCtl_.ensure(1);
   }
-  if (results) {
-    std::vector<dk::HullResultD> r(hj.size());
-    hipCheck(hipMemcpyAsync(r.data(), hres_.p, hj.size() * sizeof(dk::HullResultD), hipMemcpyDeviceToHost, stream_),
+  const dk::HullArena a{0, arena_.n};
+  h2d(arenaCtl_.p, &a, sizeof(a), "hull arena");
+}
+
+void GpuBackend::launchHulls(bool exact, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
+                             uint32_t gridBig, uint32_t gridSmallWaves) {
+  // the large-part kernel holds a workgroup per part, usually fewer than the chip's CUs at
+  // its tail: the small parts run beside it on the side stream
+  join(stream_, side_);
+  dk::HullQueue* qs = &hq_.p->q[0];
+  dk::HullQueue* qb = &hq_.p->q[1];
+  if (gridBig) {
+    hipEvent_t a = tBegin();
+    if (exact)
+      dk::k_hull_big<true><<<gridBig, dk::HBIG, 0, stream_>>>(hqBig_.p, qb, arenaCtl_.p, verts_.p, arena_.p, parts,
+                                                              hresBig_.p, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+    else
+      dk::k_hull_big<false><<<gridBig, dk::HBIG, 0, stream_>>>(hqBig_.p, qb, arenaCtl_.p, verts_.p, arena_.p, parts,
+                                                               hresBig_.p, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+    launched("k_hull_big");
+    tEnd(K_HULL_BIG, a);
+  }
+  if (gridSmallWaves) {
+    const uint32_t blocks = (gridSmallWaves + dk::HW - 1) / dk::HW;
+    hipEvent_t a = tBegin(side_);
+    if (exact)
+      dk::k_hull<true><<<blocks, 64 * dk::HW, 0, side_>>>(hqSmall_.p, qs, arenaCtl_.p, verts_.p, arena_.p, parts,
+                                                         hresSmall_.p, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+    else
+      dk::k_hull<false><<<blocks, 64 * dk::HW, 0, side_>>>(hqSmall_.p, qs, arenaCtl_.p, verts_.p, arena_.p, parts,
+                                                          hresSmall_.p, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+    launched("k_hull");
+    tEnd(K_HULL, a, side_);
+  }
+  join(side_, stream_);
+}
+
+// persistent grids of the tree levels: two workgroups per CU, four wavefronts per SIMD
+constexpr uint32_t HULL_GRID_BIG = 512, HULL_GRID_SMALL_WAVES = 4096;
+
+void GpuBackend::levelHulls(const Level& L, const dk::HullLevel& HL) {
+  const uint32_t nChild = 2 * L.nJobs;
+  hq_.ensure(1);
+  hqTag_.ensure(nChild);
+  hqRank_.ensure(nChild);
+  hqSmall_.ensure(nChild);
+  hqBig_.ensure(nChild);
+  hresSmall_.ensure(nChild);
+  hresBig_.ensure(nChild);
+  if (!arenaCtl_.p) ensureArena(0);
+  const uint32_t blocks = (nChild + 255) / 256;
+  dk::k_hq_reset<<<1, 64, 0, stream_>>>(hq_.p);
+  launched("k_hq_reset");
+  dk::k_hq_count<<<blocks, 256, 0, stream_>>>(parts_.p, L.partBase, nChild, cfg_.hullWgMinVerts, hq_.p, hqTag_.p,
+                                              hqRank_.p);
+  launched("k_hq_count");
+  dk::k_hq_place<<<blocks, 256, 0, stream_>>>(parts_.p, L.partBase, L.jobBase, nChild, hq_.p, hqTag_.p, hqRank_.p,
+                                              hqSmall_.p, hqBig_.p, arenaCtl_.p);
+  launched("k_hq_place");
+  launchHulls(false, parts_.p, nullptr, HL, std::min(HULL_GRID_BIG, nChild), std::min(HULL_GRID_SMALL_WAVES, nChild));
+}
+
+void GpuBackend::runHulls(std::vector<dk::HullJob>& hj, uint32_t* exportBuf, std::vector<dk::HullResultD>* results) {
+  results->assign(hj.size(), dk::HullResultD{});
+  if (hj.empty()) return;
+  // queues by size class, largest first (stable), split at hullWgMinVerts
+  std::vector<uint32_t> order(hj.size());
+  for (uint32_t k = 0; k < order.size(); k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return dk::hullClassOf(hj[a].nv) > dk::hullClassOf(hj[b].nv); });
+  std::vector<dk::HullJob> big, small;
+  std::vector<uint32_t> bigIdx, smallIdx;
+  uint32_t maxBig = 0, maxSmall = 0;
+  for (uint32_t k : order) {
+    if (hj[k].nv >= cfg_.hullWgMinVerts) {
+      big.push_back(hj[k]);
+      bigIdx.push_back(k);
+      maxBig = std::max(maxBig, hj[k].nv);
+    } else {
+      small.push_back(hj[k]);
+      smallIdx.push_back(k);
+      maxSmall = std::max(maxSmall, hj[k].nv);
+    }
+  }
+  // grids whose slabs fit the arena together (each group keeps one slab of its first, largest class)
+  const uint64_t slabBig = big.empty() ? 0 : dk::hullSlabWords(dk::hullClassCap(maxBig));
+  const uint64_t slabSmall = small.empty() ? 0 : dk::hullSlabWords(dk::hullClassCap(maxSmall));
+  ensureArena(slabBig + slabSmall);
+  uint64_t gBig = std::min<uint64_t>(HULL_GRID_BIG, big.size()), gSmall = std::min<uint64_t>(HULL_GRID_SMALL_WAVES, small.size());
+  while (gBig * slabBig + gSmall * slabSmall > arena_.n) {
+    if (gBig * slabBig >= gSmall * slabSmall && gBig > 1) gBig = (gBig + 1) / 2;
+    else if (gSmall > 1) gSmall = (gSmall + 1) / 2;
+    else break;
+  }
+  hq_.ensure(1);
+  hqBig_.ensure(big.size());
+  hqSmall_.ensure(small.size());
+  hresBig_.ensure(big.size());
+  hresSmall_.ensure(small.size());
+  h2d(hqBig_.p, big.data(), big.size() * sizeof(dk::HullJob), "upload hull jobs");
+  h2d(hqSmall_.p, small.data(), small.size() * sizeof(dk::HullJob), "upload hull jobs");
+  const dk::HullQueue q[2] = {{(uint32_t)small.size(), 0, {0, 0}}, {(uint32_t)big.size(), 0, {0, 0}}};
+  h2d(hq_.p->q, q, sizeof(q), "hull queues");
+  dk::HullLevel HL;
+  std::memset(&HL, 0, sizeof(HL));
+  launchHulls(true, nullptr, exportBuf, HL, (uint32_t)gBig, (uint32_t)gSmall);
+  kst_.hullJobs += hj.size();
+  std::vector<dk::HullResultD> rb(big.size()), rs(small.size());
+  if (!rb.empty())
+    hipCheck(hipMemcpyAsync(rb.data(), hresBig_.p, rb.size() * sizeof(dk::HullResultD), hipMemcpyDeviceToHost, stream_),
              "read hull results");
-    sync();
-    results->resize(hj.size());
-    for (size_t k = 0; k < order.size(); k++) (*results)[order[k]] = r[k];
-  }
+  if (!rs.empty())
+    hipCheck(hipMemcpyAsync(rs.data(), hresSmall_.p, rs.size() * sizeof(dk::HullResultD), hipMemcpyDeviceToHost,
+                            stream_),
+             "read hull results");
+  sync();
+  for (size_t k = 0; k < rb.size(); k++) (*results)[bigIdx[k]] = rb[k];
+  for (size_t k = 0; k < rs.size(); k++) (*results)[smallIdx[k]] = rs[k];
 }
 
 TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vector<std::vector<float>>& edgePlanes,
@@ -523,9 +586,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
   h2d(ctr_.p, &c0, offsetof(dk::Counters, hd), "init counters");
   // ---- stages: the tree, then each refinement round over the probe layout
   const uint32_t stages = 1 + (refine ? opts.refine->rounds : 0);
-  std::vector<dk::DPart> hostParts;
-  bool aborted = false;
-  for (uint32_t s = 0; s < stages && !aborted; s++) {
+  for (uint32_t s = 0; s < stages; s++) {
     const std::vector<Level>& levels = s == 0 ? treeLevels : probeLevels;
     if (s > 0)  // pool pointers back to the seeds; flags and counts accumulate
       h2d(ctr_.p, &c0, 4 * sizeof(uint32_t), "reset counters");
@@ -534,33 +595,6 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       const dk::LevelParams P = levelParams(L, l, depth, nSeeds);
       if (L.nJobs == 0) continue;
       runLevel(L, P);
-      // hull jobs for every child with enough points
-      const uint32_t nChild = 2 * L.nJobs;
-      hostParts.resize(nChild);
-      hipCheck(hipMemcpyAsync(hostParts.data(), parts_.p + L.partBase, nChild * sizeof(dk::DPart), hipMemcpyDeviceToHost,
-                              stream_),
-               "read child parts");
-      uint32_t ovf = 0;
-      hipCheck(hipMemcpyAsync(&ovf, &ctr_.p->overflow, sizeof(ovf), hipMemcpyDeviceToHost, stream_), "read overflow");
-      sync();
-      if (ovf) {  // a pool ran out: the caller runs these seeds again in smaller batches
-        aborted = true;
-        break;
-      }
-      std::vector<dk::HullJob> hj;
-      for (uint32_t c = 0; c < nChild; c++) {
-        const dk::DPart& p = hostParts[c];
-        if (!(p.nv >= 4 && (p.nt > 0 || (p.flags & 2u)))) continue;
-        dk::HullJob J;
-        J.part = L.partBase + c;
-        J.vOff = p.vOff;
-        J.nv = p.nv;
-        J.exportOff = dk::NONE;
-        J.slabOff = 0;
-        J.slabWords = 0;
-        J.job = L.jobBase + c / 2;
-        hj.push_back(J);
-      }
       dk::HullLevel HL;
       HL.prune = P.prune && nSeeds > 0;
       HL.itemsInBase = L.itemsInBase;
@@ -568,13 +602,12 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       HL.nSeeds = nSeeds;
       HL.depth = depth;
       HL.jobBase = L.jobBase;
-      runHulls(hj, parts_.p, nullptr, HL, nullptr);
+      levelHulls(L, HL);
       hipEvent_t e1 = tBegin();
       dk::k_record<<<(L.nJobs + 255) / 256, 256, 0, stream_>>>(P, items_.p, parts_.p);
       launched("k_record");
       tEnd(K_RECORD, e1);
     }
-    if (aborted) break;
     dk::EvalParams E;
     std::memset(&E, 0, sizeof(E));
     E.depth = depth;
@@ -622,6 +655,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
   R.npOverflow = c1.npOverflow != 0;
   R.pruned = c1.pruned;
   R.hullsDone = c1.hullsDone;
+  kst_.hullJobs += c1.hullsDone;
   for (const Level& L : treeLevels) R.jobsPerLevel.push_back(L.nJobs);
   R.gpuMs = nowMs() - t0;
   kst_.faceTests += c1.faceTests;
@@ -730,11 +764,9 @@ std::vector<HullResult> GpuBackend::hullMeshes(const std::vector<MeshRef>& meshe
     J.slabWords = 0;
     J.job = dk::NONE;
   }
-  dk::HullLevel HL;
-  std::memset(&HL, 0, sizeof(HL));
   std::vector<dk::HullResultD> res;
   clearWork();
-  runHulls(hj, nullptr, export_.p, HL, &res);
+  runHulls(hj, export_.p, &res);
   collectWork();
   std::vector<uint32_t> faces(exportTotal);
   hipCheck(hipMemcpyAsync(faces.data(), export_.p, exportTotal * 4, hipMemcpyDeviceToHost, stream_), "read hull faces");

@@ -1676,37 +1676,86 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
 // candidates per round: exported hulls (the same for both group sizes), tree candidates
 constexpr uint32_t KE = 32, KF_BIG = 64, KF_W = 16;
 
-// small parts: one wavefront per part, HW parts per block
+// ---- persistent hull kernels fed by work queues ----------------------------------------
+// A fixed grid of groups (wavefronts or workgroups) takes jobs from a queue's head until it is
+// empty. Queues are ordered by size class, largest first, so a group's first job is its largest:
+// the group takes one slab from the arena, sized for the top of that class, and keeps it for its
+// later jobs. A slab that does not fit the arena sets OVF_ARENA (the run is repeated with a
+// larger arena); the job then reports slab too small.
+
+// the largest point count of nv's size class [2^k, 2^(k+1))
+__host__ __device__ inline uint32_t hullClassCap(uint32_t nv) {
+  if (nv < 2) return nv;
+  uint32_t k = 31;
+  while (!(nv >> k)) k--;
+  return k >= 31 ? 0xffffffffu : (2u << k) - 1u;
+}
+__host__ __device__ inline uint32_t hullClassOf(uint32_t nv) {
+  uint32_t k = 0;
+  while (k < 31 && (nv >> (k + 1))) k++;
+  return k;
+}
+
+// a slab for a job of nv points: the group's current one when it is large enough, else a new one
+// from the arena; returns the slab's words (0 when the arena is spent)
+__device__ __forceinline__ uint32_t hull_slab(uint32_t nv, unsigned long long* off, uint32_t words,
+                                              HullArena* __restrict__ ar, Counters* __restrict__ ctr) {
+  if (hullSlabWords(nv) <= words) return words;
+  const uint64_t want = hullSlabWords(hullClassCap(nv));
+  const unsigned long long at = atomicAdd(&ar->top, (unsigned long long)want);
+  if (at + want > ar->cap || want > 0xffffffffull) {
+    atomicOr(&ctr->overflow, OVF_ARENA);
+    return 0;
+  }
+  *off = at;
+  return (uint32_t)want;
+}
+
+// small parts: one wavefront per part, HW wavefronts per block
 template <bool EXACT>
-__global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hjobs, uint32_t nJobsH,
-                                                  const float* __restrict__ verts, uint32_t* __restrict__ arena,
-                                                  DPart* __restrict__ parts, HullResultD* __restrict__ results,
-                                                  uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items,
-                                                  const DJob* __restrict__ jobs, HullLevel HL,
-                                                  Counters* __restrict__ ctr) {
+__global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hjobs, HullQueue* __restrict__ q,
+                                                  HullArena* __restrict__ ar, const float* __restrict__ verts,
+                                                  uint32_t* __restrict__ arena, DPart* __restrict__ parts,
+                                                  HullResultD* __restrict__ results, uint32_t* __restrict__ exportBuf,
+                                                  const DItem* __restrict__ items, const DJob* __restrict__ jobs,
+                                                  HullLevel HL, Counters* __restrict__ ctr) {
   const uint32_t wid = threadIdx.x >> 6;
-  const uint32_t hj = blockIdx.x * HW + wid;
-  if (hj >= nJobsH) return;
   constexpr uint32_t KW = EXACT ? KE : KF_W;
   __shared__ uint32_t s_seed[HW][80];
   __shared__ HullBatch<EXACT, KW, LCAP_W, HT_W, 1, NCAP_W> s_batch[HW];
   const WaveGrp g;
-  hull_job(g, s_batch[wid], hjobs[hj], hj, s_seed[wid], verts, arena, parts, results, exportBuf, items, jobs, HL,
-                 ctr, &ctr->faceTests, &ctr->hullBytes);
+  const uint32_t count = q->count;
+  unsigned long long off = 0;
+  uint32_t words = 0;
+  for (;;) {
+    uint32_t j = 0;
+    if (lane_id() == 0) {
+      j = atomicAdd(&q->head, 1u);
+      if (j < count) words = hull_slab(hjobs[j].nv, &off, words, ar, ctr);
+    }
+    j = rdlane(j, 0);
+    if (j >= count) break;
+    words = rdlane(words, 0);
+    off = ((unsigned long long)rdlane((uint32_t)(off >> 32), 0) << 32) | rdlane((uint32_t)off, 0);
+    HullJob J = hjobs[j];
+    J.slabOff = off;
+    J.slabWords = words;
+    hull_job(g, s_batch[wid], J, j, s_seed[wid], verts, arena, parts, results, exportBuf, items, jobs, HL, ctr,
+             &ctr->faceTests, &ctr->hullBytes);
+    wave_sync();
+  }
 }
 
 // large parts: one workgroup of HBIG threads per part. Rounds are latency-bound,
 // so two workgroups share a CU (4 waves per SIMD: at most 128 VGPRs; the LDS
 // batch is under half of the CU's 160 KB).
 template <bool EXACT>
-__global__ __launch_bounds__(HBIG, 4) void k_hull_big(const HullJob* __restrict__ hjobs, uint32_t nJobsH,
-                                                   const float* __restrict__ verts, uint32_t* __restrict__ arena,
-                                                   DPart* __restrict__ parts, HullResultD* __restrict__ results,
-                                                   uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items,
-                                                   const DJob* __restrict__ jobs, HullLevel HL,
-                                                   Counters* __restrict__ ctr) {
-  const uint32_t hj = blockIdx.x;
-  if (hj >= nJobsH) return;
+__global__ __launch_bounds__(HBIG, 4) void k_hull_big(const HullJob* __restrict__ hjobs, HullQueue* __restrict__ q,
+                                                   HullArena* __restrict__ ar, const float* __restrict__ verts,
+                                                   uint32_t* __restrict__ arena, DPart* __restrict__ parts,
+                                                   HullResultD* __restrict__ results, uint32_t* __restrict__ exportBuf,
+                                                   const DItem* __restrict__ items, const DJob* __restrict__ jobs,
+                                                   HullLevel HL, Counters* __restrict__ ctr) {
   constexpr uint32_t KB = EXACT ? KE : KF_BIG;
   __shared__ uint32_t s_seed[80];
   __shared__ uint32_t s_u[HBIG / 64];
@@ -1714,13 +1763,110 @@ __global__ __launch_bounds__(HBIG, 4) void k_hull_big(const HullJob* __restrict_
   __shared__ unsigned long long s_l[3 * (HBIG / 64)];
   __shared__ uint32_t s_m[4 * 16 * (HBIG / 64)];
   __shared__ HullBatch<EXACT, KB, LCAP_BIG, HT_BIG, HBIG / 64, NCAP_BIG> s_batch;
+  __shared__ uint32_t s_j, s_words;
+  __shared__ unsigned long long s_off;
   BlockGrp<HBIG> g;
   g.su = s_u;
   g.sf = s_f;
   g.sl = s_l;
   g.sm = s_m;
-  hull_job(g, s_batch, hjobs[hj], hj, s_seed, verts, arena, parts, results, exportBuf, items, jobs, HL, ctr,
-                 &ctr->faceTestsBig, &ctr->hullBytesBig);
+  const uint32_t count = q->count;
+  if (threadIdx.x == 0) {
+    s_off = 0;
+    s_words = 0;
+  }
+  for (;;) {
+    __syncthreads();  // the previous job is done with the shared state
+    if (threadIdx.x == 0) {
+      const uint32_t j = atomicAdd(&q->head, 1u);
+      s_j = j;
+      if (j < count) {
+        unsigned long long o = s_off;
+        s_words = hull_slab(hjobs[j].nv, &o, s_words, ar, ctr);
+        s_off = o;
+      }
+    }
+    __syncthreads();
+    const uint32_t j = s_j;
+    if (j >= count) break;
+    HullJob J = hjobs[j];
+    J.slabOff = s_off;
+    J.slabWords = s_words;
+    hull_job(g, s_batch, J, j, s_seed, verts, arena, parts, results, exportBuf, items, jobs, HL, ctr,
+             &ctr->faceTestsBig, &ctr->hullBytesBig);
+  }
+}
+
+// ---- hull queues of a tree level, built on the device ----------------------------------
+// Every child part with at least four points (and triangles, or no topology at the last
+// level) goes to the workgroup queue (at least `split` points) or the wavefront queue, by
+// size class, largest class first; within a class in any order (a hull does not depend on
+// which group computes it or when).
+struct HullQueues {
+  uint32_t bucket[2][32];  // jobs per size class: [0] wavefront queue, [1] workgroup queue
+  HullQueue q[2];
+};
+
+__global__ void k_hq_reset(HullQueues* __restrict__ hq) {
+  const uint32_t t = threadIdx.x;
+  if (t < 64) hq->bucket[t >> 5][t & 31] = 0;
+}
+
+// per child: its queue and size class, and its rank within them
+__global__ __launch_bounds__(256) void k_hq_count(const DPart* __restrict__ parts, uint32_t partBase, uint32_t nChild,
+                                                  uint32_t split, HullQueues* __restrict__ hq,
+                                                  uint32_t* __restrict__ tag, uint32_t* __restrict__ rank) {
+  __shared__ uint32_t s_cnt[64], s_base[64];
+  const uint32_t t = threadIdx.x, c = blockIdx.x * 256 + t;
+  if (t < 64) s_cnt[t] = 0;
+  __syncthreads();
+  uint32_t tg = NONE, local = 0;
+  if (c < nChild) {
+    const DPart p = parts[partBase + c];
+    if (p.nv >= 4 && (p.nt > 0 || (p.flags & 2u))) {
+      tg = (p.nv >= split ? 32u : 0u) + hullClassOf(p.nv);
+      local = atomicAdd(&s_cnt[tg], 1u);
+    }
+  }
+  __syncthreads();
+  if (t < 64) s_base[t] = s_cnt[t] ? atomicAdd(&hq->bucket[t >> 5][t & 31], s_cnt[t]) : 0u;
+  __syncthreads();
+  if (c < nChild) {
+    tag[c] = tg;
+    rank[c] = tg == NONE ? 0u : s_base[tg] + local;
+  }
+}
+
+// per queued child: its job record at (jobs of larger classes) + rank; the queue heads and the arena
+__global__ __launch_bounds__(256) void k_hq_place(const DPart* __restrict__ parts, uint32_t partBase, uint32_t jobBase,
+                                                  uint32_t nChild, HullQueues* __restrict__ hq,
+                                                  const uint32_t* __restrict__ tag, const uint32_t* __restrict__ rank,
+                                                  HullJob* __restrict__ qSmall, HullJob* __restrict__ qBig,
+                                                  HullArena* __restrict__ ar) {
+  const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x < 2) {
+    uint32_t n = 0;
+    for (int k = 0; k < 32; k++) n += hq->bucket[threadIdx.x][k];
+    hq->q[threadIdx.x].count = n;
+    hq->q[threadIdx.x].head = 0;
+    if (threadIdx.x == 0) ar->top = 0;
+  }
+  if (c >= nChild) return;
+  const uint32_t tg = tag[c];
+  if (tg == NONE) return;
+  const uint32_t big = tg >> 5, cls = tg & 31;
+  uint32_t at = rank[c];
+  for (uint32_t k = cls + 1; k < 32; k++) at += hq->bucket[big][k];
+  const DPart p = parts[partBase + c];
+  HullJob J;
+  J.part = partBase + c;
+  J.vOff = p.vOff;
+  J.nv = p.nv;
+  J.exportOff = NONE;
+  J.slabOff = 0;
+  J.slabWords = 0;
+  J.job = jobBase + c / 2;
+  (big ? qBig : qSmall)[at] = J;
 }
 
 }  // namespace dk

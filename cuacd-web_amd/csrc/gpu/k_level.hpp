@@ -110,7 +110,8 @@ __global__ __launch_bounds__(LT) void k_lvl_items(LevelParams P, DItem* __restri
   __shared__ uint32_t s_ok[LT];
   __shared__ uint32_t s_base, s_live;
   const DItem it = items[P.itemsInBase + item];
-  bool live = it.flags == 1 && it.np > 0;
+  // a pool already ran out in this run: it will be run again, so no further cuts
+  bool live = it.flags == 1 && it.np > 0 && ctr->overflow == 0;
   const uint32_t partId = live ? it.partIds[it.np - 1] : 0;
   DPart part;
   part.nv = part.nt = part.ne = 0;

@@ -26,7 +26,7 @@ struct TreeConfig {
   uint64_t triCap = 512ull << 20;        // u32
   uint64_t edgeCap = 16ull << 20;        // edges (8 words each)
   uint64_t scratchWords = 256ull << 20;  // plan/emit side flags, 2 words per parent vertex per job
-  uint64_t hullBudgetBytes = 4ull << 30;  // in-flight hull slabs
+  uint64_t hullBudgetBytes = 16ull << 30;  // hull slab arena (one slab per hull group in flight)
   uint32_t hullMaxVerts = 4096;
   bool pruneLastLevel = true;
   // Hausdorff samples per mesh: 0 = the reference rule max(1000, floor(2000 * area)),
@@ -101,7 +101,7 @@ struct PartView {
 struct TreeRunResult {
   std::vector<SeedResult> seeds;
   bool overflow = false;
-  uint32_t overflowPools = 0;  // which pools ran out: 1 scratch, 2 vertices, 4 triangles, 8 edges, 16 tetrahedra
+  uint32_t overflowPools = 0;  // which pools ran out: 1 scratch, 2 vertices, 4 triangles, 8 edges, 32 hull arena
   bool hullFail = false;
   uint32_t hullFailCount = 0;
   bool npOverflow = false;
@@ -178,7 +178,7 @@ class Backend {
  public:
   virtual KernelStats kernelStats() const { return KernelStats(); }
   // Doubles the tree pools named by overflow bits (1 scratch, 2 vertices, 4 triangles,
-  // 8 edges, 16 tetrahedra, which are sized from the triangle and edge pools) up to
+  // 8 edges, 32 hull slab arena) up to
   // their element limits; false when none of them could grow. A backend without
   // pools never overflows and grows nothing.
   virtual bool growPools(uint32_t overflowBits) {

@@ -671,13 +671,21 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
         std::fprintf(stderr, "[iter %u] tree run: %zu seeds, hulls %llu, pruned %u, overflow pools %u, %.1f ms\n", iter,
                      refs.size(), (unsigned long long)res.hullsDone, res.pruned, res.overflowPools, nowMs() - tt0);
       if (res.overflow) {
+        // pools the caller left at their defaults may grow (explicit sizes are limits)
+        const uint32_t growable = (o.scratchWords ? 0u : 1u) | (o.vertexCap ? 0u : 2u) | (o.triCap ? 0u : 4u) |
+                                  (o.edgeCap ? 0u : 8u) | (o.hullBudgetBytes ? 0u : 32u);
+        // the hull arena holds one slab per hull group in flight, whatever the seed count: grow it
+        // rather than split the seeds
+        if ((res.overflowPools & 32u) && (growable & 32u) && poolGrowths_ < 8 && be_.growPools(32u)) {
+          poolGrowths_++;
+          if (o.verbose) std::fprintf(stderr, "[iter %u] hull arena grown (%u)\n", iter, poolGrowths_);
+          continue;
+        }
         if (batchN == 1) {
-          // one part alone does not fit: double the pools that ran out and run it again,
-          // when the caller left them at their defaults (explicit sizes are limits)
-          const uint32_t growable = (o.scratchWords ? 0u : 1u) | (o.vertexCap ? 0u : 2u) | (o.triCap ? 0u : 4u | 16u) |
-                                    (o.edgeCap ? 0u : 8u);
-          if ((res.overflowPools & ~growable) == 0 && poolGrowths_ < 4 && be_.growPools(res.overflowPools)) {
+          // one part alone does not fit: double the pools that ran out and run it again
+          if ((res.overflowPools & ~growable) == 0 && poolGrowths_ < 8 && be_.growPools(res.overflowPools)) {
             poolGrowths_++;
+            treeBatch_ = 0;  // the grown pools may hold every seed again
             if (o.verbose)
               std::fprintf(stderr, "[iter %u] pools %u grown for a single part (%u)\n", iter, res.overflowPools,
                            poolGrowths_);

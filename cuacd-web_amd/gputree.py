@@ -22,7 +22,7 @@ class TreeConfig:
     tri_cap: int = 512 << 20
     edge_cap: int = 16 << 20
     scratch_words: int = 256 << 20
-    hull_budget_bytes: int = 4 << 30
+    hull_budget_bytes: int = 16 << 30
     hull_max_verts: int = 4096
     prune_last_level: bool = True
     hausdorff_samples: int = 0
@@ -73,7 +73,7 @@ class TreeRunResult:
     hulls_done: int
     gpu_ms: float
     levels: list = field(default_factory=list)
-    overflow_pools: int = 0  # bits: 1 scratch, 2 vertices, 4 triangles, 8 edges, 16 tetrahedra
+    overflow_pools: int = 0  # bits: 1 scratch, 2 vertices, 4 triangles, 8 edges, 32 hull arena
 
 
 class GpuTree:

@@ -75,7 +75,8 @@ typedef struct cuacd_seed_result {
 /* src/gpu/tree.ts:75-85 TreeRunResult flags and stats */
 typedef struct cuacd_run_stats {
   /* overflow: 0, or the pools that ran out (1 scratch, 2 vertices, 4 triangles,
-     8 edges, 16 tetrahedra); nonzero means the run must be repeated with fewer seeds */
+     8 edges, 32 hull slab arena); nonzero means the run must be repeated with fewer seeds
+     or larger pools */
   int32_t overflow, hull_fail, np_overflow;
   uint32_t hull_fail_count, pruned;
   uint32_t jobs_per_level[4];
