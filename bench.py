@@ -294,6 +294,7 @@ def main():
         "step_breakdown_ms": {k: round(st[k], 1) for k in ("tree_ms", "hull_ms", "hausdorff_ms", "cut_ms",
                                                              "concave_ms", "merge_ms", "total_ms")},
         "roofline": roofline(kst, workload),
+        "work_counters_per_step": {n: kst[n] // a.steps for n in COUNTERS},
         "cpu_baseline": None,
     }
     if a.plain:

@@ -225,6 +225,7 @@ class GpuBackend : public Backend {
   // last run
   std::vector<uint32_t> lastLevelCounts_;
   uint32_t lastPartCount_ = 0;
+  uint32_t maxSeedNv_ = 0;  // the largest seed of the current run (bounds the hull slabs)
 };
 
 std::vector<Level> GpuBackend::layout(uint32_t nSeeds, uint32_t depth, uint32_t w0, uint32_t ne0) const {
@@ -444,7 +445,19 @@ void GpuBackend::levelHulls(const Level& L, const dk::HullLevel& HL) {
   dk::k_hq_place<<<blocks, 256, 0, stream_>>>(parts_.p, L.partBase, L.jobBase, nChild, hq_.p, hqTag_.p, hqRank_.p,
                                               hqSmall_.p, hqBig_.p, arenaCtl_.p);
   launched("k_hq_place");
-  launchHulls(false, parts_.p, nullptr, HL, std::min(HULL_GRID_BIG, nChild), std::min(HULL_GRID_SMALL_WAVES, nChild));
+  // Grids whose slabs fit the arena together. A child has about as many points as its parent at
+  // most, so the size class of the largest seed bounds every group's slab; a part above it still
+  // gets a slab while the arena lasts (and the run reports the arena's overflow if it does not).
+  const uint32_t split = cfg_.hullWgMinVerts;
+  const uint64_t slabSmall = dk::hullSlabWords(dk::hullClassCap(std::min(split > 0 ? split - 1 : 0, maxSeedNv_)));
+  const uint64_t slabBig = dk::hullSlabWords(dk::hullClassCap(std::max(maxSeedNv_, split)));
+  uint64_t gSmall = std::min<uint64_t>(HULL_GRID_SMALL_WAVES, nChild);
+  uint64_t gBig = std::min<uint64_t>(HULL_GRID_BIG, nChild);
+  while (gBig * slabBig + gSmall * slabSmall > arena_.n && (gBig > 1 || gSmall > 1)) {
+    if (gBig * slabBig >= gSmall * slabSmall && gBig > 1) gBig = (gBig + 1) / 2;
+    else if (gSmall > 1) gSmall = (gSmall + 1) / 2;
+  }
+  launchHulls(false, parts_.p, nullptr, HL, (uint32_t)gBig, (uint32_t)gSmall);
 }
 
 void GpuBackend::runHulls(std::vector<dk::HullJob>& hj, uint32_t* exportBuf, std::vector<dk::HullResultD>* results) {
@@ -520,6 +533,8 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
   const bool refine = opts.refine && opts.refine->rounds > 0 && opts.refine->probes >= 3;
   const uint32_t nProbes = refine ? std::min(opts.refine->probes, edgeStride_) : 0;
   const std::vector<Level> treeLevels = layout(nSeeds, depth, width0, nEdge0);
+  maxSeedNv_ = 0;
+  for (const MeshRef& m : seeds) maxSeedNv_ = std::max(maxSeedNv_, m.nv);
   const std::vector<Level> probeLevels = refine ? layout(nSeeds, depth, 0, nProbes) : std::vector<Level>();
   size_t totalJobs = 0, probeJobs = 0;
   for (const Level& L : treeLevels) totalJobs += L.nJobs;
@@ -883,7 +898,11 @@ static void hdLaunch(hipStream_t s, const HdPlan& P, const HdDev& D, Timer&& tim
   if (!P.tiles.empty()) {
     up(D.tiles, P.tiles.data(), P.tiles.size() * sizeof(dk::HdTile), "upload tiles");
     timed(K_HD_DIST, true);
-    dk::k_hd_dist<<<(uint32_t)P.tiles.size(), dk::HDT, 0, s>>>(D.tiles, (uint32_t)P.tiles.size(), D.samples, D.rec,
+    const uint32_t blocks = ((uint32_t)P.tiles.size() + dk::HDT / 64 - 1) / (dk::HDT / 64);
+    dk::k_hd_probe<<<blocks, 256, 0, s>>>(D.tiles, (uint32_t)P.tiles.size(), D.samples, D.rec, D.chunks, D.bits,
+                                          D.evals);
+    hipCheck(hipGetLastError(), "k_hd_probe");
+    dk::k_hd_dist<<<blocks, dk::HDT, 0, s>>>(D.tiles, (uint32_t)P.tiles.size(), D.samples, D.rec,
                                                                D.chunks, D.bits, D.evals);
     hipCheck(hipGetLastError(), "k_hd_dist");
     timed(K_HD_DIST, false);

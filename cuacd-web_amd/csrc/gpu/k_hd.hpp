@@ -3,11 +3,14 @@
 // Samples: area-proportional counts per triangle (computed once on the host),
 // Wang-hash barycentrics (src/gpu/hausdorff_shader.ts:16-23, :88-99).
 // Distances: per-triangle quantities are computed once (k_hd_prep), chunks of
-// 64 triangles get a bounding box (k_hd_chunks), and a wave measures one
-// sample at a time against a chunk's 64 triangles (one per lane), visiting
-// only the chunks that could hold the sample's nearest triangle; the directed
-// maximum of the per-sample minimum squared distance goes to an atomicMax on
-// the pair (non-negative f32 bits order like their values).
+// 64 triangles get a bounding box (k_hd_chunks). A probe pass measures one
+// sample per tile exactly (k_hd_probe: a wave per sample, a triangle per lane);
+// then a wave takes a tile of 64 samples, one per lane, stages each chunk it
+// cannot rule out in LDS and evaluates its 64 triangles for every lane
+// (k_hd_dist). Lanes whose best distance cannot raise the pair's running
+// maximum stop early. The directed maximum of the per-sample minimum squared
+// distance goes to an atomicMax on the pair (non-negative f32 bits order like
+// their values), so the result is exact and independent of the order.
 #pragma once
 #include "dnum.hpp"
 #include "dtypes.hpp"
@@ -82,6 +85,10 @@ __device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, f
   return t + az * bz;
 }
 __device__ __forceinline__ float fminx(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ unsigned long long wave_sum_u64_hd(unsigned long long v) {
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
 
 // ---- per-triangle quantities, evaluated once with exactly the expressions of
 // the per-pair distance (the oracle evaluates them per pair), so the distance
@@ -284,102 +291,223 @@ __device__ __forceinline__ bool hd_chunk_far(float px, float py, float pz, const
   return lb > 0.0 && lb * lb > (double)best;
 }
 
-constexpr int HDT = 256;  // threads per block
-constexpr int HDS = 4;    // samples per tile, one per wave: many waves in flight hide the record loads
-// Each wave takes HDS / 4 samples of its tile one at a time. For a sample, the 64
-// lanes hold the 64 triangles of a chunk (each lane reads its own distance
-// record); the sample's best distance is the wave minimum. The chunk whose
-// box is nearest the sample is taken first, then every chunk that the box
-// test cannot rule out against the best so far (re-tested after each chunk).
+// The exact nearest squared distance from one sample to the triangles of a mesh, one wavefront
+// per sample (the 64 lanes hold a chunk's 64 triangles): the chunk whose box is nearest first,
+// then every chunk the point-to-box test cannot rule out against the best so far.
+__device__ float hd_sample_min(float px, float py, float pz, const float4* __restrict__ C4,
+                               const float4* __restrict__ R4, uint32_t nt, unsigned long long* ev) {
+  const uint32_t lane = lane_id();
+  const uint32_t nch = (nt + HDC - 1) / HDC;
+  auto chunk_min = [&](uint32_t c) {
+    const uint32_t t = c * HDC + lane;
+    float d = INFINITY;
+    if (t < nt) {
+      float T[32];
+      const float4* r = R4 + 8 * (size_t)t;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const float4 v = r[q];
+        T[4 * q] = v.x;
+        T[4 * q + 1] = v.y;
+        T[4 * q + 2] = v.z;
+        T[4 * q + 3] = v.w;
+      }
+      d = tri_sq_pre(px, py, pz, T);
+      *ev += 1;
+    }
+    return wave_ext_f<false>(d);
+  };
+  float dmin = INFINITY;
+  uint32_t cmin = 0xffffffffu;
+  for (uint32_t c = lane; c < nch; c += 64) {
+    const float4 lo = C4[2 * c], hi = C4[2 * c + 1];
+    const float dx = fmaxf(0.f, fmaxf(lo.x - px, px - hi.x)), dy = fmaxf(0.f, fmaxf(lo.y - py, py - hi.y));
+    const float dz = fmaxf(0.f, fmaxf(lo.z - pz, pz - hi.z));
+    const float d = dx * dx + dy * dy + dz * dz;
+    if (d < dmin) {
+      dmin = d;
+      cmin = c;
+    }
+  }
+  wave_argext<false>(dmin, cmin);
+  const uint32_t c0 = cmin < nch ? cmin : 0u;
+  float best = chunk_min(c0);
+  for (uint32_t g0 = 0; g0 < nch; g0 += 64) {
+    const uint32_t c = g0 + lane;
+    uint64_t need = __ballot(c < nch && c != c0 && !hd_chunk_far(px, py, pz, C4[2 * c], C4[2 * c + 1], best));
+    while (need) {
+      const uint32_t k = (uint32_t)__ffsll((unsigned long long)need) - 1;
+      need &= need - 1;
+      const float d = chunk_min(g0 + k);
+      if (d < best) {
+        best = d;
+        need &= __ballot(c < nch && !hd_chunk_far(px, py, pz, C4[2 * c], C4[2 * c + 1], best));
+      }
+    }
+  }
+  return best;
+}
+
+// A first pass over one sample of every tile (its first): their exact distances raise each
+// pair's running maximum before the tiles run, so that most lanes of the tiles close after
+// their first chunk.
+__global__ __launch_bounds__(256) void k_hd_probe(const HdTile* __restrict__ tiles, uint32_t nTiles,
+                                                  const float* __restrict__ samples, const float4* __restrict__ rec,
+                                                  const float4* __restrict__ chunks, uint32_t* __restrict__ result,
+                                                  unsigned long long* evals) {
+  const uint32_t b = (blockIdx.x * 256 + threadIdx.x) >> 6;
+  if (b >= nTiles) return;  // whole wavefronts
+  const HdTile Tl = tiles[b];
+  const float* sp = samples + 3 * (size_t)Tl.sBegin;
+  unsigned long long ev = 0;
+  const float d = hd_sample_min(sp[0], sp[1], sp[2], chunks + 2 * (size_t)Tl.chOff, rec + 8 * (size_t)Tl.gTri,
+                                Tl.nt, &ev);
+  const unsigned long long evs = wave_sum_u64_hd(ev);
+  if (lane_id() == 0) {
+    atomicMax(result + Tl.pair, __float_as_uint(d));
+    atomicAdd(evals, evs);
+  }
+}
+
+// the box-to-box form of the same test: no point of the samples' box [slo, shi] can have a
+// triangle of the chunk nearer than sqrt(best): the boxes' distance is below every point's distance
+// to the chunk box, and their farthest corner distance above every point's far distance
+__device__ __forceinline__ bool hd_chunk_far_box(const float* slo, const float* shi, const float4& lo, const float4& hi,
+                                                 float best) {
+  const double nx = fmax(0.0, fmax((double)lo.x - (double)shi[0], (double)slo[0] - (double)hi.x));
+  const double ny = fmax(0.0, fmax((double)lo.y - (double)shi[1], (double)slo[1] - (double)hi.y));
+  const double nz = fmax(0.0, fmax((double)lo.z - (double)shi[2], (double)slo[2] - (double)hi.z));
+  const double fx = fmax(fabs((double)shi[0] - (double)lo.x), fabs((double)hi.x - (double)slo[0]));
+  const double fy = fmax(fabs((double)shi[1] - (double)lo.y), fabs((double)hi.y - (double)slo[1]));
+  const double fz = fmax(fabs((double)shi[2] - (double)lo.z), fabs((double)hi.z - (double)slo[2]));
+  const double lb = sqrt(nx * nx + ny * ny + nz * nz) - 1e-4 * (double)lo.w * sqrt(fx * fx + fy * fy + fz * fz);
+  return lb > 0.0 && lb * lb > (double)best;
+}
+
+// The point-to-box test of hd_chunk_far in f32, never less conservative: the near distance is
+// shrunk by 1e-6 of itself and the margin enlarged by 1% (f32 rounding of the near and far
+// distances is below 1e-6 of them, the margin at least 2e-4 of the near distance), so it rules
+// a chunk out only when hd_chunk_far does.
+__device__ __forceinline__ bool hd_chunk_far_f32(float px, float py, float pz, const float4& lo, const float4& hi,
+                                                 float best) {
+  const float nx = fmaxf(0.f, fmaxf(lo.x - px, px - hi.x)), ny = fmaxf(0.f, fmaxf(lo.y - py, py - hi.y));
+  const float nz = fmaxf(0.f, fmaxf(lo.z - pz, pz - hi.z));
+  const float fx = fmaxf(fabsf(px - lo.x), fabsf(px - hi.x)), fy = fmaxf(fabsf(py - lo.y), fabsf(py - hi.y));
+  const float fz = fmaxf(fabsf(pz - lo.z), fabsf(pz - hi.z));
+  const float near = sqrtf(nx * nx + ny * ny + nz * nz) * (1.0f - 1e-6f);
+  const float lb = near - 1.01e-4f * lo.w * sqrtf(fx * fx + fy * fy + fz * fz);
+  return lb > 0.0f && lb * lb > best;
+}
+
+constexpr int HDT = 256;  // threads per block: four tiles, one per wavefront
+constexpr int HDS = 64;   // samples per tile, one per lane
+// A wavefront measures its 64 samples (one per lane) against the other mesh chunk by chunk: a
+// chunk's 64 distance records are staged in the wave's LDS tile once and every lane evaluates all
+// of them (the same record for every lane: broadcast reads). The chunk nearest the samples' box is
+// taken first; after it, a chunk is staged only when the box-to-box test cannot rule it out
+// against the largest best distance of the lanes still open. A lane closes when its best distance
+// is at most the pair's running maximum (read from the result word): it cannot raise the maximum,
+// so the maximum, taken over exact per-sample minima of the lanes that stay open, is exact.
 __global__ __launch_bounds__(HDT) void k_hd_dist(const HdTile* __restrict__ tiles, uint32_t nTiles,
                                                  const float* __restrict__ samples, const float4* __restrict__ rec,
                                                  const float4* __restrict__ chunks, uint32_t* __restrict__ result,
                                                  unsigned long long* evals) {
-  const uint32_t b = blockIdx.x;
-  if (b >= nTiles) return;
+  __shared__ float4 s_rec[HDT / 64][HDC * 8];  // 8 KB per wave
+  const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t b = blockIdx.x * (HDT / 64) + w;
+  if (b >= nTiles) return;  // whole wavefronts; nothing below synchronizes the block
   const HdTile Tl = tiles[b];
-  __shared__ float s_red[HDT / 64];
-  __shared__ unsigned long long s_ev[HDT / 64];
-  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  float4* L = s_rec[w];
+  const bool act = lane < Tl.sCount;
+  float px = 0.f, py = 0.f, pz = 0.f;
+  if (act) {
+    const float* sp = samples + 3 * ((size_t)Tl.sBegin + lane);
+    px = sp[0];
+    py = sp[1];
+    pz = sp[2];
+  }
+  float slo[3] = {act ? px : INFINITY, act ? py : INFINITY, act ? pz : INFINITY};
+  float shi[3] = {act ? px : -INFINITY, act ? py : -INFINITY, act ? pz : -INFINITY};
+  for (int k = 0; k < 3; k++) {
+    slo[k] = wave_ext_f<false>(slo[k]);
+    shi[k] = wave_ext_f<true>(shi[k]);
+  }
   const uint32_t nch = (Tl.nt + HDC - 1) / HDC;
   const float4* C4 = chunks + 2 * (size_t)Tl.chOff;
   const float4* R4 = rec + 8 * (size_t)Tl.gTri;
-  float worst = 0.0f;
+  float worst = __uint_as_float(__hip_atomic_load(result + Tl.pair, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  float best = act ? INFINITY : 0.0f;
+  bool open = act;
   unsigned long long ev = 0;
-  constexpr uint32_t SPW = HDS / (HDT / 64);
-  const uint32_t s1 = (w + 1) * SPW < Tl.sCount ? (w + 1) * SPW : Tl.sCount;
-  for (uint32_t si = w * SPW; si < s1; si++) {
-    const float* sp = samples + 3 * ((size_t)Tl.sBegin + si);
-    const float px = sp[0], py = sp[1], pz = sp[2];
-    // squared distance from the sample to the box of chunk c (f32: ordering only)
-    auto boxd = [&](uint32_t c) {
-      const float4 lo = C4[2 * c], hi = C4[2 * c + 1];
-      const float dx = fmaxf(0.f, fmaxf(lo.x - px, px - hi.x)), dy = fmaxf(0.f, fmaxf(lo.y - py, py - hi.y));
-      const float dz = fmaxf(0.f, fmaxf(lo.z - pz, pz - hi.z));
-      return dx * dx + dy * dy + dz * dz;
-    };
-    // the distances from the sample to the triangles of chunk c, one per lane
-    auto chunk_min = [&](uint32_t c) {
-      const uint32_t t = c * HDC + lane;
-      float d = INFINITY;
-      if (t < Tl.nt) {
-        float T[32];
-        const float4* r = R4 + 8 * (size_t)t;
+  // stage chunk c and take its triangles' distances into every open lane's best
+  auto eval_chunk = [&](uint32_t c) {
+    const uint32_t t0 = c * HDC;
+    const uint32_t n = Tl.nt - t0 < (uint32_t)HDC ? Tl.nt - t0 : (uint32_t)HDC;
+    wave_sync();  // every lane is done reading the previous chunk
+    if (lane < n) {
+      const float4* r = R4 + 8 * (size_t)(t0 + lane);
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const float4 v = r[q];
-          T[4 * q] = v.x;
-          T[4 * q + 1] = v.y;
-          T[4 * q + 2] = v.z;
-          T[4 * q + 3] = v.w;
-        }
-        d = tri_sq_pre(px, py, pz, T);
-      }
-      ev += t < Tl.nt ? 1u : 0u;
-      return d;
-    };
-    float dmin = INFINITY;
-    uint32_t cmin = 0xffffffffu;
-    for (uint32_t c = lane; c < nch; c += 64) {
-      const float d = boxd(c);
-      if (d < dmin) {
-        dmin = d;
-        cmin = c;
-      }
+      for (int q = 0; q < 8; q++) L[8 * lane + q] = r[q];
     }
-    wave_argext<false>(dmin, cmin);
-    const uint32_t c0 = cmin < nch ? cmin : 0u;
-    float best = wave_ext_f<false>(chunk_min(c0));
-    for (uint32_t g0 = 0; g0 < nch; g0 += 64) {
-      const uint32_t c = g0 + lane;
-      uint64_t need = __ballot(c < nch && c != c0 && !hd_chunk_far(px, py, pz, C4[2 * c], C4[2 * c + 1], best));
-      while (need) {
-        const uint32_t k = (uint32_t)__ffsll((unsigned long long)need) - 1;
-        need &= need - 1;
-        const float d = wave_ext_f<false>(chunk_min(g0 + k));
-        if (d < best) {
-          best = d;
-          // a nearer triangle: drop the chunks it now rules out
-          need &= __ballot(c < nch && !hd_chunk_far(px, py, pz, C4[2 * c], C4[2 * c + 1], best));
-        }
+    wave_sync();
+    if (open) {
+      for (uint32_t k = 0; k < n; k++) {
+        const float d = tri_sq_pre(px, py, pz, reinterpret_cast<const float*>(L + 8 * k));
+        best = d < best ? d : best;
       }
+      ev += n;
     }
-    worst = best > worst ? best : worst;
+  };
+  // the largest best distance of the open lanes (0 when every lane is closed)
+  auto open_max = [&]() { return wave_ext_f<true>(open ? best : 0.0f); };
+  // the chunk nearest the samples' box first
+  float dmin = INFINITY;
+  uint32_t cmin = 0xffffffffu;
+  for (uint32_t c = lane; c < nch; c += 64) {
+    const float4 lo = C4[2 * c], hi = C4[2 * c + 1];
+    const float dx = fmaxf(0.f, fmaxf(lo.x - shi[0], slo[0] - hi.x));
+    const float dy = fmaxf(0.f, fmaxf(lo.y - shi[1], slo[1] - hi.y));
+    const float dz = fmaxf(0.f, fmaxf(lo.z - shi[2], slo[2] - hi.z));
+    const float d = dx * dx + dy * dy + dz * dz;
+    if (d < dmin) {
+      dmin = d;
+      cmin = c;
+    }
   }
+  wave_argext<false>(dmin, cmin);
+  const uint32_t c0 = cmin < nch ? cmin : 0u;
+  if (nch) {
+    eval_chunk(c0);
+    open = open && best > worst;
+  }
+  float omax = open_max();
+  for (uint32_t g0 = 0; g0 < nch && omax > 0.0f; g0 += 64) {
+    worst = __uint_as_float(__hip_atomic_load(result + Tl.pair, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    open = open && best > worst;
+    omax = open_max();
+    const uint32_t c = g0 + lane;
+    const float4 clo = c < nch ? C4[2 * c] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 chi = c < nch ? C4[2 * c + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // candidates: chunks the samples' box cannot rule out; each is staged only when some open lane's
+    // own point cannot rule it out either
+    uint64_t cand = __ballot(c < nch && c != c0 && !hd_chunk_far_box(slo, shi, clo, chi, omax));
+    while (cand && omax > 0.0f) {
+      const uint32_t k = (uint32_t)__ffsll((unsigned long long)cand) - 1;
+      cand &= cand - 1;
+      const float4 lo = make_float4(__shfl(clo.x, k, 64), __shfl(clo.y, k, 64), __shfl(clo.z, k, 64),
+                                    __shfl(clo.w, k, 64));
+      const float4 hi = make_float4(__shfl(chi.x, k, 64), __shfl(chi.y, k, 64), __shfl(chi.z, k, 64), 0.0f);
+      if (__ballot(open && !hd_chunk_far_f32(px, py, pz, lo, hi, best)) == 0) continue;
+      eval_chunk(g0 + k);
+      open = open && best > worst;
+      omax = open_max();
+    }
+  }
+  const float tileMax = wave_ext_f<true>(act ? best : 0.0f);
+  const unsigned long long evs = wave_sum_u64_hd(ev);
   if (lane == 0) {
-    s_red[w] = worst;
-    s_ev[w] = ev;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float wv = s_red[0];
-    unsigned long long e = s_ev[0];
-    for (int k = 1; k < HDT / 64; k++) {
-      wv = s_red[k] > wv ? s_red[k] : wv;
-      e += s_ev[k];
-    }
-    atomicMax(result + Tl.pair, __float_as_uint(wv));
-    atomicAdd(evals, e);
+    atomicMax(result + Tl.pair, __float_as_uint(tileMax));
+    atomicAdd(evals, evs);
   }
 }
 

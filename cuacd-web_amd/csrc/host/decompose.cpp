@@ -691,7 +691,8 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
                            poolGrowths_);
             continue;
           }
-          throw std::runtime_error("GPU pool overflow with a single part; increase tree pool sizes");
+          throw std::runtime_error("GPU pool overflow with a single part (pools " + std::to_string(res.overflowPools) +
+                                   ": 1 scratch, 2 vertices, 4 triangles, 8 edges, 32 hull arena); increase tree pool sizes");
         }
         batchN = std::max<size_t>(1, batchN >> 1);
         treeBatch_ = batchN;
