@@ -110,6 +110,7 @@ TREE_API = {
     "last_faces": (P(u32), [C.c_void_p, u32, P(u32)]),
     "hausdorff": (C.c_int, [C.c_void_p, P(MeshBatch), P(MeshBatch), P(f64)]),
     "sample_mesh": (C.c_int, [C.c_void_p, P(f32), u32, P(u32), u32, P(f32), u64, P(u64)]),
+    "concave_edge_planes_batch": (C.c_int, [C.c_void_p, P(MeshBatch), u32, f64, f64, P(u32), P(f32), P(u32)]),
     "rv_cost": (f64, [f64, f64]),
     "decompose_options_default": (None, [P(DecomposeOptions)]),
     "decompose": (C.c_int, [P(f32), u32, P(u32), u32, P(DecomposeOptions), P(C.c_void_p)]),

@@ -17,6 +17,7 @@
 #include "../host/sampling.hpp"
 #include "k_hd.hpp"
 #include "k_hull.hpp"
+#include "k_concave.hpp"
 #include "k_level.hpp"
 
 namespace cuacd {
@@ -95,6 +96,8 @@ class GpuBackend : public Backend {
   std::vector<double> hausdorff(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b) override;
   std::vector<HullResult> hullsWithHausdorff(const std::vector<MeshRef>& meshes, std::vector<double>* hd) override;
   std::vector<float> sampleMesh(const MeshRef& m) override;
+  bool concavePlanes(const std::vector<MeshRef>& meshes, const ConcaveOptions& o, const std::vector<uint32_t>& seeds,
+                     std::vector<std::vector<float>>* out) override;
   std::vector<ItemView> levelItems(uint32_t level) override;
   bool readPart(uint32_t partIdx, PartView* pv, Mesh* mesh) override;
 
@@ -208,6 +211,12 @@ class GpuBackend : public Backend {
   DevBuf<dk::HullJob> hqSmall_, hqBig_;
   DevBuf<dk::HullResultD> hresSmall_, hresBig_;
   DevBuf<uint32_t> hqTag_, hqRank_;
+  // concave-edge planes: the batch, its edge CSR, concave edges, reservoirs, planes
+  DevBuf<dk::CcMesh> ccMeshes_;
+  DevBuf<float> ccVerts_;
+  DevBuf<uint32_t> ccTris_, ccDeg_, ccOff_, ccCursor_, ccKey_, ccDst_, ccCnt_, ccPos_, ccNp_;
+  DevBuf<uint4> ccEdges_, ccResv_;
+  DevBuf<float4> ccPlanes_;
   DevBuf<uint32_t> export_;
   DevBuf<float> hdVerts_, hdSamples_;
   DevBuf<uint32_t> hdTris_, hdCounts_;
@@ -944,6 +953,120 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
   hipCheck(hipMemcpyAsync(bits.data(), ctr_.p->hd, n * 4, hipMemcpyDeviceToHost, stream_), "read hd results");
   collectWork();
   return hdDistances(bits);
+}
+
+bool GpuBackend::concavePlanes(const std::vector<MeshRef>& meshes, const ConcaveOptions& o,
+                               const std::vector<uint32_t>& seeds, std::vector<std::vector<float>>* out) {
+  out->assign(meshes.size(), std::vector<float>());
+  if (meshes.empty() || o.nConcaveEdges == 0) return true;
+  std::vector<dk::CcMesh> cm(meshes.size());
+  std::vector<float> pv;
+  std::vector<uint32_t> pt;
+  uint64_t nVert = 0, nTri = 0;
+  for (size_t k = 0; k < meshes.size(); k++) {
+    dk::CcMesh& M = cm[k];
+    M.vOff = (uint32_t)pv.size();
+    M.tOff = (uint32_t)pt.size();
+    M.nv = meshes[k].nv;
+    M.nt = meshes[k].nt;
+    M.gVert = (uint32_t)nVert;
+    M.gTri = (uint32_t)nTri;
+    M.seed = seeds[k];
+    M.pad = 0;
+    pv.insert(pv.end(), meshes[k].verts, meshes[k].verts + 3ull * meshes[k].nv);
+    if (meshes[k].nt) pt.insert(pt.end(), meshes[k].tris, meshes[k].tris + 3ull * meshes[k].nt);
+    nVert += meshes[k].nv;
+    nTri += meshes[k].nt;
+  }
+  if (3 * nTri > 0xffffffffull || pv.size() > 0xffffffffull)
+    throw std::length_error("concave-edge planes: batch exceeds the edge scratch");
+  const uint32_t nm = (uint32_t)meshes.size(), nv = (uint32_t)nVert, nt = (uint32_t)nTri;
+  const uint32_t stride = 4 * o.nConcaveEdges;  // at most four planes per kept edge
+  ccMeshes_.ensure(nm);
+  ccVerts_.ensure(pv.size());
+  ccTris_.ensure(pt.size());
+  ccDeg_.ensure(nv);
+  ccCursor_.ensure(nv);
+  ccOff_.ensure(nv + 1ull);
+  ccKey_.ensure(3ull * nt);
+  ccDst_.ensure(3ull * nt);
+  ccCnt_.ensure(nv);
+  ccPos_.ensure(nv + 1ull);
+  ccEdges_.ensure(3ull * nt);  // at most one concave edge per directed edge
+  ccResv_.ensure((size_t)nm * o.nConcaveEdges);
+  ccPlanes_.ensure((size_t)nm * stride);
+  ccNp_.ensure(nm);
+  const uint64_t nChunks = std::max<uint64_t>(nv, 1) / dk::SCAN_CH + 2;
+  scanStatus_.ensure(std::max<uint64_t>(scanStatus_.n, nChunks));
+  scanVals_.ensure(std::max<uint64_t>(scanVals_.n, 2 * nChunks));
+  h2d(ccMeshes_.p, cm.data(), nm * sizeof(dk::CcMesh), "upload concave meshes");
+  h2d(ccVerts_.p, pv.data(), pv.size() * 4, "upload concave verts");
+  h2d(ccTris_.p, pt.data(), pt.size() * 4, "upload concave tris");
+  auto fill0 = [&](uint32_t* p, uint32_t n) {
+    if (!n) return;
+    dk::k_fill_u32<<<std::min<uint32_t>((n + 255) / 256, 2048), 256, 0, stream_>>>(p, n, 0u);
+    launched("k_fill_u32");
+  };
+  auto scanU32 = [&](const uint32_t* in, uint32_t* outp, uint32_t n) {
+    const uint32_t ch = std::max<uint32_t>(1, (n + dk::SCAN_CH - 1) / dk::SCAN_CH);
+    dk::k_scan_reset<<<std::min<uint32_t>((ch + 255) / 256, 1024), 256, 0, stream_>>>(scanStatus_.p, ch, scanCtl_.p);
+    launched("k_scan_reset");
+    uint32_t* vals = reinterpret_cast<uint32_t*>(scanVals_.p);
+    dk::k_cc_scan<<<std::min<uint32_t>(ch, 1024), dk::SCAN_T, 0, stream_>>>(dk::U32Scan{in, outp, n}, scanStatus_.p,
+                                                                             vals, vals + ch, scanCtl_.p);
+    launched("k_cc_scan");
+  };
+  const uint32_t tb = (nt + 255) / 256, vb = (nv + 255) / 256;
+  fill0(ccDeg_.p, nv);
+  fill0(ccCursor_.p, nv);
+  if (nt) {
+    dk::k_cc_degree<<<tb, 256, 0, stream_>>>(ccMeshes_.p, nm, nt, ccTris_.p, ccDeg_.p);
+    launched("k_cc_degree");
+  }
+  scanU32(ccDeg_.p, ccOff_.p, nv);
+  if (nt) {
+    dk::k_cc_fill<<<tb, 256, 0, stream_>>>(ccMeshes_.p, nm, nt, ccTris_.p, ccOff_.p, ccCursor_.p, ccKey_.p, ccDst_.p);
+    launched("k_cc_fill");
+  }
+  const cuacd::ConcaveCut cut = cuacd::concaveCut(o.concaveThreshold);
+  if (nv) {
+    dk::k_cc_sort<<<vb, 256, 0, stream_>>>(nv, ccOff_.p, ccKey_.p, ccDst_.p);
+    launched("k_cc_sort");
+    dk::k_cc_edges<<<vb, 256, 0, stream_>>>(ccMeshes_.p, nm, nv, ccVerts_.p, ccTris_.p, ccOff_.p, ccKey_.p, ccDst_.p,
+                                            cut, ccCnt_.p, nullptr, nullptr);
+    launched("k_cc_edges");
+  }
+  scanU32(ccCnt_.p, ccPos_.p, nv);
+  if (nv) {
+    dk::k_cc_edges<<<vb, 256, 0, stream_>>>(ccMeshes_.p, nm, nv, ccVerts_.p, ccTris_.p, ccOff_.p, ccKey_.p, ccDst_.p,
+                                            cut, ccCnt_.p, ccPos_.p, ccEdges_.p);
+    launched("k_cc_edges");
+  }
+  dk::k_cc_reservoir<<<(nm + 63) / 64, 64, 0, stream_>>>(ccMeshes_.p, nm, ccVerts_.p, ccTris_.p, ccPos_.p, ccEdges_.p,
+                                                         o.nConcaveEdges, o.concaveEps, ccResv_.p, ccPlanes_.p,
+                                                         ccNp_.p);
+  launched("k_cc_reservoir");
+  std::vector<float4> planes((size_t)nm * stride);
+  hipCheck(hipMemcpyAsync(planes.data(), ccPlanes_.p, planes.size() * sizeof(float4), hipMemcpyDeviceToHost, stream_),
+           "read concave planes");
+  std::vector<uint32_t> np(nm);
+  hipCheck(hipMemcpyAsync(np.data(), ccNp_.p, nm * 4, hipMemcpyDeviceToHost, stream_), "read concave plane counts");
+  dk::ScanCtl sc;
+  hipCheck(hipMemcpyAsync(&sc, scanCtl_.p, sizeof(sc), hipMemcpyDeviceToHost, stream_), "read scan state");
+  sync();
+  if (sc.error | sc.sticky) {
+    hipCheck(hipMemset(scanCtl_.p, 0, sizeof(dk::ScanCtl)), "clear scan control");
+    throw DeviceError("device scan: a chunk's look-back waited past its limit");
+  }
+  for (uint32_t k = 0; k < nm; k++) {
+    const uint32_t n = std::min(np[k], stride);
+    std::vector<float>& v = (*out)[k];
+    for (uint32_t i = 0; i < n; i++) {
+      const float4& q = planes[(size_t)k * stride + i];
+      v.insert(v.end(), {q.x, q.y, q.z, q.w});
+    }
+  }
+  return true;
 }
 
 // the sample set k_hd_samples writes for one mesh, read back (a debugging and test view of the

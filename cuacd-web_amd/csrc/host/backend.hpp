@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "concave.hpp"
 #include "mesh.hpp"
 
 namespace cuacd {
@@ -200,6 +201,17 @@ class Backend {
   // the Hausdorff sample set of one mesh (xyz interleaved), as the distance kernels see it
   // (src/cpu/hausdorff.ts:27-67 sampleMesh, src/gpu/hausdorff_shader.ts:58-86)
   virtual std::vector<float> sampleMesh(const MeshRef& m) = 0;
+  // concaveEdgePlanes (src/cpu/concave.ts:25) of every mesh, seeds[k] for mesh k, on the device:
+  // (*out)[k] holds mesh k's planes (4 floats each). false when this backend has no device
+  // implementation (the caller then runs the host one, which gives the same planes).
+  virtual bool concavePlanes(const std::vector<MeshRef>& meshes, const ConcaveOptions& o,
+                             const std::vector<uint32_t>& seeds, std::vector<std::vector<float>>* out) {
+    (void)meshes;
+    (void)o;
+    (void)seeds;
+    (void)out;
+    return false;
+  }
   // last run's items of one level (0 = seeds) and part descriptors
   virtual std::vector<ItemView> levelItems(uint32_t level) = 0;
   virtual bool readPart(uint32_t partIdx, PartView* pv, Mesh* mesh) = 0;

@@ -22,6 +22,16 @@ bool faceNormal(const std::vector<float>& V, const std::vector<uint32_t>& T, uin
 }
 }  // namespace
 
+ConcaveCut concaveCut(double thr) {
+  ConcaveCut k;
+  const double a0 = thr - M_PI, a1 = M_PI - thr;  // alpha > a0 (d > 0), alpha < a1 (d <= 0)
+  k.modeAbove = a0 < 0 ? 1 : (a0 >= M_PI ? 0 : 2);
+  k.cosAbove = std::cos(a0);
+  k.modeBelow = a1 <= 0 ? 0 : (a1 > M_PI ? 1 : 2);
+  k.cosBelow = std::cos(a1);
+  return k;
+}
+
 std::vector<float> concaveEdgePlanes(const Mesh& m, const ConcaveOptions& o) {
   std::vector<float> out;
   const auto& V = m.verts;
@@ -40,6 +50,7 @@ std::vector<float> concaveEdgePlanes(const Mesh& m, const ConcaveOptions& o) {
     rng = rng * 1664525u + 1013904223u;
     return rng;
   };
+  const ConcaveCut cut = concaveCut(o.concaveThreshold);
   std::vector<std::array<uint32_t, 4>> reservoir;
   uint64_t seen = 0;
   std::vector<uint32_t> partners;
@@ -58,7 +69,6 @@ std::vector<float> concaveEdgePlanes(const Mesh& m, const ConcaveOptions& o) {
       if (!faceNormal(V, T, f0, n0) || !faceNormal(V, T, f1, n1)) continue;
       double cosA = n0[0] * n1[0] + n0[1] * n1[1] + n0[2] * n1[2];
       cosA = std::max(-1.0, std::min(1.0, cosA));
-      const double alpha = std::acos(cosA);
       int64_t vOther = -1;
       for (int i = 0; i < 3; i++) {
         const uint32_t vv = T[3 * f0 + i];
@@ -71,8 +81,7 @@ std::vector<float> concaveEdgePlanes(const Mesh& m, const ConcaveOptions& o) {
       const double dx = (double)V[3 * vOther] - V[3 * v0], dy = (double)V[3 * vOther + 1] - V[3 * v0 + 1],
                    dz = (double)V[3 * vOther + 2] - V[3 * v0 + 2];
       const double d = dx * n1[0] + dy * n1[1] + dz * n1[2];
-      const double dihedral = d > 0 ? M_PI + alpha : M_PI - alpha;
-      if (!(dihedral > o.concaveThreshold)) continue;
+      if (!concaveEdge(cosA, d, cut)) continue;
       seen++;
       if (reservoir.size() < o.nConcaveEdges) {
         reservoir.push_back({v0, v1, f0, f1});

@@ -628,14 +628,26 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
     if (seeds.empty()) break;
     const double tcc = nowMs();
     if (iter < o.concaveIters && o.nConcaveEdges > 0) {
-      parallelFor(seeds.size(), [&](size_t k) {
-        ConcaveOptions co;
-        co.nConcaveEdges = o.nConcaveEdges;
-        co.concaveEps = o.concaveEps;
-        co.concaveThreshold = o.concaveThreshold;
-        co.seed = iter * 7919u + seeds[k].pi;
-        seeds[k].edge = concaveEdgePlanes(S[seeds[k].m].parts[seeds[k].pi].mesh, co);
-      });
+      ConcaveOptions co;
+      co.nConcaveEdges = o.nConcaveEdges;
+      co.concaveEps = o.concaveEps;
+      co.concaveThreshold = o.concaveThreshold;
+      std::vector<MeshRef> refs;
+      std::vector<uint32_t> cseeds;
+      for (const Seed& sd : seeds) {
+        refs.push_back(refOf(S[sd.m].parts[sd.pi].mesh));
+        cseeds.push_back(iter * 7919u + sd.pi);
+      }
+      std::vector<std::vector<float>> planes;
+      if (be_.concavePlanes(refs, co, cseeds, &planes)) {
+        for (size_t k = 0; k < seeds.size(); k++) seeds[k].edge = std::move(planes[k]);
+      } else {
+        parallelFor(seeds.size(), [&](size_t k) {
+          ConcaveOptions ck = co;
+          ck.seed = cseeds[k];
+          seeds[k].edge = concaveEdgePlanes(S[seeds[k].m].parts[seeds[k].pi].mesh, ck);
+        });
+      }
     }
     st.concaveMs += nowMs() - tcc;
     // tree search over every mesh's seeds, halving the batch on pool overflow

@@ -225,6 +225,24 @@ class GpuTree:
                                              n.value, C.byref(n)))
         return out[: 3 * n.value].reshape(-1, 3)
 
+    def concave_edge_planes(self, meshes, seeds, n_concave_edges=32, concave_eps=0.005, concave_threshold=3.49):
+        """concaveEdgePlanes (src/cpu/concave.ts:25) of every mesh on this backend's device, mesh k with
+        reservoir seed seeds[k]: a list of float32 (n, 4) plane arrays."""
+        n = len(meshes)
+        if n == 0:
+            return []
+        if len(seeds) != n:
+            raise CuacdError(F.ERR_ARG, "concave_edge_planes: one seed per mesh")
+        b = _batch(meshes)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32)
+        slot = 16 * n_concave_edges
+        out = np.zeros(max(slot * n, 1), dtype=np.float32)
+        cnt = np.zeros(n, dtype=np.uint32)
+        check(self.api, self.api.concave_edge_planes_batch(
+            self._h, C.byref(b), n_concave_edges, concave_eps, concave_threshold, sd.ctypes.data_as(C.POINTER(F.u32)),
+            out.ctypes.data_as(C.POINTER(F.f32)), cnt.ctypes.data_as(C.POINTER(F.u32))))
+        return [out[slot * k: slot * k + 4 * int(cnt[k])].reshape(-1, 4).copy() for k in range(n)]
+
     def hausdorff(self, pairs):
         """hausdorff(pairs) (src/gpu/tree.ts:318): float64 array, one per (a, b) pair."""
         n = len(pairs)

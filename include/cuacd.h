@@ -256,6 +256,13 @@ int cuacd_plane_cut(const float* verts, uint32_t nv, const uint32_t* tris, uint3
 int cuacd_concave_edge_planes(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
                               uint32_t n_concave_edges, double concave_eps, double concave_threshold, uint32_t seed,
                               float* out, uint32_t cap, uint32_t* n);
+/* concaveEdgePlanes (src/cpu/concave.ts:25) over a batch on the tree's device, mesh k with seed seeds[k]
+ * (the reservoir's LCG seed): out holds 16 * n_concave_edges floats per mesh (four planes per kept edge at
+ * most), n_planes[k] the planes written for mesh k. Equals cuacd_concave_edge_planes per mesh bit for bit;
+ * the reference computes them on the host one part at a time (src/decompose.ts:284-288). */
+int cuacd_concave_edge_planes_batch(cuacd_tree* t, const cuacd_mesh_batch* meshes, uint32_t n_concave_edges,
+                                    double concave_eps, double concave_threshold, const uint32_t* seeds, float* out,
+                                    uint32_t* n_planes);
 int32_t cuacd_is_closed(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt);
 double cuacd_signed_volume(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt);
 /* binary STL, welded by exact position (src/mesh.ts:166-212) */

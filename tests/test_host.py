@@ -10,6 +10,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from fixtures import l_prism, square_torus, stairs
@@ -108,6 +109,19 @@ def test_concave_edge_planes(cw):
     assert len(cw.concave_edge_planes(l_prism(cw), 8, 0.005, 3.49)) == 16
     assert len(cw.concave_edge_planes(cw.l_shape(), 8, 0.005, 3.49)) == 0
     assert len(cw.concave_edge_planes(cw.unit_cube(), 8, 0.005, 3.49)) == 0
+
+
+def test_concave_edge_planes_batch_entry(cw, oracle):
+    """The batched entry point on a backend without a device pass (the oracle's) runs the host
+    function per mesh: the same planes as one call per mesh with that mesh's seed."""
+    meshes = [l_prism(cw), cw.unit_cube(), cw.generate("blobs", 20260200, [3, 24, -0.6, 0.6, 0.12, 0.30, 0.5])]
+    t = cw.GpuTree(api=oracle, width=3, width2=3, depth=1, n_concave_edges=4)
+    got = t.concave_edge_planes(meshes, [5, 6, 7], n_concave_edges=4)
+    t.destroy()
+    for k, m in enumerate(meshes):
+        want = np.asarray(cw.concave_edge_planes(m, 4, 0.005, 3.49, seed=5 + k), np.float32).reshape(-1, 4)
+        assert np.array_equal(got[k].view(np.uint32), want.view(np.uint32))
+    assert [len(g) for g in got[:2]] == [4, 0] and len(got[2]) == 16
 
 
 def test_generators_closed(cw):
