@@ -3,6 +3,8 @@
 // Hausdorff batches. Mirrors the GpuTree class of src/gpu/tree.ts:116-816.
 #include <hip/hip_runtime.h>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -214,8 +216,10 @@ class GpuBackend : public Backend {
   // concave-edge planes: the batch, its edge CSR, concave edges, reservoirs, planes
   DevBuf<dk::CcMesh> ccMeshes_;
   DevBuf<float> ccVerts_;
-  DevBuf<uint32_t> ccTris_, ccDeg_, ccOff_, ccCursor_, ccKey_, ccDst_, ccCnt_, ccPos_, ccNp_;
-  DevBuf<uint4> ccEdges_, ccResv_;
+  DevBuf<uint32_t> ccTris_, ccVals_, ccFlag_, ccPos_, ccNp_;
+  DevBuf<unsigned long long> ccKeys_;
+  DevBuf<uint8_t> ccSortTmp_;
+  DevBuf<uint4> ccRec_, ccEdges_, ccResv_;
   DevBuf<float4> ccPlanes_;
   DevBuf<uint32_t> export_;
   DevBuf<float> hdVerts_, hdSamples_;
@@ -963,6 +967,7 @@ bool GpuBackend::concavePlanes(const std::vector<MeshRef>& meshes, const Concave
   std::vector<float> pv;
   std::vector<uint32_t> pt;
   uint64_t nVert = 0, nTri = 0;
+  uint32_t maxNv = 1;
   for (size_t k = 0; k < meshes.size(); k++) {
     dk::CcMesh& M = cm[k];
     M.vOff = (uint32_t)pv.size();
@@ -977,74 +982,70 @@ bool GpuBackend::concavePlanes(const std::vector<MeshRef>& meshes, const Concave
     if (meshes[k].nt) pt.insert(pt.end(), meshes[k].tris, meshes[k].tris + 3ull * meshes[k].nt);
     nVert += meshes[k].nv;
     nTri += meshes[k].nt;
+    maxNv = std::max(maxNv, meshes[k].nv);
   }
-  if (3 * nTri > 0xffffffffull || pv.size() > 0xffffffffull)
-    throw std::length_error("concave-edge planes: batch exceeds the edge scratch");
-  const uint32_t nm = (uint32_t)meshes.size(), nv = (uint32_t)nVert, nt = (uint32_t)nTri;
+  if (3 * nTri >= 0xffffffffull || pv.size() > 0xffffffffull || nVert >= 0xffffffffull)
+    throw std::length_error("concave-edge planes: batch exceeds 32-bit edge numbering");
+  const uint32_t nm = (uint32_t)meshes.size(), nt = (uint32_t)nTri, ne = 3 * nt;
+  if (ne == 0) return true;  // point sets only: no edges
+  auto bits = [](uint64_t x) {  // bits to hold 0 .. x
+    uint32_t b = 1;
+    while (b < 64 && (x >> b)) b++;
+    return b;
+  };
+  const uint32_t shift = bits(maxNv - 1), endBit = shift + bits(nVert ? nVert - 1 : 0);
   const uint32_t stride = 4 * o.nConcaveEdges;  // at most four planes per kept edge
   ccMeshes_.ensure(nm);
   ccVerts_.ensure(pv.size());
   ccTris_.ensure(pt.size());
-  ccDeg_.ensure(nv);
-  ccCursor_.ensure(nv);
-  ccOff_.ensure(nv + 1ull);
-  ccKey_.ensure(3ull * nt);
-  ccDst_.ensure(3ull * nt);
-  ccCnt_.ensure(nv);
-  ccPos_.ensure(nv + 1ull);
-  ccEdges_.ensure(3ull * nt);  // at most one concave edge per directed edge
+  ccKeys_.ensure(2ull * ne);
+  ccVals_.ensure(2ull * ne);
+  ccFlag_.ensure(ne);
+  ccPos_.ensure(ne + 1ull);
+  ccRec_.ensure(ne);
+  ccEdges_.ensure(ne);
   ccResv_.ensure((size_t)nm * o.nConcaveEdges);
   ccPlanes_.ensure((size_t)nm * stride);
   ccNp_.ensure(nm);
-  const uint64_t nChunks = std::max<uint64_t>(nv, 1) / dk::SCAN_CH + 2;
+  const uint64_t nChunks = (uint64_t)std::max<uint32_t>(ne, 1) / dk::SCAN_CH + 2;
   scanStatus_.ensure(std::max<uint64_t>(scanStatus_.n, nChunks));
   scanVals_.ensure(std::max<uint64_t>(scanVals_.n, 2 * nChunks));
   h2d(ccMeshes_.p, cm.data(), nm * sizeof(dk::CcMesh), "upload concave meshes");
   h2d(ccVerts_.p, pv.data(), pv.size() * 4, "upload concave verts");
   h2d(ccTris_.p, pt.data(), pt.size() * 4, "upload concave tris");
-  auto fill0 = [&](uint32_t* p, uint32_t n) {
-    if (!n) return;
-    dk::k_fill_u32<<<std::min<uint32_t>((n + 255) / 256, 2048), 256, 0, stream_>>>(p, n, 0u);
-    launched("k_fill_u32");
-  };
-  auto scanU32 = [&](const uint32_t* in, uint32_t* outp, uint32_t n) {
-    const uint32_t ch = std::max<uint32_t>(1, (n + dk::SCAN_CH - 1) / dk::SCAN_CH);
+  unsigned long long *kIn = ccKeys_.p, *kOut = ccKeys_.p + ne;
+  uint32_t *vIn = ccVals_.p, *vOut = ccVals_.p + ne;
+  {
+    dk::k_cc_keys<<<(nt + 255) / 256, 256, 0, stream_>>>(ccMeshes_.p, nm, nt, ccTris_.p, shift, kIn, vIn);
+    launched("k_cc_keys");
+    size_t tmpBytes = 0;
+    hipCheck(rocprim::radix_sort_pairs(nullptr, tmpBytes, kIn, kOut, vIn, vOut, ne, 0, endBit, stream_),
+             "radix sort size");
+    ccSortTmp_.ensure(tmpBytes);
+    hipCheck(rocprim::radix_sort_pairs(ccSortTmp_.p, tmpBytes, kIn, kOut, vIn, vOut, ne, 0, endBit, stream_),
+             "radix sort");
+    const cuacd::ConcaveCut cut = cuacd::concaveCut(o.concaveThreshold);
+    dk::k_cc_test<<<(ne + 255) / 256, 256, 0, stream_>>>(ccMeshes_.p, nm, ne, kOut, vOut, shift, ccVerts_.p,
+                                                         ccTris_.p, cut, ccFlag_.p, ccRec_.p);
+    launched("k_cc_test");
+  }
+  {
+    const uint32_t ch = std::max<uint32_t>(1, (ne + dk::SCAN_CH - 1) / dk::SCAN_CH);
     dk::k_scan_reset<<<std::min<uint32_t>((ch + 255) / 256, 1024), 256, 0, stream_>>>(scanStatus_.p, ch, scanCtl_.p);
     launched("k_scan_reset");
     uint32_t* vals = reinterpret_cast<uint32_t*>(scanVals_.p);
-    dk::k_cc_scan<<<std::min<uint32_t>(ch, 1024), dk::SCAN_T, 0, stream_>>>(dk::U32Scan{in, outp, n}, scanStatus_.p,
-                                                                             vals, vals + ch, scanCtl_.p);
+    dk::k_cc_scan<<<std::min<uint32_t>(ch, 1024), dk::SCAN_T, 0, stream_>>>(dk::U32Scan{ccFlag_.p, ccPos_.p, ne},
+                                                                             scanStatus_.p, vals, vals + ch,
+                                                                             scanCtl_.p);
     launched("k_cc_scan");
-  };
-  const uint32_t tb = (nt + 255) / 256, vb = (nv + 255) / 256;
-  fill0(ccDeg_.p, nv);
-  fill0(ccCursor_.p, nv);
-  if (nt) {
-    dk::k_cc_degree<<<tb, 256, 0, stream_>>>(ccMeshes_.p, nm, nt, ccTris_.p, ccDeg_.p);
-    launched("k_cc_degree");
   }
-  scanU32(ccDeg_.p, ccOff_.p, nv);
-  if (nt) {
-    dk::k_cc_fill<<<tb, 256, 0, stream_>>>(ccMeshes_.p, nm, nt, ccTris_.p, ccOff_.p, ccCursor_.p, ccKey_.p, ccDst_.p);
-    launched("k_cc_fill");
+  {
+    dk::k_cc_compact<<<(ne + 255) / 256, 256, 0, stream_>>>(ne, ccFlag_.p, ccPos_.p, ccRec_.p, ccEdges_.p);
+    launched("k_cc_compact");
   }
-  const cuacd::ConcaveCut cut = cuacd::concaveCut(o.concaveThreshold);
-  if (nv) {
-    dk::k_cc_sort<<<vb, 256, 0, stream_>>>(nv, ccOff_.p, ccKey_.p, ccDst_.p);
-    launched("k_cc_sort");
-    dk::k_cc_edges<<<vb, 256, 0, stream_>>>(ccMeshes_.p, nm, nv, ccVerts_.p, ccTris_.p, ccOff_.p, ccKey_.p, ccDst_.p,
-                                            cut, ccCnt_.p, nullptr, nullptr);
-    launched("k_cc_edges");
-  }
-  scanU32(ccCnt_.p, ccPos_.p, nv);
-  if (nv) {
-    dk::k_cc_edges<<<vb, 256, 0, stream_>>>(ccMeshes_.p, nm, nv, ccVerts_.p, ccTris_.p, ccOff_.p, ccKey_.p, ccDst_.p,
-                                            cut, ccCnt_.p, ccPos_.p, ccEdges_.p);
-    launched("k_cc_edges");
-  }
-  dk::k_cc_reservoir<<<(nm + 63) / 64, 64, 0, stream_>>>(ccMeshes_.p, nm, ccVerts_.p, ccTris_.p, ccPos_.p, ccEdges_.p,
-                                                         o.nConcaveEdges, o.concaveEps, ccResv_.p, ccPlanes_.p,
-                                                         ccNp_.p);
+  dk::k_cc_reservoir<<<(nm + 63) / 64, 64, 0, stream_>>>(ccMeshes_.p, nm, ccVerts_.p, ccTris_.p, kOut, ne, shift,
+                                                         ccPos_.p, ccEdges_.p, o.nConcaveEdges, o.concaveEps,
+                                                         ccResv_.p, ccPlanes_.p, ccNp_.p);
   launched("k_cc_reservoir");
   std::vector<float4> planes((size_t)nm * stride);
   hipCheck(hipMemcpyAsync(planes.data(), ccPlanes_.p, planes.size() * sizeof(float4), hipMemcpyDeviceToHost, stream_),

@@ -1,13 +1,14 @@
 // k_concave.hpp — concave-edge candidate planes of a batch of seed meshes on the device (HIP,
 // gfx950), the same planes as the host's concaveEdgePlanes (src/cpu/concave.ts:25-92) bit for bit:
-//   k_cc_degree / scan / k_cc_fill / k_cc_sort   directed-edge CSR of every mesh, each vertex's
-//                                                 edges in (triangle, corner) order as the host's
-//   k_cc_edges (count, scan, write)               the concave edges in sorted (v0 < v1) order: a
-//                                                 thread per vertex, FP64 face normals, the cosine
-//                                                 form of the dihedral test
-//   k_cc_reservoir                                a thread per mesh: the LCG reservoir over its
-//                                                 concave edges (a sequential stream by definition)
-//                                                 and the planes of the kept edges
+//   k_cc_keys + radix sort     every directed edge keyed (batch vertex a, b): one group per edge,
+//                              groups in (mesh, v0, v1) order, the host's edge order; O(E log E)
+//                              however uneven the vertex degrees (cap fans of cut parts reach
+//                              thousands of edges at one vertex)
+//   k_cc_test / scan / compact a thread per sorted entry: each edge v0 < v1 with a reverse edge tests
+//                              its first faces (FP64 normals, the cosine form of the dihedral test);
+//                              the concave ones compacted in order
+//   k_cc_reservoir             a thread per mesh: the LCG reservoir over its concave edges (a
+//                              sequential stream by definition) and the planes of the kept edges
 #pragma once
 #include "../host/concave.hpp"
 #include "dnum.hpp"
@@ -33,16 +34,6 @@ __device__ __forceinline__ uint32_t cc_mesh_of(const CcMesh* __restrict__ m, uin
   return lo;
 }
 
-// directed edges per vertex
-__global__ void k_cc_degree(const CcMesh* __restrict__ meshes, uint32_t nMeshes, uint32_t nTri,
-                            const uint32_t* __restrict__ tris, uint32_t* __restrict__ deg) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nTri) return;
-  const CcMesh M = meshes[cc_mesh_of(meshes, nMeshes, g, [](const CcMesh& m) { return m.gTri; })];
-  const uint32_t* T = tris + M.tOff + 3 * (size_t)(g - M.gTri);
-  for (int e = 0; e < 3; e++) atomicAdd(&deg[M.gVert + T[e]], 1u);
-}
-
 // exclusive scan of a u32 array of n entries into out[0..n], out[n] = total
 struct U32Scan {
   typedef uint32_t T;
@@ -60,39 +51,19 @@ __global__ __launch_bounds__(SCAN_T) void k_cc_scan(U32Scan s, uint32_t* status,
   scan_chunks(s, status, agg, inc, ctl);
 }
 
-// each directed edge a -> b at a's entries, keyed by 3 face + corner (the host's fill order)
-__global__ void k_cc_fill(const CcMesh* __restrict__ meshes, uint32_t nMeshes, uint32_t nTri,
-                          const uint32_t* __restrict__ tris, const uint32_t* __restrict__ off,
-                          uint32_t* __restrict__ cursor, uint32_t* __restrict__ key, uint32_t* __restrict__ dst) {
+// every directed edge a -> b of triangle g (corner e) as key (batch vertex of a, b), value 3 g + e:
+// sorted by key, a group of equal keys is one directed edge, its faces in value order
+__global__ void k_cc_keys(const CcMesh* __restrict__ meshes, uint32_t nMeshes, uint32_t nTri,
+                          const uint32_t* __restrict__ tris, uint32_t shift, unsigned long long* __restrict__ keys,
+                          uint32_t* __restrict__ vals) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nTri) return;
   const CcMesh M = meshes[cc_mesh_of(meshes, nMeshes, g, [](const CcMesh& m) { return m.gTri; })];
-  const uint32_t f = g - M.gTri;
-  const uint32_t* T = tris + M.tOff + 3 * (size_t)f;
+  const uint32_t* T = tris + M.tOff + 3 * (size_t)(g - M.gTri);
   for (uint32_t e = 0; e < 3; e++) {
     const uint32_t a = T[e], b = T[e == 2 ? 0 : e + 1];
-    const uint32_t k = off[M.gVert + a] + atomicAdd(&cursor[M.gVert + a], 1u);
-    key[k] = 3 * f + e;
-    dst[k] = b;
-  }
-}
-
-// a vertex's entries in key order (insertion sort: vertex degrees are small)
-__global__ void k_cc_sort(uint32_t nVert, const uint32_t* __restrict__ off, uint32_t* __restrict__ key,
-                          uint32_t* __restrict__ dst) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nVert) return;
-  const uint32_t b = off[v], e = off[v + 1];
-  for (uint32_t i = b + 1; i < e; i++) {
-    const uint32_t k = key[i], d = dst[i];
-    uint32_t j = i;
-    while (j > b && key[j - 1] > k) {
-      key[j] = key[j - 1];
-      dst[j] = dst[j - 1];
-      j--;
-    }
-    key[j] = k;
-    dst[j] = d;
+    keys[3 * (size_t)g + e] = ((unsigned long long)(M.gVert + a) << shift) | b;
+    vals[3 * (size_t)g + e] = 3 * g + e;
   }
 }
 
@@ -110,74 +81,100 @@ __device__ __forceinline__ bool cc_normal(const float* V, const uint32_t* T, uin
   return true;
 }
 
-// the first face of directed edge a -> b in a's (key-ordered) entries, or -1
-__device__ __forceinline__ int64_t cc_find(const uint32_t* off, const uint32_t* key, const uint32_t* dst, uint32_t ga,
-                                           uint32_t b) {
-  for (uint32_t k = off[ga]; k < off[ga + 1]; k++)
-    if (dst[k] == b) return key[k] / 3;
-  return -1;
+// first index of the sorted keys at or above k
+__device__ __forceinline__ uint32_t cc_lower(const unsigned long long* __restrict__ keys, uint32_t n,
+                                             unsigned long long k) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
 }
 
-// the concave edges (v0, v1 > v0) of one vertex in ascending v1 order; count only (out == nullptr)
-// or write (v0, v1, f0, f1) records from out
-__global__ void k_cc_edges(const CcMesh* __restrict__ meshes, uint32_t nMeshes, uint32_t nVert,
-                           const float* __restrict__ verts, const uint32_t* __restrict__ tris,
-                           const uint32_t* __restrict__ off, const uint32_t* __restrict__ key,
-                           const uint32_t* __restrict__ dst, cuacd::ConcaveCut cut, uint32_t* __restrict__ cnt,
-                           const uint32_t* __restrict__ pos, uint4* __restrict__ out) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nVert) return;
-  const CcMesh M = meshes[cc_mesh_of(meshes, nMeshes, g, [](const CcMesh& m) { return m.gVert; })];
-  const uint32_t v0 = g - M.gVert;
-  const float* V = verts + M.vOff;
-  const uint32_t* T = tris + M.tOff;
-  uint32_t n = 0, last = v0;
-  for (;;) {
-    // the next partner: the smallest destination above the last one
-    uint32_t v1 = 0xffffffffu;
-    for (uint32_t k = off[g]; k < off[g + 1]; k++) {
-      const uint32_t b = dst[k];
-      if (b > last && b < v1) v1 = b;
-    }
-    if (v1 == 0xffffffffu) break;
-    last = v1;
-    const int64_t fa = cc_find(off, key, dst, g, v1), fb = cc_find(off, key, dst, M.gVert + v1, v0);
-    if (fa < 0 || fb < 0) continue;
-    const uint32_t f0 = (uint32_t)(fa < fb ? fa : fb), f1 = (uint32_t)(fa < fb ? fb : fa);
-    double n0[3], n1[3];
-    if (!cc_normal(V, T, f0, n0) || !cc_normal(V, T, f1, n1)) continue;
-    double c = n0[0] * n1[0] + n0[1] * n1[1] + n0[2] * n1[2];
-    c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
-    int64_t vo = -1;
-    for (int i = 0; i < 3; i++) {
-      const uint32_t vv = T[3 * f0 + i];
-      if (vv != v0 && vv != v1) {
-        vo = vv;
-        break;
+// the smallest value of the group of key k starting at i (the directed edge's first face), or -1
+__device__ __forceinline__ int64_t cc_first(const unsigned long long* __restrict__ keys,
+                                            const uint32_t* __restrict__ vals, uint32_t n, uint32_t i,
+                                            unsigned long long k) {
+  if (i >= n || keys[i] != k) return -1;
+  uint32_t v = vals[i];
+  for (uint32_t j = i + 1; j < n && keys[j] == k; j++) v = vals[j] < v ? vals[j] : v;
+  return v;
+}
+
+// a thread per sorted directed edge: the first entry of each edge v0 -> v1 with v0 < v1 whose
+// reverse edge exists tests its two first faces (FP64 normals, the cosine form of the dihedral
+// test) and, when concave, writes flag 1 and the record (v0, v1, f0 < f1); groups in key order
+// are the host's (v0, ascending v1) order
+__global__ void k_cc_test(const CcMesh* __restrict__ meshes, uint32_t nMeshes, uint32_t n,
+                          const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ vals,
+                          uint32_t shift, const float* __restrict__ verts, const uint32_t* __restrict__ tris,
+                          cuacd::ConcaveCut cut, uint32_t* __restrict__ flag, uint4* __restrict__ rec) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long k = keys[i];
+  uint32_t f = 0;
+  const unsigned long long mask = (1ull << shift) - 1;
+  const uint32_t ga = (uint32_t)(k >> shift), v1 = (uint32_t)(k & mask);
+  const CcMesh M = meshes[cc_mesh_of(meshes, nMeshes, ga, [](const CcMesh& m) { return m.gVert; })];
+  const uint32_t v0 = ga - M.gVert;
+  if (v0 < v1 && (i == 0 || keys[i - 1] != k)) {
+    const unsigned long long rk = ((unsigned long long)(M.gVert + v1) << shift) | v0;
+    const int64_t fa = cc_first(keys, vals, n, i, k), fb = cc_first(keys, vals, n, cc_lower(keys, n, rk), rk);
+    if (fa >= 0 && fb >= 0) {
+      const uint32_t ta = (uint32_t)fa / 3 - M.gTri, tb = (uint32_t)fb / 3 - M.gTri;
+      const uint32_t f0 = ta < tb ? ta : tb, f1 = ta < tb ? tb : ta;
+      const float* V = verts + M.vOff;
+      const uint32_t* T = tris + M.tOff;
+      double n0[3], n1[3];
+      if (cc_normal(V, T, f0, n0) && cc_normal(V, T, f1, n1)) {
+        double c = n0[0] * n1[0] + n0[1] * n1[1] + n0[2] * n1[2];
+        c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
+        int64_t vo = -1;
+        for (int q = 0; q < 3; q++) {
+          const uint32_t vv = T[3 * f0 + q];
+          if (vv != v0 && vv != v1) {
+            vo = vv;
+            break;
+          }
+        }
+        if (vo >= 0) {
+          const double dx = (double)V[3 * vo] - V[3 * v0], dy = (double)V[3 * vo + 1] - V[3 * v0 + 1],
+                       dz = (double)V[3 * vo + 2] - V[3 * v0 + 2];
+          const double d = dx * n1[0] + dy * n1[1] + dz * n1[2];
+          if (cuacd::concaveEdge(c, d, cut)) {
+            f = 1;
+            rec[i] = make_uint4(v0, v1, f0, f1);
+          }
+        }
       }
     }
-    if (vo < 0) continue;
-    const double dx = (double)V[3 * vo] - V[3 * v0], dy = (double)V[3 * vo + 1] - V[3 * v0 + 1],
-                 dz = (double)V[3 * vo + 2] - V[3 * v0 + 2];
-    const double d = dx * n1[0] + dy * n1[1] + dz * n1[2];
-    if (!cuacd::concaveEdge(c, d, cut)) continue;
-    if (out) out[pos[g] + n] = make_uint4(v0, v1, f0, f1);
-    n++;
   }
-  if (!out) cnt[g] = n;
+  flag[i] = f;
+}
+
+// the flagged records in order: edges[pos[i]] = rec[i]
+__global__ void k_cc_compact(uint32_t n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                             const uint4* __restrict__ rec, uint4* __restrict__ edges) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) edges[pos[i]] = rec[i];
 }
 
 // a thread per mesh: the LCG reservoir of nKeep edges over its concave edges in order
 // (src/cpu/concave.ts:38-71), then per kept edge two face-offset planes and, unless the normals
 // are nearly opposite, two bisector planes: 4 nKeep plane slots per mesh, the count in count[m]
 __global__ void k_cc_reservoir(const CcMesh* __restrict__ meshes, uint32_t nMeshes, const float* __restrict__ verts,
-                               const uint32_t* __restrict__ tris, const uint32_t* __restrict__ pos,
+                               const uint32_t* __restrict__ tris, const unsigned long long* __restrict__ keys,
+                               uint32_t nKeys, uint32_t shift, const uint32_t* __restrict__ pos,
                                const uint4* __restrict__ edges, uint32_t nKeep, double eps, uint4* __restrict__ resv,
                                float4* __restrict__ planes, uint32_t* __restrict__ count) {
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nMeshes) return;
   const CcMesh M = meshes[m];
-  const uint32_t e0 = pos[M.gVert], e1 = pos[M.gVert + M.nv];
+  // the mesh's concave edges: the records of its key range (keys of vertices gVert .. gVert + nv)
+  const uint32_t e0 = pos[cc_lower(keys, nKeys, (unsigned long long)M.gVert << shift)],
+                 e1 = pos[cc_lower(keys, nKeys, (unsigned long long)(M.gVert + M.nv) << shift)];
   uint4* R = resv + (size_t)m * nKeep;
   uint32_t rng = M.seed, kept = 0;  // nKeep >= 1 (the caller launches nothing for 0)
   unsigned long long seen = 0;

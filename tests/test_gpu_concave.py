@@ -12,8 +12,13 @@ pytestmark = pytest.mark.gpu
 
 
 def _meshes(cw):
-    return [l_prism(cw), cw.unit_cube(), stairs(cw), square_torus(cw), cw.l_shape(),
-            cw.generate("blobs", 20260200, [3, 24, -0.6, 0.6, 0.12, 0.30, 0.5]),
+    blob = cw.generate("blobs", 20260200, [3, 24, -0.6, 0.6, 0.12, 0.30, 0.5])
+    # cut parts close their cross sections with fans: vertices of very high degree
+    pos, neg = cw.plane_cut(blob, np.array([0.3, 0.4, 0.866, -0.783], np.float32))
+    # a repeated triangle: directed edges with two faces each (the first one counts)
+    lp = l_prism(cw)
+    dup = cw.Mesh(lp.verts, np.concatenate([lp.tris, lp.tris[:3]]))
+    return [l_prism(cw), cw.unit_cube(), stairs(cw), square_torus(cw), cw.l_shape(), blob, pos, neg, dup,
             cw.generate("mix", 20260400, [3]), cw.generate("staircase", 5, [6]),
             cw.generate("torus", 11, [0.3, 0.1, 1.0, 24])]
 
