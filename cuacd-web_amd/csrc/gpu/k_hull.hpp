@@ -606,7 +606,10 @@ __device__ __forceinline__ void compact_cand(const Grp& g, HullCtx& C) {
 template <typename Grp>
 __device__ __forceinline__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p) {
   const uint32_t t = g.tid();
-  if (C.nf > 2 * C.nalive + 128 || C.nf + C.nv + 8 > C.fmax) compact_faces(g, C);
+  // room for the new faces: a visible disk of nvis faces has at most nvis + 2 <= nalive + 2
+  // horizon edges; compacted, the array has fmax - nalive >= nalive + 2 slots free, since a
+  // hull of nv points has at most 2 nv - 4 faces and fmax = 4 nv + 256
+  if (C.nf > 2 * C.nalive + 128 || C.nf + C.nalive + 2 > C.fmax) compact_faces(g, C);
   const int32_t* qp = C.Q + 3 * p;
   // visible faces in creation order
   uint32_t nvis = 0;
@@ -1110,7 +1113,18 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     const uint32_t v = on ? S.nh[lane] : 0u;
     const uint32_t incl = wave_incl_scan(v);
     const uint64_t over = __ballot(on && C.nf + incl > C.fmax);
-    if (over) act &= low_mask<Mask>((uint32_t)__ffsll((unsigned long long)over) - 1);  // never in EXACT rounds
+    // Candidates from the first that does not fit wait (never in EXACT rounds: with nv + 168
+    // slots free a round's at most 128 + 64 faces fit, or 4 nv - 4 when nv < 24). Candidate 0
+    // always takes part; when even its faces do not fit, it goes in alone by the one-at-a-time
+    // path, which compacts first and then has room.
+    if (over & 1ull) {
+      g.sync();  // every thread has read the round's shared counts
+      for (uint32_t e = t; e < nL; e += Grp::G) C.vmask[S.lface[e]] = 0ull;
+      g.sync();
+      hull_insert(g, C, S.cpt[0]);
+      return 1;
+    }
+    if (over) act &= low_mask<Mask>((uint32_t)__ffsll((unsigned long long)over) - 1);
     if (t < K) S.noff[t] = incl - v;  // read after the barrier below
   }
   g.sync();

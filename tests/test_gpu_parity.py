@@ -105,6 +105,16 @@ def test_blob_depth3(cw, oracle):
     assert rg.pruned > 0 or checked > 0
 
 
+def flat_cone(n_rim):
+    """A cone over a circle of n_rim points, its apex above, and one point just below the base's
+    centre: inserted last (its distance is the smallest), it sees every face of the base at once,
+    nearly half the hull's faces (n_rim - 2 of 2 n_rim - 2), more than a round's visible-face list
+    holds."""
+    a = 2 * np.pi * np.arange(n_rim) / n_rim
+    rim = np.stack([np.cos(a), np.sin(a), np.zeros(n_rim)], axis=1)
+    return np.concatenate([rim, [[0, 0, 1], [0.01, 0.02, -1e-4]]]).astype(np.float32)
+
+
 def _hull_cases(cw):
     rng = np.random.default_rng(7)
     cases = [cw.unit_cube().verts, cw.l_shape().verts, park_miller_cloud(1000)]
@@ -113,6 +123,7 @@ def _hull_cases(cw):
     cases.append(np.round(rng.random((500, 3)) * 4).astype(np.float32))  # many coplanar / duplicate points
     cases.append(np.concatenate([rng.random((50, 2)), np.zeros((50, 1))], axis=1).astype(np.float32))  # flat
     cases.append(np.zeros((3, 3), np.float32))  # too few points
+    cases.append(flat_cone(3000))  # its last point sees the whole base: 2998 of 5998 faces at once
     sphere = rng.standard_normal((4000, 3))
     cases.append((sphere / np.linalg.norm(sphere, axis=1, keepdims=True)).astype(np.float32))  # all extreme
     return [cw.Mesh(v, np.zeros((0, 3), np.uint32)) for v in cases]

@@ -225,6 +225,22 @@ def test_hull_against_qhull_on_clouds(cw, oracle):
         assert cw.is_closed(m) and cw.signed_volume(m) > 0
 
 
+def test_hull_point_seeing_the_whole_base(cw, oracle):
+    """A cone over a 3000-point circle plus a point just below the base's centre: that point goes in
+    last and sees all 2998 base faces at once (nearly half the hull). The hull's vertices are qhull's
+    and its volume is the polytope's up to FP64 rounding."""
+    from scipy.spatial import ConvexHull
+    a = 2 * np.pi * np.arange(3000) / 3000
+    pts = np.concatenate([np.stack([np.cos(a), np.sin(a), np.zeros(3000)], axis=1),
+                          [[0, 0, 1], [0.01, 0.02, -1e-4]]]).astype(np.float32)
+    tris, vol = _hull(cw, oracle, pts)
+    h = ConvexHull(pts.astype(np.float64))
+    assert set(int(x) for x in tris.reshape(-1)) == set(int(x) for x in h.vertices)
+    assert 3001 in set(int(x) for x in tris.reshape(-1))
+    assert abs(vol - h.volume) < 1e-12 * h.volume
+    assert len(tris) == 2 * len(h.vertices) - 4
+
+
 REF_QH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "libref_qh.so")
 
 
