@@ -75,6 +75,7 @@ HULL_TESTS = {"hull_big": "face_tests_big", "hull": "face_tests"}
 COUNTERS = ("face_tests", "face_tests_big", "emit_bytes", "hd_evals", "hull_jobs", "hull_bytes", "hull_bytes_big")
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector
+LDS_PEAK_GBS = 78600.0      # MI355X LDS: 128 B per clock per CU, 256 CUs at 2.4 GHz
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 TUNED_FILE = os.path.join(PKG_DIR, "tuned.json")  # written by cuacd-web_amd/autotune.py
 
@@ -116,21 +117,29 @@ def roofline(kst, workload):
         achieved, peak, unit, bound = work_per_launch / (avg_ms * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s", "hbm"
     else:
         achieved, peak, unit, bound = work_per_launch / (avg_ms * 1e-3) / 1e12, FP32_PEAK_TFLOPS, "TFLOP/s", "valu"
-    traffic = None
+    traffic = counters = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
         if t.get("kernel") == name and t.get("workload") == workload:
             traffic = t["hbm_bytes_per_launch"]
+        # instruction-mix counters of the same kernel from the committed SQ profile of this workload
+        if t.get("workload") == workload:
+            counters = {k: v for k, v in t.get("sq", {}).items() if k.split("<")[0] == "k_" + name}
     effort = None
     if name in HULL_TESTS:
+        # the candidate hull's own bound is LDS: each point-face test reads a 16-byte plane and
+        # costs about 8 FP32 operations (the kernel counts its tests)
         tests = kst[HULL_TESTS[name]]
-        flops = 8.0 * tests / (kst["ms"][name] * 1e-3)
-        effort = dict(point_face_tests_per_s=tests / (kst["ms"][name] * 1e-3), tests_per_launch=tests / launches,
-                      valu_tflops=flops / 1e12, valu_frac=flops / 1e12 / FP32_PEAK_TFLOPS)
+        secs = kst["ms"][name] * 1e-3
+        flops = 8.0 * tests / secs
+        effort = dict(point_face_tests_per_s=tests / secs, tests_per_launch=tests / launches,
+                      valu_tflops=flops / 1e12, valu_frac=flops / 1e12 / FP32_PEAK_TFLOPS,
+                      lds_plane_gbs=16.0 * tests / secs / 1e9, lds_frac=16.0 * tests / secs / 1e9 / LDS_PEAK_GBS)
     return dict(bound=bound, achieved=achieved, peak=peak, unit=unit, frac=achieved / peak, traffic=traffic,
                 kernel=name, launches=kst["launches"][name], avg_launch_ms=avg_ms,
                 work_per_launch=work_per_launch, work_counter=counter, work_per_unit=per_unit, effort=effort,
+                counters=counters,
                 kernel_ms={n: round(v, 3) for n, v in kst["ms"].items()},
                 kernel_launches=dict(kst["launches"]))
 
