@@ -20,8 +20,9 @@ Beside it, in the same JSON line:
   cpu_baseline the CPU oracle (the C++ restatement of the path, OpenMP over the host's cores) decomposing
                a sample of the mix in full, merge included, with every result checked equal to the GPU's;
                all cores, and one thread in a child process on a smaller sample.
-Each workload runs with the hull kernel split that cuacd-web_amd/autotune.py measured for it
-(cuacd-web_amd/tuned.json), when that file exists; results do not depend on it.
+Each workload runs with the knobs cuacd-web_amd/autotune.py measured for it (the hull kernel split,
+and the number of engines a batch is split over; cuacd-web_amd/tuned.json), when that file exists;
+results do not depend on them.
 """
 import argparse
 import ctypes
@@ -80,13 +81,14 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 TUNED_FILE = os.path.join(PKG_DIR, "tuned.json")  # written by cuacd-web_amd/autotune.py
 
 
-def tuned_split(workload):
-    """The hull kernel split (hull_wg_min_verts) the autotuner measured best for a workload; 0 (the
-    library default) when it has not been tuned. The split changes timing only, never results."""
+def tuned(workload):
+    """The knobs the autotuner measured best for a workload (hull_wg_min_verts, and batch_lanes for
+    batches); the library defaults for knobs it has not tuned. Knobs change timing only, never results."""
     if not os.path.exists(TUNED_FILE):
-        return 0
+        return {}
     with open(TUNED_FILE) as f:
-        return int(json.load(f).get(workload, {}).get("hull_wg_min_verts", 0))
+        t = json.load(f).get(workload, {})
+    return {k: int(t[k]) for k in ("hull_wg_min_verts", "batch_lanes") if k in t}
 
 
 def sum_kernels(kernel_dicts):
@@ -260,7 +262,7 @@ def main():
         copy_probe(a.copy_probe)
     meshes = [cw.generate("mix", MIX["seed"], [k]) for k in range(a.meshes)]
     tris = sum(m.nt for m in meshes)
-    opts = dict(MIX["options"], hull_wg_min_verts=tuned_split("mix"))
+    opts = dict(MIX["options"], **tuned("mix"))
     eng = cw.Decomposer(batch_tris=tris, **opts)
     for w in range(a.warmup):
         t0 = time.perf_counter()
@@ -323,7 +325,7 @@ def main():
                              ratio_batched_vs_plain=line["value"] / (len(sel) / tp), identical=same)
     if a.blob_steps:
         mesh = cw.generate("blobs", BLOB["seed"], BLOB["params"])
-        bopts = dict(BLOB["options"], hull_wg_min_verts=tuned_split("blob"))
+        bopts = dict(BLOB["options"], **tuned("blob"))
         beng = cw.Decomposer(**bopts)
         beng.decompose(mesh)  # warm
         bt, br = [], None

@@ -54,7 +54,7 @@ class DecomposeOptions(C.Structure):
                 ("merge_plane_tol", f64), ("verbose", u32), ("hausdorff_samples", u32),
                 ("vertex_cap", u64), ("tri_cap", u64), ("edge_cap", u64), ("scratch_words", u64),
                 ("hull_budget_bytes", u64), ("hull_max_verts", u32), ("prune_last_level", i32), ("device", i32),
-                ("hull_wg_min_verts", u32)]
+                ("hull_wg_min_verts", u32), ("batch_lanes", u32)]
 
 
 class PartDesc(C.Structure):

@@ -45,8 +45,22 @@ struct DevBuf {
     p = nullptr;
     n = 0;
     const size_t c = std::max<size_t>(count, 1);
+    reserveCheck(c * sizeof(T));
     hipCheck(hipMalloc(&p, c * sizeof(T)), "hipMalloc");
     n = c;
+  }
+  // A large allocation must leave DEVICE_RESERVE bytes free: the runtime takes kernel scratch
+  // (register spills) from device memory at launch, and a launch that cannot get it aborts its
+  // queue instead of returning an error. Such a request fails here, as a capacity error.
+  static void reserveCheck(size_t bytes) {
+    constexpr size_t DEVICE_RESERVE = 4ull << 30;
+    if (bytes < (256ull << 20)) return;
+    size_t freeB = 0, totalB = 0;
+    hipCheck(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
+    if (freeB < bytes || freeB - bytes < DEVICE_RESERVE)
+      throw std::length_error("device memory: a pool of " + std::to_string(bytes >> 20) + " MiB leaves less than " +
+                              std::to_string(DEVICE_RESERVE >> 30) + " GiB of the " + std::to_string(freeB >> 20) +
+                              " MiB free (fewer batch lanes or smaller pools fit)");
   }
 };
 
@@ -91,6 +105,7 @@ class GpuBackend : public Backend {
   KernelStats kernelStats() const override { return kst_; }
   const TreeConfig& config() const override { return cfg_; }
   uint32_t planeSlots() const override { return edgeStride_; }
+  void bindThread() override { hipCheck(hipSetDevice(cfg_.device), "hipSetDevice"); }
 
   TreeRunResult run(const std::vector<MeshRef>& seeds, const std::vector<std::vector<float>>& edgePlanes,
                     const RunOptions& opts) override;

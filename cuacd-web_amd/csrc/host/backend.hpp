@@ -178,6 +178,9 @@ struct KernelStats {
 class Backend {
  public:
   virtual KernelStats kernelStats() const { return KernelStats(); }
+  // makes this backend's device current on the calling host thread (a backend driven from a
+  // thread other than the one that created it)
+  virtual void bindThread() {}
   // Doubles the tree pools named by overflow bits (1 scratch, 2 vertices, 4 triangles,
   // 8 edges, 32 hull slab arena) up to
   // their element limits; false when none of them could grow. A backend without

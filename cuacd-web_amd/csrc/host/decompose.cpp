@@ -55,7 +55,9 @@ TreeConfig treeConfigForBatch(const DecomposeOptions& o, uint64_t totalTris) {
   if (!o.triCap) c.triCap = std::min<uint64_t>(c.triCap * 2 * scale, 0xffffffffull);
   if (!o.edgeCap) c.edgeCap *= scale;
   if (!o.scratchWords) c.scratchWords *= scale;
-  if (!o.hullBudgetBytes) c.hullBudgetBytes *= scale;
+  // the hull arena stays at its default: the hull grids are bounded by what fits it (a slab
+  // per group of the largest seed's size class), and the default holds the full grids for
+  // seeds of up to 32k points
   return c;
 }
 
@@ -818,6 +820,85 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
     s.R.kernels = ks;
     out.push_back(std::move(s.R));
   }
+  return out;
+}
+
+uint32_t batchLanesFor(const DecomposeOptions& o, size_t nMeshes) {
+  if (nMeshes < 2) return 1;
+  const uint32_t n = o.batchLanes ? o.batchLanes : 2;
+  return (uint32_t)std::min<size_t>(n, nMeshes);
+}
+
+std::vector<std::vector<uint32_t>> splitLanes(const std::vector<const Mesh*>& inputs, uint32_t nLanes) {
+  nLanes = std::max(1u, nLanes);
+  std::vector<uint32_t> order(inputs.size());
+  for (uint32_t k = 0; k < order.size(); k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return inputs[a]->nt() > inputs[b]->nt(); });
+  std::vector<uint64_t> load(nLanes, 0);
+  std::vector<std::vector<uint32_t>> lanes(nLanes);
+  for (uint32_t k : order) {
+    const uint32_t l = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[l] += inputs[k]->nt() + 1;
+    lanes[l].push_back(k);
+  }
+  for (auto& l : lanes) std::sort(l.begin(), l.end());
+  return lanes;
+}
+
+std::vector<DecomposeResult> decomposeLanes(const std::vector<Decomposer*>& lanes,
+                                            const std::vector<const Mesh*>& inputs) {
+  if (lanes.empty()) throw std::invalid_argument("decomposeLanes: no engine");
+  if (lanes.size() == 1 || inputs.size() < 2) return lanes[0]->decomposeBatch(inputs);
+  const double t0 = nowMs();
+  const std::vector<std::vector<uint32_t>> split = splitLanes(inputs, (uint32_t)lanes.size());
+  std::vector<std::vector<DecomposeResult>> res(lanes.size());
+  std::vector<std::exception_ptr> err(lanes.size());
+  auto work = [&](size_t l) {
+    try {
+      if (split[l].empty()) return;
+      lanes[l]->be_.bindThread();
+      std::vector<const Mesh*> mine;
+      for (uint32_t k : split[l]) mine.push_back(inputs[k]);
+      res[l] = lanes[l]->decomposeBatch(mine);
+    } catch (...) {
+      err[l] = std::current_exception();
+    }
+  };
+  std::vector<std::thread> th;
+  for (size_t l = 1; l < lanes.size(); l++) th.emplace_back(work, l);
+  work(0);
+  for (std::thread& x : th) x.join();
+  for (const std::exception_ptr& e : err)
+    if (e) std::rethrow_exception(e);
+  // the call's stats: the engines' counts and times summed, the call's wall time
+  DecomposeStats st;
+  KernelStats ks;
+  for (size_t l = 0; l < lanes.size(); l++) {
+    if (res[l].empty()) continue;
+    const DecomposeStats& a = res[l][0].stats;
+    st.hausdorffMs += a.hausdorffMs;
+    st.cutMs += a.cutMs;
+    st.concaveMs += a.concaveMs;
+    st.hullMs += a.hullMs;
+    st.treeMs += a.treeMs;
+    st.mergeMs += a.mergeMs;
+    st.treeRuns += a.treeRuns;
+    st.hullFailRuns += a.hullFailRuns;
+    st.candidateHulls += a.candidateHulls;
+    st.pruned += a.pruned;
+    st.treeGpuMs += a.treeGpuMs;
+    ks.add(res[l][0].kernels);
+  }
+  st.totalMs = nowMs() - t0;
+  std::vector<DecomposeResult> out(inputs.size());
+  for (size_t l = 0; l < lanes.size(); l++)
+    for (size_t q = 0; q < split[l].size(); q++) {
+      DecomposeResult& r = res[l][q];
+      r.stats = st;
+      r.kernels = ks;
+      out[split[l][q]] = std::move(r);
+    }
   return out;
 }
 

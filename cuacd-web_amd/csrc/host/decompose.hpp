@@ -65,14 +65,16 @@ struct DecomposeOptions {
   uint32_t hullMaxVerts = 0;
   int pruneLastLevel = -1;  // -1 = default (on)
   uint32_t hullWgMinVerts = 0;  // 0 = default (TreeConfig::hullWgMinVerts)
+  uint32_t batchLanes = 0;      // engines a batch is split over; 0 = default (2 for a batch)
 };
 
 // TreeConfig a Decomposer runs with (src/decompose.ts:141-148)
 TreeConfig treeConfigFor(const DecomposeOptions& o);
-// The tree pools and the hull arena for a batch of meshes: the single-mesh defaults
-// scaled with the batch's total triangle count (x1 to x4 per 400k triangles; the
-// vertex pool twice that), where the options leave them at their defaults. One tree run then holds more seeds, so
-// fewer runs overflow a pool and are run again in halves.
+// The tree pools for a batch of meshes: the single-mesh defaults scaled with the batch's
+// total triangle count (x1 to x4 per 400k triangles; the vertex and triangle pools twice
+// that), where the options leave them at their defaults; the hull arena keeps its default.
+// One tree run then holds more seeds, so fewer runs overflow a pool and are run again in
+// halves.
 TreeConfig treeConfigForBatch(const DecomposeOptions& o, uint64_t totalTris);
 
 // src/merge.ts:54-66 MergeStats
@@ -149,6 +151,7 @@ class Decomposer {
   void fillHausdorff(std::vector<State>& S);
   std::vector<Part> makeParts(const std::vector<const Mesh*>& meshes);
   std::vector<float> refineSpacing(const std::vector<const Mesh*>& meshes) const;
+  friend std::vector<DecomposeResult> decomposeLanes(const std::vector<Decomposer*>&, const std::vector<const Mesh*>&);
   Backend& be_;
   DecomposeOptions o_;
   size_t treeBatch_ = 0;  // largest seed count that last fit the tree pools (0: unknown)
@@ -157,5 +160,19 @@ class Decomposer {
 
 // number of host worker threads for per-part host work (cuts, concave planes)
 unsigned hostThreads();
+
+// A batch split over several engines (a backend and its decomposer each), the engines running at
+// once on host threads, so that one engine's host phases (cuts, merge bookkeeping, part assembly)
+// overlap another's device work. Meshes go to engines by triangle count (largest first, to the
+// least loaded engine) and keep their input order within an engine. A mesh's result equals
+// decompose() of it alone whatever engine runs it, so the split changes no result. Every result
+// carries the call's stats: counts and kernel times summed over the engines, the phase times
+// summed too (they overlap in time), totalMs the call's wall time.
+std::vector<DecomposeResult> decomposeLanes(const std::vector<Decomposer*>& lanes,
+                                            const std::vector<const Mesh*>& inputs);
+// the meshes of each of nLanes engines (indices into inputs)
+std::vector<std::vector<uint32_t>> splitLanes(const std::vector<const Mesh*>& inputs, uint32_t nLanes);
+// engines for a batch of nMeshes: the option, else 2, at most one per mesh
+uint32_t batchLanesFor(const DecomposeOptions& o, size_t nMeshes);
 
 }  // namespace cuacd

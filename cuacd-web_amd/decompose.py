@@ -70,6 +70,7 @@ class DecomposeOptions:
     prune_last_level: int = -1
     device: int = 0
     hull_wg_min_verts: int = 0  # hull kernel split (0: 1024); a performance knob, results unchanged
+    batch_lanes: int = 0  # engines a batch is split over, run at once (0: 2); results unchanged
 
     def raw(self):
         r = F.DecomposeOptions()

@@ -219,6 +219,7 @@ static napi_value Decompose(napi_env env, napi_callback_info info) {
   OPT_U32(tree, "hullMaxVerts", hull_max_verts);
   OPT_I32(tree, "pruneLastLevel", prune_last_level);
   OPT_U32(tree, "hullWgMinVerts", hull_wg_min_verts);
+  OPT_U32(tree, "batchLanes", batch_lanes);
 
   cuacd_result* r = NULL;
   if (L.decompose((const float*)pv, (uint32_t)(nvf / 3), (const uint32_t*)pt, (uint32_t)(ntu / 3), &o, &r) !=

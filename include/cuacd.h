@@ -163,6 +163,8 @@ typedef struct cuacd_decompose_options {
   int32_t prune_last_level;
   int32_t device;
   uint32_t hull_wg_min_verts; /* as in cuacd_tree_config; 0 = 1024 */
+  uint32_t batch_lanes; /* engines a batch is split over, run at once so that one engine's host phases
+                           overlap another's device work; results do not depend on it; 0 = 2 */
 } cuacd_decompose_options;
 
 /* src/part.ts:9-22 Part, flattened */

@@ -117,3 +117,32 @@ def test_engine_reuse_oracle(cw, oracle):
 @pytest.mark.gpu
 def test_engine_reuse_gpu(cw, oracle):
     _engine_cases(cw, cw.load_product(), oracle)
+
+
+def test_lane_split_balances_triangles(cw, oracle):
+    """A batch goes to engines by triangle count, largest first to the least loaded engine; every
+    engine count gives the same results, and the stats are the call's (summed over engines)."""
+    meshes = _meshes(cw)
+    ref = cw.decompose_batch(meshes, api=oracle, batch_lanes=1, **OPTS)
+    for lanes in (2, 3, 8):
+        got = cw.decompose_batch(meshes, api=oracle, batch_lanes=lanes, **OPTS)
+        for ra, rb in zip(ref, got):
+            _same(ra, rb)
+        call = ("total_ms", "tree_ms", "cut_ms", "candidate_hulls", "pruned")
+        assert all(all(r.stats[k] == got[0].stats[k] for k in call) for r in got)
+        assert got[0].stats["candidate_hulls"] == ref[0].stats["candidate_hulls"]
+        assert got[0].stats["pruned"] == ref[0].stats["pruned"]
+
+
+@pytest.mark.gpu
+def test_lanes_gpu_equal(cw, oracle):
+    meshes = _meshes(cw) + [cw.generate("mix", 20260400, [k]) for k in (1, 2, 14)]
+    ref = [cw.decompose(m, api=oracle, **OPTS) for m in meshes]
+    for lanes in (1, 2, 3):
+        print(f"batch over {lanes} engines", flush=True)
+        eng = cw.Decomposer(batch_tris=sum(m.nt for m in meshes), batch_lanes=lanes, **OPTS)
+        for _ in range(2):  # the engines' pools persist across calls
+            got = eng.decompose_batch(meshes)
+            for ra, rb in zip(ref, got):
+                _same(ra, rb)
+        eng.destroy()
