@@ -33,7 +33,8 @@ class SeedResult(C.Structure):
 
 class RunStats(C.Structure):
     _fields_ = [("overflow", i32), ("hull_fail", i32), ("np_overflow", i32), ("hull_fail_count", u32),
-                ("pruned", u32), ("jobs_per_level", u32 * 4), ("hulls_done", u64), ("gpu_ms", f64)]
+                ("pruned", u32), ("jobs_per_level", u32 * 4), ("hulls_done", u64), ("gpu_ms", f64),
+                ("last_level_chunks", u32), ("pad", u32)]
 
 
 class Item(C.Structure):

@@ -43,7 +43,8 @@ struct DEdge {  // open boundary edge p -> q on plane `plane` (a job index)
 constexpr uint32_t OVF_SCRATCH = 1, OVF_VERTS = 2, OVF_TRIS = 4, OVF_EDGES = 8, OVF_ARENA = 32;
 
 struct Counters {
-  uint32_t vTop, tTop, eTop, pad0;  // pool bump pointers (floats, u32, edges)
+  uint32_t vTop, tTop, eTop;  // pool bump pointers (floats, u32, edges)
+  uint32_t ovfLevels;         // bit l: a tree pool ran out at level l
   uint32_t overflow, hullFail, npOverflow, pruned;  // overflow: OVF_* bits of the pools that ran out
   unsigned long long hullsDone;
   unsigned long long faceTests;     // point-face record evaluations, wave-per-part hull kernel
@@ -54,6 +55,7 @@ struct Counters {
   unsigned long long hullBytesBig;  // the same for the workgroup-per-part hull kernel
   unsigned long long pad1;
   unsigned long long sTop;          // scratch words reserved by plan in the current level (64-bit: cannot wrap)
+  uint32_t markV, markT, markE, pad2;  // pool tops at the start of a level run in chunks
   uint32_t hd[1024];  // Hausdorff results (f32 bits of the squared distance)
 };
 

@@ -254,6 +254,8 @@ class GpuBackend : public Backend {
   std::vector<uint32_t> lastLevelCounts_;
   uint32_t lastPartCount_ = 0;
   uint32_t maxSeedNv_ = 0;  // the largest seed of the current run (bounds the hull slabs)
+  // part ranges of the last run's levels that ran in chunks: their geometry was not kept
+  std::vector<std::pair<uint32_t, uint32_t>> streamed_;
 };
 
 std::vector<Level> GpuBackend::layout(uint32_t nSeeds, uint32_t depth, uint32_t w0, uint32_t ne0) const {
@@ -616,73 +618,115 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
   }
   h2d(edgePlanes_.p, ep.data(), ep.size() * sizeof(float4), "upload edge planes");
   std::vector<dk::SeedState> st(nSeeds);
-  for (uint32_t k = 0; k < nSeeds; k++) {
-    std::memset(&st[k], 0, sizeof(st[k]));
-    st[k].cut = dk::NONE;
-    st[k].spacing = refine && k < opts.refine->spacing.size() ? opts.refine->spacing[k] : 0.0f;
-  }
-  h2d(state_.p, st.data(), nSeeds * sizeof(dk::SeedState), "upload seed state");
   dk::Counters c0;
   std::memset(&c0, 0, sizeof(c0));
   c0.vTop = (uint32_t)pv.size();
   c0.tTop = (uint32_t)pt.size();
-  h2d(ctr_.p, &c0, offsetof(dk::Counters, hd), "init counters");
-  // ---- stages: the tree, then each refinement round over the probe layout
+  // A run whose only overflow is a tree pool at its last level runs again with that level in
+  // chunks of input items (4x more each time): the last level's children are read only by their
+  // hulls and records, so the chunks reuse one pool range, and the results are the same.
   const uint32_t stages = 1 + (refine ? opts.refine->rounds : 0);
-  for (uint32_t s = 0; s < stages; s++) {
-    const std::vector<Level>& levels = s == 0 ? treeLevels : probeLevels;
-    if (s > 0)  // pool pointers back to the seeds; flags and counts accumulate
-      h2d(ctr_.p, &c0, 4 * sizeof(uint32_t), "reset counters");
-    for (uint32_t l = 0; l < depth; l++) {
-      const Level& L = levels[l];
-      const dk::LevelParams P = levelParams(L, l, depth, nSeeds);
-      if (L.nJobs == 0) continue;
-      runLevel(L, P);
-      dk::HullLevel HL;
-      HL.prune = P.prune && nSeeds > 0;
-      HL.itemsInBase = L.itemsInBase;
-      HL.nItemsIn = L.nItemsIn;
-      HL.nSeeds = nSeeds;
-      HL.depth = depth;
-      HL.jobBase = L.jobBase;
-      levelHulls(L, HL);
-      hipEvent_t e1 = tBegin();
-      dk::k_record<<<(L.nJobs + 255) / 256, 256, 0, stream_>>>(P, items_.p, parts_.p);
-      launched("k_record");
-      tEnd(K_RECORD, e1);
+  uint32_t maxLastItems = 1;
+  for (const std::vector<Level>* lv : {&treeLevels, &probeLevels})
+    if (!lv->empty()) maxLastItems = std::max(maxLastItems, lv->back().nItemsIn);
+  uint32_t chunks = 1;
+  dk::Counters c1;
+  for (;;) {
+    for (uint32_t k = 0; k < nSeeds; k++) {
+      std::memset(&st[k], 0, sizeof(st[k]));
+      st[k].cut = dk::NONE;
+      st[k].spacing = refine && k < opts.refine->spacing.size() ? opts.refine->spacing[k] : 0.0f;
     }
-    dk::EvalParams E;
-    std::memset(&E, 0, sizeof(E));
-    E.depth = depth;
-    E.nSeeds = nSeeds;
-    E.w0t = levels[0].totalWidth;
-    E.nProbes = nProbes;
-    E.mode = s == 0 ? 0 : 1;
-    E.jobBase0 = levels[0].jobBase;
-    E.edgeStride = edgeStride_;
-    for (uint32_t l = 0; l < depth; l++) {
-      E.outBase[l] = levels[l].itemsOutBase;
-      E.nJobs[l] = levels[l].nJobs;
+    h2d(state_.p, st.data(), nSeeds * sizeof(dk::SeedState), "upload seed state");
+    h2d(ctr_.p, &c0, offsetof(dk::Counters, hd), "init counters");
+    streamed_.clear();
+    // ---- stages: the tree, then each refinement round over the probe layout
+    for (uint32_t s = 0; s < stages; s++) {
+      const std::vector<Level>& levels = s == 0 ? treeLevels : probeLevels;
+      if (s > 0)  // pool pointers back to the seeds; flags and counts accumulate
+        h2d(ctr_.p, &c0, 3 * sizeof(uint32_t), "reset counters");
+      for (uint32_t l = 0; l < depth; l++) {
+        const Level& L = levels[l];
+        const dk::LevelParams P = levelParams(L, l, depth, nSeeds);
+        if (L.nJobs == 0) continue;
+        dk::HullLevel HL;
+        HL.prune = P.prune && nSeeds > 0;
+        HL.itemsInBase = L.itemsInBase;
+        HL.nItemsIn = L.nItemsIn;
+        HL.nSeeds = nSeeds;
+        HL.depth = depth;
+        HL.jobBase = L.jobBase;
+        const uint32_t nc = (l == depth - 1 && l > 0) ? std::min(chunks, L.nItemsIn) : 1;
+        const uint32_t per = (L.nItemsIn + nc - 1) / nc;
+        if (nc > 1) {
+          dk::k_pool_mark<<<1, 64, 0, stream_>>>(ctr_.p);
+          launched("k_pool_mark");
+          streamed_.push_back({L.partBase, L.partBase + 2 * L.nJobs});
+        }
+        for (uint32_t i0 = 0; i0 < L.nItemsIn; i0 += per) {
+          const uint32_t i1 = std::min(L.nItemsIn, i0 + per);
+          Level S = L;
+          S.itemsInBase = L.itemsInBase + i0;
+          S.nItemsIn = i1 - i0;
+          S.itemsOutBase = L.itemsOutBase + i0 * L.totalWidth;
+          S.jobBase = L.jobBase + i0 * L.totalWidth;
+          S.partBase = L.partBase + 2 * i0 * L.totalWidth;
+          S.nJobs = S.nItemsIn * L.totalWidth;
+          dk::LevelParams PS = P;
+          PS.itemsInBase = S.itemsInBase;
+          PS.nItemsIn = S.nItemsIn;
+          PS.itemsOutBase = S.itemsOutBase;
+          PS.jobBase = S.jobBase;
+          PS.partBase = S.partBase;
+          PS.nJobs = S.nJobs;
+          PS.itemChunk0 = i0;
+          runLevel(S, PS);
+          levelHulls(S, HL);
+          hipEvent_t e1 = tBegin();
+          dk::k_record<<<(S.nJobs + 255) / 256, 256, 0, stream_>>>(PS, items_.p, parts_.p);
+          launched("k_record");
+          tEnd(K_RECORD, e1);
+          if (nc > 1) {
+            dk::k_pool_rewind<<<1, 64, 0, stream_>>>(ctr_.p);
+            launched("k_pool_rewind");
+          }
+        }
+      }
+      dk::EvalParams E;
+      std::memset(&E, 0, sizeof(E));
+      E.depth = depth;
+      E.nSeeds = nSeeds;
+      E.w0t = levels[0].totalWidth;
+      E.nProbes = nProbes;
+      E.mode = s == 0 ? 0 : 1;
+      E.jobBase0 = levels[0].jobBase;
+      E.edgeStride = edgeStride_;
+      for (uint32_t l = 0; l < depth; l++) {
+        E.outBase[l] = levels[l].itemsOutBase;
+        E.nJobs[l] = levels[l].nJobs;
+      }
+      hipEvent_t e2 = tBegin();
+      dk::k_evaluate<<<nSeeds, dk::LT, 0, stream_>>>(E, items_.p, jobPlanes_.p, edgePlanes_.p, state_.p);
+      launched("k_evaluate");
+      tEnd(K_EVALUATE, e2);
+      lastLevelCounts_.assign(1, nSeeds);
+      for (const Level& L : levels) lastLevelCounts_.push_back(L.nJobs);
     }
-    hipEvent_t e2 = tBegin();
-    dk::k_evaluate<<<nSeeds, dk::LT, 0, stream_>>>(E, items_.p, jobPlanes_.p, edgePlanes_.p, state_.p);
-    launched("k_evaluate");
-    tEnd(K_EVALUATE, e2);
-    lastLevelCounts_.assign(1, nSeeds);
-    for (const Level& L : levels) lastLevelCounts_.push_back(L.nJobs);
+    hipCheck(hipMemcpyAsync(st.data(), state_.p, nSeeds * sizeof(dk::SeedState), hipMemcpyDeviceToHost, stream_),
+             "read seed state");
+    hipCheck(hipMemcpyAsync(&c1, ctr_.p, offsetof(dk::Counters, hd), hipMemcpyDeviceToHost, stream_), "read counters");
+    dk::ScanCtl sc;
+    hipCheck(hipMemcpyAsync(&sc, scanCtl_.p, sizeof(sc), hipMemcpyDeviceToHost, stream_), "read scan state");
+    sync();
+    if (sc.error | sc.sticky) {
+      hipCheck(hipMemset(scanCtl_.p, 0, sizeof(dk::ScanCtl)), "clear scan control");
+      throw DeviceError("device scan: a chunk's look-back waited past its limit");
+    }
+    const bool lastOnly = depth >= 2 && (c1.overflow & ~dk::OVF_ARENA) != 0 && c1.ovfLevels == (1u << (depth - 1));
+    if (!lastOnly || chunks >= maxLastItems) break;
+    chunks *= 4;
   }
   lastPartCount_ = (uint32_t)(nSeeds + 2 * totalJobs);
-  hipCheck(hipMemcpyAsync(st.data(), state_.p, nSeeds * sizeof(dk::SeedState), hipMemcpyDeviceToHost, stream_),
-           "read seed state");
-  dk::Counters c1;
-  hipCheck(hipMemcpyAsync(&c1, ctr_.p, offsetof(dk::Counters, hd), hipMemcpyDeviceToHost, stream_), "read counters");
-  dk::ScanCtl sc;
-  hipCheck(hipMemcpyAsync(&sc, scanCtl_.p, sizeof(sc), hipMemcpyDeviceToHost, stream_), "read scan state");
-  sync();
-  if (sc.error | sc.sticky) {
-    hipCheck(hipMemset(scanCtl_.p, 0, sizeof(dk::ScanCtl)), "clear scan control");
-    throw DeviceError("device scan: a chunk's look-back waited past its limit");
-  }
   for (uint32_t k = 0; k < nSeeds; k++) {
     SeedResult r;
     r.valid = st[k].valid != 0;
@@ -701,6 +745,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
   kst_.hullJobs += c1.hullsDone;
   for (const Level& L : treeLevels) R.jobsPerLevel.push_back(L.nJobs);
   R.gpuMs = nowMs() - t0;
+  R.lastLevelChunks = streamed_.empty() ? 1 : std::min(chunks, treeLevels.back().nItemsIn);
   kst_.faceTests += c1.faceTests;
   kst_.emitBytes += c1.emitBytes;
   kst_.hdEvals += c1.hdEvals;
@@ -739,6 +784,10 @@ std::vector<ItemView> GpuBackend::levelItems(uint32_t level) {
 
 bool GpuBackend::readPart(uint32_t partIdx, PartView* pv, Mesh* mesh) {
   if (partIdx >= lastPartCount_) return false;
+  if (mesh)
+    for (const auto& r : streamed_)
+      if (partIdx >= r.first && partIdx < r.second)
+        throw std::invalid_argument("part of a last level run in chunks: its volumes are kept, its geometry is not");
   dk::DPart p;
   hipCheck(hipMemcpyAsync(&p, parts_.p + partIdx, sizeof(p), hipMemcpyDeviceToHost, stream_), "read part");
   sync();

@@ -190,6 +190,7 @@ __global__ __launch_bounds__(LT) void k_lvl_items(LevelParams P, DItem* __restri
         if (at + need > P.sCap) {
           fits = false;
           atomicOr(&ctr->overflow, OVF_SCRATCH);
+          atomicOr(&ctr->ovfLevels, 1u << P.level);
         } else {
           b = (uint32_t)at;
         }
@@ -509,6 +510,7 @@ struct AllocScan {
         ok = false;
         job.valid = 4;
         atomicOr(&ctr->overflow, ovf);
+        atomicOr(&ctr->ovfLevels, 1u << P.level);
         for (int k = 0; k < 8; k++) out.partIds[k] = 0;
         out.np = out.srcPart = out.initialCut = out.nLevels = out.flags = out.parentItem = out.cutIdx = 0;
         for (int k = 0; k < 4; k++) out.levelCosts[k] = 0.0;
@@ -958,6 +960,24 @@ __global__ __launch_bounds__(LT) void k_lvl_emit(LevelParams P, const DJob* __re
       add_limbs(a, acc[sd].d.value());
       for (int q = 0; q < 3; q++) add_limbs(a + 3 * (q + 1), acc[sd].n[q].value());
     }
+  }
+}
+
+// A last level run in chunks of input items: its children are read only by their hulls and
+// records, so every chunk's children take the same pool range. The tops at the level's start
+// are kept (k_pool_mark) and restored after each chunk's hulls (k_pool_rewind).
+__global__ void k_pool_mark(Counters* __restrict__ ctr) {
+  if (threadIdx.x == 0) {
+    ctr->markV = ctr->vTop;
+    ctr->markT = ctr->tTop;
+    ctr->markE = ctr->eTop;
+  }
+}
+__global__ void k_pool_rewind(Counters* __restrict__ ctr) {
+  if (threadIdx.x == 0) {
+    ctr->vTop = ctr->markV;
+    ctr->tTop = ctr->markT;
+    ctr->eTop = ctr->markE;
   }
 }
 

@@ -110,6 +110,7 @@ struct TreeRunResult {
   std::vector<uint32_t> jobsPerLevel;
   uint64_t hullsDone = 0;  // child hulls completed (exact or pruned), all stages
   double gpuMs = 0;
+  uint32_t lastLevelChunks = 1;  // chunks of input items the last level ran in
 };
 
 struct HullResult {

@@ -74,6 +74,7 @@ class TreeRunResult:
     gpu_ms: float
     levels: list = field(default_factory=list)
     overflow_pools: int = 0  # bits: 1 scratch, 2 vertices, 4 triangles, 8 edges, 32 hull arena
+    last_level_chunks: int = 1  # chunks of input items the last level ran in (1: one pass)
 
 
 class GpuTree:
@@ -145,7 +146,7 @@ class GpuTree:
                                         tuple(r.plane) if r.valid else None))
         res = TreeRunResult(seeds_out, bool(st.overflow), bool(st.hull_fail), st.hull_fail_count,
                             bool(st.np_overflow), st.pruned, [x for x in st.jobs_per_level if x], st.hulls_done,
-                            st.gpu_ms, overflow_pools=st.overflow)
+                            st.gpu_ms, overflow_pools=st.overflow, last_level_chunks=st.last_level_chunks)
         if read_levels and n:
             res.levels = [self.level_items(l) for l in range(len(res.jobs_per_level) + 1)]
         return res

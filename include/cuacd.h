@@ -82,6 +82,9 @@ typedef struct cuacd_run_stats {
   uint32_t jobs_per_level[4];
   uint64_t hulls_done;
   double gpu_ms;
+  uint32_t last_level_chunks; /* 1, or the chunks of input items the last level ran in (its children then
+                                 keep no geometry: cuacd_read_part returns their volumes only) */
+  uint32_t pad;
 } cuacd_run_stats;
 
 /* src/gpu/tree.ts:48-58 ItemView */

@@ -120,22 +120,26 @@ def test_mix256_batched_equals_single(cw):
 
 
 @pytest.mark.gpu
-@pytest.mark.timeout(300)
-@pytest.mark.skip(reason="the last level (175,104 jobs) needs streamed pools; enabled with the streamed last level")
+@pytest.mark.timeout(600)
 def test_wide_lookahead_tree(cw):
     """One tree run of the wide form (width 48, width2 24, 64 concave-edge planes, depth 3) on a 44k
-    triangle blob: 304 + 7,296 + 175,104 jobs. No pool overflows; every level-0 job's two children sum
-    to the seed's volume; every exact child hull holds at least the child's volume; and the chosen cut,
-    cost and plane equal the brute force over all leaf paths of the run's own tables."""
+    triangle blob: 304 + 7,296 + 175,104 jobs. Its last level's 350k children hold about 3G vertex
+    floats, more than the pool (at most 2^32 floats), so that level runs in chunks of items that reuse
+    one pool range. No pool overflows; every level-0 job's two children sum to the seed's volume;
+    every exact child hull holds at least the child's volume; and the chosen cut, cost and plane equal
+    the brute force over all leaf paths of the run's own tables."""
     mesh = cw.normalize(cw.generate("blobs", 20260500, [12, 72, -0.6, 0.6, 0.12, 0.30, 0.5]))[0]
     planes = [cw.concave_edge_planes(mesh, n_concave_edges=64, seed=0)]
     m = 0.015
-    t = cw.GpuTree(width=48, width2=24, depth=3, n_concave_edges=64, min_edge_dist=m)
+    t = cw.GpuTree(width=48, width2=24, depth=3, n_concave_edges=64, min_edge_dist=m, tri_cap=1 << 30,
+                   vertex_cap=2 << 30)
     print(f"\nwide: tree run, {mesh.nt} triangles", flush=True)
     r = t.run([mesh], planes)
-    print(f"wide: jobs {r.jobs_per_level}, hulls {r.hulls_done}, pruned {r.pruned}, {r.gpu_ms:.0f} ms", flush=True)
+    print(f"wide: jobs {r.jobs_per_level}, hulls {r.hulls_done}, pruned {r.pruned}, {r.gpu_ms:.0f} ms, "
+          f"last level in {r.last_level_chunks} chunks", flush=True)
     assert r.jobs_per_level == [304, 7296, 175104]
     assert not r.overflow and not r.np_overflow and not r.hull_fail
+    assert r.last_level_chunks > 1
     vol = cw.volume(mesh)
     for it in r.levels[1]:
         if it["flags"] == 1:

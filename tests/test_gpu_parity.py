@@ -160,6 +160,39 @@ def test_hausdorff_bit_exact(cw, oracle):
         assert a[1] == b[1] and a[2] == b[2] and np.array_equal(a[0], b[0])
 
 
+def test_chunked_last_level_equals_one_pass(cw):
+    """A last level whose children do not fit the vertex pool runs in chunks of input items that
+    reuse one pool range: the chosen cuts, every level's item table and every part's volumes equal
+    the one-pass run with a large pool; the chunked level's parts report their volumes but no
+    geometry."""
+    mesh = cw.normalize(cw.generate("blobs", 20260201, [5, 20]))[0]
+    planes = [cw.concave_edge_planes(mesh, n_concave_edges=8, seed=3)]
+    cfg = dict(width=12, width2=6, depth=3, n_concave_edges=8, min_edge_dist=0.0125)
+    big = cw.GpuTree(**cfg)
+    ra = big.run([mesh, mesh], planes * 2)
+    assert ra.last_level_chunks == 1 and not ra.overflow
+    nparts = 2 + 2 * sum(ra.jobs_per_level)
+    first_last = 2 + 2 * sum(ra.jobs_per_level[:-1])
+    pa = [big.read_part(p, geometry=False) for p in range(nparts)]
+    base = sum(p["nv"] for p in pa[:first_last])
+    last = sum(p["nv"] for p in pa[first_last:])
+    assert last > 4 * base
+    small = cw.GpuTree(vertex_cap=3 * (base + last // 3), **cfg)
+    rb = small.run([mesh, mesh], planes * 2)
+    print(f"chunked last level: {rb.last_level_chunks} chunks, vertex floats {3 * base} + {3 * last}", flush=True)
+    assert rb.last_level_chunks > 1 and not rb.overflow
+    assert [(s.best_cut, s.best_cost, s.plane) for s in ra.seeds] == [(s.best_cut, s.best_cost, s.plane) for s in rb.seeds]
+    assert ra.levels == rb.levels
+    for p in range(nparts):
+        q = small.read_part(p, geometry=False)
+        for k in ("nv", "nt", "mesh_vol", "hull_vol", "flags", "n_cand"):
+            assert q[k] == pa[p][k], (p, k)
+    with pytest.raises(cw.CuacdError):
+        small.read_part(first_last, geometry=True)
+    big.destroy()
+    small.destroy()
+
+
 def test_pool_overflow_names_the_pool(cw):
     """A run whose pools are too small reports which pools ran out (1 scratch, 2 vertices, 4 triangles,
     8 edges) and stops there; with pools that just fit, the result equals the run with the default
