@@ -616,7 +616,6 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       ep[(size_t)k * edgeStride_ + i] = make_float4(e[0], e[1], e[2], e[3]);
     }
   }
-  h2d(edgePlanes_.p, ep.data(), ep.size() * sizeof(float4), "upload edge planes");
   std::vector<dk::SeedState> st(nSeeds);
   dk::Counters c0;
   std::memset(&c0, 0, sizeof(c0));
@@ -638,6 +637,9 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       st[k].spacing = refine && k < opts.refine->spacing.size() ? opts.refine->spacing[k] : 0.0f;
     }
     h2d(state_.p, st.data(), nSeeds * sizeof(dk::SeedState), "upload seed state");
+    // every attempt starts from the caller's planes: a refinement round's evaluate writes its
+    // probe planes over them
+    h2d(edgePlanes_.p, ep.data(), ep.size() * sizeof(float4), "upload edge planes");
     h2d(ctr_.p, &c0, offsetof(dk::Counters, hd), "init counters");
     streamed_.clear();
     // ---- stages: the tree, then each refinement round over the probe layout

@@ -193,6 +193,31 @@ def test_chunked_last_level_equals_one_pass(cw):
     small.destroy()
 
 
+def test_chunked_last_level_with_refinement(cw, oracle):
+    """The same with concave-edge planes and two refinement rounds: the first attempt overflows the
+    vertex pool at the last level and the run starts again in chunks. A refinement round writes its
+    probe planes where the caller's concave-edge planes live, so every attempt must start from the
+    caller's planes again; the chosen cuts, costs and planes equal the oracle's one-pass run."""
+    mesh = cw.normalize(cw.generate("blobs", 20260201, [5, 20]))[0]
+    planes = [cw.concave_edge_planes(mesh, n_concave_edges=8, seed=3)] * 2
+    cfg = dict(width=12, width2=6, depth=2, n_concave_edges=8, min_edge_dist=0.0125)
+    refine = dict(rounds=2, probes=8, spacing=[0.05, 0.05])
+    o = cw.GpuTree(api=oracle, **cfg)
+    ro = o.run([mesh, mesh], planes, refine=refine)
+    nparts = 2 + 2 * sum(ro.jobs_per_level)
+    first_last = 2 + 2 * sum(ro.jobs_per_level[:-1])
+    pa = [o.read_part(p, geometry=False) for p in range(nparts)]
+    base = sum(p["nv"] for p in pa[:first_last])
+    last = sum(p["nv"] for p in pa[first_last:])
+    small = cw.GpuTree(vertex_cap=3 * (base + last // 3), **cfg)
+    rb = small.run([mesh, mesh], planes, refine=refine)
+    print(f"chunked last level with refinement: {rb.last_level_chunks} chunks", flush=True)
+    assert rb.last_level_chunks > 1 and not rb.overflow
+    assert [(s.best_cut, s.best_cost, s.plane) for s in ro.seeds] == [(s.best_cut, s.best_cost, s.plane) for s in rb.seeds]
+    o.destroy()
+    small.destroy()
+
+
 def test_pool_overflow_names_the_pool(cw):
     """A run whose pools are too small reports which pools ran out (1 scratch, 2 vertices, 4 triangles,
     8 edges) and stops there; with pools that just fit, the result equals the run with the default
