@@ -142,6 +142,11 @@ HOST_API = {
     "is_closed": (i32, [P(f32), u32, P(u32), u32]),
     "signed_volume": (f64, [P(f32), u32, P(u32), u32]),
     "parse_stl": (C.c_int, [P(C.c_uint8), u64, P(C.c_void_p)]),
+    "parse_obj": (C.c_int, [C.c_char_p, u64, P(C.c_void_p)]),
+    "to_obj": (C.c_int, [P(MeshBatch), C.c_char_p, u64, P(u64)]),
+    "clean_hull_faces": (C.c_int, [P(f32), u32, P(u32), u32, f64, P(u32), u32, P(u32)]),
+    "compact_hull": (C.c_int, [P(f32), u32, P(u32), u32, P(f32), P(u32), P(u32), P(f64)]),
+    "hull_eps": (f64, [P(f32), u32]),
     "generate": (C.c_int, [u32, u64, P(f64), u32, P(C.c_void_p)]),
     "hausdorff_noalloc": (C.c_int, [P(MeshBatch), P(MeshBatch), u32, P(f64)]),
 }

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "build")
 
-HOST_SRCS = ["mesh.cpp", "cut.cpp", "concave.cpp", "sampling.cpp", "decompose.cpp", "gen.cpp", "capi_host.cpp"]
+HOST_SRCS = ["mesh.cpp", "cut.cpp", "concave.cpp", "sampling.cpp", "decompose.cpp", "gen.cpp", "qhull.cpp", "capi_host.cpp"]
 HIP_SRCS = ["gpu_backend.hip"]
 
 # -ffp-contract=off everywhere: every float expression rounds exactly as written.

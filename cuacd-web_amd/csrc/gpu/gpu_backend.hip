@@ -21,6 +21,7 @@
 #include "k_hull.hpp"
 #include "k_concave.hpp"
 #include "k_level.hpp"
+#include "k_merge.hpp"
 
 namespace cuacd {
 
@@ -115,6 +116,8 @@ class GpuBackend : public Backend {
   std::vector<float> sampleMesh(const MeshRef& m) override;
   bool concavePlanes(const std::vector<MeshRef>& meshes, const ConcaveOptions& o, const std::vector<uint32_t>& seeds,
                      std::vector<std::vector<float>>* out) override;
+  void mergePairs(const std::vector<cuacd::MergePart>& parts, const std::vector<uint32_t>& off, double threshold,
+                  std::vector<uint8_t>* pass) override;
   std::vector<ItemView> levelItems(uint32_t level) override;
   bool readPart(uint32_t partIdx, PartView* pv, Mesh* mesh) override;
 
@@ -237,6 +240,10 @@ class GpuBackend : public Backend {
   DevBuf<uint4> ccRec_, ccEdges_, ccResv_;
   DevBuf<float4> ccPlanes_;
   DevBuf<uint32_t> export_;
+  DevBuf<cuacd::MergePart> mgParts_;  // the merge pass's pair rejections
+  DevBuf<uint32_t> mgOff_;
+  DevBuf<unsigned long long> mgBase_;
+  DevBuf<uint8_t> mgPass_;
   DevBuf<float> hdVerts_, hdSamples_;
   DevBuf<uint32_t> hdTris_, hdCounts_;
   DevBuf<dk::HdMesh> hdMeshes_;
@@ -755,6 +762,35 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
   kst_.hullBytes += c1.hullBytes;
   kst_.hullBytesBig += c1.hullBytesBig;
   return R;
+}
+
+void GpuBackend::mergePairs(const std::vector<cuacd::MergePart>& parts, const std::vector<uint32_t>& off,
+                            double threshold, std::vector<uint8_t>* pass) {
+  pass->clear();
+  if (off.size() < 2) return;
+  const uint32_t nl = (uint32_t)(off.size() - 1);
+  std::vector<unsigned long long> base(nl);
+  uint64_t total = 0;
+  for (uint32_t l = 0; l < nl; l++) {
+    if (off[l + 1] < off[l] || off[l + 1] > parts.size()) throw std::invalid_argument("merge pairs: bad list offsets");
+    base[l] = total;
+    total += cuacd::merge_pairs_of(off[l + 1] - off[l]);
+  }
+  pass->assign(total, 0);
+  if (!total) return;
+  // sized with headroom: the lists shrink and grow over the rounds of a merge
+  mgParts_.ensure(std::max<size_t>(parts.size(), 4096));
+  mgOff_.ensure(std::max<size_t>(off.size(), 1024));
+  mgBase_.ensure(std::max<size_t>(nl, 1024));
+  mgPass_.ensure(std::max<size_t>(total, 1u << 20));
+  h2d(mgParts_.p, parts.data(), parts.size() * sizeof(cuacd::MergePart), "upload merge parts");
+  h2d(mgOff_.p, off.data(), off.size() * sizeof(uint32_t), "upload merge lists");
+  h2d(mgBase_.p, base.data(), nl * sizeof(unsigned long long), "upload merge pair bases");
+  dk::k_merge_pairs<<<nl, 256, 0, stream_>>>(mgParts_.p, mgOff_.p, mgBase_.p, threshold,
+                                             (threshold / 2) * (threshold / 2), mgPass_.p);
+  launched("k_merge_pairs");
+  hipCheck(hipMemcpyAsync(pass->data(), mgPass_.p, total, hipMemcpyDeviceToHost, stream_), "read merge pairs");
+  sync();
 }
 
 std::vector<ItemView> GpuBackend::levelItems(uint32_t level) {

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "concave.hpp"
+#include "cost.hpp"
 #include "mesh.hpp"
 
 namespace cuacd {
@@ -215,6 +216,19 @@ class Backend {
     (void)seeds;
     (void)out;
     return false;
+  }
+  // The merge pass's cheap rejections over every pair of several parts lists at once
+  // (src/merge.ts:192-204): list l holds parts [off[l], off[l + 1]); pass receives, list after
+  // list, one byte per pair (i < j) in row order (merge_row), 1 when the pair is to be hulled.
+  // This implementation is the host loop; a device backend evaluates the pairs in one launch.
+  virtual void mergePairs(const std::vector<MergePart>& parts, const std::vector<uint32_t>& off, double threshold,
+                          std::vector<uint8_t>* pass) {
+    const double halfT2 = (threshold / 2) * (threshold / 2);
+    pass->clear();
+    for (size_t l = 0; l + 1 < off.size(); l++)
+      for (uint32_t i = off[l]; i < off[l + 1]; i++)
+        for (uint32_t j = i + 1; j < off[l + 1]; j++)
+          pass->push_back(merge_pair_passes(parts[i], parts[j], threshold, halfT2) ? 1 : 0);
   }
   // last run's items of one level (0 = seeds) and part descriptors
   virtual std::vector<ItemView> levelItems(uint32_t level) = 0;

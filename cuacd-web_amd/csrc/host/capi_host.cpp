@@ -9,8 +9,10 @@
 #include "../../../include/cuacd.h"
 #include "concave.hpp"
 #include "cut.hpp"
+#include "decompose.hpp"
 #include "gen.hpp"
 #include "mesh.hpp"
+#include "qhull.hpp"
 
 #define CUACD_EXPORT extern "C" __attribute__((visibility("default")))
 
@@ -98,7 +100,7 @@ CUACD_EXPORT int cuacd_parse_stl(const uint8_t* buf, uint64_t len, cuacd_meshes*
   std::vector<uint8_t> b(buf, buf + len);
   std::unique_ptr<cuacd_meshes> m(new cuacd_meshes);
   cuacd::Mesh mesh;
-  if (!cuacd::parseBinarySTL(b, &mesh)) return CUACD_ERR_ARG;
+  if (!cuacd::parseSTL(b, &mesh)) return CUACD_ERR_ARG;
   m->m.push_back(std::move(mesh));
   *out = m.release();
   return CUACD_OK;
@@ -136,3 +138,68 @@ CUACD_EXPORT int cuacd_generate(uint32_t kind, uint64_t seed, const double* p, u
     return CUACD_ERR_ARG;
   }
 }
+
+CUACD_EXPORT int cuacd_parse_obj(const char* text, uint64_t len, cuacd_meshes** out) {
+  try {
+    std::unique_ptr<cuacd_meshes> m(new cuacd_meshes);
+    m->m.push_back(cuacd::parseOBJ(std::string(text, text + len)));
+    *out = m.release();
+    return CUACD_OK;
+  } catch (const std::exception&) {
+    return CUACD_ERR_RUNTIME;
+  }
+}
+
+CUACD_EXPORT int cuacd_to_obj(const cuacd_mesh_batch* parts, char* out, uint64_t cap, uint64_t* len) {
+  try {
+    std::vector<cuacd::Mesh> ms;
+    for (uint32_t k = 0; k < parts->n; k++) {
+      const uint32_t nt = parts->nt ? parts->nt[k] : 0;
+      ms.push_back(meshOf(parts->verts + 3ull * parts->v_off[k], parts->nv[k],
+                          nt ? parts->tris + 3ull * parts->t_off[k] : nullptr, nt));
+    }
+    std::vector<const cuacd::Mesh*> ptrs;
+    for (const cuacd::Mesh& m : ms) ptrs.push_back(&m);
+    const std::string s = cuacd::toOBJ(ptrs);
+    *len = s.size();
+    if (out && cap) std::memcpy(out, s.data(), std::min<uint64_t>(cap, s.size()));
+    return CUACD_OK;
+  } catch (const std::invalid_argument&) {
+    return CUACD_ERR_ARG;
+  } catch (const std::exception&) {
+    return CUACD_ERR_RUNTIME;
+  }
+}
+
+CUACD_EXPORT int cuacd_clean_hull_faces(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, double eps,
+                                        uint32_t* out, uint32_t nt_cap, uint32_t* nt_out) {
+  try {
+    std::vector<uint32_t> T(tris, tris + 3ull * nt);
+    for (uint32_t x : T)
+      if (x >= nv) return CUACD_ERR_ARG;
+    const std::vector<uint32_t> c = cuacd::cleanHullFaces(verts, T, eps);
+    *nt_out = (uint32_t)(c.size() / 3);
+    if (out) std::memcpy(out, c.data(), 4 * std::min<size_t>(c.size(), 3ull * nt_cap));
+    return CUACD_OK;
+  } catch (const std::exception&) {
+    return CUACD_ERR_RUNTIME;
+  }
+}
+
+CUACD_EXPORT int cuacd_compact_hull(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, float* hull_verts,
+                                    uint32_t* hull_tris, uint32_t* nv_out, double* volume) {
+  try {
+    std::vector<uint32_t> T(tris, tris + 3ull * nt);
+    for (uint32_t x : T)
+      if (x >= nv) return CUACD_ERR_ARG;
+    const cuacd::Mesh m = cuacd::compactHull(verts, T, volume);
+    *nv_out = m.nv();
+    if (hull_verts && !m.verts.empty()) std::memcpy(hull_verts, m.verts.data(), 4 * m.verts.size());
+    if (hull_tris && !m.tris.empty()) std::memcpy(hull_tris, m.tris.data(), 4 * m.tris.size());
+    return CUACD_OK;
+  } catch (const std::exception&) {
+    return CUACD_ERR_RUNTIME;
+  }
+}
+
+CUACD_EXPORT double cuacd_hull_eps(const float* verts, uint32_t nv) { return cuacd::hullEpsFor(verts, nv); }

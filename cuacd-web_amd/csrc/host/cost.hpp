@@ -53,4 +53,34 @@ CUACD_HD inline double rv_cost(double meshVol, double hullVol) {
   return K_RV * det_cbrt(THREE_OVER_4PI * gap);
 }
 
+// one part as the merge pass's cheap rejections see it: volumes and f32 bounding box
+struct MergePart {
+  double meshVol, hullVol;
+  float lo[3], hi[3];
+};
+
+// src/merge.ts:78-85 bboxGap2: squared gap between two boxes (0 when they overlap), FP64
+CUACD_HD inline double bbox_gap2(const MergePart& a, const MergePart& b) {
+  double g2 = 0;
+  for (int k = 0; k < 3; k++) {
+    const double u = (double)a.lo[k] - b.hi[k], v = (double)b.lo[k] - a.hi[k];
+    double g = u > v ? u : v;
+    g = g > 0.0 ? g : 0.0;
+    g2 += g * g;
+  }
+  return g2;
+}
+
+// src/merge.ts:198-199: a pair of parts is hulled unless its rv lower bound (hulls summed) reaches
+// the threshold or the boxes lie at least half the threshold apart (halfT2 = (threshold / 2)^2)
+CUACD_HD inline bool merge_pair_passes(const MergePart& a, const MergePart& b, double threshold, double halfT2) {
+  if (rv_cost(a.meshVol + b.meshVol, a.hullVol + b.hullVol) >= threshold) return false;
+  if (bbox_gap2(a, b) >= halfT2) return false;
+  return true;
+}
+
+// pairs (i < j) of a list of n parts, row by row: pair (i, j) at merge_row(n, i) + j - i - 1
+CUACD_HD inline uint64_t merge_row(uint64_t n, uint64_t i) { return i * n - i * (i + 1) / 2; }
+CUACD_HD inline uint64_t merge_pairs_of(uint64_t n) { return n < 2 ? 0 : n * (n - 1) / 2; }
+
 }  // namespace cuacd

@@ -16,6 +16,7 @@
 
 #include "concave.hpp"
 #include "cut.hpp"
+#include "qhull.hpp"
 
 namespace cuacd {
 
@@ -90,15 +91,6 @@ Mesh compactHull(const float* hv, const std::vector<uint32_t>& tris, double* vol
 // ---- merge pass ------------------------------------------------------------
 
 namespace {
-
-double bboxGap2(const Box3& a, const Box3& b) {
-  double g2 = 0;
-  for (int k = 0; k < 3; k++) {
-    const double g = std::max(0.0, std::max((double)a.lo[k] - b.hi[k], (double)b.lo[k] - a.hi[k]));
-    g2 += g * g;
-  }
-  return g2;
-}
 
 bool triCoplanar(const std::vector<float>& V, const std::vector<uint32_t>& T, size_t t, const Plane& p, double tol) {
   for (int e = 0; e < 3; e++) {
@@ -243,6 +235,7 @@ struct MergeRun {
   std::unordered_map<uint64_t, Box3> boxes;
   std::map<std::pair<uint64_t, uint64_t>, PairEval> cache;
   uint64_t nextId = 1u << 30;
+  double hullEps = 2e-5;  // output-hull tolerance of the list's merged hulls
   bool done = false;
   // this round
   std::vector<std::pair<uint32_t, uint32_t>> todo, candidates;
@@ -268,15 +261,15 @@ struct MergeRun {
 // list's decisions are its own (src/merge.ts:170-303), so every list ends exactly
 // as it would alone.
 std::vector<std::vector<Part>> mergeHullsMany(Backend& be, std::vector<std::vector<Part>> inputs, const MergeOptions& o,
-                                              std::vector<MergeStats>* statsOut) {
+                                              std::vector<MergeStats>* statsOut, const std::vector<double>* hullEps) {
   const double t0 = nowMs();
   std::vector<MergeRun> R(inputs.size());
   for (size_t r = 0; r < R.size(); r++) {
+    R[r].hullEps = hullEps && r < hullEps->size() ? (*hullEps)[r] : o.hullEps;
     R[r].parts = std::move(inputs[r]);
     for (Part& p : R[r].parts)
       if (p.uid == 0) p.uid = R[r].nextId++;
   }
-  const double halfT2 = (o.threshold / 2) * (o.threshold / 2);
   const size_t batch = 1024;
   double hullMs = 0, hdMs = 0;
   for (uint32_t round = 0; round < o.maxRounds; round++) {
@@ -285,12 +278,33 @@ std::vector<std::vector<Part>> mergeHullsMany(Backend& be, std::vector<std::vect
       if (!m.done && m.parts.size() > 1) act.push_back(&m);
       else m.done = true;
     if (act.empty()) break;
+    // the cheap rejections of every pair of every list in one backend call (one launch on the device)
+    std::vector<MergePart> mp;
+    std::vector<uint32_t> mOff(1, 0);
+    for (MergeRun* m : act) {
+      for (const Part& p : m->parts) {
+        const Box3& b = m->boxOf(p);
+        MergePart q;
+        q.meshVol = p.meshVol;
+        q.hullVol = p.hullVol;
+        for (int k = 0; k < 3; k++) {
+          q.lo[k] = b.lo[k];
+          q.hi[k] = b.hi[k];
+        }
+        mp.push_back(q);
+      }
+      mOff.push_back((uint32_t)mp.size());
+    }
+    std::vector<uint8_t> pass;
+    be.mergePairs(mp, mOff, o.threshold, &pass);
     // candidate pairs of each list; the merged hull of each new pair's two hull vertex sets
+    uint64_t pairBase = 0;
     for (MergeRun* m : act) {
       m->stats.rounds = round + 1;
       m->todo.clear();
       m->candidates.clear();
       const std::vector<Part>& parts = m->parts;
+      const uint64_t n = parts.size();
       for (uint32_t i = 0; i < parts.size(); i++) {
         for (uint32_t j = i + 1; j < parts.size(); j++) {
           const Part& A = parts[i];
@@ -299,12 +313,12 @@ std::vector<std::vector<Part>> mergeHullsMany(Backend& be, std::vector<std::vect
             m->candidates.push_back({i, j});
             continue;
           }
-          if (rv_cost(A.meshVol + B.meshVol, A.hullVol + B.hullVol) >= o.threshold) continue;
-          if (bboxGap2(m->boxOf(A), m->boxOf(B)) >= halfT2) continue;
+          if (!pass[pairBase + merge_row(n, i) + j - i - 1]) continue;
           m->todo.push_back({i, j});
           m->candidates.push_back({i, j});
         }
       }
+      pairBase += merge_pairs_of(n);
       m->evals.assign(m->todo.size(), PairEval());
       m->pointSets.assign(m->todo.size(), {});
       for (size_t k = 0; k < m->todo.size(); k++) {
@@ -344,7 +358,9 @@ std::vector<std::vector<Part>> mergeHullsMany(Backend& be, std::vector<std::vect
           const double rv = rv_cost(A.meshVol + B.meshVol, h.volume);
           ev.cost = rv;
           if (rv < o.threshold) {
-            ev.hull = compactHull(m->pointSets[k].data(), h.tris, nullptr);
+            // src/merge.ts:213: the merged hull's faces cleaned for output
+            ev.hull = compactHull(m->pointSets[k].data(), cleanHullFaces(m->pointSets[k].data(), h.tris, m->hullEps),
+                                  nullptr);
             ev.hasHull = true;
           }
         });
@@ -494,6 +510,7 @@ struct Decomposer::State {
   DecomposeResult R;
   std::vector<Part> parts;
   uint64_t nextUid = 1;
+  double hullEps = 2e-5;  // output-hull tolerance from the input's coordinates (src/decompose.ts:230-236)
   bool active = true;
 };
 
@@ -520,7 +537,7 @@ void Decomposer::fillHausdorff(std::vector<State>& S) {
   }
 }
 
-std::vector<Part> Decomposer::makeParts(const std::vector<const Mesh*>& meshes) {
+std::vector<Part> Decomposer::makeParts(const std::vector<const Mesh*>& meshes, const std::vector<double>& hullEps) {
   std::vector<Part> out(meshes.size());
   const size_t batch = 256;
   for (size_t s = 0; s < meshes.size(); s += batch) {
@@ -537,7 +554,8 @@ std::vector<Part> Decomposer::makeParts(const std::vector<const Mesh*>& meshes) 
       const HullResult& h = hulls[q];
       Part& p = out[k];
       p.mesh = *meshes[k];
-      p.hull = compactHull(meshes[k]->verts.data(), h.tris, nullptr);
+      // src/decompose.ts:189: output hulls cleaned; the kernel's volume stays the cost model's
+      p.hull = compactHull(meshes[k]->verts.data(), cleanHullFaces(meshes[k]->verts.data(), h.tris, hullEps[k]), nullptr);
       p.meshVol = volume(*meshes[k]);
       p.hullVol = h.volume;
       p.hasHausdorff = std::isfinite(hd[q]);
@@ -578,6 +596,7 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
       throw std::invalid_argument(
           "input mesh is not closed (open or non-manifold edges); repair or remesh it first, or set requireClosed: false");
     starts[m] = *inputs[m];
+    S[m].hullEps = hullEpsFor(inputs[m]->verts.data(), inputs[m]->nv());
     if (o.normalize) {
       starts[m] = normalize(*inputs[m], &S[m].R.norm);
       S[m].R.normalized = true;
@@ -586,7 +605,9 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
   {
     std::vector<const Mesh*> ms;
     for (const Mesh& m : starts) ms.push_back(&m);
-    std::vector<Part> made = makeParts(ms);
+    std::vector<double> eps;
+    for (const State& x : S) eps.push_back(x.hullEps);
+    std::vector<Part> made = makeParts(ms, eps);
     for (size_t m = 0; m < S.size(); m++) {
       made[m].uid = S[m].nextUid++;
       S[m].parts.push_back(std::move(made[m]));
@@ -760,7 +781,9 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
     }
     st.cutMs += nowMs() - ta0;
     const double th0 = nowMs();
-    std::vector<Part> made = makeParts(halves);
+    std::vector<double> halfEps;
+    for (size_t k : applied) halfEps.insert(halfEps.end(), 2, S[seeds[k].m].hullEps);
+    std::vector<Part> made = makeParts(halves, halfEps);
     st.hullMs += nowMs() - th0;
     // per mesh, in seed order: the positive half replaces the part, the negative half is appended
     for (size_t q = 0; q < applied.size(); q++) {
@@ -792,7 +815,9 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
         lists.push_back(std::move(S[m].parts));
       }
     std::vector<MergeStats> mst;
-    std::vector<std::vector<Part>> merged = mergeHullsMany(be_, std::move(lists), mo, &mst);
+    std::vector<double> listEps;
+    for (uint32_t m : who) listEps.push_back(S[m].hullEps);
+    std::vector<std::vector<Part>> merged = mergeHullsMany(be_, std::move(lists), mo, &mst, &listEps);
     for (size_t q = 0; q < who.size(); q++) {
       S[who[q]].parts = std::move(merged[q]);
       S[who[q]].R.merge = mst[q];

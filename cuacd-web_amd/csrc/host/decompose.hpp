@@ -119,6 +119,7 @@ struct MergeOptions {
   uint32_t maxRounds = 32;
   double planeTol = 1e-3;
   uint32_t verbose = 0;
+  double hullEps = 2e-5;  // src/merge.ts:34-35,48: tolerance of the merged hulls' output cleanup
 };
 
 // src/cpu/hull.ts:156-185 compactHull: first-use vertex order, outward by volume sign
@@ -131,8 +132,9 @@ Mesh filteredConcat(const Mesh& a, const Mesh& b, const std::vector<Plane>& plan
 // src/merge.ts:170-303 mergeHulls
 std::vector<Part> mergeHulls(Backend& be, const std::vector<Part>& input, const MergeOptions& o, MergeStats* stats);
 // the same over several independent parts lists at once, their device work in shared batches
+// (hullEps: each list's output-hull tolerance, else o.hullEps)
 std::vector<std::vector<Part>> mergeHullsMany(Backend& be, std::vector<std::vector<Part>> inputs, const MergeOptions& o,
-                                              std::vector<MergeStats>* stats);
+                                              std::vector<MergeStats>* stats, const std::vector<double>* hullEps = nullptr);
 
 // src/decompose.ts:128-398 Decomposer
 class Decomposer {
@@ -149,7 +151,9 @@ class Decomposer {
  private:
   struct State;
   void fillHausdorff(std::vector<State>& S);
-  std::vector<Part> makeParts(const std::vector<const Mesh*>& meshes);
+  // parts of the meshes: hulls and Hausdorff distances in shared device batches, each output hull
+  // cleaned with its mesh's tolerance
+  std::vector<Part> makeParts(const std::vector<const Mesh*>& meshes, const std::vector<double>& hullEps);
   std::vector<float> refineSpacing(const std::vector<const Mesh*>& meshes) const;
   friend std::vector<DecomposeResult> decomposeLanes(const std::vector<Decomposer*>&, const std::vector<const Mesh*>&);
   Backend& be_;

@@ -2,9 +2,12 @@
 #include "mesh.hpp"
 
 #include <algorithm>
+#include <charconv>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
+#include <regex>
 #include <tuple>
 
 namespace cuacd {
@@ -191,6 +194,215 @@ bool parseBinarySTL(const std::vector<uint8_t>& buf, Mesh* out) {
   }
   *out = weld(soup);
   return true;
+}
+
+namespace {
+
+bool jsSpace(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f'; }
+
+// Number(token) of a token with no white space (ECMAScript StringToNumber): "" is 0; decimal
+// literals with optional sign, exponent, leading or trailing point; Infinity; 0x/0o/0b integers
+double jsNumber(const std::string& t) {
+  if (t.empty()) return 0.0;
+  if (t == "Infinity" || t == "+Infinity") return INFINITY;
+  if (t == "-Infinity") return -INFINITY;
+  if (t.size() > 2 && t[0] == '0' && (t[1] == 'x' || t[1] == 'X' || t[1] == 'o' || t[1] == 'O' || t[1] == 'b' || t[1] == 'B')) {
+    const int base = (t[1] == 'x' || t[1] == 'X') ? 16 : ((t[1] == 'o' || t[1] == 'O') ? 8 : 2);
+    double v = 0;
+    for (size_t i = 2; i < t.size(); i++) {
+      const char c = t[i];
+      int d = c >= '0' && c <= '9' ? c - '0' : (c >= 'a' && c <= 'f' ? c - 'a' + 10 : (c >= 'A' && c <= 'F' ? c - 'A' + 10 : 99));
+      if (d >= base) return NAN;
+      v = v * base + d;
+    }
+    return v;
+  }
+  static const std::regex dec("[+-]?(([0-9]+\\.?[0-9]*)|(\\.[0-9]+))([eE][+-]?[0-9]+)?");
+  if (!std::regex_match(t, dec)) return NAN;
+  return std::strtod(t.c_str(), nullptr);
+}
+
+// parseInt(s, 10): leading white space, an optional sign, the longest run of digits; NaN without digits
+double jsParseInt(const std::string& s) {
+  size_t i = 0;
+  while (i < s.size() && jsSpace(s[i])) i++;
+  bool neg = false;
+  if (i < s.size() && (s[i] == '+' || s[i] == '-')) neg = s[i++] == '-';
+  double v = 0;
+  size_t d0 = i;
+  while (i < s.size() && s[i] >= '0' && s[i] <= '9') v = v * 10 + (s[i++] - '0');
+  if (i == d0) return NAN;
+  return neg ? -v : v;
+}
+
+// ToUint32: what a Uint32Array stores for a number
+uint32_t jsToUint32(double x) {
+  if (!std::isfinite(x)) return 0;
+  const double t = std::trunc(x);
+  const double m = std::fmod(t, 4294967296.0);
+  return (uint32_t)(int64_t)(m < 0 ? m + 4294967296.0 : m);
+}
+
+std::vector<std::string> splitWs(const std::string& s) {  // s.split(/\s+/)
+  std::vector<std::string> out(1);
+  size_t i = 0;
+  while (i < s.size()) {
+    if (jsSpace(s[i])) {
+      while (i < s.size() && jsSpace(s[i])) i++;
+      out.emplace_back();
+    } else {
+      out.back().push_back(s[i++]);
+    }
+  }
+  return out;
+}
+
+std::string jsTrim(const std::string& s) {
+  size_t a = 0, b = s.size();
+  while (a < b && jsSpace(s[a])) a++;
+  while (b > a && jsSpace(s[b - 1])) b--;
+  return s.substr(a, b - a);
+}
+
+Mesh weldDoubles(const std::vector<double>& pos) {  // src/mesh.ts:196-212 over number[] positions
+  std::map<std::tuple<uint64_t, uint64_t, uint64_t>, uint32_t> idx;
+  Mesh m;
+  for (size_t i = 0; i + 2 < pos.size(); i += 3) {
+    uint64_t k[3];
+    for (int a = 0; a < 3; a++) {
+      const double x = pos[i + a] == 0.0 ? 0.0 : pos[i + a];  // "-0" and "0" print alike
+      std::memcpy(&k[a], &x, 8);
+    }
+    const auto key = std::make_tuple(k[0], k[1], k[2]);
+    auto it = idx.find(key);
+    uint32_t id;
+    if (it == idx.end()) {
+      id = m.nv();
+      idx.emplace(key, id);
+      for (int a = 0; a < 3; a++) m.verts.push_back((float)pos[i + a]);
+    } else {
+      id = it->second;
+    }
+    m.tris.push_back(id);
+  }
+  return m;
+}
+
+}  // namespace
+
+bool parseSTL(const std::vector<uint8_t>& buf, Mesh* out) {
+  const size_t hn = std::min<size_t>(80, buf.size());
+  std::string head(buf.begin(), buf.begin() + hn);
+  size_t h0 = 0;
+  while (h0 < head.size() && jsSpace(head[h0])) h0++;
+  bool looksBinary = false;
+  if (buf.size() >= 84) {
+    uint32_t n;
+    std::memcpy(&n, buf.data() + 80, 4);
+    looksBinary = (uint64_t)84 + (uint64_t)n * 50 == buf.size();
+  }
+  const bool isBinary = !(head.compare(h0, 5, "solid") == 0 && !looksBinary);
+  if (isBinary) {
+    if (buf.size() < 84) return false;
+    uint32_t n;
+    std::memcpy(&n, buf.data() + 80, 4);
+    if (n && (uint64_t)84 + (uint64_t)(n - 1) * 50 + 48 > buf.size()) return false;  // a triangle past the end
+    std::vector<float> soup;
+    soup.reserve((size_t)n * 9);
+    size_t off = 84;
+    for (uint32_t i = 0; i < n; i++) {
+      off += 12;
+      for (int k = 0; k < 9; k++) {
+        float f;
+        std::memcpy(&f, buf.data() + off + 4 * k, 4);
+        soup.push_back(f);
+      }
+      off += 38;
+    }
+    *out = weld(soup);
+    return true;
+  }
+  const std::string text(buf.begin(), buf.end());
+  static const std::regex re("vertex\\s+([-+0-9.eE]+)\\s+([-+0-9.eE]+)\\s+([-+0-9.eE]+)");
+  std::vector<double> pos;
+  for (auto it = std::sregex_iterator(text.begin(), text.end(), re); it != std::sregex_iterator(); ++it)
+    for (int k = 1; k <= 3; k++) pos.push_back(jsNumber((*it)[k].str()));
+  *out = weldDoubles(pos);
+  return true;
+}
+
+Mesh parseOBJ(const std::string& text) {
+  Mesh m;
+  std::vector<uint32_t> tris;
+  size_t at = 0;
+  while (at <= text.size()) {
+    size_t e = text.find('\n', at);
+    if (e == std::string::npos) e = text.size();
+    const std::string line = text.substr(at, e - at);
+    at = e + 1;
+    if (line.compare(0, 2, "v ") == 0) {
+      const std::vector<std::string> p = splitWs(line);
+      for (int k = 1; k <= 3; k++) m.verts.push_back((float)(k < (int)p.size() ? jsNumber(p[k]) : NAN));
+    } else if (line.compare(0, 2, "f ") == 0) {
+      const std::vector<std::string> tok = splitWs(jsTrim(line));
+      std::vector<double> p;
+      for (size_t i = 1; i < tok.size(); i++) p.push_back(jsParseInt(tok[i].substr(0, tok[i].find('/'))) - 1);
+      for (size_t i = 1; i + 1 < p.size(); i++) {
+        m.tris.push_back(jsToUint32(p[0]));
+        m.tris.push_back(jsToUint32(p[i]));
+        m.tris.push_back(jsToUint32(p[i + 1]));
+      }
+    }
+  }
+  return m;
+}
+
+std::string jsNumberString(double x) {
+  if (std::isnan(x)) return "NaN";
+  if (x == 0) return "0";
+  if (std::isinf(x)) return x > 0 ? "Infinity" : "-Infinity";
+  std::string sign = x < 0 ? "-" : "";
+  char buf[64];
+  const auto r = std::to_chars(buf, buf + sizeof(buf), std::fabs(x), std::chars_format::scientific);
+  const std::string sci(buf, r.ptr);  // d[.ddd]e[+-]xx: the shortest round-trip digits
+  const size_t ep = sci.find('e');
+  std::string digits;
+  for (size_t i = 0; i < ep; i++)
+    if (sci[i] != '.') digits.push_back(sci[i]);
+  const int k = (int)digits.size();
+  const int n = std::atoi(sci.c_str() + ep + 1) + 1;  // the value is 0.digits x 10^n
+  std::string s;
+  if (k <= n && n <= 21) {
+    s = digits + std::string(n - k, '0');
+  } else if (0 < n && n <= 21) {
+    s = digits.substr(0, n) + "." + digits.substr(n);
+  } else if (-6 < n && n <= 0) {
+    s = "0." + std::string(-n, '0') + digits;
+  } else {
+    const int e = n - 1;
+    s = digits.substr(0, 1);
+    if (k > 1) s += "." + digits.substr(1);
+    s += std::string("e") + (e >= 0 ? "+" : "-") + std::to_string(std::abs(e));
+  }
+  return sign + s;
+}
+
+std::string toOBJ(const std::vector<const Mesh*>& parts) {
+  if (parts.empty()) return "\n";  // [].join("\n") + "\n"
+  std::string out;
+  uint64_t base = 1;
+  for (size_t i = 0; i < parts.size(); i++) {
+    const Mesh& m = *parts[i];
+    out += "o part_" + std::to_string(i) + "\n";
+    for (size_t k = 0; k + 2 < m.verts.size(); k += 3)
+      out += "v " + jsNumberString(m.verts[k]) + " " + jsNumberString(m.verts[k + 1]) + " " +
+             jsNumberString(m.verts[k + 2]) + "\n";
+    for (size_t k = 0; k + 2 < m.tris.size(); k += 3)
+      out += "f " + std::to_string(m.tris[k] + base) + " " + std::to_string(m.tris[k + 1] + base) + " " +
+             std::to_string(m.tris[k + 2] + base) + "\n";
+    base += m.nv();
+  }
+  return out;
 }
 
 }  // namespace cuacd

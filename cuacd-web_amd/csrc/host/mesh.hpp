@@ -54,5 +54,17 @@ Mesh denormalize(const Mesh& m, const Normalization& n);
 // src/mesh.ts:166-186 parseSTL (binary STL; vertices welded by exact position, src/mesh.ts:195-212)
 bool parseBinarySTL(const std::vector<uint8_t>& buf, Mesh* out);
 Mesh weld(const std::vector<float>& soup);
+// src/mesh.ts:166-193 parseSTL, binary or ASCII: binary unless the header starts with "solid"
+// (after leading white space) and the size is not that of a binary file; ASCII vertices are
+// welded by their double value before rounding to f32 (src/mesh.ts:196-212). false on a
+// truncated binary file.
+bool parseSTL(const std::vector<uint8_t>& buf, Mesh* out);
+// src/mesh.ts:215-229 parseOBJ: "v " and "f " lines, polygons fan-triangulated, indices as
+// JavaScript's parseInt gives them (1-based; "i/j/k" takes i), coordinates as Number() does
+Mesh parseOBJ(const std::string& text);
+// src/mesh.ts:231-242 toOBJ: "o part_<i>", v and f lines, numbers printed as JavaScript prints them
+std::string toOBJ(const std::vector<const Mesh*>& parts);
+// JavaScript's Number.prototype.toString() of a double (shortest round trip, ECMAScript layout)
+std::string jsNumberString(double x);
 
 }  // namespace cuacd

@@ -112,6 +112,7 @@ def concave_edge_planes(m, n_concave_edges=16, concave_eps=0.005, concave_thresh
 
 
 def parse_stl(data: bytes):
+    """parseSTL (src/mesh.ts:166-193): binary or ASCII STL, welded by exact position."""
     api = load_product()
     buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
     h = C.c_void_p()
@@ -129,3 +130,57 @@ def generate(kind, seed, params=()):
     h = C.c_void_p()
     check(api, api.generate(k, seed, p, len(params), C.byref(h)))
     return _take_meshes(api, h)[0]
+
+
+def parse_obj(text):
+    """parseOBJ (src/mesh.ts:215-229): v/f lines, polygons fan-triangulated."""
+    api = load_product()
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    h = C.c_void_p()
+    check(api, api.parse_obj(data, len(data), C.byref(h)))
+    return _take_meshes(api, h)[0]
+
+
+def to_obj(parts):
+    """toOBJ (src/mesh.ts:231-242): one "o part_<i>" object per mesh, numbers printed as JavaScript prints them."""
+    from .gputree import _batch
+    api = load_product()
+    b = _batch(list(parts))
+    n = C.c_uint64()
+    check(api, api.to_obj(C.byref(b), None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    check(api, api.to_obj(C.byref(b), buf, n.value, C.byref(n)))
+    return buf.raw[: n.value].decode()
+
+
+def clean_hull_faces(verts, tris, eps=2e-5):
+    """cleanHullFaces (src/cpu/hull.ts:87-154): the cleaned faces (uint32 (n, 3)) of an exact hull `tris` over `verts`."""
+    api = load_product()
+    v = np.ascontiguousarray(np.asarray(verts, dtype=np.float32).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.uint32).reshape(-1, 3))
+    cap = 2 * v.shape[0] + t.shape[0] + 8
+    out = np.zeros((cap, 3), dtype=np.uint32)
+    n = u32()
+    check(api, api.clean_hull_faces(v.ctypes.data_as(C.POINTER(f32)), v.shape[0], t.ctypes.data_as(C.POINTER(u32)),
+                                    t.shape[0], eps, out.ctypes.data_as(C.POINTER(u32)), cap, C.byref(n)))
+    assert n.value <= cap
+    return out[: n.value].copy()
+
+
+def compact_hull(verts, tris):
+    """compactHull (src/cpu/hull.ts:156-185): (hull Mesh over the used vertices, volume)."""
+    api = load_product()
+    v = np.ascontiguousarray(np.asarray(verts, dtype=np.float32).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.uint32).reshape(-1, 3))
+    hv = np.zeros((3 * max(t.shape[0], 1), 3), dtype=np.float32)
+    ht = np.zeros((max(t.shape[0], 1), 3), dtype=np.uint32)
+    nv, vol = u32(), f64()
+    check(api, api.compact_hull(v.ctypes.data_as(C.POINTER(f32)), v.shape[0], t.ctypes.data_as(C.POINTER(u32)), t.shape[0],
+                                hv.ctypes.data_as(C.POINTER(f32)), ht.ctypes.data_as(C.POINTER(u32)), C.byref(nv),
+                                C.byref(vol)))
+    return Mesh(hv[: nv.value].copy(), ht[: t.shape[0]].copy()), vol.value
+
+
+def hull_eps(m):
+    """The output-hull tolerance of a decomposition of mesh m (src/decompose.ts:230-236)."""
+    return load_product().hull_eps(m.verts.ctypes.data_as(C.POINTER(f32)), m.nv)

@@ -270,8 +270,26 @@ int cuacd_concave_edge_planes_batch(cuacd_tree* t, const cuacd_mesh_batch* meshe
                                     uint32_t* n_planes);
 int32_t cuacd_is_closed(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt);
 double cuacd_signed_volume(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt);
-/* binary STL, welded by exact position (src/mesh.ts:166-212) */
+/* parseSTL(buf) (src/mesh.ts:166-193), binary or ASCII, welded by exact position (src/mesh.ts:196-212) */
 int cuacd_parse_stl(const uint8_t* buf, uint64_t len, cuacd_meshes** out);
+/* parseOBJ(text) (src/mesh.ts:215-229): v/f lines, polygons fan-triangulated; *out gets one mesh */
+int cuacd_parse_obj(const char* text, uint64_t len, cuacd_meshes** out);
+/* toOBJ(parts) (src/mesh.ts:231-242): the text, its first `cap` bytes into out (NUL not written),
+ * *len its full length (call with cap = 0 to size the buffer) */
+int cuacd_to_obj(const cuacd_mesh_batch* parts, char* out, uint64_t cap, uint64_t* len);
+/* cleanHullFaces(V, T, eps) (src/cpu/hull.ts:87-154): the faces of an exact hull over `verts` rebuilt in
+ * double precision with tolerance eps (the reference's CPU-hull convention, src/cpu/hull.ts:19-20); out
+ * (room for nt_cap faces) receives the cleaned faces, or the raw ones when the rebuild is rejected;
+ * *nt_out their count (never more than the raw count plus the faces of a hull of its vertices) */
+int cuacd_clean_hull_faces(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, double eps,
+                           uint32_t* out, uint32_t nt_cap, uint32_t* nt_out);
+/* compactHull(hv, tris) (src/cpu/hull.ts:156-185): the used vertices in first-use order into hull_verts
+ * (room for 3 * 3 * nt floats suffices), the faces re-indexed and wound outward into hull_tris (3 * nt),
+ * *nv_out the vertex count, *volume the enclosed volume */
+int cuacd_compact_hull(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, float* hull_verts,
+                       uint32_t* hull_tris, uint32_t* nv_out, double* volume);
+/* the output-hull tolerance of a decomposition of this input (src/decompose.ts:230-236) */
+double cuacd_hull_eps(const float* verts, uint32_t nv);
 /* synthetic meshes: kind 3 = mesh params[0] of the uneven batch mix with seed base `seed`;
  * kind 0 staircase (steps), 1 blobs (n_blobs, grid), 2 torus (major, minor,
  * twist), all closed, float32, normalized into the unit cube [0,1]^3 */

@@ -53,3 +53,24 @@ extern "C" __attribute__((visibility("default"))) int ref_qh_hull(const double* 
   for (size_t i = 0; i < v.size() && i < 3ull * vcap; i++) vertsOut[i] = v[i];
   return (int)(t.size() / 3);
 }
+
+// The same call with the hull's triangles: Kuukka's vertex buffer (hull vertices only, in its own
+// order) into vertsOut (3 doubles each, at most vcap vertices), triangles indexing it into trisOut
+// (at most tcap); returns the triangle count. The reference's cleanHullFaces maps these vertices back
+// to its candidates by coordinates (src/cpu/hull.ts:120-131).
+extern "C" __attribute__((visibility("default"))) int ref_qh_tris(const double* pts, int n, double eps, uint32_t* trisOut,
+                                                                  uint32_t tcap, double* vertsOut, uint32_t vcap,
+                                                                  uint32_t* nvOut) {
+  quickhull::QuickHull<double> qh;
+  auto hull = qh.getConvexHull(pts, (size_t)n, true, false, eps);
+  const auto& ib = hull.getIndexBuffer();
+  const auto& vb = hull.getVertexBuffer();
+  *nvOut = (uint32_t)vb.size();
+  for (size_t i = 0; i < vb.size() && i < vcap; i++) {
+    vertsOut[3 * i] = vb[i].x;
+    vertsOut[3 * i + 1] = vb[i].y;
+    vertsOut[3 * i + 2] = vb[i].z;
+  }
+  for (size_t i = 0; i < ib.size() && i < 3ull * tcap; i++) trisOut[i] = (uint32_t)ib[i];
+  return (int)(ib.size() / 3);
+}

@@ -89,7 +89,32 @@ def test_merge_l_boxes_stay_apart(cw, oracle):
     # tests/merge.test.ts:54-62
     boxes = [cw.make_box((0, 0, 0), (1, 1, 1)), cw.make_box((1, 0, 0), (2, 1, 0.5))]
     r = cw.merge_hulls(boxes, threshold=0.05)
+    _same(r, cw.merge_hulls(boxes, api=oracle, threshold=0.05))
     assert len(r.parts) == 2 and r.merge["pairs_hulled"] == 1 and r.merge["merges"] == 0
+
+
+def test_merge_many_parts(cw, oracle):
+    """A 4 x 4 x 2 grid of blocks cut from a box (32 parts, 496 pairs in the first round): the pairs the
+    device's pair pass keeps (rv lower bound and box gap, one launch per round) and every later
+    decision equal the oracle's host loop; adjacent blocks merge over several rounds, blocks far apart
+    are never hulled together."""
+    parts = [cw.make_box((0, 0, 0), (2, 2, 1))]
+    for axis, cuts in ((0, (0.5, 1.0, 1.5)), (1, (0.5, 1.0, 1.5)), (2, (0.5,))):
+        for c in cuts:
+            nxt = []
+            for p in parts:
+                plane = [0.0, 0.0, 0.0, -c]
+                plane[axis] = 1.0
+                r = cw.plane_cut(p, plane)
+                nxt += [r[1], r[0]] if r else [p]
+            parts = nxt
+    assert len(parts) == 32
+    rg = cw.merge_hulls(parts, threshold=0.05)
+    ro = cw.merge_hulls(parts, api=oracle, threshold=0.05)
+    _same(rg, ro)
+    assert rg.merge["rounds"] >= 3 and rg.merge["merges"] >= 16
+    assert 32 <= rg.merge["pairs_hulled"] < 496  # the far pairs were rejected without a hull
+    assert abs(sum(p.mesh_vol for p in rg.parts) - 4.0) < 1e-9
 
 
 def test_stl_49160_runs(cw, oracle):

@@ -102,7 +102,9 @@ def test_blob_depth3(cw, oracle):
     planes = [cw.concave_edge_planes(m, n_concave_edges=8, seed=3)]
     rg, checked = compare_runs(cw, oracle, [m], planes, refine=dict(rounds=1, probes=4, spacing=[0.05]), width=6,
                                width2=3, depth=3, n_concave_edges=8, min_edge_dist=0.0125)
-    assert rg.pruned > 0 or checked > 0
+    # the last level's branch-and-bound prunes about half of its hulls (538 of 1,092 in the oracle),
+    # and every kept part of levels 1-2 was compared above
+    assert rg.pruned > 0.3 * rg.hulls_done and checked > 100
 
 
 def flat_cone(n_rim):

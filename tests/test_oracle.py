@@ -225,6 +225,19 @@ def test_hull_against_qhull_on_clouds(cw, oracle):
         assert cw.is_closed(m) and cw.signed_volume(m) > 0
 
 
+def test_hull_against_qhull_on_blob_surfaces(cw, oracle):
+    """Smooth closed surfaces (many nearly coplanar points): the exported hull's vertex set is qhull's
+    exactly (334 of 1,332, 881 of 3,688 and 1,046 of 7,050 points) and its volume the polytope's."""
+    from scipy.spatial import ConvexHull
+    for params, nh in (([3, 32, -0.6, 0.6, 0.12, 0.30, 0.5], 334), ([6, 40], 881), ([12, 48], 1046)):
+        blob = cw.generate("blobs", 20260200, params)
+        tris, vol = _hull(cw, oracle, blob.verts)
+        h = ConvexHull(blob.verts.astype(np.float64))
+        assert set(int(x) for x in tris.reshape(-1)) == set(int(x) for x in h.vertices)
+        assert len(h.vertices) == nh
+        assert abs(vol - h.volume) < 1e-9 * h.volume
+
+
 def test_hull_point_seeing_the_whole_base(cw, oracle):
     """A cone over a 3000-point circle plus a point just below the base's centre: that point goes in
     last and sees all 2998 base faces at once (nearly half the hull). The hull's vertices are qhull's
