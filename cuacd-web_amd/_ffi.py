@@ -14,7 +14,7 @@ class TreeConfig(C.Structure):
                 ("min_edge_dist", f32), ("hull_max_verts", u32),
                 ("vertex_cap", u64), ("tri_cap", u64), ("edge_cap", u64), ("scratch_words", u64),
                 ("hull_budget_bytes", u64), ("prune_last_level", i32), ("hausdorff_samples", u32),
-                ("device", i32), ("hull_wg_min_verts", u32)]
+                ("device", i32), ("hull_wg_min_verts", u32), ("use_graph", u32)]
 
 
 class MeshBatch(C.Structure):
@@ -47,6 +47,17 @@ class PartInfo(C.Structure):
                 ("n_cand", u32), ("n_ins", u32), ("pad", u32), ("mesh_vol", f64), ("hull_vol", f64)]
 
 
+class Progress(C.Structure):
+    """src/decompose.ts:70-82 ProgressInfo (cuacd_progress)."""
+    _fields_ = [("phase", i32), ("iteration", i32), ("n_parts", u32), ("n_cutting", u32), ("worst_cost", f64),
+                ("fraction", f64), ("message", C.c_char_p), ("gpu_ms", f64), ("cpu_ms", f64), ("lane", u32),
+                ("pad", u32)]
+
+
+PROGRESS_FN = C.CFUNCTYPE(None, P(Progress), C.c_void_p)
+PHASES = ["start", "select", "tree", "cut", "hull", "iteration", "merge", "done"]  # src/decompose.ts:68
+
+
 class DecomposeOptions(C.Structure):
     _fields_ = [("threshold", f64), ("max_iters", u32), ("width", u32), ("width2", u32), ("depth", u32),
                 ("max_n_cutting", u32), ("n_concave_edges", u32), ("concave_eps", f64), ("concave_threshold", f64),
@@ -55,7 +66,8 @@ class DecomposeOptions(C.Structure):
                 ("merge_plane_tol", f64), ("verbose", u32), ("hausdorff_samples", u32),
                 ("vertex_cap", u64), ("tri_cap", u64), ("edge_cap", u64), ("scratch_words", u64),
                 ("hull_budget_bytes", u64), ("hull_max_verts", u32), ("prune_last_level", i32), ("device", i32),
-                ("hull_wg_min_verts", u32), ("batch_lanes", u32)]
+                ("hull_wg_min_verts", u32), ("batch_lanes", u32), ("tree_graph", u32), ("on_progress", PROGRESS_FN),
+                ("progress_user", C.c_void_p), ("abort_flag", P(i32))]
 
 
 class PartDesc(C.Structure):
@@ -80,14 +92,14 @@ KERNEL_NAMES = ["plan", "scan", "emit", "hull", "hull_big", "record", "evaluate"
 class KernelStats(C.Structure):
     _fields_ = [("ms", f64 * NUM_KERNELS), ("launches", u64 * NUM_KERNELS), ("face_tests", u64),
                 ("emit_bytes", u64), ("hd_evals", u64), ("hull_jobs", u64), ("face_tests_big", u64),
-                ("hull_bytes", u64), ("hull_bytes_big", u64)]
+                ("hull_bytes", u64), ("hull_bytes_big", u64), ("graph_ms", f64), ("graphs", u64)]
 
     def as_dict(self):
         return dict(ms={n: self.ms[i] for i, n in enumerate(KERNEL_NAMES)},
                     launches={n: self.launches[i] for i, n in enumerate(KERNEL_NAMES)},
                     face_tests=self.face_tests, face_tests_big=self.face_tests_big, emit_bytes=self.emit_bytes,
                     hd_evals=self.hd_evals, hull_jobs=self.hull_jobs, hull_bytes=self.hull_bytes,
-                    hull_bytes_big=self.hull_bytes_big)
+                    hull_bytes_big=self.hull_bytes_big, graph_ms=self.graph_ms, graphs=self.graphs)
 
 
 class CutRecord(C.Structure):
@@ -152,7 +164,7 @@ HOST_API = {
 }
 
 # error codes (include/cuacd.h)
-OK, ERR_ARG, ERR_SCRATCH, ERR_DEVICE, ERR_RUNTIME = 0, 1, 2, 3, 4
+OK, ERR_ARG, ERR_SCRATCH, ERR_DEVICE, ERR_RUNTIME, ERR_ABORTED = 0, 1, 2, 3, 4, 5
 
 
 class Api:

@@ -35,6 +35,12 @@ def load_product():
     return _API
 
 
+class AbortError(CuacdError):
+    """The call saw its abort flag set (src/decompose.ts:238-240 AbortError)."""
+
+
 def check(api, code):
+    if code == 5:
+        raise AbortError(code, api.error())
     if code != 0:
         raise CuacdError(code, api.error())

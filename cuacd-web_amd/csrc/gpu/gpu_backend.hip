@@ -148,11 +148,13 @@ class GpuBackend : public Backend {
     return e;
   }
   hipEvent_t tBegin(hipStream_t s = nullptr) {
+    if (capturing_) return nullptr;  // events captured into a graph cannot be timed
     hipEvent_t a = event();
     hipCheck(hipEventRecord(a, s ? s : stream_), "hipEventRecord");
     return a;
   }
   void tEnd(int k, hipEvent_t a, hipStream_t s = nullptr) {
+    if (!a) return;
     hipEvent_t b = event();
     hipCheck(hipEventRecord(b, s ? s : stream_), "hipEventRecord");
     pending_.push_back(Pending{k, a, b});
@@ -191,6 +193,10 @@ class GpuBackend : public Backend {
   void ensurePools();
   // one tree level: plans, counts, child pools, geometry and volumes (kernels of k_level.hpp)
   void runLevel(const Level& L, const dk::LevelParams& P);
+  // the per-level tables and hull queues of a level of nJobs jobs, allocated ahead (runLevel and
+  // levelHulls then allocate nothing, as graph capture requires)
+  void reserveLevel(uint32_t nJobs);
+  bool capturing_ = false;  // a tree run is being captured into a graph
   template <typename S, typename T>
   void scan(S s, uint32_t maxElems, T* vals);
   // hull the given jobs (slab offsets are assigned here, in arena-sized chunks)
@@ -355,21 +361,40 @@ void GpuBackend::scan(S s, uint32_t maxElems, T* vals) {
   launched("device scan");
 }
 
-void GpuBackend::runLevel(const Level& L, const dk::LevelParams& P) {
-  // tiles per job: up to 64, fewer when the level has many jobs, so a level has about
-  // TILE_CAP tiles at most (or one per job)
-  const uint32_t maxTiles = std::max<uint32_t>(1, std::min<uint32_t>(64, dk::TILE_CAP / std::max<uint32_t>(L.nJobs, 1)));
-  const uint64_t tileBound = (uint64_t)L.nJobs * maxTiles;
+namespace {
+// tiles per job: up to 64, fewer when the level has many jobs, so a level has about TILE_CAP tiles
+// at most (or one per job)
+uint32_t maxTilesOf(uint32_t nJobs) {
+  return std::max<uint32_t>(1, std::min<uint32_t>(64, dk::TILE_CAP / std::max<uint32_t>(nJobs, 1)));
+}
+}  // namespace
+
+void GpuBackend::reserveLevel(uint32_t nJobs) {
+  const uint64_t tileBound = (uint64_t)nJobs * maxTilesOf(nJobs);
   if (tileBound >= 0xffffffffull / dk::TC_N) throw std::length_error("tree level exceeds the tile scratch");
-  const uint64_t maxElems = std::max<uint64_t>(tileBound, L.nJobs);
+  const uint64_t maxElems = std::max<uint64_t>(tileBound, nJobs);
   const uint64_t nChunks = (maxElems + dk::SCAN_CH - 1) / dk::SCAN_CH + 1;
-  jobTiles_.ensure(L.nJobs);
-  jobStart_.ensure(L.nJobs + 1ull);
+  jobTiles_.ensure(nJobs);
+  jobStart_.ensure(nJobs + 1ull);
   tileCnt_.ensure(tileBound * dk::TC_N);
   tilePre_.ensure((tileBound + 1) * dk::TC_N);
   scanStatus_.ensure(nChunks);
   scanVals_.ensure(2 * nChunks * dk::TC_N);
-  vacc_.ensure(2ull * dk::VOL_LIMBS * L.nJobs);
+  vacc_.ensure(2ull * dk::VOL_LIMBS * nJobs);
+  const uint32_t nChild = 2 * nJobs;
+  hq_.ensure(1);
+  hqTag_.ensure(nChild);
+  hqRank_.ensure(nChild);
+  hqSmall_.ensure(nChild);
+  hqBig_.ensure(nChild);
+  hresSmall_.ensure(nChild);
+  hresBig_.ensure(nChild);
+}
+
+void GpuBackend::runLevel(const Level& L, const dk::LevelParams& P) {
+  const uint32_t maxTiles = maxTilesOf(L.nJobs);
+  const uint64_t tileBound = (uint64_t)L.nJobs * maxTiles;
+  reserveLevel(L.nJobs);
   constexpr uint32_t GRID = 2048;  // tile kernels: 8192 waves over the level's tiles
   hipEvent_t e0 = tBegin();
   dk::k_lvl_items<<<L.nItemsIn, dk::LT, 0, stream_>>>(P, items_.p, parts_.p, jobs_.p, jobPlanes_.p, edgePlanes_.p,
@@ -465,13 +490,7 @@ constexpr uint32_t HULL_GRID_BIG = 512, HULL_GRID_SMALL_WAVES = 4096;
 
 void GpuBackend::levelHulls(const Level& L, const dk::HullLevel& HL) {
   const uint32_t nChild = 2 * L.nJobs;
-  hq_.ensure(1);
-  hqTag_.ensure(nChild);
-  hqRank_.ensure(nChild);
-  hqSmall_.ensure(nChild);
-  hqBig_.ensure(nChild);
-  hresSmall_.ensure(nChild);
-  hresBig_.ensure(nChild);
+  reserveLevel(L.nJobs);
   if (!arenaCtl_.p) ensureArena(0);
   const uint32_t blocks = (nChild + 255) / 256;
   dk::k_hq_reset<<<1, 64, 0, stream_>>>(hq_.p);
@@ -649,11 +668,30 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
     h2d(edgePlanes_.p, ep.data(), ep.size() * sizeof(float4), "upload edge planes");
     h2d(ctr_.p, &c0, offsetof(dk::Counters, hd), "init counters");
     streamed_.clear();
+    // A graph run allocates every per-level table first (for each level, and each chunk size of
+    // the last), then captures the stages on the stream and launches them as one graph.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    hipEvent_t g0 = nullptr, g1 = nullptr;
+    if (cfg_.useGraph) {
+      for (const std::vector<Level>* lv : {&treeLevels, &probeLevels})
+        for (uint32_t l = 0; l < lv->size(); l++) {
+          const Level& L = (*lv)[l];
+          reserveLevel(L.nJobs);
+          const uint32_t nc = (l == depth - 1 && l > 0) ? std::min(chunks, L.nItemsIn) : 1;
+          if (nc > 1) reserveLevel(((L.nItemsIn + nc - 1) / nc) * L.totalWidth);
+        }
+      hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      capturing_ = true;
+    }
+    try {
     // ---- stages: the tree, then each refinement round over the probe layout
     for (uint32_t s = 0; s < stages; s++) {
       const std::vector<Level>& levels = s == 0 ? treeLevels : probeLevels;
-      if (s > 0)  // pool pointers back to the seeds; flags and counts accumulate
-        h2d(ctr_.p, &c0, 3 * sizeof(uint32_t), "reset counters");
+      if (s > 0) {  // pool pointers back to the seeds; flags and counts accumulate
+        dk::k_pool_set<<<1, 64, 0, stream_>>>(ctr_.p, c0.vTop, c0.tTop, c0.eTop);
+        launched("k_pool_set");
+      }
       for (uint32_t l = 0; l < depth; l++) {
         const Level& L = levels[l];
         const dk::LevelParams P = levelParams(L, l, depth, nSeeds);
@@ -721,12 +759,41 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       lastLevelCounts_.assign(1, nSeeds);
       for (const Level& L : levels) lastLevelCounts_.push_back(L.nJobs);
     }
+    } catch (...) {
+      if (capturing_) {
+        capturing_ = false;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+      }
+      throw;
+    }
+    if (cfg_.useGraph) {
+      capturing_ = false;
+      hipCheck(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
+      hipCheck(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      g0 = event();
+      g1 = event();
+      hipCheck(hipEventRecord(g0, stream_), "hipEventRecord");
+      hipCheck(hipGraphLaunch(graphExec, stream_), "hipGraphLaunch");
+      hipCheck(hipEventRecord(g1, stream_), "hipEventRecord");
+    }
     hipCheck(hipMemcpyAsync(st.data(), state_.p, nSeeds * sizeof(dk::SeedState), hipMemcpyDeviceToHost, stream_),
              "read seed state");
     hipCheck(hipMemcpyAsync(&c1, ctr_.p, offsetof(dk::Counters, hd), hipMemcpyDeviceToHost, stream_), "read counters");
     dk::ScanCtl sc;
     hipCheck(hipMemcpyAsync(&sc, scanCtl_.p, sizeof(sc), hipMemcpyDeviceToHost, stream_), "read scan state");
     sync();
+    if (graphExec) {
+      float ms = 0.f;
+      hipCheck(hipEventElapsedTime(&ms, g0, g1), "hipEventElapsedTime");
+      kst_.graphMs += ms;
+      kst_.graphs++;
+      evPool_.push_back(g0);
+      evPool_.push_back(g1);
+      hipCheck(hipGraphExecDestroy(graphExec), "hipGraphExecDestroy");
+      hipCheck(hipGraphDestroy(graph), "hipGraphDestroy");
+    }
     if (sc.error | sc.sticky) {
       hipCheck(hipMemset(scanCtl_.p, 0, sizeof(dk::ScanCtl)), "clear scan control");
       throw DeviceError("device scan: a chunk's look-back waited past its limit");

@@ -973,6 +973,15 @@ __global__ void k_pool_mark(Counters* __restrict__ ctr) {
     ctr->markE = ctr->eTop;
   }
 }
+// pool tops back to the seeds' at the start of a refinement round (a kernel, not a copy, so the
+// round can sit in a captured graph)
+__global__ void k_pool_set(Counters* __restrict__ ctr, uint32_t vTop, uint32_t tTop, uint32_t eTop) {
+  if (threadIdx.x == 0) {
+    ctr->vTop = vTop;
+    ctr->tTop = tTop;
+    ctr->eTop = eTop;
+  }
+}
 __global__ void k_pool_rewind(Counters* __restrict__ ctr) {
   if (threadIdx.x == 0) {
     ctr->vTop = ctr->markV;

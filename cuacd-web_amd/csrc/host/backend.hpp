@@ -38,6 +38,9 @@ struct TreeConfig {
   // Parts with at least this many points get a workgroup in the hull kernel, the
   // rest a wavefront. Tree volumes and exported face lists are the same either way.
   uint32_t hullWgMinVerts = 1024;
+  // Each tree run captured into one HIP graph and launched once (per-kernel event timing is then
+  // not available: the run is timed as a whole). Results do not depend on it.
+  bool useGraph = false;
 };
 
 // A HIP runtime failure (reported as CUACD_ERR_DEVICE at the C ABI).
@@ -146,6 +149,8 @@ struct KernelStats {
   uint64_t hullJobs = 0;
   uint64_t hullBytes = 0;     // wave-per-part hull: algorithmic HBM bytes (12 nv + 16 + 12 per exported face)
   uint64_t hullBytesBig = 0;  // workgroup-per-part hull: the same
+  double graphMs = 0;         // tree runs captured as HIP graphs: their device time, as a whole
+  uint64_t graphs = 0;
   void add(const KernelStats& o) {
     for (int k = 0; k < K_COUNT; k++) {
       ms[k] += o.ms[k];
@@ -158,6 +163,8 @@ struct KernelStats {
     hullJobs += o.hullJobs;
     hullBytes += o.hullBytes;
     hullBytesBig += o.hullBytesBig;
+    graphMs += o.graphMs;
+    graphs += o.graphs;
   }
   // the work since an earlier snapshot of the same counters
   KernelStats since(const KernelStats& before) const {
@@ -173,6 +180,8 @@ struct KernelStats {
     d.hullJobs -= before.hullJobs;
     d.hullBytes -= before.hullBytes;
     d.hullBytesBig -= before.hullBytesBig;
+    d.graphMs -= before.graphMs;
+    d.graphs -= before.graphs;
     return d;
   }
 };

@@ -42,6 +42,7 @@ TreeConfig treeConfigFor(const DecomposeOptions& o) {
   if (o.pruneLastLevel >= 0) c.pruneLastLevel = o.pruneLastLevel != 0;
   if (o.hullWgMinVerts) c.hullWgMinVerts = o.hullWgMinVerts;
   c.hausdorffSamples = o.hausdorffSamples;
+  c.useGraph = o.treeGraph;
   return c;
 }
 
@@ -273,6 +274,7 @@ std::vector<std::vector<Part>> mergeHullsMany(Backend& be, std::vector<std::vect
   const size_t batch = 1024;
   double hullMs = 0, hdMs = 0;
   for (uint32_t round = 0; round < o.maxRounds; round++) {
+    if (o.abortFlag && *o.abortFlag) throw AbortedError("merge aborted");
     std::vector<MergeRun*> act;
     for (MergeRun& m : R)
       if (!m.done && m.parts.size() > 1) act.push_back(&m);
@@ -328,6 +330,15 @@ std::vector<std::vector<Part>> mergeHullsMany(Backend& be, std::vector<std::vect
         m->pointSets[k].insert(m->pointSets[k].end(), B.hull.verts.begin(), B.hull.verts.end());
       }
     }
+    uint32_t roundParts = 0, roundCand = 0;
+    if (o.onRound) {
+      for (MergeRun* m : act) {
+        roundParts += (uint32_t)m->parts.size();
+        roundCand += (uint32_t)m->candidates.size();
+      }
+      o.onRound(round, roundParts, roundCand, -1);
+    }
+    uint32_t roundMerges = 0;
     const double th0 = nowMs();
     {
       std::vector<std::pair<MergeRun*, size_t>> items;
@@ -468,6 +479,7 @@ std::vector<std::vector<Part>> mergeHullsMany(Backend& be, std::vector<std::vect
       if (o.verbose)
         std::fprintf(stderr, "[merge round %u] parts=%zu candidates=%zu hulled=%zu hausdorff=%zu merged=%u\n", round,
                      parts.size(), m->candidates.size(), m->todo.size(), m->hdIdx.size(), merges);
+      roundMerges += merges;
       if (merges == 0) {
         m->done = true;
         continue;
@@ -477,6 +489,7 @@ std::vector<std::vector<Part>> mergeHullsMany(Backend& be, std::vector<std::vect
       parts.swap(next);
       m->stats.merges += merges;
     }
+    if (o.onRound) o.onRound(round, roundParts - roundMerges, roundCand, (int)roundMerges);
   }
   std::vector<std::vector<Part>> out(R.size());
   const double total = nowMs() - t0;
@@ -590,6 +603,54 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
   poolGrowths_ = 0;
   std::vector<State> S(inputs.size());
   DecomposeStats st;  // the batch's counters and times
+  // src/decompose.ts:238-252 checkAbort and report: a monotone estimate from the worst part cost (log
+  // scale from the first iteration's worst cost down to the threshold) and the iteration count, with
+  // fixed sub-steps inside an iteration. A batch reports its totals (parts, cutting seeds) and its
+  // worst cost over every mesh.
+  double worst0 = 0, lastFrac = 0;
+  auto checkAbort = [&]() {
+    if (o.abortFlag && *o.abortFlag) throw AbortedError("decomposition aborted");
+  };
+  auto estimate = [&](double worst, uint32_t iter) {
+    double f = (double)iter / std::max<uint32_t>(1, o.maxIters);
+    if (worst0 > o.threshold && worst > 0)
+      f = std::max(f, std::log(worst0 / std::max(worst, o.threshold)) / std::log(worst0 / o.threshold));
+    return std::min(1.0, std::max(0.0, f));
+  };
+  auto report = [&](int phase, int iteration, size_t nParts, size_t nCutting, double worst, double frac,
+                    const std::string& msg) {
+    lastFrac = std::max(lastFrac, std::min(1.0, frac));
+    if (!o.onProgress) return;
+    ProgressInfo pi;
+    pi.phase = phase;
+    pi.iteration = iteration;
+    pi.nParts = (uint32_t)nParts;
+    pi.nCutting = (uint32_t)nCutting;
+    pi.worstCost = worst;
+    pi.fraction = lastFrac;
+    pi.message = msg;
+    pi.gpuMs = st.treeMs + st.hullMs + st.hausdorffMs;
+    pi.cpuMs = st.cutMs + st.concaveMs;
+    pi.lane = lane_;
+    if (progressMu_) {
+      std::lock_guard<std::mutex> g(*progressMu_);
+      o.onProgress(pi);
+    } else {
+      o.onProgress(pi);
+    }
+  };
+  auto totalParts = [&]() {
+    size_t n = 0;
+    for (const State& x : S) n += x.parts.size();
+    return n;
+  };
+  auto fmt4 = [](double x) {
+    char b[64];
+    std::snprintf(b, sizeof(b), "%.4f", x);
+    return std::string(b);
+  };
+  checkAbort();
+  report(PH_START, -1, 0, 0, 0, 0.01, "hull of the input");
   std::vector<Mesh> starts(inputs.size());
   for (size_t m = 0; m < inputs.size(); m++) {
     if (o.requireClosed && !isClosed(*inputs[m]))
@@ -619,7 +680,10 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
     std::vector<float> edge;
     SeedResult res;
   };
+  uint32_t iterations = 0;
   for (uint32_t iter = 0; iter < o.maxIters; iter++) {
+    iterations = iter + 1;
+    checkAbort();
     const double tc0 = nowMs();
     fillHausdorff(S);
     st.hausdorffMs += nowMs() - tc0;
@@ -648,7 +712,22 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
       }
       for (size_t k = skip; k < above.size(); k++) seeds.push_back(Seed{m, above[k], {}, {}});
     }
-    if (seeds.empty()) break;
+    double worstAll = 0;
+    for (uint32_t m = 0; m < S.size(); m++) worstAll = std::max(worstAll, worst[m]);
+    if (iter == 0) worst0 = worstAll;
+    // this iteration spans from the current estimate part of the way to the end
+    const double est = estimate(worstAll, iter);
+    const double span = 0.87 * std::max(0.02, (1 - est) / 3);
+    const double base = std::max(lastFrac, 0.03 + 0.87 * est);
+    const size_t nParts = totalParts();
+    if (seeds.empty()) {
+      report(PH_SELECT, (int)iter, nParts, 0, worstAll, base,
+             "iteration " + std::to_string(iter) + ": " + std::to_string(nParts) + " parts, every part is under the threshold");
+      break;
+    }
+    report(PH_SELECT, (int)iter, nParts, seeds.size(), worstAll, base,
+           "iteration " + std::to_string(iter) + ": " + std::to_string(nParts) + " parts, worst cost " + fmt4(worstAll) +
+               ", cutting " + std::to_string(seeds.size()));
     const double tcc = nowMs();
     if (iter < o.concaveIters && o.nConcaveEdges > 0) {
       ConcaveOptions co;
@@ -673,6 +752,9 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
       }
     }
     st.concaveMs += nowMs() - tcc;
+    checkAbort();
+    report(PH_TREE, (int)iter, nParts, seeds.size(), worstAll, base + 0.1 * span,
+           "iteration " + std::to_string(iter) + ": lookahead search over " + std::to_string(seeds.size()) + " parts");
     // tree search over every mesh's seeds, halving the batch on pool overflow
     size_t batchN = treeBatch_ ? std::min(treeBatch_, seeds.size()) : seeds.size(), start = 0;
     while (start < seeds.size()) {
@@ -738,6 +820,9 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
       for (size_t k = start; k < e; k++) seeds[k].res = res.seeds[k - start];
       start = e;
     }
+    checkAbort();
+    report(PH_CUT, (int)iter, nParts, seeds.size(), worstAll, base + 0.6 * span,
+           "iteration " + std::to_string(iter) + ": cutting " + std::to_string(seeds.size()) + " parts");
     // apply the winning cuts on the host (worker threads), then hull all new halves in shared batches
     const double ta0 = nowMs();
     std::vector<CutResult> cuts(seeds.size());
@@ -780,6 +865,10 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
       s.R.cuts.push_back(rec);
     }
     st.cutMs += nowMs() - ta0;
+    checkAbort();
+    report(PH_HULL, (int)iter, nParts, seeds.size(), worstAll, base + 0.8 * span,
+           "iteration " + std::to_string(iter) + ": hulls and Hausdorff distances of " + std::to_string(halves.size()) +
+               " halves");
     const double th0 = nowMs();
     std::vector<double> halfEps;
     for (size_t k : applied) halfEps.insert(halfEps.end(), 2, S[seeds[k].m].hullEps);
@@ -793,6 +882,8 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
       s.parts[seeds[applied[q]].pi] = std::move(made[2 * q]);
       s.parts.push_back(std::move(made[2 * q + 1]));
     }
+    report(PH_ITERATION, (int)iter, totalParts(), seeds.size(), worstAll, base + span,
+           "iteration " + std::to_string(iter) + " done: " + std::to_string(totalParts()) + " parts");
     if (o.verbose)
       for (uint32_t m = 0; m < S.size(); m++)
         if (S[m].active)
@@ -801,12 +892,21 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
   fillHausdorff(S);
   if (o.mergeHulls) {
     // every mesh with more than one part merges, the lists sharing device batches
+    checkAbort();
     const double tm0 = nowMs();
+    const double mergeBase = std::max(lastFrac, 0.9);
+    report(PH_MERGE, (int)iterations, totalParts(), 0, 0, mergeBase, "merging hulls: " + std::to_string(totalParts()) + " parts");
     MergeOptions mo;
     mo.threshold = o.threshold;
     mo.maxRounds = o.mergeRounds;
     mo.planeTol = o.mergePlaneTol;
     mo.verbose = o.verbose;
+    mo.abortFlag = o.abortFlag;
+    mo.onRound = [&](uint32_t round, uint32_t np, uint32_t cand, int merged) {
+      report(PH_MERGE, (int)iterations, np, 0, 0, mergeBase + (1 - mergeBase) * (1 - std::pow(0.5, round + 1)),
+             "merge round " + std::to_string(round) + ": " + std::to_string(np) + " parts, " + std::to_string(cand) +
+                 " candidate pairs" + (merged >= 0 ? ", " + std::to_string(merged) + " merged" : ""));
+    };
     std::vector<uint32_t> who;
     std::vector<std::vector<Part>> lists;
     for (uint32_t m = 0; m < S.size(); m++)
@@ -836,6 +936,11 @@ std::vector<DecomposeResult> Decomposer::decomposeBatch(const std::vector<const 
       }
     }
     s.R.parts = std::move(s.parts);
+  }
+  {
+    size_t n = 0;
+    for (const State& x : S) n += x.R.parts.size();
+    report(PH_DONE, (int)iterations, n, 0, 0, 1, "done: " + std::to_string(n) + " parts");
   }
   st.totalMs = nowMs() - tStart;
   const KernelStats ks = be_.kernelStats().since(ks0);
@@ -877,6 +982,11 @@ std::vector<DecomposeResult> decomposeLanes(const std::vector<Decomposer*>& lane
   if (lanes.size() == 1 || inputs.size() < 2) return lanes[0]->decomposeBatch(inputs);
   const double t0 = nowMs();
   const std::vector<std::vector<uint32_t>> split = splitLanes(inputs, (uint32_t)lanes.size());
+  std::mutex progressMu;
+  for (size_t l = 0; l < lanes.size(); l++) {
+    lanes[l]->lane_ = (uint32_t)l;
+    lanes[l]->progressMu_ = &progressMu;
+  }
   std::vector<std::vector<DecomposeResult>> res(lanes.size());
   std::vector<std::exception_ptr> err(lanes.size());
   auto work = [&](size_t l) {
@@ -894,6 +1004,7 @@ std::vector<DecomposeResult> decomposeLanes(const std::vector<Decomposer*>& lane
   for (size_t l = 1; l < lanes.size(); l++) th.emplace_back(work, l);
   work(0);
   for (std::thread& x : th) x.join();
+  for (size_t l = 0; l < lanes.size(); l++) lanes[l]->progressMu_ = nullptr;
   for (const std::exception_ptr& e : err)
     if (e) std::rethrow_exception(e);
   // the call's stats: the engines' counts and times summed, the call's wall time

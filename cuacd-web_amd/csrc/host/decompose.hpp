@@ -1,6 +1,8 @@
 // decompose.hpp — the iteration driver and the merge pass (host), mirroring
 // src/decompose.ts, src/merge.ts and src/part.ts.
 #pragma once
+#include <functional>
+#include <mutex>
 #include <array>
 #include <cstdint>
 #include <map>
@@ -38,6 +40,24 @@ inline double partCost(const Part& p) {
   return rv > hd ? rv : hd;
 }
 
+// src/decompose.ts:68-82 ProgressPhase / ProgressInfo (phases in the reference's order)
+enum ProgressPhase { PH_START = 0, PH_SELECT, PH_TREE, PH_CUT, PH_HULL, PH_ITERATION, PH_MERGE, PH_DONE };
+struct ProgressInfo {
+  int phase = PH_START;
+  int iteration = -1;
+  uint32_t nParts = 0, nCutting = 0;
+  double worstCost = 0, fraction = 0;
+  std::string message;
+  double gpuMs = 0, cpuMs = 0;
+  uint32_t lane = 0;  // the engine reporting, for a batch split over several engines
+};
+using ProgressFn = std::function<void(const ProgressInfo&)>;
+
+// the caller's abort flag was set (src/decompose.ts:238-240, src/merge.ts:189 AbortError)
+struct AbortedError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
 // src/decompose.ts:26-108 DecomposeOptions / DEFAULT_OPTIONS (the GPU is the only backend)
 struct DecomposeOptions {
   double threshold = 0.05;
@@ -66,6 +86,9 @@ struct DecomposeOptions {
   int pruneLastLevel = -1;  // -1 = default (on)
   uint32_t hullWgMinVerts = 0;  // 0 = default (TreeConfig::hullWgMinVerts)
   uint32_t batchLanes = 0;      // engines a batch is split over; 0 = default (2 for a batch)
+  bool treeGraph = false;       // each tree run captured into one HIP graph (TreeConfig::useGraph)
+  ProgressFn onProgress;                   // src/decompose.ts:63 onProgress
+  const volatile int32_t* abortFlag = nullptr;  // src/decompose.ts:62 signal: nonzero aborts at the next boundary
 };
 
 // TreeConfig a Decomposer runs with (src/decompose.ts:141-148)
@@ -120,6 +143,10 @@ struct MergeOptions {
   double planeTol = 1e-3;
   uint32_t verbose = 0;
   double hullEps = 2e-5;  // src/merge.ts:34-35,48: tolerance of the merged hulls' output cleanup
+  const volatile int32_t* abortFlag = nullptr;  // src/merge.ts:189: checked at every round
+  // src/merge.ts:205,279 onRound(round, nParts, candidates, merged; -1 before the round's matching),
+  // for the lists of a mergeHullsMany call summed
+  std::function<void(uint32_t, uint32_t, uint32_t, int)> onRound;
 };
 
 // src/cpu/hull.ts:156-185 compactHull: first-use vertex order, outward by volume sign
@@ -159,6 +186,8 @@ class Decomposer {
   Backend& be_;
   DecomposeOptions o_;
   size_t treeBatch_ = 0;  // largest seed count that last fit the tree pools (0: unknown)
+  uint32_t lane_ = 0;          // this engine's index in a batch split over several
+  std::mutex* progressMu_ = nullptr;  // serializes the engines' progress callbacks
   uint32_t poolGrowths_ = 0;  // pool doublings after a single part overflowed (at most 4)
 };
 

@@ -71,13 +71,49 @@ class DecomposeOptions:
     device: int = 0
     hull_wg_min_verts: int = 0  # hull kernel split (0: 1024); a performance knob, results unchanged
     batch_lanes: int = 0  # engines a batch is split over, run at once (0: 2); results unchanged
+    tree_graph: bool = False  # each tree run captured into one HIP graph; results unchanged
+    on_progress: object = None  # src/decompose.ts:63 onProgress: called with a dict of ProgressInfo fields
+    abort: object = None  # src/decompose.ts:62 signal: an AbortFlag; set() stops the call with AbortError
 
     def raw(self):
         r = F.DecomposeOptions()
         for f in fields(self):
+            if f.name in ("on_progress", "abort"):
+                continue
             v = getattr(self, f.name)
             setattr(r, f.name, int(v) if isinstance(v, bool) else v)
+        keep = []
+        if self.on_progress is not None:
+            fn = self.on_progress
+
+            def tramp(p, _user):
+                q = p.contents
+                fn(dict(phase=F.PHASES[q.phase], iteration=q.iteration, n_parts=q.n_parts, n_cutting=q.n_cutting,
+                        worst_cost=q.worst_cost, fraction=q.fraction, message=q.message.decode(), gpu_ms=q.gpu_ms,
+                        cpu_ms=q.cpu_ms, lane=q.lane))
+            cb = F.PROGRESS_FN(tramp)
+            r.on_progress = cb
+            keep.append(cb)
+        if self.abort is not None:
+            r.abort_flag = C.pointer(self.abort.flag)
+            keep.append(self.abort)
+        r._keep = keep  # the callback and the flag live as long as the options that point at them
         return r
+
+
+class AbortFlag:
+    """An abort signal for decompose calls (src/decompose.ts:62 AbortSignal): set() makes the call stop at
+    its next iteration, phase or merge-round boundary with AbortError."""
+
+    def __init__(self):
+        self.flag = C.c_int32(0)
+
+    def set(self):
+        self.flag.value = 1
+
+    @property
+    def aborted(self):
+        return bool(self.flag.value)
 
 
 DEFAULT_OPTIONS = DecomposeOptions()
@@ -169,6 +205,7 @@ class Decomposer:
         h = C.c_void_p()
         check(self.api, self.api.decomposer_create(C.byref(raw), batch_tris, C.byref(h)))
         self._h = h
+        self._raw = raw  # the engine keeps the progress callback and abort flag these point at
 
     def decompose(self, mesh):
         h = C.c_void_p()

@@ -21,6 +21,7 @@ extern "C" {
 #define CUACD_ERR_SCRATCH 2  /* capacity error; message contains "scratch" (src/merge.ts:251) */
 #define CUACD_ERR_DEVICE 3   /* HIP runtime failure or no GPU */
 #define CUACD_ERR_RUNTIME 4  /* pool overflow with a single part, np overflow, open input */
+#define CUACD_ERR_ABORTED 5  /* the caller's abort flag was set (src/decompose.ts:238-240 AbortError) */
 
 typedef struct cuacd_tree cuacd_tree;
 typedef struct cuacd_result cuacd_result;
@@ -40,6 +41,10 @@ typedef struct cuacd_tree_config {
    * rest a wavefront; 0 = 1024. A tuning knob with no reference counterpart:
    * results do not depend on it. */
   uint32_t hull_wg_min_verts;
+  /* 1: each tree run (all levels, hulls and refinement rounds) is captured into one HIP graph and
+   * launched once, as the reference encodes a run into one command buffer (src/gpu/tree.ts:559-606);
+   * 0: the same kernels launched one by one, each timed with HIP events. Results are the same. */
+  uint32_t use_graph;
 } cuacd_tree_config;
 
 /* A batch of meshes packed as the reference uploads them (src/gpu/tree.ts:240-262):
@@ -149,6 +154,28 @@ int cuacd_hausdorff_noalloc(const cuacd_mesh_batch* a, const cuacd_mesh_batch* b
 /* rvCost(meshVol, hullVol) (src/part.ts:24-26), the FP64 cost the tree's record kernel evaluates */
 double cuacd_rv_cost(double mesh_vol, double hull_vol);
 
+/* src/decompose.ts:68-82 ProgressPhase / ProgressInfo */
+#define CUACD_PHASE_START 0
+#define CUACD_PHASE_SELECT 1
+#define CUACD_PHASE_TREE 2
+#define CUACD_PHASE_CUT 3
+#define CUACD_PHASE_HULL 4
+#define CUACD_PHASE_ITERATION 5
+#define CUACD_PHASE_MERGE 6
+#define CUACD_PHASE_DONE 7
+typedef struct cuacd_progress {
+  int32_t phase;     /* what is about to run (ITERATION and DONE: what just finished) */
+  int32_t iteration; /* -1 at START */
+  uint32_t n_parts, n_cutting;
+  double worst_cost;
+  double fraction;   /* monotone estimate in [0, 1] */
+  const char* message;
+  double gpu_ms, cpu_ms;
+  uint32_t lane;     /* the engine reporting, for a batch split over several engines (0 otherwise) */
+  uint32_t pad;
+} cuacd_progress;
+typedef void (*cuacd_progress_fn)(const cuacd_progress* p, void* user);
+
 /* src/decompose.ts:26-108 DecomposeOptions (GPU backends only) */
 typedef struct cuacd_decompose_options {
   double threshold;
@@ -168,6 +195,13 @@ typedef struct cuacd_decompose_options {
   uint32_t hull_wg_min_verts; /* as in cuacd_tree_config; 0 = 1024 */
   uint32_t batch_lanes; /* engines a batch is split over, run at once so that one engine's host phases
                            overlap another's device work; results do not depend on it; 0 = 2 */
+  uint32_t tree_graph;  /* as cuacd_tree_config.use_graph */
+  cuacd_progress_fn on_progress;      /* src/decompose.ts:63 onProgress; NULL: none. Called on the calling
+                                         thread, or on an engine's thread for a batch (serialized) */
+  void* progress_user;
+  const volatile int32_t* abort_flag; /* src/decompose.ts:62 signal: when *abort_flag becomes nonzero the call
+                                         stops at the next iteration, phase or merge-round boundary and
+                                         returns CUACD_ERR_ABORTED; NULL: never */
 } cuacd_decompose_options;
 
 /* src/part.ts:9-22 Part, flattened */
@@ -208,6 +242,9 @@ typedef struct cuacd_kernel_stats {
   uint64_t face_tests_big; /* the workgroup-per-part hull kernel's share */
   uint64_t hull_bytes;     /* wave-per-part hull: algorithmic HBM bytes, 12 nv + 16 (+ 12 per exported face) */
   uint64_t hull_bytes_big; /* workgroup-per-part hull: the same */
+  double graph_ms;         /* tree runs captured as HIP graphs (use_graph): their device time as a whole
+                              (their kernels are then not in ms[]) */
+  uint64_t graphs;
 } cuacd_kernel_stats;
 /* accumulated over the life of the handle */
 int cuacd_tree_kernel_stats(cuacd_tree* t, cuacd_kernel_stats* s);

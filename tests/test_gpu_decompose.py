@@ -140,3 +140,15 @@ def test_hull_kernel_split_does_not_change_results(cw):
     for split in (1, 1 << 30):  # every part in the workgroup kernel, then every part in the wave kernel
         print(f"hull_wg_min_verts={split}", flush=True)
         _same(ref, cw.decompose(mesh, hull_wg_min_verts=split, **opts))
+
+
+def test_decompose_tree_graph(cw):
+    """decompose with every tree run captured into a HIP graph (tree_graph) equals the eager
+    decomposition."""
+    mesh = cw.generate("blobs", 20260210, [6, 40, -0.6, 0.6, 0.12, 0.30, 0.5])
+    opts = dict(threshold=0.05, width=12, width2=6, depth=3, n_concave_edges=8, refine_rounds=2, refine_probes=8,
+                merge_hulls=True, hausdorff_samples=500)
+    ref = cw.decompose(mesh, **opts)
+    got = cw.decompose(mesh, tree_graph=True, **opts)
+    _same(ref, got)
+    assert got.kernels["graphs"] == got.stats["tree_runs"] and ref.kernels["graphs"] == 0

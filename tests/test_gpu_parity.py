@@ -263,3 +263,42 @@ def test_pool_overflow_names_the_pool(cw):
     assert r.overflow_pools == 0 and r.hulls_done == ref.hulls_done
     for a, b in zip(r.seeds, ref.seeds):
         assert (a.best_cut, a.best_cost, a.plane) == (b.best_cut, b.best_cost, b.plane)
+
+
+def test_tree_graph_equals_eager(cw):
+    """A tree run captured into one HIP graph (all levels, hull queues, refinement rounds; use_graph)
+    gives the eager run's tables, cuts and part volumes, run after run with eager runs in between; a
+    run whose last level overflows the vertex pool is captured again in chunks and still matches."""
+    meshes = [cw.normalize(m)[0] for m in (stairs(cw), square_torus(cw), l_prism(cw))]
+    meshes.append(cw.normalize(cw.generate("blobs", 20260201, [5, 20]))[0])
+    planes = [cw.concave_edge_planes(m, n_concave_edges=8, seed=5 + i) for i, m in enumerate(meshes)]
+    refine = dict(rounds=2, probes=8, spacing=[0.05, 0.08, 0.1, 0.05])
+    cfg = dict(width=12, width2=6, depth=3, n_concave_edges=8, min_edge_dist=0.0125)
+    eager, graph = cw.GpuTree(**cfg), cw.GpuTree(use_graph=True, **cfg)
+    want = eager.run(meshes, planes, refine=refine)
+    nparts = len(meshes) + 2 * sum(want.jobs_per_level)
+    vols = [eager.read_part(p, geometry=False) for p in range(nparts)]
+    for k in range(3):
+        print(f"graph run {k}", flush=True)
+        got = graph.run(meshes, planes, refine=refine)
+        assert [(s.best_cut, s.best_cost, s.plane) for s in got.seeds] == [(s.best_cut, s.best_cost, s.plane) for s in want.seeds]
+        assert got.levels == want.levels and got.hulls_done == want.hulls_done and got.pruned == want.pruned
+        for p in range(0, nparts, 7):
+            q = graph.read_part(p, geometry=False)
+            assert (q["mesh_vol"], q["hull_vol"], q["nv"]) == (vols[p]["mesh_vol"], vols[p]["hull_vol"], vols[p]["nv"])
+        again = eager.run(meshes, planes, refine=refine)  # eager work between the graph runs
+        assert again.levels == want.levels
+    ks = graph.kernel_stats()
+    assert ks["graphs"] == 3 and ks["graph_ms"] > 0 and ks["launches"]["emit"] == 0
+    # the chunked last level: the first capture overflows, the run is captured again in chunks (the
+    # pool sized from the tree stage's own parts, read after a run without refinement)
+    eager.run(meshes, planes)
+    first_last = nparts - 2 * want.jobs_per_level[-1]
+    base = sum(eager.read_part(p, geometry=False)["nv"] for p in range(first_last))
+    last = sum(eager.read_part(p, geometry=False)["nv"] for p in range(first_last, nparts))
+    small = cw.GpuTree(use_graph=True, vertex_cap=3 * (base + last // 3), **cfg)
+    got = small.run(meshes, planes, refine=refine)
+    assert got.last_level_chunks > 1 and not got.overflow
+    assert [(s.best_cut, s.best_cost, s.plane) for s in got.seeds] == [(s.best_cut, s.best_cost, s.plane) for s in want.seeds]
+    for t in (eager, graph, small):
+        t.destroy()
