@@ -120,6 +120,8 @@ TREE_API = {
     "read_part": (C.c_int, [C.c_void_p, u32, P(PartInfo), P(f32), u32, P(u32), u32]),
     "hull_meshes": (C.c_int, [C.c_void_p, P(MeshBatch), P(f64), P(u32)]),
     "hulls_with_hausdorff": (C.c_int, [C.c_void_p, P(MeshBatch), P(f64), P(u32), P(f64)]),
+    "hull_meshes_opts": (C.c_int, [C.c_void_p, P(MeshBatch), u32, i32, P(f64), P(u32)]),
+    "tree_part_vols": (C.c_int, [C.c_void_p, P(u32), P(u32), P(f64), P(f64), u32]),
     "last_faces": (P(u32), [C.c_void_p, u32, P(u32)]),
     "hausdorff": (C.c_int, [C.c_void_p, P(MeshBatch), P(MeshBatch), P(f64)]),
     "sample_mesh": (C.c_int, [C.c_void_p, P(f32), u32, P(u32), u32, P(f32), u64, P(u64)]),

@@ -111,6 +111,8 @@ class GpuBackend : public Backend {
   TreeRunResult run(const std::vector<MeshRef>& seeds, const std::vector<std::vector<float>>& edgePlanes,
                     const RunOptions& opts) override;
   std::vector<HullResult> hullMeshes(const std::vector<MeshRef>& meshes) override;
+  std::vector<HullResult> hullMeshesStaged(const std::vector<MeshRef>& meshes, uint32_t stage, bool noExport) override;
+  void partVolumes(uint32_t first, uint32_t n, double* meshVol, double* hullVol) override;
   std::vector<double> hausdorff(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b) override;
   std::vector<HullResult> hullsWithHausdorff(const std::vector<MeshRef>& meshes, std::vector<double>* hd) override;
   std::vector<float> sampleMesh(const MeshRef& m) override;
@@ -201,7 +203,8 @@ class GpuBackend : public Backend {
   void scan(S s, uint32_t maxElems, T* vals);
   // hull the given jobs (slab offsets are assigned here, in arena-sized chunks)
   // exported hulls of host meshes: the jobs in queue order, results in the same order
-  void runHulls(std::vector<dk::HullJob>& hj, uint32_t* exportBuf, std::vector<dk::HullResultD>* results);
+  void runHulls(std::vector<dk::HullJob>& hj, uint32_t* exportBuf, std::vector<dk::HullResultD>* results,
+                uint32_t stage = 0);
   // every child of a tree level with enough points: queues built on the device
   void levelHulls(const Level& L, const dk::HullLevel& HL);
   // the persistent hull kernels over the queues in hq_ (wavefront queue on the side stream)
@@ -516,7 +519,8 @@ void GpuBackend::levelHulls(const Level& L, const dk::HullLevel& HL) {
   launchHulls(false, parts_.p, nullptr, HL, (uint32_t)gBig, (uint32_t)gSmall);
 }
 
-void GpuBackend::runHulls(std::vector<dk::HullJob>& hj, uint32_t* exportBuf, std::vector<dk::HullResultD>* results) {
+void GpuBackend::runHulls(std::vector<dk::HullJob>& hj, uint32_t* exportBuf, std::vector<dk::HullResultD>* results,
+                          uint32_t stage) {
   results->assign(hj.size(), dk::HullResultD{});
   if (hj.empty()) return;
   // queues by size class, largest first (stable), split at hullWgMinVerts
@@ -559,6 +563,7 @@ void GpuBackend::runHulls(std::vector<dk::HullJob>& hj, uint32_t* exportBuf, std
   h2d(hq_.p->q, q, sizeof(q), "hull queues");
   dk::HullLevel HL;
   std::memset(&HL, 0, sizeof(HL));
+  HL.stage = stage;
   launchHulls(true, nullptr, exportBuf, HL, (uint32_t)gBig, (uint32_t)gSmall);
   kst_.hullJobs += hj.size();
   std::vector<dk::HullResultD> rb(big.size()), rs(small.size());
@@ -860,6 +865,20 @@ void GpuBackend::mergePairs(const std::vector<cuacd::MergePart>& parts, const st
   sync();
 }
 
+void GpuBackend::partVolumes(uint32_t first, uint32_t n, double* meshVol, double* hullVol) {
+  if ((uint64_t)first + n > lastPartCount_) throw std::invalid_argument("part index out of range");
+  std::vector<dk::DPart> p(n);
+  if (n) {
+    hipCheck(hipMemcpyAsync(p.data(), parts_.p + first, n * sizeof(dk::DPart), hipMemcpyDeviceToHost, stream_),
+             "read parts");
+    sync();
+  }
+  for (uint32_t k = 0; k < n; k++) {
+    meshVol[k] = p[k].meshVol;
+    hullVol[k] = p[k].hullVol;
+  }
+}
+
 std::vector<ItemView> GpuBackend::levelItems(uint32_t level) {
   std::vector<ItemView> out;
   if (level >= lastLevelCounts_.size()) return out;
@@ -936,6 +955,12 @@ void GpuBackend::uploadMeshes(const std::vector<MeshRef>& meshes, std::vector<ui
 }
 
 std::vector<HullResult> GpuBackend::hullMeshes(const std::vector<MeshRef>& meshes) {
+  return hullMeshesStaged(meshes, 0, false);
+}
+
+std::vector<HullResult> GpuBackend::hullMeshesStaged(const std::vector<MeshRef>& meshes, uint32_t stage,
+                                                     bool noExport) {
+  if (stage > 3) throw std::invalid_argument("hullStage must be 0..3");
   std::vector<HullResult> out(meshes.size());
   if (meshes.empty()) return out;
   lastLevelCounts_.clear();
@@ -956,24 +981,26 @@ std::vector<HullResult> GpuBackend::hullMeshes(const std::vector<MeshRef>& meshe
     J.part = (uint32_t)k;
     J.vOff = vOff[k];
     J.nv = meshes[k].nv;
-    J.exportOff = (uint32_t)exOff[k];
+    J.exportOff = noExport ? dk::NONE : (uint32_t)exOff[k];
     J.slabOff = 0;
     J.slabWords = 0;
     J.job = dk::NONE;
   }
   std::vector<dk::HullResultD> res;
   clearWork();
-  runHulls(hj, export_.p, &res);
+  runHulls(hj, export_.p, &res, stage);
   collectWork();
-  std::vector<uint32_t> faces(exportTotal);
-  hipCheck(hipMemcpyAsync(faces.data(), export_.p, exportTotal * 4, hipMemcpyDeviceToHost, stream_), "read hull faces");
+  std::vector<uint32_t> faces(noExport ? 0 : exportTotal);
+  if (!noExport)
+    hipCheck(hipMemcpyAsync(faces.data(), export_.p, exportTotal * 4, hipMemcpyDeviceToHost, stream_), "read hull faces");
   sync();
   for (size_t k = 0; k < meshes.size(); k++) {
     HullResult& h = out[k];
     h.ok = res[k].status <= 1;
     h.failCode = res[k].status == 3 ? 3 : 0;
     h.volume = h.ok ? res[k].volume : -1.0;
-    if (h.ok) h.tris.assign(faces.begin() + exOff[k], faces.begin() + exOff[k] + 3ull * res[k].nFaces);
+    h.nFaces = res[k].nFaces;
+    if (h.ok && !noExport) h.tris.assign(faces.begin() + exOff[k], faces.begin() + exOff[k] + 3ull * res[k].nFaces);
   }
   return out;
 }

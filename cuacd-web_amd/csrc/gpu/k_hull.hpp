@@ -1255,6 +1255,9 @@ __device__ __forceinline__ void hull_grow(const Grp& g, HullCtx& C, HullBatch<EX
 
 struct HullLevel {  // branch-and-bound inputs at the last tree level
   uint32_t prune, itemsInBase, nItemsIn, nSeeds, depth, jobBase;
+  // stop after a stage, for timing (src/gpu/tree.ts:382 hullStage): 1 the seeds, 2 the seed hull,
+  // 3 the outside points assigned; 0 the whole hull. A stopped hull reports the grid volume so far.
+  uint32_t stage;
 };
 
 // one hull job on one group; seed points in `seed` (80 entries of group-shared memory).
@@ -1474,6 +1477,9 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
     }
     if (i3 == NONE) {
       done = true;  // all seeds coplanar: volume 0
+    } else if (HL.stage == 1) {
+      done = true;
+      res.status = 0;
     } else {
       uint32_t a = i0, b = i1, c = i2;
       if (bs > 0) {
@@ -1501,6 +1507,11 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
       hull_grow(g, C, S);
       if (phase == 2) break;
       res.status = 0;
+      if (HL.stage == 2) {
+        res.volume = hull_volume(g, C);
+        done = true;
+        break;
+      }
       if (HL.prune) {
         // branch-and-bound: rough-hull rv lower bound vs the seed's best upper bound
         const double vRough = hull_volume(g, C);
@@ -1639,6 +1650,10 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
         C.ncand = nc;
         res.nCand = nc;
         g.sync();
+        if (HL.stage == 3) {
+          res.volume = hull_volume(g, C);
+          done = true;
+        }
       }
       if (done) break;
       }

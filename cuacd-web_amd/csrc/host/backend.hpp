@@ -122,6 +122,7 @@ struct HullResult {
   double volume = 0;
   std::vector<uint32_t> tris;  // outward triangles over the mesh's own vertex indices
   uint32_t failCode = 0;
+  uint32_t nFaces = 0;  // faces of the hull (also when they are not exported)
 };
 
 // Device time and algorithmic work per kernel family, accumulated over the
@@ -238,6 +239,27 @@ class Backend {
       for (uint32_t i = off[l]; i < off[l + 1]; i++)
         for (uint32_t j = i + 1; j < off[l + 1]; j++)
           pass->push_back(merge_pair_passes(parts[i], parts[j], threshold, halfT2) ? 1 : 0);
+  }
+  // src/gpu/tree.ts:382 hullMeshes(meshes, {hullStage, noExport}): stop after a stage (1 the seeds,
+  // 2 the seed hull, 3 the outside points assigned; 0 whole hulls), for timing; noExport leaves the
+  // faces on the device (tris empty, nFaces counted). A backend without stages offers only stage 0.
+  virtual std::vector<HullResult> hullMeshesStaged(const std::vector<MeshRef>& meshes, uint32_t stage, bool noExport) {
+    if (stage != 0) throw std::invalid_argument("hull stages are not available on this backend");
+    std::vector<HullResult> r = hullMeshes(meshes);
+    for (HullResult& h : r) {
+      h.nFaces = (uint32_t)(h.tris.size() / 3);
+      if (noExport) h.tris.clear();
+    }
+    return r;
+  }
+  // src/gpu/tree.ts:604-659 readParts: mesh and hull volumes of parts [first, first + n) of the last run
+  virtual void partVolumes(uint32_t first, uint32_t n, double* meshVol, double* hullVol) {
+    for (uint32_t k = 0; k < n; k++) {
+      PartView pv;
+      if (!readPart(first + k, &pv, nullptr)) throw std::invalid_argument("part index out of range");
+      meshVol[k] = pv.meshVol;
+      hullVol[k] = pv.hullVol;
+    }
   }
   // last run's items of one level (0 = seeds) and part descriptors
   virtual std::vector<ItemView> levelItems(uint32_t level) = 0;

@@ -76,6 +76,7 @@ class TreeRunResult:
     levels: list = field(default_factory=list)
     overflow_pools: int = 0  # bits: 1 scratch, 2 vertices, 4 triangles, 8 edges, 32 hull arena
     last_level_chunks: int = 1  # chunks of input items the last level ran in (1: one pass)
+    part_vols: tuple = None  # (base, (n, 2) mesh and hull volumes) of the level-0 children, when read_parts
 
 
 class GpuTree:
@@ -120,8 +121,11 @@ class GpuTree:
     def vis_max(self):
         return self.api.vis_max(self._h)
 
-    def run(self, seeds, edge_planes=None, depth=None, width=None, n_edge_cuts=None, refine=None, read_levels=True):
-        """run(seeds, edgePlanes, opts) (src/gpu/tree.ts:449). refine = dict(rounds, probes, spacing)."""
+    def run(self, seeds, edge_planes=None, depth=None, width=None, n_edge_cuts=None, refine=None, read_levels=True,
+            read_parts=False):
+        """run(seeds, edgePlanes, opts) (src/gpu/tree.ts:449). refine = dict(rounds, probes, spacing).
+        read_parts: the result's part_vols = (base, float64 (n, 2) of mesh and hull volumes) of the level-0
+        children (src/gpu/tree.ts:83-84 partVols)."""
         n = len(seeds)
         b = _batch(seeds)
         slots = self.plane_slots
@@ -151,6 +155,14 @@ class GpuTree:
                             st.gpu_ms, overflow_pools=st.overflow, last_level_chunks=st.last_level_chunks)
         if read_levels and n:
             res.levels = [self.level_items(l) for l in range(len(res.jobs_per_level) + 1)]
+        if read_parts:
+            base, cnt = F.u32(), F.u32()
+            check(self.api, self.api.tree_part_vols(self._h, C.byref(base), C.byref(cnt), None, None, 0))
+            mv = np.zeros(max(cnt.value, 1), np.float64)
+            hv = np.zeros(max(cnt.value, 1), np.float64)
+            check(self.api, self.api.tree_part_vols(self._h, C.byref(base), C.byref(cnt), mv.ctypes.data_as(C.POINTER(F.f64)),
+                                                    hv.ctypes.data_as(C.POINTER(F.f64)), cnt.value))
+            res.part_vols = (base.value, np.stack([mv[: cnt.value], hv[: cnt.value]], axis=1))
         return res
 
     def level_items(self, level):
@@ -189,16 +201,24 @@ class GpuTree:
             return np.zeros((0, 3), dtype=np.uint32)
         return np.ctypeslib.as_array(p, shape=(nf.value * 3,)).copy().reshape(-1, 3)
 
-    def hull_meshes(self, meshes):
-        """hullMeshes (src/gpu/tree.ts:382): [(tris, volume) or None]."""
+    def hull_meshes(self, meshes, hull_stage=0, no_export=False):
+        """hullMeshes(meshes, {hullStage, noExport}) (src/gpu/tree.ts:382): [(tris, volume) or None];
+        with no_export, [(face count, volume) or None]."""
         n = len(meshes)
         if n == 0:
             return []
         b = _batch(meshes)
         vol = np.zeros(n, dtype=np.float64)
         nf = np.zeros(n, dtype=np.uint32)
-        check(self.api, self.api.hull_meshes(self._h, C.byref(b), vol.ctypes.data_as(C.POINTER(F.f64)),
-                                             nf.ctypes.data_as(C.POINTER(F.u32))))
+        if hull_stage or no_export:
+            check(self.api, self.api.hull_meshes_opts(self._h, C.byref(b), hull_stage, 1 if no_export else 0,
+                                                      vol.ctypes.data_as(C.POINTER(F.f64)),
+                                                      nf.ctypes.data_as(C.POINTER(F.u32))))
+            if no_export:
+                return [None if vol[k] < 0 else (int(nf[k]), float(vol[k])) for k in range(n)]
+        else:
+            check(self.api, self.api.hull_meshes(self._h, C.byref(b), vol.ctypes.data_as(C.POINTER(F.f64)),
+                                                 nf.ctypes.data_as(C.POINTER(F.u32))))
         return [None if vol[k] < 0 else (self._faces(k), float(vol[k])) for k in range(n)]
 
     def hulls_with_hausdorff(self, meshes):

@@ -131,6 +131,15 @@ int cuacd_read_part(cuacd_tree* t, uint32_t part, cuacd_part_info* info, float* 
 /* hullMeshes(meshes) (src/gpu/tree.ts:382): volume[k] (< 0 on failure), nfaces[k];
  * faces via cuacd_last_faces until the next call */
 int cuacd_hull_meshes(cuacd_tree* t, const cuacd_mesh_batch* meshes, double* volume, uint32_t* nfaces);
+/* hullMeshes(meshes, {hullStage, noExport}) (src/gpu/tree.ts:382): hull_stage stops every hull after a
+ * stage, for timing (1 the seeds, 2 the seed hull, 3 the outside points assigned; volume[k] then the
+ * grid volume so far; 0 whole hulls); no_export leaves the faces on the device (nfaces[k] still counted,
+ * cuacd_last_faces empty) */
+int cuacd_hull_meshes_opts(cuacd_tree* t, const cuacd_mesh_batch* meshes, uint32_t hull_stage, int32_t no_export,
+                           double* volume, uint32_t* nfaces);
+/* run(..., {readParts}).partVols (src/gpu/tree.ts:83-84, :604-659): mesh and hull volumes of the last
+ * run's level-0 children, parts *base .. *base + *n - 1 (*n = 2 x level-0 jobs; at most cap written) */
+int cuacd_tree_part_vols(cuacd_tree* t, uint32_t* base, uint32_t* n, double* mesh_vol, double* hull_vol, uint32_t cap);
 /* hullsWithHausdorff(meshes) (src/gpu/tree.ts:671): <= 1024 meshes, also Hausdorff(mesh, hull) */
 int cuacd_hulls_with_hausdorff(cuacd_tree* t, const cuacd_mesh_batch* meshes, double* volume, uint32_t* nfaces,
                                double* hausdorff);

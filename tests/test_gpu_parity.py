@@ -302,3 +302,21 @@ def test_tree_graph_equals_eager(cw):
     assert [(s.best_cut, s.best_cost, s.plane) for s in got.seeds] == [(s.best_cut, s.best_cost, s.plane) for s in want.seeds]
     for t in (eager, graph, small):
         t.destroy()
+
+
+def test_hull_stages_and_no_export(cw, oracle):
+    """hullMeshes(meshes, {hullStage, noExport}) (src/gpu/tree.ts:382): a hull stopped after the seeds
+    has no volume yet, after the seed hull and after the assignment the seed hull's grid volume (a lower
+    bound of the whole hull's), and no_export gives the exported call's volumes and face counts."""
+    g = cw.GpuTree(width=3, width2=3, depth=1, n_concave_edges=0)
+    cases = _hull_cases(cw)[:9] + [cw.Mesh(flat_cone(3000), np.zeros((0, 3), np.uint32))]
+    full = g.hull_meshes(cases)
+    bare = g.hull_meshes(cases, no_export=True)
+    assert [None if f is None else (len(f[0]), f[1]) for f in full] == bare
+    st = {s: g.hull_meshes(cases, hull_stage=s, no_export=True) for s in (1, 2, 3)}
+    for k, f in enumerate(full):
+        if f is None:
+            continue
+        assert st[1][k][1] == 0.0
+        assert st[2][k][1] == st[3][k][1] <= f[1] * (1 + 1e-6)
+    g.destroy()

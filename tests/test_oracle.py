@@ -403,3 +403,32 @@ def test_rv_cost_closed_form(cw, oracle):
     for r in (0.01, 0.1, 0.5, 1.0):
         gap = 4.0 / 3.0 * math.pi * r ** 3
         assert abs(cw.rv_cost(1.0, 1.0 + gap) - 0.3 * r) < 1e-12
+
+
+def test_run_read_parts(cw, oracle):
+    """run(..., {readParts}).partVols (src/gpu/tree.ts:83-84, :604-659): the level-0 children's mesh and
+    hull volumes, parts nSeeds .. nSeeds + 2 jobs - 1, equal readPart of each; each job's two children
+    sum to the seed's volume."""
+    from fixtures import stairs
+    m = cw.normalize(stairs(cw))[0]
+    t = tree(cw, oracle, width=6, width2=3, depth=2, n_concave_edges=0)
+    r = t.run([m], read_parts=True)
+    base, vols = r.part_vols
+    assert base == 1 and len(vols) == 2 * r.jobs_per_level[0]
+    for k in range(len(vols)):
+        d = t.read_part(base + k, geometry=False)
+        assert (d["mesh_vol"], d["hull_vol"]) == tuple(vols[k])
+    v = cw.volume(m)
+    for j in range(r.jobs_per_level[0]):
+        a, b = vols[2 * j, 0], vols[2 * j + 1, 0]
+        if a > 0 and b > 0:
+            assert abs(a + b - v) < 1e-9 * v
+
+
+def test_hull_meshes_no_export(cw, oracle):
+    """hullMeshes(meshes, {noExport}) (src/gpu/tree.ts:382, :392): volumes and face counts without the faces."""
+    t = tree(cw, oracle, width=3, width2=3, depth=1, n_concave_edges=0)
+    cloud = cw.Mesh(park_miller_cloud(), np.zeros((0, 3), np.uint32))
+    full = t.hull_meshes([cw.unit_cube(), cloud])
+    bare = t.hull_meshes([cw.unit_cube(), cloud], no_export=True)
+    assert [(len(f[0]), f[1]) for f in full] == [tuple(b) for b in bare]
