@@ -702,6 +702,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
         const dk::LevelParams P = levelParams(L, l, depth, nSeeds);
         if (L.nJobs == 0) continue;
         dk::HullLevel HL;
+        std::memset(&HL, 0, sizeof(HL));  // stage 0: whole hulls
         HL.prune = P.prune && nSeeds > 0;
         HL.itemsInBase = L.itemsInBase;
         HL.nItemsIn = L.nItemsIn;
