@@ -508,7 +508,8 @@ void GpuBackend::levelHulls(const Level& L, const dk::HullLevel& HL) {
   // most, so the size class of the largest seed bounds every group's slab; a part above it still
   // gets a slab while the arena lasts (and the run reports the arena's overflow if it does not).
   const uint32_t split = cfg_.hullWgMinVerts;
-  const uint64_t slabSmall = dk::hullSlabWords(dk::hullClassCap(std::min(split > 0 ? split - 1 : 0, maxSeedNv_)));
+  const uint64_t slabSmall =
+      dk::hullSlabWords(dk::hullClassCap(std::min(std::min(split > 0 ? split - 1 : 0, dk::MAX_NV_C16), maxSeedNv_)));
   const uint64_t slabBig = dk::hullSlabWords(dk::hullClassCap(std::max(maxSeedNv_, split)));
   uint64_t gSmall = std::min<uint64_t>(HULL_GRID_SMALL_WAVES, nChild);
   uint64_t gBig = std::min<uint64_t>(HULL_GRID_BIG, nChild);

@@ -52,6 +52,9 @@ constexpr uint32_t FW = 20;                 // words per face record (five uint4
 // batch shape: LDS visible-face capacity, edge-hash slots, staged new faces
 constexpr uint32_t LCAP_BIG = 512, HT_BIG = 4096, NCAP_BIG = 256;
 constexpr uint32_t LCAP_W = 128, HT_W = 1024, NCAP_W = 48;
+// tree candidates in the wavefront kernel: parts below 2^16 points, so the round state holds vertex
+// indices in 16 bits and directed edges in 32, and stages fewer new faces (occupancy is bound by LDS)
+constexpr uint32_t NCAP_WC = 16, MAX_NV_C16 = 65535;
 
 __host__ __device__ inline uint32_t hullFmax(uint32_t nv) { return 4 * nv + 256; }
 __host__ __device__ inline uint64_t hullSlabWords(uint32_t nv) {
@@ -676,10 +679,17 @@ __device__ __forceinline__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p
 }
 
 // group-shared state of one insertion round (EXACT: exported hulls, see hull_round)
-template <bool EXACT_, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
+template <bool EXACT_, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP, bool C16 = false>
 struct HullBatch {
   static constexpr bool EXACT = EXACT_;
   using Mask = typename std::conditional<(K > 32), unsigned long long, uint32_t>::type;  // one bit per candidate
+  // C16: every vertex index below 2^16 (16-bit list corners, 32-bit directed-edge keys)
+  using VIdx = typename std::conditional<C16, uint16_t, uint32_t>::type;
+  using HKey = typename std::conditional<C16, uint32_t, unsigned long long>::type;
+  __device__ static __forceinline__ HKey edge_key(uint32_t a, uint32_t b) {
+    if constexpr (C16) return (a << 16) | b;  // never ~0: an edge joins two different vertices
+    else return ((unsigned long long)a << 32) | b;
+  }
   FaceRec nrec[NCAP];  // the round's first new faces
   unsigned long long ckey[K];  // candidate keys (descending when EXACT)
   uint32_t cpt[K];
@@ -688,13 +698,14 @@ struct HullBatch {
   uint32_t nvis[K], nh[K];                              // list entries and horizon edges per candidate
   uint32_t boff[K], bcnt[K];                            // per-candidate buckets of list entries
   uint32_t noff[K];                                     // first new face of each candidate
-  uint32_t lface[LCAP], la[LCAP], lb[LCAP], lc[LCAP];  // visible faces (any candidate)
+  uint32_t lface[LCAP];                                 // visible faces (any candidate)
+  VIdx la[LCAP], lb[LCAP], lc[LCAP];                    // their corners
   uint8_t lj[LCAP], lflag[LCAP];                       // candidate, horizon edge bits
   uint16_t bkt[LCAP];                                   // list entries grouped by candidate
   uint32_t pface[LCAP / 2];
   Mask pmask[LCAP / 2], pund[LCAP / 2];  // faces the f32 filter left undecided
   uint16_t ppre[LCAP / 2];                // their undecided candidates, prefix counts
-  unsigned long long hkey[HT];                          // directed edges (a << 32 | b); seed-hull faces in phase 2
+  HKey hkey[HT];                                        // directed edges (edge_key); seeds before the rounds
   uint8_t hj[HT];
   unsigned long long wkey[NW][EXACT ? K : K / NW];
   unsigned long long wB[NW];
@@ -737,12 +748,15 @@ constexpr uint32_t HULL_LIST_CAP = 128;
 // Otherwise (tree candidates, which use only the exact grid volume, the same
 // for every insertion order): candidates are the largest key of each group of
 // G / K lanes.
-template <typename Grp, bool EXACT, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
-__device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& S,
+template <typename Grp, bool EXACT, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP, bool C16>
+__device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBatch<EXACT, K, LCAP, HT, NW, NCAP, C16>& S,
                                                uint32_t kEff) {
   static_assert(K <= 64 && (HT & (HT - 1)) == 0 && HT >= 6 * LCAP && NW == Grp::NW && LCAP <= 65536, "batch shape");
   static_assert(EXACT ? (K <= 32 && LCAP >= HULL_LIST_CAP) : (K % NW == 0 && K / NW <= 64), "batch shape");
-  using Mask = typename HullBatch<EXACT, K, LCAP, HT, NW, NCAP>::Mask;
+  using Batch = HullBatch<EXACT, K, LCAP, HT, NW, NCAP, C16>;
+  using Mask = typename Batch::Mask;
+  using HK = typename Batch::HKey;
+  constexpr HK HEMPTY = (HK)~(HK)0;
   constexpr uint32_t LC = EXACT ? HULL_LIST_CAP : LCAP;
   const uint32_t t = g.tid();
   const uint32_t lane = lane_id();
@@ -751,7 +765,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   // boundary has at most two edges more than it has faces, so a round makes at most
   // HULL_LIST_CAP + 2 K faces; compacted, the array has 2 nv + 256 slots free
   if (C.nf > 2 * C.nalive + 128 || C.nf + C.nv + 168 > C.fmax) compact_faces(g, C);
-  for (uint32_t s = t; s < HT; s += Grp::G) S.hkey[s] = ~0ull;
+  for (uint32_t s = t; s < HT; s += Grp::G) S.hkey[s] = HEMPTY;
   if (t < K) {
     S.nvis[t] = 0;
     S.nh[t] = 0;
@@ -1040,9 +1054,9 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     const uint32_t v3[3] = {S.la[e], S.lb[e], S.lc[e]};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      const unsigned long long key = ((unsigned long long)v3[k] << 32) | v3[k == 2 ? 0 : k + 1];
+      const HK key = Batch::edge_key(v3[k], v3[k == 2 ? 0 : k + 1]);
       uint32_t s = edge_hash(key, HT);
-      while (atomicCAS(&S.hkey[s], ~0ull, key) != ~0ull) s = (s + 1) & (HT - 1);
+      while (atomicCAS(&S.hkey[s], HEMPTY, key) != HEMPTY) s = (s + 1) & (HT - 1);
       S.hj[s] = S.lj[e];
     }
   }
@@ -1063,16 +1077,16 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     int tjs[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      const unsigned long long twin = ((unsigned long long)v3[k == 2 ? 0 : k + 1] << 32) | v3[k];
+      const HK twin = Batch::edge_key(v3[k == 2 ? 0 : k + 1], v3[k]);
       uint32_t s = edge_hash(twin, HT);
       int tj = -1;
       for (;;) {
-        const unsigned long long hk = S.hkey[s];
+        const HK hk = S.hkey[s];
         if (hk == twin) {
           tj = S.hj[s];
           break;
         }
-        if (hk == ~0ull) break;
+        if (hk == HEMPTY) break;
         s = (s + 1) & (HT - 1);
       }
       tjs[k] = tj;
@@ -1244,8 +1258,8 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
 }
 
 // insertion rounds until no point is outside (K candidates every round)
-template <typename Grp, bool EXACT, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
-__device__ __forceinline__ void hull_grow(const Grp& g, HullCtx& C, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& S) {
+template <typename Grp, bool EXACT, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP, bool C16>
+__device__ __forceinline__ void hull_grow(const Grp& g, HullCtx& C, HullBatch<EXACT, K, LCAP, HT, NW, NCAP, C16>& S) {
   const uint32_t maxRounds = C.nv + 8;
   for (uint32_t r = 0; r < maxRounds; r++) {
     if (hull_round(g, C, S, K) == 0) return;
@@ -1256,14 +1270,15 @@ __device__ __forceinline__ void hull_grow(const Grp& g, HullCtx& C, HullBatch<EX
 struct HullLevel {  // branch-and-bound inputs at the last tree level
   uint32_t prune, itemsInBase, nItemsIn, nSeeds, depth, jobBase;
   // stop after a stage, for timing (src/gpu/tree.ts:382 hullStage): 1 the seeds, 2 the seed hull,
-  // 3 the outside points assigned; 0 the whole hull. A stopped hull reports the grid volume so far.
+  // 3 the outside points assigned; 0 the whole hull. A stopped hull reports the volume of its faces
+  // so far (the grid volume for tree candidates, the surface volume for exported hulls).
   uint32_t stage;
 };
 
 // one hull job on one group; seed points in `seed` (80 entries of group-shared memory).
 // EXACT: exported hulls, whose face list is the same for every group size
-template <typename Grp, bool EXACT, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP>
-__device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& S, const dk::HullJob& J, uint32_t hj, uint32_t* seed,
+template <typename Grp, bool EXACT, uint32_t K, uint32_t LCAP, uint32_t HT, uint32_t NW, uint32_t NCAP, bool C16>
+__device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP, C16>& S, const dk::HullJob& J, uint32_t hj, uint32_t* seed,
                          const float* __restrict__ verts,
                          uint32_t* __restrict__ arena, DPart* __restrict__ parts, HullResultD* __restrict__ results,
                          uint32_t* __restrict__ exportBuf, const DItem* __restrict__ items, const DJob* __restrict__ jobs,
@@ -1304,7 +1319,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
   C.rejK = 8.0f * (float)(1 << (GB - 21));
   bool done = false;
   if (J.nv < 4) done = true;
-  else if (hullSlabWords(J.nv) > J.slabWords) {
+  else if (hullSlabWords(J.nv) > J.slabWords || (C16 && J.nv > MAX_NV_C16)) {
     res.status = 3;
     done = true;
   }
@@ -1507,8 +1522,9 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
       hull_grow(g, C, S);
       if (phase == 2) break;
       res.status = 0;
-      if (HL.stage == 2) {
-        res.volume = hull_volume(g, C);
+      if (HL.stage == 2) {  // exported hulls quantize each axis on its own: their surface's volume
+        if constexpr (EXACT) res.volume = hull_volume_f32(g, C);
+        else res.volume = hull_volume(g, C);
         done = true;
         break;
       }
@@ -1545,8 +1561,11 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
         // keeps the first-in-creation-order choice.
         for (uint32_t f = t; f < C.nf; f += Grp::G) C.fkey[f] = 0ull;
         if (t == 0) S.nOrph = 0;  // outside points so far (read after the barriers below)
-        uint4* stage = reinterpret_cast<uint4*>(S.hkey);
-        constexpr uint32_t SCAP = (uint32_t)(sizeof(S.hkey) / 48);
+        // (the round state is dead in this phase: the faces are staged over all of it up to nOrph,
+        // the one field the phase uses)
+        uint4* stage = reinterpret_cast<uint4*>(&S);
+        const uint32_t SCAP =
+            (uint32_t)((reinterpret_cast<const char*>(&S.nOrph) - reinterpret_cast<const char*>(&S)) / 48);
         uint32_t ns = 0;
         for (uint32_t base = 0; base < C.nf; base += Grp::G) {
           const uint32_t f = base + t;
@@ -1651,7 +1670,8 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP>& 
         res.nCand = nc;
         g.sync();
         if (HL.stage == 3) {
-          res.volume = hull_volume(g, C);
+          if constexpr (EXACT) res.volume = hull_volume_f32(g, C);
+          else res.volume = hull_volume(g, C);
           done = true;
         }
       }
@@ -1742,7 +1762,7 @@ __device__ __forceinline__ uint32_t hull_slab(uint32_t nv, unsigned long long* o
 
 // small parts: one wavefront per part, HW wavefronts per block
 template <bool EXACT>
-__global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hjobs, HullQueue* __restrict__ q,
+__global__ __launch_bounds__(64 * HW, EXACT ? 2 : 4) void k_hull(const HullJob* __restrict__ hjobs, HullQueue* __restrict__ q,
                                                   HullArena* __restrict__ ar, const float* __restrict__ verts,
                                                   uint32_t* __restrict__ arena, DPart* __restrict__ parts,
                                                   HullResultD* __restrict__ results, uint32_t* __restrict__ exportBuf,
@@ -1750,8 +1770,11 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
                                                   HullLevel HL, Counters* __restrict__ ctr) {
   const uint32_t wid = threadIdx.x >> 6;
   constexpr uint32_t KW = EXACT ? KE : KF_W;
-  __shared__ uint32_t s_seed[HW][80];
-  __shared__ HullBatch<EXACT, KW, LCAP_W, HT_W, 1, NCAP_W> s_batch[HW];
+  // tree candidates: the compact round state, the seeds kept in its edge hash (dead until the
+  // first round); exported hulls: the full state and seeds of their own
+  constexpr bool C16 = !EXACT;
+  __shared__ uint32_t s_seed[EXACT ? HW : 1][80];
+  __shared__ HullBatch<EXACT, KW, LCAP_W, HT_W, 1, EXACT ? NCAP_W : NCAP_WC, C16> s_batch[HW];
   const WaveGrp g;
   const uint32_t count = q->count;
   unsigned long long off = 0;
@@ -1769,7 +1792,8 @@ __global__ __launch_bounds__(64 * HW) void k_hull(const HullJob* __restrict__ hj
     HullJob J = hjobs[j];
     J.slabOff = off;
     J.slabWords = words;
-    hull_job(g, s_batch[wid], J, j, s_seed[wid], verts, arena, parts, results, exportBuf, items, jobs, HL, ctr,
+    uint32_t* seed = EXACT ? s_seed[EXACT ? wid : 0] : reinterpret_cast<uint32_t*>(s_batch[wid].hkey);
+    hull_job(g, s_batch[wid], J, j, seed, verts, arena, parts, results, exportBuf, items, jobs, HL, ctr,
              &ctr->faceTests, &ctr->hullBytes);
     wave_sync();
   }
@@ -1853,7 +1877,7 @@ __global__ __launch_bounds__(256) void k_hq_count(const DPart* __restrict__ part
   if (c < nChild) {
     const DPart p = parts[partBase + c];
     if (p.nv >= 4 && (p.nt > 0 || (p.flags & 2u))) {
-      tg = (p.nv >= split ? 32u : 0u) + hullClassOf(p.nv);
+      tg = (p.nv >= split || p.nv > MAX_NV_C16 ? 32u : 0u) + hullClassOf(p.nv);
       local = atomicAdd(&s_cnt[tg], 1u);
     }
   }

@@ -23,11 +23,13 @@ namespace cuacd {
 
 namespace {
 
+constexpr uint32_t NIL = 0xffffffffu;
+
 struct QFace {
   uint32_t v[3];
   int32_t adj[3];  // face across edge v[k] -> v[k + 1]
   double n[3], d, nn;
-  std::vector<uint32_t> pts;
+  uint32_t head = NIL, count = 0;  // the outside set: a list through Hull::next
   uint32_t far = 0;
   double farD = 0;
   uint64_t seen = 0;  // iteration of the last visibility test
@@ -38,6 +40,7 @@ struct Hull {
   const double* P;
   double eps2;
   std::vector<QFace> F;
+  std::vector<uint32_t> next;  // per point: the next point of its face's outside set
 
   const double* pt(uint32_t i) const { return P + 3 * (size_t)i; }
   uint32_t make(uint32_t a, uint32_t b, uint32_t c) {
@@ -68,11 +71,15 @@ struct Hull {
     QFace& f = F[fi];
     const double D = dist(f, i);
     if (!(D > 0 && D * D > eps2 * f.nn)) return false;
-    if (f.pts.empty() || D > f.farD) {
+    if (f.count == 0 || D > f.farD) {
       f.farD = D;
       f.far = i;
     }
-    f.pts.push_back(i);
+    // appended at the tail would keep the insertion order; the order of a set changes nothing
+    // but which of two equally distant points is farthest, and that is decided above in order
+    next[i] = f.head;
+    f.head = i;
+    f.count++;
     return true;
   }
   int edgeOf(uint32_t fi, uint32_t a, uint32_t b) const {
@@ -108,6 +115,8 @@ bool hullEps(const double* P, uint32_t n, double eps, std::vector<uint32_t>* tri
   Hull H;
   H.P = P;
   H.eps2 = epsA * epsA;
+  H.F.reserve(2 * (size_t)n + 16);
+  H.next.assign(n, NIL);
   // simplex: the farthest pair of extremes, the point farthest from their line, the point
   // farthest from their plane
   uint32_t i0 = ext[0], i1 = ext[1];
@@ -177,17 +186,22 @@ bool hullEps(const double* P, uint32_t n, double eps, std::vector<uint32_t>* tri
       if (H.offer(f, i)) break;
   }
   for (uint32_t f : fs)
-    if (!H.F[f].pts.empty()) queue.push_back(f);
+    if (H.F[f].count) queue.push_back(f);
   uint64_t iter = 0;
   std::vector<uint32_t> visible, stack;
   std::vector<std::pair<uint32_t, int>> horizon;  // (visible face, edge) with a hidden face across
-  std::unordered_map<uint32_t, size_t> fromVertex;
+  std::vector<std::pair<uint32_t, uint32_t>> fromVertex;  // (first vertex, horizon entry)
   std::vector<uint32_t> orphans, loop;
   std::vector<uint8_t> onLoop;
+  auto startingAt = [&](uint32_t v) -> size_t {
+    for (const auto& e : fromVertex)
+      if (e.first == v) return e.second;
+    return SIZE_MAX;
+  };
   while (!queue.empty()) {
     const uint32_t top = queue.front();
     queue.pop_front();
-    if (!H.F[top].alive || H.F[top].pts.empty()) continue;
+    if (!H.F[top].alive || !H.F[top].count) continue;
     iter++;
     const uint32_t eye = H.F[top].far;
     // the visible region: a walk over adjacency from the top face
@@ -216,7 +230,9 @@ bool hullEps(const double* P, uint32_t n, double eps, std::vector<uint32_t>* tri
     bool ok = true;
     for (size_t h = 0; h < horizon.size() && ok; h++) {
       const QFace& f = H.F[horizon[h].first];
-      ok = fromVertex.emplace(f.v[horizon[h].second], h).second;
+      const uint32_t v0 = f.v[horizon[h].second];
+      ok = startingAt(v0) == SIZE_MAX;
+      fromVertex.push_back({v0, (uint32_t)h});
     }
     loop.clear();
     if (ok) {
@@ -231,38 +247,47 @@ bool hullEps(const double* P, uint32_t n, double eps, std::vector<uint32_t>* tri
         loop.push_back((uint32_t)h);
         const QFace& f = H.F[horizon[h].first];
         const int k = horizon[h].second;
-        const auto it = fromVertex.find(f.v[k == 2 ? 0 : k + 1]);
-        if (it == fromVertex.end()) {
+        const size_t nx = startingAt(f.v[k == 2 ? 0 : k + 1]);
+        if (nx == SIZE_MAX) {
           ok = false;
           break;
         }
-        h = it->second;
+        h = nx;
       }
       ok = ok && h == 0 && loop.size() == horizon.size();
     }
     if (!ok) {
       // no single horizon loop (numerical trouble): the point is dropped
-      std::vector<uint32_t>& tp = H.F[top].pts;
-      tp.erase(std::find(tp.begin(), tp.end(), eye));
-      for (size_t q = 0; q < tp.size(); q++) {
-        const double D = H.dist(H.F[top], tp[q]);
-        if (q == 0 || D > H.F[top].farD) {
-          H.F[top].farD = D;
-          H.F[top].far = tp[q];
+      QFace& T = H.F[top];
+      std::vector<uint32_t> rest;
+      for (uint32_t i = T.head; i != NIL; i = H.next[i])
+        if (i != eye) rest.push_back(i);
+      T.head = NIL;
+      T.count = 0;
+      for (auto it = rest.rbegin(); it != rest.rend(); ++it) {  // the list's order kept
+        const double D = H.dist(T, *it);
+        if (T.count == 0 || D > T.farD) {
+          T.farD = D;
+          T.far = *it;
         }
+        H.next[*it] = T.head;
+        T.head = *it;
+        T.count++;
       }
       for (uint32_t f : visible) H.F[f].vis = false;
-      if (!tp.empty()) queue.push_back(top);
+      if (T.count) queue.push_back(top);
       continue;
     }
     // new faces (a, b, eye) along the loop, linked to the hidden faces and to each other
     orphans.clear();
     for (uint32_t f : visible) {
-      H.F[f].alive = false;
-      H.F[f].vis = false;
-      for (uint32_t i : H.F[f].pts)
+      QFace& V = H.F[f];
+      V.alive = false;
+      V.vis = false;
+      for (uint32_t i = V.head; i != NIL; i = H.next[i])
         if (i != eye) orphans.push_back(i);
-      std::vector<uint32_t>().swap(H.F[f].pts);
+      V.head = NIL;
+      V.count = 0;
     }
     const size_t nh = loop.size();
     const uint32_t first = (uint32_t)H.F.size();
@@ -283,7 +308,7 @@ bool hullEps(const double* P, uint32_t n, double eps, std::vector<uint32_t>* tri
       for (size_t s = 0; s < nh; s++)
         if (H.offer(first + (uint32_t)s, i)) break;
     for (size_t s = 0; s < nh; s++)
-      if (!H.F[first + s].pts.empty()) queue.push_back(first + (uint32_t)s);
+      if (H.F[first + s].count) queue.push_back(first + (uint32_t)s);
   }
   for (const QFace& f : H.F)
     if (f.alive) tris->insert(tris->end(), {f.v[0], f.v[1], f.v[2]});
@@ -309,17 +334,88 @@ double signedVolumeOf(const float* V, const std::vector<uint32_t>& T) {
 
 // src/mesh.ts:103-118 isClosed over the triangles alone: every directed edge once, and its reverse
 bool closedTris(const std::vector<uint32_t>& T) {
-  std::unordered_map<uint64_t, uint32_t> cnt;
-  cnt.reserve(T.size() * 2);
+  std::vector<uint64_t> e;
+  e.reserve(T.size());
   for (size_t t = 0; t + 2 < T.size(); t += 3)
-    for (int k = 0; k < 3; k++) cnt[((uint64_t)T[t + k] << 32) | T[t + (k == 2 ? 0 : k + 1)]]++;
-  for (const auto& e : cnt) {
-    if (e.second != 1) return false;
-    const auto it = cnt.find((e.first << 32) | (e.first >> 32));
-    if (it == cnt.end() || it->second != 1) return false;
+    for (int k = 0; k < 3; k++) e.push_back(((uint64_t)T[t + k] << 32) | T[t + (k == 2 ? 0 : k + 1)]);
+  std::sort(e.begin(), e.end());
+  for (size_t i = 0; i < e.size(); i++) {
+    if (i + 1 < e.size() && e[i + 1] == e[i]) return false;  // a directed edge twice
+    const uint64_t r = (e[i] << 32) | (e[i] >> 32);
+    if (!std::binary_search(e.begin(), e.end(), r)) return false;  // no reverse
   }
   return true;
 }
+
+// The containment test of the cleanup: does any candidate lie more than tol above the plane n.x = d?
+// Candidates are kept in clusters of spatially sorted points, each with its box; a cluster whose box
+// lies below the plane with a margin for rounding cannot hold such a point and is skipped, so the
+// answer is that of testing every point (n . q - d > tol, as src/cpu/hull.ts:146 evaluates it).
+struct Clusters {
+  static constexpr uint32_t CS = 32;
+  std::vector<double> x, y, z;                // candidate coordinates, cluster by cluster
+  std::vector<double> cx, cy, cz, hx, hy, hz;  // box centres and half extents per cluster
+  double mag = 0;                             // largest coordinate magnitude
+  void build(const float* V, const std::vector<uint32_t>& cand) {
+    const size_t n = cand.size();
+    std::vector<std::pair<uint64_t, uint32_t>> key(n);
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t c : cand)
+      for (int a = 0; a < 3; a++) {
+        lo[a] = std::min(lo[a], (double)V[3 * (size_t)c + a]);
+        hi[a] = std::max(hi[a], (double)V[3 * (size_t)c + a]);
+      }
+    for (size_t k = 0; k < n; k++) {  // Morton order on a 2^10 grid
+      uint64_t m = 0;
+      uint32_t q[3];
+      for (int a = 0; a < 3; a++) {
+        const double e = hi[a] - lo[a];
+        q[a] = e > 0 ? (uint32_t)std::min(1023.0, (V[3 * (size_t)cand[k] + a] - lo[a]) / e * 1023.0) : 0;
+      }
+      for (int b = 9; b >= 0; b--)
+        for (int a = 0; a < 3; a++) m = (m << 1) | ((q[a] >> b) & 1u);
+      key[k] = {m, cand[k]};
+    }
+    std::sort(key.begin(), key.end());
+    x.resize(n);
+    y.resize(n);
+    z.resize(n);
+    for (size_t k = 0; k < n; k++) {
+      const float* v = V + 3 * (size_t)key[k].second;
+      x[k] = v[0];
+      y[k] = v[1];
+      z[k] = v[2];
+      mag = std::max(mag, std::max(std::fabs(x[k]), std::max(std::fabs(y[k]), std::fabs(z[k]))));
+    }
+    const size_t nc = (n + CS - 1) / CS;
+    cx.resize(nc), cy.resize(nc), cz.resize(nc), hx.resize(nc), hy.resize(nc), hz.resize(nc);
+    for (size_t c = 0; c < nc; c++) {
+      double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (size_t k = c * CS; k < std::min(n, (c + 1) * CS); k++) {
+        l[0] = std::min(l[0], x[k]), h[0] = std::max(h[0], x[k]);
+        l[1] = std::min(l[1], y[k]), h[1] = std::max(h[1], y[k]);
+        l[2] = std::min(l[2], z[k]), h[2] = std::max(h[2], z[k]);
+      }
+      cx[c] = 0.5 * (l[0] + h[0]), cy[c] = 0.5 * (l[1] + h[1]), cz[c] = 0.5 * (l[2] + h[2]);
+      hx[c] = 0.5 * (h[0] - l[0]), hy[c] = 0.5 * (h[1] - l[1]), hz[c] = 0.5 * (h[2] - l[2]);
+    }
+  }
+  // true when some candidate q has nx q0 + ny q1 + nz q2 - d > tol (|n| = 1)
+  bool anyAbove(double nx, double ny, double nz, double d, double tol) const {
+    // the box bound n.c + |n|.h is within a few ulps of the points' own values; the margin covers
+    // that rounding, so a skipped cluster holds no point whose computed value exceeds tol
+    const double margin = 64 * 2.2e-16 * (3 * mag + std::fabs(d) + std::fabs(tol));
+    const size_t n = x.size(), nc = cx.size();
+    const double anx = std::fabs(nx), any = std::fabs(ny), anz = std::fabs(nz);
+    for (size_t c = 0; c < nc; c++) {
+      const double ub = nx * cx[c] + ny * cy[c] + nz * cz[c] + (anx * hx[c] + any * hy[c] + anz * hz[c]);
+      if (ub - d + margin <= tol) continue;
+      for (size_t k = c * CS; k < std::min(n, (c + 1) * CS); k++)
+        if (nx * x[k] + ny * y[k] + nz * z[k] - d > tol) return true;
+    }
+    return false;
+  }
+};
 
 struct KeyHash {
   size_t operator()(const std::array<uint32_t, 3>& k) const {
@@ -348,6 +444,7 @@ std::vector<uint32_t> cleanHullFaces(const float* V, const std::vector<uint32_t>
   const double tol = 5 * eps * extent;
   std::vector<double> pts;
   std::vector<uint32_t> tris, out;
+  Clusters cl;
   for (int round = 0; round < 6; round++) {
     const size_t n = cand.size();
     if (n < 4) return T;
@@ -375,6 +472,7 @@ std::vector<uint32_t> cleanHullFaces(const float* V, const std::vector<uint32_t>
     // degenerate faces: the vertex between the other two is dropped
     std::vector<uint32_t> drop;
     bool ok = true;
+    cl.build(V, cand);
     for (size_t t = 0; t + 2 < out.size() && ok; t += 3) {
       const uint32_t ia = out[t], ib = out[t + 1], ic = out[t + 2];
       const float* A = V + 3 * (size_t)ia;
@@ -395,13 +493,7 @@ std::vector<uint32_t> cleanHullFaces(const float* V, const std::vector<uint32_t>
       ny /= l;
       nz /= l;
       const double d = nx * A[0] + ny * A[1] + nz * A[2];
-      for (uint32_t q : cand) {
-        const float* Q = V + 3 * (size_t)q;
-        if (nx * Q[0] + ny * Q[1] + nz * Q[2] - d > tol) {
-          ok = false;
-          break;
-        }
-      }
+      if (cl.anyAbove(nx, ny, nz, d, tol)) ok = false;
     }
     if (!ok) return T;
     if (drop.empty()) return out;
