@@ -249,6 +249,7 @@ class GpuBackend : public Backend {
   DevBuf<uint4> ccRec_, ccEdges_, ccResv_;
   DevBuf<float4> ccPlanes_;
   DevBuf<uint32_t> export_;
+  DevBuf<double> seedUb_;             // branch-and-bound bound per seed at the last level
   DevBuf<cuacd::MergePart> mgParts_;  // the merge pass's pair rejections
   DevBuf<uint32_t> mgOff_;
   DevBuf<unsigned long long> mgBase_;
@@ -662,6 +663,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
     if (!lv->empty()) maxLastItems = std::max(maxLastItems, lv->back().nItemsIn);
   uint32_t chunks = 1;
   dk::Counters c1;
+  seedUb_.ensure(nSeeds);  // allocated ahead: a captured run allocates nothing
   for (;;) {
     for (uint32_t k = 0; k < nSeeds; k++) {
       std::memset(&st[k], 0, sizeof(st[k]));
@@ -705,6 +707,12 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
         dk::HullLevel HL;
         std::memset(&HL, 0, sizeof(HL));  // stage 0: whole hulls
         HL.prune = P.prune && nSeeds > 0;
+        if (HL.prune) {  // the seeds' bounds, once for the level (every chunk reads the same input items)
+          seedUb_.ensure(nSeeds);
+          dk::k_seed_ub<<<nSeeds, 256, 0, stream_>>>(items_.p, L.itemsInBase, L.nItemsIn / nSeeds, depth, seedUb_.p);
+          launched("k_seed_ub");
+          HL.seedUb = seedUb_.p;
+        }
         HL.itemsInBase = L.itemsInBase;
         HL.nItemsIn = L.nItemsIn;
         HL.nSeeds = nSeeds;

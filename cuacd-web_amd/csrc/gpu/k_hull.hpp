@@ -1273,7 +1273,39 @@ struct HullLevel {  // branch-and-bound inputs at the last tree level
   // 3 the outside points assigned; 0 the whole hull. A stopped hull reports the volume of its faces
   // so far (the grid volume for tree candidates, the surface volume for exported hulls).
   uint32_t stage;
+  const double* seedUb;  // per seed: the best leaf-path bound of the level's input items (k_seed_ub)
 };
+
+// The branch-and-bound upper bound of each seed at the last level, once per level: the smallest
+// partial cost plus the last level cost carried to the full depth, over the seed's input items
+// that are still open (src/gpu/shaders.ts:1294-1331). A workgroup per seed; the minimum is the
+// same whatever the order.
+__global__ __launch_bounds__(256) void k_seed_ub(const DItem* __restrict__ items, uint32_t itemsInBase,
+                                                 uint32_t perSeed, uint32_t depth, double* __restrict__ ubOut) {
+  __shared__ double s_min[4];
+  const uint32_t s = blockIdx.x;
+  double ub = 1e30;
+  for (uint32_t i = threadIdx.x; i < perSeed; i += blockDim.x) {
+    const DItem& it = items[itemsInBase + s * perSeed + i];
+    if (it.flags == 1 && it.nLevels > 0) {
+      double ps = 0.0;
+      for (uint32_t l = 0; l < it.nLevels; l++) ps += it.levelCosts[l];
+      const double bnd = ps + it.levelCosts[it.nLevels - 1] * (double)(depth - it.nLevels);
+      ub = bnd < ub ? bnd : ub;
+    }
+  }
+  for (int m = 32; m >= 1; m >>= 1) {
+    const double o = __shfl_xor(ub, m, 64);
+    ub = o < ub ? o : ub;
+  }
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = ub;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = s_min[0];
+    for (uint32_t w = 1; w < blockDim.x / 64; w++) r = s_min[w] < r ? s_min[w] : r;
+    ubOut[s] = r;
+  }
+}
 
 // one hull job on one group; seed points in `seed` (80 entries of group-shared memory).
 // EXACT: exported hulls, whose face list is the same for every group size
@@ -1534,18 +1566,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP, C
         const DItem pit = items[HL.itemsInBase + jobs[J.job].parentItem];
         double partial = 0.0;
         for (uint32_t l = 0; l < pit.nLevels; l++) partial += pit.levelCosts[l];
-        const uint32_t perSeed = HL.nItemsIn / HL.nSeeds;
-        double ub = 1e30;
-        for (uint32_t i = t; i < perSeed; i += Grp::G) {
-          const DItem& it = items[HL.itemsInBase + pit.srcPart * perSeed + i];
-          if (it.flags == 1 && it.nLevels > 0) {
-            double ps = 0.0;
-            for (uint32_t l = 0; l < it.nLevels; l++) ps += it.levelCosts[l];
-            const double bnd = ps + it.levelCosts[it.nLevels - 1] * (double)(HL.depth - it.nLevels);
-            ub = bnd < ub ? bnd : ub;
-          }
-        }
-        ub = g.mind(ub);
+        const double ub = HL.seedUb[pit.srcPart];
         const double meshVol = parts ? parts[J.part].meshVol : 0.0;
         if (partial + cuacd::rv_cost(meshVol, vRough) > ub) {
           res.status = 2;
