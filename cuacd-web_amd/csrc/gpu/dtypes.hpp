@@ -64,7 +64,9 @@ struct LevelParams {
   uint32_t partBase, nJobs, width, nEdgeCuts;
   uint32_t totalWidth, level, depth, lastLevel;
   uint32_t nSeeds, edgeStride, prune, itemChunk0;  // itemChunk0: first input item of a partial level
-  float minEdgeDist, padf[3];
+  float minEdgeDist;
+  uint32_t leavesOnly;  // k_lvl_items: apply the leaf rule to the items and stop (no planes)
+  float padf[2];
   uint64_t vCap, tCap, eCap, sCap;
 };
 

@@ -707,7 +707,15 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
         dk::HullLevel HL;
         std::memset(&HL, 0, sizeof(HL));  // stage 0: whole hulls
         HL.prune = P.prune && nSeeds > 0;
-        if (HL.prune) {  // the seeds' bounds, once for the level (every chunk reads the same input items)
+        if (HL.prune) {
+          // The seeds' bounds, once for the level, over its input items once the leaf rule has marked
+          // every item of the level (an early leaf is no open path): a pass of k_lvl_items that only
+          // marks, then the bounds, then the level (in chunks or not) as usual.
+          dk::LevelParams PL = P;
+          PL.leavesOnly = 1;
+          dk::k_lvl_items<<<L.nItemsIn, dk::LT, 0, stream_>>>(PL, items_.p, parts_.p, jobs_.p, jobPlanes_.p, edgePlanes_.p,
+                                                               verts_.p, jobTiles_.p, 1, ctr_.p);
+          launched("k_lvl_items (leaves)");
           seedUb_.ensure(nSeeds);
           dk::k_seed_ub<<<nSeeds, 256, 0, stream_>>>(items_.p, L.itemsInBase, L.nItemsIn / nSeeds, depth, seedUb_.p);
           launched("k_seed_ub");

@@ -1437,11 +1437,13 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP, C
           seed[2 * (d0 + k) + 1] = in[k];
         }
     } else {
-      for (int k0 = 0; k0 < 40; k0 += 4) {
-        float mx[4], mn[4];
-        uint32_t ix[4], in[4];
+      // a wavefront: SD directions per pass over the points (the points are read 40 / SD times)
+      constexpr int SD = 8;
+      for (int k0 = 0; k0 < 40; k0 += SD) {
+        float mx[SD], mn[SD];
+        uint32_t ix[SD], in[SD];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < SD; k++) {
           mx[k] = -1e30f;
           mn[k] = 1e30f;
           ix[k] = NONE;
@@ -1451,7 +1453,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP, C
           const int32_t* q = C.Q + 3 * i;
           const float x = (float)q[0], y = (float)q[1], z = (float)q[2];
 #pragma unroll
-          for (int k = 0; k < 4; k++) {
+          for (int k = 0; k < SD; k++) {
             float d = KDIR[k0 + k][0] * x;
             d = d + KDIR[k0 + k][1] * y;
             d = d + KDIR[k0 + k][2] * z;
@@ -1465,10 +1467,10 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP, C
             }
           }
         }
-        g.template argext<4>(mx, ix, mn, in);
+        g.template argext<SD>(mx, ix, mn, in);
         if (t == 0)
 #pragma unroll
-          for (int k = 0; k < 4; k++) {
+          for (int k = 0; k < SD; k++) {
             seed[2 * (k0 + k)] = ix[k];
             seed[2 * (k0 + k) + 1] = in[k];
           }

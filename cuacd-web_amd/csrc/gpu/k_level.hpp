@@ -144,6 +144,7 @@ __global__ __launch_bounds__(LT) void k_lvl_items(LevelParams P, DItem* __restri
       }
     }
   }
+  if (P.leavesOnly) return;  // a pass that only marks the level's early leaves (before its bounds)
   const uint32_t maxSize = part.nt > part.nv ? (part.nt > part.ne ? part.nt : part.ne) : (part.nv > part.ne ? part.nv : part.ne);
   uint32_t nT = (maxSize + TILE_MIN - 1) / TILE_MIN;
   nT = nT < 1 ? 1 : (nT > maxTiles ? maxTiles : nT);

@@ -320,3 +320,32 @@ def test_hull_stages_and_no_export(cw, oracle):
         assert st[1][k][1] == 0.0
         assert st[2][k][1] == st[3][k][1] <= f[1] * (1 + 1e-6)
     g.destroy()
+
+
+def test_chunked_last_level_with_early_leaves(cw, oracle):
+    """Branch-and-bound at the last level bounds each seed by its open input items, so every item the
+    leaf rule ends early must be marked before any hull is bounded: with the last level in chunks (items
+    of later chunks not yet planned) the cuts, costs and tables equal the oracle's one-pass run. The
+    flattened L prism at a 0.55 margin has early leaves at level 1 and full-depth paths."""
+    base = cw.normalize(l_prism(cw))[0]
+    flat = base.verts.copy()
+    flat[:, 2] *= 0.3
+    seeds = [base, cw.Mesh(flat.astype(np.float32), base.tris)]
+    cfg = dict(width=6, width2=3, depth=3, n_concave_edges=0, min_edge_dist=0.55)
+    o = cw.GpuTree(api=oracle, **cfg)
+    ro = o.run(seeds)
+    assert sum(1 for lvl in ro.levels[1:-1] for it in lvl if it["flags"] == 2) > 0
+    nparts = len(seeds) + 2 * sum(ro.jobs_per_level)
+    first_last = nparts - 2 * ro.jobs_per_level[-1]
+    pa = [o.read_part(p, geometry=False) for p in range(nparts)]
+    b = sum(p["nv"] for p in pa[:first_last])
+    last = sum(p["nv"] for p in pa[first_last:])
+    for caps in ({}, dict(vertex_cap=3 * (b + max(last // 3, 64)))):
+        g = cw.GpuTree(**cfg, **caps)
+        rg = g.run(seeds)
+        print(f"chunks {rg.last_level_chunks}", flush=True)
+        assert not rg.overflow and (rg.last_level_chunks > 1) == bool(caps)
+        assert [(s.best_cut, s.best_cost, s.plane) for s in rg.seeds] == [(s.best_cut, s.best_cost, s.plane) for s in ro.seeds]
+        assert rg.levels[:-1] == ro.levels[:-1] and rg.pruned == ro.pruned
+        g.destroy()
+    o.destroy()
