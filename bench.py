@@ -9,9 +9,10 @@ batches. The engine (device pools sized for the batch) is created once, before t
 reference's Decomposer is (src/decompose.ts:135). `value` is meshes/s over the timed steps.
 
 Beside it, in the same JSON line:
-  plain        the same 256 meshes decomposed one at a time on one engine with single-mesh pools (the
-               plain per-mesh GPU path), its meshes/s, the batched/plain ratio, and whether every
-               result (cuts, parts, volumes, distances, hull faces) is identical to the batched one;
+  plain        the same 256 meshes decomposed one at a time on one engine with single-mesh pools and the
+               knobs tuned for the plain decompose (the plain per-mesh GPU path), its meshes/s, the
+               batched/plain ratio, and whether every result (cuts, parts, volumes, distances, hull
+               faces) is identical to the batched one;
   blob         the 12-blob 96^3 mesh (56,464 triangles, threshold 0.05, width 12, width2 6, depth 3,
                16 concave-edge planes, refinement 2 x 8, merge, 3000 samples per part) decomposed
                plain, the single-mesh workload;
@@ -20,9 +21,9 @@ Beside it, in the same JSON line:
   cpu_baseline the CPU oracle (the C++ restatement of the path, OpenMP over the host's cores) decomposing
                a sample of the mix in full, merge included, with every result checked equal to the GPU's;
                all cores, and one thread in a child process on a smaller sample.
-Each workload runs with the knobs cuacd-web_amd/autotune.py measured for it (the hull kernel split,
-and the number of engines a batch is split over; cuacd-web_amd/tuned.json), when that file exists;
-results do not depend on them.
+Each workload runs with the knobs cuacd-web_amd/autotune.py measured for it (the kernels' launch
+shapes, the hull kernel split, graph capture and the number of engines a batch is split over;
+cuacd-web_amd/tuned.json), when that file exists; results do not depend on them.
 """
 import argparse
 import ctypes
@@ -82,24 +83,26 @@ TUNED_FILE = os.path.join(PKG_DIR, "tuned.json")  # written by cuacd-web_amd/aut
 
 
 def tuned(workload):
-    """The knobs the autotuner measured best for a workload (hull_wg_min_verts, and batch_lanes for
-    batches); the library defaults for knobs it has not tuned. Knobs change timing only, never results."""
-    if not os.path.exists(TUNED_FILE):
-        return {}
-    with open(TUNED_FILE) as f:
-        t = json.load(f).get(workload, {})
-    return {k: int(t[k]) for k in ("hull_wg_min_verts", "batch_lanes") if k in t}
+    """The decompose options the autotuner measured best for a workload (the kernels' launch shapes,
+    the hull split, graph capture and, for batches, the engine count); the library defaults for knobs
+    it has not tuned. Knobs change timing only, never results."""
+    _package()
+    from cuacd_web_amd.autotune import options
+    return options(workload, TUNED_FILE)
 
 
 def sum_kernels(kernel_dicts):
     tot = None
     for k in kernel_dicts:
         if tot is None:
-            tot = dict(ms=dict(k["ms"]), launches=dict(k["launches"]), **{n: k[n] for n in COUNTERS})
+            tot = dict(ms=dict(k["ms"]), launches=dict(k["launches"]), graphs=k["graphs"], graph_ms=k["graph_ms"],
+                       **{n: k[n] for n in COUNTERS})
             continue
         for n in k["ms"]:
             tot["ms"][n] += k["ms"][n]
             tot["launches"][n] += k["launches"][n]
+        tot["graphs"] += k["graphs"]
+        tot["graph_ms"] += k["graph_ms"]
         for n in COUNTERS:
             tot[n] += k[n]
     return tot
@@ -107,7 +110,9 @@ def sum_kernels(kernel_dicts):
 
 def roofline(kst, workload):
     """The kernel with the most device time over the timed steps, priced against its bound.
-    Times come from HIP events the library records around each launch on that launch's stream."""
+    Times come from HIP events the library records around each launch on that launch's stream; a hull
+    launch inside a tree run captured into a HIP graph (which events cannot time) times itself on the
+    device's wall clock, from its first group's start to its last group's end."""
     name = max(KERNEL_WORK, key=lambda n: kst["ms"][n])
     counter, per_unit, kind = KERNEL_WORK[name]
     launches = max(kst["launches"][name], 1)
@@ -143,7 +148,10 @@ def roofline(kst, workload):
                 work_per_launch=work_per_launch, work_counter=counter, work_per_unit=per_unit, effort=effort,
                 counters=counters,
                 kernel_ms={n: round(v, 3) for n, v in kst["ms"].items()},
-                kernel_launches=dict(kst["launches"]))
+                kernel_launches=dict(kst["launches"]),
+                timing=("HIP events; hull launches inside the tree runs' graphs on the device wall clock "
+                        f"({kst['graphs']} graphs, {kst['graph_ms']:.1f} ms; the other kernels of a graph are "
+                        "timed with it as a whole)") if kst["graphs"] else "HIP events")
 
 
 def copy_probe(mib):
@@ -310,7 +318,10 @@ def main():
     }
     if a.plain:
         sel = list(range(0, a.meshes, a.plain))
-        peng = cw.Decomposer(**opts)
+        # one mesh per call: the knobs tuned for the plain decompose (the batch's engine count and graph
+        # capture are tuned for batches)
+        popts = dict(MIX["options"], **tuned("blob"))
+        peng = cw.Decomposer(**popts)
         t0 = time.perf_counter()
         plain = []
         for n, k in enumerate(sel):
@@ -321,7 +332,7 @@ def main():
         peng.destroy()
         same = all(_same(p, batched[k]) for p, k in zip(plain, sel))
         log(f"plain: {len(sel)} meshes in {tp:.2f} s, identical: {same}")
-        line["plain"] = dict(meshes=len(sel), every=a.plain, meshes_per_s=len(sel) / tp,
+        line["plain"] = dict(meshes=len(sel), every=a.plain, options=popts, meshes_per_s=len(sel) / tp,
                              ratio_batched_vs_plain=line["value"] / (len(sel) / tp), identical=same)
     if a.blob_steps:
         mesh = cw.generate("blobs", BLOB["seed"], BLOB["params"])

@@ -9,12 +9,30 @@ u32, i32, u64, f32, f64 = C.c_uint32, C.c_int32, C.c_uint64, C.c_float, C.c_doub
 P = C.POINTER
 
 
+class Tuning(C.Structure):
+    """cuacd_tuning: launch shapes of the device kernels (0 = default); results do not depend on them."""
+    _fields_ = [("hull_wg_threads", u32), ("hull_grid_groups", u32), ("hull_grid_waves", u32), ("tile_min", u32),
+                ("tile_max", u32), ("hd_tile_samples", u32)]
+
+
+TUNING_KNOBS = tuple(n for n, _ in Tuning._fields_)
+
+
+def tuning(knobs):
+    """A Tuning block from a dict of knob values (None or {}: every knob at its default)."""
+    knobs = dict(knobs or {})
+    bad = sorted(set(knobs) - set(TUNING_KNOBS))
+    if bad:
+        raise ValueError(f"unknown tuning knobs {bad}; known: {list(TUNING_KNOBS)}")
+    return Tuning(**{k: int(v) for k, v in knobs.items()})
+
+
 class TreeConfig(C.Structure):
     _fields_ = [("width", u32), ("width2", u32), ("depth", u32), ("n_concave_edges", u32),
                 ("min_edge_dist", f32), ("hull_max_verts", u32),
                 ("vertex_cap", u64), ("tri_cap", u64), ("edge_cap", u64), ("scratch_words", u64),
                 ("hull_budget_bytes", u64), ("prune_last_level", i32), ("hausdorff_samples", u32),
-                ("device", i32), ("hull_wg_min_verts", u32), ("use_graph", u32)]
+                ("device", i32), ("hull_wg_min_verts", u32), ("use_graph", u32), ("tuning", Tuning)]
 
 
 class MeshBatch(C.Structure):
@@ -67,7 +85,7 @@ class DecomposeOptions(C.Structure):
                 ("vertex_cap", u64), ("tri_cap", u64), ("edge_cap", u64), ("scratch_words", u64),
                 ("hull_budget_bytes", u64), ("hull_max_verts", u32), ("prune_last_level", i32), ("device", i32),
                 ("hull_wg_min_verts", u32), ("batch_lanes", u32), ("tree_graph", u32), ("on_progress", PROGRESS_FN),
-                ("progress_user", C.c_void_p), ("abort_flag", P(i32))]
+                ("progress_user", C.c_void_p), ("abort_flag", P(i32)), ("tuning", Tuning)]
 
 
 class PartDesc(C.Structure):
@@ -133,6 +151,7 @@ TREE_API = {
     "decompose_batch": (C.c_int, [P(MeshBatch), P(DecomposeOptions), P(C.c_void_p)]),
     "decomposer_create": (C.c_int, [P(DecomposeOptions), u64, P(C.c_void_p)]),
     "decomposer_destroy": (None, [C.c_void_p]),
+    "decomposer_tune": (i32, [C.c_void_p, P(DecomposeOptions)]),
     "decomposer_decompose": (C.c_int, [C.c_void_p, P(f32), u32, P(u32), u32, P(C.c_void_p)]),
     "decomposer_decompose_batch": (C.c_int, [C.c_void_p, P(MeshBatch), P(C.c_void_p)]),
     "result_num_parts": (u32, [C.c_void_p]),

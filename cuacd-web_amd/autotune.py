@@ -1,20 +1,34 @@
-"""Autotuner for the performance knobs of the decomposer: the hull kernel split
-`hull_wg_min_verts` and, for batches, the number of engines `batch_lanes`.
+"""Autotuner for the performance knobs of the decomposer: the launch shapes of its kernels and, for
+batches, the number of engines.
 
-The hull kernel gives a part a whole workgroup (512 threads) when the part has at least
-`hull_wg_min_verts` points, and one wavefront (four parts per block) otherwise. A small part leaves
-most of a workgroup idle; a large part on one wavefront runs its insertion rounds 8x narrower. A
-batch is split over `batch_lanes` engines (a backend and its decomposer each) that run at once on
-host threads, so that one engine's host phases overlap another's device work; more engines overlap
-more but share the device and host cores. Neither knob changes results: a tree candidate's volume
-does not depend on the insertion order, an exported hull's face list does not depend on the group
-size, and a mesh's decomposition does not depend on the engine or the other meshes of its batch. So
-they are chosen by measurement, one knob after the other (the split first, with the default
-engines, then the engines with the best split). Every candidate runs the workload, the
-decompositions are checked to be identical, and the lowest median wall time wins.
+Knobs, in the order they are tuned:
+  batch_lanes        batches: engines a batch is split over (a backend and its decomposer each), run at
+                     once on host threads so that one engine's host phases overlap another's device work
+  hull_wg_min_verts  parts with at least this many points get a workgroup in the hull kernel, the rest a
+                     wavefront (four parts per block): a small part leaves most of a workgroup idle, a
+                     large part on one wavefront runs its insertion rounds 8x narrower
+  hull_wg_threads    waves per part of the workgroup hull kernel: 4 waves (256 threads, a half-size LDS
+                     batch, four workgroups per CU) or 8 (512 threads, two per CU)
+  hull_grid_groups   persistent workgroups of that kernel per launch
+  hull_grid_waves    persistent wavefronts of the wavefront-per-part hull kernel
+  tile_min, tile_max tree levels: the least parent elements per tile and the most tiles per job (a tile
+                     is one wavefront's share of a cut)
+  hd_tile_samples    Hausdorff: samples per wavefront tile (fewer samples per tile: more tiles in flight,
+                     each closing sooner)
+  tree_graph         each tree run captured into one HIP graph
 
-The result goes into a JSON file keyed by workload (default `tuned.json` beside this file). Callers
-pass the values back as options; bench.py reads the file.
+None changes a result: tree tables and volumes are the same bits for every tiling (prefix sums in
+parent order, exact sums), a hull does not depend on the group that computes it, a Hausdorff maximum is
+exact over exact minima, and a mesh's decomposition does not depend on its engine or batch. So they are
+chosen by measurement, one knob after the other (coordinate descent from the defaults, each knob at the
+best value found so far for the ones before it, the engine count first). Every candidate runs the
+workload, the decompositions are checked to be identical to the first one, and the lowest median wall
+time wins; the candidates of a knob take turns within each repeat, so that a drift of the device's
+speed over the minutes of tuning weighs on all of them alike.
+
+The result goes into a JSON file keyed by workload (default `tuned.json` beside this file): the chosen
+decompose options under "options", the medians per candidate. bench.py reads it; `options(workload)`
+returns what to pass to decompose.
 
     python cuacd-web_amd/autotune.py --workload blob
     python cuacd-web_amd/autotune.py --workload mix --repeats 2
@@ -29,11 +43,19 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tuned.json")
-# candidates per knob; 1 << 30 puts every part on a wavefront (measured only for single meshes:
-# on the batched mix its largest parts take minutes on one wavefront)
+# knob -> where it goes: a decompose option, or a field of the tuning block (cuacd_tuning)
+TUNING_FIELDS = ("hull_wg_threads", "hull_grid_groups", "hull_grid_waves", "tile_min", "tile_max", "hd_tile_samples")
+# candidates per knob and workload kind (every part on a wavefront, a split of 2^30, was measured on the
+# blob: 31 s against 0.4 s, and is no longer a candidate)
 KNOBS = {
-    "plain": (("hull_wg_min_verts", (256, 512, 1024, 2048, 4096, 1 << 30)),),
-    "batched": (("hull_wg_min_verts", (512, 1024, 2048, 4096)), ("batch_lanes", (1, 2, 3, 4))),
+    "plain": (("hull_wg_min_verts", (256, 512, 1024, 2048)), ("hull_wg_threads", (256, 512)),
+              ("hull_grid_groups", (256, 512, 1024)), ("hull_grid_waves", (1024, 4096)),
+              ("tile_min", (64, 256, 1024)), ("tile_max", (16, 64)), ("hd_tile_samples", (16, 32, 64)),
+              ("tree_graph", (0, 1))),
+    "batched": (("batch_lanes", (2, 3, 4)), ("hull_wg_min_verts", (1024, 2048, 4096)), ("hull_wg_threads", (256, 512)),
+                ("hull_grid_groups", (256, 512, 1024)), ("hull_grid_waves", (2048, 4096, 8192)),
+                ("tile_min", (128, 256, 1024)), ("tile_max", (16, 64)), ("hd_tile_samples", (32, 64)),
+                ("tree_graph", (0, 1))),
 }
 
 # The two workloads the library is measured on: one 56k-triangle mesh through the plain decompose,
@@ -58,8 +80,26 @@ def _package():
     return mod
 
 
+def as_options(knobs):
+    """Decompose options for a dict of knob values: tuning-block knobs go under "tuning"."""
+    out = {k: v for k, v in knobs.items() if k not in TUNING_FIELDS}
+    t = {k: v for k, v in knobs.items() if k in TUNING_FIELDS}
+    if t:
+        out["tuning"] = t
+    return out
+
+
+def options(workload, path=DEFAULT_OUT):
+    """The decompose options the autotuner chose for `workload` ({} when it has not run)."""
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        entry = json.load(f).get(workload, {})
+    return as_options(entry.get("options", {}))
+
+
 def _signature(results):
-    """What must not change with the split: cut records and every part's geometry, volumes and distance."""
+    """What must not change with the knobs: cut records and every part's geometry, volumes and distance."""
     sig = []
     for r in results:
         sig.append(tuple((c["iter"], c["part"], c["best_cut"], c["applied"], c["best_cost"], c["plane"]) for c in r.cuts))
@@ -71,53 +111,70 @@ def _signature(results):
 
 def tune(workload, repeats=3, log=print):
     """Best value and median wall seconds per candidate of every knob of `workload` (a WORKLOADS key);
-    raises if any candidate changes a decomposition."""
+    raises if any candidate changes a decomposition.
+
+    The device's speed drifts over minutes of load (the same setting measured several percent slower
+    a minute later), so the candidates of a knob are timed in turn within every repeat, not one after
+    the other: the engine count first, each run on a fresh engine made and warmed outside the timed
+    call; then the other knobs on one warmed engine, retuned between runs (Decomposer.tune)."""
     cw = _package()
     w = WORKLOADS[workload]
     if w["kind"] == "plain":
-        mesh = cw.generate(w["generator"], w["seed"], w["params"])
-
-        def run(knobs):
-            return [cw.decompose(mesh, **knobs, **w["options"])]
-        run({})  # warm
+        meshes = [cw.generate(w["generator"], w["seed"], w["params"])]
+        batch_tris = 0
     else:
         meshes = [cw.generate(w["generator"], w["seed"], [k]) for k in range(w["count"])]
-        tris = sum(m.nt for m in meshes)
-        engines = {}
+        batch_tris = sum(m.nt for m in meshes)
+    ref = []
 
-        def run(knobs):
-            # one engine per setting, made and warmed outside the timed calls, as bench.py times it
-            key = tuple(sorted(knobs.items()))
-            if key not in engines:
-                for e in engines.values():
-                    e.destroy()
-                engines.clear()
-                engines[key] = cw.Decomposer(batch_tris=tris, **knobs, **w["options"])
-                engines[key].decompose_batch(meshes)
-            return engines[key].decompose_batch(meshes)
-    ref = None
+    def timed(e, label):
+        t0 = time.perf_counter()
+        res = e.decompose_batch(meshes) if batch_tris else [e.decompose(meshes[0])]
+        dt = time.perf_counter() - t0
+        sig = _signature(res)
+        if not ref:
+            ref.append(sig)
+        elif sig != ref[0]:
+            raise RuntimeError(f"{label} changed the decomposition")
+        log(f"{workload}: {label}: {dt:.3f} s", flush=True)
+        return dt
+
+    def engine(knobs):
+        e = cw.Decomposer(batch_tris=batch_tris, **as_options(knobs), **w["options"])
+        e.decompose_batch(meshes) if batch_tris else e.decompose(meshes[0])  # warm
+        return e
+
     chosen, medians = {}, {}
+
+    def settle(knob, times):
+        med = {v: statistics.median(ts) for v, ts in times.items()}
+        chosen[knob] = min(med, key=lambda v: (med[v], v))
+        medians[knob] = {str(v): round(t, 4) for v, t in med.items()}
+        log(f"{workload}: {knob} -> {chosen[knob]} (" + ", ".join(f"{v}: {t:.3f} s" for v, t in med.items()) + ")",
+            flush=True)
+
+    knobs = dict(KNOBS[w["kind"]])
+    if "batch_lanes" in knobs:
+        times = {v: [] for v in knobs["batch_lanes"]}
+        for r in range(repeats):
+            for v in knobs["batch_lanes"]:
+                e = engine({"batch_lanes": v})
+                times[v].append(timed(e, f"batch_lanes={v} run {r}"))
+                e.destroy()
+        settle("batch_lanes", times)
+    e = engine({k: v for k, v in chosen.items()})
+    fixed = set(chosen)
     for knob, candidates in KNOBS[w["kind"]]:
-        times = {}
-        for value in candidates:
-            ts = []
-            for r in range(repeats):
-                knobs = dict(chosen, **{knob: value})
-                if r == 0 and w["kind"] == "batched":
-                    run(knobs)  # makes and warms the engine
-                t0 = time.perf_counter()
-                res = run(knobs)
-                ts.append(time.perf_counter() - t0)
-                log(f"{workload}: {knob}={value} run {r}: {ts[-1]:.3f} s", flush=True)
-                sig = _signature(res)
-                if ref is None:
-                    ref = sig
-                elif sig != ref:
-                    raise RuntimeError(f"{knob}={value} changed the decomposition")
-            times[value] = statistics.median(ts)
-            log(f"{workload}: {knob}={value}: median {times[value]:.3f} s over {repeats}", flush=True)
-        chosen[knob] = min(times, key=lambda k: (times[k], k))
-        medians[knob] = {str(k): round(v, 4) for k, v in times.items()}
+        if knob in fixed:
+            continue
+        times = {v: [] for v in candidates}
+        for r in range(repeats):
+            for v in candidates:
+                live = {k: x for k, x in dict(chosen, **{knob: v}).items() if k not in fixed}
+                e.tune(**as_options(live))
+                times[v].append(timed(e, f"{knob}={v} run {r}"))
+        settle(knob, times)
+    e.destroy()
     return chosen, medians
 
 
@@ -129,7 +186,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     chosen, medians = tune(a.workload, repeats=a.repeats)
     table = json.load(open(a.out)) if os.path.exists(a.out) else {}
-    table[a.workload] = dict(chosen, median_s=medians, repeats=a.repeats,
+    table[a.workload] = dict(options=chosen, median_s=medians, repeats=a.repeats,
                              command=f"python cuacd-web_amd/autotune.py --workload {a.workload} --repeats {a.repeats}")
     with open(a.out, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)

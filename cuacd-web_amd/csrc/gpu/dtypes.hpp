@@ -66,7 +66,8 @@ struct LevelParams {
   uint32_t nSeeds, edgeStride, prune, itemChunk0;  // itemChunk0: first input item of a partial level
   float minEdgeDist;
   uint32_t leavesOnly;  // k_lvl_items: apply the leaf rule to the items and stop (no planes)
-  float padf[2];
+  uint32_t tileMin;     // a tile covers at least this many parent elements (a job has at most maxTiles)
+  uint32_t padu;
   uint64_t vCap, tCap, eCap, sCap;
 };
 

@@ -80,10 +80,14 @@ class GpuBackend : public Backend {
   explicit GpuBackend(const TreeConfig& cfg) : cfg_(cfg), edgeStride_(edgeStrideOf(cfg)) {
     if (cfg.width % 3 != 0 || cfg.width2 % 3 != 0) throw std::invalid_argument("width and width2 must be multiples of 3");
     if (cfg.depth < 1 || cfg.depth > 4) throw std::invalid_argument("depth must be in 1..4");
+    checkTuning(cfg.tuning);
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw DeviceError("no HIP device available");
     if (cfg.device < 0 || cfg.device >= n) throw std::invalid_argument("device ordinal out of range");
     hipCheck(hipSetDevice(cfg.device), "hipSetDevice");
+    int wallKHz = 0;
+    hipCheck(hipDeviceGetAttribute(&wallKHz, hipDeviceAttributeWallClockRate, cfg.device), "hipDeviceGetAttribute");
+    wallKHz_ = wallKHz;
     hipCheck(hipStreamCreate(&stream_), "hipStreamCreate");
     hipCheck(hipStreamCreate(&side_), "hipStreamCreate");
     ctr_.ensure(1);
@@ -192,6 +196,12 @@ class GpuBackend : public Backend {
   std::vector<Level> layout(uint32_t nSeeds, uint32_t depth, uint32_t w0, uint32_t ne0) const;
   dk::LevelParams levelParams(const Level& L, uint32_t level, uint32_t depth, uint32_t nSeeds) const;
   bool growPools(uint32_t overflowBits) override;
+  void retune(uint32_t hullWgMinVerts, bool useGraph, const Tuning& t) override {
+    checkTuning(t);
+    cfg_.hullWgMinVerts = hullWgMinVerts;
+    cfg_.useGraph = useGraph;
+    cfg_.tuning = t;
+  }
   void ensurePools();
   // one tree level: plans, counts, child pools, geometry and volumes (kernels of k_level.hpp)
   void runLevel(const Level& L, const dk::LevelParams& P);
@@ -199,6 +209,26 @@ class GpuBackend : public Backend {
   // levelHulls then allocate nothing, as graph capture requires)
   void reserveLevel(uint32_t nJobs);
   bool capturing_ = false;  // a tree run is being captured into a graph
+  // device wall-clock times of the hull launches of a captured run (HullLevel::clk)
+  DevBuf<unsigned long long> clk_;
+  uint32_t clkSlots_ = 0;
+  std::vector<std::pair<bool, bool>> clkUse_;  // per slot: the workgroup and the wavefront kernel ran
+  double wallKHz_ = 0.0;                       // the device's wall clock (kHz)
+  void readClocks() {
+    if (clkUse_.empty() || wallKHz_ <= 0.0) return;
+    std::vector<unsigned long long> c(4 * clkUse_.size());
+    hipCheck(hipMemcpy(c.data(), clk_.p, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost), "read clocks");
+    for (size_t s = 0; s < clkUse_.size(); s++)
+      for (int k = 0; k < 2; k++) {
+        const bool ran = k == 0 ? clkUse_[s].first : clkUse_[s].second;
+        const unsigned long long a = c[4 * s + 2 * k], b = c[4 * s + 2 * k + 1];
+        if (!ran || b < a) continue;
+        const int fam = k == 0 ? K_HULL_BIG : K_HULL;
+        kst_.ms[fam] += (double)(b - a) / wallKHz_;
+        kst_.launches[fam]++;
+      }
+    clkUse_.clear();
+  }
   template <typename S, typename T>
   void scan(S s, uint32_t maxElems, T* vals);
   // hull the given jobs (slab offsets are assigned here, in arena-sized chunks)
@@ -311,6 +341,7 @@ dk::LevelParams GpuBackend::levelParams(const Level& L, uint32_t level, uint32_t
   P.edgeStride = edgeStride_;
   P.prune = (level == depth - 1 && level > 0 && cfg_.pruneLastLevel) ? 1 : 0;
   P.minEdgeDist = cfg_.minEdgeDist;
+  P.tileMin = cfg_.tuning.tileMin;
   P.vCap = std::min<uint64_t>(cfg_.vertexCap, 0xffffffffull);
   P.tCap = std::min<uint64_t>(cfg_.triCap, 0xffffffffull);
   P.eCap = std::min<uint64_t>(cfg_.edgeCap, 0xffffffffull);
@@ -366,15 +397,15 @@ void GpuBackend::scan(S s, uint32_t maxElems, T* vals) {
 }
 
 namespace {
-// tiles per job: up to 64, fewer when the level has many jobs, so a level has about TILE_CAP tiles
-// at most (or one per job)
-uint32_t maxTilesOf(uint32_t nJobs) {
-  return std::max<uint32_t>(1, std::min<uint32_t>(64, dk::TILE_CAP / std::max<uint32_t>(nJobs, 1)));
+// tiles per job: up to tileMax, fewer when the level has many jobs, so a level has about TILE_CAP
+// tiles at most (or one per job)
+uint32_t maxTilesOf(uint32_t nJobs, uint32_t tileMax) {
+  return std::max<uint32_t>(1, std::min<uint32_t>(tileMax, dk::TILE_CAP / std::max<uint32_t>(nJobs, 1)));
 }
 }  // namespace
 
 void GpuBackend::reserveLevel(uint32_t nJobs) {
-  const uint64_t tileBound = (uint64_t)nJobs * maxTilesOf(nJobs);
+  const uint64_t tileBound = (uint64_t)nJobs * maxTilesOf(nJobs, cfg_.tuning.tileMax);
   if (tileBound >= 0xffffffffull / dk::TC_N) throw std::length_error("tree level exceeds the tile scratch");
   const uint64_t maxElems = std::max<uint64_t>(tileBound, nJobs);
   const uint64_t nChunks = (maxElems + dk::SCAN_CH - 1) / dk::SCAN_CH + 1;
@@ -396,7 +427,7 @@ void GpuBackend::reserveLevel(uint32_t nJobs) {
 }
 
 void GpuBackend::runLevel(const Level& L, const dk::LevelParams& P) {
-  const uint32_t maxTiles = maxTilesOf(L.nJobs);
+  const uint32_t maxTiles = maxTilesOf(L.nJobs, cfg_.tuning.tileMax);
   const uint64_t tileBound = (uint64_t)L.nJobs * maxTiles;
   reserveLevel(L.nJobs);
   constexpr uint32_t GRID = 2048;  // tile kernels: 8192 waves over the level's tiles
@@ -459,38 +490,54 @@ void GpuBackend::ensureArena(uint64_t minWords) {
 void GpuBackend::launchHulls(bool exact, dk::DPart* parts, uint32_t* exportBuf, const dk::HullLevel& HL,
                              uint32_t gridBig, uint32_t gridSmallWaves) {
   // the large-part kernel holds a workgroup per part, usually fewer than the chip's CUs at
-  // its tail: the small parts run beside it on the side stream
-  join(stream_, side_);
+  // its tail: the small parts run beside it on the side stream. Not in a captured run: device
+  // memory that kernels on a stream forked into a capture use is not returned by hipFree once
+  // the graph has run (measured: the hull arena and vertex pool stayed allocated after the
+  // engine was destroyed), so a captured run launches both kernels on the main stream.
+  const bool fork = !capturing_;
+  hipStream_t side = fork ? side_ : stream_;
+  // a captured launch times itself (HL.clk) in the next clock slot
+  dk::HullLevel HLc = HL;
+  HLc.clk = nullptr;
+  if (capturing_ && clkUse_.size() < clkSlots_) {
+    HLc.clk = clk_.p + 4 * clkUse_.size();
+    clkUse_.push_back({gridBig > 0, gridSmallWaves > 0});
+  }
+  if (fork) join(stream_, side_);
   dk::HullQueue* qs = &hq_.p->q[0];
   dk::HullQueue* qb = &hq_.p->q[1];
   if (gridBig) {
     hipEvent_t a = tBegin();
-    if (exact)
-      dk::k_hull_big<true><<<gridBig, dk::HBIG, 0, stream_>>>(hqBig_.p, qb, arenaCtl_.p, verts_.p, arena_.p, parts,
-                                                              hresBig_.p, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+    const uint32_t nt = cfg_.tuning.hullWgThreads;
+    if (exact && nt == 256)
+      dk::k_hull_big<true, 256><<<gridBig, 256, 0, stream_>>>(hqBig_.p, qb, arenaCtl_.p, verts_.p, arena_.p, parts,
+                                                              hresBig_.p, exportBuf, items_.p, jobs_.p, HLc, ctr_.p);
+    else if (exact)
+      dk::k_hull_big<true, 512><<<gridBig, 512, 0, stream_>>>(hqBig_.p, qb, arenaCtl_.p, verts_.p, arena_.p, parts,
+                                                              hresBig_.p, exportBuf, items_.p, jobs_.p, HLc, ctr_.p);
+    else if (nt == 256)
+      dk::k_hull_big<false, 256><<<gridBig, 256, 0, stream_>>>(hqBig_.p, qb, arenaCtl_.p, verts_.p, arena_.p, parts,
+                                                               hresBig_.p, exportBuf, items_.p, jobs_.p, HLc, ctr_.p);
     else
-      dk::k_hull_big<false><<<gridBig, dk::HBIG, 0, stream_>>>(hqBig_.p, qb, arenaCtl_.p, verts_.p, arena_.p, parts,
-                                                               hresBig_.p, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+      dk::k_hull_big<false, 512><<<gridBig, 512, 0, stream_>>>(hqBig_.p, qb, arenaCtl_.p, verts_.p, arena_.p, parts,
+                                                               hresBig_.p, exportBuf, items_.p, jobs_.p, HLc, ctr_.p);
     launched("k_hull_big");
     tEnd(K_HULL_BIG, a);
   }
   if (gridSmallWaves) {
     const uint32_t blocks = (gridSmallWaves + dk::HW - 1) / dk::HW;
-    hipEvent_t a = tBegin(side_);
+    hipEvent_t a = tBegin(side);
     if (exact)
-      dk::k_hull<true><<<blocks, 64 * dk::HW, 0, side_>>>(hqSmall_.p, qs, arenaCtl_.p, verts_.p, arena_.p, parts,
-                                                         hresSmall_.p, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+      dk::k_hull<true><<<blocks, 64 * dk::HW, 0, side>>>(hqSmall_.p, qs, arenaCtl_.p, verts_.p, arena_.p, parts,
+                                                         hresSmall_.p, exportBuf, items_.p, jobs_.p, HLc, ctr_.p);
     else
-      dk::k_hull<false><<<blocks, 64 * dk::HW, 0, side_>>>(hqSmall_.p, qs, arenaCtl_.p, verts_.p, arena_.p, parts,
-                                                          hresSmall_.p, exportBuf, items_.p, jobs_.p, HL, ctr_.p);
+      dk::k_hull<false><<<blocks, 64 * dk::HW, 0, side>>>(hqSmall_.p, qs, arenaCtl_.p, verts_.p, arena_.p, parts,
+                                                          hresSmall_.p, exportBuf, items_.p, jobs_.p, HLc, ctr_.p);
     launched("k_hull");
-    tEnd(K_HULL, a, side_);
+    tEnd(K_HULL, a, side);
   }
-  join(side_, stream_);
+  if (fork) join(side_, stream_);
 }
-
-// persistent grids of the tree levels: two workgroups per CU, four wavefronts per SIMD
-constexpr uint32_t HULL_GRID_BIG = 512, HULL_GRID_SMALL_WAVES = 4096;
 
 void GpuBackend::levelHulls(const Level& L, const dk::HullLevel& HL) {
   const uint32_t nChild = 2 * L.nJobs;
@@ -512,8 +559,8 @@ void GpuBackend::levelHulls(const Level& L, const dk::HullLevel& HL) {
   const uint64_t slabSmall =
       dk::hullSlabWords(dk::hullClassCap(std::min(std::min(split > 0 ? split - 1 : 0, dk::MAX_NV_C16), maxSeedNv_)));
   const uint64_t slabBig = dk::hullSlabWords(dk::hullClassCap(std::max(maxSeedNv_, split)));
-  uint64_t gSmall = std::min<uint64_t>(HULL_GRID_SMALL_WAVES, nChild);
-  uint64_t gBig = std::min<uint64_t>(HULL_GRID_BIG, nChild);
+  uint64_t gSmall = std::min<uint64_t>(cfg_.tuning.hullGridWaves, nChild);
+  uint64_t gBig = std::min<uint64_t>(cfg_.tuning.hullGridGroups, nChild);
   while (gBig * slabBig + gSmall * slabSmall > arena_.n && (gBig > 1 || gSmall > 1)) {
     if (gBig * slabBig >= gSmall * slabSmall && gBig > 1) gBig = (gBig + 1) / 2;
     else if (gSmall > 1) gSmall = (gSmall + 1) / 2;
@@ -548,7 +595,8 @@ void GpuBackend::runHulls(std::vector<dk::HullJob>& hj, uint32_t* exportBuf, std
   const uint64_t slabBig = big.empty() ? 0 : dk::hullSlabWords(dk::hullClassCap(maxBig));
   const uint64_t slabSmall = small.empty() ? 0 : dk::hullSlabWords(dk::hullClassCap(maxSmall));
   ensureArena(slabBig + slabSmall);
-  uint64_t gBig = std::min<uint64_t>(HULL_GRID_BIG, big.size()), gSmall = std::min<uint64_t>(HULL_GRID_SMALL_WAVES, small.size());
+  uint64_t gBig = std::min<uint64_t>(cfg_.tuning.hullGridGroups, big.size());
+  uint64_t gSmall = std::min<uint64_t>(cfg_.tuning.hullGridWaves, small.size());
   while (gBig * slabBig + gSmall * slabSmall > arena_.n) {
     if (gBig * slabBig >= gSmall * slabSmall && gBig > 1) gBig = (gBig + 1) / 2;
     else if (gSmall > 1) gSmall = (gSmall + 1) / 2;
@@ -689,6 +737,12 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
           const uint32_t nc = (l == depth - 1 && l > 0) ? std::min(chunks, L.nItemsIn) : 1;
           if (nc > 1) reserveLevel(((L.nItemsIn + nc - 1) / nc) * L.totalWidth);
         }
+      // clock slots of the captured hull launches, one per level and stage (and chunk), set eagerly
+      clkSlots_ = stages * (depth + chunks) + 4;
+      clk_.ensure(4ull * clkSlots_);
+      clkUse_.clear();
+      dk::k_clk_init<<<(4 * clkSlots_ + 255) / 256, 256, 0, stream_>>>(clk_.p, clkSlots_);
+      launched("k_clk_init");
       hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
       capturing_ = true;
     }
@@ -816,6 +870,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       evPool_.push_back(g1);
       hipCheck(hipGraphExecDestroy(graphExec), "hipGraphExecDestroy");
       hipCheck(hipGraphDestroy(graph), "hipGraphDestroy");
+      readClocks();
     }
     if (sc.error | sc.sticky) {
       hipCheck(hipMemset(scanCtl_.p, 0, sizeof(dk::ScanCtl)), "clear scan control");
@@ -1024,8 +1079,8 @@ std::vector<HullResult> GpuBackend::hullMeshesStaged(const std::vector<MeshRef>&
 }
 
 // The host-side layout of one Hausdorff batch: meshes 0..n-1 are the A sides, n..2n-1 the B sides;
-// per-triangle sample counts from the host rule (src/cpu/hausdorff.ts:27-48), one tile per HDS
-// samples of one direction of one pair (src/cpu/hausdorff.ts:226-231: a pair with an empty side
+// per-triangle sample counts from the host rule (src/cpu/hausdorff.ts:27-48), one tile per
+// `tileSamples` (at most HDS) samples of one direction of one pair (src/cpu/hausdorff.ts:226-231: a pair with an empty side
 // measures 0).
 struct HdPlan {
   std::vector<float> pv;
@@ -1036,7 +1091,8 @@ struct HdPlan {
   size_t nPairs = 0;
 };
 
-static HdPlan hdPlan(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b, uint32_t fixedRes) {
+static HdPlan hdPlan(const std::vector<MeshRef>& a, const std::vector<MeshRef>& b, uint32_t fixedRes,
+                     uint32_t tileSamples) {
   HdPlan P;
   const size_t n = a.size();
   P.nPairs = n;
@@ -1069,11 +1125,11 @@ static HdPlan hdPlan(const std::vector<MeshRef>& a, const std::vector<MeshRef>& 
       const dk::HdMesh& S = P.hm[dir == 0 ? p : n + p];
       const dk::HdMesh& O = P.hm[dir == 0 ? n + p : p];
       const uint32_t ns = P.counts[S.cOff + S.nt];
-      for (uint32_t s0 = 0; s0 < ns; s0 += dk::HDS) {
+      for (uint32_t s0 = 0; s0 < ns; s0 += tileSamples) {
         dk::HdTile t;
         t.pair = (uint32_t)p;
         t.sBegin = S.sOff + s0;
-        t.sCount = std::min<uint32_t>(dk::HDS, ns - s0);
+        t.sCount = std::min<uint32_t>(tileSamples, ns - s0);
         t.gTri = O.gTri;
         t.nt = O.nt;
         t.chOff = O.chOff;
@@ -1150,7 +1206,7 @@ std::vector<double> GpuBackend::hausdorff(const std::vector<MeshRef>& a, const s
   const size_t n = a.size();
   if (n == 0) return {};
   if (n > 1024) throw std::invalid_argument("hausdorff: at most 1024 pairs per call");
-  const HdPlan P = hdPlan(a, b, cfg_.hausdorffSamples);
+  const HdPlan P = hdPlan(a, b, cfg_.hausdorffSamples, cfg_.tuning.hdTileSamples);
   hdVerts_.ensure(P.pv.size());
   hdTris_.ensure(P.pt.size());
   hdCounts_.ensure(P.counts.size());
@@ -1378,7 +1434,7 @@ CUACD_EXPORT int cuacd_hausdorff_noalloc(const cuacd_mesh_batch* a, const cuacd_
     if (a->n != b->n) throw std::invalid_argument("hausdorff: batches differ in size");
     if (a->n > 1024) throw std::invalid_argument("hausdorff: at most 1024 pairs per call");
     if (a->n == 0) return;
-    const cuacd::HdPlan P = cuacd::hdPlan(toRefs(a), toRefs(b), hausdorffSamples);
+    const cuacd::HdPlan P = cuacd::hdPlan(toRefs(a), toRefs(b), hausdorffSamples, dk::HDS);
     int device = 0;
     Carve c{noallocScratch(&device)};
     const cuacd::HdDev D{c.take<float>(P.pv.size()), c.take<uint32_t>(P.pt.size()), c.take<uint32_t>(P.counts.size()),

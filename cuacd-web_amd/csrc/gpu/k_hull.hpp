@@ -45,12 +45,14 @@
 namespace dk {
 
 constexpr int HW = 4;         // parts per block in the wave-per-part hull kernel
-constexpr int HBIG = 512;     // threads per part in the workgroup-per-part hull kernel
 constexpr uint32_t ALIVE = 0x80000000u;
 constexpr uint32_t FW = 20;                 // words per face record (five uint4 words, one array each)
 
-// batch shape: LDS visible-face capacity, edge-hash slots, staged new faces
+// batch shape: LDS visible-face capacity, edge-hash slots, staged new faces. The workgroup kernel's
+// batch scales with its threads, so that it runs at four waves per SIMD either way: two workgroups
+// of 512 threads per CU (under 80 KB each), or four of 256 (under 40 KB each)
 constexpr uint32_t LCAP_BIG = 512, HT_BIG = 4096, NCAP_BIG = 256;
+constexpr uint32_t LCAP_B4 = 256, HT_B4 = 2048, NCAP_B4 = 96;
 constexpr uint32_t LCAP_W = 128, HT_W = 1024, NCAP_W = 48;
 // tree candidates in the wavefront kernel: parts below 2^16 points, so the round state holds vertex
 // indices in 16 bits and directed edges in 32, and stages fewer new faces (occupancy is bound by LDS)
@@ -1274,7 +1276,24 @@ struct HullLevel {  // branch-and-bound inputs at the last tree level
   // so far (the grid volume for tree candidates, the surface volume for exported hulls).
   uint32_t stage;
   const double* seedUb;  // per seed: the best leaf-path bound of the level's input items (k_seed_ub)
+  // A launch that HIP events cannot time (one captured into a graph) times itself on the device's
+  // wall clock: the earliest start and the latest end of its groups, at clk[0] and clk[1] for the
+  // workgroup kernel, clk[2] and clk[3] for the wavefront kernel (nullptr: not timed).
+  unsigned long long* clk;
 };
+
+// the clock slots of a captured run's hull launches: starts at the largest value, ends at 0
+__global__ void k_clk_init(unsigned long long* __restrict__ c, uint32_t nSlots) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 4 * nSlots) c[i] = (i & 1) ? 0ull : ~0ull;
+}
+
+__device__ __forceinline__ void hull_clock(unsigned long long* c, bool end) {
+  if (c) {
+    if (end) atomicMax(c + 1, (unsigned long long)wall_clock64());
+    else atomicMin(c, (unsigned long long)wall_clock64());
+  }
+}
 
 // The branch-and-bound upper bound of each seed at the last level, once per level: the smallest
 // partial cost plus the last level cost carried to the full depth, over the seed's input items
@@ -1802,6 +1821,8 @@ __global__ __launch_bounds__(64 * HW, EXACT ? 2 : 4) void k_hull(const HullJob* 
   const uint32_t count = q->count;
   unsigned long long off = 0;
   uint32_t words = 0;
+  unsigned long long* const clk = HL.clk ? HL.clk + 2 : nullptr;
+  if (lane_id() == 0) hull_clock(clk, false);
   for (;;) {
     uint32_t j = 0;
     if (lane_id() == 0) {
@@ -1820,13 +1841,15 @@ __global__ __launch_bounds__(64 * HW, EXACT ? 2 : 4) void k_hull(const HullJob* 
              &ctr->faceTests, &ctr->hullBytes);
     wave_sync();
   }
+  if (lane_id() == 0) hull_clock(clk, true);
 }
 
-// large parts: one workgroup of HBIG threads per part. Rounds are latency-bound,
-// so two workgroups share a CU (4 waves per SIMD: at most 128 VGPRs; the LDS
-// batch is under half of the CU's 160 KB).
-template <bool EXACT>
-__global__ __launch_bounds__(HBIG, 4) void k_hull_big(const HullJob* __restrict__ hjobs, HullQueue* __restrict__ q,
+// large parts: one workgroup of NT threads (256 or 512) per part. Rounds are
+// latency-bound, so several workgroups share a CU at 4 waves per SIMD (at most
+// 128 VGPRs): two of 512 threads (an LDS batch under half of the CU's 160 KB) or
+// four of 256 (half that batch).
+template <bool EXACT, int NT>
+__global__ __launch_bounds__(NT, 4) void k_hull_big(const HullJob* __restrict__ hjobs, HullQueue* __restrict__ q,
                                                    HullArena* __restrict__ ar, const float* __restrict__ verts,
                                                    uint32_t* __restrict__ arena, DPart* __restrict__ parts,
                                                    HullResultD* __restrict__ results, uint32_t* __restrict__ exportBuf,
@@ -1834,14 +1857,16 @@ __global__ __launch_bounds__(HBIG, 4) void k_hull_big(const HullJob* __restrict_
                                                    HullLevel HL, Counters* __restrict__ ctr) {
   constexpr uint32_t KB = EXACT ? KE : KF_BIG;
   __shared__ uint32_t s_seed[80];
-  __shared__ uint32_t s_u[HBIG / 64];
-  __shared__ float s_f[HBIG / 64];
-  __shared__ unsigned long long s_l[3 * (HBIG / 64)];
-  __shared__ uint32_t s_m[4 * 16 * (HBIG / 64)];
-  __shared__ HullBatch<EXACT, KB, LCAP_BIG, HT_BIG, HBIG / 64, NCAP_BIG> s_batch;
+  __shared__ uint32_t s_u[NT / 64];
+  __shared__ float s_f[NT / 64];
+  __shared__ unsigned long long s_l[3 * (NT / 64)];
+  __shared__ uint32_t s_m[4 * 16 * (NT / 64)];
+  static_assert(NT == 512 || NT == 256, "workgroup sizes");
+  constexpr bool W8 = NT == 512;
+  __shared__ HullBatch<EXACT, KB, W8 ? LCAP_BIG : LCAP_B4, W8 ? HT_BIG : HT_B4, NT / 64, W8 ? NCAP_BIG : NCAP_B4> s_batch;
   __shared__ uint32_t s_j, s_words;
   __shared__ unsigned long long s_off;
-  BlockGrp<HBIG> g;
+  BlockGrp<NT> g;
   g.su = s_u;
   g.sf = s_f;
   g.sl = s_l;
@@ -1850,6 +1875,7 @@ __global__ __launch_bounds__(HBIG, 4) void k_hull_big(const HullJob* __restrict_
   if (threadIdx.x == 0) {
     s_off = 0;
     s_words = 0;
+    hull_clock(HL.clk, false);
   }
   for (;;) {
     __syncthreads();  // the previous job is done with the shared state
@@ -1871,6 +1897,7 @@ __global__ __launch_bounds__(HBIG, 4) void k_hull_big(const HullJob* __restrict_
     hull_job(g, s_batch, J, j, s_seed, verts, arena, parts, results, exportBuf, items, jobs, HL, ctr,
              &ctr->faceTestsBig, &ctr->hullBytesBig);
   }
+  if (threadIdx.x == 0) hull_clock(HL.clk, true);
 }
 
 // ---- hull queues of a tree level, built on the device ----------------------------------

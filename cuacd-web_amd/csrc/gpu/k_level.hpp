@@ -40,7 +40,6 @@
 namespace dk {
 
 constexpr uint32_t LT = 256;          // threads per block of the tile kernels (four tiles in flight)
-constexpr uint32_t TILE_MIN = 256;    // smallest tile (triangles, vertices or edges)
 constexpr uint32_t TILE_CAP = 1u << 22;  // tiles per level are kept below this (bounds the tile tables)
 
 // per-tile counts, in this order
@@ -146,7 +145,7 @@ __global__ __launch_bounds__(LT) void k_lvl_items(LevelParams P, DItem* __restri
   }
   if (P.leavesOnly) return;  // a pass that only marks the level's early leaves (before its bounds)
   const uint32_t maxSize = part.nt > part.nv ? (part.nt > part.ne ? part.nt : part.ne) : (part.nv > part.ne ? part.nv : part.ne);
-  uint32_t nT = (maxSize + TILE_MIN - 1) / TILE_MIN;
+  uint32_t nT = (maxSize + P.tileMin - 1) / P.tileMin;
   nT = nT < 1 ? 1 : (nT > maxTiles ? maxTiles : nT);
   // planes, 256 cuts at a time; ok cuts get side-flag scratch in one reservation per item
   for (uint32_t c0 = 0; c0 < W; c0 += LT) {

@@ -16,6 +16,30 @@
 namespace cuacd {
 
 // src/gpu/tree.ts:14-46 TreeConfig / DEFAULT_TREE_CONFIG
+// Launch shapes of the device kernels. No value changes a result: tree tables, volumes, hull face
+// lists and distances are the same bits whatever the tiling, the group size or the grid (every
+// position comes from prefix sums in parent order, every volume from an exact sum, a hull does not
+// depend on the group that computes it, and a Hausdorff maximum is exact over exact minima). They
+// are chosen per workload by measurement.
+struct Tuning {
+  uint32_t hullWgThreads = 512;   // threads per part of the workgroup hull kernel: 256 or 512
+  uint32_t hullGridGroups = 512;  // persistent workgroups of that kernel per launch (two per CU)
+  uint32_t hullGridWaves = 4096;  // persistent wavefronts of the wavefront-per-part hull kernel
+  uint32_t tileMin = 256;         // tree levels: a tile covers at least this many parent elements
+  uint32_t tileMax = 64;          // tree levels: tiles per job at most
+  uint32_t hdTileSamples = 64;    // Hausdorff: samples per wavefront tile
+};
+
+// throws std::invalid_argument for a value outside the kernels' limits
+inline void checkTuning(const Tuning& t) {
+  if (t.hullWgThreads != 256 && t.hullWgThreads != 512) throw std::invalid_argument("hull_wg_threads must be 256 or 512");
+  if (t.hullGridGroups < 1 || t.hullGridGroups > 65536) throw std::invalid_argument("hull_grid_groups must be in 1..65536");
+  if (t.hullGridWaves < 1 || t.hullGridWaves > (1u << 20)) throw std::invalid_argument("hull_grid_waves must be in 1..2^20");
+  if (t.tileMin < 1 || t.tileMin > (1u << 24)) throw std::invalid_argument("tile_min must be in 1..2^24");
+  if (t.tileMax < 1 || t.tileMax > 64) throw std::invalid_argument("tile_max must be in 1..64");
+  if (t.hdTileSamples < 1 || t.hdTileSamples > 64) throw std::invalid_argument("hd_tile_samples must be in 1..64");
+}
+
 struct TreeConfig {
   uint32_t width = 60;
   uint32_t width2 = 5;
@@ -41,6 +65,7 @@ struct TreeConfig {
   // Each tree run captured into one HIP graph and launched once (per-kernel event timing is then
   // not available: the run is timed as a whole). Results do not depend on it.
   bool useGraph = false;
+  Tuning tuning;
 };
 
 // A HIP runtime failure (reported as CUACD_ERR_DEVICE at the C ABI).
@@ -200,6 +225,13 @@ class Backend {
   virtual bool growPools(uint32_t overflowBits) {
     (void)overflowBits;
     return false;
+  }
+  // The performance knobs of a live engine (the hull split, graph capture, the launch shapes): the
+  // next calls run with them. Results do not depend on them; a backend without kernels ignores them.
+  virtual void retune(uint32_t hullWgMinVerts, bool useGraph, const Tuning& t) {
+    (void)hullWgMinVerts;
+    (void)useGraph;
+    checkTuning(t);
   }
   virtual ~Backend() {}
   virtual const TreeConfig& config() const = 0;

@@ -43,7 +43,19 @@ TreeConfig treeConfigFor(const DecomposeOptions& o) {
   if (o.hullWgMinVerts) c.hullWgMinVerts = o.hullWgMinVerts;
   c.hausdorffSamples = o.hausdorffSamples;
   c.useGraph = o.treeGraph;
+  checkTuning(o.tuning);
+  c.tuning = o.tuning;
   return c;
+}
+
+void Decomposer::retune(const DecomposeOptions& o) {
+  DecomposeOptions n = o_;
+  n.hullWgMinVerts = o.hullWgMinVerts;
+  n.treeGraph = o.treeGraph;
+  n.tuning = o.tuning;
+  const TreeConfig c = treeConfigFor(n);
+  be_.retune(c.hullWgMinVerts, c.useGraph, c.tuning);
+  o_ = n;
 }
 
 TreeConfig treeConfigForBatch(const DecomposeOptions& o, uint64_t totalTris) {

@@ -87,6 +87,7 @@ struct DecomposeOptions {
   uint32_t hullWgMinVerts = 0;  // 0 = default (TreeConfig::hullWgMinVerts)
   uint32_t batchLanes = 0;      // engines a batch is split over; 0 = default (2 for a batch)
   bool treeGraph = false;       // each tree run captured into one HIP graph (TreeConfig::useGraph)
+  Tuning tuning;                // launch shapes of the device kernels (results do not depend on them)
   ProgressFn onProgress;                   // src/decompose.ts:63 onProgress
   const volatile int32_t* abortFlag = nullptr;  // src/decompose.ts:62 signal: nonzero aborts at the next boundary
 };
@@ -174,6 +175,9 @@ class Decomposer {
   // each result equal decompose() of that mesh alone; stats and kernel times are the
   // batch's. decompose(m) is the batch of one.
   std::vector<DecomposeResult> decomposeBatch(const std::vector<const Mesh*>& inputs);
+  // takes the performance knobs of `o` (hullWgMinVerts, treeGraph, tuning) for the next calls;
+  // every other field of `o` is ignored
+  void retune(const DecomposeOptions& o);
 
  private:
   struct State;

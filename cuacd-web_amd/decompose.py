@@ -74,11 +74,12 @@ class DecomposeOptions:
     tree_graph: bool = False  # each tree run captured into one HIP graph; results unchanged
     on_progress: object = None  # src/decompose.ts:63 onProgress: called with a dict of ProgressInfo fields
     abort: object = None  # src/decompose.ts:62 signal: an AbortFlag; set() stops the call with AbortError
+    tuning: dict = None  # launch shapes of the engines' kernels (F.TUNING_KNOBS; 0 or absent: default)
 
     def raw(self):
         r = F.DecomposeOptions()
         for f in fields(self):
-            if f.name in ("on_progress", "abort"):
+            if f.name in ("on_progress", "abort", "tuning"):
                 continue
             v = getattr(self, f.name)
             setattr(r, f.name, int(v) if isinstance(v, bool) else v)
@@ -97,6 +98,7 @@ class DecomposeOptions:
         if self.abort is not None:
             r.abort_flag = C.pointer(self.abort.flag)
             keep.append(self.abort)
+        r.tuning = F.tuning(self.tuning)
         r._keep = keep  # the callback and the flag live as long as the options that point at them
         return r
 
@@ -218,6 +220,17 @@ class Decomposer:
         hs = (C.c_void_p * max(len(meshes), 1))()
         check(self.api, self.api.decomposer_decompose_batch(self._h, C.byref(b), hs))
         return [_read_result(self.api, C.c_void_p(hs[k])) for k in range(len(meshes))]
+
+    def tune(self, **knobs):
+        """Sets the engine's performance knobs for its next calls (hull_wg_min_verts, tree_graph, tuning; a
+        knob not given goes back to its default); its pools are kept. Results do not depend on them."""
+        o = DecomposeOptions()
+        for k, v in knobs.items():
+            if k not in ("hull_wg_min_verts", "tree_graph", "tuning"):
+                raise ValueError(f"{k} is fixed when the engine is made")
+            setattr(o, k, v)
+        raw = o.raw()
+        check(self.api, self.api.decomposer_tune(self._h, C.byref(raw)))
 
     def destroy(self):
         if self._h:

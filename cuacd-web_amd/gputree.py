@@ -29,6 +29,7 @@ class TreeConfig:
     device: int = 0
     hull_wg_min_verts: int = 0  # parts with at least this many points: a workgroup each (0: 1024); results unchanged
     use_graph: bool = False  # each run captured into one HIP graph and launched once; results unchanged
+    tuning: dict = None  # launch shapes of the kernels (F.TUNING_KNOBS; 0 or absent: default); results unchanged
 
 
 DEFAULT_TREE_CONFIG = TreeConfig()
@@ -91,7 +92,7 @@ class GpuTree:
         raw = F.TreeConfig(c.width, c.width2, c.depth, c.n_concave_edges, c.min_edge_dist, c.hull_max_verts,
                            c.vertex_cap, c.tri_cap, c.edge_cap, c.scratch_words, c.hull_budget_bytes,
                            1 if c.prune_last_level else 0, c.hausdorff_samples, c.device, c.hull_wg_min_verts,
-                           1 if c.use_graph else 0)
+                           1 if c.use_graph else 0, F.tuning(c.tuning))
         h = C.c_void_p()
         check(self.api, self.api.tree_create(C.byref(raw), C.byref(h)))
         self._h = h

@@ -176,6 +176,24 @@ static int opt_num(napi_env env, napi_value obj, const char* key, double* out) {
 #define OPT_F64(obj, key, field) \
   if (opt_num(env, obj, key, &d)) o.field = d
 
+/* obj.tuning: the kernels' launch shapes (cuacd_tuning) under camelCase names; absent keys keep t */
+static void opt_tuning(napi_env env, napi_value obj, cuacd_tuning* t) {
+  bool has = false;
+  napi_value v;
+  napi_valuetype vt;
+  if (!obj || napi_has_named_property(env, obj, "tuning", &has) != napi_ok || !has) return;
+  if (napi_get_named_property(env, obj, "tuning", &v) != napi_ok || napi_typeof(env, v, &vt) != napi_ok ||
+      vt != napi_object)
+    return;
+  double d;
+  if (opt_num(env, v, "hullWgThreads", &d)) t->hull_wg_threads = (uint32_t)d;
+  if (opt_num(env, v, "hullGridGroups", &d)) t->hull_grid_groups = (uint32_t)d;
+  if (opt_num(env, v, "hullGridWaves", &d)) t->hull_grid_waves = (uint32_t)d;
+  if (opt_num(env, v, "tileMin", &d)) t->tile_min = (uint32_t)d;
+  if (opt_num(env, v, "tileMax", &d)) t->tile_max = (uint32_t)d;
+  if (opt_num(env, v, "hdTileSamples", &d)) t->hd_tile_samples = (uint32_t)d;
+}
+
 static napi_value typed(napi_env env, napi_typedarray_type type, size_t n, size_t elem, void** data) {
   napi_value ab, arr;
   if (napi_create_arraybuffer(env, n * elem, data, &ab) != napi_ok) return NULL;
@@ -451,6 +469,7 @@ static napi_value CreateTree(napi_env env, napi_callback_info info) {
   OPT_I32(c, "device", device);
   OPT_U32(c, "hullWgMinVerts", hull_wg_min_verts);
   OPT_U32(c, "useGraph", use_graph);
+  opt_tuning(env, c, &o.tuning);
   TreeBox* b = calloc(1, sizeof *b);
   const int rc = L.tree_create(&o, &b->t);
   if (rc != CUACD_OK) {
@@ -812,6 +831,7 @@ static napi_value Decompose(napi_env env, napi_callback_info info) {
   OPT_U32(tree, "hullWgMinVerts", hull_wg_min_verts);
   OPT_U32(tree, "batchLanes", batch_lanes);
   OPT_U32(tree, "useGraph", tree_graph);
+  opt_tuning(env, tree, &o.tuning);
   Progress prog;
   memset(&prog, 0, sizeof prog);
   prog.env = env;

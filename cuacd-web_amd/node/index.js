@@ -29,7 +29,8 @@ function decompose(mesh, options) {
 }
 
 class GpuTree {
-  // new GpuTree(cfg) (src/gpu/tree.ts:139): cfg with TreeConfig's names, plus useGraph
+  // new GpuTree(cfg) (src/gpu/tree.ts:139): cfg with TreeConfig's names, plus useGraph and
+  // tuning {hullWgThreads, hullGridGroups, hullGridWaves, tileMin, tileMax, hdTileSamples}
   constructor(cfg) {
     ready();
     this.handle = addon.createTree(cfg || {});

@@ -28,6 +28,18 @@ typedef struct cuacd_result cuacd_result;
 typedef struct cuacd_meshes cuacd_meshes;
 typedef struct cuacd_decomposer cuacd_decomposer;
 
+/* Launch shapes of the device kernels: a tuning block with no reference counterpart. Results do
+ * not depend on any of them (tree tables, volumes, hull faces and distances are the same bits);
+ * the autotuner (cuacd-web_amd/autotune.py) picks them per workload. 0 = the default in brackets. */
+typedef struct cuacd_tuning {
+  uint32_t hull_wg_threads;  /* threads per part of the workgroup hull kernel: 256 or 512 [512] */
+  uint32_t hull_grid_groups; /* persistent workgroups of that kernel per launch, 1..65536 [512] */
+  uint32_t hull_grid_waves;  /* persistent wavefronts of the wavefront-per-part hull kernel, 1..2^20 [4096] */
+  uint32_t tile_min;         /* tree levels: least parent elements (triangles, vertices, edges) per tile [256] */
+  uint32_t tile_max;         /* tree levels: most tiles per job, 1..64 [64] */
+  uint32_t hd_tile_samples;  /* Hausdorff: samples per wavefront tile, 1..64 [64] */
+} cuacd_tuning;
+
 /* src/gpu/tree.ts:14-46 TreeConfig (+ device ordinal, Hausdorff sample count) */
 typedef struct cuacd_tree_config {
   uint32_t width, width2, depth, n_concave_edges;
@@ -45,6 +57,7 @@ typedef struct cuacd_tree_config {
    * launched once, as the reference encodes a run into one command buffer (src/gpu/tree.ts:559-606);
    * 0: the same kernels launched one by one, each timed with HIP events. Results are the same. */
   uint32_t use_graph;
+  cuacd_tuning tuning;
 } cuacd_tree_config;
 
 /* A batch of meshes packed as the reference uploads them (src/gpu/tree.ts:240-262):
@@ -211,6 +224,7 @@ typedef struct cuacd_decompose_options {
   const volatile int32_t* abort_flag; /* src/decompose.ts:62 signal: when *abort_flag becomes nonzero the call
                                          stops at the next iteration, phase or merge-round boundary and
                                          returns CUACD_ERR_ABORTED; NULL: never */
+  cuacd_tuning tuning;                /* launch shapes of the engines' kernels (results do not depend on them) */
 } cuacd_decompose_options;
 
 /* src/part.ts:9-22 Part, flattened */
@@ -273,6 +287,11 @@ int cuacd_decompose_batch(const cuacd_mesh_batch* meshes, const cuacd_decompose_
  * triangles (as cuacd_decompose_batch sizes them per call); 0 sizes them for single meshes. */
 int cuacd_decomposer_create(const cuacd_decompose_options* o, uint64_t batch_tris, cuacd_decomposer** out);
 void cuacd_decomposer_destroy(cuacd_decomposer* d);
+/* No reference counterpart (a tuning hook for autotuners): the engine takes the performance knobs of
+ * `o` (hull_wg_min_verts, tree_graph and the tuning block, with 0 meaning the default as at create) for
+ * its next calls, keeping its pools; every other field of `o` is ignored. Results do not depend on
+ * these knobs. The number of engines (batch_lanes) is fixed at create. */
+int cuacd_decomposer_tune(cuacd_decomposer* d, const cuacd_decompose_options* o);
 /* Decomposer.decompose(mesh) (src/decompose.ts:214-398); kernel stats of the result cover this call */
 int cuacd_decomposer_decompose(cuacd_decomposer* d, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
                                cuacd_result** out);

@@ -152,3 +152,25 @@ def test_decompose_tree_graph(cw):
     got = cw.decompose(mesh, tree_graph=True, **opts)
     _same(ref, got)
     assert got.kernels["graphs"] == got.stats["tree_runs"] and ref.kernels["graphs"] == 0
+
+
+def test_decompose_tuning(cw):
+    """decompose and a batch with non-default launch shapes equal the default decompositions."""
+    mesh = cw.generate("blobs", 20260211, [6, 40, -0.6, 0.6, 0.12, 0.30, 0.5])
+    opts = dict(threshold=0.05, width=12, width2=6, depth=3, n_concave_edges=8, refine_rounds=1, refine_probes=4,
+                merge_hulls=True, hausdorff_samples=500, hull_wg_min_verts=256)
+    ref = cw.decompose(mesh, **opts)
+    tuned = dict(hull_wg_threads=256, hull_grid_groups=1024, tile_min=64, tile_max=16, hd_tile_samples=16)
+    _same(ref, cw.decompose(mesh, tuning=tuned, **opts))
+    e = cw.Decomposer(**opts)
+    e.tune(hull_wg_min_verts=256, tree_graph=True, tuning=tuned)  # a live engine retuned between calls
+    _same(ref, e.decompose(mesh))
+    e.tune(hull_wg_min_verts=4096)
+    _same(ref, e.decompose(mesh))
+    e.destroy()
+    meshes = [cw.generate("mix", 20260400 + k, [1]) for k in range(3)]
+    bopts = dict(threshold=0.1, width=12, depth=3, merge_hulls=True, hausdorff_samples=500)
+    want = cw.decompose_batch(meshes, **bopts)
+    got = cw.decompose_batch(meshes, tuning=tuned, **bopts)
+    for a, b in zip(want, got):
+        _same(a, b)

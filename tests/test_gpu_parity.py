@@ -288,8 +288,13 @@ def test_tree_graph_equals_eager(cw):
             assert (q["mesh_vol"], q["hull_vol"], q["nv"]) == (vols[p]["mesh_vol"], vols[p]["hull_vol"], vols[p]["nv"])
         again = eager.run(meshes, planes, refine=refine)  # eager work between the graph runs
         assert again.levels == want.levels
-    ks = graph.kernel_stats()
+    ks, ke = graph.kernel_stats(), eager.kernel_stats()
     assert ks["graphs"] == 3 and ks["graph_ms"] > 0 and ks["launches"]["emit"] == 0
+    # the captured hull launches time themselves on the device clock: the eager runs' launches (4
+    # runs) per graph run (3), each launch inside its graph's own time (one stream in a graph)
+    for k in ("hull", "hull_big"):
+        assert ks["launches"][k] * 4 == ke["launches"][k] * 3 and (ks["ms"][k] > 0) == (ks["launches"][k] > 0), k
+    assert ks["launches"]["hull"] > 0 and ks["ms"]["hull"] + ks["ms"]["hull_big"] <= ks["graph_ms"]
     # the chunked last level: the first capture overflows, the run is captured again in chunks (the
     # pool sized from the tree stage's own parts, read after a run without refinement)
     eager.run(meshes, planes)
@@ -349,3 +354,73 @@ def test_chunked_last_level_with_early_leaves(cw, oracle):
         assert rg.levels[:-1] == ro.levels[:-1] and rg.pruned == ro.pruned
         g.destroy()
     o.destroy()
+
+
+# launch shapes: the 256-thread hull workgroup with its half-size LDS batch, grids far below and above
+# the chip's group slots, one tile per job and tiles of a single element, Hausdorff tiles of 1..33 samples
+TUNINGS = [dict(hull_wg_threads=256), dict(hull_wg_threads=256, hull_grid_groups=1024, hull_grid_waves=512),
+           dict(hull_grid_groups=3, hull_grid_waves=5), dict(tile_min=1), dict(tile_min=1 << 20),
+           dict(tile_max=1, tile_min=64), dict(hd_tile_samples=1), dict(hd_tile_samples=33, tile_max=7)]
+
+
+def test_tuning_leaves_results_unchanged(cw):
+    """Every launch shape (cuacd_tuning) gives the default's tree tables, cuts, part volumes, exported
+    hull faces and Hausdorff distances, bit for bit. The large-part kernel takes every part of at least
+    64 points here, so both of its workgroup sizes run the tree candidates and the exported hulls."""
+    meshes = [cw.normalize(m)[0] for m in (stairs(cw), square_torus(cw), l_prism(cw))]
+    meshes.append(cw.normalize(cw.generate("blobs", 20260201, [5, 24]))[0])
+    planes = [cw.concave_edge_planes(m, n_concave_edges=8, seed=5 + i) for i, m in enumerate(meshes)]
+    refine = dict(rounds=1, probes=4, spacing=[0.05, 0.08, 0.1, 0.05])
+    cfg = dict(width=12, width2=6, depth=3, n_concave_edges=8, min_edge_dist=0.0125, hull_wg_min_verts=64)
+    hulls = _hull_cases(cw)
+    base = cw.GpuTree(**cfg)
+    want = base.run(meshes, planes, refine=refine)
+    nparts = len(meshes) + 2 * sum(want.jobs_per_level)
+    vols = [base.read_part(p, geometry=False) for p in range(0, nparts, 5)]
+    want_h = base.hull_meshes(hulls)
+    pairs = [(m, cw.Mesh(m.verts, h[0])) for m, h in zip(meshes, base.hull_meshes(meshes))]
+    pairs.append((cw.unit_cube(), cw.make_box((0.1, 0, 0), (1.1, 1, 1))))
+    want_d = base.hausdorff(pairs)
+    base.destroy()
+    for t in TUNINGS:
+        print(f"tuning {t}", flush=True)
+        g = cw.GpuTree(tuning=t, **cfg)
+        got = g.run(meshes, planes, refine=refine)
+        assert [(s.best_cut, s.best_cost, s.plane) for s in got.seeds] == [(s.best_cut, s.best_cost, s.plane) for s in want.seeds]
+        assert got.levels == want.levels and got.hulls_done == want.hulls_done and got.pruned == want.pruned
+        for k, p in enumerate(range(0, nparts, 5)):
+            q = g.read_part(p, geometry=False)
+            assert (q["mesh_vol"], q["hull_vol"], q["nv"], q["nt"]) == (vols[k]["mesh_vol"], vols[k]["hull_vol"], vols[k]["nv"], vols[k]["nt"])
+        for k, (a, b) in enumerate(zip(g.hull_meshes(hulls), want_h)):
+            assert (a is None) == (b is None), k
+            if a is not None:
+                assert a[1] == b[1] and np.array_equal(a[0], b[0]), (t, k)
+        assert np.array_equal(g.hausdorff(pairs), want_d), t
+        g.destroy()
+
+
+def test_tree_graph_returns_device_memory(cw):
+    """Engines that captured their runs into graphs give all their device memory back when destroyed
+    (the hull arena alone is 16 GiB, so a leak shows at once); the hull kernels of a captured run are
+    launched on one stream, since memory used on a stream forked into a capture is not returned."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+
+    def free_gib():
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        assert hip.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) == 0
+        return f.value / 2 ** 30
+
+    mesh = cw.normalize(cw.generate("blobs", 20260201, [5, 24]))[0]
+    cfg = dict(width=12, width2=6, depth=3, hull_wg_min_verts=256)
+    g = cw.GpuTree(**cfg)
+    want = g.run([mesh])
+    g.destroy()
+    f0 = free_gib()
+    for k in range(3):
+        g = cw.GpuTree(use_graph=True, **cfg)
+        got = g.run([mesh])
+        assert got.levels == want.levels
+        g.destroy()
+        print(f"graph engine {k}: {f0 - free_gib():.2f} GiB not returned", flush=True)
+    assert f0 - free_gib() < 2.0

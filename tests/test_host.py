@@ -167,3 +167,44 @@ def test_oracle_exports_tree_surface(cw, oracle):
     from cuacd_web_amd._ffi import TREE_API
     for n in TREE_API:
         assert hasattr(oracle.lib, "oracle_" + n), n
+
+
+def test_tuning_block_defaults_and_limits(cw, oracle):
+    """cuacd_tuning: tree_config_default reports the kernels' launch shapes; a value outside the kernels'
+    limits is an argument error when a tree or a decompose call takes it; an unknown knob name fails in
+    the binding."""
+    from cuacd_web_amd import _ffi as F
+    c = F.TreeConfig()
+    oracle.tree_config_default(ctypes.byref(c))
+    got = {k: getattr(c.tuning, k) for k in F.TUNING_KNOBS}
+    assert got == dict(hull_wg_threads=512, hull_grid_groups=512, hull_grid_waves=4096, tile_min=256, tile_max=64,
+                       hd_tile_samples=64)
+    for bad in (dict(hull_wg_threads=384), dict(hull_wg_threads=1024), dict(tile_max=65), dict(hd_tile_samples=65),
+                dict(hull_grid_groups=1 << 17), dict(tile_min=1 << 25)):
+        with pytest.raises(cw.CuacdError) as e:
+            cw.GpuTree(api=oracle, width=3, width2=3, depth=1, tuning=bad)
+        assert e.value.code == F.ERR_ARG and next(iter(bad)) in str(e.value)
+        with pytest.raises(cw.CuacdError):
+            cw.decompose(cw.unit_cube(), api=oracle, tuning=bad)
+    with pytest.raises(ValueError):
+        cw.GpuTree(api=oracle, tuning=dict(waves_per_part=4))
+    t = cw.GpuTree(api=oracle, width=3, width2=3, depth=1, tuning=dict(hull_wg_threads=256, tile_min=1, tile_max=1))
+    assert t.run([cw.unit_cube()]).seeds[0].best_cost == 0.0
+    t.destroy()
+
+
+def test_decomposer_tune(cw, oracle):
+    """cuacd_decomposer_tune: a live engine takes new knobs between calls, with the same results; a value
+    outside the kernels' limits is an argument error; the engine count is fixed at create."""
+    from cuacd_web_amd import _ffi as F
+    e = cw.Decomposer(api=oracle, threshold=0.05, width=6, width2=3, depth=2, max_iters=2)
+    want = e.decompose(cw.l_shape())
+    e.tune(hull_wg_min_verts=64, tree_graph=True, tuning=dict(hull_wg_threads=256, tile_min=1))
+    got = e.decompose(cw.l_shape())
+    assert [p.mesh_vol for p in got.parts] == [p.mesh_vol for p in want.parts] and got.cuts == want.cuts
+    with pytest.raises(cw.CuacdError) as err:
+        e.tune(tuning=dict(hd_tile_samples=0x100))
+    assert err.value.code == F.ERR_ARG
+    with pytest.raises(ValueError):
+        e.tune(batch_lanes=3)
+    e.destroy()
