@@ -88,12 +88,15 @@ class GpuBackend : public Backend {
     int wallKHz = 0;
     hipCheck(hipDeviceGetAttribute(&wallKHz, hipDeviceAttributeWallClockRate, cfg.device), "hipDeviceGetAttribute");
     wallKHz_ = wallKHz;
-    hipCheck(hipStreamCreate(&stream_), "hipStreamCreate");
-    hipCheck(hipStreamCreate(&side_), "hipStreamCreate");
+    // Non-blocking streams, and no work on the null stream: a batch runs several engines at once on
+    // host threads, and null-stream work from one engine would wait on another's streams, which
+    // invalidates that engine's graph capture when it is capturing a tree run.
+    hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hipCheck(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreate");
     ctr_.ensure(1);
     scanCtl_.ensure(1);
     lvlCounts_.ensure(1);
-    hipCheck(hipMemset(scanCtl_.p, 0, sizeof(dk::ScanCtl)), "clear scan control");
+    clearScanCtl();
   }
   ~GpuBackend() override {
     if (stream_) (void)hipStreamSynchronize(stream_);
@@ -172,6 +175,10 @@ class GpuBackend : public Backend {
     hipCheck(hipStreamWaitEvent(to, e, 0), "hipStreamWaitEvent");
     joinEv_.push_back(e);
   }
+  void clearScanCtl() {
+    hipCheck(hipMemsetAsync(scanCtl_.p, 0, sizeof(dk::ScanCtl), stream_), "clear scan control");
+    hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
   // work counters of the kernels (cleared at the start of every entry point, summed at its end)
   void clearWork() {
     const unsigned long long z[6] = {0, 0, 0, 0, 0, 0};
@@ -217,7 +224,9 @@ class GpuBackend : public Backend {
   void readClocks() {
     if (clkUse_.empty() || wallKHz_ <= 0.0) return;
     std::vector<unsigned long long> c(4 * clkUse_.size());
-    hipCheck(hipMemcpy(c.data(), clk_.p, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost), "read clocks");
+    hipCheck(hipMemcpyAsync(c.data(), clk_.p, c.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_),
+             "read clocks");
+    hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     for (size_t s = 0; s < clkUse_.size(); s++)
       for (int k = 0; k < 2; k++) {
         const bool ran = k == 0 ? clkUse_[s].first : clkUse_[s].second;
@@ -873,7 +882,7 @@ TreeRunResult GpuBackend::run(const std::vector<MeshRef>& seeds, const std::vect
       readClocks();
     }
     if (sc.error | sc.sticky) {
-      hipCheck(hipMemset(scanCtl_.p, 0, sizeof(dk::ScanCtl)), "clear scan control");
+      clearScanCtl();
       throw DeviceError("device scan: a chunk's look-back waited past its limit");
     }
     const bool lastOnly = depth >= 2 && (c1.overflow & ~dk::OVF_ARENA) != 0 && c1.ovfLevels == (1u << (depth - 1));
@@ -1326,7 +1335,7 @@ bool GpuBackend::concavePlanes(const std::vector<MeshRef>& meshes, const Concave
   hipCheck(hipMemcpyAsync(&sc, scanCtl_.p, sizeof(sc), hipMemcpyDeviceToHost, stream_), "read scan state");
   sync();
   if (sc.error | sc.sticky) {
-    hipCheck(hipMemset(scanCtl_.p, 0, sizeof(dk::ScanCtl)), "clear scan control");
+    clearScanCtl();
     throw DeviceError("device scan: a chunk's look-back waited past its limit");
   }
   for (uint32_t k = 0; k < nm; k++) {

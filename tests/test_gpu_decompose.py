@@ -174,3 +174,20 @@ def test_decompose_tuning(cw):
     got = cw.decompose_batch(meshes, tuning=tuned, **bopts)
     for a, b in zip(want, got):
         _same(a, b)
+
+
+def test_batch_lanes_with_graph_capture(cw):
+    """Four engines at once, each capturing its tree runs into graphs while the others launch, copy and
+    synchronize: every capture completes and every result equals the eager single-engine batch, call
+    after call (the engines' streams do not synchronize with null-stream work of other threads)."""
+    meshes = [cw.generate("mix", 20260400, [k]) for k in range(48)]
+    opts = dict(threshold=0.1, width=12, depth=3, merge_hulls=True, hausdorff_samples=500)
+    want = cw.decompose_batch(meshes, batch_lanes=1, **opts)
+    e = cw.Decomposer(batch_tris=sum(m.nt for m in meshes), batch_lanes=4, tree_graph=True, **opts)
+    for k in range(3):
+        print(f"graph batch {k}", flush=True)
+        got = e.decompose_batch(meshes)
+        for a, b in zip(want, got):
+            _same(a, b)
+        assert got[0].kernels["graphs"] > 0
+    e.destroy()

@@ -83,11 +83,24 @@ def test_node_decompose_reports_library_errors(cw, oracle):
     assert out == {"error": str(e.value)}
 
 
+@pytest.mark.skipif(not _node_ready(), reason="node or the built addon is missing")
+def test_node_tuning_block(cw, oracle):
+    """options.tree.tuning carries the launch shapes under camelCase names: a value outside the kernels'
+    limits is the library's argument error, a valid one leaves the parts unchanged."""
+    bad = _run_node(cw.l_shape(), ORACLE_LIB, "oracle_", dict(JS_OPTS, tree=dict(tuning=dict(hullWgThreads=384))))
+    assert "hull_wg_threads" in bad["error"]
+    good = dict(JS_OPTS, tree=dict(JS_OPTS["tree"], useGraph=True, tuning=dict(hullWgThreads=256, tileMin=64, tileMax=8,
+                                                                               hdTileSamples=16)))
+    _compare(_run_node(cw.l_shape(), ORACLE_LIB, "oracle_", good), cw.decompose(cw.l_shape(), api=oracle, **PY_OPTS))
+
+
 @pytest.mark.gpu
 @pytest.mark.skipif(not _node_ready(), reason="node or the built addon is missing")
 def test_node_decompose_matches_python_on_the_gpu(cw):
     mesh = stairs(cw)
     _compare(_run_node(mesh, PRODUCT_LIB, "cuacd_", JS_OPTS), cw.decompose(mesh, **PY_OPTS))
+    tuned = dict(JS_OPTS, tree=dict(JS_OPTS["tree"], useGraph=True, tuning=dict(hullWgThreads=256, tileMin=64)))
+    _compare(_run_node(mesh, PRODUCT_LIB, "cuacd_", tuned), cw.decompose(mesh, **PY_OPTS))
 
 
 TREE_SCRIPT = """
