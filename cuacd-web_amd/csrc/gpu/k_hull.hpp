@@ -46,7 +46,7 @@ namespace dk {
 
 constexpr int HW = 4;         // parts per block in the wave-per-part hull kernel
 constexpr uint32_t ALIVE = 0x80000000u;
-constexpr uint32_t FW = 20;                 // words per face record (five uint4 words, one array each)
+constexpr uint32_t FW = 12;                 // words per face record (three uint4 words, one array each)
 
 // batch shape: LDS visible-face capacity, edge-hash slots, staged new faces. The workgroup kernel's
 // batch scales with its threads, so that it runs at four waves per SIMD either way: two workgroups
@@ -62,7 +62,7 @@ __host__ __device__ inline uint32_t hullFmax(uint32_t nv) { return 4 * nv + 256;
 __host__ __device__ inline uint64_t hullSlabWords(uint32_t nv) {
   const uint64_t f = hullFmax(nv);
   const uint64_t c = nv > 80 ? nv : 80;
-  return ((26ull * f + 3ull * nv + 18ull * c) + 63) & ~63ull;
+  return ((18ull * f + 3ull * nv + 18ull * c) + 63) & ~63ull;
 }
 
 __constant__ float KDIR[40][3] = {
@@ -334,9 +334,13 @@ struct HullCtx {
   float rejK;                // cheap-rejection factor 8 2^(G-21): the rounding grows with the coordinates
 };
 
-// Face records live in five arrays of uint4 (word k of face f at F4[k * fmax + f]):
-//   0: (a | ALIVE, b, c, n.qa)   1: (n0, n1, n2, bound factor)   2: (qa, |n|)   3: (qb, 0)   4: (qc, 0)
-// so a wave scanning consecutive faces reads contiguous 16-byte words.
+// Face records live in three arrays of uint4 (word k of face f at F4[k * fmax + f]):
+//   0: (a | ALIVE, b, c, n.qa)   1: (n0, n1, n2, bound factor)   2: (qa, |n|)
+// so a wave scanning consecutive faces reads contiguous 16-byte words. The other
+// corners' grid coordinates, needed only by exact tests, volumes and new faces,
+// are read from the part's grid points Q by corner index (every record is made
+// from those points). A record built in registers or LDS (FaceRec) carries them
+// as words 3 and 4.
 __device__ __forceinline__ uint4* face_row(const HullCtx& C, int k) {
   return reinterpret_cast<uint4*>(C.F) + (size_t)k * C.fmax;
 }
@@ -393,7 +397,7 @@ __device__ __forceinline__ FaceRec face_make(uint32_t a, uint32_t b, uint32_t c,
 }
 __device__ __forceinline__ void face_store(HullCtx& C, uint32_t f, const FaceRec& R) {
 #pragma unroll
-  for (int k = 0; k < 5; k++) face_row(C, k)[f] = R.w[k];
+  for (int k = 0; k < 3; k++) face_row(C, k)[f] = R.w[k];
   C.fkey[f] = 0ull;
   C.vmask[f] = 0ull;
 }
@@ -423,14 +427,16 @@ __device__ __forceinline__ int orient_w(const uint4& w2, const uint4& w3, const 
   const int32_t qc[3] = {(int32_t)w4.x, (int32_t)w4.y, (int32_t)w4.z};
   return orient_q(qa, qb, qc, qp);
 }
-// exact orientation of qp against face f (corner a given in w2, b and c read from words 3 and 4)
+// exact orientation of qp against face f (corner a given in w2, b and c from the grid points)
 __device__ __forceinline__ int orient_rec(const HullCtx& C, uint32_t f, const uint4& w2, const int32_t* qp) {
-  const uint4 w3 = face_word(C, f, 3);
-  const uint4 w4 = face_word(C, f, 4);
+  const uint4 w0 = face_word(C, f, 0);
   const int32_t qa[3] = {(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z};
-  const int32_t qb[3] = {(int32_t)w3.x, (int32_t)w3.y, (int32_t)w3.z};
-  const int32_t qc[3] = {(int32_t)w4.x, (int32_t)w4.y, (int32_t)w4.z};
-  return orient_q(qa, qb, qc, qp);
+  return orient_q(qa, C.Q + 3 * (size_t)w0.y, C.Q + 3 * (size_t)w0.z, qp);
+}
+// grid point v as a record word
+__device__ __forceinline__ uint4 grid_word(const HullCtx& C, uint32_t v) {
+  const int32_t* q = C.Q + 3 * (size_t)v;
+  return make_uint4((uint32_t)q[0], (uint32_t)q[1], (uint32_t)q[2], 0u);
 }
 
 // f32 filter of a face plane against a quantized point: 1 outside, -1 inside
@@ -512,8 +518,9 @@ template <typename Grp>
 __device__ __forceinline__ double hull_volume(const Grp& g, const HullCtx& C) {
   i128 acc = 0;
   for (uint32_t f = g.tid(); f < C.nf; f += Grp::G) {
-    if (!(face_a(C, f) & ALIVE)) continue;
-    acc += grid_det(face_word(C, f, 2), face_word(C, f, 3), face_word(C, f, 4), C.gridH);
+    const uint4 w0 = face_word(C, f, 0);
+    if (!(w0.x & ALIVE)) continue;
+    acc += grid_det(face_word(C, f, 2), grid_word(C, w0.y), grid_word(C, w0.z), C.gridH);
   }
   acc = g.sum(acc);
   const i128 a = acc < 0 ? -acc : acc;
@@ -550,11 +557,11 @@ __device__ __forceinline__ void compact_faces(const Grp& g, HullCtx& C) {
   uint32_t w = 0;
   for (uint32_t base = 0; base < C.nf; base += Grp::G) {
     const uint32_t f = base + g.tid();
-    uint4 r[5];
+    uint4 r[3];
     unsigned long long key = 0;
     bool alive = false;
     if (f < C.nf) {
-      for (int k = 0; k < 5; k++) r[k] = face_word(C, f, k);
+      for (int k = 0; k < 3; k++) r[k] = face_word(C, f, k);
       key = C.fkey[f];
       alive = (r[0].x & ALIVE) != 0;
     }
@@ -563,7 +570,7 @@ __device__ __forceinline__ void compact_faces(const Grp& g, HullCtx& C) {
     g.sync();  // every thread holds its record before any thread overwrites a slot
     if (f < C.nf) C.map[f] = alive ? dst : NONE;
     if (alive) {
-      for (int k = 0; k < 5; k++) face_row(C, k)[dst] = r[k];
+      for (int k = 0; k < 3; k++) face_row(C, k)[dst] = r[k];
       C.fkey[dst] = key;
     }
     w += cnt;
@@ -1101,10 +1108,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       // the two old faces on edge ab (the new face's plane keeps every point
       // below both of them below itself): a shared face is already a
       // conflict, so these edges are the only places to check.
-      const uint4 w2 = face_word(C, S.lface[e], 2), w3 = face_word(C, S.lface[e], 3), w4 = face_word(C, S.lface[e], 4);
-      const int32_t q3[3][3] = {{(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z},
-                                {(int32_t)w3.x, (int32_t)w3.y, (int32_t)w3.z},
-                                {(int32_t)w4.x, (int32_t)w4.y, (int32_t)w4.z}};
+      const int32_t* q3[3] = {C.Q + 3 * (size_t)v3[0], C.Q + 3 * (size_t)v3[1], C.Q + 3 * (size_t)v3[2]};
 #pragma unroll
       for (int k = 0; k < 3; k++)
         if (cross & (1u << k)) {
@@ -1152,7 +1156,6 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     if (!((act >> j) & 1u)) continue;
     const uint32_t flags = S.lflag[e];
     if (!flags) continue;
-    const uint4 w2 = face_word(C, S.lface[e], 2), w3 = face_word(C, S.lface[e], 3), w4 = face_word(C, S.lface[e], 4);
     uint32_t off = S.noff[j];
     {
       const uint32_t fe = S.lface[e], b0 = S.boff[j], nb = S.nvis[j];
@@ -1163,9 +1166,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     }
     uint32_t rel = off;
     const uint32_t v3[3] = {S.la[e], S.lb[e], S.lc[e]};
-    const int32_t q3[3][3] = {{(int32_t)w2.x, (int32_t)w2.y, (int32_t)w2.z},
-                              {(int32_t)w3.x, (int32_t)w3.y, (int32_t)w3.z},
-                              {(int32_t)w4.x, (int32_t)w4.y, (int32_t)w4.z}};
+    const int32_t* q3[3] = {C.Q + 3 * (size_t)v3[0], C.Q + 3 * (size_t)v3[1], C.Q + 3 * (size_t)v3[2]};
     const int32_t qp[3] = {S.cq[j][0], S.cq[j][1], S.cq[j][2]};
 #pragma unroll
     for (int k = 0; k < 3; k++)
@@ -1355,7 +1356,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP, C
   C.vmask = reinterpret_cast<unsigned long long*>(slab + (FW + 2) * fm);
   C.vis = slab + (FW + 4) * fm;
   C.map = C.vis + fm;
-  C.orph = C.map + fm;  // 16-byte aligned: 26 f is a multiple of 4
+  C.orph = C.map + fm;  // 16-byte aligned: 18 f is a multiple of 4
   C.cand = reinterpret_cast<uint4*>(C.orph + 8 * (size_t)cn);
   C.cand2 = C.cand + cn;
   C.Q = reinterpret_cast<int32_t*>(C.cand2 + cn);
