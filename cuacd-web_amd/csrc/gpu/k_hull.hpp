@@ -335,8 +335,9 @@ struct HullCtx {
 };
 
 // Face records live in three arrays of uint4 (word k of face f at F4[k * fmax + f]):
-//   0: (a | ALIVE, b, c, n.qa)   1: (n0, n1, n2, bound factor)   2: (qa, |n|)
-// so a wave scanning consecutive faces reads contiguous 16-byte words. The other
+//   0: (a | ALIVE, b, c, bound factor s)   1: (n0, n1, n2, rejection threshold)   2: (qa, |n|)
+// so a wave scanning consecutive faces reads contiguous 16-byte words; the cheap
+// rejection reads word 1 alone (a dead face's threshold is +inf: it rejects every point). The other
 // corners' grid coordinates, needed only by exact tests, volumes and new faces,
 // are read from the part's grid points Q by corner index (every record is made
 // from those points). A record built in registers or LDS (FaceRec) carries them
@@ -353,30 +354,40 @@ __device__ __forceinline__ unsigned long long pkey(float d, uint32_t i) {
 }
 
 // Cheap rejection against a face plane. With grid coordinates in [0, 2^G],
-// c = n.p - n.qa evaluated with fused multiply-adds (d = n.qa the same way)
+// n.p - n.qa evaluated with fused multiply-adds (off = n.qa the same way)
 // differs from the exact det N.(p - qa) by at most 3 eps |n|inf (|p|1 + 2 |qa|1)
 // for the rounding of the two dot products plus |n - N|inf |p - qa|1 <= 2 eps s1 s2 3 2^G
 // for the f32 normal; with |n|inf <= s1 s2 that is below 33 2^(G-24) s1 s2
-// (eps = 2^-24): 4.2 s1 s2 on the 2^21 grid, 33 s1 s2 on the 2^24 grid.
-// So c < -8 2^(G-21) s1 s2 proves the point strictly inside, and the f32 filter
-// and the exact test would say the same; any other point needs those tests.
+// (eps = 2^-24): 4.2 s1 s2 on the 2^21 grid, 33 s1 s2 on the 2^24 grid. The record
+// keeps one threshold thr = off - 8 2^(G-21) s1 s2 (rounded once, which adds at most
+// eps |thr| <= 3.1 2^(G-24) s1 s2, as |off| <= 3 2^G s1 s2), and the test is
+// c = n.p - thr < 0 with fused multiply-adds: below 37 2^(G-24) s1 s2 of error in
+// all, inside the 64 2^(G-24) s1 s2 margin. So c < 0 proves the point strictly
+// inside, and the f32 filter and the exact test would say the same; any other
+// point needs those tests.
 __device__ __forceinline__ float plane_offset(const uint4& w1, const uint4& w2) {
   return fmaf(__uint_as_float(w1.x), (float)(int32_t)w2.x,
               fmaf(__uint_as_float(w1.y), (float)(int32_t)w2.y, __uint_as_float(w1.z) * (float)(int32_t)w2.z));
 }
-__device__ __forceinline__ bool maybe_outside(const uint4& w1, float off, float lim, const float4& p) {
-  const float c = fmaf(__uint_as_float(w1.x), p.x, fmaf(__uint_as_float(w1.y), p.y, fmaf(__uint_as_float(w1.z), p.z, -off)));
-  return c >= lim;
+__device__ __forceinline__ bool maybe_outside(const uint4& w1, const float4& p) {
+  const float c = fmaf(__uint_as_float(w1.x), p.x,
+                       fmaf(__uint_as_float(w1.y), p.y, fmaf(__uint_as_float(w1.z), p.z, -__uint_as_float(w1.w))));
+  return c >= 0.0f;
 }
-__device__ __forceinline__ float reject_limit(const uint4& w1, float k) { return -k * __uint_as_float(w1.w); }
+constexpr uint32_t THR_DEAD = 0x7f800000u;  // +inf: the threshold of a dead face
+// marks face f dead: first corner without the alive bit, threshold +inf
+__device__ __forceinline__ void face_kill(const HullCtx& C, uint32_t f, uint32_t a) {
+  C.F[4 * (size_t)f] = a;
+  reinterpret_cast<uint32_t*>(C.F)[4 * ((size_t)C.fmax + f) + 3] = THR_DEAD;
+}
 
 struct FaceRec {
   uint4 w[5];
 };
 
-// record of face (a, b, c) with corner grid coordinates qa, qb, qc
+// record of face (a, b, c) with corner grid coordinates qa, qb, qc (rejK: 8 2^(G-21))
 __device__ __forceinline__ FaceRec face_make(uint32_t a, uint32_t b, uint32_t c, const int32_t* qa, const int32_t* qb,
-                                             const int32_t* qc) {
+                                             const int32_t* qc, float rejK) {
   const float e1[3] = {(float)qb[0] - (float)qa[0], (float)qb[1] - (float)qa[1], (float)qb[2] - (float)qa[2]};
   const float e2[3] = {(float)qc[0] - (float)qa[0], (float)qc[1] - (float)qa[1], (float)qc[2] - (float)qa[2]};
   const float n0 = e1[1] * e2[2] - e1[2] * e2[1];
@@ -388,9 +399,11 @@ __device__ __forceinline__ FaceRec face_make(uint32_t a, uint32_t b, uint32_t c,
   nn = nn + n1 * n1;
   nn = nn + n2 * n2;
   FaceRec R;
-  R.w[1] = make_uint4(__float_as_uint(n0), __float_as_uint(n1), __float_as_uint(n2), __float_as_uint(s1 * s2));
+  const float s = s1 * s2;
+  R.w[1] = make_uint4(__float_as_uint(n0), __float_as_uint(n1), __float_as_uint(n2), 0u);
   R.w[2] = make_uint4((uint32_t)qa[0], (uint32_t)qa[1], (uint32_t)qa[2], __float_as_uint(sqrtf(nn)));
-  R.w[0] = make_uint4(a | ALIVE, b, c, __float_as_uint(plane_offset(R.w[1], R.w[2])));
+  R.w[1].w = __float_as_uint(plane_offset(R.w[1], R.w[2]) - rejK * s);
+  R.w[0] = make_uint4(a | ALIVE, b, c, __float_as_uint(s));
   R.w[3] = make_uint4((uint32_t)qb[0], (uint32_t)qb[1], (uint32_t)qb[2], 0u);
   R.w[4] = make_uint4((uint32_t)qc[0], (uint32_t)qc[1], (uint32_t)qc[2], 0u);
   return R;
@@ -403,7 +416,7 @@ __device__ __forceinline__ void face_store(HullCtx& C, uint32_t f, const FaceRec
 }
 __device__ __forceinline__ void face_write_q(HullCtx& C, uint32_t f, uint32_t a, uint32_t b, uint32_t c,
                                              const int32_t* qa, const int32_t* qb, const int32_t* qc) {
-  face_store(C, f, face_make(a, b, c, qa, qb, qc));
+  face_store(C, f, face_make(a, b, c, qa, qb, qc, C.rejK));
 }
 __device__ __forceinline__ void face_write(HullCtx& C, uint32_t f, uint32_t a, uint32_t b, uint32_t c) {
   face_write_q(C, f, a, b, c, C.Q + 3 * a, C.Q + 3 * b, C.Q + 3 * c);
@@ -440,8 +453,8 @@ __device__ __forceinline__ uint4 grid_word(const HullCtx& C, uint32_t v) {
 }
 
 // f32 filter of a face plane against a quantized point: 1 outside, -1 inside
-// or on the plane, 0 undecided (the exact test decides)
-__device__ __forceinline__ int face_filter(const uint4& w1, const uint4& w2, const int32_t* qp, float* detOut) {
+// or on the plane, 0 undecided (the exact test decides); s the record's bound factor
+__device__ __forceinline__ int face_filter(const uint4& w1, const uint4& w2, float s, const int32_t* qp, float* detOut) {
   const float d0 = (float)qp[0] - (float)(int32_t)w2.x;
   const float d1 = (float)qp[1] - (float)(int32_t)w2.y;
   const float d2 = (float)qp[2] - (float)(int32_t)w2.z;
@@ -449,7 +462,7 @@ __device__ __forceinline__ int face_filter(const uint4& w1, const uint4& w2, con
   det = det + __uint_as_float(w1.y) * d1;
   det = det + __uint_as_float(w1.z) * d2;
   const float sd = (fabsf(d0) + fabsf(d1)) + fabsf(d2);
-  const float bound = (1e-6f * __uint_as_float(w1.w)) * sd;
+  const float bound = (1e-6f * s) * sd;
   *detOut = det;
   return det > bound ? 1 : (det < -bound ? -1 : 0);
 }
@@ -459,10 +472,9 @@ __device__ __forceinline__ int face_filter(const uint4& w1, const uint4& w2, con
 __device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, const int32_t* qp) {
   const uint4 w0 = face_word(C, f, 0), w1 = face_word(C, f, 1), w2 = face_word(C, f, 2);
   if (!(w0.x & ALIVE)) return -1e30f;
-  if (!maybe_outside(w1, __uint_as_float(w0.w), reject_limit(w1, C.rejK), make_float4((float)qp[0], (float)qp[1], (float)qp[2], 0.0f)))
-    return -1.0f;
+  if (!maybe_outside(w1, make_float4((float)qp[0], (float)qp[1], (float)qp[2], 0.0f))) return -1.0f;
   float det;
-  const int s = face_filter(w1, w2, qp, &det);
+  const int s = face_filter(w1, w2, __uint_as_float(w0.w), qp, &det);
   if (s > 0) {
     const float nl = __uint_as_float(w2.w);
     return det / (nl > 1e-30f ? nl : 1e-30f);
@@ -470,13 +482,11 @@ __device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, const i
   if (s < 0) return -1.0f;
   return orient_rec(C, f, w2, qp) > 0 ? 1e-30f : -1.0f;
 }
-__device__ __forceinline__ float face_eval_rec(const FaceRec& R, const int32_t* qp, float rejK) {
+__device__ __forceinline__ float face_eval_rec(const FaceRec& R, const int32_t* qp) {
   if (!(R.w[0].x & ALIVE)) return -1e30f;
-  if (!maybe_outside(R.w[1], __uint_as_float(R.w[0].w), reject_limit(R.w[1], rejK),
-                     make_float4((float)qp[0], (float)qp[1], (float)qp[2], 0.0f)))
-    return -1.0f;
+  if (!maybe_outside(R.w[1], make_float4((float)qp[0], (float)qp[1], (float)qp[2], 0.0f))) return -1.0f;
   float det;
-  const int s = face_filter(R.w[1], R.w[2], qp, &det);
+  const int s = face_filter(R.w[1], R.w[2], __uint_as_float(R.w[0].w), qp, &det);
   if (s > 0) {
     const float nl = __uint_as_float(R.w[2].w);
     return det / (nl > 1e-30f ? nl : 1e-30f);
@@ -667,7 +677,7 @@ __device__ __forceinline__ void hull_insert(const Grp& g, HullCtx& C, uint32_t p
   }
   g.sync();
   for (uint32_t vi = t; vi < nvis; vi += Grp::G) {
-    face_a(C, C.vis[vi]) &= ~ALIVE;
+    face_kill(C, C.vis[vi], face_a(C, C.vis[vi]) & ~ALIVE);
     C.fkey[C.vis[vi]] = 0ull;
   }
   C.nf = f0 + nnew;
@@ -931,59 +941,57 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     // in a pending list and get their exact tests after the loop, one face per thread
     constexpr uint32_t VB = Grp::G > 64 ? 2 : 4;  // fewer records in flight where registers are capped
     for (uint32_t f0 = t; f0 < C.nf; f0 += VB * Grp::G) {
-      uint4 h0[VB], h1[VB], h2[VB];
+      // word 1 (the plane and its rejection threshold) of every face of the group
+      uint4 h1[VB];
 #pragma unroll
       for (uint32_t b = 0; b < VB; b++) {
         const uint32_t f = f0 + b * Grp::G;
-        if (f < C.nf) {
-          h0[b] = face_word(C, f, 0);
-          h1[b] = face_word(C, f, 1);
-          h2[b] = face_word(C, f, 2);
-        } else {
-          h0[b] = h1[b] = h2[b] = make_uint4(0u, 0u, 0u, 0u);
-        }
+        h1[b] = f < C.nf ? face_word(C, f, 1) : make_uint4(0u, 0u, 0u, THR_DEAD);
       }
       // every candidate against the faces of the group by the cheap rejection;
-      // the f32 filter (and the exact test) only for the pairs it leaves
+      // the f32 filter (and the exact test) only for the pairs it leaves, which are the
+      // only faces whose words 0 and 2 are read (a dead face rejects every candidate)
       // (candidates in descending order, each 32-bit half of the mask built by shift
       // and or, so candidate j lands on bit j; the count is the group's, read once)
-      float off[VB], lim[VB];
       uint32_t mlo[VB], mhi[VB];
       Mask maybe[VB];
 #pragma unroll
-      for (uint32_t b = 0; b < VB; b++) {
-        off[b] = __uint_as_float(h0[b].w);
-        lim[b] = reject_limit(h1[b], C.rejK);
-        mlo[b] = mhi[b] = 0u;
-      }
+      for (uint32_t b = 0; b < VB; b++) mlo[b] = mhi[b] = 0u;
       const int mu = (int)__builtin_amdgcn_readfirstlane(m);
 #pragma unroll 4
       for (int j = mu - 1; j >= 32; j--) {
         const float4 pj = S.cqf[j];
 #pragma unroll
-        for (uint32_t b = 0; b < VB; b++) mhi[b] = (mhi[b] << 1) | (maybe_outside(h1[b], off[b], lim[b], pj) ? 1u : 0u);
+        for (uint32_t b = 0; b < VB; b++) mhi[b] = (mhi[b] << 1) | (maybe_outside(h1[b], pj) ? 1u : 0u);
       }
 #pragma unroll 4
       for (int j = (mu < 32 ? mu : 32) - 1; j >= 0; j--) {
         const float4 pj = S.cqf[j];
 #pragma unroll
-        for (uint32_t b = 0; b < VB; b++) mlo[b] = (mlo[b] << 1) | (maybe_outside(h1[b], off[b], lim[b], pj) ? 1u : 0u);
+        for (uint32_t b = 0; b < VB; b++) mlo[b] = (mlo[b] << 1) | (maybe_outside(h1[b], pj) ? 1u : 0u);
       }
 #pragma unroll
       for (uint32_t b = 0; b < VB; b++) maybe[b] = (Mask)(((unsigned long long)mhi[b] << 32) | mlo[b]);
 #pragma unroll
       for (uint32_t b = 0; b < VB; b++) {
         const uint32_t f = f0 + b * Grp::G;
-        if (!(h0[b].x & ALIVE)) continue;
+        if (f >= C.nf) continue;
+        C.tests += m;
+        if (!maybe[b]) {
+          if (pass > 0) C.vmask[f] = 0ull;  // a later pass clears what an earlier one marked
+          continue;
+        }
+        const uint4 h0 = face_word(C, f, 0), h2 = face_word(C, f, 2);
+        if (!(h0.x & ALIVE)) continue;
+        const float sb = __uint_as_float(h0.w);
         Mask mask = 0, und = 0;
         for (Mask u = maybe[b]; u; u &= u - 1) {
           const uint32_t j = mask_ffs(u) - 1;
           float det;
-          const int sgn = face_filter(h1[b], h2[b], S.cq[j], &det);
+          const int sgn = face_filter(h1[b], h2, sb, S.cq[j], &det);
           mask |= sgn > 0 ? ((Mask)1 << j) : (Mask)0;
           und |= sgn == 0 ? ((Mask)1 << j) : (Mask)0;
         }
-        C.tests += m;
         if (und) {
           const uint32_t pp = atomicAdd(&S.nPend, 1u);
           if (pp < LCAP / 2) {
@@ -994,10 +1002,10 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
           }
           for (Mask u = und; u; u &= u - 1) {
             const uint32_t j = mask_ffs(u) - 1;
-            if (orient_rec(C, f, h2[b], S.cq[j]) > 0) mask |= (Mask)1 << j;
+            if (orient_rec(C, f, h2, S.cq[j]) > 0) mask |= (Mask)1 << j;
           }
         }
-        finish(f, mask, h0[b].x & ~ALIVE, h0[b].y, h0[b].z);
+        finish(f, mask, h0.x & ~ALIVE, h0.y, h0.z);
       }
     }
     g.sync();
@@ -1113,9 +1121,9 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
       for (int k = 0; k < 3; k++)
         if (cross & (1u << k)) {
           const int k1 = k == 2 ? 0 : k + 1;
-          const FaceRec R = face_make(v3[k], v3[k1], S.cpt[j], q3[k], q3[k1], S.cq[j]);
+          const FaceRec R = face_make(v3[k], v3[k1], S.cpt[j], q3[k], q3[k1], S.cq[j], C.rejK);
           C.tests++;
-          if (face_eval_rec(R, S.cq[tjs[k]], C.rejK) > 0.0f) atomicOr(&S.cutA, (Mask)1 << tjs[k]);
+          if (face_eval_rec(R, S.cq[tjs[k]]) > 0.0f) atomicOr(&S.cutA, (Mask)1 << tjs[k]);
         }
     }
     S.lflag[e] = (uint8_t)flags;
@@ -1172,7 +1180,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     for (int k = 0; k < 3; k++)
       if (flags & (1u << k)) {
         const int k1 = k == 2 ? 0 : k + 1;
-        const FaceRec R = face_make(v3[k], v3[k1], S.cpt[j], q3[k], q3[k1], qp);
+        const FaceRec R = face_make(v3[k], v3[k1], S.cpt[j], q3[k], q3[k1], qp, C.rejK);
         face_store(C, C.nf + rel, R);
         if (rel < NCAP) S.nrec[rel] = R;
         rel++;
@@ -1223,7 +1231,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
     uint32_t bf = NONE;
     float bd = -1.0f;
     for (uint32_t rel = off; rel < r1; rel++) {
-      const float dd = rel < NCAP ? face_eval_rec(S.nrec[rel], qi, C.rejK) : face_eval(C, C.nf + rel, qi);
+      const float dd = rel < NCAP ? face_eval_rec(S.nrec[rel], qi) : face_eval(C, C.nf + rel, qi);
       if (dd > 0.0f && dd > bd) {
         bd = dd;
         bf = C.nf + rel;
@@ -1243,7 +1251,7 @@ __device__ __forceinline__ uint32_t hull_round(const Grp& g, HullCtx& C, HullBat
   for (uint32_t e = t; e < nL; e += Grp::G) {
     C.vmask[S.lface[e]] = 0ull;  // every face with a mask is listed
     if (!((acc >> S.lj[e]) & 1u)) continue;
-    face_a(C, S.lface[e]) = S.la[e];  // first corner without the alive bit
+    face_kill(C, S.lface[e], S.la[e]);
     C.fkey[S.lface[e]] = 0ull;
   }
   for (uint32_t q = t; q < no; q += Grp::G) {
@@ -1618,7 +1626,7 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP, C
           if (alive && ns + rk < SCAP) {
             stage[3 * (ns + rk)] = face_word(C, f, 1);
             stage[3 * (ns + rk) + 1] = face_word(C, f, 2);
-            stage[3 * (ns + rk) + 2] = make_uint4(f, face_word(C, f, 0).w, 0u, 0u);
+            stage[3 * (ns + rk) + 2] = make_uint4(f, face_word(C, f, 0).w, 0u, 0u);  // (face, bound factor)
           }
           ns += cnt;
         }
@@ -1663,16 +1671,16 @@ __device__ void hull_job(const Grp& g, HullBatch<EXACT, K, LCAP, HT, NW, NCAP, C
                 det = det + n1 * d1;
                 det = det + n2 * d2;
                 const float sd = ((fabsf(d0) + fabsf(d1)) + fabsf(d2)) + ((hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]));
-                keep = !(det < -((1e-6f * __uint_as_float(w1.w)) * sd));
+                keep = !(det < -((1e-6f * __uint_as_float(stage[3 * sl + 2].y)) * sd));
               }
               for (uint64_t km = __ballot(keep); km; km &= km - 1) {
                 if (i >= C.nv) continue;
                 const uint32_t s = s0 + (uint32_t)__ffsll((unsigned long long)km) - 1;
                 const uint4 w1 = stage[3 * s], w2 = stage[3 * s + 1];
                 C.tests++;
-                if (!maybe_outside(w1, __uint_as_float(stage[3 * s + 2].y), reject_limit(w1, C.rejK), pf)) continue;
+                if (!maybe_outside(w1, pf)) continue;
                 float det;
-                const int sg = face_filter(w1, w2, qi, &det);
+                const int sg = face_filter(w1, w2, __uint_as_float(stage[3 * s + 2].y), qi, &det);
                 float dd;
                 if (sg > 0) {
                   const float nl = __uint_as_float(w2.w);

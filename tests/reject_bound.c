@@ -4,9 +4,9 @@
  * against the exact integer determinant on random and near-coplanar cases.
  *
  * Kernel formulas: normal n = e1 x e2 in f32 from integer corner differences,
- * s = (|e1|1) (|e2|1), off = fma(n0, qa0, fma(n1, qa1, n2 qa2)),
- * c = fma(n0, p0, fma(n1, p1, fma(n2, p2, -off))), rejected when c < -8 2^(G-21) s
- * on a 2^G grid (G = 21 for tree candidates, 24 for exported hulls). */
+ * s = (|e1|1) (|e2|1), off = fma(n0, qa0, fma(n1, qa1, n2 qa2)), the record's
+ * threshold thr = off - (8 2^(G-21)) s, c = fma(n0, p0, fma(n1, p1, fma(n2, p2, -thr))),
+ * rejected when c < 0 on a 2^G grid (G = 21 for tree candidates, 24 for exported hulls). */
 #include <math.h>
 #include <stdint.h>
 
@@ -23,11 +23,11 @@ static int G = 21;
 static int32_t coord(void) { return (int32_t)(rng() % ((1u << G) + 1)); }
 
 /* out[0] rejections, out[1] violations (rejected but det >= 0), out[2] pairs
- * with det < 0 that were not rejected, out[3] worst (c - det) / s scaled by 1e6 */
+ * with det < 0 that were not rejected, out[3] worst |c - (det + rejK s)| / s scaled by 1e6 */
 void reject_check(uint64_t seed, int trials, int bits, int64_t* out) {
   rng_state = seed | 1;
   G = bits;
-  const float lim = -8.0f * (float)(1 << (bits - 21));
+  const float rejK = 8.0f * (float)(1 << (bits - 21));
   out[0] = out[1] = out[2] = out[3] = 0;
   double worst = 0;
   for (int t = 0; t < trials; t++) {
@@ -68,18 +68,20 @@ void reject_check(uint64_t seed, int trials, int bits, int64_t* out) {
     const float s2 = (fabsf(e2[0]) + fabsf(e2[1])) + fabsf(e2[2]);
     const float s = s1 * s2;
     const float off = fmaf(n0, (float)qa[0], fmaf(n1, (float)qa[1], n2 * (float)qa[2]));
-    const float c = fmaf(n0, (float)p[0], fmaf(n1, (float)p[1], fmaf(n2, (float)p[2], -off)));
+    const float thr = off - rejK * s;
+    const float c = fmaf(n0, (float)p[0], fmaf(n1, (float)p[1], fmaf(n2, (float)p[2], -thr)));
     const int64_t E1[3] = {(int64_t)qb[0] - qa[0], (int64_t)qb[1] - qa[1], (int64_t)qb[2] - qa[2]};
     const int64_t E2[3] = {(int64_t)qc[0] - qa[0], (int64_t)qc[1] - qa[1], (int64_t)qc[2] - qa[2]};
     const int64_t D[3] = {(int64_t)p[0] - qa[0], (int64_t)p[1] - qa[1], (int64_t)p[2] - qa[2]};
     const int64_t N[3] = {E1[1] * E2[2] - E1[2] * E2[1], E1[2] * E2[0] - E1[0] * E2[2], E1[0] * E2[1] - E1[1] * E2[0]};
     const i128 det = (i128)N[0] * D[0] + (i128)N[1] * D[1] + (i128)N[2] * D[2];
-    const int rejected = c < lim * s;
+    const int rejected = c < 0.0f;
     if (rejected) out[0]++;
     if (rejected && det >= 0) out[1]++;
     if (!rejected && det < 0) out[2]++;
     if (s > 0) {
-      const double err = fabs((double)c - (double)det) / (double)s;
+      /* c estimates det + rejK s: its error, per s */
+      const double err = fabs((double)c - ((double)det + (double)rejK * (double)s)) / (double)s;
       if (err > worst) worst = err;
     }
   }
