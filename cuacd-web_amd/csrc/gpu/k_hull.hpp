@@ -468,11 +468,13 @@ __device__ __forceinline__ int face_filter(const uint4& w1, const uint4& w2, flo
 }
 
 // > 0: point qp strictly outside face f (f32 distance, 1e-30 when only the
-// exact test decided); <= 0: inside, on the plane, or a dead face
+// exact test decided); <= 0: inside, on the plane, or a dead face. Words 0 and 2
+// are read only when the cheap rejection (word 1; dead faces reject) leaves the point.
 __device__ __forceinline__ float face_eval(const HullCtx& C, uint32_t f, const int32_t* qp) {
-  const uint4 w0 = face_word(C, f, 0), w1 = face_word(C, f, 1), w2 = face_word(C, f, 2);
-  if (!(w0.x & ALIVE)) return -1e30f;
+  const uint4 w1 = face_word(C, f, 1);
   if (!maybe_outside(w1, make_float4((float)qp[0], (float)qp[1], (float)qp[2], 0.0f))) return -1.0f;
+  const uint4 w0 = face_word(C, f, 0), w2 = face_word(C, f, 2);
+  if (!(w0.x & ALIVE)) return -1e30f;
   float det;
   const int s = face_filter(w1, w2, __uint_as_float(w0.w), qp, &det);
   if (s > 0) {
