@@ -9,10 +9,12 @@ batches. The engine (device pools sized for the batch) is created once, before t
 reference's Decomposer is (src/decompose.ts:135). `value` is meshes/s over the timed steps.
 
 Beside it, in the same JSON line:
-  plain        the same 256 meshes decomposed one at a time on one engine with single-mesh pools and the
-               knobs tuned for the plain decompose (the plain per-mesh GPU path), its meshes/s, the
-               batched/plain ratio, and whether every result (cuts, parts, volumes, distances, hull
-               faces) is identical to the batched one;
+  plain        the same 256 meshes decomposed one at a time on one warm engine with single-mesh pools
+               and the knobs tuned for the plain decompose (the plain per-mesh GPU path at its best), its
+               meshes/s, the batched/plain ratio, and whether every result (cuts, parts, volumes,
+               distances, hull faces) is identical to the batched one;
+  plain_api    every 4th mesh through the drop-in call decompose(mesh, options), which makes and releases
+               its device state per call as the reference's decompose() does, with the same checks;
   blob         the 12-blob 96^3 mesh (56,464 triangles, threshold 0.05, width 12, width2 6, depth 3,
                16 concave-edge planes, refinement 2 x 8, merge, 3000 samples per part) decomposed
                plain, the single-mesh workload;
@@ -252,6 +254,8 @@ def main():
     ap.add_argument("--meshes", type=int, default=MIX["n"], help="first N meshes of the mix (default all 256)")
     ap.add_argument("--plain", type=int, default=1, metavar="K",
                     help="time the plain per-mesh path on every K-th mesh of the mix (1: all; 0: skip)")
+    ap.add_argument("--plain-api", type=int, default=4, metavar="K",
+                    help="time decompose(mesh, options) calls (device state made per call) on every K-th mesh (0: skip)")
     ap.add_argument("--blob-steps", type=int, default=3, help="plain decompositions of the 56k-triangle blob (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -334,6 +338,18 @@ def main():
         log(f"plain: {len(sel)} meshes in {tp:.2f} s, identical: {same}")
         line["plain"] = dict(meshes=len(sel), every=a.plain, options=popts, meshes_per_s=len(sel) / tp,
                              ratio_batched_vs_plain=line["value"] / (len(sel) / tp), identical=same)
+    if a.plain_api:
+        # the drop-in call decompose(mesh, options) per mesh: each call makes and releases its device
+        # state, as the reference's decompose() makes and destroys a Decomposer (src/decompose.ts:407-414)
+        sel = list(range(0, a.meshes, a.plain_api))
+        popts = dict(MIX["options"], **tuned("blob"))
+        t0 = time.perf_counter()
+        api = [cw.decompose(meshes[k], **popts) for k in sel]
+        ta = time.perf_counter() - t0
+        same = all(_same(p, batched[k]) for p, k in zip(api, sel))
+        log(f"plain decompose calls: {len(sel)} meshes in {ta:.2f} s, identical: {same}")
+        line["plain_api"] = dict(meshes=len(sel), every=a.plain_api, meshes_per_s=len(sel) / ta,
+                                 ratio_batched_vs_plain_api=line["value"] / (len(sel) / ta), identical=same)
     if a.blob_steps:
         mesh = cw.generate("blobs", BLOB["seed"], BLOB["params"])
         bopts = dict(BLOB["options"], **tuned("blob"))

@@ -12,3 +12,4 @@ from .mesh import normalize, plane_cut, concave_edge_planes, parse_stl, generate
 from .mesh import parse_obj, to_obj, clean_hull_faces, compact_hull, hull_eps  # noqa: F401
 from .gputree import GpuTree, TreeConfig, DEFAULT_TREE_CONFIG, HD_MAX_PAIRS  # noqa: F401
 from .decompose import decompose, decompose_batch, merge_hulls, Decomposer, Part, DecomposeOptions, DEFAULT_OPTIONS, K_RV, rv_cost, part_cost, AbortFlag  # noqa: F401
+from .decompose import release_cached_memory  # noqa: F401

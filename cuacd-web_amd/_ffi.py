@@ -182,6 +182,7 @@ HOST_API = {
     "hull_eps": (f64, [P(f32), u32]),
     "generate": (C.c_int, [u32, u64, P(f64), u32, P(C.c_void_p)]),
     "hausdorff_noalloc": (C.c_int, [P(MeshBatch), P(MeshBatch), u32, P(f64)]),
+    "release_cached_memory": (None, []),
 }
 
 # error codes (include/cuacd.h)

@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <iterator>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <type_traits>
@@ -31,37 +33,128 @@ void hipCheck(hipError_t e, const char* what) {
   if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Device blocks of 64 MiB and more are kept for reuse when an engine releases them. An engine made
+// and released per call (decompose(mesh, options) makes and destroys its device state, as the
+// reference's decompose() does) would otherwise map and unmap some 20 GiB of pools every call,
+// which this runtime does at about 30 ms per GiB once its deferred frees are spent (measured: a
+// 16 GiB free took 477 ms). A request takes the smallest cached block of its device that holds it,
+// whatever its size (cached memory is held either way; a batch engine's large blocks serve the
+// single-mesh engines made after it). The cache goes back to the device when a request does not fit
+// the device's free memory, when it holds more than MAX_CACHED of a device, and on
+// cuacd_release_cached_memory.
+class BlockCache {
+ public:
+  static constexpr size_t MIN_BYTES = 64ull << 20, MAX_CACHED = 128ull << 30;
+  static BlockCache& get() {
+    static BlockCache* c = new BlockCache;  // never destroyed: engines released at exit still return blocks
+    return *c;
+  }
+  // a block of at least `bytes` on the current device; *cap its size
+  void* take(size_t bytes, int dev, size_t* cap) {
+    if (bytes >= MIN_BYTES) {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto& m = blocks_[dev];
+      auto it = m.lower_bound(bytes);
+      if (it != m.end()) {
+        void* p = it->second;
+        *cap = it->first;
+        cached_[dev] -= it->first;
+        m.erase(it);
+        return p;
+      }
+    }
+    if (!fits(bytes)) {
+      release(dev);
+      reserveCheck(bytes);
+    }
+    void* p = nullptr;
+    hipCheck(hipMalloc(&p, bytes), "hipMalloc");
+    *cap = bytes;
+    return p;
+  }
+  void give(void* p, size_t cap, int dev) {
+    if (cap < MIN_BYTES) {
+      (void)hipFree(p);
+      return;
+    }
+    std::vector<void*> drop;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      auto& m = blocks_[dev];
+      m.emplace(cap, p);
+      cached_[dev] += cap;
+      while (cached_[dev] > MAX_CACHED && !m.empty()) {  // largest blocks go back first
+        auto last = std::prev(m.end());
+        cached_[dev] -= last->first;
+        drop.push_back(last->second);
+        m.erase(last);
+      }
+    }
+    for (void* q : drop) (void)hipFree(q);
+  }
+  // every cached block of device dev (every device when dev < 0) back to the device
+  void release(int dev) {
+    std::vector<void*> drop;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (auto& kv : blocks_) {
+        if (dev >= 0 && kv.first != dev) continue;
+        for (auto& b : kv.second) drop.push_back(b.second);
+        kv.second.clear();
+        cached_[kv.first] = 0;
+      }
+    }
+    for (void* q : drop) (void)hipFree(q);
+  }
+  // A large allocation must leave DEVICE_RESERVE bytes free: the runtime takes kernel scratch
+  // (register spills) from device memory at launch, and a launch that cannot get it aborts its
+  // queue instead of returning an error. Such a request fails here, as a capacity error.
+  static constexpr size_t DEVICE_RESERVE = 4ull << 30;
+  static bool fits(size_t bytes) {
+    if (bytes < (256ull << 20)) return true;
+    size_t freeB = 0, totalB = 0;
+    hipCheck(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
+    return freeB >= bytes && freeB - bytes >= DEVICE_RESERVE;
+  }
+  static void reserveCheck(size_t bytes) {
+    if (fits(bytes)) return;
+    size_t freeB = 0, totalB = 0;
+    hipCheck(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
+    throw std::length_error("device memory: a pool of " + std::to_string(bytes >> 20) + " MiB leaves less than " +
+                            std::to_string(DEVICE_RESERVE >> 30) + " GiB of the " + std::to_string(freeB >> 20) +
+                            " MiB free (fewer batch lanes or smaller pools fit)");
+  }
+
+ private:
+  std::mutex mu_;
+  std::map<int, std::multimap<size_t, void*>> blocks_;
+  std::map<int, size_t> cached_;
+};
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
-  size_t n = 0;
+  size_t n = 0;    // elements asked for
+  size_t cap = 0;  // bytes of the block
+  int dev = 0;
   DevBuf() {}
   DevBuf(const DevBuf&) = delete;
+  // an engine's buffers are released after its streams are synchronized: the block is idle and
+  // goes to the cache
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) BlockCache::get().give(p, cap, dev);
   }
+  // a larger block; the old one may still be read by work in flight, so it goes back through
+  // hipFree, which waits for it, not to the cache (another engine could take it at once)
   void ensure(size_t count) {
     if (count <= n && p) return;
     if (p) hipCheck(hipFree(p), "hipFree");
     p = nullptr;
     n = 0;
     const size_t c = std::max<size_t>(count, 1);
-    reserveCheck(c * sizeof(T));
-    hipCheck(hipMalloc(&p, c * sizeof(T)), "hipMalloc");
+    hipCheck(hipGetDevice(&dev), "hipGetDevice");
+    p = static_cast<T*>(BlockCache::get().take(c * sizeof(T), dev, &cap));
     n = c;
-  }
-  // A large allocation must leave DEVICE_RESERVE bytes free: the runtime takes kernel scratch
-  // (register spills) from device memory at launch, and a launch that cannot get it aborts its
-  // queue instead of returning an error. Such a request fails here, as a capacity error.
-  static void reserveCheck(size_t bytes) {
-    constexpr size_t DEVICE_RESERVE = 4ull << 30;
-    if (bytes < (256ull << 20)) return;
-    size_t freeB = 0, totalB = 0;
-    hipCheck(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
-    if (freeB < bytes || freeB - bytes < DEVICE_RESERVE)
-      throw std::length_error("device memory: a pool of " + std::to_string(bytes >> 20) + " MiB leaves less than " +
-                              std::to_string(DEVICE_RESERVE >> 30) + " GiB of the " + std::to_string(freeB >> 20) +
-                              " MiB free (fewer batch lanes or smaller pools fit)");
   }
 };
 
@@ -1398,6 +1491,10 @@ std::vector<HullResult> GpuBackend::hullsWithHausdorff(const std::vector<MeshRef
 #define CUACD_API(name) cuacd_##name
 #define CUACD_MAKE_BACKEND(cfg) (new cuacd::GpuBackend(cfg))
 #include "../host/capi_body.inc"
+
+// No counterpart in the reference: the device blocks released engines keep for reuse go back to
+// the devices (a caller that needs the memory for other work).
+CUACD_EXPORT void cuacd_release_cached_memory(void) { cuacd::BlockCache::get().release(-1); }
 
 // ---- the allocation-free Hausdorff entry point ---------------------------------------------------
 // Every device array it uses lives in `scratch`, one static array per GPU that the runtime maps with

@@ -166,6 +166,12 @@ def _read_result(api, h):
     return DecomposeResult(parts, s.iterations, stats, merge, (s.norm_scale, tuple(s.norm_center)), cuts, ks.as_dict())
 
 
+def release_cached_memory(api=None):
+    """Gives the device blocks that released engines keep for reuse back to the devices
+    (cuacd_release_cached_memory); the next engine allocates afresh."""
+    (api or load_product()).release_cached_memory()
+
+
 def decompose(mesh, options=None, api=None, **overrides):
     """decompose(mesh, options) (src/decompose.ts:407): one device, decompose, release."""
     api = api or load_product()

@@ -292,6 +292,12 @@ void cuacd_decomposer_destroy(cuacd_decomposer* d);
  * its next calls, keeping its pools; every other field of `o` is ignored. Results do not depend on
  * these knobs. The number of engines (batch_lanes) is fixed at create. */
 int cuacd_decomposer_tune(cuacd_decomposer* d, const cuacd_decompose_options* o);
+/* No reference counterpart: device blocks of 64 MiB and more that released engines (trees,
+ * decomposers, one-shot decompose calls) leave behind are kept for reuse by the next engine, so an
+ * engine per call does not map and unmap its pools every call; the cache gives memory back by
+ * itself when an allocation would not fit, and keeps at most 128 GiB per device. This returns every
+ * cached block to its device now. */
+void cuacd_release_cached_memory(void);
 /* Decomposer.decompose(mesh) (src/decompose.ts:214-398); kernel stats of the result cover this call */
 int cuacd_decomposer_decompose(cuacd_decomposer* d, const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt,
                                cuacd_result** out);

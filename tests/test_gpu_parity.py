@@ -416,11 +416,45 @@ def test_tree_graph_returns_device_memory(cw):
     g = cw.GpuTree(**cfg)
     want = g.run([mesh])
     g.destroy()
+    cw.release_cached_memory()  # released engines keep their blocks for reuse until this
     f0 = free_gib()
     for k in range(3):
         g = cw.GpuTree(use_graph=True, **cfg)
         got = g.run([mesh])
         assert got.levels == want.levels
         g.destroy()
+        cw.release_cached_memory()
         print(f"graph engine {k}: {f0 - free_gib():.2f} GiB not returned", flush=True)
     assert f0 - free_gib() < 2.0
+
+
+def test_released_engines_reuse_device_blocks(cw):
+    """Engines made and released one after another (as decompose(mesh, options) does per call) reuse
+    the blocks the previous one released instead of allocating afresh: device memory in use stays at
+    one engine's pools, and release_cached_memory gives them back."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+
+    def free_gib():
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        assert hip.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) == 0
+        return f.value / 2 ** 30
+
+    mesh = cw.normalize(cw.generate("blobs", 20260201, [5, 24]))[0]
+    cw.release_cached_memory()
+    f0 = free_gib()
+    g = cw.GpuTree(width=12, width2=6, depth=2)
+    want = g.run([mesh])
+    g.destroy()
+    one = f0 - free_gib()  # one engine's pools, kept by the cache
+    print(f"one engine's blocks: {one:.1f} GiB", flush=True)
+    assert one > 8.0  # the 16 GiB hull arena at least in part, and the pools
+    for k in range(4):
+        g = cw.GpuTree(width=12, width2=6, depth=2)
+        assert g.run([mesh]).levels == want.levels
+        g.destroy()
+        assert abs((f0 - free_gib()) - one) < 1.0, k  # the same blocks again
+    r = cw.decompose(cw.l_shape(), max_iters=2)
+    assert len(r.parts) >= 1 and abs((f0 - free_gib()) - one) < 1.0
+    cw.release_cached_memory()
+    assert f0 - free_gib() < 1.0
