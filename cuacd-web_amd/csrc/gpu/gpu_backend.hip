@@ -44,7 +44,9 @@ void hipCheck(hipError_t e, const char* what) {
 // cuacd_release_cached_memory.
 class BlockCache {
  public:
-  static constexpr size_t MIN_BYTES = 64ull << 20, MAX_CACHED = 128ull << 30;
+  // MAX_CACHED holds a whole batch's engines: freeing their blocks (some 200 GiB) made the driver
+  // slow every launch of the next seconds (one-shot calls after a batch took 2-10x their time)
+  static constexpr size_t MIN_BYTES = 64ull << 20, MAX_CACHED = 256ull << 30;
   static BlockCache& get() {
     static BlockCache* c = new BlockCache;  // never destroyed: engines released at exit still return blocks
     return *c;

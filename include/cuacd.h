@@ -295,7 +295,7 @@ int cuacd_decomposer_tune(cuacd_decomposer* d, const cuacd_decompose_options* o)
 /* No reference counterpart: device blocks of 64 MiB and more that released engines (trees,
  * decomposers, one-shot decompose calls) leave behind are kept for reuse by the next engine, so an
  * engine per call does not map and unmap its pools every call; the cache gives memory back by
- * itself when an allocation would not fit, and keeps at most 128 GiB per device. This returns every
+ * itself when an allocation would not fit, and keeps at most 256 GiB per device. This returns every
  * cached block to its device now. */
 void cuacd_release_cached_memory(void);
 /* Decomposer.decompose(mesh) (src/decompose.ts:214-398); kernel stats of the result cover this call */
