@@ -1,15 +1,15 @@
-// qhull.cpp — double-precision convex hull with a distance tolerance (host), and the
+// qhull.cpp — the reference CPU hull restated in double precision (host), and the
 // output-hull cleanup of src/cpu/hull.ts:87-154 on top of it.
 //
-// The hull is an incremental one of our own: a simplex from extreme points, outside
-// sets per face, a face queue, the visible region found by a walk over face
-// adjacency from the face whose farthest point is inserted, a horizon loop, a fan of
-// new faces, and the orphaned points handed to the new faces. Its tolerance rules
-// are those of the reference's CPU hull (Kuukka's quickhull as wasm/qh.cpp:16-25
-// calls it): eps scales with the largest absolute extreme coordinate, a point joins
-// the first face whose plane it lies more than that distance above, a face is
-// visible from the inserted point when the point is on its positive side, and a
-// point whose horizon does not close into one loop is dropped.
+// hullEps follows the steps of Antti Kuukka's quickhull as the reference builds it
+// (wasm/quickhull/QuickHull.cpp:53-514, called by wasm/qh.cpp:16-25 with CCW output
+// and compacted vertices), so that the cleaned output hulls are the reference's own:
+// the same extreme points, tetrahedron, tolerance (eps times the largest absolute
+// extreme coordinate), face queue (first in, first out), depth-first search of the
+// visible faces, horizon loop, recycling of dead face and half-edge slots, point
+// hand-over to the first new face a point lies beyond, and the depth-first face
+// order of the output. Every product and sum is evaluated in the order the reference
+// writes it (the host code is built without contraction, as the reference's wasm is).
 #include "qhull.hpp"
 
 #include <algorithm>
@@ -17,301 +17,416 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <unordered_map>
 
 namespace cuacd {
 
 namespace {
 
-constexpr uint32_t NIL = 0xffffffffu;
+constexpr size_t NOSLOT = std::numeric_limits<size_t>::max();
 
-struct QFace {
-  uint32_t v[3];
-  int32_t adj[3];  // face across edge v[k] -> v[k + 1]
-  double n[3], d, nn;
-  uint32_t head = NIL, count = 0;  // the outside set: a list through Hull::next
-  uint32_t far = 0;
-  double farD = 0;
-  uint64_t seen = 0;  // iteration of the last visibility test
-  bool vis = false, alive = true;
+struct V3 {
+  double x, y, z;
+};
+inline V3 sub(const V3& a, const V3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline double dot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline double len2(const V3& a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+inline double dist2(const V3& a, const V3& b) {
+  const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  return dx * dx + dy * dy + dz * dz;
+}
+// (a - c) x (b - c), QuickHull's triangle normal (MathUtils.hpp:27-39)
+inline V3 triNormal(const V3& a, const V3& b, const V3& c) {
+  const double x = a.x - c.x, y = a.y - c.y, z = a.z - c.z;
+  const double rx = b.x - c.x, ry = b.y - c.y, rz = b.z - c.z;
+  return {y * rz - z * ry, z * rx - x * rz, x * ry - y * rx};
+}
+
+struct QPlane {
+  V3 n{0, 0, 0};
+  double d = 0, nn = 0;  // d = -n.p for a point p of the plane, nn = |n|^2
+  QPlane() = default;
+  QPlane(const V3& N, const V3& p) : n(N), d(-dot(N, p)), nn(N.x * N.x + N.y * N.y + N.z * N.z) {}
+  double eval(const V3& q) const { return dot(n, q) + d; }
 };
 
-struct Hull {
-  const double* P;
-  double eps2;
-  std::vector<QFace> F;
-  std::vector<uint32_t> next;  // per point: the next point of its face's outside set
+struct HalfEdge {
+  size_t end = NOSLOT, opp = 0, face = 0, next = 0;  // end NOSLOT: a dead slot
+};
 
-  const double* pt(uint32_t i) const { return P + 3 * (size_t)i; }
-  uint32_t make(uint32_t a, uint32_t b, uint32_t c) {
-    QFace f;
-    f.v[0] = a;
-    f.v[1] = b;
-    f.v[2] = c;
-    f.adj[0] = f.adj[1] = f.adj[2] = -1;
-    const double* A = pt(a);
-    const double* B = pt(b);
-    const double* C = pt(c);
-    const double u[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
-    const double w[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
-    f.n[0] = u[1] * w[2] - u[2] * w[1];
-    f.n[1] = u[2] * w[0] - u[0] * w[2];
-    f.n[2] = u[0] * w[1] - u[1] * w[0];
-    f.d = -(f.n[0] * A[0] + f.n[1] * A[1] + f.n[2] * A[2]);
-    f.nn = f.n[0] * f.n[0] + f.n[1] * f.n[1] + f.n[2] * f.n[2];
-    F.push_back(std::move(f));
-    return (uint32_t)(F.size() - 1);
+struct QFace {
+  size_t he = NOSLOT;  // NOSLOT: a dead slot
+  QPlane pl;
+  double farD = 0;
+  size_t far = 0;
+  size_t seenIter = 0;
+  bool visible = false, queued = false;
+  uint8_t horizonBits = 0;
+  int pts = -1;  // its outside points: a list of the point pool, or none
+};
+
+struct QHull {
+  std::vector<V3> P;
+  double eps = 0, eps2 = 0;
+  std::vector<QFace> F;
+  std::vector<HalfEdge> H;
+  std::vector<size_t> deadF, deadH;
+  std::vector<std::vector<uint32_t>> lists;  // outside-point lists (reused when a face dies)
+  std::vector<int> freeLists;
+
+  std::array<size_t, 3> edgesOf(const QFace& f) const { return {f.he, H[f.he].next, H[H[f.he].next].next}; }
+  std::array<size_t, 3> vertsOf(const QFace& f) const {
+    const size_t e0 = f.he, e1 = H[e0].next, e2 = H[e1].next;
+    return {H[e0].end, H[e1].end, H[e2].end};
   }
-  double dist(const QFace& f, uint32_t i) const {
-    const double* p = pt(i);
-    return f.n[0] * p[0] + f.n[1] * p[1] + f.n[2] * p[2] + f.d;
+  size_t newFace() {
+    if (!deadF.empty()) {
+      const size_t i = deadF.back();
+      deadF.pop_back();
+      F[i].farD = 0;  // a recycled slot keeps its other fields until they are set
+      return i;
+    }
+    F.emplace_back();
+    return F.size() - 1;
   }
-  // point i joins face f when it lies beyond the tolerance above its plane
-  bool offer(uint32_t fi, uint32_t i) {
-    QFace& f = F[fi];
-    const double D = dist(f, i);
-    if (!(D > 0 && D * D > eps2 * f.nn)) return false;
-    if (f.count == 0 || D > f.farD) {
+  size_t newEdge() {
+    if (!deadH.empty()) {
+      const size_t i = deadH.back();
+      deadH.pop_back();
+      return i;
+    }
+    H.emplace_back();
+    return H.size() - 1;
+  }
+  int takeList() {
+    if (!freeLists.empty()) {
+      const int l = freeLists.back();
+      freeLists.pop_back();
+      lists[l].clear();
+      return l;
+    }
+    lists.emplace_back();
+    return (int)lists.size() - 1;
+  }
+  void giveList(int l) { freeLists.push_back(l); }
+  int killFace(size_t i) {
+    QFace& f = F[i];
+    f.he = NOSLOT;
+    deadF.push_back(i);
+    const int l = f.pts;
+    f.pts = -1;
+    return l;
+  }
+  void killEdge(size_t i) {
+    H[i].end = NOSLOT;
+    deadH.push_back(i);
+  }
+  // QuickHull.hpp:195-209: point i joins face f when it lies beyond the tolerance
+  bool offer(QFace& f, uint32_t i) {
+    const double D = f.pl.eval(P[i]);
+    if (!(D > 0 && D * D > eps2 * f.pl.nn)) return false;
+    if (f.pts < 0) f.pts = takeList();
+    lists[f.pts].push_back(i);
+    if (D > f.farD) {
       f.farD = D;
       f.far = i;
     }
-    // appended at the tail would keep the insertion order; the order of a set changes nothing
-    // but which of two equally distant points is farthest, and that is decided above in order
-    next[i] = f.head;
-    f.head = i;
-    f.count++;
     return true;
   }
-  int edgeOf(uint32_t fi, uint32_t a, uint32_t b) const {
-    const QFace& f = F[fi];
-    for (int k = 0; k < 3; k++)
-      if (f.v[k] == a && f.v[k == 2 ? 0 : k + 1] == b) return k;
-    return -1;
+  // the tetrahedron a, b, c, d: four faces and twelve half-edges in the reference's slots
+  // (Structs/Mesh.hpp:108-209)
+  void tetrahedron(size_t a, size_t b, size_t c, size_t d) {
+    F.assign(4, QFace());
+    H.assign(12, HalfEdge());
+    deadF.clear();
+    deadH.clear();
+    // (end vertex, opposite, face, next) of AB BC CA | AC CD DA | BA AD DB | CB BD DC
+    const size_t e[12][4] = {{b, 6, 0, 1},  {c, 9, 0, 2},  {a, 3, 0, 0},  {c, 2, 1, 4},
+                             {d, 11, 1, 5}, {a, 7, 1, 3},  {a, 0, 2, 7},  {d, 5, 2, 8},
+                             {b, 10, 2, 6}, {b, 1, 3, 10}, {d, 8, 3, 11}, {c, 4, 3, 9}};
+    for (int k = 0; k < 12; k++) H[k] = HalfEdge{e[k][0], e[k][1], e[k][2], e[k][3]};
+    for (int k = 0; k < 4; k++) F[k].he = (size_t)(3 * k);
   }
+  bool reorder(std::vector<size_t>& hz) const {
+    for (size_t i = 0; i + 1 < hz.size(); i++) {
+      const size_t endV = H[hz[i]].end;
+      bool found = false;
+      for (size_t j = i + 1; j < hz.size(); j++)
+        if (H[H[hz[j]].opp].end == endV) {
+          std::swap(hz[i + 1], hz[j]);
+          found = true;
+          break;
+        }
+      if (!found) return false;
+    }
+    return true;
+  }
+  void setupTetrahedron(const std::array<size_t, 6>& ext, size_t n0);
+  void run();
 };
 
-double dist2(const double* a, const double* b) {
-  const double x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
-  return x * x + y * y + z * z;
+// QuickHull.cpp:391-496
+void QHull::setupTetrahedron(const std::array<size_t, 6>& ext, size_t n0) {
+  const size_t n = P.size();
+  if (n <= 4) {
+    size_t v[4] = {0, std::min<size_t>(1, n - 1), std::min<size_t>(2, n - 1), std::min<size_t>(3, n - 1)};
+    const QPlane pl(triNormal(P[v[0]], P[v[1]], P[v[2]]), P[v[0]]);
+    if (pl.eval(P[v[3]]) >= 0) std::swap(v[0], v[1]);
+    tetrahedron(v[0], v[1], v[2], v[3]);
+    return;
+  }
+  double maxD = eps2;
+  size_t s0 = 0, s1 = 0;
+  for (size_t i = 0; i < 6; i++)
+    for (size_t j = i + 1; j < 6; j++) {
+      const double d = dist2(P[ext[i]], P[ext[j]]);
+      if (d > maxD) {
+        maxD = d;
+        s0 = ext[i];
+        s1 = ext[j];
+      }
+    }
+  if (maxD == eps2) {  // a single point, within the tolerance
+    tetrahedron(0, std::min<size_t>(1, n - 1), std::min<size_t>(2, n - 1), std::min<size_t>(3, n - 1));
+    return;
+  }
+  // the point farthest from the line s0 s1
+  const V3 rs = P[s0], rv = sub(P[s1], P[s0]);
+  const double rInv = 1 / len2(rv);
+  maxD = eps2;
+  size_t maxI = NOSLOT;
+  for (size_t i = 0; i < n; i++) {
+    const V3 s = sub(P[i], rs);
+    const double t = dot(s, rv);
+    const double d = len2(s) - t * t * rInv;
+    if (d > maxD) {
+      maxD = d;
+      maxI = i;
+    }
+  }
+  if (maxD == eps2) {  // the points lie on a line
+    auto other = [&](size_t skip3) {
+      for (size_t i = 0; i < n; i++) {
+        const V3& q = P[i];
+        auto ne = [&](const V3& o) { return q.x != o.x || q.y != o.y || q.z != o.z; };
+        if (ne(P[s0]) && ne(P[s1]) && (skip3 == NOSLOT || ne(P[skip3]))) return i;
+      }
+      return s0;
+    };
+    const size_t third = other(NOSLOT);
+    tetrahedron(s0, s1, third, other(third));
+    return;
+  }
+  size_t base[3] = {s0, s1, maxI};
+  const V3 bv[3] = {P[base[0]], P[base[1]], P[base[2]]};
+  const V3 N = triNormal(bv[0], bv[1], bv[2]);
+  const QPlane tri(N, bv[0]);
+  double maxE = eps;
+  maxI = 0;
+  for (size_t i = 0; i < n; i++) {
+    const double d = std::fabs(tri.eval(P[i]));
+    if (d > maxE) {
+      maxE = d;
+      maxI = i;
+    }
+  }
+  if (maxE == eps) {  // the points lie in a plane: one point off it gives the hull a volume
+    const V3 N1 = triNormal(bv[1], bv[2], bv[0]);
+    P.push_back({N1.x + P[0].x, N1.y + P[0].y, N1.z + P[0].z});
+    maxI = P.size() - 1;
+  }
+  if (tri.eval(P[maxI]) >= 0) std::swap(base[0], base[1]);
+  tetrahedron(base[0], base[1], base[2], maxI);
+  for (QFace& f : F) {
+    const auto v = vertsOf(f);
+    f.pl = QPlane(triNormal(P[v[0]], P[v[1]], P[v[2]]), P[v[0]]);
+  }
+  for (size_t i = 0; i < n0; i++)
+    for (QFace& f : F)
+      if (offer(f, (uint32_t)i)) break;
+}
+
+// QuickHull.cpp:98-296
+void QHull::run() {
+  std::deque<size_t> queue;
+  for (size_t i = 0; i < 4; i++)
+    if (F[i].pts >= 0 && !lists[F[i].pts].empty()) {
+      queue.push_back(i);
+      F[i].queued = true;
+    }
+  std::vector<std::pair<size_t, size_t>> maybe;  // (face, half-edge entered by)
+  std::vector<size_t> visible, horizon, newFaces, newEdges;
+  std::vector<int> orphanLists;
+  size_t iter = 0;
+  while (!queue.empty()) {
+    iter++;
+    const size_t top = queue.front();
+    queue.pop_front();
+    QFace& tf = F[top];
+    tf.queued = false;
+    if (tf.pts < 0 || tf.he == NOSLOT) continue;
+    const size_t eyeI = tf.far;
+    const V3 eye = P[eyeI];
+    // the visible faces, depth first from the top face; a hidden face reached over an edge
+    // makes that edge a horizon edge
+    horizon.clear();
+    visible.clear();
+    maybe.assign(1, {top, NOSLOT});
+    while (!maybe.empty()) {
+      const auto [fi, from] = maybe.back();
+      maybe.pop_back();
+      QFace& f = F[fi];
+      if (f.seenIter == iter) {
+        if (f.visible) continue;
+      } else {
+        f.seenIter = iter;
+        if (f.pl.eval(eye) > 0) {
+          f.visible = true;
+          f.horizonBits = 0;
+          visible.push_back(fi);
+          for (size_t e : edgesOf(f))
+            if (H[e].opp != from) maybe.push_back({H[H[e].opp].face, e});
+          continue;
+        }
+      }
+      f.visible = false;
+      horizon.push_back(from);
+      QFace& vf = F[H[from].face];
+      const auto es = edgesOf(vf);
+      vf.horizonBits |= (uint8_t)(1u << (es[0] == from ? 0 : (es[1] == from ? 1 : 2)));
+    }
+    const size_t nh = horizon.size();
+    if (!reorder(horizon)) {
+      // no single horizon loop (numerical trouble): the point leaves its face, which is not
+      // queued again
+      auto& L = lists[tf.pts];
+      L.erase(std::find(L.begin(), L.end(), (uint32_t)eyeI));
+      if (L.empty()) {
+        giveList(tf.pts);
+        tf.pts = -1;
+      }
+      continue;
+    }
+    // the visible faces die; their non-horizon half-edges serve the new faces first
+    newFaces.clear();
+    newEdges.clear();
+    orphanLists.clear();
+    size_t used = 0;
+    for (size_t fi : visible) {
+      const auto es = edgesOf(F[fi]);
+      for (int j = 0; j < 3; j++)
+        if (!(F[fi].horizonBits & (1u << j))) {
+          if (used < 2 * nh) {
+            newEdges.push_back(es[j]);
+            used++;
+          } else {
+            killEdge(es[j]);
+          }
+        }
+      const int l = killFace(fi);
+      if (l >= 0) orphanLists.push_back(l);
+    }
+    while (used < 2 * nh) {
+      newEdges.push_back(newEdge());
+      used++;
+    }
+    // a fan of new faces (a, b, eye) along the horizon loop
+    for (size_t i = 0; i < nh; i++) {
+      const size_t ab = horizon[i];
+      const size_t a = H[H[ab].opp].end, b = H[ab].end;
+      const size_t nf = newFace();
+      newFaces.push_back(nf);
+      const size_t ca = newEdges[2 * i], bc = newEdges[2 * i + 1];
+      H[ab].next = bc;
+      H[bc].next = ca;
+      H[ca].next = ab;
+      H[bc].face = nf;
+      H[ca].face = nf;
+      H[ab].face = nf;
+      H[ca].end = a;
+      H[bc].end = eyeI;
+      F[nf].pl = QPlane(triNormal(P[a], P[b], eye), eye);
+      F[nf].he = ab;
+      H[ca].opp = newEdges[i > 0 ? 2 * i - 1 : 2 * nh - 1];
+      H[bc].opp = newEdges[((i + 1) * 2) % (2 * nh)];
+    }
+    // the dead faces' points go to the first new face they lie beyond
+    for (int l : orphanLists) {
+      for (uint32_t p : lists[l]) {
+        if (p == eyeI) continue;
+        for (size_t j = 0; j < nh; j++)
+          if (offer(F[newFaces[j]], p)) break;
+      }
+      giveList(l);
+    }
+    for (size_t nf : newFaces)
+      if (F[nf].pts >= 0 && !F[nf].queued) {
+        queue.push_back(nf);
+        F[nf].queued = true;
+      }
+  }
 }
 
 }  // namespace
 
-bool hullEps(const double* P, uint32_t n, double eps, std::vector<uint32_t>* tris) {
+bool hullEps(const double* pts, uint32_t n, double eps, std::vector<uint32_t>* tris) {
   tris->clear();
-  if (n < 4) return false;
-  // extreme points per axis (lowest index on ties) and the scale of the cloud
-  uint32_t ext[6];
-  for (int a = 0; a < 3; a++) {
-    ext[2 * a] = ext[2 * a + 1] = 0;
-    for (uint32_t i = 1; i < n; i++) {
-      if (P[3 * (size_t)i + a] > P[3 * (size_t)ext[2 * a] + a]) ext[2 * a] = i;
-      if (P[3 * (size_t)i + a] < P[3 * (size_t)ext[2 * a + 1] + a]) ext[2 * a + 1] = i;
+  if (n == 0) return false;
+  QHull Q;
+  Q.P.resize(n);
+  for (uint32_t i = 0; i < n; i++) Q.P[i] = {pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]};
+  // extreme points (QuickHull.cpp:302-336): a point that raises an axis maximum is not
+  // tested against that axis' minimum
+  std::array<size_t, 6> ext{0, 0, 0, 0, 0, 0};
+  double ev[6] = {Q.P[0].x, Q.P[0].x, Q.P[0].y, Q.P[0].y, Q.P[0].z, Q.P[0].z};
+  for (uint32_t i = 1; i < n; i++) {
+    const double c[3] = {Q.P[i].x, Q.P[i].y, Q.P[i].z};
+    for (int a = 0; a < 3; a++) {
+      if (c[a] > ev[2 * a]) {
+        ev[2 * a] = c[a];
+        ext[2 * a] = i;
+      } else if (c[a] < ev[2 * a + 1]) {
+        ev[2 * a + 1] = c[a];
+        ext[2 * a + 1] = i;
+      }
     }
   }
   double scale = 0;
-  for (int k = 0; k < 6; k++) scale = std::max(scale, std::fabs(P[3 * (size_t)ext[k] + k / 2]));
-  const double epsA = eps * scale;
-  Hull H;
-  H.P = P;
-  H.eps2 = epsA * epsA;
-  H.F.reserve(2 * (size_t)n + 16);
-  H.next.assign(n, NIL);
-  // simplex: the farthest pair of extremes, the point farthest from their line, the point
-  // farthest from their plane
-  uint32_t i0 = ext[0], i1 = ext[1];
-  double best = -1;
-  for (int a = 0; a < 6; a++)
-    for (int b = a + 1; b < 6; b++) {
-      const double d = dist2(H.pt(ext[a]), H.pt(ext[b]));
-      if (d > best) {
-        best = d;
-        i0 = ext[a];
-        i1 = ext[b];
-      }
-    }
-  if (!(best > H.eps2)) return false;
-  const double* p0 = H.pt(i0);
-  const double* p1 = H.pt(i1);
-  const double e[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
-  const double ee = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
-  uint32_t i2 = 0;
-  best = -1;
-  for (uint32_t i = 0; i < n; i++) {
-    const double* p = H.pt(i);
-    const double q[3] = {p[0] - p0[0], p[1] - p0[1], p[2] - p0[2]};
-    const double c[3] = {q[1] * e[2] - q[2] * e[1], q[2] * e[0] - q[0] * e[2], q[0] * e[1] - q[1] * e[0]};
-    const double d = (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) / ee;  // squared distance from the line
-    if (d > best) {
-      best = d;
-      i2 = i;
-    }
+  for (int k = 0; k < 6; k++) {
+    const V3& v = Q.P[ext[k]];
+    const double a = std::fabs(k / 2 == 0 ? v.x : (k / 2 == 1 ? v.y : v.z));
+    if (a > scale) scale = a;
   }
-  if (!(best > H.eps2)) return false;
-  const double* p2 = H.pt(i2);
-  const double u[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
-  const double w[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
-  const double nr[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
-  const double nl2 = nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2];
-  uint32_t i3 = 0;
-  double bs = 0;
-  best = -1;
-  for (uint32_t i = 0; i < n; i++) {
-    const double* p = H.pt(i);
-    const double s = nr[0] * (p[0] - p0[0]) + nr[1] * (p[1] - p0[1]) + nr[2] * (p[2] - p0[2]);
-    if (std::fabs(s) > best) {
-      best = std::fabs(s);
-      bs = s;
-      i3 = i;
+  Q.eps = eps * scale;
+  Q.eps2 = Q.eps * Q.eps;
+  Q.setupTetrahedron(ext, n);
+  Q.run();
+  const size_t extra = Q.P.size() > n ? n : NOSLOT;  // the planar case's added point reads as point 0
+  // output (ConvexHull.hpp:96-146): faces depth first from the lowest live slot, neighbours over
+  // the face's half-edges in order, each face written (v0, v2, v1)
+  std::vector<uint8_t> done(Q.F.size(), 0);
+  std::vector<size_t> stack;
+  for (size_t i = 0; i < Q.F.size(); i++)
+    if (Q.F[i].he != NOSLOT) {
+      stack.push_back(i);
+      break;
     }
+  while (!stack.empty()) {
+    const size_t top = stack.back();
+    stack.pop_back();
+    if (done[top]) continue;
+    done[top] = 1;
+    const auto es = Q.edgesOf(Q.F[top]);
+    for (size_t e : es) {
+      const size_t a = Q.H[Q.H[e].opp].face;
+      if (!done[a] && Q.F[a].he != NOSLOT) stack.push_back(a);
+    }
+    auto v = Q.vertsOf(Q.F[top]);
+    for (auto& x : v)
+      if (x == extra) x = 0;
+    tris->insert(tris->end(), {(uint32_t)v[0], (uint32_t)v[2], (uint32_t)v[1]});
   }
-  if (!(best * best > H.eps2 * nl2)) return false;
-  // faces wound so that the fourth point lies below each
-  uint32_t a = i0, b = i1, c = i2;
-  if (bs > 0) std::swap(b, c);
-  const uint32_t f0 = H.make(a, b, c), f1 = H.make(a, i3, b), f2 = H.make(b, i3, c), f3 = H.make(c, i3, a);
-  const uint32_t fs[4] = {f0, f1, f2, f3};
-  for (uint32_t x : fs)
-    for (uint32_t y : fs) {
-      if (x == y) continue;
-      for (int k = 0; k < 3; k++) {
-        const uint32_t va = H.F[x].v[k], vb = H.F[x].v[k == 2 ? 0 : k + 1];
-        if (H.edgeOf(y, vb, va) >= 0) H.F[x].adj[k] = (int32_t)y;
-      }
-    }
-  std::deque<uint32_t> queue;
-  for (uint32_t i = 0; i < n; i++) {
-    if (i == i0 || i == i1 || i == i2 || i == i3) continue;
-    for (uint32_t f : fs)
-      if (H.offer(f, i)) break;
-  }
-  for (uint32_t f : fs)
-    if (H.F[f].count) queue.push_back(f);
-  uint64_t iter = 0;
-  std::vector<uint32_t> visible, stack;
-  std::vector<std::pair<uint32_t, int>> horizon;  // (visible face, edge) with a hidden face across
-  std::vector<std::pair<uint32_t, uint32_t>> fromVertex;  // (first vertex, horizon entry)
-  std::vector<uint32_t> orphans, loop;
-  std::vector<uint8_t> onLoop;
-  auto startingAt = [&](uint32_t v) -> size_t {
-    for (const auto& e : fromVertex)
-      if (e.first == v) return e.second;
-    return SIZE_MAX;
-  };
-  while (!queue.empty()) {
-    const uint32_t top = queue.front();
-    queue.pop_front();
-    if (!H.F[top].alive || !H.F[top].count) continue;
-    iter++;
-    const uint32_t eye = H.F[top].far;
-    // the visible region: a walk over adjacency from the top face
-    visible.clear();
-    horizon.clear();
-    stack.assign(1, top);
-    H.F[top].seen = iter;
-    H.F[top].vis = true;
-    while (!stack.empty()) {
-      const uint32_t f = stack.back();
-      stack.pop_back();
-      visible.push_back(f);
-      for (int k = 0; k < 3; k++) {
-        const uint32_t g = (uint32_t)H.F[f].adj[k];
-        QFace& G = H.F[g];
-        if (G.seen != iter) {
-          G.seen = iter;
-          G.vis = H.dist(G, eye) > 0;
-          if (G.vis) stack.push_back(g);
-        }
-        if (!G.vis) horizon.push_back({f, k});
-      }
-    }
-    // the horizon as one loop of directed edges
-    fromVertex.clear();
-    bool ok = true;
-    for (size_t h = 0; h < horizon.size() && ok; h++) {
-      const QFace& f = H.F[horizon[h].first];
-      const uint32_t v0 = f.v[horizon[h].second];
-      ok = startingAt(v0) == SIZE_MAX;
-      fromVertex.push_back({v0, (uint32_t)h});
-    }
-    loop.clear();
-    if (ok) {
-      size_t h = 0;
-      onLoop.assign(horizon.size(), 0);
-      for (size_t s = 0; s < horizon.size(); s++) {
-        if (onLoop[h]) {  // a shorter cycle: the horizon is not one loop
-          ok = false;
-          break;
-        }
-        onLoop[h] = 1;
-        loop.push_back((uint32_t)h);
-        const QFace& f = H.F[horizon[h].first];
-        const int k = horizon[h].second;
-        const size_t nx = startingAt(f.v[k == 2 ? 0 : k + 1]);
-        if (nx == SIZE_MAX) {
-          ok = false;
-          break;
-        }
-        h = nx;
-      }
-      ok = ok && h == 0 && loop.size() == horizon.size();
-    }
-    if (!ok) {
-      // no single horizon loop (numerical trouble): the point is dropped
-      QFace& T = H.F[top];
-      std::vector<uint32_t> rest;
-      for (uint32_t i = T.head; i != NIL; i = H.next[i])
-        if (i != eye) rest.push_back(i);
-      T.head = NIL;
-      T.count = 0;
-      for (auto it = rest.rbegin(); it != rest.rend(); ++it) {  // the list's order kept
-        const double D = H.dist(T, *it);
-        if (T.count == 0 || D > T.farD) {
-          T.farD = D;
-          T.far = *it;
-        }
-        H.next[*it] = T.head;
-        T.head = *it;
-        T.count++;
-      }
-      for (uint32_t f : visible) H.F[f].vis = false;
-      if (T.count) queue.push_back(top);
-      continue;
-    }
-    // new faces (a, b, eye) along the loop, linked to the hidden faces and to each other
-    orphans.clear();
-    for (uint32_t f : visible) {
-      QFace& V = H.F[f];
-      V.alive = false;
-      V.vis = false;
-      for (uint32_t i = V.head; i != NIL; i = H.next[i])
-        if (i != eye) orphans.push_back(i);
-      V.head = NIL;
-      V.count = 0;
-    }
-    const size_t nh = loop.size();
-    const uint32_t first = (uint32_t)H.F.size();
-    for (size_t s = 0; s < nh; s++) {
-      const auto& he = horizon[loop[s]];
-      const uint32_t va = H.F[he.first].v[he.second], vb = H.F[he.first].v[he.second == 2 ? 0 : he.second + 1];
-      const uint32_t hidden = (uint32_t)H.F[he.first].adj[he.second];
-      const uint32_t nf = H.make(va, vb, eye);
-      H.F[nf].adj[0] = (int32_t)hidden;
-      H.F[hidden].adj[H.edgeOf(hidden, vb, va)] = (int32_t)nf;
-    }
-    for (size_t s = 0; s < nh; s++) {
-      const uint32_t nf = first + (uint32_t)s;
-      H.F[nf].adj[1] = (int32_t)(first + (uint32_t)((s + 1) % nh));   // b -> eye: the next face's eye -> b
-      H.F[nf].adj[2] = (int32_t)(first + (uint32_t)((s + nh - 1) % nh));  // eye -> a: the previous face's a -> eye
-    }
-    for (uint32_t i : orphans)
-      for (size_t s = 0; s < nh; s++)
-        if (H.offer(first + (uint32_t)s, i)) break;
-    for (size_t s = 0; s < nh; s++)
-      if (H.F[first + s].count) queue.push_back(first + (uint32_t)s);
-  }
-  for (const QFace& f : H.F)
-    if (f.alive) tris->insert(tris->end(), {f.v[0], f.v[1], f.v[2]});
   return true;
 }
 

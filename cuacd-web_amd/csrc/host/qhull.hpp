@@ -6,12 +6,11 @@
 
 namespace cuacd {
 
-// Convex hull of n points (xyz doubles) with the reference CPU hull's tolerance
-// convention (src/cpu/hull.ts:19-20): eps is relative to the largest absolute
-// extreme coordinate s, and a point joins a face's outside set only when it lies
-// more than eps * s above the face's plane. Triangles index the input points and
-// are wound counter-clockwise seen from outside. false when the points span no
-// volume (fewer than 4 points, or all within the tolerance of a plane or line).
+// The reference CPU hull (Kuukka's quickhull, wasm/qh.cpp:16-25 qh_compute) of n
+// points (xyz doubles) with tolerance eps relative to the largest absolute extreme
+// coordinate: the same faces in the same order, each triangle indexing the input
+// points (the reference indexes its compacted vertex list instead). Degenerate
+// clouds give the reference's degenerate tetrahedra. false only when n is 0.
 bool hullEps(const double* pts, uint32_t n, double eps, std::vector<uint32_t>* tris);
 
 // src/cpu/hull.ts:87-154 cleanHullFaces: the faces of an exact (grid-quantized)

@@ -1,16 +1,76 @@
 """Output-hull cleanup (src/cpu/hull.ts:87-185) and the host mesh formats (src/mesh.ts:166-242).
 
-cleanHullFaces rebuilds an exact hull in double precision with a tolerance. Where the hull is unique
-(points in general position) the library's faces equal those of the reference's steps run over the
-reference's own CPU hull (tests/ref_clean.py); on smooth surfaces a hull with a tolerance may keep a
-different few of the near-coplanar vertices (its choice depends on the insertion order), so there the
-checks are the reference's own acceptance rules: closed, the raw volume within 10 eps, every raw vertex
-within 5 eps extent, no face thinner than eps extent, vertices a subset of the raw hull's."""
+cleanHullFaces rebuilds an exact hull in double precision with a tolerance, with the library's
+restatement of the reference's CPU hull (Kuukka's quickhull, its steps and order). That hull is compared
+with the reference's own quickhull (oracle/_ref) face for face, and the cleaned face lists with the
+reference's cleanup steps run over it (tests/ref_clean.py), in order, on general-position clouds and on
+the smooth surfaces of decomposed parts, where a hull with a tolerance keeps some near-coplanar points
+and not others; the reference's acceptance rules (closed, the raw volume within 10 eps, every raw vertex
+within 5 eps extent, no face thinner than eps extent, vertices a subset of the raw hull's) are checked
+besides."""
 import numpy as np
 import pytest
 
 import ref_clean as rc
 from fixtures import park_miller_cloud
+
+
+@pytest.fixture(scope="module")
+def our_qh(tmp_path_factory):
+    """The library's hull restatement (qhull.cpp) with a C driver, built for the test."""
+    import ctypes
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    root = __import__("os").path.dirname(__import__("os").path.dirname(__import__("os").path.abspath(__file__)))
+    out = tmp_path_factory.mktemp("qh") / "libourqh.so"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", "-o", str(out),
+                    f"{root}/tests/qhull_driver.cpp", f"{root}/cuacd-web_amd/csrc/host/qhull.cpp"], check=True)
+    lib = ctypes.CDLL(str(out))
+    lib.our_qh_tris.restype = ctypes.c_int
+
+    def run(pts, eps):
+        p = np.ascontiguousarray(pts, np.float64).reshape(-1)
+        n = len(p) // 3
+        cap = 2 * n + 8
+        t = np.zeros(3 * cap, np.uint32)
+        nt = lib.our_qh_tris(p.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, ctypes.c_double(eps),
+                             t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), cap)
+        return t[: 3 * nt].reshape(-1, 3)
+    return run
+
+
+def _clouds(cw, oracle):
+    rng = np.random.default_rng(11)
+    out = [rng.standard_normal((n, 3)) for n in (5, 9, 60, 300, 3000)]
+    s = rng.standard_normal((2000, 3))
+    out.append(s / np.linalg.norm(s, axis=1, keepdims=True))  # every point extreme
+    out.append(np.round(rng.random((800, 3)) * 5))  # duplicates and coplanar runs
+    out.append(np.concatenate([rng.random((60, 2)), np.zeros((60, 1))], axis=1))  # planar
+    out.append(np.outer(rng.random(30), [1.0, 2.0, -0.5]))  # collinear
+    out.append(np.ones((7, 3)))  # one point
+    out += [rng.standard_normal((k, 3)) for k in (1, 2, 3, 4)]
+    out.append(park_miller_cloud().astype(np.float64))
+    for kind, seed, params in (("blobs", 20260200, [6, 40]), ("mix", 20260400, [1]), ("torus", 3, [0.3, 0.1, 1.0])):
+        r = cw.decompose(cw.generate(kind, seed, params), api=oracle, max_iters=2, threshold=0.05, width=12,
+                         merge_hulls=False)
+        out += [p.mesh.verts.astype(np.float64) for p in r.parts[:4]]  # smooth part surfaces
+    return out
+
+
+@pytest.mark.skipif(not __import__("os").path.exists(rc.REF_QH), reason="oracle/_ref not built (needs the reference tree)")
+def test_hull_equals_reference_quickhull(cw, oracle, our_qh):
+    """The library's hull (qhull.cpp) gives the reference quickhull's faces in the reference's order, on
+    general-position clouds, smooth part surfaces with near-coplanar points, points all on a sphere,
+    duplicates, planar, collinear and single-point clouds, and 1 to 4 points (faces compared by their
+    corners' coordinates: the reference numbers its compacted vertices)."""
+    for k, pts in enumerate(_clouds(cw, oracle)):
+        for eps in (2e-5, 1e-3):
+            ours = our_qh(pts, eps)
+            rv, rt = rc.ref_hull_tris(pts, eps)
+            assert ours.shape == rt.shape, (k, eps, ours.shape, rt.shape)
+            assert np.array_equal(pts[ours.reshape(-1)], rv[rt.reshape(-1)]), (k, eps)
 
 
 def _exported(cw, oracle, meshes):
@@ -48,7 +108,7 @@ def test_clean_general_position_equals_reference(cw, oracle):
         m = cw.Mesh(pts, np.zeros((0, 3), np.uint32))
         tris, _ = _exported(cw, oracle, [m])[0]
         ours = cw.clean_hull_faces(pts, tris)
-        assert rc.canon(ours) == rc.canon(rc.ref_clean_hull_faces(pts, tris)), len(pts)
+        assert np.array_equal(ours, rc.ref_clean_hull_faces(pts, tris)), len(pts)
         _check_accepted(pts, tris, ours, 2e-5)
 
 
@@ -77,7 +137,7 @@ def test_clean_drops_grid_artifacts(cw):
 
 def test_clean_accepted_on_part_hulls(cw, oracle):
     """Exported hulls of decomposed parts (blob, staircase, torus): every cleaned list passes the
-    acceptance rules; against the reference's steps over its own hull the vertex counts agree within 3 %."""
+    acceptance rules and equals, in order, the reference's steps over its own hull."""
     meshes = []
     for kind, seed, params in (("blobs", 20260200, [6, 40]), ("mix", 20260400, [1]), ("torus", 3, [0.3, 0.1, 1.0])):
         r = cw.decompose(cw.generate(kind, seed, params), api=oracle, max_iters=2, threshold=0.05, width=12,
@@ -88,9 +148,7 @@ def test_clean_accepted_on_part_hulls(cw, oracle):
         C = cw.clean_hull_faces(m.verts, tris)
         _check_accepted(m.verts, tris, C, 2e-5)
         if have_ref:
-            R = rc.ref_clean_hull_faces(m.verts, tris)
-            a, b = len(set(C.reshape(-1).tolist())), len(set(R.reshape(-1).tolist()))
-            assert abs(a - b) <= max(2, 0.03 * b), (a, b)
+            assert np.array_equal(C, rc.ref_clean_hull_faces(m.verts, tris))
 
 
 def test_decomposed_hulls_are_cleaned(cw, oracle):
