@@ -458,3 +458,33 @@ def test_released_engines_reuse_device_blocks(cw):
     assert len(r.parts) >= 1 and abs((f0 - free_gib()) - one) < 1.0
     cw.release_cached_memory()
     assert f0 - free_gib() < 1.0
+
+
+def test_engines_on_threads_share_the_block_cache(cw):
+    """Engines made, run and released at once on several host threads (the cache's blocks handed
+    between them under its lock) give the results of one engine run alone."""
+    import threading
+    meshes = [cw.normalize(m)[0] for m in (stairs(cw), square_torus(cw), l_prism(cw))]
+    cfg = dict(width=12, width2=6, depth=3)
+    g = cw.GpuTree(**cfg)
+    want = [g.run([m]).levels for m in meshes]
+    g.destroy()
+    errors = []
+
+    def work(k):
+        try:
+            for r in range(3):
+                t = cw.GpuTree(**cfg)
+                got = t.run([meshes[(k + r) % 3]]).levels
+                t.destroy()
+                if got != want[(k + r) % 3]:
+                    errors.append((k, r))
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append((k, repr(e)))
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
