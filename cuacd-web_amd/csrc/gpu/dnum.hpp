@@ -2,9 +2,10 @@
 //
 // Every FP32 expression is written in the evaluation order the host side of
 // this library documents (left to right, no contraction: the kernels are built
-// with -ffp-contract=off), exact products use __builtin_fmaf, and volume sums
-// go into an int128 fixed-point accumulator so that the order in which lanes
-// and waves combine their partial sums cannot change a single bit.
+// with -ffp-contract=off), exact products are formed in FP64 (split by fma where
+// they exceed 53 bits), and volume sums go into an int128 fixed-point accumulator
+// so that the order in which lanes and waves combine their partial sums cannot
+// change a single bit.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -18,22 +19,22 @@ constexpr uint32_t NONE = 0xffffffffu;
 constexpr float PC_EPS_F = 1e-6f;  // src/gpu/shaders.ts:162
 
 // ---- fixed point: value * 2^64 truncated toward zero
-__device__ __forceinline__ i128 fix_f32(float x) {
-  const uint32_t b = __float_as_uint(x);
-  uint32_t e = (b >> 23) & 0xffu;
-  uint32_t m = b & 0x7fffffu;
+__device__ __forceinline__ i128 fix_f64(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  uint32_t e = (uint32_t)(b >> 52) & 0x7ffu;
+  unsigned long long m = b & 0xfffffffffffffull;
   if (e == 0) {
     if (m == 0) return 0;
     e = 1;
   } else {
-    m |= 0x800000u;
+    m |= 0x10000000000000ull;
   }
-  const int sh = (int)e - 86;
+  const int sh = (int)e - 1011;  // x * 2^64 = m * 2^(e - 1075 + 64)
   i128 v;
   if (sh >= 0) v = (i128)m << sh;
-  else if (sh > -24) v = (i128)(m >> (-sh));
+  else if (sh > -53) v = (i128)(m >> (-sh));
   else v = 0;
-  return (b >> 31) ? -v : v;
+  return (b >> 63) ? -v : v;
 }
 __device__ __forceinline__ double fix_to_f64(i128 a) {
   const int64_t hi = (int64_t)(a >> 64);
@@ -43,27 +44,27 @@ __device__ __forceinline__ double fix_to_f64(i128 a) {
 __device__ __forceinline__ i128 fix_abs(i128 a) { return a < 0 ? -a : a; }
 
 // a 128-bit two's-complement accumulator in two 64-bit words; adding the fixed
-// point value of a float (x * 2^64 truncated toward zero, as fix_f32) needs only
-// 64-bit shifts, so the sum equals the i128 sum of fix_f32 terms exactly
+// point value of a double (x * 2^64 truncated toward zero, as fix_f64) needs only
+// 64-bit shifts, so the sum equals the i128 sum of fix_f64 terms exactly
 struct Acc128 {
   unsigned long long lo = 0, hi = 0;
-  __device__ __forceinline__ void add_f32(float x, bool neg) {
-    const uint32_t b = __float_as_uint(x);
-    const uint32_t eb = (b >> 23) & 0xffu;
-    const uint32_t m = eb == 0 ? (b & 0x7fffffu) : ((b & 0x7fffffu) | 0x800000u);
-    const int sh = (int)(eb == 0 ? 1u : eb) - 86;
+  __device__ __forceinline__ void add_f64(double x, bool neg) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const uint32_t eb = (uint32_t)(b >> 52) & 0x7ffu;
+    const unsigned long long m = eb == 0 ? (b & 0xfffffffffffffull) : ((b & 0xfffffffffffffull) | 0x10000000000000ull);
+    const int sh = (int)(eb == 0 ? 1u : eb) - 1011;
     unsigned long long vlo, vhi;
     if (sh < 0) {
-      vlo = sh > -24 ? (unsigned long long)(m >> (uint32_t)(-sh)) : 0ull;
+      vlo = sh > -53 ? (m >> (uint32_t)(-sh)) : 0ull;
       vhi = 0;
     } else if (sh < 64) {
-      vlo = (unsigned long long)m << sh;
-      vhi = sh == 0 ? 0ull : ((unsigned long long)m >> (64 - sh));
+      vlo = m << sh;
+      vhi = sh == 0 ? 0ull : (m >> (64 - sh));
     } else {
       vlo = 0;
-      vhi = (unsigned long long)m << (sh - 64);
+      vhi = m << (sh - 64);
     }
-    if (((b >> 31) != 0) != neg) {
+    if (((b >> 63) != 0) != neg) {
       vlo = ~vlo + 1ull;
       vhi = ~vhi + (vlo == 0ull ? 1ull : 0ull);
     }
@@ -73,24 +74,24 @@ struct Acc128 {
   __device__ __forceinline__ i128 value() const { return (i128)(((unsigned __int128)hi << 64) | lo); }
 };
 
+// x*y*z exactly as two doubles: x*y is exact in FP64 (48 significant bits), and
+// TwoProduct by fma splits its product with z into a rounded product and its exact error
+__device__ __forceinline__ void triple_f64(float x, float y, float z, double* p, double* e) {
+  const double xy = (double)x * (double)y;
+  *p = xy * (double)z;
+  *e = __builtin_fma(xy, (double)z, -*p);
+}
 __device__ __forceinline__ void acc_triple(i128& acc, float x, float y, float z, bool neg) {
-  const float p = x * y;
-  const float e = __builtin_fmaf(x, y, -p);
-  const float p2 = p * z;
-  const float e2 = __builtin_fmaf(p, z, -p2);
-  const float e3 = e * z;
-  const i128 s = fix_f32(p2) + fix_f32(e2) + fix_f32(e3);
+  double p, e;
+  triple_f64(x, y, z, &p, &e);
+  const i128 s = fix_f64(p) + fix_f64(e);
   acc += neg ? -s : s;
 }
 __device__ __forceinline__ void acc_triple(Acc128& acc, float x, float y, float z, bool neg) {
-  const float p = x * y;
-  const float e = __builtin_fmaf(x, y, -p);
-  const float p2 = p * z;
-  const float e2 = __builtin_fmaf(p, z, -p2);
-  const float e3 = e * z;
-  acc.add_f32(p2, neg);
-  acc.add_f32(e2, neg);
-  acc.add_f32(e3, neg);
+  double p, e;
+  triple_f64(x, y, z, &p, &e);
+  acc.add_f64(p, neg);
+  acc.add_f64(e, neg);
 }
 template <typename A>
 __device__ __forceinline__ void acc_det(A& acc, const float* a, const float* b, const float* c, bool neg) {
@@ -101,12 +102,9 @@ __device__ __forceinline__ void acc_det(A& acc, const float* a, const float* b, 
   acc_triple(acc, a[2], b[0], c[1], neg);
   acc_triple(acc, a[2], b[1], c[0], !neg);
 }
-// x*y as two f32 terms whose sum is exact (TwoProduct by fma), added with the given sign
+// x*y, exact in FP64, added with the given sign
 __device__ __forceinline__ void acc_prod(Acc128& acc, float x, float y, bool neg) {
-  const float p = x * y;
-  const float e = __builtin_fmaf(x, y, -p);
-  acc.add_f32(p, neg);
-  acc.add_f32(e, neg);
+  acc.add_f64((double)x * (double)y, neg);
 }
 // A child's volume terms about an anchor o applied once, at the end:
 //   sum det(a-o, b-o, c-o) = sum det(a, b, c) - o . sum (b x c + c x a + a x b)

@@ -2,7 +2,8 @@
 // product). Scalar arithmetic of the lookahead path, written out op by op so
 // that the CPU restatement produces the same bits as the GPU kernels:
 // every FP32 expression below is evaluated left to right without contraction
-// (build with -ffp-contract=off), products that must be exact use fmaf, and
+// (build with -ffp-contract=off), products that must be exact are formed in FP64
+// (split by fma where they exceed 53 bits), and
 // sums that must not depend on order go into an int128 fixed-point accumulator.
 #pragma once
 #include <cmath>
@@ -16,23 +17,23 @@ constexpr uint32_t NONE = 0xffffffffu;
 constexpr float PC_EPS_F = 1e-6f;  // src/gpu/shaders.ts:162, src/cpu/cut.ts:13
 
 // ---- order-independent fixed-point sums (value * 2^64, truncated toward zero)
-inline i128 fix_f32(float x) {
-  uint32_t b;
-  std::memcpy(&b, &x, 4);
-  uint32_t e = (b >> 23) & 0xffu;
-  uint32_t m = b & 0x7fffffu;
+inline i128 fix_f64(double x) {
+  uint64_t b;
+  std::memcpy(&b, &x, 8);
+  uint32_t e = (uint32_t)(b >> 52) & 0x7ffu;
+  uint64_t m = b & 0xfffffffffffffull;
   if (e == 0) {
     if (m == 0) return 0;
     e = 1;
   } else {
-    m |= 0x800000u;
+    m |= 0x10000000000000ull;
   }
-  const int sh = (int)e - 86;  // x * 2^64 = m * 2^(e - 150 + 64)
+  const int sh = (int)e - 1011;  // x * 2^64 = m * 2^(e - 1075 + 64)
   i128 v;
   if (sh >= 0) v = (i128)m << sh;
-  else if (sh > -24) v = (i128)(m >> (-sh));
+  else if (sh > -53) v = (i128)(m >> (-sh));
   else v = 0;
-  return (b >> 31) ? -v : v;
+  return (b >> 63) ? -v : v;
 }
 inline double fix_to_f64(i128 a) {
   const int64_t hi = (int64_t)(a >> 64);
@@ -41,15 +42,14 @@ inline double fix_to_f64(i128 a) {
 }
 inline i128 fix_abs(i128 a) { return a < 0 ? -a : a; }
 
-// x*y*z as three f32 terms whose sum is exact up to the last (rounded) term,
-// accumulated with the given sign (error-free TwoProduct via fmaf)
+// x*y*z exactly as two doubles (x*y is exact in FP64, 48 significant bits; TwoProduct
+// by fma splits its product with z into the rounded product and its exact error), each
+// truncated to the fixed point and accumulated with the given sign
 inline void acc_triple(i128& acc, float x, float y, float z, bool neg) {
-  const float p = x * y;
-  const float e = std::fmaf(x, y, -p);
-  const float p2 = p * z;
-  const float e2 = std::fmaf(p, z, -p2);
-  const float e3 = e * z;
-  const i128 s = fix_f32(p2) + fix_f32(e2) + fix_f32(e3);
+  const double xy = (double)x * (double)y;
+  const double p = xy * (double)z;
+  const double e = std::fma(xy, (double)z, -p);
+  const i128 s = fix_f64(p) + fix_f64(e);
   acc += neg ? -s : s;
 }
 // det[a; b; c] = a . (b x c)
@@ -61,11 +61,9 @@ inline void acc_det(i128& acc, const float* a, const float* b, const float* c, b
   acc_triple(acc, a[2], b[0], c[1], neg);
   acc_triple(acc, a[2], b[1], c[0], !neg);
 }
-// x*y as two f32 terms whose sum is exact (TwoProduct by fmaf), accumulated with the given sign
+// x*y, exact in FP64, accumulated with the given sign
 inline void acc_prod(i128& acc, float x, float y, bool neg) {
-  const float p = x * y;
-  const float e = std::fmaf(x, y, -p);
-  const i128 s = fix_f32(p) + fix_f32(e);
+  const i128 s = fix_f64((double)x * (double)y);
   acc += neg ? -s : s;
 }
 // acc += u x w, componentwise
