@@ -402,8 +402,9 @@ __device__ __forceinline__ bool hd_chunk_far_f32(float px, float py, float pz, c
 constexpr int HDT = 256;  // threads per block: four tiles, one per wavefront
 constexpr int HDS = 64;   // samples per tile, one per lane
 // A wavefront measures its 64 samples (one per lane) against the other mesh chunk by chunk: a
-// chunk's 64 distance records are staged in the wave's LDS tile once and every lane evaluates all
-// of them (the same record for every lane: broadcast reads). The chunk nearest the samples' box is
+// chunk's 64 distance records are staged in the wave's LDS tile once and every open lane evaluates
+// each of them (the same record for every lane: broadcast reads) unless the point-to-box test on the
+// triangle's own box rules it out for that lane; a record no open lane needs is skipped. The chunk nearest the samples' box is
 // taken first; after it, a chunk is staged only when the box-to-box test cannot rule it out
 // against the largest best distance of the lanes still open. A lane closes when its best distance
 // is at most the pair's running maximum (read from the result word): it cannot raise the maximum,
@@ -450,12 +451,21 @@ __global__ __launch_bounds__(HDT) void k_hd_dist(const HdTile* __restrict__ tile
       for (int q = 0; q < 8; q++) L[8 * lane + q] = r[q];
     }
     wave_sync();
-    if (open) {
-      for (uint32_t k = 0; k < n; k++) {
-        const float d = tri_sq_pre(px, py, pz, reinterpret_cast<const float*>(L + 8 * k));
+    for (uint32_t k = 0; k < n; k++) {
+      const float* T = reinterpret_cast<const float*>(L + 8 * k);
+      // the chunk test on the triangle's own box and weight: skipped when no open lane can get
+      // a distance below its best from it
+      const float4 lo = make_float4(fminf(fminf(T[0], T[3]), T[6]), fminf(fminf(T[1], T[4]), T[7]),
+                                    fminf(fminf(T[2], T[5]), T[8]), T[31]);
+      const float4 hi = make_float4(fmaxf(fmaxf(T[0], T[3]), T[6]), fmaxf(fmaxf(T[1], T[4]), T[7]),
+                                    fmaxf(fmaxf(T[2], T[5]), T[8]), 0.0f);
+      const bool need = open && !hd_chunk_far_f32(px, py, pz, lo, hi, best);
+      if (__ballot(need) == 0) continue;
+      if (need) {
+        const float d = tri_sq_pre(px, py, pz, T);
         best = d < best ? d : best;
+        ev++;
       }
-      ev += n;
     }
   };
   // the largest best distance of the open lanes (0 when every lane is closed)
