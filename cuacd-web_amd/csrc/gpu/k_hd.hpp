@@ -6,9 +6,9 @@
 // 64 triangles get a bounding box (k_hd_chunks). A probe pass measures one
 // sample per tile exactly (k_hd_probe: a wave per sample, a triangle per lane);
 // then a wave takes a tile of 64 samples, one per lane, stages each chunk it
-// cannot rule out in LDS and evaluates its 64 triangles for every lane
-// (k_hd_dist). Lanes whose best distance cannot raise the pair's running
-// maximum stop early. The directed maximum of the per-sample minimum squared
+// cannot rule out in LDS and evaluates its triangles for every lane that the
+// same test on the triangle's own box cannot rule out (k_hd_dist). Lanes whose
+// best distance cannot raise the pair's running maximum stop early. The directed maximum of the per-sample minimum squared
 // distance goes to an atomicMax on the pair (non-negative f32 bits order like
 // their values), so the result is exact and independent of the order.
 #pragma once
@@ -399,6 +399,26 @@ __device__ __forceinline__ bool hd_chunk_far_f32(float px, float py, float pz, c
   return lb > 0.0f && lb * lb > best;
 }
 
+// The chunk test for one triangle (its corner box, its own weight w) without square roots. With
+// m = 1.01e-4 w (at least hd_chunk_far's 1e-4 w), near - m far > sqrt(best) follows from
+// near^2 > (m^2 + m) far^2 + (1 + m) best, since 2 m far sqrt(best) <= m (far^2 + best); the
+// factors 0.99998 and 1.00002 cover the f32 rounding of both sides (below 1e-6 of each), so the
+// test rules a triangle out only when hd_chunk_far on its box would.
+__device__ __forceinline__ bool hd_tri_far(float px, float py, float pz, const float* T, float best) {
+  const float lx = fminf(fminf(T[0], T[3]), T[6]), ly = fminf(fminf(T[1], T[4]), T[7]);
+  const float lz = fminf(fminf(T[2], T[5]), T[8]);
+  const float hx = fmaxf(fmaxf(T[0], T[3]), T[6]), hy = fmaxf(fmaxf(T[1], T[4]), T[7]);
+  const float hz = fmaxf(fmaxf(T[2], T[5]), T[8]);
+  const float nx = fmaxf(0.f, fmaxf(lx - px, px - hx)), ny = fmaxf(0.f, fmaxf(ly - py, py - hy));
+  const float nz = fmaxf(0.f, fmaxf(lz - pz, pz - hz));
+  const float fx = fmaxf(fabsf(px - lx), fabsf(px - hx)), fy = fmaxf(fabsf(py - ly), fabsf(py - hy));
+  const float fz = fmaxf(fabsf(pz - lz), fabsf(pz - hz));
+  const float n2 = nx * nx + ny * ny + nz * nz;
+  const float f2 = fx * fx + fy * fy + fz * fz;
+  const float m = 1.01e-4f * T[31];
+  return n2 * 0.99998f > ((m * m + m) * f2 + (1.0f + m) * best) * 1.00002f;
+}
+
 constexpr int HDT = 256;  // threads per block: four tiles, one per wavefront
 constexpr int HDS = 64;   // samples per tile, one per lane
 // A wavefront measures its 64 samples (one per lane) against the other mesh chunk by chunk: a
@@ -453,13 +473,8 @@ __global__ __launch_bounds__(HDT) void k_hd_dist(const HdTile* __restrict__ tile
     wave_sync();
     for (uint32_t k = 0; k < n; k++) {
       const float* T = reinterpret_cast<const float*>(L + 8 * k);
-      // the chunk test on the triangle's own box and weight: skipped when no open lane can get
-      // a distance below its best from it
-      const float4 lo = make_float4(fminf(fminf(T[0], T[3]), T[6]), fminf(fminf(T[1], T[4]), T[7]),
-                                    fminf(fminf(T[2], T[5]), T[8]), T[31]);
-      const float4 hi = make_float4(fmaxf(fmaxf(T[0], T[3]), T[6]), fmaxf(fmaxf(T[1], T[4]), T[7]),
-                                    fmaxf(fmaxf(T[2], T[5]), T[8]), 0.0f);
-      const bool need = open && !hd_chunk_far_f32(px, py, pz, lo, hi, best);
+      // skipped when no open lane can get a distance below its best from it
+      const bool need = open && !hd_tri_far(px, py, pz, T, best);
       if (__ballot(need) == 0) continue;
       if (need) {
         const float d = tri_sq_pre(px, py, pz, T);
